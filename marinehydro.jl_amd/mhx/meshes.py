@@ -1,0 +1,272 @@
+"""Panel meshes in the reference's StaticArraysMesh layout (src/meshes.jl:42-51).
+
+A `Mesh` holds, as contiguous numpy arrays:
+  vertices (nv, 3) float64, faces (nf, 4) int64 **1-based**, centers (nf, 3),
+  normals (nf, 3), areas (nf,), radii (nf,)
+i.e. exactly the memory of Julia's Vector{SVector{3,Float64}} / Vector{SVector{4,Int}},
+so the C ABI receives the reference's own arrays without a copy.
+
+Face properties follow the Capytaine conventions the reference imports
+(src/meshes.jl:122-142, faces_centers / faces_normals / faces_areas /
+faces_radiuses): a face whose first and last vertex indices coincide is a
+triangle; quads are split into triangles (0,1,2) and (0,2,3) for area and
+area-weighted centre, the quad normal is (v2-v0) x (v3-v1), the radius is the
+largest centre-to-vertex distance.
+
+The generators build bodies of revolution from a meridian profile the way
+Capytaine's mesh_sphere does (profile from the bottom pole upwards, rotated
+counter-clockwise about z, face (i, i+n, i+n+1, i+1)), so the reference's
+notebook sphere (radius 1, resolution (6, 6), immersed part, 18 panels) is
+reproduced vertex for vertex.
+"""
+import numpy as np
+
+
+class Mesh:
+    def __init__(self, vertices, faces, centers=None, normals=None, areas=None, radii=None):
+        self.vertices = np.ascontiguousarray(vertices, dtype=np.float64).reshape(-1, 3)
+        f = np.ascontiguousarray(faces, dtype=np.int64).reshape(-1, 4)
+        if f.size and f.min() < 1:
+            raise ValueError("faces must be 1-based (StaticArraysMesh convention)")
+        if f.size and f.max() > self.vertices.shape[0]:
+            raise ValueError("face index beyond the vertex count")
+        self.faces = f
+        if centers is None:
+            centers, normals, areas, radii = face_properties(self.vertices, self.faces)
+        self.centers = np.ascontiguousarray(centers, dtype=np.float64).reshape(-1, 3)
+        self.normals = np.ascontiguousarray(normals, dtype=np.float64).reshape(-1, 3)
+        self.areas = np.ascontiguousarray(areas, dtype=np.float64).reshape(-1)
+        self.radii = np.ascontiguousarray(radii, dtype=np.float64).reshape(-1)
+        nf = self.faces.shape[0]
+        for name, arr, shape in (("centers", self.centers, (nf, 3)), ("normals", self.normals, (nf, 3)),
+                                 ("areas", self.areas, (nf,)), ("radii", self.radii, (nf,))):
+            if arr.shape != shape:
+                raise ValueError(f"{name} must have shape {shape}, got {arr.shape}")
+
+    @property
+    def nfaces(self):
+        return self.faces.shape[0]
+
+    @property
+    def nvertices(self):
+        return self.vertices.shape[0]
+
+    nf = nfaces
+
+    def copy(self):
+        return Mesh(self.vertices.copy(), self.faces.copy(), self.centers.copy(), self.normals.copy(),
+                    self.areas.copy(), self.radii.copy())
+
+    def scaled(self, s):
+        """Uniform scaling of every field, as the reference's radius-derivative tests do."""
+        return Mesh(self.vertices * s, self.faces, self.centers * s, self.normals, self.areas * s * s, self.radii * s)
+
+
+def face_properties(vertices, faces):
+    """Capytaine's compute_faces_properties on 1-based faces."""
+    V = vertices
+    F = faces - 1
+    nf = F.shape[0]
+    centers = np.zeros((nf, 3))
+    normals = np.zeros((nf, 3))
+    areas = np.zeros(nf)
+    tri = F[:, 0] == F[:, 3]
+    quad = ~tri
+    t = F[tri]
+    if t.size:
+        cr = np.cross(V[t[:, 1]] - V[t[:, 0]], V[t[:, 2]] - V[t[:, 0]])
+        ta = np.linalg.norm(cr, axis=1)
+        normals[tri] = cr / ta[:, None]
+        areas[tri] = ta / 2.0
+        centers[tri] = np.sum(V[t[:, :3]], axis=1) / 3.0
+    q = F[quad]
+    if q.size:
+        qn = np.cross(V[q[:, 2]] - V[q[:, 0]], V[q[:, 3]] - V[q[:, 1]])
+        normals[quad] = qn / np.linalg.norm(qn, axis=1)[:, None]
+        a1 = np.linalg.norm(np.cross(V[q[:, 1]] - V[q[:, 0]], V[q[:, 2]] - V[q[:, 0]]), axis=1) * 0.5
+        a2 = np.linalg.norm(np.cross(V[q[:, 3]] - V[q[:, 0]], V[q[:, 2]] - V[q[:, 0]]), axis=1) * 0.5
+        areas[quad] = a1 + a2
+        c1 = np.sum(V[q[:, :3]], axis=1) / 3.0
+        c2 = (np.sum(V[q[:, 2:4]], axis=1) + V[q[:, 0]]) / 3.0
+        centers[quad] = (a1[:, None] * c1 + a2[:, None] * c2) / areas[quad][:, None]
+    rv = centers[None, :, :] - np.moveaxis(V[F], 0, 1)
+    radii = np.max(np.linalg.norm(rv, axis=2), axis=0)
+    return centers, normals, areas, radii
+
+
+def revolution(profile_rz, nphi, keep=None):
+    """Rotate a meridian profile [(r0, z0), ...] (bottom to top) about the z axis.
+
+    Points with r == 0 are on the axis and are merged, turning the adjacent faces
+    into triangles stored as (a, b, c, a).  Face (i, i+n, i+n+1, i+1) orders
+    vertices so that the normal points away from the axis for a profile that
+    climbs from the bottom pole.  `keep(i_profile_segment)` filters segments.
+    """
+    prof = np.asarray(profile_rz, dtype=np.float64)
+    npr = prof.shape[0]
+    dphi = 2 * np.pi / nphi
+    verts = []
+    index = {}
+
+    def vid(ip, jphi):
+        r, z = prof[ip]
+        key = (ip, 0) if r == 0.0 else (ip, jphi % nphi)
+        if key not in index:
+            ang = (jphi % nphi) * dphi
+            index[key] = len(verts) + 1
+            verts.append((r * np.cos(ang), r * np.sin(ang), z))
+        return index[key]
+
+    faces = []
+    for j in range(nphi):
+        for i in range(npr - 1):
+            if keep is not None and not keep(i):
+                continue
+            f = [vid(i, j), vid(i, j + 1), vid(i + 1, j + 1), vid(i + 1, j)]
+            if f[0] == f[1]:        # bottom pole: (P, P, a, b) -> (P, a, b, P)
+                f = [f[0], f[2], f[3], f[0]]
+            elif f[2] == f[3]:      # top pole: (a, b, P, P) -> (P, a, b, P)
+                f = [f[2], f[0], f[1], f[2]]
+            faces.append(f)
+    return Mesh(np.array(verts), np.array(faces, dtype=np.int64))
+
+
+def sphere(radius=1.0, ntheta=10, nphi=10, immersed=True):
+    """Capytaine-style UV sphere: ntheta meridian segments (pole to pole), nphi sectors.
+    With `immersed` only the segments below z=0 are kept (ntheta even)."""
+    theta = np.linspace(0.0, np.pi, ntheta + 1)
+    prof = np.stack([radius * np.sin(theta), -radius * np.cos(theta)], axis=1)
+    prof[0, 0] = 0.0
+    prof[-1, 0] = 0.0
+    if immersed:
+        if ntheta % 2:
+            raise ValueError("immersed sphere needs an even ntheta")
+        prof = prof[: ntheta // 2 + 1]
+        prof[-1, 1] = 0.0
+    return revolution(prof, nphi)
+
+
+def hemisphere(radius=1.0, nlat=32, nlon=64):
+    """Submerged hemisphere: nlat latitude rows from the bottom pole to z=0, nlon sectors."""
+    theta = np.linspace(0.0, np.pi / 2, nlat + 1)
+    prof = np.stack([radius * np.sin(theta), -radius * np.cos(theta)], axis=1)
+    prof[0, 0] = 0.0
+    prof[-1, 1] = 0.0
+    return revolution(prof, nlon)
+
+
+def cylinder(radius=5.0, draft=10.0, ncirc=32, nvert=24, nrings=8):
+    """Vertical truncated cylinder: bottom disk of nrings rings (centre ring triangles)
+    and a side of nvert rows, ncirc sectors: ncirc * (nvert + nrings) panels."""
+    rr = np.linspace(0.0, radius, nrings + 1)
+    bottom = np.stack([rr, np.full_like(rr, -draft)], axis=1)
+    zz = np.linspace(-draft, 0.0, nvert + 1)
+    side = np.stack([np.full_like(zz, radius), zz], axis=1)
+    prof = np.concatenate([bottom, side[1:]], axis=0)
+    return revolution(prof, ncirc)
+
+
+def capped_cylinder(radius=4.0, side_depth=8.0, ncirc=64, nside=40, ncap=24):
+    """Vertical cylinder (z in [-side_depth, 0]) closed below by a hemispherical cap:
+    draft = side_depth + radius, ncirc * (nside + ncap) panels."""
+    th = np.linspace(0.0, np.pi / 2, ncap + 1)
+    cap = np.stack([radius * np.sin(th), -side_depth - radius * np.cos(th)], axis=1)
+    cap[0, 0] = 0.0
+    zz = np.linspace(-side_depth, 0.0, nside + 1)
+    side = np.stack([np.full_like(zz, radius), zz], axis=1)
+    prof = np.concatenate([cap, side[1:]], axis=0)
+    return revolution(prof, ncirc)
+
+
+def _box_counts(L, W, T, n):
+    best = None
+    for nx in range(1, 257):
+        for ny in range(1, 257):
+            rem = n - nx * ny
+            if rem <= 0:
+                break
+            if rem % (2 * (nx + ny)):
+                continue
+            nz = rem // (2 * (nx + ny))
+            h = np.array([L / nx, W / ny, T / nz])
+            score = np.log(h.max() / h.min())
+            if best is None or score < best[0]:
+                best = (score, nx, ny, nz)
+    if best is None:
+        raise ValueError(f"no box grid with exactly {n} panels")
+    return best[1:]
+
+
+def box(length, width, draft, n):
+    """Immersed part of a rectangular box (centred on the z axis, top at z=0) with
+    exactly n panels: bottom nx*ny, two sides nx*nz, two ends ny*nz."""
+    nx, ny, nz = _box_counts(length, width, draft, n)
+    xs = np.linspace(-length / 2, length / 2, nx + 1)
+    ys = np.linspace(-width / 2, width / 2, ny + 1)
+    zs = np.linspace(-draft, 0.0, nz + 1)
+    verts, faces, index = [], [], {}
+
+    def vid(p):
+        key = tuple(np.round(p, 12))
+        if key not in index:
+            index[key] = len(verts) + 1
+            verts.append(p)
+        return index[key]
+
+    def grid(u, v, f):  # quads with outward orientation given by f(u,v)
+        for a in range(len(u) - 1):
+            for b in range(len(v) - 1):
+                p = [f(u[a], v[b]), f(u[a + 1], v[b]), f(u[a + 1], v[b + 1]), f(u[a], v[b + 1])]
+                faces.append([vid(np.array(q, dtype=np.float64)) for q in p])
+
+    grid(ys, xs, lambda y, x: (x, y, -draft))                 # bottom, normal -z
+    grid(xs, zs, lambda x, z: (x, -width / 2, z))             # side y=-W/2, normal -y
+    grid(zs, xs, lambda z, x: (x, width / 2, z))              # side y=+W/2, normal +y
+    grid(zs, ys, lambda z, y: (-length / 2, y, z))            # end x=-L/2, normal -x
+    grid(ys, zs, lambda y, z: (length / 2, y, z))             # end x=+L/2, normal +x
+    m = Mesh(np.array(verts), np.array(faces, dtype=np.int64))
+    assert m.nfaces == n
+    return m
+
+
+def sphere_with_count(radius, n):
+    """Immersed hemisphere with exactly n in {256, ..., 4096} panels."""
+    table = {256: (8, 32), 512: (16, 32), 1024: (16, 64), 2048: (32, 64), 4096: (32, 128)}
+    nlat, nlon = table[n]
+    return hemisphere(radius, nlat, nlon)
+
+
+def cylinder_with_count(radius, draft, n):
+    table = {256: 16, 512: 32, 1024: 32, 2048: 64, 4096: 64}
+    nc = table[n]
+    rows = n // nc
+    nr = int(np.clip(round(rows * radius / (radius + draft)), 1, rows - 1))
+    return cylinder(radius, draft, nc, rows - nr, nr)
+
+
+def random_bodies(seed=20261203, nbodies=8):
+    """Seeded uneven workload: spheres (R~U[0.5,3]), cylinders (R~U[1,6], draft~U[2,12]),
+    boxes (sides~U[1,10], half immersed), panel counts from {256,...,4096},
+    16-64 log-uniform frequencies in [0.1, 3.0] rad/s each."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for b in range(nbodies):
+        kind = ["sphere", "cylinder", "box"][int(rng.integers(0, 3))]
+        n = int(rng.choice([256, 512, 1024, 2048, 4096]))
+        if kind == "sphere":
+            R = rng.uniform(0.5, 3.0)
+            mesh = sphere_with_count(R, n)
+            desc = dict(kind=kind, radius=R)
+        elif kind == "cylinder":
+            R = rng.uniform(1.0, 6.0); T = rng.uniform(2.0, 12.0)
+            mesh = cylinder_with_count(R, T, n)
+            desc = dict(kind=kind, radius=R, draft=T)
+        else:
+            L, W, H = rng.uniform(1.0, 10.0, size=3)
+            mesh = box(L, W, H / 2, n)
+            desc = dict(kind=kind, sides=(L, W, H))
+        nw = int(rng.integers(16, 65))
+        omegas = np.sort(np.exp(rng.uniform(np.log(0.1), np.log(3.0), size=nw)))
+        desc.update(n=n, nw=nw)
+        out.append((mesh, omegas, desc))
+    return out
