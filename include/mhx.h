@@ -1,0 +1,118 @@
+/*
+ * mhx.h — C ABI of libmhx, the MI355X (gfx950) boundary-element hot path of
+ * MarineHydro.jl: Rankine + wave Green-function panel integrals, dense complex
+ * S/D assembly, dense complex LU solve, pressure integration into added mass,
+ * radiation damping and excitation force.
+ *
+ * Plain pointers and sizes only.  All host buffers are owned by the caller;
+ * device memory is owned by the context.  Every call is synchronous and
+ * returns MHX_OK (0) or a negative status; mhx_last_error() gives the message.
+ * A context is not thread-safe: use one per host thread (Julia task).
+ *
+ * Array conventions are those of the reference's Julia types, so Julia passes
+ * its own arrays without a copy:
+ *   vertices  nv x 3 (x,y,z per vertex)           Vector{SVector{3,Float64}}
+ *   faces     nf x 4 int64, 1-based               Vector{SVector{4,Int}}
+ *   centers, normals nf x 3; areas, radii nf      (src/meshes.jl:42-51)
+ *   complex   interleaved (re, im) doubles        ComplexF64 / double _Complex
+ *   matrices  n x n column-major                  Matrix{ComplexF64}
+ *   dofs      ndof consecutive n x 3 column-major Matrix{Float64}(n, 3) each
+ *             (src/bodies.jl:74-118)
+ */
+#ifndef MHX_H
+#define MHX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes; the Julia glue maps EINVAL -> ArgumentError, EDIM -> DimensionMismatch */
+#define MHX_OK 0
+#define MHX_EINVAL (-1)     /* bad argument value */
+#define MHX_EDIM (-2)       /* inconsistent sizes */
+#define MHX_ESINGULAR (-3)  /* exactly zero pivot (LAPACK info > 0) */
+#define MHX_EDEVICE (-4)    /* HIP runtime failure */
+#define MHX_ENOMEM (-5)     /* device allocation failure */
+#define MHX_EQUAD (-6)      /* adaptive quadrature interval budget exhausted */
+
+/* wave Green functions: GFWu() and ExactGuevelDelhommeau() of src/formulations.jl:26-37 */
+#define MHX_GF_WU 0
+#define MHX_GF_DELHOMMEAU 1
+
+/* solve schedule of mhx_hydro_coeffs */
+#define MHX_MODE_PER_OMEGA 0   /* one assembly + one LU per frequency, all right-hand sides together */
+#define MHX_MODE_PER_PROBLEM 1 /* reference semantics: assembly + LU per radiation/diffraction problem */
+
+typedef struct mhx_ctx mhx_ctx;
+typedef struct mhx_body mhx_body;
+
+typedef struct {
+    double rho;     /* fluid density (problem kwarg rho, src/problems_and_results.jl:30) */
+    double g;       /* gravity (src/constants.jl:15) */
+    int device;     /* HIP device ordinal */
+    int timing;     /* 1: record per-phase HIP events (see mhx_phase_time) */
+} mhx_options;
+
+typedef struct {
+    int64_t nv, nf;
+    const double* vertices;
+    const int64_t* faces;
+    const double* centers;
+    const double* normals;
+    const double* areas;
+    const double* radii;
+} mhx_mesh;
+
+/* context lifetime */
+int mhx_create(mhx_ctx** ctx, const mhx_options* opt);
+void mhx_destroy(mhx_ctx* ctx);
+const char* mhx_last_error(const mhx_ctx* ctx);
+int mhx_set_constants(mhx_ctx* ctx, double rho, double g); /* set_rho!/set_g! (src/constants.jl:21-29) */
+
+/* assemble_matrices(gfs, mesh, k; direct, include_identity) for gfs = (Rankine(), RankineReflected(), gf)
+ * (src/matrix_assembly.jl:164-208).  S_out, D_out: n x n complex column-major host buffers. */
+int mhx_assemble(mhx_ctx* ctx, const mhx_mesh* mesh, double k, int gf, int direct, int include_identity,
+                 double* S_out, double* D_out);
+
+/* solve(D, S, bc; direct) (src/matrix_assembly.jl:436-449): direct phi = D \ (S bc);
+ * indirect sources = D \ bc, phi = S sources.  bc, phi_out, sources_out: n x nrhs complex
+ * (sources_out may be NULL; it is left untouched for the direct method). */
+int mhx_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* D, const double* S, const double* bc, int direct,
+              double* phi_out, double* sources_out);
+
+/* zgetrf + zgetrs on the device (LinearSolve LUFactorization, src/matrix_assembly.jl:411-416):
+ * X = A \ B (trans = 0) or A^H \ B (trans = 2); B overwritten by X.  Returns MHX_ESINGULAR on a zero pivot. */
+int mhx_lu_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* A, int trans, double* B);
+
+/* hydrodynamic_coefficients(body, (wave_frequencies, radiating_dofs=all, wave_directions), formulation)
+ * (src/problems_and_results.jl:464-495) with every dof radiating and influenced:
+ *   A_out, B_out  ndof x ndof x nw          (influenced, radiating, omega)
+ *   Fd_out, Fk_out ndof x nw x nb complex   (influenced, omega, beta); excitation = Fd + Fk */
+int mhx_hydro_coeffs(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const double* dofs, int64_t nw,
+                     const double* omegas, int64_t nb, const double* betas, int gf, int direct, int mode,
+                     double* A_out, double* B_out, double* Fd_out, double* Fk_out);
+
+/* device-resident body (FloatingBody with its rigid dofs, src/bodies.jl:9-51): geometry and the
+ * wavenumber-independent Rankine part stay on the device across calls */
+int mhx_body_create(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const double* dofs, mhx_body** body);
+void mhx_body_destroy(mhx_body* body);
+int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* body, int64_t nw, const double* omegas, int64_t nb, const double* betas,
+                    int gf, int direct, int mode, double* A_out, double* B_out, double* Fd_out, double* Fk_out);
+
+/* per-phase timing (HIP events on the context's stream, enabled by mhx_options.timing):
+ * phase: "rankine", "wave_table" (exact wave term), "wave" (S, D formation), "lu", "solve", "forces";
+ * returns total ms and launch count since reset */
+int mhx_phase_time(mhx_ctx* ctx, const char* phase, double* total_ms, int64_t* count);
+void mhx_reset_timing(mhx_ctx* ctx);
+
+/* work of the exact Guevel-Delhommeau wave term since the last mhx_reset_timing: counts =
+ * {integrand evaluations, E1 power-series terms, E1 continued-fraction steps}; flops = the
+ * algorithmic FP64 operation count derived from them (no counterpart in the reference;
+ * used for roofline reporting) */
+int mhx_exact_work(mhx_ctx* ctx, int64_t counts[3], double* flops);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

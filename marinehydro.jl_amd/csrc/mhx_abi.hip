@@ -1,0 +1,547 @@
+// C ABI of libmhx (declared in include/mhx.h): contexts, device-resident bodies,
+// and the host orchestration of one frequency solve
+// (reference: src/solve.jl:47-78, src/problems_and_results.jl:464-495).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mhx.h"
+#include "mhx_internal.h"
+
+
+
+struct PhaseAcc {
+    double ms = 0.0;
+    int64_t count = 0;
+};
+
+struct mhx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double rho = 1000.0, g = 9.81;
+    int timing = 0;
+    std::string err;
+    std::map<std::string, PhaseAcc> phases;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    unsigned long long quad_work[3] = {0, 0, 0};  // exact wave term: evaluations, series terms, CF steps
+};
+
+struct mhx_body {
+    mhx_ctx* ctx = nullptr;
+    int n = 0, ndof = 0;
+    PanelGeom* geom = nullptr;
+    double* dofs = nullptr;
+    double* SR = nullptr;
+    double* DR[2] = {nullptr, nullptr};  // [direct] : K_R (indirect) or D_R (direct)
+    bool have_rankine[2] = {false, false};
+    // per-frequency work
+    cplx *S = nullptr, *D = nullptr, *WG = nullptr, *WR = nullptr;
+    cplx *B = nullptr, *Y = nullptr, *work = nullptr, *F = nullptr, *FK = nullptr;
+    double* betas = nullptr;
+    int cap_R = 0, cap_nb = 0;
+    int *ipiv = nullptr, *perm = nullptr, *wperm = nullptr, *info = nullptr;
+    unsigned long long* status = nullptr;
+};
+
+namespace {
+
+int fail(mhx_ctx* c, int code, const std::string& msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define HCHK(call)                                                                                   \
+    do {                                                                                             \
+        hipError_t e_ = (call);                                                                      \
+        if (e_ != hipSuccess)                                                                        \
+            return fail(ctx, MHX_EDEVICE, std::string(#call) + ": " + hipGetErrorString(e_));        \
+    } while (0)
+
+template <typename T>
+int dalloc(mhx_ctx* ctx, T** p, size_t count) {
+    if (*p) return MHX_OK;
+    hipError_t e = hipMalloc((void**)p, sizeof(T) * (count ? count : 1));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(ctx, MHX_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    return MHX_OK;
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+struct Phase {
+    mhx_ctx* c;
+    std::string name;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    Phase(mhx_ctx* c_, const char* nm) : c(c_), name(nm) {
+        if (!c->timing) return;
+        hipEventCreate(&e0);
+        hipEventCreate(&e1);
+        hipEventRecord(e0, c->stream);
+    }
+    ~Phase() {
+        if (!c->timing) return;
+        hipEventRecord(e1, c->stream);
+        c->pending.push_back({name, {e0, e1}});
+    }
+};
+
+void flush_timing(mhx_ctx* c) {
+    for (auto& p : c->pending) {
+        float ms = 0.f;
+        hipEventSynchronize(p.second.second);
+        hipEventElapsedTime(&ms, p.second.first, p.second.second);
+        auto& a = c->phases[p.first];
+        a.ms += ms;
+        a.count += 1;
+        hipEventDestroy(p.second.first);
+        hipEventDestroy(p.second.second);
+    }
+    c->pending.clear();
+}
+
+int check_mesh(mhx_ctx* ctx, const mhx_mesh* m) {
+    if (!m || !m->vertices || !m->faces || !m->centers || !m->normals || !m->areas || !m->radii)
+        return fail(ctx, MHX_EINVAL, "mesh: null array");
+    if (m->nf <= 0 || m->nv <= 0) return fail(ctx, MHX_EDIM, "mesh: empty");
+    if (m->nf > (1 << 30)) return fail(ctx, MHX_EDIM, "mesh: too many faces");
+    for (int64_t i = 0; i < 4 * m->nf; i++)
+        if (m->faces[i] < 1 || m->faces[i] > m->nv) return fail(ctx, MHX_EDIM, "mesh: face index out of 1..nv");
+    return MHX_OK;
+}
+
+int ensure_rankine(mhx_ctx* ctx, mhx_body* b, int direct) {
+    if (b->have_rankine[direct]) return MHX_OK;
+    const size_t nn = (size_t)b->n * b->n;
+    int rc;
+    if ((rc = dalloc(ctx, &b->SR, nn))) return rc;
+    if ((rc = dalloc(ctx, &b->DR[direct], nn))) return rc;
+    {
+        Phase ph(ctx, "rankine");
+        mhx::launch_rankine(ctx->stream, b->n, b->geom, direct, b->SR, b->DR[direct]);
+    }
+    HCHK(hipGetLastError());
+    b->have_rankine[direct] = true;
+    return MHX_OK;
+}
+
+int ensure_work(mhx_ctx* ctx, mhx_body* b, int R, int nb, bool table) {
+    const size_t n = b->n, nn = n * n;
+    int rc;
+    if ((rc = dalloc(ctx, &b->S, nn))) return rc;
+    if ((rc = dalloc(ctx, &b->D, nn))) return rc;
+    if (table) {
+        if ((rc = dalloc(ctx, &b->WG, nn))) return rc;
+        if ((rc = dalloc(ctx, &b->WR, nn))) return rc;
+    }
+    if (R > b->cap_R || nb > b->cap_nb) {
+        dfree(b->B); dfree(b->Y); dfree(b->work); dfree(b->F); dfree(b->FK); dfree(b->betas);
+        b->cap_R = R > 16 ? R : 16;
+        b->cap_nb = nb > 8 ? nb : 8;
+    }
+    const int cr = b->cap_R;
+    if ((rc = dalloc(ctx, &b->B, n * cr))) return rc;
+    if ((rc = dalloc(ctx, &b->Y, n * cr))) return rc;
+    if ((rc = dalloc(ctx, &b->work, 16 * n * cr + 2 * n * cr))) return rc;
+    if ((rc = dalloc(ctx, &b->F, (size_t)b->ndof * cr))) return rc;
+    if ((rc = dalloc(ctx, &b->FK, (size_t)b->ndof * b->cap_nb))) return rc;
+    if ((rc = dalloc(ctx, &b->betas, (size_t)b->cap_nb))) return rc;
+    if ((rc = dalloc(ctx, &b->ipiv, n))) return rc;
+    if ((rc = dalloc(ctx, &b->perm, n))) return rc;
+    if ((rc = dalloc(ctx, &b->wperm, (size_t)mhx::kPermWork))) return rc;
+    if ((rc = dalloc(ctx, &b->info, 1))) return rc;
+    if ((rc = dalloc(ctx, &b->status, 4))) return rc;
+    return MHX_OK;
+}
+
+// wave assembly into b->S, b->D for wavenumber k
+int assemble_into(mhx_ctx* ctx, mhx_body* b, double k, int gf, int direct, int include_identity) {
+    if (gf == MHX_GF_WU) {
+        Phase ph(ctx, "wave");
+        mhx::launch_combine_wu(ctx->stream, b->n, b->geom, b->SR, b->DR[direct], k, direct, include_identity, b->S,
+                               b->D);
+    } else {
+        HCHK(hipMemsetAsync(b->status, 0, 4 * sizeof(unsigned long long), ctx->stream));
+        {
+            Phase ph(ctx, "wave_table");
+            mhx::launch_delhommeau_table(ctx->stream, b->n, b->geom, k, 1e-8, 1e-8, b->WG, b->WR, b->status);
+        }
+        Phase ph(ctx, "wave");
+        mhx::launch_combine_table(ctx->stream, b->n, b->geom, b->SR, b->DR[direct], k, direct, include_identity,
+                                  b->WG, b->WR, b->S, b->D);
+    }
+    HCHK(hipGetLastError());
+    return MHX_OK;
+}
+
+int check_status(mhx_ctx* ctx, mhx_body* b, int gf) {
+    if (gf != MHX_GF_DELHOMMEAU) return MHX_OK;
+    unsigned long long st[4] = {0, 0, 0, 0};
+    HCHK(hipMemcpyAsync(st, b->status, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
+    HCHK(hipStreamSynchronize(ctx->stream));
+    for (int q = 0; q < 3; q++) ctx->quad_work[q] += st[q + 1];
+    if (st[0]) return fail(ctx, MHX_EQUAD, "exact Delhommeau quadrature exceeded its interval budget");
+    return MHX_OK;
+}
+
+// factor b->D in place and solve for the R columns of b->B; phi into b->Y
+int factor_and_solve(mhx_ctx* ctx, mhx_body* b, int R, int direct) {
+    const int n = b->n;
+    HCHK(hipMemsetAsync(b->info, 0, sizeof(int), ctx->stream));
+    {
+        Phase ph(ctx, "lu");
+        int rc = mhx::getrf_batched(ctx->stream, n, b->D, n, (int64_t)n * n, 1, b->ipiv, n, b->wperm, b->info);
+        if (rc) return fail(ctx, MHX_EDIM, "LU: order above 4096 is not supported");
+    }
+    HCHK(hipGetLastError());
+    {
+        Phase ph(ctx, "solve");
+        cplx* W2 = b->work + (size_t)16 * n * b->cap_R;
+        if (direct) {
+            mhx::launch_matmul_nr(ctx->stream, n, R, b->S, b->B, b->Y, b->work);
+            mhx::getrs_batched(ctx->stream, 0, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, b->Y, 0, W2, 0, 1);
+        } else {
+            HCHK(hipMemcpyAsync(b->work, b->B, sizeof(cplx) * (size_t)n * R, hipMemcpyDeviceToDevice, ctx->stream));
+            // sources (in place in b->B)
+            mhx::getrs_batched(ctx->stream, 0, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, b->B, 0, W2, 0, 1);
+            mhx::launch_matmul_nr(ctx->stream, n, R, b->S, b->B, b->Y, b->work);
+        }
+    }
+    HCHK(hipGetLastError());
+    int info = 0;
+    HCHK(hipMemcpyAsync(&info, b->info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HCHK(hipStreamSynchronize(ctx->stream));
+    if (info) return fail(ctx, MHX_ESINGULAR, "LU: exactly zero pivot at column " + std::to_string(info));
+    return MHX_OK;
+}
+
+int body_upload(mhx_ctx* ctx, const mhx_mesh* m, int64_t ndof, const double* dofs, mhx_body** out) {
+    int rc = check_mesh(ctx, m);
+    if (rc) return rc;
+    if (ndof < 0 || (ndof > 0 && !dofs)) return fail(ctx, MHX_EINVAL, "dofs: null with ndof > 0");
+    mhx_body* b = new mhx_body();
+    b->ctx = ctx;
+    b->n = (int)m->nf;
+    b->ndof = (int)ndof;
+    const size_t n = m->nf, nv = m->nv;
+    double *dv = nullptr, *dc = nullptr, *dn = nullptr, *da = nullptr, *dr = nullptr;
+    int64_t* df = nullptr;
+    auto cleanup = [&]() { dfree(dv); dfree(dc); dfree(dn); dfree(da); dfree(dr); dfree(df); };
+    if ((rc = dalloc(ctx, &dv, 3 * nv)) || (rc = dalloc(ctx, &df, 4 * n)) || (rc = dalloc(ctx, &dc, 3 * n)) ||
+        (rc = dalloc(ctx, &dn, 3 * n)) || (rc = dalloc(ctx, &da, n)) || (rc = dalloc(ctx, &dr, n)) ||
+        (rc = dalloc(ctx, &b->geom, n)) || (rc = dalloc(ctx, &b->dofs, 3 * n * (ndof ? ndof : 1)))) {
+        cleanup();
+        mhx_body_destroy(b);
+        return rc;
+    }
+    hipStream_t st = ctx->stream;
+    hipMemcpyAsync(dv, m->vertices, sizeof(double) * 3 * nv, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(df, m->faces, sizeof(int64_t) * 4 * n, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dc, m->centers, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dn, m->normals, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(da, m->areas, sizeof(double) * n, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dr, m->radii, sizeof(double) * n, hipMemcpyHostToDevice, st);
+    if (ndof) hipMemcpyAsync(b->dofs, dofs, sizeof(double) * 3 * n * ndof, hipMemcpyHostToDevice, st);
+    mhx::launch_panel_geometry(st, (int)n, dv, df, dc, dn, da, dr, b->geom);
+    hipError_t e = hipStreamSynchronize(st);
+    cleanup();
+    if (e != hipSuccess || (e = hipGetLastError()) != hipSuccess) {
+        mhx_body_destroy(b);
+        return fail(ctx, MHX_EDEVICE, std::string("body upload: ") + hipGetErrorString(e));
+    }
+    *out = b;
+    return MHX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhx_create(mhx_ctx** out, const mhx_options* opt) {
+    if (!out) return MHX_EINVAL;
+    *out = nullptr;
+    mhx_ctx* c = new mhx_ctx();
+    if (opt) {
+        c->rho = opt->rho;
+        c->g = opt->g;
+        c->device = opt->device;
+        c->timing = opt->timing;
+    }
+    if (!(c->g > 0) || !(c->rho > 0)) {
+        delete c;
+        return MHX_EINVAL;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0 || c->device < 0 || c->device >= ndev) {
+        delete c;
+        return MHX_EDEVICE;
+    }
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return MHX_EDEVICE;
+    }
+    *out = c;
+    return MHX_OK;
+}
+
+void mhx_destroy(mhx_ctx* c) {
+    if (!c) return;
+    flush_timing(c);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char* mhx_last_error(const mhx_ctx* c) { return c ? c->err.c_str() : "null context"; }
+
+int mhx_set_constants(mhx_ctx* ctx, double rho, double g) {
+    if (!ctx) return MHX_EINVAL;
+    if (!(rho > 0) || !(g > 0)) return fail(ctx, MHX_EINVAL, "rho and g must be positive");
+    ctx->rho = rho;
+    ctx->g = g;
+    return MHX_OK;
+}
+
+int mhx_body_create(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const double* dofs, mhx_body** body) {
+    if (!ctx || !body) return MHX_EINVAL;
+    hipSetDevice(ctx->device);
+    return body_upload(ctx, mesh, ndof, dofs, body);
+}
+
+void mhx_body_destroy(mhx_body* b) {
+    if (!b) return;
+    dfree(b->geom); dfree(b->dofs); dfree(b->SR); dfree(b->DR[0]); dfree(b->DR[1]);
+    dfree(b->S); dfree(b->D); dfree(b->WG); dfree(b->WR); dfree(b->B); dfree(b->Y); dfree(b->work);
+    dfree(b->F); dfree(b->FK); dfree(b->betas); dfree(b->ipiv); dfree(b->perm); dfree(b->wperm);
+    dfree(b->info); dfree(b->status);
+    delete b;
+}
+
+int mhx_assemble(mhx_ctx* ctx, const mhx_mesh* mesh, double k, int gf, int direct, int include_identity, double* S_out,
+                 double* D_out) {
+    if (!ctx) return MHX_EINVAL;
+    if (!S_out || !D_out) return fail(ctx, MHX_EINVAL, "assemble: null output");
+    if (gf != MHX_GF_WU && gf != MHX_GF_DELHOMMEAU) return fail(ctx, MHX_EINVAL, "unknown Green function");
+    if (!(k > 0) || !std::isfinite(k)) return fail(ctx, MHX_EINVAL, "wavenumber must be positive and finite");
+    hipSetDevice(ctx->device);
+    direct = direct ? 1 : 0;
+    mhx_body* b = nullptr;
+    int rc = body_upload(ctx, mesh, 0, nullptr, &b);
+    if (rc) return rc;
+    if (!(rc = ensure_rankine(ctx, b, direct)) && !(rc = ensure_work(ctx, b, 1, 0, gf != MHX_GF_WU)) &&
+        !(rc = assemble_into(ctx, b, k, gf, direct, include_identity ? 1 : 0)) && !(rc = check_status(ctx, b, gf))) {
+        const size_t bytes = sizeof(cplx) * (size_t)b->n * b->n;
+        hipError_t e1 = hipMemcpyAsync(S_out, b->S, bytes, hipMemcpyDeviceToHost, ctx->stream);
+        hipError_t e2 = hipMemcpyAsync(D_out, b->D, bytes, hipMemcpyDeviceToHost, ctx->stream);
+        hipError_t e3 = hipStreamSynchronize(ctx->stream);
+        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) rc = fail(ctx, MHX_EDEVICE, "assemble: copy back");
+    }
+    flush_timing(ctx);
+    mhx_body_destroy(b);
+    return rc;
+}
+
+int mhx_lu_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* A, int trans, double* Bh) {
+    if (!ctx) return MHX_EINVAL;
+    if (n <= 0 || nrhs <= 0) return fail(ctx, MHX_EDIM, "lu_solve: empty system");
+    if (nrhs > 16) return fail(ctx, MHX_EDIM, "lu_solve: at most 16 right-hand sides per call");
+    if (n > 4096) return fail(ctx, MHX_EDIM, "lu_solve: order above 4096 is not supported");
+    if (!A || !Bh) return fail(ctx, MHX_EINVAL, "lu_solve: null buffer");
+    if (trans != 0 && trans != 2) return fail(ctx, MHX_EINVAL, "lu_solve: trans must be 0 or 2");
+    hipSetDevice(ctx->device);
+    cplx *dA = nullptr, *dB = nullptr, *dW = nullptr;
+    int *ipiv = nullptr, *perm = nullptr, *wperm = nullptr, *info = nullptr;
+    int rc;
+    auto cleanup = [&]() { dfree(dA); dfree(dB); dfree(dW); dfree(ipiv); dfree(perm); dfree(wperm); dfree(info); };
+    if ((rc = dalloc(ctx, &dA, (size_t)n * n)) || (rc = dalloc(ctx, &dB, (size_t)n * nrhs)) ||
+        (rc = dalloc(ctx, &dW, (size_t)2 * n * nrhs)) || (rc = dalloc(ctx, &ipiv, n)) || (rc = dalloc(ctx, &perm, n)) ||
+        (rc = dalloc(ctx, &wperm, (size_t)mhx::kPermWork)) || (rc = dalloc(ctx, &info, 1))) {
+        cleanup();
+        return rc;
+    }
+    hipStream_t st = ctx->stream;
+    hipMemcpyAsync(dA, A, sizeof(cplx) * n * n, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dB, Bh, sizeof(cplx) * n * nrhs, hipMemcpyHostToDevice, st);
+    hipMemsetAsync(info, 0, sizeof(int), st);
+    {
+        Phase ph(ctx, "lu");
+        mhx::getrf_batched(st, (int)n, dA, (int)n, n * n, 1, ipiv, n, wperm, info);
+    }
+    {
+        Phase ph(ctx, "solve");
+        mhx::getrs_batched(st, trans, (int)n, (int)nrhs, dA, (int)n, 0, ipiv, n, perm, n, dB, 0, dW, 0, 1);
+    }
+    int hinfo = 0;
+    hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st);
+    hipMemcpyAsync(Bh, dB, sizeof(cplx) * n * nrhs, hipMemcpyDeviceToHost, st);
+    hipError_t e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = hipGetLastError();
+    flush_timing(ctx);
+    cleanup();
+    if (e != hipSuccess) return fail(ctx, MHX_EDEVICE, std::string("lu_solve: ") + hipGetErrorString(e));
+    if (hinfo) return fail(ctx, MHX_ESINGULAR, "LU: exactly zero pivot at column " + std::to_string(hinfo));
+    return MHX_OK;
+}
+
+int mhx_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* D, const double* S, const double* bc, int direct,
+              double* phi_out, double* sources_out) {
+    if (!ctx) return MHX_EINVAL;
+    if (n <= 0 || nrhs <= 0) return fail(ctx, MHX_EDIM, "solve: empty system");
+    if (nrhs > 16) return fail(ctx, MHX_EDIM, "solve: at most 16 right-hand sides per call");
+    if (n > 4096) return fail(ctx, MHX_EDIM, "solve: order above 4096 is not supported");
+    if (!D || !S || !bc || !phi_out) return fail(ctx, MHX_EINVAL, "solve: null buffer");
+    hipSetDevice(ctx->device);
+    mhx_body b;
+    b.ctx = ctx;
+    b.n = (int)n;
+    b.ndof = 0;
+    int rc = ensure_work(ctx, &b, (int)nrhs, 0, false);
+    if (!rc) {
+        hipStream_t st = ctx->stream;
+        hipMemcpyAsync(b.D, D, sizeof(cplx) * n * n, hipMemcpyHostToDevice, st);
+        hipMemcpyAsync(b.S, S, sizeof(cplx) * n * n, hipMemcpyHostToDevice, st);
+        hipMemcpyAsync(b.B, bc, sizeof(cplx) * n * nrhs, hipMemcpyHostToDevice, st);
+        rc = factor_and_solve(ctx, &b, (int)nrhs, direct ? 1 : 0);
+        if (!rc) {
+            hipMemcpyAsync(phi_out, b.Y, sizeof(cplx) * n * nrhs, hipMemcpyDeviceToHost, st);
+            if (!direct && sources_out)
+                hipMemcpyAsync(sources_out, b.B, sizeof(cplx) * n * nrhs, hipMemcpyDeviceToHost, st);
+            if (hipStreamSynchronize(st) != hipSuccess) rc = fail(ctx, MHX_EDEVICE, "solve: copy back");
+        }
+    }
+    flush_timing(ctx);
+    dfree(b.S); dfree(b.D); dfree(b.B); dfree(b.Y); dfree(b.work); dfree(b.F); dfree(b.FK); dfree(b.betas);
+    dfree(b.ipiv); dfree(b.perm); dfree(b.wperm); dfree(b.info); dfree(b.status);
+    return rc;
+}
+
+int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas, int64_t nb, const double* betas, int gf,
+                    int direct, int mode, double* A_out, double* B_out, double* Fd_out, double* Fk_out) {
+    if (!ctx || !b) return MHX_EINVAL;
+    if (nw < 0 || nb < 0 || (nw && !omegas) || (nb && !betas)) return fail(ctx, MHX_EINVAL, "coeffs: bad frequency/heading arrays");
+    if (gf != MHX_GF_WU && gf != MHX_GF_DELHOMMEAU) return fail(ctx, MHX_EINVAL, "unknown Green function");
+    if (mode != MHX_MODE_PER_OMEGA && mode != MHX_MODE_PER_PROBLEM) return fail(ctx, MHX_EINVAL, "unknown mode");
+    const int ndof = b->ndof, n = b->n;
+    const int R = ndof + (int)nb;
+    if (R <= 0) return fail(ctx, MHX_EDIM, "coeffs: no radiating dof and no heading");
+    if (R > 16) return fail(ctx, MHX_EDIM, "coeffs: ndof + nbeta must be at most 16");
+    if (n > 4096) return fail(ctx, MHX_EDIM, "coeffs: more than 4096 panels is not supported");
+    if ((ndof && (!A_out || !B_out)) || (nb && ndof && (!Fd_out || !Fk_out)))
+        return fail(ctx, MHX_EINVAL, "coeffs: null output");
+    for (int64_t w = 0; w < nw; w++)
+        if (!(omegas[w] > 0) || !std::isfinite(omegas[w])) return fail(ctx, MHX_EINVAL, "omega must be positive and finite");
+    hipSetDevice(ctx->device);
+    direct = direct ? 1 : 0;
+    int rc;
+    if ((rc = ensure_work(ctx, b, R, (int)nb, gf != MHX_GF_WU))) return rc;
+    if ((rc = ensure_rankine(ctx, b, direct))) return rc;
+    hipStream_t st = ctx->stream;
+    if (nb) HCHK(hipMemcpyAsync(b->betas, betas, sizeof(double) * nb, hipMemcpyHostToDevice, st));
+    std::vector<cplx> F((size_t)ndof * R), FK((size_t)ndof * (nb ? nb : 1));
+    for (int64_t iw = 0; iw < nw; iw++) {
+        const double w = omegas[iw];
+        const double k = w * w / ctx->g;  // compute_wavenumber (src/waves.jl:133-134)
+        mhx::launch_bc(st, n, b->geom, b->dofs, ndof, b->betas, (int)nb, w, k, ctx->g, b->B);
+        if (mode == MHX_MODE_PER_OMEGA) {
+            if ((rc = assemble_into(ctx, b, k, gf, direct, 1)) || (rc = check_status(ctx, b, gf))) break;
+            if ((rc = factor_and_solve(ctx, b, R, direct))) break;
+        } else {
+            // one assembly (Rankine part included, as the reference recomputes it) and one LU per problem
+            cplx* phi_all = b->work + (size_t)16 * n * b->cap_R;  // reuse as staging for the R solutions
+            std::vector<cplx> hB((size_t)n * R), hphi((size_t)n * R);
+            HCHK(hipMemcpyAsync(hB.data(), b->B, sizeof(cplx) * n * R, hipMemcpyDeviceToHost, st));
+            HCHK(hipStreamSynchronize(st));
+            for (int r = 0; r < R && !rc; r++) {
+                b->have_rankine[direct] = false;
+                if ((rc = ensure_rankine(ctx, b, direct))) break;
+                if ((rc = assemble_into(ctx, b, k, gf, direct, 1)) || (rc = check_status(ctx, b, gf))) break;
+                HCHK(hipMemcpyAsync(b->B, hB.data() + (size_t)n * r, sizeof(cplx) * n, hipMemcpyHostToDevice, st));
+                if ((rc = factor_and_solve(ctx, b, 1, direct))) break;
+                HCHK(hipMemcpyAsync(hphi.data() + (size_t)n * r, b->Y, sizeof(cplx) * n, hipMemcpyDeviceToHost, st));
+                HCHK(hipStreamSynchronize(st));
+            }
+            (void)phi_all;
+            if (rc) break;
+            HCHK(hipMemcpyAsync(b->Y, hphi.data(), sizeof(cplx) * n * R, hipMemcpyHostToDevice, st));
+        }
+        if (ndof > 0) {
+            Phase ph(ctx, "forces");
+            mhx::launch_forces(st, n, b->geom, b->dofs, ndof, b->Y, R, ctx->rho, w, k, ctx->g, b->betas, (int)nb, b->F,
+                               b->FK);
+        }
+        HCHK(hipGetLastError());
+        HCHK(hipMemcpyAsync(F.data(), b->F, sizeof(cplx) * ndof * R, hipMemcpyDeviceToHost, st));
+        if (nb) HCHK(hipMemcpyAsync(FK.data(), b->FK, sizeof(cplx) * ndof * nb, hipMemcpyDeviceToHost, st));
+        HCHK(hipStreamSynchronize(st));
+        // added_mass / radiation_damping (src/problems_and_results.jl:207-208), packing (:325-380)
+        for (int r = 0; r < R; r++)
+            for (int d = 0; d < ndof; d++) {
+                const cplx f = F[d + (size_t)ndof * r];
+                if (r < ndof) {
+                    A_out[d + (size_t)ndof * (r + (size_t)ndof * iw)] = f.x / (w * w);
+                    B_out[d + (size_t)ndof * (r + (size_t)ndof * iw)] = f.y / w;
+                } else {
+                    const size_t o = d + (size_t)ndof * (iw + (size_t)nw * (r - ndof));
+                    Fd_out[2 * o] = f.x;
+                    Fd_out[2 * o + 1] = f.y;
+                }
+            }
+        for (int bb = 0; bb < nb; bb++)
+            for (int d = 0; d < ndof; d++) {
+                const size_t o = d + (size_t)ndof * (iw + (size_t)nw * bb);
+                Fk_out[2 * o] = FK[d + (size_t)ndof * bb].x;
+                Fk_out[2 * o + 1] = FK[d + (size_t)ndof * bb].y;
+            }
+    }
+    flush_timing(ctx);
+    return rc;
+}
+
+int mhx_hydro_coeffs(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const double* dofs, int64_t nw,
+                     const double* omegas, int64_t nb, const double* betas, int gf, int direct, int mode, double* A_out,
+                     double* B_out, double* Fd_out, double* Fk_out) {
+    if (!ctx) return MHX_EINVAL;
+    mhx_body* b = nullptr;
+    int rc = mhx_body_create(ctx, mesh, ndof, dofs, &b);
+    if (rc) return rc;
+    rc = mhx_body_coeffs(ctx, b, nw, omegas, nb, betas, gf, direct, mode, A_out, B_out, Fd_out, Fk_out);
+    mhx_body_destroy(b);
+    return rc;
+}
+
+int mhx_phase_time(mhx_ctx* ctx, const char* phase, double* total_ms, int64_t* count) {
+    if (!ctx || !phase) return MHX_EINVAL;
+    flush_timing(ctx);
+    auto it = ctx->phases.find(phase);
+    if (total_ms) *total_ms = it == ctx->phases.end() ? 0.0 : it->second.ms;
+    if (count) *count = it == ctx->phases.end() ? 0 : it->second.count;
+    return MHX_OK;
+}
+
+void mhx_reset_timing(mhx_ctx* ctx) {
+    if (!ctx) return;
+    flush_timing(ctx);
+    ctx->phases.clear();
+    ctx->quad_work[0] = ctx->quad_work[1] = ctx->quad_work[2] = 0;
+}
+
+int mhx_exact_work(mhx_ctx* ctx, int64_t counts[3], double* flops) {
+    if (!ctx) return MHX_EINVAL;
+    if (counts)
+        for (int q = 0; q < 3; q++) counts[q] = (int64_t)ctx->quad_work[q];
+    if (flops)
+        *flops = MHX_FLOPS_PER_EVAL * ctx->quad_work[0] + MHX_FLOPS_PER_SERIES * ctx->quad_work[1] +
+                 MHX_FLOPS_PER_CF * ctx->quad_work[2];
+    return MHX_OK;
+}
+
+}  // extern "C"

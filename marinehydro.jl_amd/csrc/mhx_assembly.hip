@@ -1,0 +1,398 @@
+// Influence-matrix assembly, boundary conditions, right-hand-side products and
+// pressure integration for gfx950.
+//
+// Reference: src/matrix_assembly.jl:274-369 (assemble_birk_rankine,
+// assemble_wu_centers), :56-88 (broadcast assembly of the exact Delhommeau
+// term), src/waves.jl:170-303 (BCs, Froude-Krylov, pressure integration),
+// src/solve.jl:47-55.
+//
+// Layout: matrices are n x n column-major (Julia Matrix layout); a workgroup
+// owns 256 consecutive rows (field panels, one per thread) and TJ columns
+// (source panels) whose geometry is staged in LDS, so every store of a column
+// is one coalesced 2 KB (real) or 4 KB (complex) segment.
+#include "mhx_internal.h"
+
+namespace mhx {
+
+#define TJ 8
+static const double kScale = -1.0 / (4 * MHX_PI);
+
+__global__ void panel_geometry_kernel(int n, const double* __restrict__ verts, const int64_t* __restrict__ faces,
+                                      const double* __restrict__ centers, const double* __restrict__ normals,
+                                      const double* __restrict__ areas, const double* __restrict__ radii,
+                                      PanelGeom* __restrict__ geom) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double V[12];
+    for (int k = 0; k < 4; k++) {
+        int64_t v = faces[4 * j + k] - 1;
+        for (int c = 0; c < 3; c++) V[3 * k + c] = verts[3 * v + c];
+    }
+    PanelGeom g;
+    birk_panel_geometry(V, g);
+    for (int c = 0; c < 3; c++) {
+        g.c[c] = centers[3 * j + c];
+        g.nrm[c] = normals[3 * j + c];
+    }
+    g.radius = radii[j];
+    g.area = areas[j];
+    g.pad[0] = g.pad[1] = g.pad[2] = g.pad[3] = 0.0;
+    geom[j] = g;
+}
+
+void launch_panel_geometry(hipStream_t st, int n, const double* verts, const int64_t* faces, const double* centers,
+                           const double* normals, const double* areas, const double* radii, PanelGeom* geom) {
+    panel_geometry_kernel<<<(n + 127) / 128, 128, 0, st>>>(n, verts, faces, centers, normals, areas, radii, geom);
+}
+
+__device__ __forceinline__ void stage_columns(const PanelGeom* __restrict__ geom, int n, int j0, PanelGeom* sg) {
+    const double* src = reinterpret_cast<const double*>(geom + j0);
+    double* dst = reinterpret_cast<double*>(sg);
+    const int cnt = min(TJ, n - j0) * 32;
+    for (int t = threadIdx.x; t < cnt; t += blockDim.x) dst[t] = src[t];
+    __syncthreads();
+}
+
+// assemble_birk_rankine inner loop (matrix_assembly.jl:301-324) for the tuple
+// (Rankine(), RankineReflected()), without the identity jump.
+__global__ void __launch_bounds__(256) rankine_kernel(int n, const PanelGeom* __restrict__ geom, int direct,
+                                                      double* __restrict__ SR, double* __restrict__ DR) {
+    __shared__ PanelGeom sg[TJ];
+    const int j0 = blockIdx.y * TJ;
+    stage_columns(geom, n, j0, sg);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const PanelGeom& gi = geom[i];
+    const double c1[3] = {gi.c[0], gi.c[1], gi.c[2]};
+    const double c1r[3] = {gi.c[0], gi.c[1], -gi.c[2]};
+    const double ni[3] = {gi.nrm[0], gi.nrm[1], gi.nrm[2]};
+    const int jn = min(TJ, n - j0);
+#pragma unroll 1
+    for (int jj = 0; jj < jn; jj++) {
+        const PanelGeom& g = sg[jj];
+        const double* nv = direct ? g.nrm : ni;
+        double phi, gs[3], phir, gr[3];
+        vrankine(c1, g, phi, gs);
+        vrankine(c1r, g, phir, gr);
+        double s = 0.0, d = 0.0;
+        s += phi;
+        if (direct) {
+            d += nv[0] * gs[0] + nv[1] * gs[1] + nv[2] * gs[2];
+        } else {
+            d += nv[0] * (-gs[0]) + nv[1] * (-gs[1]) + nv[2] * (-gs[2]);
+            gr[0] = -gr[0];
+            gr[1] = -gr[1];
+        }
+        s += 1.0 * phir;
+        d += 1.0 * (nv[0] * gr[0] + nv[1] * gr[1] + nv[2] * gr[2]);
+        const int64_t idx = i + (int64_t)n * (j0 + jj);
+        SR[idx] = kScale * s;
+        DR[idx] = kScale * d;
+    }
+}
+
+void launch_rankine(hipStream_t st, int n, const PanelGeom* geom, int direct, double* SR, double* DR) {
+    dim3 grid((n + 255) / 256, (n + TJ - 1) / TJ);
+    rankine_kernel<<<grid, 256, 0, st>>>(n, geom, direct, SR, DR);
+}
+
+// _wu_sd (wu.jl:112-127) + the scale of _assemble_wu_centers_loop (matrix_assembly.jl:343-369)
+__global__ void __launch_bounds__(256) combine_wu_kernel(int n, const PanelGeom* __restrict__ geom,
+                                                         const double* __restrict__ SR, const double* __restrict__ DR,
+                                                         double k, int direct, int include_identity,
+                                                         cplx* __restrict__ S, cplx* __restrict__ D) {
+    __shared__ PanelGeom sg[TJ];
+    const int j0 = blockIdx.y * TJ;
+    stage_columns(geom, n, j0, sg);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const PanelGeom& gi = geom[i];
+    const double x0 = gi.c[0], x1 = gi.c[1], x2 = gi.c[2];
+    const double ni0 = gi.nrm[0], ni1 = gi.nrm[1], ni2 = gi.nrm[2];
+    const int jn = min(TJ, n - j0);
+#pragma unroll 1
+    for (int jj = 0; jj < jn; jj++) {
+        const PanelGeom& g = sg[jj];
+        const int j = j0 + jj;
+        double hh, vv;
+        reduced_hv(x0, x1, x2, g.c[0], g.c[1], g.c[2], k, hh, vv);
+        cplx G, Gh, Gv;
+        wu_term(hh, vv, G, Gh, Gv);
+        double dx1 = direct ? g.c[0] - x0 : x0 - g.c[0];
+        double dx2 = direct ? g.c[1] - x1 : x1 - g.c[1];
+        double dr1 = 0.0, dr2 = 0.0;
+        if (fabs(hh) > 1e-6) {
+            dr1 = k * k / hh * dx1;
+            dr2 = k * k / hh * dx2;
+        }
+        const double n0 = direct ? g.nrm[0] : ni0, n1 = direct ? g.nrm[1] : ni1, n2 = direct ? g.nrm[2] : ni2;
+        cplx g0 = cscale(k, cscale(dr1, Gh)), g1 = cscale(k, cscale(dr2, Gh)), g2 = cscale(k, cscale(k, Gv));
+        cplx s = cscale(g.area, cscale(k, G));
+        cplx dsum = cadd(cadd(cscale(n0, g0), cscale(n1, g1)), cscale(n2, g2));
+        cplx d = cscale(g.area, dsum);
+        const int64_t idx = i + (int64_t)n * j;
+        double sr = SR[idx], dr = DR[idx];
+        if (include_identity && i == j) dr += 0.5;
+        S[idx] = cmk(sr + kScale * s.x, kScale * s.y);
+        D[idx] = cmk(dr + kScale * d.x, kScale * d.y);
+    }
+}
+
+void launch_combine_wu(hipStream_t st, int n, const PanelGeom* geom, const double* SR, const double* DR, double k,
+                       int direct, int include_identity, cplx* S, cplx* D) {
+    dim3 grid((n + 255) / 256, (n + TJ - 1) / TJ);
+    combine_wu_kernel<<<grid, 256, 0, st>>>(n, geom, SR, DR, k, direct, include_identity, S, D);
+}
+
+// Exact Guevel-Delhommeau dimensionless (G, dG/dr) per unordered panel pair.
+// The reduced coordinates are bitwise symmetric in (i, j), so one evaluation
+// serves both matrix entries.
+__global__ void __launch_bounds__(256) delhommeau_table_kernel(int n, const PanelGeom* __restrict__ geom, double k,
+                                                               double rtol, double atol, cplx* __restrict__ WG,
+                                                               cplx* __restrict__ WR,
+                                                               unsigned long long* __restrict__ status) {
+    if (blockIdx.x > blockIdx.y) return;
+    const int i = blockIdx.x * 16 + threadIdx.x;
+    const int j = blockIdx.y * 16 + threadIdx.y;
+    QuadWork wk;
+    if (i < n && j < n && i <= j) {
+        const PanelGeom& gi = geom[i];
+        const PanelGeom& gj = geom[j];
+        double hh, vv;
+        reduced_hv(gi.c[0], gi.c[1], gi.c[2], gj.c[0], gj.c[1], gj.c[2], k, hh, vv);
+        cplx Gr, G;
+        bool ok1 = hquad<true>(hh, vv, rtol, atol, Gr, wk);
+        bool ok0 = hquad<false>(hh, vv, rtol, atol, G, wk);
+        if (!(ok0 && ok1)) atomicOr(status, 1ull);
+        G = cscale(4.0, G);
+        Gr = cscale(4.0, Gr);
+        WG[i + (int64_t)n * j] = G;
+        WR[i + (int64_t)n * j] = Gr;
+        if (i != j) {
+            WG[j + (int64_t)n * i] = G;
+            WR[j + (int64_t)n * i] = Gr;
+        }
+    }
+    // one atomic per wave for the work tally
+    unsigned long long e = wk.evals, se = wk.series, cf = wk.cf;
+    for (int off = 32; off > 0; off >>= 1) {
+        e += __shfl_xor(e, off);
+        se += __shfl_xor(se, off);
+        cf += __shfl_xor(cf, off);
+    }
+    const int lane = (threadIdx.y * 16 + threadIdx.x) & 63;
+    if (lane == 0 && e) {
+        atomicAdd(status + 1, e);
+        atomicAdd(status + 2, se);
+        atomicAdd(status + 3, cf);
+    }
+}
+
+void launch_delhommeau_table(hipStream_t st, int n, const PanelGeom* geom, double k, double rtol, double atol, cplx* WG,
+                             cplx* WR, unsigned long long* status) {
+    dim3 grid((n + 15) / 16, (n + 15) / 16);
+    delhommeau_table_kernel<<<grid, dim3(16, 16), 0, st>>>(n, geom, k, rtol, atol, WG, WR, status);
+}
+
+// with_reduced_coordinates_derivative (abstract_greens_function.jl:140-185) and
+// the one-point integrals (exact_Guevel_Delhommeau.jl:61-75) from a (G, dG/dr) table
+__global__ void __launch_bounds__(256) combine_table_kernel(int n, const PanelGeom* __restrict__ geom,
+                                                            const double* __restrict__ SR, const double* __restrict__ DR,
+                                                            double k, int direct, int include_identity,
+                                                            const cplx* __restrict__ WG, const cplx* __restrict__ WR,
+                                                            cplx* __restrict__ S, cplx* __restrict__ D) {
+    __shared__ PanelGeom sg[TJ];
+    const int j0 = blockIdx.y * TJ;
+    stage_columns(geom, n, j0, sg);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const PanelGeom& gi = geom[i];
+    const double x0 = gi.c[0], x1 = gi.c[1], x2 = gi.c[2];
+    const double ni0 = gi.nrm[0], ni1 = gi.nrm[1], ni2 = gi.nrm[2];
+    const int jn = min(TJ, n - j0);
+#pragma unroll 1
+    for (int jj = 0; jj < jn; jj++) {
+        const PanelGeom& g = sg[jj];
+        const int j = j0 + jj;
+        const int64_t idx = i + (int64_t)n * j;
+        double hh, vv;
+        reduced_hv(x0, x1, x2, g.c[0], g.c[1], g.c[2], k, hh, vv);
+        const cplx G = WG[idx], Gh = WR[idx];
+        const double inv = 2.0 / hypot(hh, vv);
+        const cplx Gv = cmk(G.x + inv, G.y);
+        double dx1 = direct ? g.c[0] - x0 : x0 - g.c[0];
+        double dx2 = direct ? g.c[1] - x1 : x1 - g.c[1];
+        double dr1 = 0.0, dr2 = 0.0;
+        if (fabs(hh) > 1e-6) {
+            dr1 = k * k / hh * dx1;
+            dr2 = k * k / hh * dx2;
+        }
+        const double n0 = direct ? g.nrm[0] : ni0, n1 = direct ? g.nrm[1] : ni1, n2 = direct ? g.nrm[2] : ni2;
+        const double a2 = g.area;
+        cplx g0 = cscale(k, cscale(dr1, Gh)), g1 = cscale(k, cscale(dr2, Gh)), g2 = cscale(k, cscale(k, Gv));
+        cplx s = cscale(a2, cscale(k, G));
+        cplx d = cadd(cadd(cscale(n0, cscale(a2, g0)), cscale(n1, cscale(a2, g1))), cscale(n2, cscale(a2, g2)));
+        double sr = SR[idx], dr = DR[idx];
+        if (include_identity && i == j) dr += 0.5;
+        S[idx] = cmk(sr + kScale * s.x, kScale * s.y);
+        D[idx] = cmk(dr + kScale * d.x, kScale * d.y);
+    }
+}
+
+void launch_combine_table(hipStream_t st, int n, const PanelGeom* geom, const double* SR, const double* DR, double k,
+                          int direct, int include_identity, const cplx* WG, const cplx* WR, cplx* S, cplx* D) {
+    dim3 grid((n + 255) / 256, (n + TJ - 1) / TJ);
+    combine_table_kernel<<<grid, 256, 0, st>>>(n, geom, SR, DR, k, direct, include_identity, WG, WR, S, D);
+}
+
+// ---------------------------------------------------------------------------
+// boundary conditions: radiation_bc (waves.jl:183-211), _airy_bc (waves.jl:266-282)
+// ---------------------------------------------------------------------------
+__global__ void bc_kernel(int n, const PanelGeom* __restrict__ geom, const double* __restrict__ dofs, int ndof,
+                          const double* __restrict__ betas, int nb, double omega, double k, double g,
+                          cplx* __restrict__ B) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const PanelGeom& gi = geom[i];
+    const double* nv = gi.nrm;
+    for (int r = 0; r < ndof; r++) {
+        const double* dof = dofs + (int64_t)3 * n * r;
+        double disp = nv[0] * dof[i] + nv[1] * dof[i + n] + nv[2] * dof[i + 2 * n];
+        B[i + (int64_t)n * r] = cmk(-0.0 * omega * disp, -omega * disp);
+    }
+    for (int b = 0; b < nb; b++) {
+        const double beta = betas[b];
+        const double cb = cos(beta), sb = sin(beta);
+        const double wbar = gi.c[0] * cb + gi.c[1] * sb;
+        const double cih = exp(k * gi.c[2]);
+        double s, c;
+        sincos(k * wbar, &s, &c);
+        const double am = g * k / omega;
+        const cplx amp = cmk(am * c, am * s);
+        const cplx v0 = cscale(cih, cscale(cb, amp)), v1 = cscale(cih, cscale(sb, amp));
+        const cplx v2 = cscale(cih, cmk(amp.y, -amp.x));  // -i * amp
+        const cplx dot = cadd(cadd(cscale(nv[0], v0), cscale(nv[1], v1)), cscale(nv[2], v2));
+        B[i + (int64_t)n * (ndof + b)] = cmk(-dot.x, -dot.y);
+    }
+}
+
+void launch_bc(hipStream_t st, int n, const PanelGeom* geom, const double* dofs, int ndof, const double* betas, int nb,
+               double omega, double k, double g, cplx* B) {
+    bc_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, geom, dofs, ndof, betas, nb, omega, k, g, B);
+}
+
+// ---------------------------------------------------------------------------
+// Y = S X for R right-hand sides (_mulvec, matrix_assembly.jl:380-390).  HBM-bound
+// on S: each thread owns a row, the j range is split across NSPLIT blocks.
+// ---------------------------------------------------------------------------
+#define NSPLIT 16
+#define XCH 64
+
+template <int RB>
+__global__ void __launch_bounds__(256) matmul_nr_kernel(int n, int R, const cplx* __restrict__ S,
+                                                        const cplx* __restrict__ X, cplx* __restrict__ part) {
+    __shared__ cplx xs[XCH][RB];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int s = blockIdx.y;
+    const int per = (n + NSPLIT - 1) / NSPLIT;
+    const int jb = s * per, je = min(n, jb + per);
+    cplx acc[RB];
+#pragma unroll
+    for (int c = 0; c < RB; c++) acc[c] = cmk(0.0, 0.0);
+    for (int j0 = jb; j0 < je; j0 += XCH) {
+        const int cnt = min(XCH, je - j0);
+        __syncthreads();
+        for (int t = threadIdx.x; t < XCH * RB; t += 256) {
+            int jj = t % XCH, c = t / XCH;
+            xs[jj][c] = (jj < cnt && c < R) ? X[(j0 + jj) + (int64_t)n * c] : cmk(0.0, 0.0);
+        }
+        __syncthreads();
+        if (i < n)
+            for (int jj = 0; jj < cnt; jj++) {
+                const cplx a = S[i + (int64_t)n * (j0 + jj)];
+#pragma unroll
+                for (int c = 0; c < RB; c++) acc[c] = cadd(acc[c], cmul(a, xs[jj][c]));
+            }
+    }
+    if (i < n)
+#pragma unroll
+        for (int c = 0; c < RB; c++)
+            if (c < R) part[((int64_t)s * R + c) * n + i] = acc[c];
+}
+
+__global__ void reduce_parts_kernel(int n, int R, const cplx* __restrict__ part, cplx* __restrict__ Y) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n * R) return;
+    cplx s = cmk(0.0, 0.0);
+    for (int q = 0; q < NSPLIT; q++) s = cadd(s, part[(int64_t)q * n * R + t]);
+    Y[t] = s;
+}
+
+void launch_matmul_nr(hipStream_t st, int n, int R, const cplx* S, const cplx* X, cplx* Y, cplx* work) {
+    dim3 grid((n + 255) / 256, NSPLIT);
+    if (R <= 4)
+        matmul_nr_kernel<4><<<grid, 256, 0, st>>>(n, R, S, X, work);
+    else if (R <= 8)
+        matmul_nr_kernel<8><<<grid, 256, 0, st>>>(n, R, S, X, work);
+    else
+        matmul_nr_kernel<16><<<grid, 256, 0, st>>>(n, R, S, X, work);
+    const int64_t tot = (int64_t)n * R;
+    reduce_parts_kernel<<<(tot + 255) / 256, 256, 0, st>>>(n, R, work, Y);
+}
+
+// ---------------------------------------------------------------------------
+// forces: integrate_pressure (waves.jl:154-178) with p = i rho omega phi
+// (solve.jl:50) and the Froude-Krylov pressure (waves.jl:259-264, 294-303)
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) forces_kernel(int n, const PanelGeom* __restrict__ geom,
+                                                     const double* __restrict__ dofs, int ndof,
+                                                     const cplx* __restrict__ phi, int R, double rho, double omega,
+                                                     double k, double g, const double* __restrict__ betas, int nb,
+                                                     cplx* __restrict__ F, cplx* __restrict__ FK) {
+    const int d = blockIdx.x;       // influenced dof
+    const int r = blockIdx.y;       // right-hand side (r < R) or FK heading (R + b)
+    const double* dof = dofs + (int64_t)3 * n * d;
+    const double ro = rho * omega;
+    cplx acc = cmk(0.0, 0.0);
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const PanelGeom& gi = geom[i];
+        double disp = dof[i] * gi.nrm[0] + dof[i + n] * gi.nrm[1] + dof[i + 2 * n] * gi.nrm[2];
+        cplx p;
+        if (r < R) {
+            const cplx ph = phi[i + (int64_t)n * r];
+            p = cmk(-ro * ph.y, ro * ph.x);
+        } else {
+            const double beta = betas[r - R];
+            const double wbar = gi.c[0] * cos(beta) + gi.c[1] * sin(beta);
+            double s, c;
+            sincos(k * wbar, &s, &c);
+            const double amp = g / omega * exp(k * gi.c[2]);
+            const cplx ph = cmk(amp * s, -amp * c);  // (-i g/omega) e^{kz} e^{i k wbar}
+            p = cmk(-omega * rho * ph.y, omega * rho * ph.x);
+        }
+        acc = cadd(acc, cscale(-disp * gi.area, p));
+    }
+    __shared__ double red[2][256];
+    red[0][threadIdx.x] = acc.x;
+    red[1][threadIdx.x] = acc.y;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + off];
+            red[1][threadIdx.x] += red[1][threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if (r < R) F[d + (int64_t)ndof * r] = cmk(red[0][0], red[1][0]);
+        else FK[d + (int64_t)ndof * (r - R)] = cmk(red[0][0], red[1][0]);
+    }
+}
+
+void launch_forces(hipStream_t st, int n, const PanelGeom* geom, const double* dofs, int ndof, const cplx* phi, int R,
+                   double rho, double omega, double k, double g, const double* betas, int nb, cplx* F, cplx* FK) {
+    dim3 grid(ndof, R + nb);
+    forces_kernel<<<grid, 256, 0, st>>>(n, geom, dofs, ndof, phi, R, rho, omega, k, g, betas, nb, F, FK);
+}
+
+}  // namespace mhx

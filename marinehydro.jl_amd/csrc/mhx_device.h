@@ -1,0 +1,426 @@
+// Device-side math for the BEM hot path on gfx950 (FP64).
+//
+//  * Birk (2021) constant-source quad panel potential and velocity
+//    (reference: src/green_functions/rankine_vectorized.jl:42-137, 241-287)
+//  * Liang-Wu-Noblesse wave term fits (src/green_functions/wu.jl:7-37, 131-435)
+//  * complex exponential integral and the exact Guevel-Delhommeau wave term
+//    integrated by adaptive Gauss-Kronrod (7,15) bisection with the
+//    termination rule of HCubature's hquadrature
+//    (src/green_functions/exact_Guevel_Delhommeau.jl:8-59)
+//
+// Every formula keeps the reference's branch structure (thresholds, signed-zero
+// behaviour of the atan2 rewrite, the 1e-18 perturbation of coincident
+// horizontal positions) so that results agree with the CPU restatement to
+// rounding.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define MHX_PI 3.14159265358979323846
+#define MHX_EULER 0.57721566490153286060651209
+
+typedef double2 cplx;
+
+__device__ __forceinline__ cplx cmk(double r, double i) { return make_double2(r, i); }
+__device__ __forceinline__ cplx cadd(cplx a, cplx b) { return cmk(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cplx csub(cplx a, cplx b) { return cmk(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cplx cmul(cplx a, cplx b) { return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ cplx cscale(double s, cplx a) { return cmk(s * a.x, s * a.y); }
+__device__ __forceinline__ double cabs_(cplx a) { return hypot(a.x, a.y); }
+// Smith's division
+__device__ __forceinline__ cplx cdiv(cplx a, cplx b) {
+    if (fabs(b.x) >= fabs(b.y)) {
+        double r = b.y / b.x, den = b.x + b.y * r;
+        return cmk((a.x + a.y * r) / den, (a.y - a.x * r) / den);
+    }
+    double r = b.x / b.y, den = b.x * r + b.y;
+    return cmk((a.x * r + a.y) / den, (a.y * r - a.x) / den);
+}
+__device__ __forceinline__ cplx cexp_(cplx z) {
+    double s, c;
+    sincos(z.y, &s, &c);
+    double e = exp(z.x);
+    return cmk(e * c, e * s);
+}
+__device__ __forceinline__ cplx clog_(cplx z) { return cmk(log(hypot(z.x, z.y)), atan2(z.y, z.x)); }
+
+// ---------------------------------------------------------------------------
+// Source-panel geometry, precomputed once per panel and staged in LDS.
+// ---------------------------------------------------------------------------
+struct PanelGeom {
+    double T[9];    // column-major [s t n] local frame (bimedian construction)
+    double qgc[3];  // area centroid
+    double lc[8];   // local corners (x, y) relative to qgc
+    double c[3];    // mesh centre (far-field source point)
+    double radius;
+    double area;    // mesh area (far field and wave term)
+    double nrm[3];  // mesh normal
+    double pad[4];
+};
+static_assert(sizeof(PanelGeom) == 32 * sizeof(double), "PanelGeom must be 256 bytes");
+
+__device__ __forceinline__ double atan_ratio(double num, double den) {
+    double flipped = (den < 0.0) ? -num : num;
+    double ad = (den < 0.0) ? -den : den;
+    return atan2(flipped, ad);
+}
+
+__device__ __forceinline__ void birk_panel_geometry(const double V[12], PanelGeom& g) {
+    double mid[4][3];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        int j = (i + 1) & 3;
+#pragma unroll
+        for (int c = 0; c < 3; c++) mid[i][c] = (V[3 * i + c] + V[3 * j + c]) / 2;
+    }
+    double mgc[3], s[3], tt[3], n[3], t[3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        mgc[c] = (mid[1][c] + mid[3][c]) / 2;
+        s[c] = mid[1][c] - mid[3][c];
+        tt[c] = mid[2][c] - mid[0][c];
+    }
+    double ns = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+    for (int c = 0; c < 3; c++) s[c] /= ns;
+    n[0] = s[1] * tt[2] - s[2] * tt[1];
+    n[1] = s[2] * tt[0] - s[0] * tt[2];
+    n[2] = s[0] * tt[1] - s[1] * tt[0];
+    double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for (int c = 0; c < 3; c++) n[c] /= nn;
+    t[0] = n[1] * s[2] - n[2] * s[1];
+    t[1] = n[2] * s[0] - n[0] * s[2];
+    t[2] = n[0] * s[1] - n[1] * s[0];
+    double nt = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+    for (int c = 0; c < 3; c++) t[c] /= nt;
+    for (int c = 0; c < 3; c++) {
+        g.T[c] = s[c];
+        g.T[3 + c] = t[c];
+        g.T[6 + c] = n[c];
+    }
+    double tc[4][2];
+    for (int k = 0; k < 4; k++) {
+        double d0 = V[3 * k] - mgc[0], d1 = V[3 * k + 1] - mgc[1], d2 = V[3 * k + 2] - mgc[2];
+        tc[k][0] = d0 * s[0] + d1 * s[1] + d2 * s[2];
+        tc[k][1] = d0 * t[0] + d1 * t[1] + d2 * t[2];
+    }
+    double A = 0, Mx = 0, My = 0;
+    for (int k = 0; k < 4; k++) {
+        int k1 = (k + 1) & 3;
+        double xk = tc[k][0], yk = tc[k][1], xk1 = tc[k1][0], yk1 = tc[k1][1];
+        A += (yk1 - yk) * (xk + xk1) / 2;
+        Mx += (xk1 - xk) * (yk * yk + yk * yk1 + yk1 * yk1) / 6;
+        My += (yk1 - yk) * (xk * xk + xk * xk1 + xk1 * xk1) / 6;
+    }
+    double clx = My / A, cly = -Mx / A;
+    for (int c = 0; c < 3; c++) g.qgc[c] = mgc[c] + clx * s[c] + cly * t[c];
+    for (int k = 0; k < 4; k++) {
+        g.lc[2 * k] = tc[k][0] - clx;
+        g.lc[2 * k + 1] = tc[k][1] - cly;
+    }
+}
+
+// birk_phi_and_velocity (rankine_vectorized.jl:241-267), including the -1/(4 pi) scale.
+__device__ __forceinline__ void birk_phi_v(double x, double y, double z, const double* lc, double& phi,
+                                           double v[3]) {
+    double sx[4], sy[4], r[4], e[4], h[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        sx[k] = x - lc[2 * k];
+        sy[k] = y - lc[2 * k + 1];
+        r[k] = sqrt(sx[k] * sx[k] + sy[k] * sy[k] + z * z);
+        e[k] = sx[k] * sx[k] + z * z;
+        h[k] = sx[k] * sy[k];
+    }
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    const bool inplane = fabs(z) <= 1e-14;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int kn = (k + 1) & 3;
+        double dx = lc[2 * kn] - lc[2 * k], dy = lc[2 * kn + 1] - lc[2 * k + 1];
+        double dk = hypot(dx, dy);
+        double rsum = r[k] + r[kn];
+        double lt_dk = log((rsum - dk) / (rsum + dk)) / dk;
+        double datan = atan_ratio(dy * e[k] - dx * h[k], dx * z * r[k]) -
+                       atan_ratio(dy * e[kn] - dx * h[kn], dx * z * r[kn]);
+        double vzk = inplane ? (datan - datan) : datan;
+        double phik = (sx[k] * dy - sy[k] * dx) * lt_dk + z * vzk;
+        if (!(dk <= 1e-14)) {
+            s0 += phik;
+            s1 += dy * lt_dk;
+            s2 += -dx * lt_dk;
+            s3 += vzk;
+        }
+    }
+    const double scale = -1.0 / (4 * MHX_PI);
+    phi = scale * s0;
+    v[0] = scale * s1;
+    v[1] = scale * s2;
+    v[2] = scale * s3;
+}
+
+// _vrankine_from_geom (rankine_vectorized.jl:273-287): integral of 1/|p - xi| over
+// the source panel and its gradient with respect to the source point.
+__device__ __forceinline__ void vrankine(const double p[3], const PanelGeom& g, double& phi, double grad[3]) {
+    double rb0 = p[0] - g.c[0], rb1 = p[1] - g.c[1], rb2 = p[2] - g.c[2];
+    double r = sqrt(rb0 * rb0 + rb1 * rb1 + rb2 * rb2);
+    if (r > 7 * g.radius) {
+        double r3 = r * r * r;
+        phi = g.area / r;
+        grad[0] = (rb0 / r3) * g.area;
+        grad[1] = (rb1 / r3) * g.area;
+        grad[2] = (rb2 / r3) * g.area;
+        return;
+    }
+    double d0 = p[0] - g.qgc[0], d1 = p[1] - g.qgc[1], d2 = p[2] - g.qgc[2];
+    double fl0 = g.T[0] * d0 + g.T[1] * d1 + g.T[2] * d2;
+    double fl1 = g.T[3] * d0 + g.T[4] * d1 + g.T[5] * d2;
+    double fl2 = g.T[6] * d0 + g.T[7] * d1 + g.T[8] * d2;
+    double pb, vl[3];
+    birk_phi_v(fl0, fl1, fl2, g.lc, pb, vl);
+    phi = -4 * MHX_PI * pb;
+#pragma unroll
+    for (int q = 0; q < 3; q++) grad[q] = 4 * MHX_PI * (g.T[q] * vl[0] + g.T[3 + q] * vl[1] + g.T[6 + q] * vl[2]);
+}
+
+// ---------------------------------------------------------------------------
+// Liang-Wu-Noblesse fits (wu.jl:131-435)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double struve_h0(double xx) {
+    if (xx <= 3.0) {
+        double yy = (xx / 3.0) * (xx / 3.0);
+        double h = 1.909859164 +
+                   (-1.909855001 + (0.687514637 + (-0.126164557 + (0.013828813 + -0.000876918 * yy) * yy) * yy) * yy) * yy;
+        return h * (xx / 3.0);
+    }
+    double yy = (3.0 / xx) * (3.0 / xx);
+    double c1 = 2.0 * (0.99999906 + (4.77228920 + (3.85542044 + 0.32303607 * yy) * yy) * yy);
+    double c2 = MHX_PI * xx * (1.0 + (4.88331068 + (4.28957333 + 0.52120508 * yy) * yy) * yy);
+    return c1 / c2 + y0(xx);
+}
+__device__ __forceinline__ double struve_h1(double xx) {
+    if (xx <= 3.0) {
+        double yy = (xx / 3.0) * (xx / 3.0);
+        return (1.909859286 +
+                (-1.145914713 + (0.294656958 + (-0.042070508 + (0.003785727 + -0.000207183 * yy) * yy) * yy) * yy) * yy) * yy;
+    }
+    double yy = (3.0 / xx) * (3.0 / xx);
+    double c1 = 2.0 * (1.00000004 + (3.92205313 + (2.64893033 + 0.27450895 * yy) * yy) * yy);
+    double c2 = MHX_PI * (1.0 + (3.81095112 + (2.26216956 + 0.10885141 * yy) * yy) * yy);
+    return c1 / c2 + y1(xx);
+}
+
+template <int DEG>
+__device__ __forceinline__ double horner_wu(const double* c, double t) {
+    double acc = (c[DEG - 1] + c[DEG] * t) * t;
+#pragma unroll
+    for (int i = DEG - 2; i >= 1; i--) acc = (c[i] + acc) * t;
+    return c[0] + acc;
+}
+
+__constant__ static const double WU_A[10] = {1.21, -13.328, 215.896, -1763.96, 8418.94, -24314.21, 42002.57, -41592.9, 21859.0, -4838.6};
+__constant__ static const double WU_B[10] = {0.938, 5.737, -67.92, 796.534, -4780.77, 17137.74, -36618.81, 44894.06, -29030.24, 7671.22};
+__constant__ static const double WU_C[8] = {1.268, -9.747, 209.653, -1397.89, 5155.67, -9844.35, 9136.4, -3272.62};
+__constant__ static const double WU_D[10] = {0.632, -40.97, 667.16, -6072.07, 31127.39, -96293.05, 181856.75, -205690.43, 128170.2, -33744.6};
+__constant__ static const double WU_DA[10] = {2.948, -24.53, 249.69, -754.85, -1187.71, 16370.75, -48811.41, 68220.87, -46688.0, 12622.25};
+__constant__ static const double WU_DB[10] = {1.11, 2.894, -76.765, 1565.35, -11336.19, 44270.15, -97014.11, 118879.26, -76209.82, 19923.28};
+__constant__ static const double WU_DC[6] = {14.19, -148.24, 847.8, -2318.58, 3168.35, -1590.27};
+
+// _wu_wave_term_and_derivs (wu.jl:28-37): G, dG/dh, dG/dv of the dimensionless wave term.
+__device__ __noinline__ void wu_term(double hh, double vv, cplx& G, cplx& Gh, cplx& Gv) {
+    double dd = sqrt(hh * hh + vv * vv);
+    double alpha = -vv / dd, beta = hh / dd, rho = dd / (1 + dd);
+    double ev = exp(vv);
+    double omr3 = (1.0 - rho) * (1.0 - rho) * (1.0 - rho);
+    double dd2 = dd * dd, dd3 = dd * dd * dd;
+    double A = horner_wu<9>(WU_A, rho), B = horner_wu<9>(WU_B, rho), C = horner_wu<7>(WU_C, rho),
+           D = horner_wu<9>(WU_D, rho);
+    double RR = (1.0 - beta) * A;
+    RR -= beta * B;
+    RR -= alpha * C / (1.0 + 6.0 * alpha * rho * (1.0 - rho));
+    RR += beta * (1.0 - beta) * D;
+    double Lp = rho * omr3 * RR;
+    double PP = log(0.5 * (dd - vv)) + 0.5772156649 - 2.0 * dd2;
+    PP = ev * PP;
+    PP = PP + dd2 - vv;
+    double L0 = 2.0 * PP / (1.0 + dd3) + 2.0 * Lp;
+    double H0 = struve_h0(hh), J0 = j0(hh);
+    double Wr = 2 * MHX_PI * H0 * ev, Wi = 2 * MHX_PI * (-J0) * ev;
+    double dA = horner_wu<9>(WU_DA, rho), dB = horner_wu<9>(WU_DB, rho), dC = horner_wu<5>(WU_DC, rho);
+    double RS = beta * dA;
+    RS -= (1.0 - alpha) * dB;
+    RS += beta * (1.0 - beta) * rho * (1.0 - 2.0 * rho) * dC;
+    double Lsp = rho * omr3 * RS;
+    double PS = (beta + hh) / (dd - vv);
+    PS = PS - 2.0 * beta + 2.0 * ev * dd - hh;
+    double QS = exp(-dd) * (1.0 - beta);
+    QS = QS * (1.0 + dd / (1.0 + dd3));
+    double Ls = 2.0 * PS / (1.0 + dd3) - 4.0 * QS + 2.0 * Lsp;
+    double H1 = struve_h1(hh), J1 = j1(hh);
+    double Whr = 2 * MHX_PI * (2 / MHX_PI - H1) * ev, Whi = 2 * MHX_PI * J1 * ev;
+    G = cmk(-L0 - Wr, -Wi);
+    Gh = cmk(-Ls - Whr, -Whi);
+    Gv = cmk(G.x + 2 / dd, G.y);
+}
+
+// reduced coordinates (wu.jl:7-18; abstract_greens_function.jl:117-138)
+__device__ __forceinline__ void reduced_hv(double x0, double x1, double x2, double y0_, double y1_, double y2_,
+                                           double k, double& hh, double& vv) {
+    double d2 = (x0 - y0_) * (x0 - y0_) + (x1 - y1_) * (x1 - y1_);
+    double dis;
+    if (d2 == 0.0)
+        dis = sqrt(d2 + 1e-18 * 1e-18) - 1e-18;
+    else
+        dis = sqrt(d2);
+    hh = k * dis;
+    vv = k * (x2 + y2_);
+}
+
+// ---------------------------------------------------------------------------
+// exp(z) E1(z).  Region rule: power series for |z|<=2, for Re z<=0 with |z|<=4,
+// and inside the cone |z|<=40, |Im z| <= -Re z (24/|z| - 0.1); elsewhere the even
+// continued fraction 1/(z+1- 1/(z+3- 4/(z+5- ...))) by modified Lentz.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool e1_use_series(cplx z) {
+    double R = hypot(z.x, z.y), y = fabs(z.y);
+    if (R <= 2.0) return true;
+    if (z.x <= 0.0 && R <= 4.0) return true;
+    if (z.x < 0.0 && R <= 40.0 && y <= (-z.x) * (24.0 / R - 0.1)) return true;
+    return false;
+}
+
+// per-thread tally of the quadrature work (integrand evaluations, power-series
+// terms, continued-fraction steps) used for the algorithmic FLOP count
+struct QuadWork {
+    unsigned evals = 0, series = 0, cf = 0;
+};
+
+__device__ __noinline__ cplx expE1(cplx z, QuadWork& wk) {
+    if (e1_use_series(z)) {
+        cplx term = cmk(1.0, 0.0), sum = cmk(0.0, 0.0);
+        cplx mz = cmk(-z.x, -z.y);
+        for (int n = 1; n < 600; n++) {
+            double inv = 1.0 / (double)n;
+            term = cscale(inv, cmul(term, mz));
+            cplx add = cscale(inv, term);
+            sum = cadd(sum, add);
+            wk.series++;
+            if (hypot(add.x, add.y) <= 1e-17 * hypot(sum.x, sum.y)) break;
+        }
+        cplx L = clog_(z);
+        cplx e1 = cmk(-MHX_EULER - L.x - sum.x, -L.y - sum.y);
+        return cmul(cexp_(z), e1);
+    }
+    const double tiny = 1e-300;
+    cplx f = cmk(z.x + 1.0, z.y);
+    if (f.x == 0.0 && f.y == 0.0) f = cmk(tiny, 0.0);
+    cplx C = f, Dd = cmk(0.0, 0.0);
+    for (int n = 1; n < 600; n++) {
+        double a = -(double)n * (double)n;
+        cplx b = cmk(z.x + (double)(2 * n + 1), z.y);
+        Dd = cadd(b, cscale(a, Dd));
+        if (Dd.x == 0.0 && Dd.y == 0.0) Dd = cmk(tiny, 0.0);
+        Dd = cdiv(cmk(1.0, 0.0), Dd);
+        C = cadd(b, cscale(a, cdiv(cmk(1.0, 0.0), C)));
+        if (C.x == 0.0 && C.y == 0.0) C = cmk(tiny, 0.0);
+        cplx d = cmul(C, Dd);
+        f = cmul(f, d);
+        wk.cf++;
+        if (hypot(d.x - 1.0, d.y) < 1e-16) break;
+    }
+    return cdiv(cmk(1.0, 0.0), f);
+}
+
+// integrands of _dimless_wave_term (exact_Guevel_Delhommeau.jl:16-23) and of
+// _d_dimless_wave_term_dr (:36-43)
+template <bool DERIV>
+__device__ __forceinline__ cplx del_integrand(double th, double r, double z, QuadWork& wk) {
+    double c = cos(th);
+    cplx zeta = cmk(z, r * c);
+    cplx ez = cexp_(zeta);
+    wk.evals++;
+    cplx w = expE1(zeta, wk);
+    cplx j = cmk(w.x / MHX_PI - ez.y, w.y / MHX_PI + ez.x);  // w/pi + i e^zeta
+    if (!DERIV) return cmk(j.x, ez.x);
+    cplx inv = cdiv(cmk(1.0, 0.0), cmk(MHX_PI * zeta.x, MHX_PI * zeta.y));
+    double qi = j.y - inv.y;
+    return cmk(-c * qi, -c * ez.y);
+}
+
+__constant__ static const double GK_X[8] = {0.991455371120812639206854697526329, 0.949107912342758524526189684047851,
+                                            0.864864423359769072789712788640926, 0.741531185599394439863864773280788,
+                                            0.586087235467691130294144845693013, 0.405845151377397166906606412076961,
+                                            0.207784955007898467600689403773245, 0.0};
+__constant__ static const double GK_WK[8] = {0.022935322010529224963732008058970, 0.063092092629978553290700663189204,
+                                             0.104790010322250183839876322541518, 0.140653259715525918745189590510238,
+                                             0.169004726639267902826583426598550, 0.190350578064785409913256402421014,
+                                             0.204432940075298892414161999234649, 0.209482141084727828012999174891714};
+__constant__ static const double GK_WG[4] = {0.129484966168869693270611432679082, 0.279705391489276667901467771423780,
+                                             0.381830050505118944950369775488975, 0.417959183673469387755102040816327};
+
+template <bool DERIV>
+__device__ __forceinline__ cplx gk15(double a, double b, double r, double z, double& err, QuadWork& wk) {
+    double c = (a + b) * 0.5, d = (b - a) * 0.5;
+    cplx f0 = del_integrand<DERIV>(c, r, z, wk);
+    cplx Ik = cscale(GK_WK[7], f0), Ig = cscale(GK_WG[3], f0);
+#pragma unroll 1
+    for (int i = 0; i < 7; i++) {
+        double dx = d * GK_X[i];
+        cplx fx = cadd(del_integrand<DERIV>(c - dx, r, z, wk), del_integrand<DERIV>(c + dx, r, z, wk));
+        Ik = cadd(Ik, cscale(GK_WK[i], fx));
+        if (i & 1) Ig = cadd(Ig, cscale(GK_WG[i >> 1], fx));
+    }
+    Ik = cscale(d, Ik);
+    Ig = cscale(d, Ig);
+    err = hypot(Ik.x - Ig.x, Ik.y - Ig.y);
+    return Ik;
+}
+
+#define MHX_MAXBOX 48
+
+// Adaptive GK(7,15) on [0, pi/2]: pop the interval of largest error, bisect,
+// stop when E <= max(rtol |I|, atol); re-sum the intervals at the end.
+// Returns false if the interval budget was exhausted.
+template <bool DERIV>
+__device__ bool hquad(double r, double z, double rtol, double atol, cplx& out, QuadWork& wk) {
+    double ba[MHX_MAXBOX], bb[MHX_MAXBOX], be[MHX_MAXBOX];
+    cplx bi[MHX_MAXBOX];
+    double E;
+    cplx I0 = gk15<DERIV>(0.0, MHX_PI / 2, r, z, E, wk);
+    ba[0] = 0.0; bb[0] = MHX_PI / 2; bi[0] = I0; be[0] = E;
+    int nb = 1;
+    cplx It = I0;
+    double Et = E;
+    bool ok = true;
+    while (Et > fmax(rtol * hypot(It.x, It.y), atol)) {
+        if (nb >= MHX_MAXBOX - 1) { ok = false; break; }
+        int im = 0;
+        for (int q = 1; q < nb; q++)
+            if (be[q] > be[im]) im = q;
+        double a = ba[im], b = bb[im], e = be[im];
+        cplx iv = bi[im];
+        ba[im] = ba[nb - 1]; bb[im] = bb[nb - 1]; be[im] = be[nb - 1]; bi[im] = bi[nb - 1];
+        nb--;
+        It = csub(It, iv);
+        Et -= e;
+        double w = (b - a) / 2, m = a + w;
+        double E1, E2;
+        cplx I1 = gk15<DERIV>(a, m, r, z, E1, wk);
+        cplx I2 = gk15<DERIV>(m, b, r, z, E2, wk);
+        It = cadd(It, cadd(I1, I2));
+        Et += E1 + E2;
+        ba[nb] = a; bb[nb] = m; bi[nb] = I1; be[nb] = E1; nb++;
+        ba[nb] = m; bb[nb] = b; bi[nb] = I2; be[nb] = E2; nb++;
+    }
+    cplx s = cmk(0.0, 0.0);
+    for (int q = 0; q < nb; q++) s = cadd(s, bi[q]);
+    out = s;
+    return ok;
+}
+
+// Algorithmic FP64 operation counts of the exact wave term (add, sub, mul = 1,
+// fma = 2, div, sqrt and each elementary function = 1), counted from the code
+// above: per integrand evaluation outside the E1 loop, per power-series term,
+// per continued-fraction step.
+#define MHX_FLOPS_PER_EVAL 41.0
+#define MHX_FLOPS_PER_SERIES 20.0
+#define MHX_FLOPS_PER_CF 45.0

@@ -1,0 +1,44 @@
+// Internal declarations shared by the HIP translation units of libmhx.
+#pragma once
+#include "mhx_device.h"
+
+namespace mhx {
+
+// ---- assembly (mhx_assembly.hip)
+void launch_panel_geometry(hipStream_t st, int n, const double* verts, const int64_t* faces, const double* centers,
+                           const double* normals, const double* areas, const double* radii, PanelGeom* geom);
+// Rankine + free-surface image parts (real, wavenumber independent): SR and D_R (direct) or K_R (indirect)
+void launch_rankine(hipStream_t st, int n, const PanelGeom* geom, int direct, double* SR, double* DR);
+// S = SR + S_wave, D = DR (+1/2 I) + D_wave for the Wu wave term
+void launch_combine_wu(hipStream_t st, int n, const PanelGeom* geom, const double* SR, const double* DR, double k,
+                       int direct, int include_identity, cplx* S, cplx* D);
+// exact Guevel-Delhommeau pair table (G, dG/dr) for i <= j mirrored to j > i.
+// status[0]: nonzero if an interval budget was exhausted; status[1..3] += integrand
+// evaluations, power-series terms, continued-fraction steps (status is not cleared here)
+void launch_delhommeau_table(hipStream_t st, int n, const PanelGeom* geom, double k, double rtol, double atol, cplx* WG,
+                             cplx* WR, unsigned long long* status);
+// S, D from a pair table of (G, dG/dr) (Delhommeau or surrogate)
+void launch_combine_table(hipStream_t st, int n, const PanelGeom* geom, const double* SR, const double* DR, double k,
+                          int direct, int include_identity, const cplx* WG, const cplx* WR, cplx* S, cplx* D);
+
+// ---- right-hand sides, products, forces (mhx_assembly.hip)
+// B[:, r] for r < ndof: -i omega (n . dof_r);  B[:, ndof+b]: Airy diffraction BC for heading b
+void launch_bc(hipStream_t st, int n, const PanelGeom* geom, const double* dofs, int ndof, const double* betas, int nb,
+               double omega, double k, double g, cplx* B);
+// Y = S X (n x n times n x R), split-K partials in work (nsplit * n * R complex)
+void launch_matmul_nr(hipStream_t st, int n, int R, const cplx* S, const cplx* X, cplx* Y, cplx* work);
+// F[d + ndof*r] = sum_i (i rho omega phi[i, r]) * -(dof_d,i . n_i) * a_i ; FK[d + ndof*b] for the Airy pressure
+void launch_forces(hipStream_t st, int n, const PanelGeom* geom, const double* dofs, int ndof, const cplx* phi, int R,
+                   double rho, double omega, double k, double g, const double* betas, int nb, cplx* F, cplx* FK);
+
+// ---- dense LU and solves (mhx_lu.hip)
+void zgemm(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cplx* A, int lda, int64_t sA, const cplx* B,
+           int ldb, int64_t sB, cplx* C, int ldc, int64_t sC, int batch);
+int getrf_batched(hipStream_t st, int n, cplx* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
+                  int* work_perm, int* info);
+int getrs_batched(hipStream_t st, int trans, int n, int R, const cplx* LU, int lda, int64_t sA, const int* ipiv,
+                  int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
+                  int batch);
+constexpr int kPermWork = 1 + 4 * 64;  // ints per matrix for getrf_batched's work_perm
+
+}  // namespace mhx

@@ -1,0 +1,160 @@
+"""ctypes binding of libmhx.so (C ABI in include/mhx.h).
+
+The library is built in-tree by `make -C marinehydro.jl_amd` (see
+__graft_entry__.build()).  There is no fallback: if the shared library is
+missing or cannot be loaded, every entry point raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "libmhx.so")
+
+MHX_OK = 0
+MHX_EINVAL = -1
+MHX_EDIM = -2
+MHX_ESINGULAR = -3
+MHX_EDEVICE = -4
+MHX_ENOMEM = -5
+MHX_EQUAD = -6
+
+GF_WU = 0
+GF_DELHOMMEAU = 1
+MODE_PER_OMEGA = 0
+MODE_PER_PROBLEM = 1
+
+EXPORTED = ["mhx_create", "mhx_destroy", "mhx_last_error", "mhx_set_constants", "mhx_assemble", "mhx_solve",
+            "mhx_lu_solve", "mhx_hydro_coeffs", "mhx_body_create", "mhx_body_destroy", "mhx_body_coeffs",
+            "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work"]
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int64)
+
+
+class Options(C.Structure):
+    _fields_ = [("rho", C.c_double), ("g", C.c_double), ("device", C.c_int), ("timing", C.c_int)]
+
+
+class CMesh(C.Structure):
+    _fields_ = [("nv", C.c_int64), ("nf", C.c_int64), ("vertices", _dp), ("faces", _ip), ("centers", _dp),
+                ("normals", _dp), ("areas", _dp), ("radii", _dp)]
+
+
+class MHXError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mhx error {code}: {msg}")
+        self.code = code
+
+
+class DimensionMismatch(ValueError):
+    """The reference's DimensionMismatch."""
+
+
+class SingularException(ArithmeticError):
+    """LAPACK-style exactly-zero pivot."""
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise OSError(f"libmhx.so not built at {LIB_PATH}; run `make -C {PKG_ROOT}`")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.mhx_create.argtypes = [C.POINTER(vp), C.POINTER(Options)]
+        L.mhx_destroy.argtypes = [vp]
+        L.mhx_destroy.restype = None
+        L.mhx_last_error.argtypes = [vp]
+        L.mhx_last_error.restype = C.c_char_p
+        L.mhx_set_constants.argtypes = [vp, C.c_double, C.c_double]
+        L.mhx_assemble.argtypes = [vp, C.POINTER(CMesh), C.c_double, C.c_int, C.c_int, C.c_int, _dp, _dp]
+        L.mhx_solve.argtypes = [vp, C.c_int64, C.c_int64, _dp, _dp, _dp, C.c_int, _dp, _dp]
+        L.mhx_lu_solve.argtypes = [vp, C.c_int64, C.c_int64, _dp, C.c_int, _dp]
+        L.mhx_hydro_coeffs.argtypes = [vp, C.POINTER(CMesh), C.c_int64, _dp, C.c_int64, _dp, C.c_int64, _dp,
+                                       C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.mhx_body_create.argtypes = [vp, C.POINTER(CMesh), C.c_int64, _dp, C.POINTER(vp)]
+        L.mhx_body_destroy.argtypes = [vp]
+        L.mhx_body_destroy.restype = None
+        L.mhx_body_coeffs.argtypes = [vp, vp, C.c_int64, _dp, C.c_int64, _dp, C.c_int, C.c_int, C.c_int,
+                                      _dp, _dp, _dp, _dp]
+        L.mhx_phase_time.argtypes = [vp, C.c_char_p, _dp, C.POINTER(C.c_int64)]
+        L.mhx_reset_timing.argtypes = [vp]
+        L.mhx_reset_timing.restype = None
+        L.mhx_exact_work.argtypes = [vp, _ip, _dp]
+        _lib = L
+    return _lib
+
+
+def ptr(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def cmesh(mesh):
+    """Borrow a Mesh's arrays as the C struct (the Mesh must outlive the call)."""
+    return CMesh(mesh.vertices.shape[0], mesh.faces.shape[0], ptr(mesh.vertices), mesh.faces.ctypes.data_as(_ip),
+                 ptr(mesh.centers), ptr(mesh.normals), ptr(mesh.areas), ptr(mesh.radii))
+
+
+class Context:
+    """One HIP stream plus timing state; wraps mhx_create / mhx_destroy."""
+
+    def __init__(self, rho=1000.0, g=9.81, device=0, timing=False):
+        self.h = C.c_void_p()
+        opt = Options(rho, g, device, 1 if timing else 0)
+        rc = lib().mhx_create(C.byref(self.h), C.byref(opt))
+        if rc != MHX_OK:
+            raise MHXError(rc, "mhx_create failed (no HIP device or bad options)")
+
+    def check(self, rc):
+        if rc == MHX_OK:
+            return
+        msg = lib().mhx_last_error(self.h).decode()
+        if rc == MHX_EINVAL:
+            raise ValueError(msg)
+        if rc == MHX_EDIM:
+            raise DimensionMismatch(msg)
+        if rc == MHX_ESINGULAR:
+            raise SingularException(msg)
+        raise MHXError(rc, msg)
+
+    def set_constants(self, rho, g):
+        self.check(lib().mhx_set_constants(self.h, rho, g))
+
+    def phase_time(self, phase):
+        ms = C.c_double()
+        cnt = C.c_int64()
+        self.check(lib().mhx_phase_time(self.h, phase.encode(), C.byref(ms), C.byref(cnt)))
+        return ms.value, cnt.value
+
+    def exact_work(self):
+        counts = np.zeros(3, dtype=np.int64)
+        fl = C.c_double()
+        self.check(lib().mhx_exact_work(self.h, counts.ctypes.data_as(_ip), C.byref(fl)))
+        return counts, fl.value
+
+    def reset_timing(self):
+        lib().mhx_reset_timing(self.h)
+
+    def close(self):
+        if self.h:
+            lib().mhx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def complex_out(shape):
+    return np.zeros(shape, dtype=np.complex128, order="F")
+
+
+def as_c128(a):
+    return np.asfortranarray(a, dtype=np.complex128)

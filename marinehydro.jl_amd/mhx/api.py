@@ -1,0 +1,394 @@
+"""Host-side mirror of MarineHydro.jl's user interface for the BEM hot path.
+
+Names, argument meaning and error behaviour follow the reference so that a
+caller (or a test ported from the reference's test suite) reads the same:
+
+  FloatingBody, translational_dofs, rotational_dofs        src/bodies.jl:9-118
+  DirectBEM, IndirectBEM, default_greens_functions         src/formulations.jl:7-37
+  RadiationProblem, DiffractionProblem                     src/problems_and_results.jl:14-150
+  solve(problem, formulation)                              src/solve.jl:59-78
+  added_mass, radiation_damping, diffraction_force,
+  froude_krylov_force, excitation_force                    src/problems_and_results.jl:202-219
+  hydrodynamic_coefficients                                src/problems_and_results.jl:464-495
+  assemble_matrices(gfs, mesh, k; direct, include_identity) src/matrix_assembly.jl:164-208
+  solve_linear(D, S, bc; direct)  (the reference's solve(D, S, bc))  src/matrix_assembly.jl:436-449
+  SETTINGS, set_rho, set_g                                 src/constants.jl:10-29
+
+Every computation runs in libmhx on the GPU; this module only validates
+arguments, keeps device contexts and packs results.  ArgumentError maps to
+ValueError, DimensionMismatch to mhx.DimensionMismatch.
+"""
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import DimensionMismatch, SingularException  # noqa: F401
+from .meshes import Mesh
+
+RIGID_DOFS = ("Surge", "Sway", "Heave", "Roll", "Pitch", "Yaw")
+
+
+# --------------------------------------------------------------------------- constants
+@dataclass
+class PhysicalConstants:
+    rho: float = 1000.0
+    g: float = 9.81
+
+
+SETTINGS = PhysicalConstants()
+
+
+def set_rho(val):
+    SETTINGS.rho = float(val)
+
+
+def set_g(val):
+    SETTINGS.g = float(val)
+
+
+def compute_wavenumber(omega, g=None):
+    """Deep-water dispersion k = omega^2 / g (src/waves.jl:133-134)."""
+    g = SETTINGS.g if g is None else g
+    return omega * omega / g
+
+
+# --------------------------------------------------------------------------- Green functions
+class GreensFunction:
+    wavenumber_independent = False
+
+
+class Rankine(GreensFunction):
+    wavenumber_independent = True
+
+
+class RankineReflected(GreensFunction):
+    wavenumber_independent = True
+
+
+class GFWu(GreensFunction):
+    code = _lib.GF_WU
+
+
+class ExactGuevelDelhommeau(GreensFunction):
+    code = _lib.GF_DELHOMMEAU
+
+
+def default_greens_functions(gf="Wu"):
+    if gf == "Wu":
+        wave = GFWu()
+    elif gf == "ExactGuevelDelhommeau":
+        wave = ExactGuevelDelhommeau()
+    else:
+        raise ValueError(f'unknown Green\'s function "{gf}"; use "Wu" or "ExactGuevelDelhommeau"')
+    return (Rankine(), RankineReflected(), wave)
+
+
+def _wave_code(gfs):
+    gfs = tuple(gfs)
+    kinds = [type(g) for g in gfs]
+    if len(gfs) != 3 or kinds[0] is not Rankine or kinds[1] is not RankineReflected or \
+            kinds[2] not in (GFWu, ExactGuevelDelhommeau):
+        raise ValueError("the GPU path supports (Rankine(), RankineReflected(), GFWu() | ExactGuevelDelhommeau())")
+    return gfs[2].code
+
+
+class BEMFormulation:
+    direct = True
+
+    def __init__(self, greens="Wu"):
+        self.greens = default_greens_functions(greens) if isinstance(greens, str) else tuple(greens)
+        self.gf = _wave_code(self.greens)
+
+
+class DirectBEM(BEMFormulation):
+    direct = True
+
+
+class IndirectBEM(BEMFormulation):
+    direct = False
+
+
+def is_direct(f):
+    return f.direct
+
+
+# --------------------------------------------------------------------------- contexts
+_contexts = {}
+
+
+def context(device=0):
+    ctx = _contexts.get(device)
+    if ctx is None:
+        ctx = _contexts[device] = _lib.Context(SETTINGS.rho, SETTINGS.g, device)
+    return ctx
+
+
+# --------------------------------------------------------------------------- bodies
+def translational_dofs(mesh, name):
+    dof = np.zeros((mesh.nfaces, 3), order="F")
+    dof[:, ("Surge", "Sway", "Heave").index(name)] = 1.0
+    return dof
+
+
+def rotational_dofs(mesh, name, rotation_center=(0.0, 0.0, 0.0)):
+    if name not in ("Roll", "Pitch", "Yaw"):
+        raise ValueError(f"unknown rotational dof {name}")
+    axis = np.zeros(3)
+    axis[("Roll", "Pitch", "Yaw").index(name)] = 1.0
+    pos = mesh.centers - np.asarray(rotation_center, dtype=np.float64)[None, :]
+    return np.asfortranarray(np.cross(axis[None, :], pos))
+
+
+class FloatingBody:
+    """FloatingBody(mesh, dofs, name) with dofs a dict name -> (n, 3) motion, or
+    FloatingBody(mesh, ["Heave", ...], rotation_center, name) for rigid dofs."""
+
+    def __init__(self, mesh, dofs, rotation_center=(0.0, 0.0, 0.0), body_name="body"):
+        if not isinstance(mesh, Mesh):
+            raise TypeError("mesh must be an mhx.meshes.Mesh")
+        self.mesh = mesh
+        self.body_name = body_name
+        if isinstance(dofs, dict):
+            items = dofs.items()
+        else:
+            items = []
+            for name in dofs:
+                name = str(name)
+                if name in ("Surge", "Sway", "Heave"):
+                    items.append((name, translational_dofs(mesh, name)))
+                else:
+                    items.append((name, rotational_dofs(mesh, name, rotation_center)))
+        self.dofs = {}
+        for name, d in items:
+            d = np.asfortranarray(d, dtype=np.float64)
+            if d.shape != (mesh.nfaces, 3):
+                raise DimensionMismatch(f"dof {name} must have shape ({mesh.nfaces}, 3)")
+            self.dofs[str(name)] = d
+        self._handle = None
+        self._handle_key = None
+
+    @property
+    def dof_names(self):
+        return list(self.dofs)
+
+    def _dof_stack(self, names):
+        return np.ascontiguousarray(np.concatenate([self.dofs[nm].ravel(order="F") for nm in names]))
+
+
+# --------------------------------------------------------------------------- problems
+@dataclass
+class RadiationProblem:
+    floatingbody: FloatingBody
+    omega: float
+    radiating_dof: str = None
+    influenced_dofs: list = None
+    rho: float = None
+    g: float = None
+
+    def __post_init__(self):
+        inf = list(self.influenced_dofs) if self.influenced_dofs is not None else self.floatingbody.dof_names
+        rad = self.radiating_dof
+        if rad is None:
+            if len(inf) != 1:
+                raise ValueError("radiating_dof must be given when the body has several dofs")
+            rad = inf[0]
+        if rad not in self.floatingbody.dofs:
+            raise ValueError("the radiating_dof Symbol must be a key of floatingbody.dofs")
+        if not set(inf) <= set(self.floatingbody.dofs):
+            raise ValueError("each influenced_dofs Symbol must be a key of floatingbody.dofs")
+        self.influenced_dofs, self.radiating_dof = inf, rad
+        self.rho = SETTINGS.rho if self.rho is None else float(self.rho)
+        self.g = SETTINGS.g if self.g is None else float(self.g)
+        self.wavenumber = compute_wavenumber(self.omega, self.g)
+
+
+@dataclass
+class DiffractionProblem:
+    floatingbody: FloatingBody
+    omega: float
+    beta: float = 0.0
+    influenced_dofs: list = None
+    rho: float = None
+    g: float = None
+
+    def __post_init__(self):
+        inf = list(self.influenced_dofs) if self.influenced_dofs is not None else self.floatingbody.dof_names
+        if not set(inf) <= set(self.floatingbody.dofs):
+            raise ValueError("each influenced_dofs Symbol must be a key of floatingbody.dofs")
+        self.influenced_dofs = inf
+        self.rho = SETTINGS.rho if self.rho is None else float(self.rho)
+        self.g = SETTINGS.g if self.g is None else float(self.g)
+        self.wavenumber = compute_wavenumber(self.omega, self.g)
+
+
+@dataclass
+class RadiationResult:
+    problem: RadiationProblem
+    forces: dict
+
+
+@dataclass
+class DiffractionResult:
+    problem: DiffractionProblem
+    forces: dict
+    fk: dict = field(default_factory=dict)
+
+
+def added_mass(res):
+    w = res.problem.omega
+    return {k: v.real / w ** 2 for k, v in res.forces.items()}
+
+
+def radiation_damping(res):
+    w = res.problem.omega
+    return {k: v.imag / w for k, v in res.forces.items()}
+
+
+def diffraction_force(res):
+    return dict(res.forces)
+
+
+def froude_krylov_force(res):
+    return dict(res.fk)
+
+
+def excitation_force(res):
+    return {k: res.forces[k] + res.fk[k] for k in res.forces}
+
+
+# --------------------------------------------------------------------------- solves
+def _body_coeffs(body, names, omegas, betas, gf, direct, mode, rho, g, device=0):
+    ctx = context(device)
+    ctx.set_constants(rho, g)
+    L = _lib.lib()
+    key = tuple(names)
+    if body._handle is None or body._handle_key != key:
+        _release(body)
+        h = _lib.C.c_void_p()
+        m = body.mesh
+        cm = _lib.cmesh(m)
+        dofs = body._dof_stack(names)
+        ctx.check(L.mhx_body_create(ctx.h, _lib.C.byref(cm), len(names), _lib.ptr(dofs), _lib.C.byref(h)))
+        body._handle, body._handle_key, body._ctx = h, key, ctx
+    nd = len(names)
+    om = np.ascontiguousarray(omegas, dtype=np.float64)
+    be = np.ascontiguousarray(betas, dtype=np.float64)
+    nw, nb = om.size, be.size
+    A = np.zeros((nd, nd, nw), order="F")
+    B = np.zeros((nd, nd, nw), order="F")
+    Fd = np.zeros((nd, nw, max(nb, 1)), dtype=np.complex128, order="F")
+    Fk = np.zeros((nd, nw, max(nb, 1)), dtype=np.complex128, order="F")
+    ctx.check(L.mhx_body_coeffs(ctx.h, body._handle, nw, _lib.ptr(om), nb, _lib.ptr(be) if nb else None, gf,
+                                int(direct), mode, _lib.ptr(A), _lib.ptr(B), _lib.ptr(Fd), _lib.ptr(Fk)))
+    return A, B, Fd[:, :, :nb], Fk[:, :, :nb]
+
+
+def _release(body):
+    if body._handle is not None:
+        _lib.lib().mhx_body_destroy(body._handle)
+        body._handle = None
+
+
+def solve(problem, formulation=None):
+    """solve(problem, DirectBEM()/IndirectBEM()) (src/solve.jl:59-65)."""
+    formulation = DirectBEM() if formulation is None else formulation
+    body = problem.floatingbody
+    names = list(problem.influenced_dofs)
+    if isinstance(problem, RadiationProblem):
+        if problem.radiating_dof not in names:
+            names = names + [problem.radiating_dof]
+        A, B, _, _ = _body_coeffs(body, names, [problem.omega], [], formulation.gf, formulation.direct,
+                                  _lib.MODE_PER_OMEGA, problem.rho, problem.g)
+        r = names.index(problem.radiating_dof)
+        w = problem.omega
+        forces = {nm: complex(A[i, r, 0] * w * w, B[i, r, 0] * w) for i, nm in enumerate(problem.influenced_dofs)}
+        return RadiationResult(problem, forces)
+    if isinstance(problem, DiffractionProblem):
+        _, _, Fd, Fk = _body_coeffs(body, names, [problem.omega], [problem.beta], formulation.gf, formulation.direct,
+                                    _lib.MODE_PER_OMEGA, problem.rho, problem.g)
+        return DiffractionResult(problem, {nm: complex(Fd[i, 0, 0]) for i, nm in enumerate(names)},
+                                 {nm: complex(Fk[i, 0, 0]) for i, nm in enumerate(names)})
+    raise TypeError("problem must be a RadiationProblem or a DiffractionProblem")
+
+
+def hydrodynamic_coefficients(floatingbody, parameters, formulation=None, schedule="per_omega"):
+    """hydrodynamic_coefficients(body, (wave_frequencies, radiating_dofs, wave_directions), formulation)
+    (src/problems_and_results.jl:464-495).  Layouts as the reference:
+    added_mass / radiation_damping (n_inf, n_rad, n_omega, 1); forces (n_inf, n_omega, n_beta, 1).
+    schedule="per_problem" reproduces the reference's one assembly + LU per problem."""
+    formulation = DirectBEM() if formulation is None else formulation
+    ws = np.atleast_1d(np.asarray(parameters["wave_frequencies"], dtype=np.float64))
+    inf = list(parameters.get("influenced_dofs", floatingbody.dof_names))
+    rad = list(parameters.get("radiating_dofs", []))
+    betas = list(parameters.get("wave_directions", []))
+    for nm in inf + rad:
+        if nm not in floatingbody.dofs:
+            raise ValueError("each influenced_dofs Symbol must be a key of floatingbody.dofs")
+    names = inf + [r for r in rad if r not in inf]
+    mode = {"per_omega": _lib.MODE_PER_OMEGA, "per_problem": _lib.MODE_PER_PROBLEM}[schedule]
+    A, B, Fd, Fk = _body_coeffs(floatingbody, names, ws, betas, formulation.gf, formulation.direct, mode,
+                                SETTINGS.rho, SETTINGS.g)
+    ii = [names.index(nm) for nm in inf]
+    rr = [names.index(nm) for nm in rad]
+    out = {}
+    if rad:
+        out["added_mass"] = A[np.ix_(ii, rr)][..., None]
+        out["radiation_damping"] = B[np.ix_(ii, rr)][..., None]
+    else:
+        out["added_mass"] = out["radiation_damping"] = np.zeros(0)
+    if betas:
+        fd = Fd[ii][..., None]
+        fk = Fk[ii][..., None]
+        out.update(diffraction_force=fd, Froude_Krylov_force=fk, excitation_force=fd + fk)
+    else:
+        out.update(diffraction_force=np.zeros(0), Froude_Krylov_force=np.zeros(0), excitation_force=np.zeros(0))
+    return out
+
+
+def assemble_matrices(green_functions, mesh, wavenumber, direct=True, include_identity=True, device=0):
+    """S, D (or K for the indirect method) as n x n complex column-major arrays."""
+    gf = _wave_code(green_functions)
+    ctx = context(device)
+    n = mesh.nfaces
+    S = _lib.complex_out((n, n))
+    D = _lib.complex_out((n, n))
+    cm = _lib.cmesh(mesh)
+    ctx.check(_lib.lib().mhx_assemble(ctx.h, _lib.C.byref(cm), float(wavenumber), gf, int(direct),
+                                      int(include_identity), _lib.ptr(S), _lib.ptr(D)))
+    return S, D
+
+
+def solve_linear(D, S, bc, direct=True, device=0):
+    """The reference's solve(D, S, bc; direct): returns (phi, sources)."""
+    D = _lib.as_c128(D)
+    S = _lib.as_c128(S)
+    bc = _lib.as_c128(bc)
+    vec = bc.ndim == 1
+    bc2 = bc.reshape(-1, 1, order="F") if vec else bc
+    n = D.shape[0]
+    if D.shape != (n, n) or S.shape != (n, n) or bc2.shape[0] != n:
+        raise DimensionMismatch("D, S must be n x n and bc must have n rows")
+    R = bc2.shape[1]
+    phi = _lib.complex_out((n, R))
+    src = _lib.complex_out((n, R))
+    ctx = context(device)
+    ctx.check(_lib.lib().mhx_solve(ctx.h, n, R, _lib.ptr(D), _lib.ptr(S), _lib.ptr(bc2), int(direct),
+                                   _lib.ptr(phi), _lib.ptr(src)))
+    if vec:
+        return phi[:, 0], (None if direct else src[:, 0])
+    return phi, (None if direct else src)
+
+
+def lu_solve(A, B, trans=0, device=0):
+    """X = A \\ B (trans=0) or A^H \\ B (trans=2) through the device LU."""
+    A = _lib.as_c128(A)
+    B = np.array(B, dtype=np.complex128, order="F", copy=True)
+    vec = B.ndim == 1
+    B2 = B.reshape(-1, 1, order="F") if vec else B
+    n = A.shape[0]
+    if A.shape != (n, n) or B2.shape[0] != n:
+        raise DimensionMismatch("A must be n x n and B must have n rows")
+    ctx = context(device)
+    ctx.check(_lib.lib().mhx_lu_solve(ctx.h, n, B2.shape[1], _lib.ptr(A), int(trans), _lib.ptr(B2)))
+    return B2[:, 0] if vec else B2

@@ -1,0 +1,100 @@
+"""CPU-side checks of the boundary: libmhx loads and exports every function
+include/mhx.h declares; host-side argument handling mirrors the reference's
+errors; meshers produce the workloads' exact panel counts with outward normals.
+No kernel is launched here."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "mhx.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(mhx_\w+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+    from mhx import _lib
+    L = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 14
+    for nm in names:
+        assert hasattr(L, nm), nm
+    assert sorted(_lib.EXPORTED) == names
+
+
+def test_no_device_is_an_error_not_a_fallback():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    from mhx import _lib
+    with pytest.raises(_lib.MHXError):
+        _lib.Context()
+
+
+def test_formulations_and_errors():
+    import mhx
+    assert mhx.DirectBEM().gf == 0 and mhx.DirectBEM("ExactGuevelDelhommeau").gf == 1
+    assert not mhx.IndirectBEM().direct
+    with pytest.raises(ValueError):
+        mhx.default_greens_functions("Delhommeau")
+    with pytest.raises(ValueError):
+        mhx.DirectBEM((mhx.Rankine(), mhx.GFWu()))
+    m = mhx.meshes.sphere(1.0, 6, 6)
+    with pytest.raises(mhx.DimensionMismatch):
+        mhx.FloatingBody(m, {"Heave": np.zeros((5, 3))})
+    body = mhx.FloatingBody(m, ["Heave", "Pitch"])
+    with pytest.raises(ValueError):
+        mhx.RadiationProblem(body, 1.0)               # several dofs, no radiating_dof
+    with pytest.raises(ValueError):
+        mhx.RadiationProblem(body, 1.0, radiating_dof="Yaw")
+    p = mhx.RadiationProblem(body, 1.0, radiating_dof="Heave", rho=1025.0)
+    assert p.wavenumber == pytest.approx(1.0 / 9.81)
+    with pytest.raises(ValueError):
+        mhx.meshes.Mesh(m.vertices, m.faces - 1)      # 0-based faces are rejected
+
+
+def test_rigid_dofs_match_reference_definition():
+    import mhx
+    m = mhx.meshes.sphere(1.0, 6, 6)
+    body = mhx.FloatingBody(m, ["Surge", "Roll", "Yaw"], rotation_center=(0.0, 0.0, -0.5))
+    np.testing.assert_array_equal(body.dofs["Surge"], np.tile([1.0, 0.0, 0.0], (18, 1)))
+    rel = m.centers - np.array([0.0, 0.0, -0.5])
+    np.testing.assert_allclose(body.dofs["Roll"], np.cross([1.0, 0.0, 0.0], rel))
+    np.testing.assert_allclose(body.dofs["Yaw"], np.cross([0.0, 0.0, 1.0], rel))
+
+
+@pytest.mark.parametrize("maker,n,area", [
+    (lambda M: M.sphere(1.0, 8, 16), 64, 2 * math.pi),
+    (lambda M: M.cylinder(5.0, 10.0, 32, 24, 8), 1024, 2 * math.pi * 5 * 10 + math.pi * 25),
+    (lambda M: M.capped_cylinder(4.0, 8.0, 64, 40, 24), 4096, 2 * math.pi * 4 * 8 + 2 * math.pi * 16),
+    (lambda M: M.hemisphere(1.0, 32, 64), 2048, 2 * math.pi),
+    (lambda M: M.box(4.0, 3.0, 2.0, 512), 512, 12 + 2 * 8 + 2 * 6),
+])
+def test_meshers(maker, n, area):
+    from mhx import meshes
+    m = maker(meshes)
+    assert m.nfaces == n
+    assert abs(m.areas.sum() - area) / area < 0.05
+    # outward normals: the normal points away from the body axis/centre for every panel
+    c = m.centers
+    inward = np.einsum("ij,ij->i", m.normals, c - np.array([0.0, 0.0, c[:, 2].min() / 2])) < -1e-9
+    assert inward.mean() < 0.05
+    assert np.all(c[:, 2] < 0)
+    assert np.allclose(np.linalg.norm(m.normals, axis=1), 1.0)
+
+
+def test_random_bodies_hit_exact_counts():
+    from mhx import meshes
+    bodies = meshes.random_bodies()
+    assert len(bodies) == 8
+    for mesh, omegas, desc in bodies:
+        assert mesh.nfaces == desc["n"] in (256, 512, 1024, 2048, 4096)
+        assert 16 <= len(omegas) <= 64
+        assert omegas.min() >= 0.1 and omegas.max() <= 3.0
