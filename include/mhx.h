@@ -43,6 +43,8 @@ extern "C" {
 /* solve schedule of mhx_hydro_coeffs */
 #define MHX_MODE_PER_OMEGA 0   /* one assembly + one LU per frequency, all right-hand sides together */
 #define MHX_MODE_PER_PROBLEM 1 /* reference semantics: assembly + LU per radiation/diffraction problem */
+#define MHX_MODE_BATCHED 2     /* frequencies in chunks: one launch sequence (assembly, batched LU, solves,
+                                  forces) per chunk; chunk size from mhx_set_batch */
 
 typedef struct mhx_ctx mhx_ctx;
 typedef struct mhx_body mhx_body;
@@ -99,6 +101,9 @@ int mhx_body_create(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const doub
 void mhx_body_destroy(mhx_body* body);
 int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* body, int64_t nw, const double* omegas, int64_t nb, const double* betas,
                     int gf, int direct, int mode, double* A_out, double* B_out, double* Fd_out, double* Fk_out);
+
+/* frequency-chunk limits of MHX_MODE_BATCHED (defaults 64 items, 64 GiB of device work memory) */
+int mhx_set_batch(mhx_ctx* ctx, int64_t max_items, double max_bytes);
 
 /* per-phase timing (HIP events on the context's stream, enabled by mhx_options.timing):
  * phase: "rankine", "wave_table" (exact wave term), "wave" (S, D formation), "lu", "solve", "forces";

@@ -29,6 +29,8 @@ struct mhx_ctx {
     std::map<std::string, PhaseAcc> phases;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
     unsigned long long quad_work[3] = {0, 0, 0};  // exact wave term: evaluations, series terms, CF steps
+    double batch_bytes = 64.0 * (1ull << 30);      // device memory budget of one frequency chunk
+    int64_t max_batch = 64;
 };
 
 struct mhx_body {
@@ -39,13 +41,17 @@ struct mhx_body {
     double* SR = nullptr;
     double* DR[2] = {nullptr, nullptr};  // [direct] : K_R (indirect) or D_R (direct)
     bool have_rankine[2] = {false, false};
-    // per-frequency work
+    // per-frequency work, cap_batch items
+    int cap_batch = 0, cap_R = 0, cap_nb = 0;
+    bool cap_table = false;
     cplx *S = nullptr, *D = nullptr, *WG = nullptr, *WR = nullptr;
     cplx *B = nullptr, *Y = nullptr, *work = nullptr, *F = nullptr, *FK = nullptr;
-    double* betas = nullptr;
-    int cap_R = 0, cap_nb = 0;
+    double *betas = nullptr, *d_ws = nullptr, *d_ks = nullptr;
     int *ipiv = nullptr, *perm = nullptr, *wperm = nullptr, *info = nullptr;
     unsigned long long* status = nullptr;
+    int64_t nn() const { return (int64_t)n * n; }
+    int64_t nR() const { return (int64_t)n * cap_R; }
+    int64_t sW() const { return 18 * nR(); }  // 16 n R split-K partials + 2 n R getrs work
 };
 
 namespace {
@@ -135,50 +141,57 @@ int ensure_rankine(mhx_ctx* ctx, mhx_body* b, int direct) {
     return MHX_OK;
 }
 
-int ensure_work(mhx_ctx* ctx, mhx_body* b, int R, int nb, bool table) {
-    const size_t n = b->n, nn = n * n;
+void free_work(mhx_body* b) {
+    dfree(b->S); dfree(b->D); dfree(b->WG); dfree(b->WR); dfree(b->B); dfree(b->Y); dfree(b->work);
+    dfree(b->F); dfree(b->FK); dfree(b->betas); dfree(b->d_ws); dfree(b->d_ks); dfree(b->ipiv); dfree(b->perm);
+    dfree(b->wperm); dfree(b->info); dfree(b->status);
+    b->cap_batch = b->cap_R = b->cap_nb = 0;
+    b->cap_table = false;
+}
+
+// work buffers for `batch` frequency items with up to R right-hand sides and nb headings
+int ensure_work(mhx_ctx* ctx, mhx_body* b, int R, int nb, bool table, int batch = 1) {
+    if (batch <= b->cap_batch && R <= b->cap_R && nb <= b->cap_nb && (!table || b->cap_table)) return MHX_OK;
+    const int nbt = batch > b->cap_batch ? batch : b->cap_batch;
+    const int cr = R > 16 ? R : 16, cnb = nb > 8 ? nb : 8;
+    const bool tab = table || b->cap_table;
+    free_work(b);
+    b->cap_batch = nbt;
+    b->cap_R = cr;
+    b->cap_nb = cnb;
+    b->cap_table = tab;
+    const size_t n = b->n, nn = n * n, B = nbt;
     int rc;
-    if ((rc = dalloc(ctx, &b->S, nn))) return rc;
-    if ((rc = dalloc(ctx, &b->D, nn))) return rc;
-    if (table) {
-        if ((rc = dalloc(ctx, &b->WG, nn))) return rc;
-        if ((rc = dalloc(ctx, &b->WR, nn))) return rc;
+    if ((rc = dalloc(ctx, &b->S, nn * B)) || (rc = dalloc(ctx, &b->D, nn * B))) return rc;
+    if (tab && ((rc = dalloc(ctx, &b->WG, nn * B)) || (rc = dalloc(ctx, &b->WR, nn * B)))) return rc;
+    if ((rc = dalloc(ctx, &b->B, n * cr * B)) || (rc = dalloc(ctx, &b->Y, n * cr * B)) ||
+        (rc = dalloc(ctx, &b->work, 18 * n * cr * B)) || (rc = dalloc(ctx, &b->F, (size_t)(b->ndof ? b->ndof : 1) * cr * B)) ||
+        (rc = dalloc(ctx, &b->FK, (size_t)(b->ndof ? b->ndof : 1) * cnb * B)) || (rc = dalloc(ctx, &b->betas, (size_t)cnb)) ||
+        (rc = dalloc(ctx, &b->d_ws, B)) || (rc = dalloc(ctx, &b->d_ks, B)) || (rc = dalloc(ctx, &b->ipiv, n * B)) ||
+        (rc = dalloc(ctx, &b->perm, n * B)) || (rc = dalloc(ctx, &b->wperm, (size_t)mhx::kPermWork * B)) ||
+        (rc = dalloc(ctx, &b->info, B)) || (rc = dalloc(ctx, &b->status, 4))) {
+        free_work(b);
+        return rc;
     }
-    if (R > b->cap_R || nb > b->cap_nb) {
-        dfree(b->B); dfree(b->Y); dfree(b->work); dfree(b->F); dfree(b->FK); dfree(b->betas);
-        b->cap_R = R > 16 ? R : 16;
-        b->cap_nb = nb > 8 ? nb : 8;
-    }
-    const int cr = b->cap_R;
-    if ((rc = dalloc(ctx, &b->B, n * cr))) return rc;
-    if ((rc = dalloc(ctx, &b->Y, n * cr))) return rc;
-    if ((rc = dalloc(ctx, &b->work, 16 * n * cr + 2 * n * cr))) return rc;
-    if ((rc = dalloc(ctx, &b->F, (size_t)b->ndof * cr))) return rc;
-    if ((rc = dalloc(ctx, &b->FK, (size_t)b->ndof * b->cap_nb))) return rc;
-    if ((rc = dalloc(ctx, &b->betas, (size_t)b->cap_nb))) return rc;
-    if ((rc = dalloc(ctx, &b->ipiv, n))) return rc;
-    if ((rc = dalloc(ctx, &b->perm, n))) return rc;
-    if ((rc = dalloc(ctx, &b->wperm, (size_t)mhx::kPermWork))) return rc;
-    if ((rc = dalloc(ctx, &b->info, 1))) return rc;
-    if ((rc = dalloc(ctx, &b->status, 4))) return rc;
     return MHX_OK;
 }
 
-// wave assembly into b->S, b->D for wavenumber k
-int assemble_into(mhx_ctx* ctx, mhx_body* b, double k, int gf, int direct, int include_identity) {
+// wave assembly of `batch` items (wavenumbers already in b->d_ks) into b->S, b->D
+int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int include_identity) {
     if (gf == MHX_GF_WU) {
         Phase ph(ctx, "wave");
-        mhx::launch_combine_wu(ctx->stream, b->n, b->geom, b->SR, b->DR[direct], k, direct, include_identity, b->S,
-                               b->D);
+        mhx::launch_combine_wu(ctx->stream, b->n, b->geom, b->SR, b->DR[direct], b->d_ks, batch, direct,
+                               include_identity, b->S, b->D, b->nn());
     } else {
         HCHK(hipMemsetAsync(b->status, 0, 4 * sizeof(unsigned long long), ctx->stream));
         {
             Phase ph(ctx, "wave_table");
-            mhx::launch_delhommeau_table(ctx->stream, b->n, b->geom, k, 1e-8, 1e-8, b->WG, b->WR, b->status);
+            mhx::launch_delhommeau_table(ctx->stream, b->n, b->geom, b->d_ks, batch, 1e-8, 1e-8, b->WG, b->WR, b->nn(),
+                                         b->status);
         }
         Phase ph(ctx, "wave");
-        mhx::launch_combine_table(ctx->stream, b->n, b->geom, b->SR, b->DR[direct], k, direct, include_identity,
-                                  b->WG, b->WR, b->S, b->D);
+        mhx::launch_combine_table(ctx->stream, b->n, b->geom, b->SR, b->DR[direct], b->d_ks, batch, direct,
+                                  include_identity, b->WG, b->WR, b->S, b->D, b->nn());
     }
     HCHK(hipGetLastError());
     return MHX_OK;
@@ -194,34 +207,47 @@ int check_status(mhx_ctx* ctx, mhx_body* b, int gf) {
     return MHX_OK;
 }
 
-// factor b->D in place and solve for the R columns of b->B; phi into b->Y
-int factor_and_solve(mhx_ctx* ctx, mhx_body* b, int R, int direct) {
+// factor the `batch` matrices b->D in place and solve for the R columns of b->B; phi into b->Y
+int factor_and_solve(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch = 1) {
     const int n = b->n;
-    HCHK(hipMemsetAsync(b->info, 0, sizeof(int), ctx->stream));
+    HCHK(hipMemsetAsync(b->info, 0, sizeof(int) * batch, ctx->stream));
     {
         Phase ph(ctx, "lu");
-        int rc = mhx::getrf_batched(ctx->stream, n, b->D, n, (int64_t)n * n, 1, b->ipiv, n, b->wperm, b->info);
+        int rc = mhx::getrf_batched(ctx->stream, n, b->D, n, b->nn(), batch, b->ipiv, n, b->wperm, b->info);
         if (rc) return fail(ctx, MHX_EDIM, "LU: order above 4096 is not supported");
     }
     HCHK(hipGetLastError());
     {
         Phase ph(ctx, "solve");
-        cplx* W2 = b->work + (size_t)16 * n * b->cap_R;
+        cplx* W2 = b->work + 16 * b->nR();
         if (direct) {
-            mhx::launch_matmul_nr(ctx->stream, n, R, b->S, b->B, b->Y, b->work);
-            mhx::getrs_batched(ctx->stream, 0, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, b->Y, 0, W2, 0, 1);
+            mhx::launch_matmul_nr(ctx->stream, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), batch);
+            mhx::getrs_batched(ctx->stream, 0, n, R, b->D, n, b->nn(), b->ipiv, n, b->perm, n, b->Y, b->nR(), W2,
+                               b->sW(), batch);
         } else {
-            HCHK(hipMemcpyAsync(b->work, b->B, sizeof(cplx) * (size_t)n * R, hipMemcpyDeviceToDevice, ctx->stream));
-            // sources (in place in b->B)
-            mhx::getrs_batched(ctx->stream, 0, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, b->B, 0, W2, 0, 1);
-            mhx::launch_matmul_nr(ctx->stream, n, R, b->S, b->B, b->Y, b->work);
+            mhx::getrs_batched(ctx->stream, 0, n, R, b->D, n, b->nn(), b->ipiv, n, b->perm, n, b->B, b->nR(), W2,
+                               b->sW(), batch);
+            mhx::launch_matmul_nr(ctx->stream, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), batch);
         }
     }
     HCHK(hipGetLastError());
-    int info = 0;
-    HCHK(hipMemcpyAsync(&info, b->info, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<int> info(batch, 0);
+    HCHK(hipMemcpyAsync(info.data(), b->info, sizeof(int) * batch, hipMemcpyDeviceToHost, ctx->stream));
     HCHK(hipStreamSynchronize(ctx->stream));
-    if (info) return fail(ctx, MHX_ESINGULAR, "LU: exactly zero pivot at column " + std::to_string(info));
+    for (int q = 0; q < batch; q++)
+        if (info[q]) return fail(ctx, MHX_ESINGULAR, "LU: exactly zero pivot at column " + std::to_string(info[q]));
+    return MHX_OK;
+}
+
+int set_freqs(mhx_ctx* ctx, mhx_body* b, const double* ws, int batch) {
+    std::vector<double> h(2 * batch);
+    for (int q = 0; q < batch; q++) {
+        h[q] = ws[q];
+        h[batch + q] = ws[q] * ws[q] / ctx->g;  // compute_wavenumber (src/waves.jl:133-134)
+    }
+    HCHK(hipMemcpyAsync(b->d_ws, h.data(), sizeof(double) * batch, hipMemcpyHostToDevice, ctx->stream));
+    HCHK(hipMemcpyAsync(b->d_ks, h.data() + batch, sizeof(double) * batch, hipMemcpyHostToDevice, ctx->stream));
+    HCHK(hipStreamSynchronize(ctx->stream));
     return MHX_OK;
 }
 
@@ -320,9 +346,7 @@ int mhx_body_create(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const doub
 void mhx_body_destroy(mhx_body* b) {
     if (!b) return;
     dfree(b->geom); dfree(b->dofs); dfree(b->SR); dfree(b->DR[0]); dfree(b->DR[1]);
-    dfree(b->S); dfree(b->D); dfree(b->WG); dfree(b->WR); dfree(b->B); dfree(b->Y); dfree(b->work);
-    dfree(b->F); dfree(b->FK); dfree(b->betas); dfree(b->ipiv); dfree(b->perm); dfree(b->wperm);
-    dfree(b->info); dfree(b->status);
+    free_work(b);
     delete b;
 }
 
@@ -337,13 +361,20 @@ int mhx_assemble(mhx_ctx* ctx, const mhx_mesh* mesh, double k, int gf, int direc
     mhx_body* b = nullptr;
     int rc = body_upload(ctx, mesh, 0, nullptr, &b);
     if (rc) return rc;
+    const double w = std::sqrt(k * ctx->g);
     if (!(rc = ensure_rankine(ctx, b, direct)) && !(rc = ensure_work(ctx, b, 1, 0, gf != MHX_GF_WU)) &&
-        !(rc = assemble_into(ctx, b, k, gf, direct, include_identity ? 1 : 0)) && !(rc = check_status(ctx, b, gf))) {
-        const size_t bytes = sizeof(cplx) * (size_t)b->n * b->n;
-        hipError_t e1 = hipMemcpyAsync(S_out, b->S, bytes, hipMemcpyDeviceToHost, ctx->stream);
-        hipError_t e2 = hipMemcpyAsync(D_out, b->D, bytes, hipMemcpyDeviceToHost, ctx->stream);
-        hipError_t e3 = hipStreamSynchronize(ctx->stream);
-        if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) rc = fail(ctx, MHX_EDEVICE, "assemble: copy back");
+        !(rc = set_freqs(ctx, b, &w, 1))) {
+        // the wavenumber itself, not k recomputed from omega
+        if (hipMemcpyAsync(b->d_ks, &k, sizeof(double), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
+            rc = fail(ctx, MHX_EDEVICE, "assemble: copy k");
+        if (!rc && !(rc = assemble_into(ctx, b, 1, gf, direct, include_identity ? 1 : 0)) &&
+            !(rc = check_status(ctx, b, gf))) {
+            const size_t bytes = sizeof(cplx) * (size_t)b->n * b->n;
+            hipError_t e1 = hipMemcpyAsync(S_out, b->S, bytes, hipMemcpyDeviceToHost, ctx->stream);
+            hipError_t e2 = hipMemcpyAsync(D_out, b->D, bytes, hipMemcpyDeviceToHost, ctx->stream);
+            hipError_t e3 = hipStreamSynchronize(ctx->stream);
+            if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) rc = fail(ctx, MHX_EDEVICE, "assemble: copy back");
+        }
     }
     flush_timing(ctx);
     mhx_body_destroy(b);
@@ -419,9 +450,31 @@ int mhx_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* D, const doub
         }
     }
     flush_timing(ctx);
-    dfree(b.S); dfree(b.D); dfree(b.B); dfree(b.Y); dfree(b.work); dfree(b.F); dfree(b.FK); dfree(b.betas);
-    dfree(b.ipiv); dfree(b.perm); dfree(b.wperm); dfree(b.info); dfree(b.status);
+    free_work(&b);
     return rc;
+}
+
+// packing of added_mass / radiation_damping (src/problems_and_results.jl:207-208, 325-380)
+static void pack(const std::vector<cplx>& F, const std::vector<cplx>& FK, int ndof, int R, int nb, int64_t iw,
+                 int64_t nw, double w, double* A_out, double* B_out, double* Fd_out, double* Fk_out) {
+    for (int r = 0; r < R; r++)
+        for (int d = 0; d < ndof; d++) {
+            const cplx f = F[d + (size_t)ndof * r];
+            if (r < ndof) {
+                A_out[d + (size_t)ndof * (r + (size_t)ndof * iw)] = f.x / (w * w);
+                B_out[d + (size_t)ndof * (r + (size_t)ndof * iw)] = f.y / w;
+            } else {
+                const size_t o = d + (size_t)ndof * (iw + (size_t)nw * (r - ndof));
+                Fd_out[2 * o] = f.x;
+                Fd_out[2 * o + 1] = f.y;
+            }
+        }
+    for (int bb = 0; bb < nb; bb++)
+        for (int d = 0; d < ndof; d++) {
+            const size_t o = d + (size_t)ndof * (iw + (size_t)nw * bb);
+            Fk_out[2 * o] = FK[d + (size_t)ndof * bb].x;
+            Fk_out[2 * o + 1] = FK[d + (size_t)ndof * bb].y;
+        }
 }
 
 int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas, int64_t nb, const double* betas, int gf,
@@ -429,7 +482,8 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
     if (!ctx || !b) return MHX_EINVAL;
     if (nw < 0 || nb < 0 || (nw && !omegas) || (nb && !betas)) return fail(ctx, MHX_EINVAL, "coeffs: bad frequency/heading arrays");
     if (gf != MHX_GF_WU && gf != MHX_GF_DELHOMMEAU) return fail(ctx, MHX_EINVAL, "unknown Green function");
-    if (mode != MHX_MODE_PER_OMEGA && mode != MHX_MODE_PER_PROBLEM) return fail(ctx, MHX_EINVAL, "unknown mode");
+    if (mode != MHX_MODE_PER_OMEGA && mode != MHX_MODE_PER_PROBLEM && mode != MHX_MODE_BATCHED)
+        return fail(ctx, MHX_EINVAL, "unknown mode");
     const int ndof = b->ndof, n = b->n;
     const int R = ndof + (int)nb;
     if (R <= 0) return fail(ctx, MHX_EDIM, "coeffs: no radiating dof and no heading");
@@ -441,66 +495,57 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
         if (!(omegas[w] > 0) || !std::isfinite(omegas[w])) return fail(ctx, MHX_EINVAL, "omega must be positive and finite");
     hipSetDevice(ctx->device);
     direct = direct ? 1 : 0;
+    const bool table = gf != MHX_GF_WU;
+    int chunk = 1;
+    if (mode == MHX_MODE_BATCHED && nw > 1) {
+        const double per = (double)n * n * sizeof(cplx) * (table ? 4 : 2) + 20.0 * n * 16 * sizeof(cplx);
+        chunk = (int)std::min<double>((double)std::min<int64_t>(nw, ctx->max_batch), std::max(1.0, ctx->batch_bytes / per));
+    }
     int rc;
-    if ((rc = ensure_work(ctx, b, R, (int)nb, gf != MHX_GF_WU))) return rc;
+    if ((rc = ensure_work(ctx, b, R, (int)nb, table, chunk))) return rc;
     if ((rc = ensure_rankine(ctx, b, direct))) return rc;
     hipStream_t st = ctx->stream;
     if (nb) HCHK(hipMemcpyAsync(b->betas, betas, sizeof(double) * nb, hipMemcpyHostToDevice, st));
-    std::vector<cplx> F((size_t)ndof * R), FK((size_t)ndof * (nb ? nb : 1));
-    for (int64_t iw = 0; iw < nw; iw++) {
-        const double w = omegas[iw];
-        const double k = w * w / ctx->g;  // compute_wavenumber (src/waves.jl:133-134)
-        mhx::launch_bc(st, n, b->geom, b->dofs, ndof, b->betas, (int)nb, w, k, ctx->g, b->B);
-        if (mode == MHX_MODE_PER_OMEGA) {
-            if ((rc = assemble_into(ctx, b, k, gf, direct, 1)) || (rc = check_status(ctx, b, gf))) break;
-            if ((rc = factor_and_solve(ctx, b, R, direct))) break;
+    std::vector<cplx> F((size_t)ndof * R * chunk), FK((size_t)ndof * (nb ? nb : 1) * chunk);
+    for (int64_t w0 = 0; w0 < nw; w0 += chunk) {
+        const int bt = (int)std::min<int64_t>(chunk, nw - w0);
+        if ((rc = set_freqs(ctx, b, omegas + w0, bt))) break;
+        mhx::launch_bc(st, n, b->geom, b->dofs, ndof, b->betas, (int)nb, b->d_ws, b->d_ks, bt, ctx->g, b->B, b->nR());
+        if (mode != MHX_MODE_PER_PROBLEM) {
+            if ((rc = assemble_into(ctx, b, bt, gf, direct, 1)) || (rc = check_status(ctx, b, gf))) break;
+            if ((rc = factor_and_solve(ctx, b, R, direct, bt))) break;
         } else {
-            // one assembly (Rankine part included, as the reference recomputes it) and one LU per problem
-            cplx* phi_all = b->work + (size_t)16 * n * b->cap_R;  // reuse as staging for the R solutions
+            // reference semantics: one assembly (Rankine part included, as the reference recomputes
+            // it) and one LU per radiation / diffraction problem
             std::vector<cplx> hB((size_t)n * R), hphi((size_t)n * R);
             HCHK(hipMemcpyAsync(hB.data(), b->B, sizeof(cplx) * n * R, hipMemcpyDeviceToHost, st));
             HCHK(hipStreamSynchronize(st));
             for (int r = 0; r < R && !rc; r++) {
                 b->have_rankine[direct] = false;
                 if ((rc = ensure_rankine(ctx, b, direct))) break;
-                if ((rc = assemble_into(ctx, b, k, gf, direct, 1)) || (rc = check_status(ctx, b, gf))) break;
+                if ((rc = assemble_into(ctx, b, 1, gf, direct, 1)) || (rc = check_status(ctx, b, gf))) break;
                 HCHK(hipMemcpyAsync(b->B, hB.data() + (size_t)n * r, sizeof(cplx) * n, hipMemcpyHostToDevice, st));
                 if ((rc = factor_and_solve(ctx, b, 1, direct))) break;
                 HCHK(hipMemcpyAsync(hphi.data() + (size_t)n * r, b->Y, sizeof(cplx) * n, hipMemcpyDeviceToHost, st));
                 HCHK(hipStreamSynchronize(st));
             }
-            (void)phi_all;
             if (rc) break;
             HCHK(hipMemcpyAsync(b->Y, hphi.data(), sizeof(cplx) * n * R, hipMemcpyHostToDevice, st));
         }
         if (ndof > 0) {
             Phase ph(ctx, "forces");
-            mhx::launch_forces(st, n, b->geom, b->dofs, ndof, b->Y, R, ctx->rho, w, k, ctx->g, b->betas, (int)nb, b->F,
-                               b->FK);
+            mhx::launch_forces(st, n, b->geom, b->dofs, ndof, b->Y, b->nR(), R, ctx->rho, b->d_ws, b->d_ks, bt, ctx->g,
+                               b->betas, (int)nb, b->F, b->FK);
         }
         HCHK(hipGetLastError());
-        HCHK(hipMemcpyAsync(F.data(), b->F, sizeof(cplx) * ndof * R, hipMemcpyDeviceToHost, st));
-        if (nb) HCHK(hipMemcpyAsync(FK.data(), b->FK, sizeof(cplx) * ndof * nb, hipMemcpyDeviceToHost, st));
+        HCHK(hipMemcpyAsync(F.data(), b->F, sizeof(cplx) * ndof * R * bt, hipMemcpyDeviceToHost, st));
+        if (nb) HCHK(hipMemcpyAsync(FK.data(), b->FK, sizeof(cplx) * ndof * nb * bt, hipMemcpyDeviceToHost, st));
         HCHK(hipStreamSynchronize(st));
-        // added_mass / radiation_damping (src/problems_and_results.jl:207-208), packing (:325-380)
-        for (int r = 0; r < R; r++)
-            for (int d = 0; d < ndof; d++) {
-                const cplx f = F[d + (size_t)ndof * r];
-                if (r < ndof) {
-                    A_out[d + (size_t)ndof * (r + (size_t)ndof * iw)] = f.x / (w * w);
-                    B_out[d + (size_t)ndof * (r + (size_t)ndof * iw)] = f.y / w;
-                } else {
-                    const size_t o = d + (size_t)ndof * (iw + (size_t)nw * (r - ndof));
-                    Fd_out[2 * o] = f.x;
-                    Fd_out[2 * o + 1] = f.y;
-                }
-            }
-        for (int bb = 0; bb < nb; bb++)
-            for (int d = 0; d < ndof; d++) {
-                const size_t o = d + (size_t)ndof * (iw + (size_t)nw * bb);
-                Fk_out[2 * o] = FK[d + (size_t)ndof * bb].x;
-                Fk_out[2 * o + 1] = FK[d + (size_t)ndof * bb].y;
-            }
+        for (int q = 0; q < bt; q++) {
+            std::vector<cplx> Fq(F.begin() + (size_t)q * ndof * R, F.begin() + (size_t)(q + 1) * ndof * R);
+            std::vector<cplx> FKq(FK.begin() + (size_t)q * ndof * nb, FK.begin() + (size_t)(q + 1) * ndof * nb);
+            pack(Fq, FKq, ndof, R, (int)nb, w0 + q, nw, omegas[w0 + q], A_out, B_out, Fd_out, Fk_out);
+        }
     }
     flush_timing(ctx);
     return rc;
@@ -532,6 +577,14 @@ void mhx_reset_timing(mhx_ctx* ctx) {
     flush_timing(ctx);
     ctx->phases.clear();
     ctx->quad_work[0] = ctx->quad_work[1] = ctx->quad_work[2] = 0;
+}
+
+int mhx_set_batch(mhx_ctx* ctx, int64_t max_items, double max_bytes) {
+    if (!ctx) return MHX_EINVAL;
+    if (max_items < 1 || !(max_bytes > 0)) return fail(ctx, MHX_EINVAL, "batch limits must be positive");
+    ctx->max_batch = max_items;
+    ctx->batch_bytes = max_bytes;
+    return MHX_OK;
 }
 
 int mhx_exact_work(mhx_ctx* ctx, int64_t counts[3], double* flops) {

@@ -99,8 +99,12 @@ void launch_rankine(hipStream_t st, int n, const PanelGeom* geom, int direct, do
 // _wu_sd (wu.jl:112-127) + the scale of _assemble_wu_centers_loop (matrix_assembly.jl:343-369)
 __global__ void __launch_bounds__(256) combine_wu_kernel(int n, const PanelGeom* __restrict__ geom,
                                                          const double* __restrict__ SR, const double* __restrict__ DR,
-                                                         double k, int direct, int include_identity,
-                                                         cplx* __restrict__ S, cplx* __restrict__ D) {
+                                                         const double* __restrict__ ks, int direct,
+                                                         int include_identity, cplx* __restrict__ Sb,
+                                                         cplx* __restrict__ Db, int64_t sMat) {
+    const double k = ks[blockIdx.z];
+    cplx* S = Sb + sMat * blockIdx.z;
+    cplx* D = Db + sMat * blockIdx.z;
     __shared__ PanelGeom sg[TJ];
     const int j0 = blockIdx.y * TJ;
     stage_columns(geom, n, j0, sg);
@@ -138,20 +142,24 @@ __global__ void __launch_bounds__(256) combine_wu_kernel(int n, const PanelGeom*
     }
 }
 
-void launch_combine_wu(hipStream_t st, int n, const PanelGeom* geom, const double* SR, const double* DR, double k,
-                       int direct, int include_identity, cplx* S, cplx* D) {
-    dim3 grid((n + 255) / 256, (n + TJ - 1) / TJ);
-    combine_wu_kernel<<<grid, 256, 0, st>>>(n, geom, SR, DR, k, direct, include_identity, S, D);
+void launch_combine_wu(hipStream_t st, int n, const PanelGeom* geom, const double* SR, const double* DR,
+                       const double* ks, int batch, int direct, int include_identity, cplx* S, cplx* D,
+                       int64_t sMat) {
+    dim3 grid((n + 255) / 256, (n + TJ - 1) / TJ, batch);
+    combine_wu_kernel<<<grid, 256, 0, st>>>(n, geom, SR, DR, ks, direct, include_identity, S, D, sMat);
 }
 
 // Exact Guevel-Delhommeau dimensionless (G, dG/dr) per unordered panel pair.
 // The reduced coordinates are bitwise symmetric in (i, j), so one evaluation
 // serves both matrix entries.
-__global__ void __launch_bounds__(256) delhommeau_table_kernel(int n, const PanelGeom* __restrict__ geom, double k,
-                                                               double rtol, double atol, cplx* __restrict__ WG,
-                                                               cplx* __restrict__ WR,
-                                                               unsigned long long* __restrict__ status) {
+__global__ void __launch_bounds__(256) delhommeau_table_kernel(int n, const PanelGeom* __restrict__ geom,
+                                                               const double* __restrict__ ks, double rtol, double atol,
+                                                               cplx* __restrict__ WGb, cplx* __restrict__ WRb,
+                                                               int64_t sMat, unsigned long long* __restrict__ status) {
     if (blockIdx.x > blockIdx.y) return;
+    const double k = ks[blockIdx.z];
+    cplx* WG = WGb + sMat * blockIdx.z;
+    cplx* WR = WRb + sMat * blockIdx.z;
     const int i = blockIdx.x * 16 + threadIdx.x;
     const int j = blockIdx.y * 16 + threadIdx.y;
     QuadWork wk;
@@ -188,19 +196,25 @@ __global__ void __launch_bounds__(256) delhommeau_table_kernel(int n, const Pane
     }
 }
 
-void launch_delhommeau_table(hipStream_t st, int n, const PanelGeom* geom, double k, double rtol, double atol, cplx* WG,
-                             cplx* WR, unsigned long long* status) {
-    dim3 grid((n + 15) / 16, (n + 15) / 16);
-    delhommeau_table_kernel<<<grid, dim3(16, 16), 0, st>>>(n, geom, k, rtol, atol, WG, WR, status);
+void launch_delhommeau_table(hipStream_t st, int n, const PanelGeom* geom, const double* ks, int batch, double rtol,
+                             double atol, cplx* WG, cplx* WR, int64_t sMat, unsigned long long* status) {
+    dim3 grid((n + 15) / 16, (n + 15) / 16, batch);
+    delhommeau_table_kernel<<<grid, dim3(16, 16), 0, st>>>(n, geom, ks, rtol, atol, WG, WR, sMat, status);
 }
 
 // with_reduced_coordinates_derivative (abstract_greens_function.jl:140-185) and
 // the one-point integrals (exact_Guevel_Delhommeau.jl:61-75) from a (G, dG/dr) table
 __global__ void __launch_bounds__(256) combine_table_kernel(int n, const PanelGeom* __restrict__ geom,
                                                             const double* __restrict__ SR, const double* __restrict__ DR,
-                                                            double k, int direct, int include_identity,
-                                                            const cplx* __restrict__ WG, const cplx* __restrict__ WR,
-                                                            cplx* __restrict__ S, cplx* __restrict__ D) {
+                                                            const double* __restrict__ ks, int direct,
+                                                            int include_identity, const cplx* __restrict__ WGb,
+                                                            const cplx* __restrict__ WRb, cplx* __restrict__ Sb,
+                                                            cplx* __restrict__ Db, int64_t sMat) {
+    const double k = ks[blockIdx.z];
+    const cplx* WG = WGb + sMat * blockIdx.z;
+    const cplx* WR = WRb + sMat * blockIdx.z;
+    cplx* S = Sb + sMat * blockIdx.z;
+    cplx* D = Db + sMat * blockIdx.z;
     __shared__ PanelGeom sg[TJ];
     const int j0 = blockIdx.y * TJ;
     stage_columns(geom, n, j0, sg);
@@ -239,18 +253,21 @@ __global__ void __launch_bounds__(256) combine_table_kernel(int n, const PanelGe
     }
 }
 
-void launch_combine_table(hipStream_t st, int n, const PanelGeom* geom, const double* SR, const double* DR, double k,
-                          int direct, int include_identity, const cplx* WG, const cplx* WR, cplx* S, cplx* D) {
-    dim3 grid((n + 255) / 256, (n + TJ - 1) / TJ);
-    combine_table_kernel<<<grid, 256, 0, st>>>(n, geom, SR, DR, k, direct, include_identity, WG, WR, S, D);
+void launch_combine_table(hipStream_t st, int n, const PanelGeom* geom, const double* SR, const double* DR,
+                          const double* ks, int batch, int direct, int include_identity, const cplx* WG,
+                          const cplx* WR, cplx* S, cplx* D, int64_t sMat) {
+    dim3 grid((n + 255) / 256, (n + TJ - 1) / TJ, batch);
+    combine_table_kernel<<<grid, 256, 0, st>>>(n, geom, SR, DR, ks, direct, include_identity, WG, WR, S, D, sMat);
 }
 
 // ---------------------------------------------------------------------------
 // boundary conditions: radiation_bc (waves.jl:183-211), _airy_bc (waves.jl:266-282)
 // ---------------------------------------------------------------------------
 __global__ void bc_kernel(int n, const PanelGeom* __restrict__ geom, const double* __restrict__ dofs, int ndof,
-                          const double* __restrict__ betas, int nb, double omega, double k, double g,
-                          cplx* __restrict__ B) {
+                          const double* __restrict__ betas, int nb, const double* __restrict__ ws,
+                          const double* __restrict__ ks, double g, cplx* __restrict__ Bb, int64_t sB) {
+    const double omega = ws[blockIdx.z], k = ks[blockIdx.z];
+    cplx* B = Bb + sB * blockIdx.z;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const PanelGeom& gi = geom[i];
@@ -277,8 +294,8 @@ __global__ void bc_kernel(int n, const PanelGeom* __restrict__ geom, const doubl
 }
 
 void launch_bc(hipStream_t st, int n, const PanelGeom* geom, const double* dofs, int ndof, const double* betas, int nb,
-               double omega, double k, double g, cplx* B) {
-    bc_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, geom, dofs, ndof, betas, nb, omega, k, g, B);
+               const double* ws, const double* ks, int batch, double g, cplx* B, int64_t sB) {
+    bc_kernel<<<dim3((n + 255) / 256, 1, batch), 256, 0, st>>>(n, geom, dofs, ndof, betas, nb, ws, ks, g, B, sB);
 }
 
 // ---------------------------------------------------------------------------
@@ -289,8 +306,12 @@ void launch_bc(hipStream_t st, int n, const PanelGeom* geom, const double* dofs,
 #define XCH 64
 
 template <int RB>
-__global__ void __launch_bounds__(256) matmul_nr_kernel(int n, int R, const cplx* __restrict__ S,
-                                                        const cplx* __restrict__ X, cplx* __restrict__ part) {
+__global__ void __launch_bounds__(256) matmul_nr_kernel(int n, int R, const cplx* __restrict__ Sb, int64_t sS,
+                                                        const cplx* __restrict__ Xb, int64_t sX,
+                                                        cplx* __restrict__ partb, int64_t sP) {
+    const cplx* S = Sb + sS * blockIdx.z;
+    const cplx* X = Xb + sX * blockIdx.z;
+    cplx* part = partb + sP * blockIdx.z;
     __shared__ cplx xs[XCH][RB];
     const int i = blockIdx.x * 256 + threadIdx.x;
     const int s = blockIdx.y;
@@ -320,7 +341,10 @@ __global__ void __launch_bounds__(256) matmul_nr_kernel(int n, int R, const cplx
             if (c < R) part[((int64_t)s * R + c) * n + i] = acc[c];
 }
 
-__global__ void reduce_parts_kernel(int n, int R, const cplx* __restrict__ part, cplx* __restrict__ Y) {
+__global__ void reduce_parts_kernel(int n, int R, const cplx* __restrict__ partb, int64_t sP, cplx* __restrict__ Yb,
+                                    int64_t sY) {
+    const cplx* part = partb + sP * blockIdx.z;
+    cplx* Y = Yb + sY * blockIdx.z;
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (t >= (int64_t)n * R) return;
     cplx s = cmk(0.0, 0.0);
@@ -328,16 +352,17 @@ __global__ void reduce_parts_kernel(int n, int R, const cplx* __restrict__ part,
     Y[t] = s;
 }
 
-void launch_matmul_nr(hipStream_t st, int n, int R, const cplx* S, const cplx* X, cplx* Y, cplx* work) {
-    dim3 grid((n + 255) / 256, NSPLIT);
+void launch_matmul_nr(hipStream_t st, int n, int R, const cplx* S, int64_t sS, const cplx* X, int64_t sX, cplx* Y,
+                      int64_t sY, cplx* work, int64_t sW, int batch) {
+    dim3 grid((n + 255) / 256, NSPLIT, batch);
     if (R <= 4)
-        matmul_nr_kernel<4><<<grid, 256, 0, st>>>(n, R, S, X, work);
+        matmul_nr_kernel<4><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW);
     else if (R <= 8)
-        matmul_nr_kernel<8><<<grid, 256, 0, st>>>(n, R, S, X, work);
+        matmul_nr_kernel<8><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW);
     else
-        matmul_nr_kernel<16><<<grid, 256, 0, st>>>(n, R, S, X, work);
+        matmul_nr_kernel<16><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW);
     const int64_t tot = (int64_t)n * R;
-    reduce_parts_kernel<<<(tot + 255) / 256, 256, 0, st>>>(n, R, work, Y);
+    reduce_parts_kernel<<<dim3((tot + 255) / 256, 1, batch), 256, 0, st>>>(n, R, work, sW, Y, sY);
 }
 
 // ---------------------------------------------------------------------------
@@ -346,9 +371,14 @@ void launch_matmul_nr(hipStream_t st, int n, int R, const cplx* S, const cplx* X
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(256) forces_kernel(int n, const PanelGeom* __restrict__ geom,
                                                      const double* __restrict__ dofs, int ndof,
-                                                     const cplx* __restrict__ phi, int R, double rho, double omega,
-                                                     double k, double g, const double* __restrict__ betas, int nb,
-                                                     cplx* __restrict__ F, cplx* __restrict__ FK) {
+                                                     const cplx* __restrict__ phib, int64_t sPhi, int R, double rho,
+                                                     const double* __restrict__ ws, const double* __restrict__ ks,
+                                                     double g, const double* __restrict__ betas, int nb,
+                                                     cplx* __restrict__ Fb, cplx* __restrict__ FKb) {
+    const double omega = ws[blockIdx.z], k = ks[blockIdx.z];
+    const cplx* phi = phib + sPhi * blockIdx.z;
+    cplx* F = Fb + (int64_t)gridDim.x * R * blockIdx.z;
+    cplx* FK = FKb + (int64_t)gridDim.x * nb * blockIdx.z;
     const int d = blockIdx.x;       // influenced dof
     const int r = blockIdx.y;       // right-hand side (r < R) or FK heading (R + b)
     const double* dof = dofs + (int64_t)3 * n * d;
@@ -389,10 +419,11 @@ __global__ void __launch_bounds__(256) forces_kernel(int n, const PanelGeom* __r
     }
 }
 
-void launch_forces(hipStream_t st, int n, const PanelGeom* geom, const double* dofs, int ndof, const cplx* phi, int R,
-                   double rho, double omega, double k, double g, const double* betas, int nb, cplx* F, cplx* FK) {
-    dim3 grid(ndof, R + nb);
-    forces_kernel<<<grid, 256, 0, st>>>(n, geom, dofs, ndof, phi, R, rho, omega, k, g, betas, nb, F, FK);
+void launch_forces(hipStream_t st, int n, const PanelGeom* geom, const double* dofs, int ndof, const cplx* phi,
+                   int64_t sPhi, int R, double rho, const double* ws, const double* ks, int batch, double g,
+                   const double* betas, int nb, cplx* F, cplx* FK) {
+    dim3 grid(ndof, R + nb, batch);
+    forces_kernel<<<grid, 256, 0, st>>>(n, geom, dofs, ndof, phi, sPhi, R, rho, ws, ks, g, betas, nb, F, FK);
 }
 
 }  // namespace mhx

@@ -9,27 +9,34 @@ void launch_panel_geometry(hipStream_t st, int n, const double* verts, const int
                            const double* normals, const double* areas, const double* radii, PanelGeom* geom);
 // Rankine + free-surface image parts (real, wavenumber independent): SR and D_R (direct) or K_R (indirect)
 void launch_rankine(hipStream_t st, int n, const PanelGeom* geom, int direct, double* SR, double* DR);
+// Batched launches: blockIdx.z indexes a frequency; ks / ws are device arrays of
+// wavenumbers / frequencies; matrices of item z start at base + z * stride.
 // S = SR + S_wave, D = DR (+1/2 I) + D_wave for the Wu wave term
-void launch_combine_wu(hipStream_t st, int n, const PanelGeom* geom, const double* SR, const double* DR, double k,
-                       int direct, int include_identity, cplx* S, cplx* D);
+void launch_combine_wu(hipStream_t st, int n, const PanelGeom* geom, const double* SR, const double* DR,
+                       const double* ks, int batch, int direct, int include_identity, cplx* S, cplx* D,
+                       int64_t sMat);
 // exact Guevel-Delhommeau pair table (G, dG/dr) for i <= j mirrored to j > i.
 // status[0]: nonzero if an interval budget was exhausted; status[1..3] += integrand
 // evaluations, power-series terms, continued-fraction steps (status is not cleared here)
-void launch_delhommeau_table(hipStream_t st, int n, const PanelGeom* geom, double k, double rtol, double atol, cplx* WG,
-                             cplx* WR, unsigned long long* status);
+void launch_delhommeau_table(hipStream_t st, int n, const PanelGeom* geom, const double* ks, int batch, double rtol,
+                             double atol, cplx* WG, cplx* WR, int64_t sMat, unsigned long long* status);
 // S, D from a pair table of (G, dG/dr) (Delhommeau or surrogate)
-void launch_combine_table(hipStream_t st, int n, const PanelGeom* geom, const double* SR, const double* DR, double k,
-                          int direct, int include_identity, const cplx* WG, const cplx* WR, cplx* S, cplx* D);
+void launch_combine_table(hipStream_t st, int n, const PanelGeom* geom, const double* SR, const double* DR,
+                          const double* ks, int batch, int direct, int include_identity, const cplx* WG,
+                          const cplx* WR, cplx* S, cplx* D, int64_t sMat);
 
 // ---- right-hand sides, products, forces (mhx_assembly.hip)
 // B[:, r] for r < ndof: -i omega (n . dof_r);  B[:, ndof+b]: Airy diffraction BC for heading b
 void launch_bc(hipStream_t st, int n, const PanelGeom* geom, const double* dofs, int ndof, const double* betas, int nb,
-               double omega, double k, double g, cplx* B);
-// Y = S X (n x n times n x R), split-K partials in work (nsplit * n * R complex)
-void launch_matmul_nr(hipStream_t st, int n, int R, const cplx* S, const cplx* X, cplx* Y, cplx* work);
-// F[d + ndof*r] = sum_i (i rho omega phi[i, r]) * -(dof_d,i . n_i) * a_i ; FK[d + ndof*b] for the Airy pressure
-void launch_forces(hipStream_t st, int n, const PanelGeom* geom, const double* dofs, int ndof, const cplx* phi, int R,
-                   double rho, double omega, double k, double g, const double* betas, int nb, cplx* F, cplx* FK);
+               const double* ws, const double* ks, int batch, double g, cplx* B, int64_t sB);
+// Y = S X (n x n times n x R); split-K partials in work (16 * n * R complex per item, stride sW)
+void launch_matmul_nr(hipStream_t st, int n, int R, const cplx* S, int64_t sS, const cplx* X, int64_t sX, cplx* Y,
+                      int64_t sY, cplx* work, int64_t sW, int batch);
+// F[d + ndof*r] = sum_i (i rho omega phi[i, r]) * -(dof_d,i . n_i) * a_i ; FK[d + ndof*b] for the Airy
+// pressure; item z writes F + z*ndof*R and FK + z*ndof*nb
+void launch_forces(hipStream_t st, int n, const PanelGeom* geom, const double* dofs, int ndof, const cplx* phi,
+                   int64_t sPhi, int R, double rho, const double* ws, const double* ks, int batch, double g,
+                   const double* betas, int nb, cplx* F, cplx* FK);
 
 // ---- dense LU and solves (mhx_lu.hip)
 void zgemm(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cplx* A, int lda, int64_t sA, const cplx* B,

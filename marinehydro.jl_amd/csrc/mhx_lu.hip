@@ -298,7 +298,7 @@ __global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, 
             for (int v = 0; v < 4; v++) {
                 double cr = 0.0, ci = 0.0;
                 if (ACCUM) {
-                    int r = row0 + wm + bi * 16 + 4 * lk + v, c = col0 + wn + bj * 16 + li;
+                    int r = row0 + wm + bi * 16 + 4 * v + lk, c = col0 + wn + bj * 16 + li;
                     if (r < m && c < nn) {
                         cplx t = C[r + (int64_t)ldc * c];
                         cr = t.x; ci = t.y;
@@ -358,7 +358,7 @@ __global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, 
         for (int bj = 0; bj < 2; bj++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                int r = row0 + wm + bi * 16 + 4 * lk + v, c = col0 + wn + bj * 16 + li;
+                int r = row0 + wm + bi * 16 + 4 * v + lk, c = col0 + wn + bj * 16 + li;
                 if (r < m && c < nn) C[r + (int64_t)ldc * c] = cmk(accr[bi][bj][v], acci[bi][bj][v]);
             }
 }

@@ -24,10 +24,11 @@ GF_WU = 0
 GF_DELHOMMEAU = 1
 MODE_PER_OMEGA = 0
 MODE_PER_PROBLEM = 1
+MODE_BATCHED = 2
 
 EXPORTED = ["mhx_create", "mhx_destroy", "mhx_last_error", "mhx_set_constants", "mhx_assemble", "mhx_solve",
             "mhx_lu_solve", "mhx_hydro_coeffs", "mhx_body_create", "mhx_body_destroy", "mhx_body_coeffs",
-            "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work"]
+            "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work", "mhx_set_batch"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int64)
@@ -86,6 +87,7 @@ def lib():
         L.mhx_reset_timing.argtypes = [vp]
         L.mhx_reset_timing.restype = None
         L.mhx_exact_work.argtypes = [vp, _ip, _dp]
+        L.mhx_set_batch.argtypes = [vp, C.c_int64, C.c_double]
         _lib = L
     return _lib
 
@@ -136,6 +138,9 @@ class Context:
         fl = C.c_double()
         self.check(lib().mhx_exact_work(self.h, counts.ctypes.data_as(_ip), C.byref(fl)))
         return counts, fl.value
+
+    def set_batch(self, max_items, max_bytes=64.0 * 2 ** 30):
+        self.check(lib().mhx_set_batch(self.h, int(max_items), float(max_bytes)))
 
     def reset_timing(self):
         lib().mhx_reset_timing(self.h)

@@ -316,7 +316,8 @@ def hydrodynamic_coefficients(floatingbody, parameters, formulation=None, schedu
     """hydrodynamic_coefficients(body, (wave_frequencies, radiating_dofs, wave_directions), formulation)
     (src/problems_and_results.jl:464-495).  Layouts as the reference:
     added_mass / radiation_damping (n_inf, n_rad, n_omega, 1); forces (n_inf, n_omega, n_beta, 1).
-    schedule="per_problem" reproduces the reference's one assembly + LU per problem."""
+    schedule="per_problem" reproduces the reference's one assembly + LU per problem;
+    schedule="batched" processes the frequencies in chunks with a batched LU."""
     formulation = DirectBEM() if formulation is None else formulation
     ws = np.atleast_1d(np.asarray(parameters["wave_frequencies"], dtype=np.float64))
     inf = list(parameters.get("influenced_dofs", floatingbody.dof_names))
@@ -326,7 +327,8 @@ def hydrodynamic_coefficients(floatingbody, parameters, formulation=None, schedu
         if nm not in floatingbody.dofs:
             raise ValueError("each influenced_dofs Symbol must be a key of floatingbody.dofs")
     names = inf + [r for r in rad if r not in inf]
-    mode = {"per_omega": _lib.MODE_PER_OMEGA, "per_problem": _lib.MODE_PER_PROBLEM}[schedule]
+    mode = {"per_omega": _lib.MODE_PER_OMEGA, "per_problem": _lib.MODE_PER_PROBLEM,
+            "batched": _lib.MODE_BATCHED}[schedule]
     A, B, Fd, Fk = _body_coeffs(floatingbody, names, ws, betas, formulation.gf, formulation.direct, mode,
                                 SETTINGS.rho, SETTINGS.g)
     ii = [names.index(nm) for nm in inf]
