@@ -77,14 +77,47 @@ def cpu_baseline(mesh, omegas, rows, threads):
                       f"({t_lu:.2f} s, numpy's OpenBLAS)"}
 
 
+def measure(ctx, run, ws, n, gf):
+    """One timed call over the frequencies ws; returns rate, phase table and roofline of the
+    dominant kernel (HIP events on the library's stream)."""
+    ctx.reset_timing()
+    t0 = time.perf_counter()
+    A, B, Fd, Fk = run(ws)
+    elapsed = time.perf_counter() - t0
+    assert np.all(np.isfinite(A)) and np.all(np.isfinite(Fd))
+    phases = {p: ctx.phase_time(p) for p in ("wave_table", "wave", "lu", "solve", "forces", "rankine")}
+    counts, flops = ctx.exact_work()
+    tw, cw = phases["wave_table"]
+    tl, cl = phases["lu"]
+    nw = len(ws)
+    if gf == "ExactGuevelDelhommeau" and tw >= tl:
+        ach = flops / (tw * 1e-3) / 1e12
+        roof = {"bound": "valu", "kernel": "delhommeau_table_kernel", "achieved": ach,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                "avg_launch_ms": tw / cw, "launches": cw, "frequencies_per_launch": nw / cw,
+                "work_per_frequency": {"integrand_evals": int(counts[0] // nw), "series_steps": int(counts[1] // nw),
+                                       "cf_steps": int(counts[2] // nw)}}
+    else:
+        fl = 8.0 / 3.0 * n ** 3 * nw
+        ach = fl / (tl * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": "getrf_batched (zgemm_mfma_kernel trailing updates)", "achieved": ach,
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                "avg_launch_ms": tl / cl, "launches": cl}
+    lu_tflops = 8.0 / 3.0 * n ** 3 * nw / (tl * 1e-3) / 1e12 if tl else None
+    per_w = {k: v[0] / nw for k, v in phases.items()}
+    return nw / elapsed, elapsed, per_w, roof, lu_tflops
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=8)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=16, help="omega-solves timed (frequencies of the sweep)")
+    ap.add_argument("--warmup", type=int, default=16, help="untimed omega-solves (the first W timed frequencies)")
     ap.add_argument("--gf", default="ExactGuevelDelhommeau", choices=["ExactGuevelDelhommeau", "Wu"])
+    ap.add_argument("--schedule", default="batched", choices=["batched", "per_omega"])
     ap.add_argument("--cpu-rows", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain per-omega comparison run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -105,47 +138,24 @@ def main():
     form = mhx.DirectBEM(args.gf)
     idx = np.linspace(0, len(omegas) - 1, args.steps).round().astype(int)
     steps_w = omegas[idx]
-    warm_w = omegas[np.linspace(0, len(omegas) - 1, max(args.warmup, 1)).round().astype(int)][:args.warmup]
+    mode = {"batched": _lib.MODE_BATCHED, "per_omega": _lib.MODE_PER_OMEGA}[args.schedule]
+    ctx.set_batch(max(args.steps, 1))
 
-    def run(ws):
-        return mhx.api._body_coeffs(body, names, ws, [0.0], form.gf, form.direct, _lib.MODE_PER_OMEGA, 1025.0, 9.81)
+    def run_with(m):
+        return lambda ws: mhx.api._body_coeffs(body, names, ws, [0.0], form.gf, form.direct, m, 1025.0, 9.81)
 
     log(f"bench: n={n}, warmup over {args.warmup} frequencies")
-    if args.warmup:
-        run(warm_w)
-    else:
-        run(omegas[:1])  # body upload + Rankine part outside the timed region
-    ctx.reset_timing()
-    log(f"bench: timing {args.steps} omega-solves")
-    t0 = time.perf_counter()
-    A, B, Fd, Fk = run(steps_w)
-    elapsed = time.perf_counter() - t0
-    ms_step = elapsed * 1e3 / args.steps
-    value = args.steps / elapsed
-    phases = {p: ctx.phase_time(p) for p in ("wave_table", "wave", "lu", "solve", "forces", "rankine")}
-    counts, flops = ctx.exact_work()
-    log("phases (ms total, launches):", phases)
-    assert np.all(np.isfinite(A)) and np.all(np.isfinite(Fd))
-
-    # dominant kernel roofline
-    tw, cw = phases["wave_table"]
-    tl, cl = phases["lu"]
-    if args.gf == "ExactGuevelDelhommeau" and tw >= tl:
-        dur = tw / cw * 1e-3
-        ach = flops / cw / dur / 1e12
-        roof = {"bound": "valu", "kernel": "delhommeau_table_kernel", "achieved": ach,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
-                "avg_ms": tw / cw, "work_per_launch": {"integrand_evals": int(counts[0] // cw),
-                                                      "series_terms": int(counts[1] // cw),
-                                                      "cf_steps": int(counts[2] // cw)}}
-    else:
-        dur = tl / cl * 1e-3
-        fl = 8.0 / 3.0 * n ** 3
-        ach = fl / dur / 1e12
-        roof = {"bound": "mfma", "kernel": "getrf (zgemm_mfma_kernel trailing updates)", "achieved": ach,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
-                "avg_ms": tl / cl}
-    lu_tflops = 8.0 / 3.0 * n ** 3 / (tl / cl * 1e-3) / 1e12 if cl else None
+    warm = steps_w[:max(args.warmup, 1)]
+    run_with(mode)(warm)   # body upload, Rankine part, buffers: outside the timed region
+    log(f"bench: timing {args.steps} omega-solves ({args.schedule})")
+    value, elapsed, phases, roof, lu_tflops = measure(ctx, run_with(mode), steps_w, n, args.gf)
+    log("per-frequency phase ms:", phases)
+    plain = None
+    if not args.no_plain and mode != _lib.MODE_PER_OMEGA:
+        log("bench: plain per-omega comparison")
+        kp = min(4, args.steps)
+        pv, pel, pph, proof, plu = measure(ctx, run_with(_lib.MODE_PER_OMEGA), steps_w[:kp], n, args.gf)
+        plain = {"value": pv, "steps": kp, "phases_ms_per_omega": pph, "lu_tflops": plu}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -154,14 +164,17 @@ def main():
 
     out = {
         "metric": METRIC, "value": value, "unit": "omega-solves/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": ms_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"plain GPU per-omega path ({args.gf}, direct): capped cylinder R=4 draft=12, "
-                               f"{n} panels, 6 dofs, beta=0, 64-frequency sweep 0.1-3.0 rad/s",
-                   "panels": n, "dofs": 6, "headings": 1, "schedule": "per_omega",
+        "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{args.schedule} GPU path, exact FP64 wave term ({args.gf}), direct method: "
+                               f"capped cylinder R=4 draft=12, {n} panels, 6 dofs, beta=0, "
+                               f"{args.steps} of the 64 frequencies in 0.1-3.0 rad/s",
+                   "panels": n, "dofs": 6, "headings": 1, "schedule": args.schedule,
                    "parallelism": "single GPU"},
-        "phases_ms": {k: v[0] / max(v[1], 1) for k, v in phases.items()},
+        "phases_ms_per_omega": phases,
         "lu_tflops": lu_tflops,
+        "plain_gpu_per_omega": plain,
+        "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -51,7 +51,7 @@ struct mhx_body {
     unsigned long long* status = nullptr;
     int64_t nn() const { return (int64_t)n * n; }
     int64_t nR() const { return (int64_t)n * cap_R; }
-    int64_t sW() const { return 18 * nR(); }  // 16 n R split-K partials + 2 n R getrs work
+    int64_t sW() const { return 16 * nR() + mhx::getrs_work(n, cap_R); }  // split-K partials + getrs work
 };
 
 namespace {
@@ -165,7 +165,7 @@ int ensure_work(mhx_ctx* ctx, mhx_body* b, int R, int nb, bool table, int batch 
     if ((rc = dalloc(ctx, &b->S, nn * B)) || (rc = dalloc(ctx, &b->D, nn * B))) return rc;
     if (tab && ((rc = dalloc(ctx, &b->WG, nn * B)) || (rc = dalloc(ctx, &b->WR, nn * B)))) return rc;
     if ((rc = dalloc(ctx, &b->B, n * cr * B)) || (rc = dalloc(ctx, &b->Y, n * cr * B)) ||
-        (rc = dalloc(ctx, &b->work, 18 * n * cr * B)) || (rc = dalloc(ctx, &b->F, (size_t)(b->ndof ? b->ndof : 1) * cr * B)) ||
+        (rc = dalloc(ctx, &b->work, (size_t)b->sW() * B)) || (rc = dalloc(ctx, &b->F, (size_t)(b->ndof ? b->ndof : 1) * cr * B)) ||
         (rc = dalloc(ctx, &b->FK, (size_t)(b->ndof ? b->ndof : 1) * cnb * B)) || (rc = dalloc(ctx, &b->betas, (size_t)cnb)) ||
         (rc = dalloc(ctx, &b->d_ws, B)) || (rc = dalloc(ctx, &b->d_ks, B)) || (rc = dalloc(ctx, &b->ipiv, n * B)) ||
         (rc = dalloc(ctx, &b->perm, n * B)) || (rc = dalloc(ctx, &b->wperm, (size_t)mhx::kPermWork * B)) ||
@@ -394,7 +394,7 @@ int mhx_lu_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* A, int tra
     int rc;
     auto cleanup = [&]() { dfree(dA); dfree(dB); dfree(dW); dfree(ipiv); dfree(perm); dfree(wperm); dfree(info); };
     if ((rc = dalloc(ctx, &dA, (size_t)n * n)) || (rc = dalloc(ctx, &dB, (size_t)n * nrhs)) ||
-        (rc = dalloc(ctx, &dW, (size_t)2 * n * nrhs)) || (rc = dalloc(ctx, &ipiv, n)) || (rc = dalloc(ctx, &perm, n)) ||
+        (rc = dalloc(ctx, &dW, (size_t)mhx::getrs_work((int)n, (int)nrhs))) || (rc = dalloc(ctx, &ipiv, n)) || (rc = dalloc(ctx, &perm, n)) ||
         (rc = dalloc(ctx, &wperm, (size_t)mhx::kPermWork)) || (rc = dalloc(ctx, &info, 1))) {
         cleanup();
         return rc;

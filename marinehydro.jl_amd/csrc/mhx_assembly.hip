@@ -169,9 +169,7 @@ __global__ void __launch_bounds__(256) delhommeau_table_kernel(int n, const Pane
         double hh, vv;
         reduced_hv(gi.c[0], gi.c[1], gi.c[2], gj.c[0], gj.c[1], gj.c[2], k, hh, vv);
         cplx Gr, G;
-        bool ok1 = hquad<true>(hh, vv, rtol, atol, Gr, wk);
-        bool ok0 = hquad<false>(hh, vv, rtol, atol, G, wk);
-        if (!(ok0 && ok1)) atomicOr(status, 1ull);
+        if (!hquad_pair(hh, vv, rtol, atol, G, Gr, wk)) atomicOr(status, 1ull);
         G = cscale(4.0, G);
         Gr = cscale(4.0, Gr);
         WG[i + (int64_t)n * j] = G;

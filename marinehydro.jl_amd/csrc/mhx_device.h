@@ -276,7 +276,8 @@ __device__ __forceinline__ void reduced_hv(double x0, double x1, double x2, doub
 }
 
 // ---------------------------------------------------------------------------
-// exp(z) E1(z).  Region rule: power series for |z|<=2, for Re z<=0 with |z|<=4,
+// exp(z) E1(z).  Region rule: power series (Horner on a_n = (-1)^n/(n n!)) for
+// |z|<=2, for Re z<=0 with |z|<=4,
 // and inside the cone |z|<=40, |Im z| <= -Re z (24/|z| - 0.1); elsewhere the even
 // continued fraction 1/(z+1- 1/(z+3- 4/(z+5- ...))) by modified Lentz.
 // ---------------------------------------------------------------------------
@@ -294,20 +295,25 @@ struct QuadWork {
     unsigned evals = 0, series = 0, cf = 0;
 };
 
+#define E1_QUAL __constant__ static
+#include "mhx_e1_table.h"
+
 __device__ __noinline__ cplx expE1(cplx z, QuadWork& wk) {
     if (e1_use_series(z)) {
-        cplx term = cmk(1.0, 0.0), sum = cmk(0.0, 0.0);
-        cplx mz = cmk(-z.x, -z.y);
-        for (int n = 1; n < 600; n++) {
-            double inv = 1.0 / (double)n;
-            term = cscale(inv, cmul(term, mz));
-            cplx add = cscale(inv, term);
-            sum = cadd(sum, add);
-            wk.series++;
-            if (hypot(add.x, add.y) <= 1e-17 * hypot(sum.x, sum.y)) break;
+        // E1 = -gamma - log z - sum_{n=1}^{N} a_n z^n, Horner, degree from the table at ceil|z|
+        int ir = (int)ceil(hypot(z.x, z.y));
+        ir = ir > 40 ? 40 : ir;
+        const int N = E1_DEG[ir];
+        double pr = E1_A[N - 1], pi = 0.0;
+        for (int q = N - 2; q >= 0; q--) {
+            const double tr = E1_A[q] + (z.x * pr - z.y * pi);
+            pi = z.x * pi + z.y * pr;
+            pr = tr;
         }
+        wk.series += N;
+        const double sr = z.x * pr - z.y * pi, si = z.x * pi + z.y * pr;
         cplx L = clog_(z);
-        cplx e1 = cmk(-MHX_EULER - L.x - sum.x, -L.y - sum.y);
+        cplx e1 = cmk(-MHX_EULER - L.x - sr, -L.y - si);
         return cmul(cexp_(z), e1);
     }
     const double tiny = 1e-300;
@@ -377,15 +383,57 @@ __device__ __forceinline__ cplx gk15(double a, double b, double r, double z, dou
 
 #define MHX_MAXBOX 48
 
-// Adaptive GK(7,15) on [0, pi/2]: pop the interval of largest error, bisect,
-// stop when E <= max(rtol |I|, atol); re-sum the intervals at the end.
-// Returns false if the interval budget was exhausted.
+// Both integrands at one node from a single E1 evaluation: .x/.y of G, then of dG/dr.
+// Values are bitwise those of del_integrand<false> and del_integrand<true>.
+__device__ __forceinline__ void del_integrand_both(double th, double r, double z, QuadWork& wk, cplx& fG, cplx& fR) {
+    double c = cos(th);
+    cplx zeta = cmk(z, r * c);
+    cplx ez = cexp_(zeta);
+    wk.evals++;
+    cplx w = expE1(zeta, wk);
+    cplx j = cmk(w.x / MHX_PI - ez.y, w.y / MHX_PI + ez.x);
+    fG = cmk(j.x, ez.x);
+    cplx inv = cdiv(cmk(1.0, 0.0), cmk(MHX_PI * zeta.x, MHX_PI * zeta.y));
+    fR = cmk(-c * (j.y - inv.y), -c * ez.y);
+}
+
+// GK(7,15) of both integrands on [a, b] with shared nodes; same accumulation order as gk15.
+__device__ __forceinline__ void gk15_both(double a, double b, double r, double z, cplx& IG, double& EG, cplx& IR,
+                                          double& ER, QuadWork& wk) {
+    double c = (a + b) * 0.5, d = (b - a) * 0.5;
+    cplx g0, r0;
+    del_integrand_both(c, r, z, wk, g0, r0);
+    cplx IkG = cscale(GK_WK[7], g0), IgG = cscale(GK_WG[3], g0);
+    cplx IkR = cscale(GK_WK[7], r0), IgR = cscale(GK_WG[3], r0);
+#pragma unroll 1
+    for (int i = 0; i < 7; i++) {
+        double dx = d * GK_X[i];
+        cplx gm, rm, gp, rp;
+        del_integrand_both(c - dx, r, z, wk, gm, rm);
+        del_integrand_both(c + dx, r, z, wk, gp, rp);
+        cplx fG = cadd(gm, gp), fR = cadd(rm, rp);
+        IkG = cadd(IkG, cscale(GK_WK[i], fG));
+        IkR = cadd(IkR, cscale(GK_WK[i], fR));
+        if (i & 1) {
+            IgG = cadd(IgG, cscale(GK_WG[i >> 1], fG));
+            IgR = cadd(IgR, cscale(GK_WG[i >> 1], fR));
+        }
+    }
+    IkG = cscale(d, IkG); IgG = cscale(d, IgG);
+    IkR = cscale(d, IkR); IgR = cscale(d, IgR);
+    EG = hypot(IkG.x - IgG.x, IkG.y - IgG.y);
+    ER = hypot(IkR.x - IgR.x, IkR.y - IgR.y);
+    IG = IkG;
+    IR = IkR;
+}
+
+// Adaptive GK(7,15) on [0, pi/2] from a given first interval estimate: pop the interval
+// of largest error, bisect, stop when E <= max(rtol |I|, atol); re-sum the intervals at
+// the end.  Returns false if the interval budget was exhausted.
 template <bool DERIV>
-__device__ bool hquad(double r, double z, double rtol, double atol, cplx& out, QuadWork& wk) {
+__device__ bool hquad_from(double r, double z, double rtol, double atol, cplx I0, double E, cplx& out, QuadWork& wk) {
     double ba[MHX_MAXBOX], bb[MHX_MAXBOX], be[MHX_MAXBOX];
     cplx bi[MHX_MAXBOX];
-    double E;
-    cplx I0 = gk15<DERIV>(0.0, MHX_PI / 2, r, z, E, wk);
     ba[0] = 0.0; bb[0] = MHX_PI / 2; bi[0] = I0; be[0] = E;
     int nb = 1;
     cplx It = I0;
@@ -417,10 +465,21 @@ __device__ bool hquad(double r, double z, double rtol, double atol, cplx& out, Q
     return ok;
 }
 
+// G and dG/dr of the exact wave term (both without the factor 4)
+__device__ __forceinline__ bool hquad_pair(double r, double z, double rtol, double atol, cplx& G, cplx& Gr,
+                                           QuadWork& wk) {
+    cplx IG, IR;
+    double EG, ER;
+    gk15_both(0.0, MHX_PI / 2, r, z, IG, EG, IR, ER, wk);
+    bool ok1 = hquad_from<true>(r, z, rtol, atol, IR, ER, Gr, wk);
+    bool ok0 = hquad_from<false>(r, z, rtol, atol, IG, EG, G, wk);
+    return ok0 && ok1;
+}
+
 // Algorithmic FP64 operation counts of the exact wave term (add, sub, mul = 1,
 // fma = 2, div, sqrt and each elementary function = 1), counted from the code
-// above: per integrand evaluation outside the E1 loop, per power-series term,
-// per continued-fraction step.
+// above: per integrand evaluation outside the E1 loop, per power-series Horner
+// step (one complex multiply-add), per continued-fraction step.
 #define MHX_FLOPS_PER_EVAL 41.0
-#define MHX_FLOPS_PER_SERIES 20.0
+#define MHX_FLOPS_PER_SERIES 8.0
 #define MHX_FLOPS_PER_CF 45.0

@@ -43,6 +43,8 @@ void zgemm(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cp
            int ldb, int64_t sB, cplx* C, int ldc, int64_t sC, int batch);
 int getrf_batched(hipStream_t st, int n, cplx* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
                   int* work_perm, int* info);
+// complex elements of getrs work per matrix (sW must be at least this)
+int64_t getrs_work(int n, int R);
 int getrs_batched(hipStream_t st, int trans, int n, int R, const cplx* LU, int lda, int64_t sA, const int* ipiv,
                   int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
                   int batch);

@@ -453,12 +453,59 @@ __global__ void permute_rows_kernel(int n, int R, const int* __restrict__ perm, 
     }
 }
 
-// One block-step of a triangular solve with TRS_B-row diagonal blocks.
-//  lower=1: forward with unit-lower L (rows [b0,b1) solved; rows below updated)
-//  lower=0: backward with non-unit upper U (rows [b0,b1) solved; rows above updated)
-//  conj=1 solves with the conjugate transpose (U^H is lower, L^H is unit upper).
-// Every block solves the diagonal block redundantly from Bw (not written in this
-// launch); block 0 stores the solution into X, the others update their Bw rows.
+// Inverses of the TRS_B x TRS_B diagonal blocks of L (unit lower) and U (upper),
+// in place in LDS (LAPACK trti2 order), one workgroup per (block, triangle, matrix).
+// Output: Inv[(blk * TRS_B + i) + TRS_B * nblk_rows...] stored as nblk consecutive
+// TRS_B x TRS_B column-major blocks per triangle and matrix.
+__global__ void __launch_bounds__(TRS_B) invert_diag_kernel(const cplx* __restrict__ Ab, int lda, int64_t sA, int n,
+                                                            cplx* __restrict__ Linv, cplx* __restrict__ Uinv,
+                                                            int64_t sInv) {
+    const cplx* A = Ab + sA * blockIdx.z;
+    const int blk = blockIdx.x, upper = blockIdx.y;
+    const int b0 = blk * TRS_B, nbk = min(TRS_B, n - b0);
+    cplx* out = (upper ? Uinv : Linv) + sInv * blockIdx.z + (int64_t)blk * TRS_B * TRS_B;
+    __shared__ cplx T[TRS_B][TRS_B];  // T[i][j]
+    const int t = threadIdx.x;
+    for (int j = 0; j < TRS_B; j++) {
+        cplx v = cmk(0.0, 0.0);
+        if (t < nbk && j < nbk) v = A[(b0 + t) + (int64_t)lda * (b0 + j)];
+        T[t][j] = v;
+    }
+    __syncthreads();
+    if (!upper) {
+        // unit lower: columns from the right; Inv[i][j] = -sum_{s=j+1..i} Inv[i][s] L[s][j]
+        for (int j = nbk - 2; j >= 0; j--) {
+            cplx acc = cmk(0.0, 0.0);
+            if (t > j && t < nbk) {
+                acc = T[t][j];  // s = i term with Inv[i][i] = 1
+                for (int s2 = j + 1; s2 < t; s2++) acc = cadd(acc, cmul(T[t][s2], T[s2][j]));
+            }
+            __syncthreads();
+            if (t > j && t < nbk) T[t][j] = cmk(-acc.x, -acc.y);
+            __syncthreads();
+        }
+        if (t < nbk) T[t][t] = cmk(1.0, 0.0);
+    } else {
+        // upper: columns from the left; Inv[j][j] = 1/U[j][j], Inv[i][j] = -Inv[j][j] sum_{s=i..j-1} Inv[i][s] U[s][j]
+        for (int j = 0; j < nbk; j++) {
+            const cplx djj = cdiv(cmk(1.0, 0.0), T[j][j]);
+            cplx acc = cmk(0.0, 0.0);
+            if (t < j)
+                for (int s2 = t; s2 < j; s2++) acc = cadd(acc, cmul(T[t][s2], T[s2][j]));
+            __syncthreads();
+            if (t < j) T[t][j] = cmk(-(djj.x * acc.x - djj.y * acc.y), -(djj.x * acc.y + djj.y * acc.x));
+            if (t == j) T[j][j] = djj;
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    // the tile holds the packed LU: keep only the inverted triangle
+    for (int j = 0; j < TRS_B; j++) {
+        const bool keep = t < nbk && j < nbk && (upper ? j >= t : j <= t);
+        out[t + TRS_B * j] = keep ? T[t][j] : cmk(0.0, 0.0);
+    }
+}
+
 __device__ __forceinline__ cplx tri_elem(const cplx* A, int lda, int i, int j, int conj) {
     // element (i, j) of op(A): A or A^H
     if (!conj) return A[i + (int64_t)lda * j];
@@ -466,39 +513,47 @@ __device__ __forceinline__ cplx tri_elem(const cplx* A, int lda, int i, int j, i
     return cmk(v.x, -v.y);
 }
 
+// One block-step of a triangular solve with TRS_B-row diagonal blocks.
+//  lower=1: forward (L, or U^H when conj); rows [b0,b1) solved, rows below updated
+//  lower=0: backward (U, or L^H when conj); rows [b0,b1) solved, rows above updated
+// x_b = op(Inv_b) y_b with the precomputed diagonal-block inverse (Inv = L_bb^{-1} or
+// U_bb^{-1}; op = conjugate transpose when conj).  Every block forms x_b from Bw (not
+// written in this launch); block 0 stores it into X, the others update their Bw rows.
 template <int RB>
 __global__ void __launch_bounds__(256) trs_step_kernel(const cplx* __restrict__ Ab, int lda, int64_t sA, int n, int R,
-                                                       int b0, int b1, int lower, int conj, cplx* __restrict__ Bwb,
-                                                       int64_t sB, cplx* __restrict__ Xb, int64_t sX) {
+                                                       int b0, int b1, int lower, int conj,
+                                                       const cplx* __restrict__ Invb, int64_t sInv,
+                                                       cplx* __restrict__ Bwb, int64_t sB, cplx* __restrict__ Xb,
+                                                       int64_t sX) {
     const int bz = blockIdx.z;
     const cplx* A = Ab + sA * bz;
+    const cplx* Inv = Invb + sInv * bz + (int64_t)(b0 / TRS_B) * TRS_B * TRS_B;
     cplx* Bw = Bwb + sB * bz;
     cplx* X = Xb + sX * bz;
     const int nbk = b1 - b0;
+    __shared__ cplx ys[TRS_B][RB];
     __shared__ cplx xs[TRS_B][RB];
     const int tid = threadIdx.x;
-    for (int t = tid; t < nbk * R; t += 256) {
-        int i = t % nbk, c = t / nbk;
-        xs[i][c] = Bw[(b0 + i) + (int64_t)n * c];
+    for (int t = tid; t < TRS_B * RB; t += 256) {
+        int i = t % TRS_B, c = t / TRS_B;
+        ys[i][c] = (i < nbk && c < R) ? Bw[(b0 + i) + (int64_t)n * c] : cmk(0.0, 0.0);
     }
     __syncthreads();
-    // effective triangle: lower-unit (lower && !conj), upper non-unit (!lower && !conj),
-    // U^H lower non-unit (lower && conj), L^H upper unit (!lower && conj)
-    const bool fwd = lower;
-    const bool unit = (lower && !conj) || (!lower && conj);
-    for (int s = 0; s < nbk; s++) {
-        const int r = fwd ? s : nbk - 1 - s;
-        if (!unit) {
-            if (tid < R) xs[r][tid] = cdiv(xs[r][tid], tri_elem(A, lda, b0 + r, b0 + r, conj));
-            __syncthreads();
+    for (int t = tid; t < TRS_B * RB; t += 256) {
+        int i = t % TRS_B, c = t / TRS_B;
+        cplx acc = cmk(0.0, 0.0);
+        for (int s2 = 0; s2 < nbk; s2++) {
+            cplx v;
+            if (!conj) v = Inv[i + TRS_B * s2];
+            else {
+                cplx w = Inv[s2 + TRS_B * i];
+                v = cmk(w.x, -w.y);
+            }
+            acc = cadd(acc, cmul(v, ys[s2][c]));
         }
-        for (int t = tid; t < nbk * R; t += 256) {
-            int i = t % nbk, c = t / nbk;
-            if ((fwd && i > r) || (!fwd && i < r))
-                xs[i][c] = csub(xs[i][c], cmul(tri_elem(A, lda, b0 + i, b0 + r, conj), xs[r][c]));
-        }
-        __syncthreads();
+        xs[i][c] = acc;
     }
+    __syncthreads();
     if (blockIdx.x == 0) {
         for (int t = tid; t < nbk * R; t += 256) {
             int i = t % nbk, c = t / nbk;
@@ -506,24 +561,24 @@ __global__ void __launch_bounds__(256) trs_step_kernel(const cplx* __restrict__ 
         }
         return;
     }
-    // update rows outside the block: below (fwd) or above (bwd)
+    const bool fwd = lower;
     const int row = fwd ? b1 + (blockIdx.x - 1) * 256 + tid : (blockIdx.x - 1) * 256 + tid;
     if (fwd ? (row >= n) : (row >= b0)) return;
     cplx acc[RB];
 #pragma unroll
     for (int c = 0; c < RB; c++) acc[c] = (c < R) ? Bw[row + (int64_t)n * c] : cmk(0.0, 0.0);
-    for (int s = 0; s < nbk; s++) {
-        cplx a = tri_elem(A, lda, row, b0 + s, conj);
+    for (int s2 = 0; s2 < nbk; s2++) {
+        cplx a = tri_elem(A, lda, row, b0 + s2, conj);
 #pragma unroll
-        for (int c = 0; c < RB; c++) acc[c] = csub(acc[c], cmul(a, xs[s][c]));
+        for (int c = 0; c < RB; c++) acc[c] = csub(acc[c], cmul(a, xs[s2][c]));
     }
 #pragma unroll
     for (int c = 0; c < RB; c++)
         if (c < R) Bw[row + (int64_t)n * c] = acc[c];
 }
 
-static void trs_pass(hipStream_t st, const cplx* A, int lda, int64_t sA, int n, int R, int lower, int conj, cplx* Bw,
-                     int64_t sB, cplx* X, int64_t sX, int batch) {
+static void trs_pass(hipStream_t st, const cplx* A, int lda, int64_t sA, int n, int R, int lower, int conj,
+                     const cplx* Inv, int64_t sInv, cplx* Bw, int64_t sB, cplx* X, int64_t sX, int batch) {
     const int nblk = (n + TRS_B - 1) / TRS_B;
     for (int s = 0; s < nblk; s++) {
         const int blk = lower ? s : nblk - 1 - s;
@@ -531,35 +586,43 @@ static void trs_pass(hipStream_t st, const cplx* A, int lda, int64_t sA, int n, 
         const int rest = lower ? n - b1 : b0;
         dim3 g(1 + (rest + 255) / 256, 1, batch);
         if (R <= 4)
-            trs_step_kernel<4><<<g, 256, 0, st>>>(A, lda, sA, n, R, b0, b1, lower, conj, Bw, sB, X, sX);
+            trs_step_kernel<4><<<g, 256, 0, st>>>(A, lda, sA, n, R, b0, b1, lower, conj, Inv, sInv, Bw, sB, X, sX);
         else if (R <= 8)
-            trs_step_kernel<8><<<g, 256, 0, st>>>(A, lda, sA, n, R, b0, b1, lower, conj, Bw, sB, X, sX);
+            trs_step_kernel<8><<<g, 256, 0, st>>>(A, lda, sA, n, R, b0, b1, lower, conj, Inv, sInv, Bw, sB, X, sX);
         else
-            trs_step_kernel<16><<<g, 256, 0, st>>>(A, lda, sA, n, R, b0, b1, lower, conj, Bw, sB, X, sX);
+            trs_step_kernel<16><<<g, 256, 0, st>>>(A, lda, sA, n, R, b0, b1, lower, conj, Inv, sInv, Bw, sB, X, sX);
     }
 }
 
-// B (n x R, leading dimension n) is overwritten with the solution; work: 2 * n * R complex per matrix.
+// B (n x R, leading dimension n) is overwritten with the solution.  work: 2 n R complex
+// plus 2 * ceil(n/64) * 64 * 64 complex (diagonal-block inverses) per matrix, stride sW.
+int64_t getrs_work(int n, int R) {
+    const int64_t nblk = (n + TRS_B - 1) / TRS_B;
+    return 2 * (int64_t)n * R + 2 * nblk * TRS_B * TRS_B;
+}
+
 int getrs_batched(hipStream_t st, int trans, int n, int R, const cplx* LU, int lda, int64_t sA, const int* ipiv,
                   int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
                   int batch) {
     if (R > MAXR) return -1;
     if (n <= 0 || R <= 0) return 0;
+    const int nblk = (n + TRS_B - 1) / TRS_B;
     cplx* W1 = work;
     cplx* W2 = work + (int64_t)n * R;
+    cplx* Linv = work + 2 * (int64_t)n * R;
+    cplx* Uinv = Linv + (int64_t)nblk * TRS_B * TRS_B;
+    invert_diag_kernel<<<dim3(nblk, 2, batch), TRS_B, 0, st>>>(LU, lda, sA, n, Linv, Uinv, sW);
     ipiv_to_perm_kernel<<<dim3(1, 1, batch), 256, 0, st>>>(ipiv, ipiv_stride, n, perm, perm_stride);
     dim3 gp((n + 255) / 256, 1, batch);
     if (trans == 0) {
         permute_rows_kernel<<<gp, 256, 0, st>>>(n, R, perm, perm_stride, B, sB, W1, sW, 0);
-        trs_pass(st, LU, lda, sA, n, R, 1, 0, W1, sW, W2, sW, batch);  // L: W1 -> W2
-        trs_pass(st, LU, lda, sA, n, R, 0, 0, W2, sW, B, sB, batch);   // U: W2 -> B
+        trs_pass(st, LU, lda, sA, n, R, 1, 0, Linv, sW, W1, sW, W2, sW, batch);  // L: W1 -> W2
+        trs_pass(st, LU, lda, sA, n, R, 0, 0, Uinv, sW, W2, sW, B, sB, batch);   // U: W2 -> B
     } else {
-        hipMemcpyAsync(W1, B, sizeof(cplx) * (size_t)n * R, hipMemcpyDeviceToDevice, st);
-        if (batch > 1)
-            for (int b = 1; b < batch; b++)
-                hipMemcpyAsync(W1 + sW * b, B + sB * b, sizeof(cplx) * (size_t)n * R, hipMemcpyDeviceToDevice, st);
-        trs_pass(st, LU, lda, sA, n, R, 1, 1, W1, sW, W2, sW, batch);  // U^H (lower)
-        trs_pass(st, LU, lda, sA, n, R, 0, 1, W2, sW, W1, sW, batch);  // L^H (unit upper)
+        for (int b = 0; b < batch; b++)
+            hipMemcpyAsync(W1 + sW * b, B + sB * b, sizeof(cplx) * (size_t)n * R, hipMemcpyDeviceToDevice, st);
+        trs_pass(st, LU, lda, sA, n, R, 1, 1, Uinv, sW, W1, sW, W2, sW, batch);  // U^H (lower)
+        trs_pass(st, LU, lda, sA, n, R, 0, 1, Linv, sW, W2, sW, W1, sW, batch);  // L^H (unit upper)
         permute_rows_kernel<<<gp, 256, 0, st>>>(n, R, perm, perm_stride, W1, sW, B, sB, 1);
     }
     return 0;

@@ -355,7 +355,8 @@ void ora_wu_wave_term_and_derivs(double hh, double vv, double out[6]) {
 /* ------------------------------------------------------------------------- */
 /* Complex exponential integral E1 (replaces SpecialFunctions.expint,        */
 /* exact_Guevel_Delhommeau.jl:5).  Region rule shared with the HIP kernel:   */
-/*  power series  E1 = -gamma - log z - sum (-z)^n/(n n!)  for |z|<=2, for    */
+/*  power series  E1 = -gamma - log z - sum a_n z^n, a_n = (-1)^n/(n n!)    */
+/*  (Horner, degree from a table at ceil|z|) for |z|<=2, for                  */
 /*  Re z<=0 with |z|<=4, and inside a cone around the negative real axis      */
 /*  (|z|<=40, Im z <= -Re z (24/|z| - 0.1)); elsewhere the even continued     */
 /*  fraction exp(z)E1(z) = 1/(z+1- 1/(z+3- 4/(z+5- ...))) by modified Lentz.  */
@@ -369,15 +370,19 @@ static int e1_use_series(cplx z) {
     return 0;
 }
 
+#define E1_QUAL static
+#include "e1_table.h"
+
+/* Horner evaluation of the truncated series, degree from the table at ceil|z| */
 static cplx e1_series(cplx z) {
-    cplx term = 1.0, sum = 0.0;
-    for (int n = 1; n < 600; n++) {
-        term *= -z / (double)n;
-        cplx add = term / (double)n;
-        sum += add;
-        if (cabs(add) <= 1e-17 * cabs(sum)) break;
-    }
-    return -EULER_GAMMA - clog(z) - sum;
+    double R = cabs(z);
+    int ir = (int)ceil(R);
+    if (ir > 40) ir = 40;
+    int N = E1_DEG[ir];
+    cplx p = E1_A[N - 1];
+    for (int q = N - 2; q >= 0; q--) p = E1_A[q] + z * p;
+    p = z * p;
+    return -EULER_GAMMA - clog(z) - p;
 }
 
 static cplx expE1_cf(cplx z) {
