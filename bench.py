@@ -3,14 +3,21 @@
 Workload (one GPU): vertical cylinder with a hemispherical cap, radius 4 m,
 draft 12 m, 4,096 panels (64 sectors x (40 side + 24 cap rows)), 64
 frequencies uniform in 0.1-3.0 rad/s, 6 rigid dofs about the origin, heading
-beta = 0, rho = 1025, g = 9.81, exact Guevel-Delhommeau wave term in FP64,
-direct method.
+beta = 0, rho = 1025, g = 9.81, direct method.
 
 A step is one omega-solve: wave-term assembly (the wavenumber-independent
 Rankine part is cached per body), one LU, all right-hand sides (6 radiation +
 1 diffraction) and pressure integration into A, B and F_ex.  K steps use K
-frequencies spread over the 64-frequency sweep.  The mesh, dofs and the cached
-Rankine part are resident on the device before the timed region.
+frequencies spread over the 64-frequency sweep.  The mesh, dofs, the cached
+Rankine part and the network weights are resident on the device before the
+timed region.
+
+The headline path is the learned surrogate of the exact Guevel-Delhommeau term
+(Wu fit + BF16-MFMA network residual, exact FP64 fallback near the free
+surface) with all K frequencies through the batched schedule.  It is compared
+with the plain GPU path (exact FP64 wave term, one frequency at a time) on a
+spread subset of the same frequencies, and its coefficients are checked
+against that exact path (relative error reported).
 
 Prints one JSON line (last line of stdout).
 """
@@ -28,6 +35,13 @@ sys.path.insert(0, os.path.join(ROOT, "marinehydro.jl_amd"))
 METRIC = "omega-solves/s at 4k panels: surrogate+batched vs plain GPU, at <=1e-3 rel err"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0
+
+
+WAVE_DESC = {"Surrogate": "learned surrogate (Wu + BF16 MFMA residual network, exact FP64 fallback)",
+             "ExactGuevelDelhommeau": "exact FP64 Guevel-Delhommeau", "Wu": "FP64 Liang-Wu-Noblesse fit"}
+DTYPE = {"Surrogate": "f64 (wave-term residual network: bf16 MFMA, f32 accumulate)",
+         "ExactGuevelDelhommeau": "f64", "Wu": "f64"}
 
 
 def log(*a):
@@ -77,6 +91,11 @@ def cpu_baseline(mesh, omegas, rows, threads):
                       f"({t_lu:.2f} s, numpy's OpenBLAS)"}
 
 
+def surrogate_flops_per_pair():
+    """Multiply-adds of one network evaluation (64 -> 4 x 128 -> 6), two flops each."""
+    return 2.0 * (64 * 128 + 3 * 128 * 128 + 128 * 6)
+
+
 def measure(ctx, run, ws, n, gf):
     """One timed call over the frequencies ws; returns rate, phase table and roofline of the
     dominant kernel (HIP events on the library's stream)."""
@@ -90,7 +109,13 @@ def measure(ctx, run, ws, n, gf):
     tw, cw = phases["wave_table"]
     tl, cl = phases["lu"]
     nw = len(ws)
-    if gf == "ExactGuevelDelhommeau" and tw >= tl:
+    if gf == "Surrogate" and tw >= tl:
+        pairs = n * (n + 1) / 2 * nw
+        ach = pairs * surrogate_flops_per_pair() / (tw * 1e-3) / 1e12
+        roof = {"bound": "mfma", "kernel": "surr_table_kernel (+ exact fallback)", "achieved": ach,
+                "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / BF16_PEAK_TFLOPS, "traffic": None,
+                "avg_launch_ms": tw / cw, "launches": cw, "frequencies_per_launch": nw / cw}
+    elif gf == "ExactGuevelDelhommeau" and tw >= tl:
         ach = flops / (tw * 1e-3) / 1e12
         roof = {"bound": "valu", "kernel": "delhommeau_table_kernel", "achieved": ach,
                 "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
@@ -105,7 +130,7 @@ def measure(ctx, run, ws, n, gf):
                 "avg_launch_ms": tl / cl, "launches": cl}
     lu_tflops = 8.0 / 3.0 * n ** 3 * nw / (tl * 1e-3) / 1e12 if tl else None
     per_w = {k: v[0] / nw for k, v in phases.items()}
-    return nw / elapsed, elapsed, per_w, roof, lu_tflops
+    return nw / elapsed, elapsed, per_w, roof, lu_tflops, (A, B, Fd, Fk)
 
 
 def main():
@@ -113,7 +138,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16, help="omega-solves timed (frequencies of the sweep)")
     ap.add_argument("--warmup", type=int, default=16, help="untimed omega-solves (the first W timed frequencies)")
-    ap.add_argument("--gf", default="ExactGuevelDelhommeau", choices=["ExactGuevelDelhommeau", "Wu"])
+    ap.add_argument("--gf", default="Surrogate", choices=["Surrogate", "ExactGuevelDelhommeau", "Wu"])
     ap.add_argument("--schedule", default="batched", choices=["batched", "per_omega"])
     ap.add_argument("--cpu-rows", type=int, default=32)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -144,18 +169,38 @@ def main():
     def run_with(m):
         return lambda ws: mhx.api._body_coeffs(body, names, ws, [0.0], form.gf, form.direct, m, 1025.0, 9.81)
 
+    if args.gf == "Surrogate":
+        t0 = time.perf_counter()
+        if os.path.exists(mhx.api.SURROGATE_WEIGHTS):
+            mhx.load_surrogate()
+            log(f"bench: loaded surrogate weights ({time.perf_counter() - t0:.2f} s)")
+        else:
+            val = mhx.train_surrogate()
+            log(f"bench: trained surrogate on the device ({time.perf_counter() - t0:.1f} s, val mse {val:.2e})")
     log(f"bench: n={n}, warmup over {args.warmup} frequencies")
     warm = steps_w[:max(args.warmup, 1)]
     run_with(mode)(warm)   # body upload, Rankine part, buffers: outside the timed region
     log(f"bench: timing {args.steps} omega-solves ({args.schedule})")
-    value, elapsed, phases, roof, lu_tflops = measure(ctx, run_with(mode), steps_w, n, args.gf)
+    value, elapsed, phases, roof, lu_tflops, res = measure(ctx, run_with(mode), steps_w, n, args.gf)
     log("per-frequency phase ms:", phases)
     plain = None
-    if not args.no_plain and mode != _lib.MODE_PER_OMEGA:
-        log("bench: plain per-omega comparison")
+    if not args.no_plain and (mode != _lib.MODE_PER_OMEGA or args.gf != "ExactGuevelDelhommeau"):
         kp = min(4, args.steps)
-        pv, pel, pph, proof, plu = measure(ctx, run_with(_lib.MODE_PER_OMEGA), steps_w[:kp], n, args.gf)
-        plain = {"value": pv, "steps": kp, "phases_ms_per_omega": pph, "lu_tflops": plu}
+        sub = np.linspace(0, args.steps - 1, kp).round().astype(int)
+        log(f"bench: plain GPU path (exact FP64, per omega) at omega = {np.round(steps_w[sub], 3).tolist()}")
+        exact_form = mhx.DirectBEM("ExactGuevelDelhommeau")
+        run_plain = lambda ws: mhx.api._body_coeffs(body, names, ws, [0.0], exact_form.gf, True,
+                                                    _lib.MODE_PER_OMEGA, 1025.0, 9.81)
+        pv, pel, pph, proof, plu, pres = measure(ctx, run_plain, steps_w[sub], n, "ExactGuevelDelhommeau")
+        errs = {}
+        pairs = (("added_mass", res[0][:, :, sub], pres[0], lambda x, q: x[:, :, q]),
+                 ("radiation_damping", res[1][:, :, sub], pres[1], lambda x, q: x[:, :, q]),
+                 ("excitation_force", (res[2] + res[3])[:, sub], pres[2] + pres[3], lambda x, q: x[:, q]))
+        for name, a, b, at in pairs:
+            errs[name] = max(float(np.linalg.norm(at(a, q) - at(b, q)) / np.linalg.norm(at(b, q))) for q in range(kp))
+        log("max relative error vs exact FP64 per omega:", errs)
+        plain = {"value": pv, "steps": kp, "omegas": steps_w[sub].tolist(), "phases_ms_per_omega": pph,
+                 "lu_tflops": plu, "max_rel_err_vs_exact": errs}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -165,8 +210,8 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "omega-solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{args.schedule} GPU path, exact FP64 wave term ({args.gf}), direct method: "
+        "scaling": "weak", "vs_baseline": None, "dtype": DTYPE[args.gf], "data": "synthetic",
+        "config": {"workload": f"{args.schedule} GPU path, wave term {WAVE_DESC[args.gf]}, direct method: "
                                f"capped cylinder R=4 draft=12, {n} panels, 6 dofs, beta=0, "
                                f"{args.steps} of the 64 frequencies in 0.1-3.0 rad/s",
                    "panels": n, "dofs": 6, "headings": 1, "schedule": args.schedule,

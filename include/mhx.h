@@ -39,6 +39,8 @@ extern "C" {
 /* wave Green functions: GFWu() and ExactGuevelDelhommeau() of src/formulations.jl:26-37 */
 #define MHX_GF_WU 0
 #define MHX_GF_DELHOMMEAU 1
+#define MHX_GF_SURROGATE 2  /* learned surrogate of the exact term (Wu fit + network residual); exact
+                               fallback outside the trained box and near the free surface */
 
 /* solve schedule of mhx_hydro_coeffs */
 #define MHX_MODE_PER_OMEGA 0   /* one assembly + one LU per frequency, all right-hand sides together */
@@ -116,6 +118,27 @@ void mhx_reset_timing(mhx_ctx* ctx);
  * algorithmic FP64 operation count derived from them (no counterpart in the reference;
  * used for roofline reporting) */
 int mhx_exact_work(mhx_ctx* ctx, int64_t counts[3], double* flops);
+
+/* Learned surrogate of the exact Guevel-Delhommeau wave term (no counterpart in the reference; it
+ * replaces the exact term of src/green_functions/exact_Guevel_Delhommeau.jl:15-59 inside assembly).
+ * The network maps (r, v) = (k R_h, k |z + zeta|) through 64 Fourier features and 4 x 128 SiLU
+ * layers to the residual (exact - GFWu) of G, dG/dr, dG/dz: real parts as 100 x residual,
+ * imaginary parts as 100 exp(min(v, 5)) x residual (they decay like exp(-v)).  It is trained on
+ * the device on ngrid^2 log-spaced jittered samples generated by the exact FP64 kernel.
+ *   train: Adam, cosine learning rate, minibatches of `batch`; val_loss = MSE on the held-out samples
+ *   save / load: versioned binary file (magic line, JSON header line, Fourier matrix, FP32 parameters)
+ *   params: copies the FP32 parameters (nparams floats) and the 32 x 2 Fourier matrix to the host
+ *   eval: the BF16-MFMA inference network at m points rv[2m] = (r, v); out[6m] its raw outputs
+ *   loss_grad: one FP32 forward/backward of the training kernels on m samples (U m x 2, Y m x 6)
+ *              with the given parameters; grad_out has nparams floats */
+int mhx_surrogate_train(mhx_ctx* ctx, int64_t ngrid, int64_t steps, int64_t batch, double lr, uint64_t seed,
+                        double* val_loss);
+int mhx_surrogate_save(mhx_ctx* ctx, const char* path);
+int mhx_surrogate_load(mhx_ctx* ctx, const char* path);
+int mhx_surrogate_params(mhx_ctx* ctx, float* params_out, float* fourier_out, int64_t* nparams);
+int mhx_surrogate_eval(mhx_ctx* ctx, int64_t m, const double* rv, float* out);
+int mhx_surrogate_loss_grad(mhx_ctx* ctx, int64_t m, const float* U, const float* Y, const float* params,
+                            const float* fourier, float* grad_out, float* loss_out);
 
 #ifdef __cplusplus
 }

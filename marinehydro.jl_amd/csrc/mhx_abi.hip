@@ -11,7 +11,7 @@
 #include <vector>
 
 #include "../../include/mhx.h"
-#include "mhx_internal.h"
+#include "mhx_surrogate.h"
 
 
 
@@ -31,6 +31,12 @@ struct mhx_ctx {
     unsigned long long quad_work[3] = {0, 0, 0};  // exact wave term: evaluations, series terms, CF steps
     double batch_bytes = 64.0 * (1ull << 30);      // device memory budget of one frequency chunk
     int64_t max_batch = 64;
+    // learned wave-term surrogate
+    bool surr_ready = false;
+    float *surr_params = nullptr, *surr_B = nullptr, *surr_bias = nullptr;
+    unsigned short* surr_W = nullptr;
+    double surr_val_loss = -1.0;
+    int64_t surr_steps = 0;
 };
 
 struct mhx_body {
@@ -49,6 +55,8 @@ struct mhx_body {
     double *betas = nullptr, *d_ws = nullptr, *d_ks = nullptr;
     int *ipiv = nullptr, *perm = nullptr, *wperm = nullptr, *info = nullptr;
     unsigned long long* status = nullptr;
+    int *fb_list = nullptr, *fb_count = nullptr;
+    int fb_cap = 0;
     int64_t nn() const { return (int64_t)n * n; }
     int64_t nR() const { return (int64_t)n * cap_R; }
     int64_t sW() const { return 16 * nR() + mhx::getrs_work(n, cap_R); }  // split-K partials + getrs work
@@ -144,7 +152,8 @@ int ensure_rankine(mhx_ctx* ctx, mhx_body* b, int direct) {
 void free_work(mhx_body* b) {
     dfree(b->S); dfree(b->D); dfree(b->WG); dfree(b->WR); dfree(b->B); dfree(b->Y); dfree(b->work);
     dfree(b->F); dfree(b->FK); dfree(b->betas); dfree(b->d_ws); dfree(b->d_ks); dfree(b->ipiv); dfree(b->perm);
-    dfree(b->wperm); dfree(b->info); dfree(b->status);
+    dfree(b->wperm); dfree(b->info); dfree(b->status); dfree(b->fb_list); dfree(b->fb_count);
+    b->fb_cap = 0;
     b->cap_batch = b->cap_R = b->cap_nb = 0;
     b->cap_table = false;
 }
@@ -182,6 +191,23 @@ int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int 
         Phase ph(ctx, "wave");
         mhx::launch_combine_wu(ctx->stream, b->n, b->geom, b->SR, b->DR[direct], b->d_ks, batch, direct,
                                include_identity, b->S, b->D, b->nn());
+    } else if (gf == MHX_GF_SURROGATE) {
+        if (!ctx->surr_ready) return fail(ctx, MHX_EINVAL, "surrogate: no trained network (train or load one)");
+        const int64_t cap = (int64_t)b->n * (b->n + 1) / 2;
+        if (!b->fb_list) {
+            int rc;
+            if ((rc = dalloc(ctx, &b->fb_list, 2 * (size_t)cap)) || (rc = dalloc(ctx, &b->fb_count, 1))) return rc;
+            b->fb_cap = (int)cap;
+        }
+        HCHK(hipMemsetAsync(b->status, 0, 4 * sizeof(unsigned long long), ctx->stream));
+        {
+            Phase ph(ctx, "wave_table");
+            mhx::launch_surr_table(ctx->stream, b->n, b->geom, b->d_ks, batch, ctx->surr_W, ctx->surr_bias,
+                                   ctx->surr_B, b->WG, b->WR, b->nn(), b->fb_list, b->fb_count, b->fb_cap, b->status);
+        }
+        Phase ph(ctx, "wave");
+        mhx::launch_combine_table(ctx->stream, b->n, b->geom, b->SR, b->DR[direct], b->d_ks, batch, direct,
+                                  include_identity, b->WG, b->WR, b->S, b->D, b->nn());
     } else {
         HCHK(hipMemsetAsync(b->status, 0, 4 * sizeof(unsigned long long), ctx->stream));
         {
@@ -198,7 +224,7 @@ int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int 
 }
 
 int check_status(mhx_ctx* ctx, mhx_body* b, int gf) {
-    if (gf != MHX_GF_DELHOMMEAU) return MHX_OK;
+    if (gf == MHX_GF_WU) return MHX_OK;
     unsigned long long st[4] = {0, 0, 0, 0};
     HCHK(hipMemcpyAsync(st, b->status, sizeof st, hipMemcpyDeviceToHost, ctx->stream));
     HCHK(hipStreamSynchronize(ctx->stream));
@@ -354,7 +380,8 @@ int mhx_assemble(mhx_ctx* ctx, const mhx_mesh* mesh, double k, int gf, int direc
                  double* D_out) {
     if (!ctx) return MHX_EINVAL;
     if (!S_out || !D_out) return fail(ctx, MHX_EINVAL, "assemble: null output");
-    if (gf != MHX_GF_WU && gf != MHX_GF_DELHOMMEAU) return fail(ctx, MHX_EINVAL, "unknown Green function");
+    if (gf != MHX_GF_WU && gf != MHX_GF_DELHOMMEAU && gf != MHX_GF_SURROGATE)
+        return fail(ctx, MHX_EINVAL, "unknown Green function");
     if (!(k > 0) || !std::isfinite(k)) return fail(ctx, MHX_EINVAL, "wavenumber must be positive and finite");
     hipSetDevice(ctx->device);
     direct = direct ? 1 : 0;
@@ -481,7 +508,8 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
                     int direct, int mode, double* A_out, double* B_out, double* Fd_out, double* Fk_out) {
     if (!ctx || !b) return MHX_EINVAL;
     if (nw < 0 || nb < 0 || (nw && !omegas) || (nb && !betas)) return fail(ctx, MHX_EINVAL, "coeffs: bad frequency/heading arrays");
-    if (gf != MHX_GF_WU && gf != MHX_GF_DELHOMMEAU) return fail(ctx, MHX_EINVAL, "unknown Green function");
+    if (gf != MHX_GF_WU && gf != MHX_GF_DELHOMMEAU && gf != MHX_GF_SURROGATE)
+        return fail(ctx, MHX_EINVAL, "unknown Green function");
     if (mode != MHX_MODE_PER_OMEGA && mode != MHX_MODE_PER_PROBLEM && mode != MHX_MODE_BATCHED)
         return fail(ctx, MHX_EINVAL, "unknown mode");
     const int ndof = b->ndof, n = b->n;
@@ -594,6 +622,287 @@ int mhx_exact_work(mhx_ctx* ctx, int64_t counts[3], double* flops) {
     if (flops)
         *flops = MHX_FLOPS_PER_EVAL * ctx->quad_work[0] + MHX_FLOPS_PER_SERIES * ctx->quad_work[1] +
                  MHX_FLOPS_PER_CF * ctx->quad_work[2];
+    return MHX_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// learned surrogate of the wave term
+// ---------------------------------------------------------------------------
+namespace {
+
+struct HostRng {
+    uint64_t s;
+    explicit HostRng(uint64_t seed) : s(seed) {}
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return (double)(next() >> 11) * (1.0 / 9007199254740992.0); }
+    double normal() {
+        double u1 = uniform(), u2 = uniform();
+        if (u1 < 1e-300) u1 = 1e-300;
+        return std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
+    }
+};
+
+int surr_alloc(mhx_ctx* ctx) {
+    int rc;
+    const int64_t P = mhx::surr_param_count();
+    if ((rc = dalloc(ctx, &ctx->surr_params, (size_t)P)) || (rc = dalloc(ctx, &ctx->surr_B, 2 * SURR_NF)) ||
+        (rc = dalloc(ctx, &ctx->surr_W, (size_t)SURR_PACKED_W)) || (rc = dalloc(ctx, &ctx->surr_bias, 4 * 128 + 32)))
+        return rc;
+    return MHX_OK;
+}
+
+// deterministic initial parameters (uniform +-1/sqrt(fan_in)) and Fourier frequencies (N(0, 4^2))
+void surr_init_host(uint64_t seed, std::vector<float>& params, std::vector<float>& B) {
+    HostRng rng(seed);
+    const int64_t P = mhx::surr_param_count();
+    params.assign(P, 0.f);
+    const int in_dim[5] = {64, 128, 128, 128, 128}, out_dim[5] = {128, 128, 128, 128, SURR_NOUT};
+    int64_t o = 0;
+    for (int l = 0; l < 5; l++) {
+        const double bound = 1.0 / std::sqrt((double)in_dim[l]);
+        for (int64_t q = 0; q < (int64_t)out_dim[l] * in_dim[l] + out_dim[l]; q++)
+            params[o++] = (float)((2.0 * rng.uniform() - 1.0) * bound);
+    }
+    B.assign(2 * SURR_NF, 0.f);
+    for (auto& b : B) b = (float)(4.0 * rng.normal());
+}
+
+struct TrainBufs {
+    mhx::SurrTrainWork w;
+    float *grad = nullptr, *m1 = nullptr, *m2 = nullptr, *loss = nullptr;
+    int* idx = nullptr;
+    void release() {
+        for (int q = 0; q < 6; q++) dfree(w.act[q]);
+        for (int q = 0; q < 5; q++) { dfree(w.z[q]); dfree(w.dz[q]); }
+        dfree(w.part); dfree(grad); dfree(m1); dfree(m2); dfree(loss); dfree(idx);
+    }
+};
+
+int train_bufs(mhx_ctx* ctx, TrainBufs& t, int M) {
+    const int64_t P = mhx::surr_param_count();
+    int rc;
+    t.w.M = M;
+    const int widths[6] = {64, 128, 128, 128, 128, 8};
+    for (int q = 0; q < 6; q++)
+        if ((rc = dalloc(ctx, &t.w.act[q], (size_t)M * widths[q]))) return rc;
+    for (int q = 0; q < 5; q++)
+        if ((rc = dalloc(ctx, &t.w.z[q], (size_t)M * 128)) || (rc = dalloc(ctx, &t.w.dz[q], (size_t)M * 128))) return rc;
+    if ((rc = dalloc(ctx, &t.w.part, (size_t)32 * 128 * 128)) || (rc = dalloc(ctx, &t.grad, (size_t)P)) ||
+        (rc = dalloc(ctx, &t.m1, (size_t)P)) || (rc = dalloc(ctx, &t.m2, (size_t)P)) || (rc = dalloc(ctx, &t.loss, 1)) ||
+        (rc = dalloc(ctx, &t.idx, (size_t)M)))
+        return rc;
+    t.w.Bf = ctx->surr_B;
+    return MHX_OK;
+}
+
+__global__ void strided_index_kernel(int M, int64_t start, int64_t stride, int* idx) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < M) idx[t] = (int)(start + stride * t);
+}
+
+}  // namespace
+
+extern "C" {
+
+int mhx_surrogate_train(mhx_ctx* ctx, int64_t ngrid, int64_t steps, int64_t batch, double lr, uint64_t seed,
+                        double* val_loss) {
+    if (!ctx) return MHX_EINVAL;
+    if (ngrid < 16 || ngrid > 4096 || steps < 1 || batch < 64 || batch > 65536 || batch % 64 || !(lr > 0))
+        return fail(ctx, MHX_EINVAL, "surrogate_train: bad grid, steps, batch (multiple of 64) or learning rate");
+    hipSetDevice(ctx->device);
+    int rc;
+    if ((rc = surr_alloc(ctx))) return rc;
+    hipStream_t st = ctx->stream;
+    std::vector<float> hp, hB;
+    surr_init_host(seed, hp, hB);
+    const int64_t P = mhx::surr_param_count();
+    HCHK(hipMemcpyAsync(ctx->surr_params, hp.data(), sizeof(float) * P, hipMemcpyHostToDevice, st));
+    HCHK(hipMemcpyAsync(ctx->surr_B, hB.data(), sizeof(float) * hB.size(), hipMemcpyHostToDevice, st));
+    const int64_t ns = ngrid * ngrid;
+    float *U = nullptr, *Y = nullptr;
+    unsigned long long* status = nullptr;
+    TrainBufs t;
+    auto cleanup = [&]() { dfree(U); dfree(Y); dfree(status); t.release(); };
+    if ((rc = dalloc(ctx, &U, 2 * (size_t)ns)) || (rc = dalloc(ctx, &Y, 6 * (size_t)ns)) || (rc = dalloc(ctx, &status, 4)) ||
+        (rc = train_bufs(ctx, t, (int)batch))) {
+        cleanup();
+        return rc;
+    }
+    HCHK(hipMemsetAsync(status, 0, 4 * sizeof(unsigned long long), st));
+    {
+        Phase ph(ctx, "surrogate_samples");
+        mhx::launch_surr_samples(st, (int)ngrid, seed ^ 0x5DEECE66Dull, U, Y, status);
+    }
+    HCHK(hipMemsetAsync(t.m1, 0, sizeof(float) * P, st));
+    HCHK(hipMemsetAsync(t.m2, 0, sizeof(float) * P, st));
+    {
+        Phase ph(ctx, "surrogate_train");
+        for (int64_t step = 1; step <= steps; step++) {
+            mhx::launch_batch_index(st, (int)batch, ns, seed * 1315423911ull + (uint64_t)step, t.idx);
+            mhx::surr_forward_backward(st, t.w, (int)batch, t.idx, U, Y, ctx->surr_params, t.grad, nullptr);
+            const double frac = (double)(step - 1) / (double)steps;
+            const float lrs = (float)(lr * 0.5 * (1.0 + std::cos(3.141592653589793 * frac)));
+            mhx::surr_adam(st, P, ctx->surr_params, t.grad, t.m1, t.m2, lrs, (int)step);
+        }
+    }
+    HCHK(hipGetLastError());
+    // validation loss over the held-out stride
+    const int64_t nval = ns / SURR_VAL_STRIDE;
+    double vsum = 0.0;
+    int64_t vcnt = 0;
+    for (int64_t v0 = 0; v0 < nval; v0 += batch) {
+        const int m = (int)std::min<int64_t>(batch, nval - v0);
+        if (m < 64) break;
+        strided_index_kernel<<<(m + 255) / 256, 256, 0, st>>>(m, v0 * SURR_VAL_STRIDE, SURR_VAL_STRIDE, t.idx);
+        mhx::surr_forward_backward(st, t.w, m, t.idx, U, Y, ctx->surr_params, t.grad, t.loss);
+        float l = 0.f;
+        HCHK(hipMemcpyAsync(&l, t.loss, sizeof(float), hipMemcpyDeviceToHost, st));
+        HCHK(hipStreamSynchronize(st));
+        vsum += (double)l * m;
+        vcnt += m;
+    }
+    unsigned long long hst[4] = {0, 0, 0, 0};
+    HCHK(hipMemcpyAsync(hst, status, sizeof hst, hipMemcpyDeviceToHost, st));
+    mhx::launch_surr_pack(st, ctx->surr_params, ctx->surr_W, ctx->surr_bias);
+    HCHK(hipStreamSynchronize(st));
+    cleanup();
+    flush_timing(ctx);
+    if (hst[0]) return fail(ctx, MHX_EQUAD, "surrogate samples: exact quadrature exceeded its interval budget");
+    ctx->surr_val_loss = vcnt ? vsum / vcnt : -1.0;
+    ctx->surr_steps = steps;
+    ctx->surr_ready = true;
+    if (val_loss) *val_loss = ctx->surr_val_loss;
+    return MHX_OK;
+}
+
+int mhx_surrogate_save(mhx_ctx* ctx, const char* path) {
+    if (!ctx || !path) return MHX_EINVAL;
+    if (!ctx->surr_ready) return fail(ctx, MHX_EINVAL, "surrogate_save: no trained network");
+    const int64_t P = mhx::surr_param_count();
+    std::vector<float> hp(P), hB(2 * SURR_NF);
+    HCHK(hipMemcpy(hp.data(), ctx->surr_params, sizeof(float) * P, hipMemcpyDeviceToHost));
+    HCHK(hipMemcpy(hB.data(), ctx->surr_B, sizeof(float) * hB.size(), hipMemcpyDeviceToHost));
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(ctx, MHX_EINVAL, std::string("surrogate_save: cannot open ") + path);
+    char hdr[512];
+    std::snprintf(hdr, sizeof hdr,
+                  "{\"format\": \"mhx-surrogate\", \"version\": 1, \"params\": %lld, \"fourier\": %d, \"outputs\": %d, "
+                  "\"scale\": %.1f, \"r_max\": %.1f, \"v_min\": %g, \"v_max\": %.1f, \"steps\": %lld, "
+                  "\"val_mse\": %.6e}\n",
+                  (long long)P, SURR_NF, SURR_NOUT, SURR_SCALE, SURR_RMAX, SURR_VMIN, SURR_VMAX,
+                  (long long)ctx->surr_steps, ctx->surr_val_loss);
+    std::fputs("MHXSURR1\n", f);
+    std::fputs(hdr, f);
+    std::fwrite(hB.data(), sizeof(float), hB.size(), f);
+    std::fwrite(hp.data(), sizeof(float), hp.size(), f);
+    std::fclose(f);
+    return MHX_OK;
+}
+
+int mhx_surrogate_load(mhx_ctx* ctx, const char* path) {
+    if (!ctx || !path) return MHX_EINVAL;
+    FILE* f = std::fopen(path, "rb");
+    if (!f) return fail(ctx, MHX_EINVAL, std::string("surrogate_load: cannot open ") + path);
+    char line[1024];
+    const int64_t P = mhx::surr_param_count();
+    std::vector<float> hp(P), hB(2 * SURR_NF);
+    bool ok = std::fgets(line, sizeof line, f) && std::strcmp(line, "MHXSURR1\n") == 0 && std::fgets(line, sizeof line, f);
+    long long pc = -1, steps = 0;
+    double vl = -1.0;
+    if (ok) {
+        const char* q = std::strstr(line, "\"params\": ");
+        ok = q && std::sscanf(q + 10, "%lld", &pc) == 1 && pc == P;
+        const char* q2 = std::strstr(line, "\"steps\": ");
+        if (q2) std::sscanf(q2 + 9, "%lld", &steps);
+        const char* q3 = std::strstr(line, "\"val_mse\": ");
+        if (q3) std::sscanf(q3 + 11, "%lf", &vl);
+    }
+    ok = ok && std::fread(hB.data(), sizeof(float), hB.size(), f) == hB.size() &&
+         std::fread(hp.data(), sizeof(float), hp.size(), f) == hp.size();
+    std::fclose(f);
+    if (!ok) return fail(ctx, MHX_EINVAL, std::string("surrogate_load: not an mhx surrogate file: ") + path);
+    hipSetDevice(ctx->device);
+    int rc;
+    if ((rc = surr_alloc(ctx))) return rc;
+    HCHK(hipMemcpyAsync(ctx->surr_params, hp.data(), sizeof(float) * P, hipMemcpyHostToDevice, ctx->stream));
+    HCHK(hipMemcpyAsync(ctx->surr_B, hB.data(), sizeof(float) * hB.size(), hipMemcpyHostToDevice, ctx->stream));
+    mhx::launch_surr_pack(ctx->stream, ctx->surr_params, ctx->surr_W, ctx->surr_bias);
+    HCHK(hipStreamSynchronize(ctx->stream));
+    ctx->surr_ready = true;
+    ctx->surr_steps = steps;
+    ctx->surr_val_loss = vl;
+    return MHX_OK;
+}
+
+int mhx_surrogate_params(mhx_ctx* ctx, float* params_out, float* fourier_out, int64_t* nparams) {
+    if (!ctx) return MHX_EINVAL;
+    const int64_t P = mhx::surr_param_count();
+    if (nparams) *nparams = P;
+    if (!params_out && !fourier_out) return MHX_OK;
+    if (!ctx->surr_ready) return fail(ctx, MHX_EINVAL, "surrogate_params: no trained network");
+    if (params_out) HCHK(hipMemcpy(params_out, ctx->surr_params, sizeof(float) * P, hipMemcpyDeviceToHost));
+    if (fourier_out) HCHK(hipMemcpy(fourier_out, ctx->surr_B, sizeof(float) * 2 * SURR_NF, hipMemcpyDeviceToHost));
+    return MHX_OK;
+}
+
+int mhx_surrogate_eval(mhx_ctx* ctx, int64_t m, const double* rv, float* out) {
+    if (!ctx) return MHX_EINVAL;
+    if (!ctx->surr_ready) return fail(ctx, MHX_EINVAL, "surrogate_eval: no trained network");
+    if (m <= 0 || !rv || !out) return fail(ctx, MHX_EINVAL, "surrogate_eval: bad arguments");
+    hipSetDevice(ctx->device);
+    double* drv = nullptr;
+    float* dout = nullptr;
+    int rc;
+    if ((rc = dalloc(ctx, &drv, 2 * (size_t)m)) || (rc = dalloc(ctx, &dout, 6 * (size_t)m))) {
+        dfree(drv); dfree(dout);
+        return rc;
+    }
+    hipMemcpyAsync(drv, rv, sizeof(double) * 2 * m, hipMemcpyHostToDevice, ctx->stream);
+    mhx::launch_surr_eval(ctx->stream, (int)m, drv, ctx->surr_W, ctx->surr_bias, ctx->surr_B, dout);
+    hipMemcpyAsync(out, dout, sizeof(float) * 6 * m, hipMemcpyDeviceToHost, ctx->stream);
+    hipError_t e = hipStreamSynchronize(ctx->stream);
+    dfree(drv); dfree(dout);
+    if (e != hipSuccess) return fail(ctx, MHX_EDEVICE, "surrogate_eval: device error");
+    return MHX_OK;
+}
+
+int mhx_surrogate_loss_grad(mhx_ctx* ctx, int64_t m, const float* U, const float* Yt, const float* params,
+                            const float* fourier, float* grad_out, float* loss_out) {
+    if (!ctx) return MHX_EINVAL;
+    if (m < 1 || m > 65536 || !U || !Yt || !params || !fourier || !grad_out)
+        return fail(ctx, MHX_EINVAL, "surrogate_loss_grad: bad arguments");
+    hipSetDevice(ctx->device);
+    const int64_t P = mhx::surr_param_count();
+    float *dU = nullptr, *dY = nullptr, *dp = nullptr, *dB = nullptr;
+    TrainBufs t;
+    int rc;
+    auto cleanup = [&]() { dfree(dU); dfree(dY); dfree(dp); dfree(dB); t.release(); };
+    if ((rc = dalloc(ctx, &dU, 2 * (size_t)m)) || (rc = dalloc(ctx, &dY, 6 * (size_t)m)) || (rc = dalloc(ctx, &dp, (size_t)P)) ||
+        (rc = dalloc(ctx, &dB, 2 * SURR_NF)) || (rc = train_bufs(ctx, t, (int)m))) {
+        cleanup();
+        return rc;
+    }
+    t.w.Bf = dB;
+    hipStream_t st = ctx->stream;
+    hipMemcpyAsync(dU, U, sizeof(float) * 2 * m, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dY, Yt, sizeof(float) * 6 * m, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dp, params, sizeof(float) * P, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dB, fourier, sizeof(float) * 2 * SURR_NF, hipMemcpyHostToDevice, st);
+    mhx::surr_forward_backward(st, t.w, (int)m, nullptr, dU, dY, dp, t.grad, t.loss);
+    float l = 0.f;
+    hipMemcpyAsync(&l, t.loss, sizeof(float), hipMemcpyDeviceToHost, st);
+    hipMemcpyAsync(grad_out, t.grad, sizeof(float) * P, hipMemcpyDeviceToHost, st);
+    hipError_t e = hipStreamSynchronize(st);
+    cleanup();
+    if (e != hipSuccess) return fail(ctx, MHX_EDEVICE, "surrogate_loss_grad: device error");
+    if (loss_out) *loss_out = l;
     return MHX_OK;
 }
 

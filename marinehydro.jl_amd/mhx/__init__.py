@@ -10,5 +10,6 @@ from .api import (SETTINGS, DiffractionProblem, DirectBEM, ExactGuevelDelhommeau
                   IndirectBEM, RadiationProblem, Rankine, RankineReflected, added_mass, assemble_matrices,
                   compute_wavenumber, context, default_greens_functions, diffraction_force, excitation_force,
                   froude_krylov_force, hydrodynamic_coefficients, is_direct, lu_solve, radiation_damping,
-                  rotational_dofs, set_g, set_rho, solve, solve_linear, translational_dofs)
+                  rotational_dofs, set_g, set_rho, solve, solve_linear, translational_dofs,
+                  SurrogateGuevelDelhommeau, train_surrogate, load_surrogate)
 from .meshes import Mesh  # noqa: F401

@@ -22,16 +22,20 @@ MHX_EQUAD = -6
 
 GF_WU = 0
 GF_DELHOMMEAU = 1
+GF_SURROGATE = 2
 MODE_PER_OMEGA = 0
 MODE_PER_PROBLEM = 1
 MODE_BATCHED = 2
 
 EXPORTED = ["mhx_create", "mhx_destroy", "mhx_last_error", "mhx_set_constants", "mhx_assemble", "mhx_solve",
             "mhx_lu_solve", "mhx_hydro_coeffs", "mhx_body_create", "mhx_body_destroy", "mhx_body_coeffs",
-            "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work", "mhx_set_batch"]
+            "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work", "mhx_set_batch", "mhx_surrogate_train",
+            "mhx_surrogate_save", "mhx_surrogate_load", "mhx_surrogate_params", "mhx_surrogate_eval",
+            "mhx_surrogate_loss_grad"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int64)
+_fp = C.POINTER(C.c_float)
 
 
 class Options(C.Structure):
@@ -88,12 +92,22 @@ def lib():
         L.mhx_reset_timing.restype = None
         L.mhx_exact_work.argtypes = [vp, _ip, _dp]
         L.mhx_set_batch.argtypes = [vp, C.c_int64, C.c_double]
+        L.mhx_surrogate_train.argtypes = [vp, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_uint64, _dp]
+        L.mhx_surrogate_save.argtypes = [vp, C.c_char_p]
+        L.mhx_surrogate_load.argtypes = [vp, C.c_char_p]
+        L.mhx_surrogate_params.argtypes = [vp, _fp, _fp, _ip]
+        L.mhx_surrogate_eval.argtypes = [vp, C.c_int64, _dp, _fp]
+        L.mhx_surrogate_loss_grad.argtypes = [vp, C.c_int64, _fp, _fp, _fp, _fp, _fp, _fp]
         _lib = L
     return _lib
 
 
 def ptr(a):
     return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def fptr(a):
+    return a.ctypes.data_as(_fp) if a is not None else None
 
 
 def cmesh(mesh):
@@ -144,6 +158,62 @@ class Context:
 
     def reset_timing(self):
         lib().mhx_reset_timing(self.h)
+
+    # ---- learned surrogate of the exact wave term ----
+    def surrogate_train(self, ngrid=2048, steps=8000, batch=4096, lr=2e-3, seed=20261203):
+        """Generate ngrid^2 exact samples on the device and train the network; returns the
+        validation MSE (scaled units) on the held-out samples."""
+        vl = C.c_double()
+        self.check(lib().mhx_surrogate_train(self.h, int(ngrid), int(steps), int(batch), float(lr), int(seed),
+                                             C.byref(vl)))
+        return vl.value
+
+    def surrogate_save(self, path):
+        self.check(lib().mhx_surrogate_save(self.h, os.fsencode(path)))
+
+    def surrogate_load(self, path):
+        self.check(lib().mhx_surrogate_load(self.h, os.fsencode(path)))
+
+    def surrogate_param_count(self):
+        n = C.c_int64()
+        self.check(lib().mhx_surrogate_params(self.h, None, None, C.byref(n)))
+        return n.value
+
+    def surrogate_params(self):
+        """(FP32 parameter vector, 32 x 2 Fourier matrix) of the trained network."""
+        n = self.surrogate_param_count()
+        p = np.zeros(n, dtype=np.float32)
+        B = np.zeros((32, 2), dtype=np.float32)
+        self.check(lib().mhx_surrogate_params(self.h, fptr(p), fptr(B), None))
+        return p, B
+
+    def surrogate_eval(self, r, v):
+        """BF16-MFMA network at points (r, v): (m, 6) raw outputs (see surrogate_residual)."""
+        rv = np.ascontiguousarray(np.stack([np.asarray(r, float).ravel(), np.asarray(v, float).ravel()], 1))
+        out = np.zeros((rv.shape[0], 6), dtype=np.float32)
+        self.check(lib().mhx_surrogate_eval(self.h, rv.shape[0], ptr(rv), fptr(out)))
+        return out
+
+    def surrogate_residual(self, r, v):
+        """The network's correction to GFWu at (r, v) (v = k|z + zeta| >= 0): (m, 6) real array
+        Re/Im of dG, dG/dr, dG/dz, i.e. the raw outputs / 100, the imaginary ones also times
+        exp(-min(v, 5))."""
+        out = self.surrogate_eval(r, v).astype(np.float64) / 100.0
+        env = np.exp(-np.minimum(np.asarray(v, float).ravel(), 5.0))
+        out[:, 1::2] *= env[:, None]
+        return out
+
+    def surrogate_loss_grad(self, U, Y, params, fourier):
+        """One FP32 forward/backward of the training kernels: (loss, gradient)."""
+        U = np.ascontiguousarray(U, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        fourier = np.ascontiguousarray(fourier, dtype=np.float32)
+        g = np.zeros_like(params)
+        loss = C.c_float()
+        self.check(lib().mhx_surrogate_loss_grad(self.h, U.shape[0], fptr(U), fptr(Y), fptr(params), fptr(fourier),
+                                                 fptr(g), C.byref(loss)))
+        return loss.value, g
 
     def close(self):
         if self.h:

@@ -18,6 +18,7 @@ Every computation runs in libmhx on the GPU; this module only validates
 arguments, keeps device contexts and packs results.  ArgumentError maps to
 ValueError, DimensionMismatch to mhx.DimensionMismatch.
 """
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -74,22 +75,30 @@ class ExactGuevelDelhommeau(GreensFunction):
     code = _lib.GF_DELHOMMEAU
 
 
+class SurrogateGuevelDelhommeau(GreensFunction):
+    """The exact wave term through the learned surrogate: GFWu plus a network residual
+    (BF16 MFMA inference), with the exact FP64 term for pairs outside the trained box
+    (k R_h > 100, k|z + zeta| > 50) and near the free surface (k|z + zeta| < 1e-3).
+    No counterpart in the reference; it stands in for ExactGuevelDelhommeau()."""
+    code = _lib.GF_SURROGATE
+
+
+GREENS = {"Wu": GFWu, "ExactGuevelDelhommeau": ExactGuevelDelhommeau, "Surrogate": SurrogateGuevelDelhommeau}
+
+
 def default_greens_functions(gf="Wu"):
-    if gf == "Wu":
-        wave = GFWu()
-    elif gf == "ExactGuevelDelhommeau":
-        wave = ExactGuevelDelhommeau()
-    else:
-        raise ValueError(f'unknown Green\'s function "{gf}"; use "Wu" or "ExactGuevelDelhommeau"')
-    return (Rankine(), RankineReflected(), wave)
+    if gf not in GREENS:
+        raise ValueError(f'unknown Green\'s function "{gf}"; use "Wu", "ExactGuevelDelhommeau" or "Surrogate"')
+    return (Rankine(), RankineReflected(), GREENS[gf]())
 
 
 def _wave_code(gfs):
     gfs = tuple(gfs)
     kinds = [type(g) for g in gfs]
     if len(gfs) != 3 or kinds[0] is not Rankine or kinds[1] is not RankineReflected or \
-            kinds[2] not in (GFWu, ExactGuevelDelhommeau):
-        raise ValueError("the GPU path supports (Rankine(), RankineReflected(), GFWu() | ExactGuevelDelhommeau())")
+            kinds[2] not in GREENS.values():
+        raise ValueError("the GPU path supports (Rankine(), RankineReflected(), GFWu() | ExactGuevelDelhommeau() | "
+                         "SurrogateGuevelDelhommeau())")
     return gfs[2].code
 
 
@@ -123,6 +132,35 @@ def context(device=0):
         ctx = _contexts[device] = _lib.Context(SETTINGS.rho, SETTINGS.g, device)
     return ctx
 
+
+# --------------------------------------------------------------------------- surrogate
+SURROGATE_WEIGHTS = os.path.join(_lib.PKG_ROOT, "weights", "delhommeau_surrogate.mhxs")
+_surrogate_ready = set()
+
+
+def train_surrogate(device=0, save_to=None, **kw):
+    """Train the wave-term network on the device (exact FP64 samples, Adam); keeps it in the
+    device context and optionally writes it to `save_to`.  Returns the validation MSE."""
+    ctx = context(device)
+    val = ctx.surrogate_train(**kw)
+    _surrogate_ready.add(device)
+    if save_to:
+        ctx.surrogate_save(save_to)
+    return val
+
+
+def load_surrogate(path=None, device=0):
+    """Load trained weights (default: the weights shipped with the package)."""
+    context(device).surrogate_load(path or SURROGATE_WEIGHTS)
+    _surrogate_ready.add(device)
+
+
+def _ensure_surrogate(gf, device=0):
+    if gf == _lib.GF_SURROGATE and device not in _surrogate_ready:
+        if not os.path.exists(SURROGATE_WEIGHTS):
+            raise ValueError("SurrogateGuevelDelhommeau needs a trained network: call mhx.train_surrogate() or "
+                             "mhx.load_surrogate(path)")
+        load_surrogate(device=device)
 
 # --------------------------------------------------------------------------- bodies
 def translational_dofs(mesh, name):
@@ -260,6 +298,7 @@ def excitation_force(res):
 # --------------------------------------------------------------------------- solves
 def _body_coeffs(body, names, omegas, betas, gf, direct, mode, rho, g, device=0):
     ctx = context(device)
+    _ensure_surrogate(gf, device)
     ctx.set_constants(rho, g)
     L = _lib.lib()
     key = tuple(names)
@@ -352,6 +391,7 @@ def assemble_matrices(green_functions, mesh, wavenumber, direct=True, include_id
     """S, D (or K for the indirect method) as n x n complex column-major arrays."""
     gf = _wave_code(green_functions)
     ctx = context(device)
+    _ensure_surrogate(gf, device)
     n = mesh.nfaces
     S = _lib.complex_out((n, n))
     D = _lib.complex_out((n, n))
