@@ -55,8 +55,8 @@ struct mhx_body {
     double *betas = nullptr, *d_ws = nullptr, *d_ks = nullptr;
     int *ipiv = nullptr, *perm = nullptr, *wperm = nullptr, *info = nullptr;
     unsigned long long* status = nullptr;
-    int *fb_list = nullptr, *fb_count = nullptr;
-    int fb_cap = 0;
+    unsigned long long *fb_list = nullptr, *fb_count = nullptr;  // surrogate fallback pairs
+    int64_t fb_cap = 0;
     int64_t nn() const { return (int64_t)n * n; }
     int64_t nR() const { return (int64_t)n * cap_R; }
     int64_t sW() const { return 16 * nR() + mhx::getrs_work(n, cap_R); }  // split-K partials + getrs work
@@ -193,11 +193,15 @@ int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int 
                                include_identity, b->S, b->D, b->nn());
     } else if (gf == MHX_GF_SURROGATE) {
         if (!ctx->surr_ready) return fail(ctx, MHX_EINVAL, "surrogate: no trained network (train or load one)");
-        const int64_t cap = (int64_t)b->n * (b->n + 1) / 2;
-        if (!b->fb_list) {
+        const int64_t cap = mhx::surr_fallback_capacity(b->n, batch);
+        if (b->fb_cap < cap) {
             int rc;
-            if ((rc = dalloc(ctx, &b->fb_list, 2 * (size_t)cap)) || (rc = dalloc(ctx, &b->fb_count, 1))) return rc;
-            b->fb_cap = (int)cap;
+            dfree(b->fb_list);
+            b->fb_list = nullptr;
+            b->fb_cap = 0;
+            if (!b->fb_count && (rc = dalloc(ctx, &b->fb_count, 1))) return rc;
+            if ((rc = dalloc(ctx, &b->fb_list, (size_t)cap))) return rc;
+            b->fb_cap = cap;
         }
         HCHK(hipMemsetAsync(b->status, 0, 4 * sizeof(unsigned long long), ctx->stream));
         {
@@ -349,6 +353,8 @@ int mhx_create(mhx_ctx** out, const mhx_options* opt) {
 void mhx_destroy(mhx_ctx* c) {
     if (!c) return;
     flush_timing(c);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    dfree(c->surr_params); dfree(c->surr_B); dfree(c->surr_bias); dfree(c->surr_W);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -526,7 +532,8 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
     const bool table = gf != MHX_GF_WU;
     int chunk = 1;
     if (mode == MHX_MODE_BATCHED && nw > 1) {
-        const double per = (double)n * n * sizeof(cplx) * (table ? 4 : 2) + 20.0 * n * 16 * sizeof(cplx);
+        double per = (double)n * n * sizeof(cplx) * (table ? 4 : 2) + 20.0 * n * 16 * sizeof(cplx);
+        if (gf == MHX_GF_SURROGATE) per += 8.0 * (double)mhx::surr_fallback_capacity((int)n, 1);  // fallback list
         chunk = (int)std::min<double>((double)std::min<int64_t>(nw, ctx->max_batch), std::max(1.0, ctx->batch_bytes / per));
     }
     int rc;
@@ -653,7 +660,7 @@ int surr_alloc(mhx_ctx* ctx) {
     int rc;
     const int64_t P = mhx::surr_param_count();
     if ((rc = dalloc(ctx, &ctx->surr_params, (size_t)P)) || (rc = dalloc(ctx, &ctx->surr_B, 2 * SURR_NF)) ||
-        (rc = dalloc(ctx, &ctx->surr_W, (size_t)SURR_PACKED_W)) || (rc = dalloc(ctx, &ctx->surr_bias, 4 * 128 + 32)))
+        (rc = dalloc(ctx, &ctx->surr_W, (size_t)SURR_PACKED_W)) || (rc = dalloc(ctx, &ctx->surr_bias, SURR_BIAS)))
         return rc;
     return MHX_OK;
 }

@@ -10,7 +10,8 @@
 #define SURR_VMIN 1e-3      // below: free-surface region, exact fallback
 #define SURR_IM_VC 5.0      // imaginary outputs are residuals relative to exp(-min(v, SURR_IM_VC))
 #define SURR_VAL_STRIDE 64  // every 64th sample is held out for validation
-#define SURR_PACKED_W (128 * 64 + 3 * 128 * 128 + 32 * 128)  // bf16 weights, output layer padded to 32
+#define SURR_PACKED_W (128 * 64 + 3 * 128 * 128 + 32 * 128)  // bf16 weights, output layer padded to 32 rows
+#define SURR_BIAS (4 * 128 + 8)
 
 namespace mhx {
 
@@ -31,9 +32,11 @@ void surr_forward_backward(hipStream_t st, SurrTrainWork& w, int M, const int* i
 void surr_adam(hipStream_t st, int64_t n, float* p, const float* g, float* m1, float* m2, float lr, int step);
 void launch_batch_index(hipStream_t st, int M, int64_t nsamples, uint64_t seed, int* idx);
 void launch_surr_pack(hipStream_t st, const float* params, unsigned short* Wb, float* bias);
+int64_t surr_fallback_capacity(int n, int batch);
 void launch_surr_table(hipStream_t st, int n, const PanelGeom* geom, const double* ks, int batch,
                        const unsigned short* Wb, const float* bias, const float* Bf, cplx* WG, cplx* WR, int64_t sMat,
-                       int* fb_list, int* fb_count, int fb_cap, unsigned long long* status);
+                       unsigned long long* fb_list, unsigned long long* fb_count, int64_t fb_cap,
+                       unsigned long long* status);
 void launch_surr_eval(hipStream_t st, int M, const double* rv, const unsigned short* Wb, const float* bias,
                       const float* Bf, float* out);
 

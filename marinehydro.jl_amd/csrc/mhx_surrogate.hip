@@ -315,10 +315,32 @@ void launch_batch_index(hipStream_t st, int M, int64_t nsamples, uint64_t seed, 
 }
 
 // ---------------------------------------------------------------------------
-// inference: BF16 MFMA, fused into the pair table
+// inference: BF16 MFMA (v_mfma_f32_32x32x16_bf16), weights resident in LDS
 // ---------------------------------------------------------------------------
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+// Each wave evaluates the network on 64 points (two column tiles of 32) in the
+// transposed form  H_next^T = W H^T: the weights are the A operand (32 neurons x 16
+// inputs per lane group, read from LDS), the activations the B operand (16 inputs x
+// 32 points).  With the 32x32 accumulator layout (lane l, register 4g + i holds row
+// 8g + 4(l/32) + i, column l%32) the eight accumulator values a lane holds for rows
+// 16b + {0..3, 8..11} + 4(l/32) are exactly the eight B-operand inputs that lane needs
+// for input block 2t + b of the next layer, so activations stay in registers from
+// layer to layer: the packed weights carry the matching input permutation
+//     input slot (block kb, lane half h, element e) <-> neuron 16 kb + 8 (e/4) + 4 h + e%4.
+// LDS layout of the packed weights (bf16x8 units, one per lane):
+//   layer 0  [tile 0..3][kb 0..3][lane 0..63]   inputs: Fourier features
+//   layer 1-3 [tile 0..3][kb 0..7][lane]
+//   output   [kb 0..7][lane]                    rows 0..5 used, 6..31 zero
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+#define SW_UNITS (16 * 64 + 3 * 32 * 64 + 8 * 64)  // 7680 bf16x8 = 122,880 B
+__host__ __device__ constexpr int sw_layer_off(int l) { return l == 0 ? 0 : (l <= 3 ? 16 * 64 + (l - 1) * 32 * 64 : 16 * 64 + 96 * 64); }
+
+struct SurrLds {
+    bf16x8 w[SW_UNITS];
+    float bias[4 * 128 + 8];
+    float fourier[2 * SURR_NF];
+};
 
 __host__ __device__ __forceinline__ unsigned short f2bf(float f) {
     union {
@@ -335,25 +357,34 @@ __device__ __forceinline__ int IN_DIM_D(int l) { return l == 0 ? 64 : 128; }
 __device__ __forceinline__ int OUT_DIM_D(int l) { return l == 4 ? SURR_NOUT : 128; }
 __device__ __forceinline__ int64_t OFF_W_D(int l) { return l == 0 ? 0 : 128 * 64 + 128 + (int64_t)(l - 1) * (128 * 128 + 128); }
 
-// FP32 parameters -> BF16 weight blocks [l][n][k] (n padded to 32 for the output layer)
+// input index (in the FP32 parameter layout) feeding slot (kb, h, e) of layer l
+__device__ __forceinline__ int surr_slot_input(int l, int kb, int h, int e) {
+    if (l == 0) {  // feature: sin (e < 4) or cos of frequency 16 h + 4 kb + e % 4
+        const int f = 16 * h + 4 * kb + (e & 3);
+        return (e < 4 ? 0 : SURR_NF) + f;
+    }
+    return 16 * kb + 8 * (e >> 2) + 4 * h + (e & 3);
+}
+
+// FP32 parameters -> packed BF16 units (layout above) and FP32 biases [l][128] + output[8]
 __global__ void surr_pack_kernel(const float* __restrict__ params, unsigned short* __restrict__ Wb,
                                  float* __restrict__ bias) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    // layer offsets in Wb: 128*64, 3 * 128*128, 32*128
-    if (t < SURR_PACKED_W) {
-        int l, o = t;
-        int64_t base;
-        if (o < 128 * 64) { l = 0; base = 0; }
-        else if (o < 128 * 64 + 3 * 128 * 128) { l = 1 + (o - 128 * 64) / (128 * 128); base = 128 * 64 + (int64_t)(l - 1) * 128 * 128; }
-        else { l = 4; base = 128 * 64 + 3 * 128 * 128; }
-        const int in = IN_DIM_D(l);
-        const int r = (int)((o - base) / in), k = (int)((o - base) % in);
+    if (t < SW_UNITS * 8) {
+        const int unit = t >> 3, e = t & 7;
+        const int lane = unit & 63, blk = unit >> 6;
+        int l, tile, kb;
+        if (blk < 16) { l = 0; tile = blk / 4; kb = blk % 4; }
+        else if (blk < 16 + 96) { l = 1 + (blk - 16) / 32; tile = ((blk - 16) % 32) / 8; kb = (blk - 16) % 8; }
+        else { l = 4; tile = 0; kb = blk - 112; }
+        const int row = 32 * tile + (lane & 31), h = lane >> 5;
+        const int in = surr_slot_input(l, kb, h, e);
         float v = 0.f;
-        if (r < OUT_DIM_D(l)) v = params[OFF_W_D(l) + (int64_t)r * in + k];
+        if (row < OUT_DIM_D(l)) v = params[OFF_W_D(l) + (int64_t)row * IN_DIM_D(l) + in];
         Wb[t] = f2bf(v);
     }
-    if (t < 4 * 128 + 32) {
-        const int l = t / 128 < 4 ? t / 128 : 4;
+    if (t < 4 * 128 + 8) {
+        const int l = t < 512 ? t / 128 : 4;
         const int r = t - 128 * l;
         float v = 0.f;
         if (r < OUT_DIM_D(l)) v = params[OFF_W_D(l) + (int64_t)OUT_DIM_D(l) * IN_DIM_D(l) + r];
@@ -362,158 +393,199 @@ __global__ void surr_pack_kernel(const float* __restrict__ params, unsigned shor
 }
 
 void launch_surr_pack(hipStream_t st, const float* params, unsigned short* Wb, float* bias) {
-    surr_pack_kernel<<<(SURR_PACKED_W + 255) / 256, 256, 0, st>>>(params, Wb, bias);
+    surr_pack_kernel<<<(SW_UNITS * 8 + 255) / 256, 256, 0, st>>>(params, Wb, bias);
 }
 
-// one wave: 32 pairs x 128 neurons on MFMA 32x32x8 (four 32-column tiles)
-template <int KIN, int NT>
-__device__ __forceinline__ void surr_layer(const unsigned short* __restrict__ act, int act_ld,
-                                           const unsigned short* __restrict__ W, f32x16 acc[NT], int lane) {
-#pragma unroll
-    for (int t = 0; t < NT; t++)
-#pragma unroll
-        for (int q = 0; q < 16; q++) acc[t][q] = 0.f;
-    const int row = lane & 31, kq = (lane >> 5) * 4;
-#pragma unroll 2
-    for (int k0 = 0; k0 < KIN; k0 += 8) {
-        const bf16x4 a = *reinterpret_cast<const bf16x4*>(act + row * act_ld + k0 + kq);
-#pragma unroll
-        for (int t = 0; t < NT; t++) {
-            const bf16x4 b = *reinterpret_cast<const bf16x4*>(W + (t * 32 + row) * KIN + k0 + kq);
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc[t], 0, 0, 0);
-        }
-    }
+// cooperative copy of the packed network into LDS (whole block)
+__device__ __forceinline__ void surr_load_lds(SurrLds& L, const unsigned short* __restrict__ Wb,
+                                              const float* __restrict__ bias, const float* __restrict__ Bf) {
+    const bf16x8* src = reinterpret_cast<const bf16x8*>(Wb);
+    for (int q = threadIdx.x; q < SW_UNITS; q += blockDim.x) L.w[q] = src[q];
+    for (int q = threadIdx.x; q < 4 * 128 + 8; q += blockDim.x) L.bias[q] = bias[q];
+    for (int q = threadIdx.x; q < 2 * SURR_NF; q += blockDim.x) L.fourier[q] = Bf[q];
+    __syncthreads();
 }
 
-#define SURR_TILE_I 16
-#define SURR_TILE_J 8
-#define SURR_ACT_LD 136  // 128 + 8 bf16 padding
+__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 
-// One workgroup (4 waves) handles a 16 x 8 block of (i, j) pairs per step (each wave
-// 32 pairs: 4 rows i x 8 columns j), for unordered pairs i <= j.  Pairs in the box go
-// through the network; others are appended to the exact fallback list.
-__global__ void __launch_bounds__(256) surr_table_kernel(int n, const PanelGeom* __restrict__ geom,
-                                                         const double* __restrict__ ks,
-                                                         const unsigned short* __restrict__ Wb,
-                                                         const float* __restrict__ bias, const float* __restrict__ Bf,
-                                                         cplx* __restrict__ WGb, cplx* __restrict__ WRb, int64_t sMat,
-                                                         int* __restrict__ fb_list, int* __restrict__ fb_count,
-                                                         int fb_cap) {
-    const int bi = blockIdx.x, bj = blockIdx.y;
-    const int i0 = bi * SURR_TILE_I, j0 = bj * SURR_TILE_J;
-    if (i0 > j0 + SURR_TILE_J - 1) return;  // whole tile below the diagonal
-    const double k = ks[blockIdx.z];
-    cplx* WG = WGb + sMat * blockIdx.z;
-    cplx* WR = WRb + sMat * blockIdx.z;
-    __shared__ unsigned short act[128 * SURR_ACT_LD];
-    __shared__ float outv[128][SURR_NOUT];
-    __shared__ float sB[2 * SURR_NF];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid < 2 * SURR_NF) sB[tid] = Bf[tid];
-    // pair of this thread (threads 0..127): local p = tid; i = i0 + p / 8, j = j0 + p % 8
-    const int p = tid & 127;
-    const int i = i0 + p / SURR_TILE_J, j = j0 + p % SURR_TILE_J;
-    double hh = 0.0, vv = 0.0;
-    bool valid = false, net = false;
-    if (i < n && j < n && i <= j) {
-        const PanelGeom& gi = geom[i];
-        const PanelGeom& gj = geom[j];
-        reduced_hv(gi.c[0], gi.c[1], gi.c[2], gj.c[0], gj.c[1], gj.c[2], k, hh, vv);
-        valid = true;
-        net = surr_in_box(hh, -vv);
-    }
-    __syncthreads();
-    // features: each of the 256 threads writes 32 of the 64 features of its pair (bf16)
-    {
-        float u1, u2;
-        surr_inputs(hh, -vv, u1, u2);
-        const int half = tid >> 7;  // features [32 half, 32 half + 32)
-        for (int f = half * 32; f < half * 32 + 32; f++) {
-            const int ff = f % SURR_NF;
-            const float ph = 6.283185307179586f * (sB[2 * ff] * u1 + sB[2 * ff + 1] * u2);
-            const float val = f < SURR_NF ? sinf(ph) : cosf(ph);
-            act[p * SURR_ACT_LD + f] = f2bf(val);
+// one hidden layer on the two column tiles: act (B operands, KB input blocks) -> act (8 blocks).
+// Output neuron tiles are done one at a time so that only 32 accumulators are live.
+template <int KB>
+__device__ __forceinline__ void surr_hidden(const SurrLds& L, int layer, int lane, bf16x8 act[2][8]) {
+    const int h = lane >> 5;
+    const bf16x8* W = L.w + sw_layer_off(layer);
+    const float* bias = L.bias + 128 * layer;
+    bf16x8 nxt[2][8];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        // keep the scheduler from hoisting the next tile's LDS loads (register pressure)
+        __builtin_amdgcn_sched_barrier(0);
+        f32x16 acc0, acc1;
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            const float4 b = *reinterpret_cast<const float4*>(bias + 32 * t + 8 * g + 4 * h);
+            acc0[4 * g + 0] = acc1[4 * g + 0] = b.x;
+            acc0[4 * g + 1] = acc1[4 * g + 1] = b.y;
+            acc0[4 * g + 2] = acc1[4 * g + 2] = b.z;
+            acc0[4 * g + 3] = acc1[4 * g + 3] = b.w;
         }
-    }
-    __syncthreads();
-    // four waves x 32 pairs: pairs [32 wave, 32 wave + 32)
-    unsigned short* myact = act + (32 * wave) * SURR_ACT_LD;
-    const unsigned short* Wl = Wb;
-    f32x16 acc[4];
-    // hidden layers
-#pragma unroll 1
-    for (int l = 0; l < 4; l++) {
-        if (l == 0) surr_layer<64, 4>(myact, SURR_ACT_LD, Wl, acc, lane);
-        else surr_layer<128, 4>(myact, SURR_ACT_LD, Wl, acc, lane);
-        Wl += (l == 0 ? 128 * 64 : 128 * 128);
-        // SiLU, back to bf16 in place (same wave: its reads above are complete)
 #pragma unroll
-        for (int t = 0; t < 4; t++)
+        for (int kb = 0; kb < KB; kb++) {
+            const bf16x8 a = W[(t * KB + kb) * 64 + lane];
+            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, act[0][kb], acc0, 0, 0, 0);
+            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, act[1][kb], acc1, 0, 0, 0);
+        }
 #pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const int row = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-                const int col = t * 32 + (lane & 31);
-                float z = acc[t][q] + bias[l * 128 + col];
-                z = z / (1.0f + __expf(-z));
-                myact[row * SURR_ACT_LD + col] = f2bf(z);
+        for (int b = 0; b < 2; b++) {
+            bf16x8 v0, v1;
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int q = 4 * (2 * b + (e >> 2)) + (e & 3);
+                v0[e] = (__bf16)silu_fast(acc0[q]);
+                v1[e] = (__bf16)silu_fast(acc1[q]);
             }
+            nxt[0][2 * t + b] = v0;
+            nxt[1][2 * t + b] = v1;
+        }
     }
-    // output layer (6 of 32 columns used)
-    {
-        f32x16 o[1];
-        surr_layer<128, 1>(myact, SURR_ACT_LD, Wl, o, lane);
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int row = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-            const int col = lane & 31;
-            if (col < SURR_NOUT) outv[32 * wave + row][col] = o[0][q] + bias[4 * 128 + col];
-        }
-    }
-    __syncthreads();
-    if (tid >= 128 || !valid) return;
-    if (!net) {
-        const int slot = atomicAdd(fb_count, 1);
-        if (slot < fb_cap) {
-            fb_list[2 * slot] = i;
-            fb_list[2 * slot + 1] = j;
-        }
-        return;
-    }
-    cplx W, Wh, Wv;
-    wu_term(hh > 0 ? hh : 1e-300, vv, W, Wh, Wv);
-    const double is = 1.0 / SURR_SCALE;
-    const double ii = is * exp(-fmin(-vv, SURR_IM_VC));
-    const cplx G = cmk(W.x + is * outv[p][0], W.y + ii * outv[p][1]);
-    const cplx Gr = cmk(Wh.x + is * outv[p][2], Wh.y + ii * outv[p][3]);
-    WG[i + (int64_t)n * j] = G;
-    WR[i + (int64_t)n * j] = Gr;
-    if (i != j) {
-        WG[j + (int64_t)n * i] = G;
-        WR[j + (int64_t)n * i] = Gr;
+    for (int kb = 0; kb < 8; kb++) {
+        act[0][kb] = nxt[0][kb];
+        act[1][kb] = nxt[1][kb];
     }
 }
 
-// exact term for the fallback pairs of one item
-__global__ void surr_fallback_kernel(int n, const PanelGeom* __restrict__ geom, const double* __restrict__ ks,
-                                     int item, const int* __restrict__ fb_list, const int* __restrict__ fb_count,
-                                     int fb_cap, cplx* __restrict__ WGb, cplx* __restrict__ WRb, int64_t sMat,
-                                     unsigned long long* __restrict__ status) {
-    const int cnt = min(*fb_count, fb_cap);
-    const double k = ks[item];
-    cplx* WG = WGb + sMat * item;
-    cplx* WR = WRb + sMat * item;
-    // grid-stride over the listed pairs (the list length is only known on the device)
-    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < cnt; t += gridDim.x * blockDim.x) {
-        const int i = fb_list[2 * t], j = fb_list[2 * t + 1];
-        const PanelGeom& gi = geom[i];
-        const PanelGeom& gj = geom[j];
-        double hh, vv;
-        reduced_hv(gi.c[0], gi.c[1], gi.c[2], gj.c[0], gj.c[1], gj.c[2], k, hh, vv);
-        QuadWork wk;
-        cplx G, Gr;
-        if (!hquad_pair(hh, vv, 1e-8, 1e-8, G, Gr, wk)) atomicOr(status, 1ull);
-        G = cscale(4.0, G);
-        Gr = cscale(4.0, Gr);
+// the network on the lane's two points (u of column tile 0 and 1); out[pt] holds the
+// output rows: lanes 0-31 outputs 0..3 (registers 0..3), lanes 32-63 outputs 4, 5 (registers 0, 1)
+__device__ __forceinline__ void surr_mlp64(const SurrLds& L, int lane, const float u1[2], const float u2[2],
+                                           f32x16 out[2]) {
+    const int h = lane >> 5;
+    bf16x8 act[2][8];
+#pragma unroll
+    for (int pt = 0; pt < 2; pt++)
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int f = 16 * h + 4 * kb + e;
+                // v_sin/v_cos take revolutions: sin(2 pi x)
+                const float x = L.fourier[2 * f] * u1[pt] + L.fourier[2 * f + 1] * u2[pt];
+                act[pt][kb][e] = (__bf16)__builtin_amdgcn_sinf(x);
+                act[pt][kb][4 + e] = (__bf16)__builtin_amdgcn_cosf(x);
+            }
+    surr_hidden<4>(L, 0, lane, act);
+    surr_hidden<8>(L, 1, lane, act);
+    surr_hidden<8>(L, 2, lane, act);
+    surr_hidden<8>(L, 3, lane, act);
+    const bf16x8* W = L.w + sw_layer_off(4);
+#pragma unroll
+    for (int pt = 0; pt < 2; pt++)
+#pragma unroll
+        for (int q = 0; q < 16; q++) out[pt][q] = 0.f;
+    if (h == 0) {
+#pragma unroll
+        for (int q = 0; q < 4; q++) out[0][q] = out[1][q] = L.bias[512 + q];
+    } else {
+        out[0][0] = out[1][0] = L.bias[512 + 4];
+        out[0][1] = out[1][1] = L.bias[512 + 5];
+    }
+#pragma unroll
+    for (int kb = 0; kb < 8; kb++) {
+        const bf16x8 a = W[kb * 64 + lane];
+        out[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, act[0][kb], out[0], 0, 0, 0);
+        out[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, act[1][kb], out[1], 0, 0, 0);
+    }
+}
+
+// triangular tile index -> (bi, bj), bi <= bj, t = bj (bj + 1) / 2 + bi
+__device__ __forceinline__ void tri_tile(int64_t t, int& bi, int& bj) {
+    int b = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((int64_t)(b + 1) * (b + 2) / 2 <= t) b++;
+    while ((int64_t)b * (b + 1) / 2 > t) b--;
+    bj = b;
+    bi = (int)(t - (int64_t)b * (b + 1) / 2);
+}
+
+#define SURR_WAVES 8
+
+// Persistent pair-table kernel over all items of the batch.  One work unit = 8 x 8 block of
+// panel pairs (i in 8 bi.., j in 8 bj.., bi <= bj; pairs with i > j are skipped); each wave takes
+// units in a strided order.  Pairs inside the trained box get Wu + network; others are
+// appended to the fallback list as (item << 40 | i << 20 | j).
+__global__ void __launch_bounds__(64 * SURR_WAVES) surr_table_kernel(
+    int n, int batch, const PanelGeom* __restrict__ geom, const double* __restrict__ ks,
+    const unsigned short* __restrict__ Wb, const float* __restrict__ bias, const float* __restrict__ Bf,
+    cplx* __restrict__ WGb, cplx* __restrict__ WRb, int64_t sMat, unsigned long long* __restrict__ fb_list,
+    unsigned long long* __restrict__ fb_count, int64_t fb_cap) {
+    __shared__ SurrLds L;
+    surr_load_lds(L, Wb, bias, Bf);
+    const int lane = threadIdx.x & 63, h = lane >> 5, c = lane & 31;
+    const int nb = (n + 7) / 8;
+    const int64_t per_item = (int64_t)nb * (nb + 1) / 2;
+    const int64_t units = per_item * batch;
+    const int64_t stride = (int64_t)gridDim.x * SURR_WAVES;
+    for (int64_t unit = (int64_t)blockIdx.x * SURR_WAVES + (threadIdx.x >> 6); unit < units; unit += stride) {
+        const int item = (int)(unit / per_item);
+        int bi, bj;
+        tri_tile(unit - (int64_t)item * per_item, bi, bj);
+        const double k = ks[item];
+        // the lane's two points: pairs c and 32 + c of the unit
+        float u1[2], u2[2];
+        double hh_me = 0.0, vv_me = 0.0;
+        bool valid_me = false;
+#pragma unroll
+        for (int pt = 0; pt < 2; pt++) {
+            const int p = 32 * pt + c;
+            const int i = 8 * bi + (p & 7), j = 8 * bj + (p >> 3);
+            double hh = 0.0, vv = 0.0;
+            const bool valid = i < n && j < n && i <= j;
+            if (valid) {
+                const PanelGeom& gi = geom[i];
+                const PanelGeom& gj = geom[j];
+                reduced_hv(gi.c[0], gi.c[1], gi.c[2], gj.c[0], gj.c[1], gj.c[2], k, hh, vv);
+            }
+            surr_inputs(hh, -vv, u1[pt], u2[pt]);
+            if (pt == h) {
+                hh_me = hh;
+                vv_me = vv;
+                valid_me = valid;
+            }
+        }
+        f32x16 out[2];
+        surr_mlp64(L, lane, u1, u2, out);
+        // outputs 0..3 of this lane's pair (32 h + c): own column tile 0, or lane c's tile 1
+        float o[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const float other = __shfl_xor(out[1][q], 32);
+            o[q] = h == 0 ? out[0][q] : other;
+        }
+        const int p = 32 * h + c;
+        const int i = 8 * bi + (p & 7), j = 8 * bj + (p >> 3);
+        const bool net = valid_me && surr_in_box(hh_me, -vv_me);
+        const bool fb = valid_me && !net;
+        // wave-aggregated append to the fallback list
+        const unsigned long long mask = __ballot(fb);
+        if (mask) {
+            unsigned long long base = 0;
+            const int leader = __ffsll((long long)mask) - 1;
+            if (lane == leader) base = atomicAdd(fb_count, (unsigned long long)__popcll(mask));
+            base = __shfl(base, leader);
+            if (fb) {
+                const unsigned long long slot = base + __popcll(mask & ((1ull << lane) - 1ull));
+                if (slot < (unsigned long long)fb_cap)
+                    fb_list[slot] = ((unsigned long long)item << 40) | ((unsigned long long)i << 20) | (unsigned)j;
+            }
+        }
+        if (!net) continue;
+        cplx W, Wh, Wv;
+        wu_term(hh_me > 0 ? hh_me : 1e-300, vv_me, W, Wh, Wv);
+        const double is = 1.0 / SURR_SCALE;
+        const double ii = is * exp(-fmin(-vv_me, SURR_IM_VC));
+        const cplx G = cmk(W.x + is * o[0], W.y + ii * o[1]);
+        const cplx Gr = cmk(Wh.x + is * o[2], Wh.y + ii * o[3]);
+        cplx* WG = WGb + sMat * item;
+        cplx* WR = WRb + sMat * item;
         WG[i + (int64_t)n * j] = G;
         WR[i + (int64_t)n * j] = Gr;
         if (i != j) {
@@ -523,79 +595,101 @@ __global__ void surr_fallback_kernel(int n, const PanelGeom* __restrict__ geom, 
     }
 }
 
-void launch_surr_table(hipStream_t st, int n, const PanelGeom* geom, const double* ks, int batch,
-                       const unsigned short* Wb, const float* bias, const float* Bf, cplx* WG, cplx* WR, int64_t sMat,
-                       int* fb_list, int* fb_count, int fb_cap, unsigned long long* status) {
-    const int ti = (n + SURR_TILE_I - 1) / SURR_TILE_I, tj = (n + SURR_TILE_J - 1) / SURR_TILE_J;
-    for (int b = 0; b < batch; b++) {
-        hipMemsetAsync(fb_count, 0, sizeof(int), st);
-        surr_table_kernel<<<dim3(ti, tj, 1), 256, 0, st>>>(n, geom, ks + b, Wb, bias, Bf, WG + sMat * b, WR + sMat * b,
-                                                           sMat, fb_list, fb_count, fb_cap);
-        // the fallback list of item b holds at most n(n+1)/2 <= fb_cap pairs
-        const int64_t maxp = (int64_t)n * (n + 1) / 2;
-        const int64_t blocks = std::min<int64_t>((maxp + 255) / 256, 2048);
-        surr_fallback_kernel<<<(int)blocks, 256, 0, st>>>(n, geom, ks, b, fb_list, fb_count, fb_cap, WG, WR, sMat,
-                                                           status);
+// exact term for the listed fallback pairs (grid-stride: the count is only known on the device)
+__global__ void surr_fallback_kernel(int n, const PanelGeom* __restrict__ geom, const double* __restrict__ ks,
+                                     const unsigned long long* __restrict__ fb_list,
+                                     const unsigned long long* __restrict__ fb_count, int64_t fb_cap,
+                                     cplx* __restrict__ WGb, cplx* __restrict__ WRb, int64_t sMat,
+                                     unsigned long long* __restrict__ status) {
+    const int64_t cnt = (int64_t)min(*fb_count, (unsigned long long)fb_cap);
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < cnt; t += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long e = fb_list[t];
+        const int item = (int)(e >> 40), i = (int)((e >> 20) & 0xFFFFF), j = (int)(e & 0xFFFFF);
+        const double k = ks[item];
+        const PanelGeom& gi = geom[i];
+        const PanelGeom& gj = geom[j];
+        double hh, vv;
+        reduced_hv(gi.c[0], gi.c[1], gi.c[2], gj.c[0], gj.c[1], gj.c[2], k, hh, vv);
+        QuadWork wk;
+        cplx G, Gr;
+        if (!hquad_pair(hh, vv, 1e-8, 1e-8, G, Gr, wk)) atomicOr(status, 1ull);
+        G = cscale(4.0, G);
+        Gr = cscale(4.0, Gr);
+        cplx* WG = WGb + sMat * item;
+        cplx* WR = WRb + sMat * item;
+        WG[i + (int64_t)n * j] = G;
+        WR[i + (int64_t)n * j] = Gr;
+        if (i != j) {
+            WG[j + (int64_t)n * i] = G;
+            WR[j + (int64_t)n * i] = Gr;
+        }
     }
 }
 
-// evaluate the network (MFMA path) at arbitrary (r, v): out[6 m + o], the raw outputs (real
-// parts: SURR_SCALE x residual; imaginary parts: SURR_SCALE exp(min(v, SURR_IM_VC)) x residual)
-__global__ void __launch_bounds__(256) surr_eval_kernel(int M, const double* __restrict__ rv,
-                                                        const unsigned short* __restrict__ Wb,
-                                                        const float* __restrict__ bias, const float* __restrict__ Bf,
-                                                        float* __restrict__ out) {
-    __shared__ unsigned short act[128 * SURR_ACT_LD];
-    __shared__ float sB[2 * SURR_NF];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    if (tid < 2 * SURR_NF) sB[tid] = Bf[tid];
-    const int p = tid & 127;
-    const int m = blockIdx.x * 128 + p;
-    double r = 0.0, v = 0.005;
-    if (m < M) { r = rv[2 * m]; v = rv[2 * m + 1]; }
-    __syncthreads();
-    float u1, u2;
-    surr_inputs(r, v, u1, u2);
-    const int half = tid >> 7;
-    for (int f = half * 32; f < half * 32 + 32; f++) {
-        const int ff = f % SURR_NF;
-        const float ph = 6.283185307179586f * (sB[2 * ff] * u1 + sB[2 * ff + 1] * u2);
-        act[p * SURR_ACT_LD + f] = f2bf(f < SURR_NF ? sinf(ph) : cosf(ph));
-    }
-    __syncthreads();
-    unsigned short* myact = act + (32 * wave) * SURR_ACT_LD;
-    const unsigned short* Wl = Wb;
-    f32x16 acc[4];
-#pragma unroll 1
-    for (int l = 0; l < 4; l++) {
-        if (l == 0) surr_layer<64, 4>(myact, SURR_ACT_LD, Wl, acc, lane);
-        else surr_layer<128, 4>(myact, SURR_ACT_LD, Wl, acc, lane);
-        Wl += (l == 0 ? 128 * 64 : 128 * 128);
+static int device_cus() {
+    int dev = 0, cus = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    return cus;
+}
+
+int64_t surr_fallback_capacity(int n, int batch) { return (int64_t)n * (n + 1) / 2 * batch; }
+
+void launch_surr_table(hipStream_t st, int n, const PanelGeom* geom, const double* ks, int batch,
+                       const unsigned short* Wb, const float* bias, const float* Bf, cplx* WG, cplx* WR, int64_t sMat,
+                       unsigned long long* fb_list, unsigned long long* fb_count, int64_t fb_cap,
+                       unsigned long long* status) {
+    hipMemsetAsync(fb_count, 0, sizeof(unsigned long long), st);
+    const int nb = (n + 7) / 8;
+    const int64_t units = (int64_t)nb * (nb + 1) / 2 * batch;
+    const int64_t blocks = std::min<int64_t>(device_cus(), (units + SURR_WAVES - 1) / SURR_WAVES);
+    surr_table_kernel<<<(int)blocks, 64 * SURR_WAVES, 0, st>>>(n, batch, geom, ks, Wb, bias, Bf, WG, WR, sMat,
+                                                               fb_list, fb_count, fb_cap);
+    surr_fallback_kernel<<<2048, 256, 0, st>>>(n, geom, ks, fb_list, fb_count, fb_cap, WG, WR, sMat, status);
+}
+
+// the network (MFMA path) at arbitrary (r, v): out[6 m + o], the raw outputs (real parts:
+// SURR_SCALE x residual; imaginary parts: SURR_SCALE exp(min(v, SURR_IM_VC)) x residual)
+__global__ void __launch_bounds__(64 * SURR_WAVES) surr_eval_kernel(int M, const double* __restrict__ rv,
+                                                                    const unsigned short* __restrict__ Wb,
+                                                                    const float* __restrict__ bias,
+                                                                    const float* __restrict__ Bf,
+                                                                    float* __restrict__ out) {
+    __shared__ SurrLds L;
+    surr_load_lds(L, Wb, bias, Bf);
+    const int lane = threadIdx.x & 63, h = lane >> 5, c = lane & 31;
+    const int units = (M + 63) / 64;
+    for (int unit = blockIdx.x * SURR_WAVES + (threadIdx.x >> 6); unit < units; unit += gridDim.x * SURR_WAVES) {
+        float u1[2], u2[2];
 #pragma unroll
-        for (int t = 0; t < 4; t++)
+        for (int pt = 0; pt < 2; pt++) {
+            const int m = 64 * unit + 32 * pt + c;
+            double r = 0.0, v = 0.005;
+            if (m < M) { r = rv[2 * m]; v = rv[2 * m + 1]; }
+            surr_inputs(r, v, u1[pt], u2[pt]);
+        }
+        f32x16 o[2];
+        surr_mlp64(L, lane, u1, u2, o);
 #pragma unroll
-            for (int q = 0; q < 16; q++) {
-                const int row = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-                const int col = t * 32 + (lane & 31);
-                float z = acc[t][q] + bias[l * 128 + col];
-                z = z / (1.0f + __expf(-z));
-                myact[row * SURR_ACT_LD + col] = f2bf(z);
+        for (int pt = 0; pt < 2; pt++) {
+            const int m = 64 * unit + 32 * pt + c;
+            if (m >= M) continue;
+            if (h == 0) {
+#pragma unroll
+                for (int q = 0; q < 4; q++) out[(int64_t)6 * m + q] = o[pt][q];
+            } else {
+                out[(int64_t)6 * m + 4] = o[pt][0];
+                out[(int64_t)6 * m + 5] = o[pt][1];
             }
-    }
-    f32x16 o[1];
-    surr_layer<128, 1>(myact, SURR_ACT_LD, Wl, o, lane);
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const int row = 8 * (q >> 2) + 4 * (lane >> 5) + (q & 3);
-        const int col = lane & 31;
-        const int mm = blockIdx.x * 128 + 32 * wave + row;
-        if (col < SURR_NOUT && mm < M) out[(int64_t)6 * mm + col] = o[0][q] + bias[4 * 128 + col];
+        }
     }
 }
 
 void launch_surr_eval(hipStream_t st, int M, const double* rv, const unsigned short* Wb, const float* bias,
                       const float* Bf, float* out) {
-    surr_eval_kernel<<<(M + 127) / 128, 256, 0, st>>>(M, rv, Wb, bias, Bf, out);
+    const int units = (M + 63) / 64;
+    const int blocks = std::max(1, std::min(device_cus(), (units + SURR_WAVES - 1) / SURR_WAVES));
+    surr_eval_kernel<<<blocks, 64 * SURR_WAVES, 0, st>>>(M, rv, Wb, bias, Bf, out);
 }
 
 }  // namespace mhx
