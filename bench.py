@@ -8,7 +8,7 @@ beta = 0, rho = 1025, g = 9.81, direct method.
 A step is one omega-solve: wave-term assembly (the wavenumber-independent
 Rankine part is cached per body), one LU, all right-hand sides (6 radiation +
 1 diffraction) and pressure integration into A, B and F_ex.  K steps use K
-frequencies spread over the 64-frequency sweep.  The mesh, dofs, the cached
+frequencies spread over the 64-frequency sweep (default: all 64, one batch).  The mesh, dofs, the cached
 Rankine part and the network weights are resident on the device before the
 timed region.
 
@@ -35,6 +35,7 @@ sys.path.insert(0, os.path.join(ROOT, "marinehydro.jl_amd"))
 METRIC = "omega-solves/s at 4k panels: surrogate+batched vs plain GPU, at <=1e-3 rel err"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 
 
@@ -123,10 +124,14 @@ def measure(ctx, run, ws, n, gf):
                 "work_per_frequency": {"integrand_evals": int(counts[0] // nw), "series_steps": int(counts[1] // nw),
                                        "cf_steps": int(counts[2] // nw)}}
     else:
+        # the surrogate path factors in FP32 (mixed-precision solver), the others in FP64
+        fp32 = gf == "Surrogate"
+        peak = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
         fl = 8.0 / 3.0 * n ** 3 * nw
         ach = fl / (tl * 1e-3) / 1e12
-        roof = {"bound": "mfma", "kernel": "getrf_batched (zgemm_mfma_kernel trailing updates)", "achieved": ach,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+        roof = {"bound": "mfma", "kernel": "getrf_batched (zgemm_mfma_kernel trailing updates, %s)" %
+                ("FP32 factor of the mixed-precision solver" if fp32 else "FP64"), "achieved": ach,
+                "peak": peak, "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
                 "avg_launch_ms": tl / cl, "launches": cl}
     lu_tflops = 8.0 / 3.0 * n ** 3 * nw / (tl * 1e-3) / 1e12 if tl else None
     per_w = {k: v[0] / nw for k, v in phases.items()}
@@ -136,8 +141,8 @@ def measure(ctx, run, ws, n, gf):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16, help="omega-solves timed (frequencies of the sweep)")
-    ap.add_argument("--warmup", type=int, default=16, help="untimed omega-solves (the first W timed frequencies)")
+    ap.add_argument("--steps", type=int, default=64, help="omega-solves timed (frequencies of the sweep)")
+    ap.add_argument("--warmup", type=int, default=64, help="untimed omega-solves (the first W timed frequencies)")
     ap.add_argument("--gf", default="Surrogate", choices=["Surrogate", "ExactGuevelDelhommeau", "Wu"])
     ap.add_argument("--schedule", default="batched", choices=["batched", "per_omega"])
     ap.add_argument("--cpu-rows", type=int, default=32)
@@ -182,7 +187,10 @@ def main():
     run_with(mode)(warm)   # body upload, Rankine part, buffers: outside the timed region
     log(f"bench: timing {args.steps} omega-solves ({args.schedule})")
     value, elapsed, phases, roof, lu_tflops, res = measure(ctx, run_with(mode), steps_w, n, args.gf)
-    log("per-frequency phase ms:", phases)
+    solved, steps, fallbacks, resid = ctx.solver_stats()
+    solver = {"mixed_precision_systems": solved, "refinement_steps": steps, "fp64_fallbacks": fallbacks,
+              "max_relative_residual": resid}
+    log("per-frequency phase ms:", phases, "solver:", solver)
     plain = None
     if not args.no_plain and (mode != _lib.MODE_PER_OMEGA or args.gf != "ExactGuevelDelhommeau"):
         kp = min(4, args.steps)
@@ -218,6 +226,7 @@ def main():
                    "parallelism": "single GPU"},
         "phases_ms_per_omega": phases,
         "lu_tflops": lu_tflops,
+        "solver": solver,
         "plain_gpu_per_omega": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "roofline": roof,

@@ -104,8 +104,24 @@ void mhx_body_destroy(mhx_body* body);
 int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* body, int64_t nw, const double* omegas, int64_t nb, const double* betas,
                     int gf, int direct, int mode, double* A_out, double* B_out, double* Fd_out, double* Fk_out);
 
-/* frequency-chunk limits of MHX_MODE_BATCHED (defaults 64 items, 64 GiB of device work memory) */
+/* frequency-chunk limits of MHX_MODE_BATCHED (defaults: 64 items, 60% of the device memory free at
+ * mhx_create) */
 int mhx_set_batch(mhx_ctx* ctx, int64_t max_items, double max_bytes);
+
+/* Dense solver precision (replaces LinearSolve's LUFactorization, src/matrix_assembly.jl:396-416):
+ *   MHX_LU_FP64   LU in FP64 with LAPACK pivoting (the reference's zgetrf/zgetrs);
+ *   MHX_LU_MIXED  LU of an FP32 copy (FP32 MFMA trailing updates), triangular solves with those
+ *                 factors in FP64 arithmetic, FP64 residuals against the FP64 matrix and iterative
+ *                 refinement to |b - A x| <= 1e-12 |b| per system; an item that does not reach it in
+ *                 4 steps (or has a singular FP32 factor) is re-solved with the FP64 LU;
+ *   MHX_LU_AUTO   (default) mixed for MHX_GF_SURROGATE, FP64 otherwise.
+ * mhx_solver_stats: counts[0] systems solved mixed, counts[1] refinement steps, counts[2] FP64
+ * fallbacks; max_residual: largest accepted relative residual (since mhx_reset_timing). */
+#define MHX_LU_AUTO 0
+#define MHX_LU_FP64 1
+#define MHX_LU_MIXED 2
+int mhx_set_lu_mode(mhx_ctx* ctx, int mode);
+int mhx_solver_stats(mhx_ctx* ctx, int64_t counts[3], double* max_residual);
 
 /* per-phase timing (HIP events on the context's stream, enabled by mhx_options.timing):
  * phase: "rankine", "wave_table" (exact wave term), "wave" (S, D formation), "lu", "solve", "forces";

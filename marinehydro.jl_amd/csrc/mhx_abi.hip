@@ -29,7 +29,7 @@ struct mhx_ctx {
     std::map<std::string, PhaseAcc> phases;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
     unsigned long long quad_work[3] = {0, 0, 0};  // exact wave term: evaluations, series terms, CF steps
-    double batch_bytes = 64.0 * (1ull << 30);      // device memory budget of one frequency chunk
+    double batch_bytes = 64.0 * (1ull << 30);      // device memory budget of one frequency chunk (set at creation)
     int64_t max_batch = 64;
     // learned wave-term surrogate
     bool surr_ready = false;
@@ -37,6 +37,10 @@ struct mhx_ctx {
     unsigned short* surr_W = nullptr;
     double surr_val_loss = -1.0;
     int64_t surr_steps = 0;
+    // dense solver: precision policy and mixed-precision statistics
+    int lu_mode = MHX_LU_AUTO;
+    int64_t refine_steps = 0, mixed_solves = 0, fp64_fallbacks = 0;
+    double max_residual = 0.0;
 };
 
 struct mhx_body {
@@ -57,6 +61,11 @@ struct mhx_body {
     unsigned long long* status = nullptr;
     unsigned long long *fb_list = nullptr, *fb_count = nullptr;  // surrogate fallback pairs
     int64_t fb_cap = 0;
+    // mixed-precision solver: FP32 factors, right-hand-side copy, residuals, norms
+    cplxf* Df = nullptr;
+    cplx *Rr = nullptr, *Tr = nullptr;
+    double* norms = nullptr;
+    int trs_epoch = 0;  // sweep counter of the triangular-solve flags in work (zeroed with work)
     int64_t nn() const { return (int64_t)n * n; }
     int64_t nR() const { return (int64_t)n * cap_R; }
     int64_t sW() const { return 16 * nR() + mhx::getrs_work(n, cap_R); }  // split-K partials + getrs work
@@ -153,7 +162,9 @@ void free_work(mhx_body* b) {
     dfree(b->S); dfree(b->D); dfree(b->WG); dfree(b->WR); dfree(b->B); dfree(b->Y); dfree(b->work);
     dfree(b->F); dfree(b->FK); dfree(b->betas); dfree(b->d_ws); dfree(b->d_ks); dfree(b->ipiv); dfree(b->perm);
     dfree(b->wperm); dfree(b->info); dfree(b->status); dfree(b->fb_list); dfree(b->fb_count);
+    dfree(b->Df); dfree(b->Rr); dfree(b->Tr); dfree(b->norms);
     b->fb_cap = 0;
+    b->trs_epoch = 0;
     b->cap_batch = b->cap_R = b->cap_nb = 0;
     b->cap_table = false;
 }
@@ -182,6 +193,8 @@ int ensure_work(mhx_ctx* ctx, mhx_body* b, int R, int nb, bool table, int batch 
         free_work(b);
         return rc;
     }
+    HCHK(hipMemsetAsync(b->work, 0, sizeof(cplx) * (size_t)b->sW() * B, ctx->stream));  // sweep flags
+    b->trs_epoch = 0;
     return MHX_OK;
 }
 
@@ -238,7 +251,32 @@ int check_status(mhx_ctx* ctx, mhx_body* b, int gf) {
 }
 
 // factor the `batch` matrices b->D in place and solve for the R columns of b->B; phi into b->Y
-int factor_and_solve(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch = 1) {
+bool use_mixed(const mhx_ctx* ctx, int gf) {
+    return ctx->lu_mode == MHX_LU_MIXED || (ctx->lu_mode == MHX_LU_AUTO && gf == MHX_GF_SURROGATE);
+}
+
+int check_info(mhx_ctx* ctx, mhx_body* b, int batch, bool* singular) {
+    std::vector<int> info(batch, 0);
+    int herr = 0;
+    int* errw = mhx::getrs_error_word(b->n, b->work + 16 * b->nR());
+    HCHK(hipMemcpyAsync(info.data(), b->info, sizeof(int) * batch, hipMemcpyDeviceToHost, ctx->stream));
+    HCHK(hipMemcpyAsync(&herr, errw, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HCHK(hipStreamSynchronize(ctx->stream));
+    if (herr) {
+        HCHK(hipMemsetAsync(errw, 0, sizeof(int), ctx->stream));
+        return fail(ctx, MHX_EDEVICE, "solve: triangular sweep gave up waiting for a block");
+    }
+    *singular = false;
+    for (int q = 0; q < batch; q++)
+        if (info[q]) {
+            *singular = true;
+            ctx->err = "LU: exactly zero pivot at column " + std::to_string(info[q]);
+        }
+    return MHX_OK;
+}
+
+// FP64 LU of D (in place) and the solves (rhs in b->B; solution potentials in b->Y)
+int solve_fp64(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
     const int n = b->n;
     HCHK(hipMemsetAsync(b->info, 0, sizeof(int) * batch, ctx->stream));
     {
@@ -253,20 +291,96 @@ int factor_and_solve(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch = 1
         if (direct) {
             mhx::launch_matmul_nr(ctx->stream, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), batch);
             mhx::getrs_batched(ctx->stream, 0, n, R, b->D, n, b->nn(), b->ipiv, n, b->perm, n, b->Y, b->nR(), W2,
-                               b->sW(), batch);
+                               b->sW(), batch, &b->trs_epoch);
         } else {
             mhx::getrs_batched(ctx->stream, 0, n, R, b->D, n, b->nn(), b->ipiv, n, b->perm, n, b->B, b->nR(), W2,
-                               b->sW(), batch);
+                               b->sW(), batch, &b->trs_epoch);
             mhx::launch_matmul_nr(ctx->stream, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), batch);
         }
     }
     HCHK(hipGetLastError());
-    std::vector<int> info(batch, 0);
-    HCHK(hipMemcpyAsync(info.data(), b->info, sizeof(int) * batch, hipMemcpyDeviceToHost, ctx->stream));
-    HCHK(hipStreamSynchronize(ctx->stream));
-    for (int q = 0; q < batch; q++)
-        if (info[q]) return fail(ctx, MHX_ESINGULAR, "LU: exactly zero pivot at column " + std::to_string(info[q]));
-    return MHX_OK;
+    bool singular = false;
+    int rc = check_info(ctx, b, batch, &singular);
+    if (rc) return rc;
+    return singular ? MHX_ESINGULAR : MHX_OK;
+}
+
+// Mixed precision: FP32 LU of a copy of D, solves with the FP32 factors in FP64 arithmetic,
+// FP64 residuals against the FP64 matrix and iterative refinement until the relative
+// residual |r| / |rhs| <= kRefineTol on every item.  Items that do not converge in
+// kRefineMax steps (or whose FP32 factor is singular) are redone with the FP64 LU.
+constexpr double kRefineTol = 1e-12;
+constexpr int kRefineMax = 4;
+
+int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
+    const int n = b->n;
+    hipStream_t st = ctx->stream;
+    int rc;
+    const size_t nRb = (size_t)b->nR() * batch;
+    if ((rc = dalloc(ctx, &b->Df, (size_t)b->nn() * b->cap_batch)) || (rc = dalloc(ctx, &b->Rr, (size_t)b->nR() * b->cap_batch)) ||
+        (rc = dalloc(ctx, &b->Tr, (size_t)b->nR() * b->cap_batch)) || (rc = dalloc(ctx, &b->norms, 2 * (size_t)b->cap_batch)))
+        return rc;
+    HCHK(hipMemsetAsync(b->info, 0, sizeof(int) * batch, st));
+    {
+        Phase ph(ctx, "lu");
+        mhx::convert_to_f32(st, b->nn() * batch, b->D, b->Df);
+        rc = mhx::getrf_batched(st, n, b->Df, n, b->nn(), batch, b->ipiv, n, b->wperm, b->info);
+        if (rc) return fail(ctx, MHX_EDIM, "LU: order above 4096 is not supported");
+    }
+    HCHK(hipGetLastError());
+    bool singular = false;
+    if ((rc = check_info(ctx, b, batch, &singular))) return rc;
+    bool converged = false;
+    if (!singular) {
+        Phase ph(ctx, "solve");
+        cplx* W2 = b->work + 16 * b->nR();
+        cplx* X = direct ? b->Y : b->B;  // the system's unknowns, solved in place
+        if (direct)
+            mhx::launch_matmul_nr(st, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), batch);
+        HCHK(hipMemcpyAsync(b->Rr, X, sizeof(cplx) * nRb, hipMemcpyDeviceToDevice, st));
+        mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, X, b->nR(), W2, b->sW(), batch,
+                           &b->trs_epoch);
+        std::vector<double> nr(2 * batch);
+        for (int it = 0; it <= kRefineMax; it++) {
+            mhx::launch_matmul_nr(st, n, R, b->D, b->nn(), X, b->nR(), b->Tr, b->nR(), b->work, b->sW(), batch);
+            HCHK(hipMemsetAsync(b->norms, 0, sizeof(double) * 2 * batch, st));
+            mhx::launch_residual(st, (int64_t)n * R, b->Rr, b->nR(), b->Tr, b->nR(), b->norms, batch);
+            HCHK(hipMemcpyAsync(nr.data(), b->norms, sizeof(double) * 2 * batch, hipMemcpyDeviceToHost, st));
+            HCHK(hipStreamSynchronize(st));
+            double worst = 0.0;
+            for (int q = 0; q < batch; q++) {
+                const double rel = nr[2 * q + 1] > 0 ? std::sqrt(nr[2 * q] / nr[2 * q + 1]) : std::sqrt(nr[2 * q]);
+                worst = std::max(worst, std::isfinite(rel) ? rel : 1e300);
+            }
+            if (worst <= kRefineTol) {
+                converged = true;
+                ctx->max_residual = std::max(ctx->max_residual, worst);
+                break;
+            }
+            if (it == kRefineMax) break;
+            mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, b->Tr, b->nR(), W2, b->sW(), batch,
+                               &b->trs_epoch, true);
+            mhx::launch_add(st, (int64_t)n * R, b->Tr, b->nR(), X, b->nR(), batch);
+            ctx->refine_steps += batch;
+        }
+        if (converged && !direct)
+            mhx::launch_matmul_nr(st, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), batch);
+        HCHK(hipGetLastError());
+    }
+    if (converged) {
+        ctx->mixed_solves += batch;
+        return MHX_OK;
+    }
+    // FP64 fallback (D is intact): restore the right-hand side and factor in FP64
+    ctx->fp64_fallbacks += batch;
+    if (!singular && !direct) HCHK(hipMemcpyAsync(b->B, b->Rr, sizeof(cplx) * nRb, hipMemcpyDeviceToDevice, st));
+    return solve_fp64(ctx, b, R, direct, batch);
+}
+
+int factor_and_solve(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch = 1, int gf = MHX_GF_WU) {
+    int rc = use_mixed(ctx, gf) ? solve_mixed(ctx, b, R, direct, batch) : solve_fp64(ctx, b, R, direct, batch);
+    if (rc == MHX_ESINGULAR) return fail(ctx, MHX_ESINGULAR, ctx->err);
+    return rc;
 }
 
 int set_freqs(mhx_ctx* ctx, mhx_body* b, const double* ws, int batch) {
@@ -346,6 +460,8 @@ int mhx_create(mhx_ctx** out, const mhx_options* opt) {
         delete c;
         return MHX_EDEVICE;
     }
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0) c->batch_bytes = 0.6 * (double)free_b;
     *out = c;
     return MHX_OK;
 }
@@ -436,22 +552,26 @@ int mhx_lu_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* A, int tra
     hipMemcpyAsync(dA, A, sizeof(cplx) * n * n, hipMemcpyHostToDevice, st);
     hipMemcpyAsync(dB, Bh, sizeof(cplx) * n * nrhs, hipMemcpyHostToDevice, st);
     hipMemsetAsync(info, 0, sizeof(int), st);
+    hipMemsetAsync(dW, 0, sizeof(cplx) * (size_t)mhx::getrs_work((int)n, (int)nrhs), st);  // sweep flags
+    int epoch = 0;
     {
         Phase ph(ctx, "lu");
         mhx::getrf_batched(st, (int)n, dA, (int)n, n * n, 1, ipiv, n, wperm, info);
     }
     {
         Phase ph(ctx, "solve");
-        mhx::getrs_batched(st, trans, (int)n, (int)nrhs, dA, (int)n, 0, ipiv, n, perm, n, dB, 0, dW, 0, 1);
+        mhx::getrs_batched(st, trans, (int)n, (int)nrhs, dA, (int)n, 0, ipiv, n, perm, n, dB, 0, dW, 0, 1, &epoch);
     }
-    int hinfo = 0;
+    int hinfo = 0, herr = 0;
     hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st);
+    hipMemcpyAsync(&herr, mhx::getrs_error_word((int)n, dW), sizeof(int), hipMemcpyDeviceToHost, st);
     hipMemcpyAsync(Bh, dB, sizeof(cplx) * n * nrhs, hipMemcpyDeviceToHost, st);
     hipError_t e = hipStreamSynchronize(st);
     if (e == hipSuccess) e = hipGetLastError();
     flush_timing(ctx);
     cleanup();
     if (e != hipSuccess) return fail(ctx, MHX_EDEVICE, std::string("lu_solve: ") + hipGetErrorString(e));
+    if (herr) return fail(ctx, MHX_EDEVICE, "lu_solve: triangular sweep gave up waiting for a block");
     if (hinfo) return fail(ctx, MHX_ESINGULAR, "LU: exactly zero pivot at column " + std::to_string(hinfo));
     return MHX_OK;
 }
@@ -534,6 +654,7 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
     if (mode == MHX_MODE_BATCHED && nw > 1) {
         double per = (double)n * n * sizeof(cplx) * (table ? 4 : 2) + 20.0 * n * 16 * sizeof(cplx);
         if (gf == MHX_GF_SURROGATE) per += 8.0 * (double)mhx::surr_fallback_capacity((int)n, 1);  // fallback list
+        if (use_mixed(ctx, gf)) per += (double)n * n * sizeof(cplxf) + 2.0 * n * 16 * sizeof(cplx);  // FP32 factors
         chunk = (int)std::min<double>((double)std::min<int64_t>(nw, ctx->max_batch), std::max(1.0, ctx->batch_bytes / per));
     }
     int rc;
@@ -548,7 +669,7 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
         mhx::launch_bc(st, n, b->geom, b->dofs, ndof, b->betas, (int)nb, b->d_ws, b->d_ks, bt, ctx->g, b->B, b->nR());
         if (mode != MHX_MODE_PER_PROBLEM) {
             if ((rc = assemble_into(ctx, b, bt, gf, direct, 1)) || (rc = check_status(ctx, b, gf))) break;
-            if ((rc = factor_and_solve(ctx, b, R, direct, bt))) break;
+            if ((rc = factor_and_solve(ctx, b, R, direct, bt, gf))) break;
         } else {
             // reference semantics: one assembly (Rankine part included, as the reference recomputes
             // it) and one LU per radiation / diffraction problem
@@ -560,7 +681,7 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
                 if ((rc = ensure_rankine(ctx, b, direct))) break;
                 if ((rc = assemble_into(ctx, b, 1, gf, direct, 1)) || (rc = check_status(ctx, b, gf))) break;
                 HCHK(hipMemcpyAsync(b->B, hB.data() + (size_t)n * r, sizeof(cplx) * n, hipMemcpyHostToDevice, st));
-                if ((rc = factor_and_solve(ctx, b, 1, direct))) break;
+                if ((rc = factor_and_solve(ctx, b, 1, direct, 1, gf))) break;
                 HCHK(hipMemcpyAsync(hphi.data() + (size_t)n * r, b->Y, sizeof(cplx) * n, hipMemcpyDeviceToHost, st));
                 HCHK(hipStreamSynchronize(st));
             }
@@ -612,6 +733,27 @@ void mhx_reset_timing(mhx_ctx* ctx) {
     flush_timing(ctx);
     ctx->phases.clear();
     ctx->quad_work[0] = ctx->quad_work[1] = ctx->quad_work[2] = 0;
+    ctx->refine_steps = ctx->mixed_solves = ctx->fp64_fallbacks = 0;
+    ctx->max_residual = 0.0;
+}
+
+int mhx_set_lu_mode(mhx_ctx* ctx, int mode) {
+    if (!ctx) return MHX_EINVAL;
+    if (mode != MHX_LU_AUTO && mode != MHX_LU_FP64 && mode != MHX_LU_MIXED)
+        return fail(ctx, MHX_EINVAL, "lu mode must be MHX_LU_AUTO, MHX_LU_FP64 or MHX_LU_MIXED");
+    ctx->lu_mode = mode;
+    return MHX_OK;
+}
+
+int mhx_solver_stats(mhx_ctx* ctx, int64_t counts[3], double* max_residual) {
+    if (!ctx) return MHX_EINVAL;
+    if (counts) {
+        counts[0] = ctx->mixed_solves;
+        counts[1] = ctx->refine_steps;
+        counts[2] = ctx->fp64_fallbacks;
+    }
+    if (max_residual) *max_residual = ctx->max_residual;
+    return MHX_OK;
 }
 
 int mhx_set_batch(mhx_ctx* ctx, int64_t max_items, double max_bytes) {

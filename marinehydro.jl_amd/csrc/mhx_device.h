@@ -20,6 +20,7 @@
 #define MHX_EULER 0.57721566490153286060651209
 
 typedef double2 cplx;
+typedef float2 cplxf;
 
 __device__ __forceinline__ cplx cmk(double r, double i) { return make_double2(r, i); }
 __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return cmk(a.x + b.x, a.y + b.y); }

@@ -15,16 +15,49 @@
 //   updated by a complex GEMM on FP64 MFMA (v_mfma_f64_16x16x4f64).
 // Every kernel takes a batch index (blockIdx.z) so many matrices of the same
 // order are factored by one launch sequence (frequency sweeps).
+//
+// The factorization is generic in the element type: complex FP64 (cplx) or
+// complex FP32 (cplxf, trailing updates on v_mfma_f32_16x16x4f32).  The FP32
+// factors serve the mixed-precision solver (FP32 LU, FP64 residuals and
+// iterative refinement); the triangular solves always run in FP64 arithmetic.
 #include "mhx_internal.h"
 
 namespace mhx {
+
+template <class C> struct Cx;
+template <> struct Cx<cplx> { typedef double R; };
+template <> struct Cx<cplxf> { typedef float R; };
+
+template <class C>
+__device__ __forceinline__ C tmk(typename Cx<C>::R r, typename Cx<C>::R i) {
+    C c;
+    c.x = r;
+    c.y = i;
+    return c;
+}
+template <class C> __device__ __forceinline__ C tsub(C a, C b) { return tmk<C>(a.x - b.x, a.y - b.y); }
+template <class C> __device__ __forceinline__ C tmul(C a, C b) {
+    return tmk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+// 1 / u (Smith's algorithm in the element's precision)
+template <class C> __device__ __forceinline__ C trecip(C u) {
+    typedef typename Cx<C>::R R;
+    if (fabs(u.x) >= fabs(u.y)) {
+        const R r = u.y / u.x, den = u.x + u.y * r;
+        return tmk<C>(R(1) / den, -r / den);
+    }
+    const R r = u.x / u.y, den = u.x * r + u.y;
+    return tmk<C>(r / den, R(-1) / den);
+}
+template <class C> __device__ __forceinline__ cplx to_d(C v) { return cmk((double)v.x, (double)v.y); }
 
 #define LW 4
 #define NB 64
 #define LEAF_THREADS 1024
 
+template <class C>
 struct MatBatch {
-    cplx* A;        // base pointer of matrix 0
+    C* A;           // base pointer of matrix 0
     int64_t stride; // elements between consecutive matrices
     int lda;
 };
@@ -34,53 +67,54 @@ struct MatBatch {
 // the other columns of the outer panel [K, KE); solve the 4-row U block of the
 // outer panel columns right of the leaf.
 // ---------------------------------------------------------------------------
-template <int ROWS>
-__global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch mb, int n, int c0, int K, int KE,
+template <int ROWS, class C>
+__global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch<C> mb, int n, int c0, int K, int KE,
                                                                   int* __restrict__ ipiv, int64_t ipiv_stride,
                                                                   int* __restrict__ info) {
+    typedef typename Cx<C>::R R;
     const int b = blockIdx.z;
-    cplx* A = mb.A + mb.stride * b;
+    C* A = mb.A + mb.stride * b;
     int* piv = ipiv + ipiv_stride * b;
     const int lda = mb.lda;
     const int tid = threadIdx.x;
-    __shared__ double red_v[LEAF_THREADS / 64];
+    __shared__ R red_v[LEAF_THREADS / 64];
     __shared__ int red_r[LEAF_THREADS / 64];
-    __shared__ cplx xrow[2][LW];
-    __shared__ cplx prow[LW];
+    __shared__ C xrow[2][LW];
+    __shared__ C prow[LW];
     __shared__ int s_piv;
     __shared__ int s_pivs[LW];
-    __shared__ cplx L11[LW][LW];
+    __shared__ C L11[LW][LW];
 
     const int wcols = min(LW, n - c0);
-    cplx x[ROWS][LW];
+    C x[ROWS][LW];
 #pragma unroll
     for (int q = 0; q < ROWS; q++) {
         int r = c0 + tid + LEAF_THREADS * q;
 #pragma unroll
-        for (int c = 0; c < LW; c++) x[q][c] = (r < n && c < wcols) ? A[r + (int64_t)lda * (c0 + c)] : cmk(0.0, 0.0);
+        for (int c = 0; c < LW; c++) x[q][c] = (r < n && c < wcols) ? A[r + (int64_t)lda * (c0 + c)] : tmk<C>(0, 0);
     }
 #pragma unroll
     for (int c = 0; c < LW; c++) {
         if (c >= wcols) break;
         const int col = c0 + c;
         // --- pivot search: max |Re|+|Im|, smallest row on ties
-        double bv = -1.0;
+        R bv = -1;
         int br = 0x7fffffff;
 #pragma unroll
         for (int q = 0; q < ROWS; q++) {
             int r = c0 + tid + LEAF_THREADS * q;
-            double v = fabs(x[q][c].x) + fabs(x[q][c].y);
+            R v = fabs(x[q][c].x) + fabs(x[q][c].y);
             if (r >= col && r < n && (v > bv || (v == bv && r < br))) { bv = v; br = r; }
         }
         for (int off = 32; off > 0; off >>= 1) {
-            double ov = __shfl_xor(bv, off);
+            R ov = __shfl_xor(bv, off);
             int orow = __shfl_xor(br, off);
             if (ov > bv || (ov == bv && orow < br)) { bv = ov; br = orow; }
         }
         if ((tid & 63) == 0) { red_v[tid >> 6] = bv; red_r[tid >> 6] = br; }
         __syncthreads();
         if (tid == 0) {
-            double v = red_v[0];
+            R v = red_v[0];
             int r = red_r[0];
             for (int w = 1; w < LEAF_THREADS / 64; w++)
                 if (red_v[w] > v || (red_v[w] == v && red_r[w] < r)) { v = red_v[w]; r = red_r[w]; }
@@ -116,17 +150,17 @@ __global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch mb, i
             if (tid == tc && q == qc)
                 for (int cc = 0; cc < LW; cc++) prow[cc] = x[q][cc];
         __syncthreads();
-        const cplx u = prow[c];
-        if (u.x != 0.0 || u.y != 0.0) {
-            const cplx inv = cdiv(cmk(1.0, 0.0), u);
+        const C u = prow[c];
+        if (u.x != 0 || u.y != 0) {
+            const C inv = trecip(u);
 #pragma unroll
             for (int q = 0; q < ROWS; q++) {
                 int r = c0 + tid + LEAF_THREADS * q;
                 if (r > col && r < n) {
-                    x[q][c] = cmul(x[q][c], inv);
+                    x[q][c] = tmul(x[q][c], inv);
 #pragma unroll
                     for (int cc = 0; cc < LW; cc++)
-                        if (cc > c) x[q][cc] = csub(x[q][cc], cmul(x[q][c], prow[cc]));
+                        if (cc > c) x[q][cc] = tsub(x[q][cc], tmul(x[q][c], prow[cc]));
                 }
             }
         } else if (tid == 0) {
@@ -152,20 +186,20 @@ __global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch mb, i
     for (int t = tid; t < nother; t += LEAF_THREADS) {
         int j = K + t;
         if (j >= c0) j += wcols;
-        cplx* colj = A + (int64_t)lda * j;
+        C* colj = A + (int64_t)lda * j;
         for (int c = 0; c < wcols; c++) {
             int r0 = c0 + c, r1 = s_pivs[c];
             if (r1 != r0) {
-                cplx t0 = colj[r0];
+                C t0 = colj[r0];
                 colj[r0] = colj[r1];
                 colj[r1] = t0;
             }
         }
         if (j > c0) {  // U12 rows c0..c0+wcols-1: unit-lower forward substitution
-            cplx u[LW];
+            C u[LW];
             for (int c = 0; c < wcols; c++) {
-                cplx s = colj[c0 + c];
-                for (int s2 = 0; s2 < c; s2++) s = csub(s, cmul(L11[c][s2], u[s2]));
+                C s = colj[c0 + c];
+                for (int s2 = 0; s2 < c; s2++) s = tsub(s, tmul(L11[c][s2], u[s2]));
                 u[c] = s;
                 colj[c0 + c] = s;
             }
@@ -175,11 +209,12 @@ __global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch mb, i
 
 // rank-LW update of the outer panel: A[r, j] -= sum_s L[r, c0+s] U[c0+s, j],
 // rows r in [c0+LW, n), columns j in [c0+LW, KE)
-__global__ void __launch_bounds__(256) panel_update_kernel(MatBatch mb, int n, int c0, int KE) {
+template <class C>
+__global__ void __launch_bounds__(256) panel_update_kernel(MatBatch<C> mb, int n, int c0, int KE) {
     const int b = blockIdx.z;
-    cplx* A = mb.A + mb.stride * b;
+    C* A = mb.A + mb.stride * b;
     const int lda = mb.lda;
-    __shared__ cplx U[LW][NB];
+    __shared__ C U[LW][NB];
     const int j0 = c0 + LW, nj = KE - j0;
     for (int t = threadIdx.x; t < LW * nj; t += blockDim.x) {
         int s = t / nj, jj = t % nj;
@@ -188,14 +223,14 @@ __global__ void __launch_bounds__(256) panel_update_kernel(MatBatch mb, int n, i
     __syncthreads();
     const int r = c0 + LW + blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    cplx l[LW];
+    C l[LW];
 #pragma unroll
     for (int s = 0; s < LW; s++) l[s] = A[r + (int64_t)lda * (c0 + s)];
     for (int jj = 0; jj < nj; jj++) {
-        cplx* p = A + r + (int64_t)lda * (j0 + jj);
-        cplx a = *p;
+        C* p = A + r + (int64_t)lda * (j0 + jj);
+        C a = *p;
 #pragma unroll
-        for (int s = 0; s < LW; s++) a = csub(a, cmul(l[s], U[s][jj]));
+        for (int s = 0; s < LW; s++) a = tsub(a, tmul(l[s], U[s][jj]));
         *p = a;
     }
 }
@@ -227,17 +262,18 @@ __global__ void build_perm_kernel(const int* __restrict__ ipiv, int64_t ipiv_str
 }
 
 // apply the permutation to columns outside [K, KE); one column per block-row of threads
-__global__ void __launch_bounds__(256) apply_perm_kernel(MatBatch mb, int n, int K, int KE, const int* __restrict__ perm,
-                                                         int64_t perm_stride) {
+template <class C>
+__global__ void __launch_bounds__(256) apply_perm_kernel(MatBatch<C> mb, int n, int K, int KE,
+                                                         const int* __restrict__ perm, int64_t perm_stride) {
     const int b = blockIdx.z;
-    cplx* A = mb.A + mb.stride * b;
+    C* A = mb.A + mb.stride * b;
     const int* pm = perm + perm_stride * b;
     const int m = pm[0];
     const int e = threadIdx.x % (2 * NB);
     const int cl = threadIdx.x / (2 * NB);
     int j = (blockIdx.x * (256 / (2 * NB))) + cl;
     const int nout = n - (KE - K);
-    cplx v = cmk(0.0, 0.0);
+    C v = tmk<C>(0, 0);
     bool act = e < m && j < nout;
     if (j >= K) j += KE - K;
     if (act) v = A[pm[1 + 2 * NB + e] + (int64_t)mb.lda * j];
@@ -246,12 +282,13 @@ __global__ void __launch_bounds__(256) apply_perm_kernel(MatBatch mb, int n, int
 }
 
 // U12 = L11^{-1} A12 for the outer panel rows [K, KE), columns [KE, n); 4 columns per block
-__global__ void __launch_bounds__(256) trsm_panel_kernel(MatBatch mb, int n, int K, int KE) {
+template <class C>
+__global__ void __launch_bounds__(256) trsm_panel_kernel(MatBatch<C> mb, int n, int K, int KE) {
     const int b = blockIdx.z;
-    cplx* A = mb.A + mb.stride * b;
+    C* A = mb.A + mb.stride * b;
     const int lda = mb.lda;
     const int nb = KE - K;
-    __shared__ cplx X[NB][4];
+    __shared__ C X[NB][4];
     const int i = threadIdx.x % NB, c = threadIdx.x / NB;
     const int j = KE + blockIdx.x * 4 + c;
     const bool act = i < nb && j < n;
@@ -259,8 +296,8 @@ __global__ void __launch_bounds__(256) trsm_panel_kernel(MatBatch mb, int n, int
     __syncthreads();
     for (int r = 0; r < nb - 1; r++) {
         if (act && i > r) {
-            cplx l = A[(K + i) + (int64_t)lda * (K + r)];
-            X[i][c] = csub(X[i][c], cmul(l, X[r][c]));
+            C l = A[(K + i) + (int64_t)lda * (K + r)];
+            X[i][c] = tsub(X[i][c], tmul(l, X[r][c]));
         }
         __syncthreads();
     }
@@ -268,27 +305,47 @@ __global__ void __launch_bounds__(256) trsm_panel_kernel(MatBatch mb, int n, int
 }
 
 // ---------------------------------------------------------------------------
-// C[m x nn] (-)= A[m x kk] * B[kk x nn], complex, column-major, on FP64 MFMA.
+// C[m x nn] (-)= A[m x kk] * B[kk x nn], complex, column-major, on MFMA
+// (v_mfma_f64_16x16x4f64 for cplx, v_mfma_f32_16x16x4f32 for cplxf).
 // 256 threads (4 waves in 2x2), 64x64 tile, K chunks of 16 staged in LDS as
-// separate real / imaginary planes.  v_mfma_f64_16x16x4f64 lane layout:
-//   A: lane l holds A[l%16][l/16], B: B[l/16][l%16],
-//   C: 4 values per lane, C[4*(l/16)+v][l%16].
+// separate real / imaginary planes.  Lane layout of both instructions:
+//   A: lane l holds A[l%16][l/16], B: B[l/16][l%16];
+//   C: 4 values per lane, f64: C[4v + l/16][l%16], f32: C[4(l/16) + v][l%16].
 // ---------------------------------------------------------------------------
 typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
 
-template <bool SUB, bool ACCUM>
-__global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, const cplx* __restrict__ Ab, int lda,
-                                                         int64_t sA, const cplx* __restrict__ Bb, int ldb, int64_t sB,
-                                                         cplx* __restrict__ Cb, int ldc, int64_t sC) {
+template <class R> struct Mfma16;
+template <> struct Mfma16<double> {
+    typedef double4_t V;
+    static __device__ __forceinline__ V run(double a, double b, V c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int v, int lk) { return 4 * v + lk; }
+};
+template <> struct Mfma16<float> {
+    typedef float4_t V;
+    static __device__ __forceinline__ V run(float a, float b, V c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int v, int lk) { return 4 * lk + v; }
+};
+
+template <bool SUB, bool ACCUM, class C>
+__global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, const C* __restrict__ Ab, int lda,
+                                                         int64_t sA, const C* __restrict__ Bb, int ldb, int64_t sB,
+                                                         C* __restrict__ Cb, int ldc, int64_t sC) {
+    typedef typename Cx<C>::R R;
+    typedef Mfma16<R> M;
     const int bz = blockIdx.z;
-    const cplx* A = Ab + sA * bz;
-    const cplx* B = Bb + sB * bz;
-    cplx* C = Cb + sC * bz;
-    __shared__ double Ar[16][64], Ai[16][64], Br[16][64], Bi[16][64];
+    const C* A = Ab + sA * bz;
+    const C* B = Bb + sB * bz;
+    C* Cm = Cb + sC * bz;
+    __shared__ R Ar[16][64], Ai[16][64], Br[16][64], Bi[16][64];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     const int row0 = blockIdx.x * 64, col0 = blockIdx.y * 64;
-    double4_t accr[2][2], acci[2][2];
+    typename M::V accr[2][2], acci[2][2];
     const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
     for (int bi = 0; bi < 2; bi++)
@@ -296,11 +353,11 @@ __global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, 
         for (int bj = 0; bj < 2; bj++) {
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                double cr = 0.0, ci = 0.0;
+                R cr = 0, ci = 0;
                 if (ACCUM) {
-                    int r = row0 + wm + bi * 16 + 4 * v + lk, c = col0 + wn + bj * 16 + li;
+                    int r = row0 + wm + bi * 16 + M::row(v, lk), c = col0 + wn + bj * 16 + li;
                     if (r < m && c < nn) {
-                        cplx t = C[r + (int64_t)ldc * c];
+                        C t = Cm[r + (int64_t)ldc * c];
                         cr = t.x; ci = t.y;
                     }
                 }
@@ -314,7 +371,7 @@ __global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, 
         for (int e = 0; e < 4; e++) {
             int idx = tid + 256 * e;
             int r = idx & 63, k = idx >> 6;
-            cplx t = cmk(0.0, 0.0);
+            C t = tmk<C>(0, 0);
             if (row0 + r < m && k0 + k < kk) t = A[(row0 + r) + (int64_t)lda * (k0 + k)];
             Ar[k][r] = SUB ? -t.x : t.x;
             Ai[k][r] = SUB ? -t.y : t.y;
@@ -324,7 +381,7 @@ __global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, 
         for (int e = 0; e < 4; e++) {
             int idx = tid + 256 * e;
             int k = idx & 15, c = idx >> 4;
-            cplx t = cmk(0.0, 0.0);
+            C t = tmk<C>(0, 0);
             if (col0 + c < nn && k0 + k < kk) t = B[(k0 + k) + (int64_t)ldb * (col0 + c)];
             Br[k][c] = t.x;
             Bi[k][c] = t.y;
@@ -332,7 +389,7 @@ __global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, 
         __syncthreads();
 #pragma unroll
         for (int ks = 0; ks < 16; ks += 4) {
-            double ar[2], ai[2], br[2], bi_[2];
+            R ar[2], ai[2], br[2], bi_[2];
 #pragma unroll
             for (int q = 0; q < 2; q++) {
                 ar[q] = Ar[ks + lk][wm + q * 16 + li];
@@ -344,10 +401,10 @@ __global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, 
             for (int bi = 0; bi < 2; bi++)
 #pragma unroll
                 for (int bj = 0; bj < 2; bj++) {
-                    accr[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[bi], br[bj], accr[bi][bj], 0, 0, 0);
-                    accr[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[bi], bi_[bj], accr[bi][bj], 0, 0, 0);
-                    acci[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[bi], bi_[bj], acci[bi][bj], 0, 0, 0);
-                    acci[bi][bj] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[bi], br[bj], acci[bi][bj], 0, 0, 0);
+                    accr[bi][bj] = M::run(ar[bi], br[bj], accr[bi][bj]);
+                    accr[bi][bj] = M::run(-ai[bi], bi_[bj], accr[bi][bj]);
+                    acci[bi][bj] = M::run(ar[bi], bi_[bj], acci[bi][bj]);
+                    acci[bi][bj] = M::run(ai[bi], br[bj], acci[bi][bj]);
                 }
         }
         __syncthreads();
@@ -358,33 +415,40 @@ __global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, 
         for (int bj = 0; bj < 2; bj++)
 #pragma unroll
             for (int v = 0; v < 4; v++) {
-                int r = row0 + wm + bi * 16 + 4 * v + lk, c = col0 + wn + bj * 16 + li;
-                if (r < m && c < nn) C[r + (int64_t)ldc * c] = cmk(accr[bi][bj][v], acci[bi][bj][v]);
+                int r = row0 + wm + bi * 16 + M::row(v, lk), c = col0 + wn + bj * 16 + li;
+                if (r < m && c < nn) Cm[r + (int64_t)ldc * c] = tmk<C>(accr[bi][bj][v], acci[bi][bj][v]);
             }
+}
+
+template <class C>
+static void gemm_t(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const C* A, int lda, int64_t sA,
+                   const C* B, int ldb, int64_t sB, C* Cm, int ldc, int64_t sC, int batch) {
+    if (m <= 0 || nn <= 0) return;
+    dim3 grid((m + 63) / 64, (nn + 63) / 64, batch);
+    if (sub && accum)
+        zgemm_mfma_kernel<true, true, C><<<grid, 256, 0, st>>>(m, nn, kk, A, lda, sA, B, ldb, sB, Cm, ldc, sC);
+    else if (!sub && accum)
+        zgemm_mfma_kernel<false, true, C><<<grid, 256, 0, st>>>(m, nn, kk, A, lda, sA, B, ldb, sB, Cm, ldc, sC);
+    else
+        zgemm_mfma_kernel<false, false, C><<<grid, 256, 0, st>>>(m, nn, kk, A, lda, sA, B, ldb, sB, Cm, ldc, sC);
 }
 
 void zgemm(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cplx* A, int lda, int64_t sA,
            const cplx* B, int ldb, int64_t sB, cplx* C, int ldc, int64_t sC, int batch) {
-    if (m <= 0 || nn <= 0) return;
-    dim3 grid((m + 63) / 64, (nn + 63) / 64, batch);
-    if (sub && accum)
-        zgemm_mfma_kernel<true, true><<<grid, 256, 0, st>>>(m, nn, kk, A, lda, sA, B, ldb, sB, C, ldc, sC);
-    else if (!sub && accum)
-        zgemm_mfma_kernel<false, true><<<grid, 256, 0, st>>>(m, nn, kk, A, lda, sA, B, ldb, sB, C, ldc, sC);
-    else
-        zgemm_mfma_kernel<false, false><<<grid, 256, 0, st>>>(m, nn, kk, A, lda, sA, B, ldb, sB, C, ldc, sC);
+    gemm_t(st, sub, accum, m, nn, kk, A, lda, sA, B, ldb, sB, C, ldc, sC, batch);
 }
 
 // ---------------------------------------------------------------------------
 // getrf driver.  ipiv: 0-based global row indices, int32, per matrix stride.
 // work_perm: int workspace of (1 + 4*NB) per matrix.  info: int per matrix (0 / first zero pivot, 1-based).
 // ---------------------------------------------------------------------------
-int getrf_batched(hipStream_t st, int n, cplx* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
-                  int* work_perm, int* info) {
+template <class C>
+static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
+                   int* work_perm, int* info) {
     if (n <= 0) return 0;
     const int rows_per = (n + LEAF_THREADS - 1) / LEAF_THREADS;
     if (rows_per > 4) return -1;  // orders above 4096 need a larger register tile
-    MatBatch mb{A, strideA, lda};
+    MatBatch<C> mb{A, strideA, lda};
     const int64_t pstride = 1 + 4 * NB;
     for (int K = 0; K < n; K += NB) {
         const int KE = min(n, K + NB);
@@ -393,28 +457,38 @@ int getrf_batched(hipStream_t st, int n, cplx* A, int lda, int64_t strideA, int 
             const int rp = (m_rows + LEAF_THREADS - 1) / LEAF_THREADS;
             dim3 g1(1, 1, batch);
             if (rp <= 1)
-                getf2_leaf_kernel<1><<<g1, LEAF_THREADS, 0, st>>>(mb, n, c0, K, KE, ipiv, ipiv_stride, info);
+                getf2_leaf_kernel<1, C><<<g1, LEAF_THREADS, 0, st>>>(mb, n, c0, K, KE, ipiv, ipiv_stride, info);
             else if (rp <= 2)
-                getf2_leaf_kernel<2><<<g1, LEAF_THREADS, 0, st>>>(mb, n, c0, K, KE, ipiv, ipiv_stride, info);
+                getf2_leaf_kernel<2, C><<<g1, LEAF_THREADS, 0, st>>>(mb, n, c0, K, KE, ipiv, ipiv_stride, info);
             else
-                getf2_leaf_kernel<4><<<g1, LEAF_THREADS, 0, st>>>(mb, n, c0, K, KE, ipiv, ipiv_stride, info);
+                getf2_leaf_kernel<4, C><<<g1, LEAF_THREADS, 0, st>>>(mb, n, c0, K, KE, ipiv, ipiv_stride, info);
             if (c0 + LW < KE && c0 + LW < n) {
                 dim3 g2((n - c0 - LW + 255) / 256, 1, batch);
-                panel_update_kernel<<<g2, 256, 0, st>>>(mb, n, c0, KE);
+                panel_update_kernel<C><<<g2, 256, 0, st>>>(mb, n, c0, KE);
             }
         }
         if (n - (KE - K) > 0) {
             build_perm_kernel<<<dim3(1, 1, batch), 64, 0, st>>>(ipiv, ipiv_stride, K, KE, work_perm, pstride);
             const int nout = n - (KE - K);
-            apply_perm_kernel<<<dim3((nout + 1) / 2, 1, batch), 256, 0, st>>>(mb, n, K, KE, work_perm, pstride);
+            apply_perm_kernel<C><<<dim3((nout + 1) / 2, 1, batch), 256, 0, st>>>(mb, n, K, KE, work_perm, pstride);
         }
         if (KE < n) {
-            trsm_panel_kernel<<<dim3((n - KE + 3) / 4, 1, batch), 256, 0, st>>>(mb, n, K, KE);
-            zgemm(st, true, true, n - KE, n - KE, KE - K, A + KE + (int64_t)lda * K, lda, strideA,
-                  A + K + (int64_t)lda * KE, lda, strideA, A + KE + (int64_t)lda * KE, lda, strideA, batch);
+            trsm_panel_kernel<C><<<dim3((n - KE + 3) / 4, 1, batch), 256, 0, st>>>(mb, n, K, KE);
+            gemm_t<C>(st, true, true, n - KE, n - KE, KE - K, A + KE + (int64_t)lda * K, lda, strideA,
+                      A + K + (int64_t)lda * KE, lda, strideA, A + KE + (int64_t)lda * KE, lda, strideA, batch);
         }
     }
     return 0;
+}
+
+int getrf_batched(hipStream_t st, int n, cplx* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
+                  int* work_perm, int* info) {
+    return getrf_t(st, n, A, lda, strideA, batch, ipiv, ipiv_stride, work_perm, info);
+}
+
+int getrf_batched(hipStream_t st, int n, cplxf* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
+                  int* work_perm, int* info) {
+    return getrf_t(st, n, A, lda, strideA, batch, ipiv, ipiv_stride, work_perm, info);
 }
 
 // ---------------------------------------------------------------------------
@@ -457,10 +531,11 @@ __global__ void permute_rows_kernel(int n, int R, const int* __restrict__ perm, 
 // in place in LDS (LAPACK trti2 order), one workgroup per (block, triangle, matrix).
 // Output: Inv[(blk * TRS_B + i) + TRS_B * nblk_rows...] stored as nblk consecutive
 // TRS_B x TRS_B column-major blocks per triangle and matrix.
-__global__ void __launch_bounds__(TRS_B) invert_diag_kernel(const cplx* __restrict__ Ab, int lda, int64_t sA, int n,
+template <class C>
+__global__ void __launch_bounds__(TRS_B) invert_diag_kernel(const C* __restrict__ Ab, int lda, int64_t sA, int n,
                                                             cplx* __restrict__ Linv, cplx* __restrict__ Uinv,
                                                             int64_t sInv) {
-    const cplx* A = Ab + sA * blockIdx.z;
+    const C* A = Ab + sA * blockIdx.z;
     const int blk = blockIdx.x, upper = blockIdx.y;
     const int b0 = blk * TRS_B, nbk = min(TRS_B, n - b0);
     cplx* out = (upper ? Uinv : Linv) + sInv * blockIdx.z + (int64_t)blk * TRS_B * TRS_B;
@@ -468,7 +543,7 @@ __global__ void __launch_bounds__(TRS_B) invert_diag_kernel(const cplx* __restri
     const int t = threadIdx.x;
     for (int j = 0; j < TRS_B; j++) {
         cplx v = cmk(0.0, 0.0);
-        if (t < nbk && j < nbk) v = A[(b0 + t) + (int64_t)lda * (b0 + j)];
+        if (t < nbk && j < nbk) v = to_d(A[(b0 + t) + (int64_t)lda * (b0 + j)]);
         T[t][j] = v;
     }
     __syncthreads();
@@ -506,126 +581,323 @@ __global__ void __launch_bounds__(TRS_B) invert_diag_kernel(const cplx* __restri
     }
 }
 
-__device__ __forceinline__ cplx tri_elem(const cplx* A, int lda, int i, int j, int conj) {
-    // element (i, j) of op(A): A or A^H
-    if (!conj) return A[i + (int64_t)lda * j];
-    cplx v = A[j + (int64_t)lda * i];
-    return cmk(v.x, -v.y);
+// ---------------------------------------------------------------------------
+// Sync-free blocked triangular solve: one persistent launch per pass.
+//   forward (lower=1): L (unit lower), or U^H when conj;  backward (lower=0): U, or L^H when conj.
+// Task (step, item) solves the TRS_B-row block blk = step (forward) or nblk-1-step (backward):
+//   acc = y_blk - sum over the blocks solved before it of op(T)[blk, c] x_c,  x_blk = op(Inv_blk) acc.
+// Tasks are dealt to workgroups in increasing step order and every workgroup runs its tasks in
+// that order; a task waits on the per-(item, block) flag of each dependency, which its producer
+// sets to this pass's epoch after its x stores have completed.  Flags and x are accessed only
+// with device-coherent (agent-scope) atomic loads and stores, so no cache-wide fence is needed:
+// the consumer issues its x loads after it has seen the flag, the producer raises the flag after
+// s_waitcnt vmcnt(0) on its x stores.  The smallest unfinished task therefore never waits on
+// anything unfinished, so the pass completes whenever the grid fits on the device at once (it is
+// sized to half of that).  A wait that exceeds TRS_SPIN_LIMIT polls (tens of milliseconds, far
+// beyond a pass) sets *err and gives up, and every later wait of the launch then gives up at
+// once (the result is invalid and the caller reports an error): no launch can hang.
+// ---------------------------------------------------------------------------
+#define TRS_SPIN_LIMIT (1 << 17)
+// s_waitcnt vmcnt(0) (gfx9-family encoding: expcnt and lgkmcnt left at their maxima)
+constexpr int kWaitVmcnt0 = 0x0F70;
+
+__device__ __forceinline__ cplx ld_agent(const cplx* p) {
+    const double* q = reinterpret_cast<const double*>(p);
+    return cmk(__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+               __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+}
+__device__ __forceinline__ void st_agent(cplx* p, cplx v) {
+    double* q = reinterpret_cast<double*>(p);
+    __hip_atomic_store(q, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// One block-step of a triangular solve with TRS_B-row diagonal blocks.
-//  lower=1: forward (L, or U^H when conj); rows [b0,b1) solved, rows below updated
-//  lower=0: backward (U, or L^H when conj); rows [b0,b1) solved, rows above updated
-// x_b = op(Inv_b) y_b with the precomputed diagonal-block inverse (Inv = L_bb^{-1} or
-// U_bb^{-1}; op = conjugate transpose when conj).  Every block forms x_b from Bw (not
-// written in this launch); block 0 stores it into X, the others update their Bw rows.
-template <int RB>
-__global__ void __launch_bounds__(256) trs_step_kernel(const cplx* __restrict__ Ab, int lda, int64_t sA, int n, int R,
-                                                       int b0, int b1, int lower, int conj,
-                                                       const cplx* __restrict__ Invb, int64_t sInv,
-                                                       cplx* __restrict__ Bwb, int64_t sB, cplx* __restrict__ Xb,
-                                                       int64_t sX) {
-    const int bz = blockIdx.z;
-    const cplx* A = Ab + sA * bz;
-    const cplx* Inv = Invb + sInv * bz + (int64_t)(b0 / TRS_B) * TRS_B * TRS_B;
-    cplx* Bw = Bwb + sB * bz;
-    cplx* X = Xb + sX * bz;
-    const int nbk = b1 - b0;
-    __shared__ cplx ys[TRS_B][RB];
+// element (a, q) of the dependency tile (a: index contiguous in memory), e = 0..15 per thread
+template <class C>
+__device__ __forceinline__ void trs_load_tile(const C* __restrict__ A, int lda, int conj, int b0, int nbk, int c0,
+                                              int ncb, int tid, C (&reg)[TRS_B * TRS_B / 256]) {
+#pragma unroll
+    for (int e = 0; e < TRS_B * TRS_B / 256; e++) {
+        const int idx = tid + 256 * e;
+        const int a = idx & (TRS_B - 1), q = idx / TRS_B;
+        C w = tmk<C>(0, 0);
+        if (!conj) {
+            if (a < nbk && q < ncb) w = A[(b0 + a) + (int64_t)lda * (c0 + q)];
+        } else {
+            if (q < nbk && a < ncb) w = A[(c0 + a) + (int64_t)lda * (b0 + q)];
+        }
+        reg[e] = w;
+    }
+}
+
+template <int RB, class C>
+__global__ void __launch_bounds__(256) trs_sweep_kernel(const C* __restrict__ Ab, int lda, int64_t sA, int n, int R,
+                                                        int lower, int conj, const cplx* __restrict__ Invb,
+                                                        int64_t sInv, const cplx* __restrict__ Yb, int64_t sY,
+                                                        cplx* __restrict__ Xb, int64_t sX, int* __restrict__ flags,
+                                                        int64_t sF, int epoch, int batch, int* __restrict__ err) {
+    constexpr int CPT = RB / 4;  // right-hand-side columns per thread
+    constexpr int TE = TRS_B * TRS_B / 256;
+    constexpr int XE = RB * TRS_B / 256;
+    __shared__ C tileT[TRS_B][TRS_B + 1];  // tileT[s][i] = op(T)[b0 + i][c0 + s]
     __shared__ cplx xs[TRS_B][RB];
-    const int tid = threadIdx.x;
-    for (int t = tid; t < TRS_B * RB; t += 256) {
-        int i = t % TRS_B, c = t / TRS_B;
-        ys[i][c] = (i < nbk && c < R) ? Bw[(b0 + i) + (int64_t)n * c] : cmk(0.0, 0.0);
-    }
-    __syncthreads();
-    for (int t = tid; t < TRS_B * RB; t += 256) {
-        int i = t % TRS_B, c = t / TRS_B;
-        cplx acc = cmk(0.0, 0.0);
-        for (int s2 = 0; s2 < nbk; s2++) {
-            cplx v;
-            if (!conj) v = Inv[i + TRS_B * s2];
-            else {
-                cplx w = Inv[s2 + TRS_B * i];
-                v = cmk(w.x, -w.y);
-            }
-            acc = cadd(acc, cmul(v, ys[s2][c]));
-        }
-        xs[i][c] = acc;
-    }
-    __syncthreads();
-    if (blockIdx.x == 0) {
-        for (int t = tid; t < nbk * R; t += 256) {
-            int i = t % nbk, c = t / nbk;
-            X[(b0 + i) + (int64_t)n * c] = xs[i][c];
-        }
-        return;
-    }
-    const bool fwd = lower;
-    const int row = fwd ? b1 + (blockIdx.x - 1) * 256 + tid : (blockIdx.x - 1) * 256 + tid;
-    if (fwd ? (row >= n) : (row >= b0)) return;
-    cplx acc[RB];
-#pragma unroll
-    for (int c = 0; c < RB; c++) acc[c] = (c < R) ? Bw[row + (int64_t)n * c] : cmk(0.0, 0.0);
-    for (int s2 = 0; s2 < nbk; s2++) {
-        cplx a = tri_elem(A, lda, row, b0 + s2, conj);
-#pragma unroll
-        for (int c = 0; c < RB; c++) acc[c] = csub(acc[c], cmul(a, xs[s2][c]));
-    }
-#pragma unroll
-    for (int c = 0; c < RB; c++)
-        if (c < R) Bw[row + (int64_t)n * c] = acc[c];
-}
-
-static void trs_pass(hipStream_t st, const cplx* A, int lda, int64_t sA, int n, int R, int lower, int conj,
-                     const cplx* Inv, int64_t sInv, cplx* Bw, int64_t sB, cplx* X, int64_t sX, int batch) {
+    const int tid = threadIdx.x, i = tid & (TRS_B - 1), rq = tid / TRS_B;
     const int nblk = (n + TRS_B - 1) / TRS_B;
-    for (int s = 0; s < nblk; s++) {
-        const int blk = lower ? s : nblk - 1 - s;
-        const int b0 = blk * TRS_B, b1 = min(n, b0 + TRS_B);
-        const int rest = lower ? n - b1 : b0;
-        dim3 g(1 + (rest + 255) / 256, 1, batch);
-        if (R <= 4)
-            trs_step_kernel<4><<<g, 256, 0, st>>>(A, lda, sA, n, R, b0, b1, lower, conj, Inv, sInv, Bw, sB, X, sX);
-        else if (R <= 8)
-            trs_step_kernel<8><<<g, 256, 0, st>>>(A, lda, sA, n, R, b0, b1, lower, conj, Inv, sInv, Bw, sB, X, sX);
-        else
-            trs_step_kernel<16><<<g, 256, 0, st>>>(A, lda, sA, n, R, b0, b1, lower, conj, Inv, sInv, Bw, sB, X, sX);
+    const int ntask = nblk * batch;
+    for (int task = blockIdx.x; task < ntask; task += gridDim.x) {
+        const int step = task / batch, item = task - step * batch;
+        const int blk = lower ? step : nblk - 1 - step;
+        const int b0 = blk * TRS_B, nbk = min(TRS_B, n - b0);
+        const C* A = Ab + sA * item;
+        const cplx* Y = Yb + sY * item;
+        cplx* X = Xb + sX * item;
+        int* fl = flags + sF * item;
+        cplx acc[CPT];
+#pragma unroll
+        for (int cc = 0; cc < CPT; cc++) {
+            const int col = rq * CPT + cc;
+            acc[cc] = (i < nbk && col < R) ? Y[(b0 + i) + (int64_t)n * col] : cmk(0.0, 0.0);
+        }
+        // the factor tiles do not depend on the flags: the next one is loaded while this one is used
+        C reg[TE];
+        if (step > 0) {
+            const int cb = lower ? 0 : nblk - 1;
+            trs_load_tile(A, lda, conj, b0, nbk, cb * TRS_B, min(TRS_B, n - cb * TRS_B), tid, reg);
+        }
+        for (int d = 0; d < step; d++) {
+            const int cb = lower ? d : nblk - 1 - d;  // dependency blocks in the order they are produced
+            const int c0 = cb * TRS_B, ncb = min(TRS_B, n - c0);
+            if (tid == 0) {
+                int spins = 0;
+                while (__hip_atomic_load(fl + cb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+                    // give up after TRS_SPIN_LIMIT polls, or at once when another wait already gave up
+                    if ((++spins & 63) == 1 &&
+                        (spins > TRS_SPIN_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
+                        atomicOr(err, 1);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            __syncthreads();  // flag seen; every thread is done with the previous tile
+#pragma unroll
+            for (int e = 0; e < TE; e++) {
+                const int idx = tid + 256 * e;
+                const int a = idx & (TRS_B - 1), q = idx / TRS_B;
+                C w = reg[e];
+                if (!conj) {
+                    tileT[q][a] = w;
+                } else {
+                    w.y = -w.y;
+                    tileT[a][q] = w;
+                }
+            }
+            cplx xv[XE];
+#pragma unroll
+            for (int e = 0; e < XE; e++) {
+                const int idx = tid + 256 * e;
+                const int s = idx % TRS_B, r = idx / TRS_B;
+                xv[e] = (s < ncb && r < R) ? ld_agent(X + (c0 + s) + (int64_t)n * r) : cmk(0.0, 0.0);
+            }
+#pragma unroll
+            for (int e = 0; e < XE; e++) {
+                const int idx = tid + 256 * e;
+                xs[idx % TRS_B][idx / TRS_B] = xv[e];
+            }
+            if (d + 1 < step) {
+                const int nb2 = lower ? d + 1 : nblk - 2 - d;
+                trs_load_tile(A, lda, conj, b0, nbk, nb2 * TRS_B, min(TRS_B, n - nb2 * TRS_B), tid, reg);
+            }
+            __syncthreads();
+#pragma unroll 8
+            for (int s = 0; s < TRS_B; s++) {
+                const cplx t = to_d(tileT[s][i]);
+#pragma unroll
+                for (int cc = 0; cc < CPT; cc++) acc[cc] = csub(acc[cc], cmul(t, xs[s][rq * CPT + cc]));
+            }
+        }
+        __syncthreads();  // all threads done with xs of the last dependency
+        // x_blk = op(Inv_blk) acc
+#pragma unroll
+        for (int cc = 0; cc < CPT; cc++) xs[i][rq * CPT + cc] = acc[cc];
+        __syncthreads();
+        const cplx* Inv = Invb + sInv * item + (int64_t)blk * TRS_B * TRS_B;
+        cplx x[CPT];
+#pragma unroll
+        for (int cc = 0; cc < CPT; cc++) x[cc] = cmk(0.0, 0.0);
+        if (i < nbk) {
+#pragma unroll 4
+            for (int s = 0; s < nbk; s++) {
+                cplx v;
+                if (!conj) v = Inv[i + TRS_B * s];
+                else {
+                    const cplx w = Inv[s + TRS_B * i];
+                    v = cmk(w.x, -w.y);
+                }
+#pragma unroll
+                for (int cc = 0; cc < CPT; cc++) x[cc] = cadd(x[cc], cmul(v, xs[s][rq * CPT + cc]));
+            }
+#pragma unroll
+            for (int cc = 0; cc < CPT; cc++) {
+                const int col = rq * CPT + cc;
+                if (col < R) st_agent(X + (b0 + i) + (int64_t)n * col, x[cc]);
+            }
+        }
+        // publish: the x stores above are device-coherent (agent-scope atomics); wait until every
+        // thread's stores have completed, then raise the flag
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        __syncthreads();  // also: every thread is done with tileT and xs before the next task
+        if (tid == 0) __hip_atomic_store(fl + blk, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 
-// B (n x R, leading dimension n) is overwritten with the solution.  work: 2 n R complex
-// plus 2 * ceil(n/64) * 64 * 64 complex (diagonal-block inverses) per matrix, stride sW.
+// persistent grid: at most half of the workgroups the device can hold at once
+template <int RB, class C>
+static int trs_grid(int ntask) {
+    int dev = 0, cus = 0, occ = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, trs_sweep_kernel<RB, C>, 256, 0) != hipSuccess || occ <= 0)
+        occ = 1;
+    return std::max(1, std::min(ntask, std::max(1, cus * occ / 2)));
+}
+
+template <class C>
+static void trs_pass(hipStream_t st, const C* A, int lda, int64_t sA, int n, int R, int lower, int conj,
+                     const cplx* Inv, int64_t sInv, const cplx* Y, int64_t sY, cplx* X, int64_t sX, int* flags,
+                     int64_t sF, int epoch, int batch, int* err) {
+    const int ntask = ((n + TRS_B - 1) / TRS_B) * batch;
+    if (R <= 4)
+        trs_sweep_kernel<4, C><<<trs_grid<4, C>(ntask), 256, 0, st>>>(A, lda, sA, n, R, lower, conj, Inv, sInv, Y, sY,
+                                                                      X, sX, flags, sF, epoch, batch, err);
+    else if (R <= 8)
+        trs_sweep_kernel<8, C><<<trs_grid<8, C>(ntask), 256, 0, st>>>(A, lda, sA, n, R, lower, conj, Inv, sInv, Y, sY,
+                                                                      X, sX, flags, sF, epoch, batch, err);
+    else
+        trs_sweep_kernel<16, C><<<trs_grid<16, C>(ntask), 256, 0, st>>>(A, lda, sA, n, R, lower, conj, Inv, sInv, Y,
+                                                                        sY, X, sX, flags, sF, epoch, batch, err);
+}
+
+// B (n x R, leading dimension n) is overwritten with the solution.  work per matrix (stride sW):
+// the wait flags of the triangular sweeps plus one error word (ceil(n/64) + 1 ints, rounded up to
+// whole complex elements; first, so that their place does not depend on R), W1, W2 (n R complex
+// each) and the diagonal-block inverses of L and U (2 ceil(n/64) 64^2 complex).  The flags must
+// be zero when the work memory is first used; *epoch (host counter, one per work buffer) is
+// advanced by every sweep.
+static int64_t flag_words(int n) { return ((n + TRS_B - 1) / TRS_B + 1 + 3) / 4; }
+
 int64_t getrs_work(int n, int R) {
     const int64_t nblk = (n + TRS_B - 1) / TRS_B;
-    return 2 * (int64_t)n * R + 2 * nblk * TRS_B * TRS_B;
+    return flag_words(n) + 2 * (int64_t)n * R + 2 * nblk * TRS_B * TRS_B;
+}
+
+template <class C>
+static int getrs_t(hipStream_t st, int trans, int n, int R, const C* LU, int lda, int64_t sA, const int* ipiv,
+                   int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
+                   int batch, bool prepared, int* epoch) {
+    if (R > MAXR) return -1;
+    if (n <= 0 || R <= 0) return 0;
+    const int nblk = (n + TRS_B - 1) / TRS_B;
+    int* flags = reinterpret_cast<int*>(work);
+    int* err = flags + nblk;  // error word of item 0
+    cplx* W1 = work + flag_words(n);
+    cplx* W2 = W1 + (int64_t)n * R;
+    cplx* Linv = W2 + (int64_t)n * R;
+    cplx* Uinv = Linv + (int64_t)nblk * TRS_B * TRS_B;
+    const int64_t sF = sW * (int64_t)(sizeof(cplx) / sizeof(int));
+    if (!prepared) {
+        invert_diag_kernel<C><<<dim3(nblk, 2, batch), TRS_B, 0, st>>>(LU, lda, sA, n, Linv, Uinv, sW);
+        ipiv_to_perm_kernel<<<dim3(1, 1, batch), 256, 0, st>>>(ipiv, ipiv_stride, n, perm, perm_stride);
+    }
+    dim3 gp((n + 255) / 256, 1, batch);
+    if (trans == 0) {
+        permute_rows_kernel<<<gp, 256, 0, st>>>(n, R, perm, perm_stride, B, sB, W1, sW, 0);
+        trs_pass(st, LU, lda, sA, n, R, 1, 0, Linv, sW, W1, sW, W2, sW, flags, sF, ++*epoch, batch, err);  // L
+        trs_pass(st, LU, lda, sA, n, R, 0, 0, Uinv, sW, W2, sW, B, sB, flags, sF, ++*epoch, batch, err);   // U
+    } else {
+        for (int b = 0; b < batch; b++)
+            hipMemcpyAsync(W1 + sW * b, B + sB * b, sizeof(cplx) * (size_t)n * R, hipMemcpyDeviceToDevice, st);
+        trs_pass(st, LU, lda, sA, n, R, 1, 1, Uinv, sW, W1, sW, W2, sW, flags, sF, ++*epoch, batch, err);  // U^H
+        trs_pass(st, LU, lda, sA, n, R, 0, 1, Linv, sW, W2, sW, W1, sW, flags, sF, ++*epoch, batch, err);  // L^H
+        permute_rows_kernel<<<gp, 256, 0, st>>>(n, R, perm, perm_stride, W1, sW, B, sB, 1);
+    }
+    return 0;
 }
 
 int getrs_batched(hipStream_t st, int trans, int n, int R, const cplx* LU, int lda, int64_t sA, const int* ipiv,
                   int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
-                  int batch) {
-    if (R > MAXR) return -1;
-    if (n <= 0 || R <= 0) return 0;
-    const int nblk = (n + TRS_B - 1) / TRS_B;
-    cplx* W1 = work;
-    cplx* W2 = work + (int64_t)n * R;
-    cplx* Linv = work + 2 * (int64_t)n * R;
-    cplx* Uinv = Linv + (int64_t)nblk * TRS_B * TRS_B;
-    invert_diag_kernel<<<dim3(nblk, 2, batch), TRS_B, 0, st>>>(LU, lda, sA, n, Linv, Uinv, sW);
-    ipiv_to_perm_kernel<<<dim3(1, 1, batch), 256, 0, st>>>(ipiv, ipiv_stride, n, perm, perm_stride);
-    dim3 gp((n + 255) / 256, 1, batch);
-    if (trans == 0) {
-        permute_rows_kernel<<<gp, 256, 0, st>>>(n, R, perm, perm_stride, B, sB, W1, sW, 0);
-        trs_pass(st, LU, lda, sA, n, R, 1, 0, Linv, sW, W1, sW, W2, sW, batch);  // L: W1 -> W2
-        trs_pass(st, LU, lda, sA, n, R, 0, 0, Uinv, sW, W2, sW, B, sB, batch);   // U: W2 -> B
-    } else {
-        for (int b = 0; b < batch; b++)
-            hipMemcpyAsync(W1 + sW * b, B + sB * b, sizeof(cplx) * (size_t)n * R, hipMemcpyDeviceToDevice, st);
-        trs_pass(st, LU, lda, sA, n, R, 1, 1, Uinv, sW, W1, sW, W2, sW, batch);  // U^H (lower)
-        trs_pass(st, LU, lda, sA, n, R, 0, 1, Linv, sW, W2, sW, W1, sW, batch);  // L^H (unit upper)
-        permute_rows_kernel<<<gp, 256, 0, st>>>(n, R, perm, perm_stride, W1, sW, B, sB, 1);
+                  int batch, int* epoch, bool prepared) {
+    return getrs_t(st, trans, n, R, LU, lda, sA, ipiv, ipiv_stride, perm, perm_stride, B, sB, work, sW, batch, prepared,
+                   epoch);
+}
+
+
+int getrs_batched(hipStream_t st, int trans, int n, int R, const cplxf* LU, int lda, int64_t sA, const int* ipiv,
+                  int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
+                  int batch, int* epoch, bool prepared) {
+    return getrs_t(st, trans, n, R, LU, lda, sA, ipiv, ipiv_stride, perm, perm_stride, B, sB, work, sW, batch, prepared,
+                   epoch);
+}
+
+int* getrs_error_word(int n, cplx* work) { return reinterpret_cast<int*>(work) + (n + TRS_B - 1) / TRS_B; }
+
+
+// ---------------------------------------------------------------------------
+// mixed-precision helpers: FP64 -> FP32 copy of a batch of matrices, residual
+// R = B - T with per-item squared norms, X += D
+// ---------------------------------------------------------------------------
+__global__ void to_f32_kernel(int64_t count, const cplx* __restrict__ src, cplxf* __restrict__ dst) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count; t += (int64_t)gridDim.x * blockDim.x)
+        dst[t] = make_float2((float)src[t].x, (float)src[t].y);
+}
+
+void convert_to_f32(hipStream_t st, int64_t count, const cplx* src, cplxf* dst) {
+    const int64_t blocks = std::min<int64_t>((count + 255) / 256, 8192);
+    to_f32_kernel<<<(int)blocks, 256, 0, st>>>(count, src, dst);
+}
+
+// per item z: T <- Bv - T over n*R elements; norms[2z] += |T|^2, norms[2z+1] += |Bv|^2
+__global__ void residual_kernel(int64_t cnt, const cplx* __restrict__ Bvb, int64_t sB, cplx* __restrict__ Tb,
+                                int64_t sT, double* __restrict__ norms) {
+    const int z = blockIdx.z;
+    const cplx* Bv = Bvb + sB * z;
+    cplx* T = Tb + sT * z;
+    double rr = 0.0, bb = 0.0;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < cnt; t += (int64_t)gridDim.x * blockDim.x) {
+        const cplx b = Bv[t], y = T[t];
+        const cplx r = cmk(b.x - y.x, b.y - y.y);
+        T[t] = r;
+        rr += r.x * r.x + r.y * r.y;
+        bb += b.x * b.x + b.y * b.y;
     }
-    return 0;
+    for (int off = 32; off > 0; off >>= 1) {
+        rr += __shfl_xor(rr, off);
+        bb += __shfl_xor(bb, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(norms + 2 * z, rr);
+        atomicAdd(norms + 2 * z + 1, bb);
+    }
+}
+
+void launch_residual(hipStream_t st, int64_t cnt, const cplx* Bv, int64_t sB, cplx* T, int64_t sT, double* norms,
+                     int batch) {
+    const int64_t blocks = std::min<int64_t>((cnt + 255) / 256, 64);
+    residual_kernel<<<dim3((int)blocks, 1, batch), 256, 0, st>>>(cnt, Bv, sB, T, sT, norms);
+}
+
+__global__ void axpy_kernel(int64_t cnt, const cplx* __restrict__ Db, int64_t sD, cplx* __restrict__ Xb, int64_t sX) {
+    const int z = blockIdx.z;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < cnt; t += (int64_t)gridDim.x * blockDim.x) {
+        const cplx d = Db[sD * z + t];
+        cplx& x = Xb[sX * z + t];
+        x = cmk(x.x + d.x, x.y + d.y);
+    }
+}
+
+void launch_add(hipStream_t st, int64_t cnt, const cplx* D, int64_t sD, cplx* X, int64_t sX, int batch) {
+    const int64_t blocks = std::min<int64_t>((cnt + 255) / 256, 256);
+    axpy_kernel<<<dim3((int)blocks, 1, batch), 256, 0, st>>>(cnt, D, sD, X, sX);
 }
 
 }  // namespace mhx

@@ -26,12 +26,15 @@ GF_SURROGATE = 2
 MODE_PER_OMEGA = 0
 MODE_PER_PROBLEM = 1
 MODE_BATCHED = 2
+LU_AUTO = 0
+LU_FP64 = 1
+LU_MIXED = 2
 
 EXPORTED = ["mhx_create", "mhx_destroy", "mhx_last_error", "mhx_set_constants", "mhx_assemble", "mhx_solve",
             "mhx_lu_solve", "mhx_hydro_coeffs", "mhx_body_create", "mhx_body_destroy", "mhx_body_coeffs",
             "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work", "mhx_set_batch", "mhx_surrogate_train",
             "mhx_surrogate_save", "mhx_surrogate_load", "mhx_surrogate_params", "mhx_surrogate_eval",
-            "mhx_surrogate_loss_grad"]
+            "mhx_surrogate_loss_grad", "mhx_set_lu_mode", "mhx_solver_stats"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int64)
@@ -92,6 +95,8 @@ def lib():
         L.mhx_reset_timing.restype = None
         L.mhx_exact_work.argtypes = [vp, _ip, _dp]
         L.mhx_set_batch.argtypes = [vp, C.c_int64, C.c_double]
+        L.mhx_set_lu_mode.argtypes = [vp, C.c_int]
+        L.mhx_solver_stats.argtypes = [vp, _ip, _dp]
         L.mhx_surrogate_train.argtypes = [vp, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_uint64, _dp]
         L.mhx_surrogate_save.argtypes = [vp, C.c_char_p]
         L.mhx_surrogate_load.argtypes = [vp, C.c_char_p]
@@ -158,6 +163,17 @@ class Context:
 
     def reset_timing(self):
         lib().mhx_reset_timing(self.h)
+
+    def set_lu_mode(self, mode):
+        """LU_AUTO (mixed precision for the surrogate wave term), LU_FP64 or LU_MIXED."""
+        self.check(lib().mhx_set_lu_mode(self.h, int(mode)))
+
+    def solver_stats(self):
+        """(systems solved in mixed precision, refinement steps, FP64 fallbacks, max residual)."""
+        c = np.zeros(3, dtype=np.int64)
+        r = C.c_double()
+        self.check(lib().mhx_solver_stats(self.h, c.ctypes.data_as(_ip), C.byref(r)))
+        return int(c[0]), int(c[1]), int(c[2]), r.value
 
     # ---- learned surrogate of the exact wave term ----
     def surrogate_train(self, ngrid=2048, steps=8000, batch=4096, lr=2e-3, seed=20261203):
