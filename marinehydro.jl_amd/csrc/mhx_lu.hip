@@ -305,7 +305,7 @@ __global__ void __launch_bounds__(256) trsm_panel_kernel(MatBatch<C> mb, int n, 
 }
 
 // ---------------------------------------------------------------------------
-// C[m x nn] (-)= A[m x kk] * B[kk x nn], complex, column-major, on MFMA
+// C[m x nn] (+)= (-)A[m x kk] * B[kk x nn], complex, column-major, on MFMA
 // (v_mfma_f64_16x16x4f64 for cplx, v_mfma_f32_16x16x4f32 for cplxf).
 // 256 threads (4 waves in 2x2), 64x64 tile, K chunks of 16 staged in LDS as
 // separate real / imaginary planes.  Lane layout of both instructions:
@@ -331,12 +331,16 @@ template <> struct Mfma16<float> {
     static __device__ __forceinline__ int row(int v, int lk) { return 4 * lk + v; }
 };
 
+// Complex product per k-step: FP64 uses the four real products (Re = ar br - ai bi, Im = ar bi +
+// ai br), rounding as the reference's zgemm; FP32 (the mixed-precision factor) uses three,
+// T1 = ar br, T2 = ai bi, T3 = (ar + ai)(br + bi): Re = T1 - T2, Im = T3 - T1 - T2.
 template <bool SUB, bool ACCUM, class C>
 __global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, const C* __restrict__ Ab, int lda,
                                                          int64_t sA, const C* __restrict__ Bb, int ldb, int64_t sB,
                                                          C* __restrict__ Cb, int ldc, int64_t sC) {
     typedef typename Cx<C>::R R;
     typedef Mfma16<R> M;
+    constexpr bool THREE = sizeof(R) == 4;
     const int bz = blockIdx.z;
     const C* A = Ab + sA * bz;
     const C* B = Bb + sB * bz;
@@ -345,26 +349,14 @@ __global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     const int row0 = blockIdx.x * 64, col0 = blockIdx.y * 64;
-    typename M::V accr[2][2], acci[2][2];
+    typename M::V t1[2][2], t2[2][2], t3[2][2];
     const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
     for (int bi = 0; bi < 2; bi++)
 #pragma unroll
-        for (int bj = 0; bj < 2; bj++) {
+        for (int bj = 0; bj < 2; bj++)
 #pragma unroll
-            for (int v = 0; v < 4; v++) {
-                R cr = 0, ci = 0;
-                if (ACCUM) {
-                    int r = row0 + wm + bi * 16 + M::row(v, lk), c = col0 + wn + bj * 16 + li;
-                    if (r < m && c < nn) {
-                        C t = Cm[r + (int64_t)ldc * c];
-                        cr = t.x; ci = t.y;
-                    }
-                }
-                accr[bi][bj][v] = cr;
-                acci[bi][bj][v] = ci;
-            }
-        }
+            for (int v = 0; v < 4; v++) t1[bi][bj][v] = t2[bi][bj][v] = t3[bi][bj][v] = 0;
     for (int k0 = 0; k0 < kk; k0 += 16) {
         // A tile: 64 rows x 16 k; thread loads 4 elements (rows contiguous)
 #pragma unroll
@@ -401,10 +393,16 @@ __global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, 
             for (int bi = 0; bi < 2; bi++)
 #pragma unroll
                 for (int bj = 0; bj < 2; bj++) {
-                    accr[bi][bj] = M::run(ar[bi], br[bj], accr[bi][bj]);
-                    accr[bi][bj] = M::run(-ai[bi], bi_[bj], accr[bi][bj]);
-                    acci[bi][bj] = M::run(ar[bi], bi_[bj], acci[bi][bj]);
-                    acci[bi][bj] = M::run(ai[bi], br[bj], acci[bi][bj]);
+                    if (THREE) {
+                        t1[bi][bj] = M::run(ar[bi], br[bj], t1[bi][bj]);
+                        t2[bi][bj] = M::run(ai[bi], bi_[bj], t2[bi][bj]);
+                        t3[bi][bj] = M::run(ar[bi] + ai[bi], br[bj] + bi_[bj], t3[bi][bj]);
+                    } else {  // t1: real part, t3: imaginary part
+                        t1[bi][bj] = M::run(ar[bi], br[bj], t1[bi][bj]);
+                        t1[bi][bj] = M::run(-ai[bi], bi_[bj], t1[bi][bj]);
+                        t3[bi][bj] = M::run(ar[bi], bi_[bj], t3[bi][bj]);
+                        t3[bi][bj] = M::run(ai[bi], br[bj], t3[bi][bj]);
+                    }
                 }
         }
         __syncthreads();
@@ -416,7 +414,22 @@ __global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, 
 #pragma unroll
             for (int v = 0; v < 4; v++) {
                 int r = row0 + wm + bi * 16 + M::row(v, lk), c = col0 + wn + bj * 16 + li;
-                if (r < m && c < nn) Cm[r + (int64_t)ldc * c] = tmk<C>(accr[bi][bj][v], acci[bi][bj][v]);
+                if (r < m && c < nn) {
+                    R re, im;
+                    if (THREE) {
+                        re = t1[bi][bj][v] - t2[bi][bj][v];
+                        im = t3[bi][bj][v] - t1[bi][bj][v] - t2[bi][bj][v];
+                    } else {
+                        re = t1[bi][bj][v];
+                        im = t3[bi][bj][v];
+                    }
+                    if (ACCUM) {
+                        const C old = Cm[r + (int64_t)ldc * c];
+                        re += old.x;
+                        im += old.y;
+                    }
+                    Cm[r + (int64_t)ldc * c] = tmk<C>(re, im);
+                }
             }
 }
 
