@@ -281,27 +281,51 @@ __global__ void __launch_bounds__(256) apply_perm_kernel(MatBatch<C> mb, int n, 
     if (act) A[pm[1 + e] + (int64_t)mb.lda * j] = v;
 }
 
-// U12 = L11^{-1} A12 for the outer panel rows [K, KE), columns [KE, n); 4 columns per block
+// U12 = L11^{-1} A12 for the outer panel rows [K, KE), columns [KE, n): forward substitution with
+// the unit-lower block L11 staged once in LDS; TRSM_COLS columns per workgroup, thread (row i,
+// column quarter q) owns TRSM_COLS / 4 columns of row i.
+#define TRSM_COLS 16
 template <class C>
 __global__ void __launch_bounds__(256) trsm_panel_kernel(MatBatch<C> mb, int n, int K, int KE) {
+    constexpr int CPQ = TRSM_COLS / 4;
     const int b = blockIdx.z;
     C* A = mb.A + mb.stride * b;
     const int lda = mb.lda;
     const int nb = KE - K;
-    __shared__ C X[NB][4];
-    const int i = threadIdx.x % NB, c = threadIdx.x / NB;
-    const int j = KE + blockIdx.x * 4 + c;
-    const bool act = i < nb && j < n;
-    if (act) X[i][c] = A[(K + i) + (int64_t)lda * j];
+    __shared__ C L[NB][NB + 1];         // L[s][i] = L11[i][s]
+    __shared__ C X[NB][TRSM_COLS + 1];
+    const int tid = threadIdx.x, i = tid & (NB - 1), q = tid / NB;
+    const int j0 = KE + blockIdx.x * TRSM_COLS;
+    for (int e = tid; e < NB * NB; e += 256) {
+        const int r = e & (NB - 1), s = e / NB;
+        L[s][r] = (r < nb && s < nb) ? A[(K + r) + (int64_t)lda * (K + s)] : tmk<C>(0, 0);
+    }
+    for (int e = tid; e < NB * TRSM_COLS; e += 256) {
+        const int r = e & (NB - 1), c = e / NB;
+        X[r][c] = (r < nb && j0 + c < n) ? A[(K + r) + (int64_t)lda * (j0 + c)] : tmk<C>(0, 0);
+    }
     __syncthreads();
+    C x[CPQ];
+#pragma unroll
+    for (int cc = 0; cc < CPQ; cc++) x[cc] = X[i][q * CPQ + cc];
     for (int r = 0; r < nb - 1; r++) {
-        if (act && i > r) {
-            C l = A[(K + i) + (int64_t)lda * (K + r)];
-            X[i][c] = tsub(X[i][c], tmul(l, X[r][c]));
+        // row r is final; rows below subtract L[i][r] x_r
+        if (i > r) {
+            const C l = L[r][i];
+#pragma unroll
+            for (int cc = 0; cc < CPQ; cc++) x[cc] = tsub(x[cc], tmul(l, X[r][q * CPQ + cc]));
+        }
+        if (i == r + 1) {
+#pragma unroll
+            for (int cc = 0; cc < CPQ; cc++) X[i][q * CPQ + cc] = x[cc];
         }
         __syncthreads();
     }
-    if (act) A[(K + i) + (int64_t)lda * j] = X[i][c];
+#pragma unroll
+    for (int cc = 0; cc < CPQ; cc++) {
+        const int j = j0 + q * CPQ + cc;
+        if (i < nb && j < n) A[(K + i) + (int64_t)lda * j] = x[cc];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -486,7 +510,7 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
             apply_perm_kernel<C><<<dim3((nout + 1) / 2, 1, batch), 256, 0, st>>>(mb, n, K, KE, work_perm, pstride);
         }
         if (KE < n) {
-            trsm_panel_kernel<C><<<dim3((n - KE + 3) / 4, 1, batch), 256, 0, st>>>(mb, n, K, KE);
+            trsm_panel_kernel<C><<<dim3((n - KE + TRSM_COLS - 1) / TRSM_COLS, 1, batch), 256, 0, st>>>(mb, n, K, KE);
             gemm_t<C>(st, true, true, n - KE, n - KE, KE - K, A + KE + (int64_t)lda * K, lda, strideA,
                       A + K + (int64_t)lda * KE, lda, strideA, A + KE + (int64_t)lda * KE, lda, strideA, batch);
         }
