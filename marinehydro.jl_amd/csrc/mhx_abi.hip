@@ -65,6 +65,7 @@ struct mhx_body {
     cplxf* Df = nullptr;
     cplx *Rr = nullptr, *Tr = nullptr;
     double* norms = nullptr;
+    bool df_ready = false;  // Df already holds the FP32 copy of D (written by the fused surrogate assembly)
     int trs_epoch = 0;  // sweep counter of the triangular-solve flags in work (zeroed with work)
     int64_t nn() const { return (int64_t)n * n; }
     int64_t nR() const { return (int64_t)n * cap_R; }
@@ -198,8 +199,13 @@ int ensure_work(mhx_ctx* ctx, mhx_body* b, int R, int nb, bool table, int batch 
     return MHX_OK;
 }
 
+bool use_mixed(const mhx_ctx* ctx, int gf) {
+    return ctx->lu_mode == MHX_LU_MIXED || (ctx->lu_mode == MHX_LU_AUTO && gf == MHX_GF_SURROGATE);
+}
+
 // wave assembly of `batch` items (wavenumbers already in b->d_ks) into b->S, b->D
 int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int include_identity) {
+    b->df_ready = false;
     if (gf == MHX_GF_WU) {
         Phase ph(ctx, "wave");
         mhx::launch_combine_wu(ctx->stream, b->n, b->geom, b->SR, b->DR[direct], b->d_ks, batch, direct,
@@ -216,15 +222,27 @@ int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int 
             if ((rc = dalloc(ctx, &b->fb_list, (size_t)cap))) return rc;
             b->fb_cap = cap;
         }
+        // fused: the pair kernel writes S, D (and the FP32 copy of D for the mixed-precision solver)
+        mhx::SurrOut out;
+        out.SR = b->SR;
+        out.DR = b->DR[direct];
+        out.S = b->S;
+        out.D = b->D;
+        out.direct = direct;
+        out.include_identity = include_identity;
+        out.sMat = b->nn();
+        if (use_mixed(ctx, gf)) {
+            int rc = dalloc(ctx, &b->Df, (size_t)b->nn() * b->cap_batch);
+            if (rc) return rc;
+            out.Df = b->Df;
+        }
         HCHK(hipMemsetAsync(b->status, 0, 4 * sizeof(unsigned long long), ctx->stream));
         {
             Phase ph(ctx, "wave_table");
             mhx::launch_surr_table(ctx->stream, b->n, b->geom, b->d_ks, batch, ctx->surr_W, ctx->surr_bias,
-                                   ctx->surr_B, b->WG, b->WR, b->nn(), b->fb_list, b->fb_count, b->fb_cap, b->status);
+                                   ctx->surr_B, out, b->fb_list, b->fb_count, b->fb_cap, b->status);
         }
-        Phase ph(ctx, "wave");
-        mhx::launch_combine_table(ctx->stream, b->n, b->geom, b->SR, b->DR[direct], b->d_ks, batch, direct,
-                                  include_identity, b->WG, b->WR, b->S, b->D, b->nn());
+        b->df_ready = out.Df != nullptr;
     } else {
         HCHK(hipMemsetAsync(b->status, 0, 4 * sizeof(unsigned long long), ctx->stream));
         {
@@ -251,10 +269,6 @@ int check_status(mhx_ctx* ctx, mhx_body* b, int gf) {
 }
 
 // factor the `batch` matrices b->D in place and solve for the R columns of b->B; phi into b->Y
-bool use_mixed(const mhx_ctx* ctx, int gf) {
-    return ctx->lu_mode == MHX_LU_MIXED || (ctx->lu_mode == MHX_LU_AUTO && gf == MHX_GF_SURROGATE);
-}
-
 int check_info(mhx_ctx* ctx, mhx_body* b, int batch, bool* singular) {
     std::vector<int> info(batch, 0);
     int herr = 0;
@@ -323,7 +337,8 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
     HCHK(hipMemsetAsync(b->info, 0, sizeof(int) * batch, st));
     {
         Phase ph(ctx, "lu");
-        mhx::convert_to_f32(st, b->nn() * batch, b->D, b->Df);
+        if (!b->df_ready) mhx::convert_to_f32(st, b->nn() * batch, b->D, b->Df);
+        b->df_ready = false;
         rc = mhx::getrf_batched(st, n, b->Df, n, b->nn(), batch, b->ipiv, n, b->wperm, b->info);
         if (rc) return fail(ctx, MHX_EDIM, "LU: order above 4096 is not supported");
     }
@@ -511,7 +526,7 @@ int mhx_assemble(mhx_ctx* ctx, const mhx_mesh* mesh, double k, int gf, int direc
     int rc = body_upload(ctx, mesh, 0, nullptr, &b);
     if (rc) return rc;
     const double w = std::sqrt(k * ctx->g);
-    if (!(rc = ensure_rankine(ctx, b, direct)) && !(rc = ensure_work(ctx, b, 1, 0, gf != MHX_GF_WU)) &&
+    if (!(rc = ensure_rankine(ctx, b, direct)) && !(rc = ensure_work(ctx, b, 1, 0, gf == MHX_GF_DELHOMMEAU)) &&
         !(rc = set_freqs(ctx, b, &w, 1))) {
         // the wavenumber itself, not k recomputed from omega
         if (hipMemcpyAsync(b->d_ks, &k, sizeof(double), hipMemcpyHostToDevice, ctx->stream) != hipSuccess)
@@ -649,7 +664,7 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
         if (!(omegas[w] > 0) || !std::isfinite(omegas[w])) return fail(ctx, MHX_EINVAL, "omega must be positive and finite");
     hipSetDevice(ctx->device);
     direct = direct ? 1 : 0;
-    const bool table = gf != MHX_GF_WU;
+    const bool table = gf == MHX_GF_DELHOMMEAU;  // (G, dG/dr) table of the exact term
     int chunk = 1;
     if (mode == MHX_MODE_BATCHED && nw > 1) {
         double per = (double)n * n * sizeof(cplx) * (table ? 4 : 2) + 20.0 * n * 16 * sizeof(cplx);

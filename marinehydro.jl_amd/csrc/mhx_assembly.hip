@@ -229,25 +229,12 @@ __global__ void __launch_bounds__(256) combine_table_kernel(int n, const PanelGe
         const int64_t idx = i + (int64_t)n * j;
         double hh, vv;
         reduced_hv(x0, x1, x2, g.c[0], g.c[1], g.c[2], k, hh, vv);
-        const cplx G = WG[idx], Gh = WR[idx];
-        const double inv = 2.0 / hypot(hh, vv);
-        const cplx Gv = cmk(G.x + inv, G.y);
-        double dx1 = direct ? g.c[0] - x0 : x0 - g.c[0];
-        double dx2 = direct ? g.c[1] - x1 : x1 - g.c[1];
-        double dr1 = 0.0, dr2 = 0.0;
-        if (fabs(hh) > 1e-6) {
-            dr1 = k * k / hh * dx1;
-            dr2 = k * k / hh * dx2;
-        }
-        const double n0 = direct ? g.nrm[0] : ni0, n1 = direct ? g.nrm[1] : ni1, n2 = direct ? g.nrm[2] : ni2;
-        const double a2 = g.area;
-        cplx g0 = cscale(k, cscale(dr1, Gh)), g1 = cscale(k, cscale(dr2, Gh)), g2 = cscale(k, cscale(k, Gv));
-        cplx s = cscale(a2, cscale(k, G));
-        cplx d = cadd(cadd(cscale(n0, cscale(a2, g0)), cscale(n1, cscale(a2, g1))), cscale(n2, cscale(a2, g2)));
-        double sr = SR[idx], dr = DR[idx];
-        if (include_identity && i == j) dr += 0.5;
-        S[idx] = cmk(sr + kScale * s.x, kScale * s.y);
-        D[idx] = cmk(dr + kScale * d.x, kScale * d.y);
+        const double xr[3] = {x0, x1, x2}, nr[3] = {ni0, ni1, ni2};
+        cplx Sv, Dv;
+        combine_entry(k, hh, vv, xr, nr, g, direct, include_identity && i == j, WG[idx], WR[idx], SR[idx], DR[idx], Sv,
+                      Dv);
+        S[idx] = Sv;
+        D[idx] = Dv;
     }
 }
 

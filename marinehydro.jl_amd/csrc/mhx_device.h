@@ -484,3 +484,30 @@ __device__ __forceinline__ bool hquad_pair(double r, double z, double rtol, doub
 #define MHX_FLOPS_PER_EVAL 41.0
 #define MHX_FLOPS_PER_SERIES 8.0
 #define MHX_FLOPS_PER_CF 45.0
+
+// One entry (row panel with centre xr / normal nr, column panel g) of S and D (or K) from the
+// wave term G, dG/dr and the Rankine parts sr, dr: with_reduced_coordinates_derivative
+// (abstract_greens_function.jl:140-185) and the one-point integrals
+// (exact_Guevel_Delhommeau.jl:61-75), scaled by -1/(4 pi); `diag` adds the identity term 1/2.
+__device__ __forceinline__ void combine_entry(double k, double hh, double vv, const double* xr, const double* nr,
+                                              const PanelGeom& g, int direct, bool diag, cplx G, cplx Gh, double sr,
+                                              double dr, cplx& S, cplx& D) {
+    const double kScaleD = -1.0 / (4 * MHX_PI);
+    const double inv = 2.0 / hypot(hh, vv);
+    const cplx Gv = cmk(G.x + inv, G.y);
+    double dx1 = direct ? g.c[0] - xr[0] : xr[0] - g.c[0];
+    double dx2 = direct ? g.c[1] - xr[1] : xr[1] - g.c[1];
+    double dr1 = 0.0, dr2 = 0.0;
+    if (fabs(hh) > 1e-6) {
+        dr1 = k * k / hh * dx1;
+        dr2 = k * k / hh * dx2;
+    }
+    const double n0 = direct ? g.nrm[0] : nr[0], n1 = direct ? g.nrm[1] : nr[1], n2 = direct ? g.nrm[2] : nr[2];
+    const double a2 = g.area;
+    cplx g0 = cscale(k, cscale(dr1, Gh)), g1 = cscale(k, cscale(dr2, Gh)), g2 = cscale(k, cscale(k, Gv));
+    cplx s = cscale(a2, cscale(k, G));
+    cplx d = cadd(cadd(cscale(n0, cscale(a2, g0)), cscale(n1, cscale(a2, g1))), cscale(n2, cscale(a2, g2)));
+    if (diag) dr += 0.5;
+    S = cmk(sr + kScaleD * s.x, kScaleD * s.y);
+    D = cmk(dr + kScaleD * d.x, kScaleD * d.y);
+}

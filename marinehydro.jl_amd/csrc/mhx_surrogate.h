@@ -33,8 +33,19 @@ void surr_adam(hipStream_t st, int64_t n, float* p, const float* g, float* m1, f
 void launch_batch_index(hipStream_t st, int M, int64_t nsamples, uint64_t seed, int* idx);
 void launch_surr_pack(hipStream_t st, const float* params, unsigned short* Wb, float* bias);
 int64_t surr_fallback_capacity(int n, int batch);
+// Destination of the surrogate pair table: with S null, the symmetric (G, dG/dr) table WG, WR
+// (combined by launch_combine_table); otherwise the finished S and D (and, when Df is set, the
+// FP32 copy of D) from the Rankine parts SR, DR.  Strides sMat per item.
+struct SurrOut {
+    cplx *WG = nullptr, *WR = nullptr;
+    const double *SR = nullptr, *DR = nullptr;
+    cplx *S = nullptr, *D = nullptr;
+    cplxf* Df = nullptr;
+    int direct = 1, include_identity = 1;
+    int64_t sMat = 0;
+};
 void launch_surr_table(hipStream_t st, int n, const PanelGeom* geom, const double* ks, int batch,
-                       const unsigned short* Wb, const float* bias, const float* Bf, cplx* WG, cplx* WR, int64_t sMat,
+                       const unsigned short* Wb, const float* bias, const float* Bf, const SurrOut& out,
                        unsigned long long* fb_list, unsigned long long* fb_count, int64_t fb_cap,
                        unsigned long long* status);
 void launch_surr_eval(hipStream_t st, int M, const double* rv, const unsigned short* Wb, const float* bias,

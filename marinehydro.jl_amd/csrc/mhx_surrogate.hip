@@ -508,15 +508,48 @@ __device__ __forceinline__ void tri_tile(int64_t t, int& bi, int& bj) {
 
 #define SURR_WAVES 8
 
+// store the pair's wave term: the (G, dG/dr) table, or (fused) the S and D entries (i, j) and
+// (j, i) with their Rankine parts, and the FP32 copy of D for the mixed-precision solver
+__device__ __forceinline__ void surr_store_pair(const SurrOut& o, int n, int item, int i, int j, double k, double hh,
+                                                double vv, const PanelGeom* __restrict__ geom, cplx G, cplx Gr) {
+    const int64_t ij = i + (int64_t)n * j, ji = j + (int64_t)n * i;
+    if (!o.S) {
+        cplx* WG = o.WG + o.sMat * item;
+        cplx* WR = o.WR + o.sMat * item;
+        WG[ij] = G;
+        WR[ij] = Gr;
+        if (i != j) {
+            WG[ji] = G;
+            WR[ji] = Gr;
+        }
+        return;
+    }
+    const PanelGeom& gi = geom[i];
+    const PanelGeom& gj = geom[j];
+    cplx* S = o.S + o.sMat * item;
+    cplx* D = o.D + o.sMat * item;
+    cplxf* Df = o.Df ? o.Df + o.sMat * item : nullptr;
+    cplx Sv, Dv;
+    combine_entry(k, hh, vv, gi.c, gi.nrm, gj, o.direct, o.include_identity && i == j, G, Gr, o.SR[ij], o.DR[ij], Sv, Dv);
+    S[ij] = Sv;
+    D[ij] = Dv;
+    if (Df) Df[ij] = make_float2((float)Dv.x, (float)Dv.y);
+    if (i != j) {
+        combine_entry(k, hh, vv, gj.c, gj.nrm, gi, o.direct, false, G, Gr, o.SR[ji], o.DR[ji], Sv, Dv);
+        S[ji] = Sv;
+        D[ji] = Dv;
+        if (Df) Df[ji] = make_float2((float)Dv.x, (float)Dv.y);
+    }
+}
+
 // Persistent pair-table kernel over all items of the batch.  One work unit = 8 x 8 block of
 // panel pairs (i in 8 bi.., j in 8 bj.., bi <= bj; pairs with i > j are skipped); each wave takes
 // units in a strided order.  Pairs inside the trained box get Wu + network; others are
 // appended to the fallback list as (item << 40 | i << 20 | j).
 __global__ void __launch_bounds__(64 * SURR_WAVES) surr_table_kernel(
     int n, int batch, const PanelGeom* __restrict__ geom, const double* __restrict__ ks,
-    const unsigned short* __restrict__ Wb, const float* __restrict__ bias, const float* __restrict__ Bf,
-    cplx* __restrict__ WGb, cplx* __restrict__ WRb, int64_t sMat, unsigned long long* __restrict__ fb_list,
-    unsigned long long* __restrict__ fb_count, int64_t fb_cap) {
+    const unsigned short* __restrict__ Wb, const float* __restrict__ bias, const float* __restrict__ Bf, SurrOut dst,
+    unsigned long long* __restrict__ fb_list, unsigned long long* __restrict__ fb_count, int64_t fb_cap) {
     __shared__ SurrLds L;
     surr_load_lds(L, Wb, bias, Bf);
     const int lane = threadIdx.x & 63, h = lane >> 5, c = lane & 31;
@@ -584,22 +617,14 @@ __global__ void __launch_bounds__(64 * SURR_WAVES) surr_table_kernel(
         const double ii = is * exp(-fmin(-vv_me, SURR_IM_VC));
         const cplx G = cmk(W.x + is * o[0], W.y + ii * o[1]);
         const cplx Gr = cmk(Wh.x + is * o[2], Wh.y + ii * o[3]);
-        cplx* WG = WGb + sMat * item;
-        cplx* WR = WRb + sMat * item;
-        WG[i + (int64_t)n * j] = G;
-        WR[i + (int64_t)n * j] = Gr;
-        if (i != j) {
-            WG[j + (int64_t)n * i] = G;
-            WR[j + (int64_t)n * i] = Gr;
-        }
+        surr_store_pair(dst, n, item, i, j, k, hh_me, vv_me, geom, G, Gr);
     }
 }
 
 // exact term for the listed fallback pairs (grid-stride: the count is only known on the device)
 __global__ void surr_fallback_kernel(int n, const PanelGeom* __restrict__ geom, const double* __restrict__ ks,
                                      const unsigned long long* __restrict__ fb_list,
-                                     const unsigned long long* __restrict__ fb_count, int64_t fb_cap,
-                                     cplx* __restrict__ WGb, cplx* __restrict__ WRb, int64_t sMat,
+                                     const unsigned long long* __restrict__ fb_count, int64_t fb_cap, SurrOut out,
                                      unsigned long long* __restrict__ status) {
     const int64_t cnt = (int64_t)min(*fb_count, (unsigned long long)fb_cap);
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < cnt; t += (int64_t)gridDim.x * blockDim.x) {
@@ -615,14 +640,7 @@ __global__ void surr_fallback_kernel(int n, const PanelGeom* __restrict__ geom, 
         if (!hquad_pair(hh, vv, 1e-8, 1e-8, G, Gr, wk)) atomicOr(status, 1ull);
         G = cscale(4.0, G);
         Gr = cscale(4.0, Gr);
-        cplx* WG = WGb + sMat * item;
-        cplx* WR = WRb + sMat * item;
-        WG[i + (int64_t)n * j] = G;
-        WR[i + (int64_t)n * j] = Gr;
-        if (i != j) {
-            WG[j + (int64_t)n * i] = G;
-            WR[j + (int64_t)n * i] = Gr;
-        }
+        surr_store_pair(out, n, item, i, j, k, hh, vv, geom, G, Gr);
     }
 }
 
@@ -636,16 +654,16 @@ static int device_cus() {
 int64_t surr_fallback_capacity(int n, int batch) { return (int64_t)n * (n + 1) / 2 * batch; }
 
 void launch_surr_table(hipStream_t st, int n, const PanelGeom* geom, const double* ks, int batch,
-                       const unsigned short* Wb, const float* bias, const float* Bf, cplx* WG, cplx* WR, int64_t sMat,
+                       const unsigned short* Wb, const float* bias, const float* Bf, const SurrOut& out,
                        unsigned long long* fb_list, unsigned long long* fb_count, int64_t fb_cap,
                        unsigned long long* status) {
     hipMemsetAsync(fb_count, 0, sizeof(unsigned long long), st);
     const int nb = (n + 7) / 8;
     const int64_t units = (int64_t)nb * (nb + 1) / 2 * batch;
     const int64_t blocks = std::min<int64_t>(device_cus(), (units + SURR_WAVES - 1) / SURR_WAVES);
-    surr_table_kernel<<<(int)blocks, 64 * SURR_WAVES, 0, st>>>(n, batch, geom, ks, Wb, bias, Bf, WG, WR, sMat,
-                                                               fb_list, fb_count, fb_cap);
-    surr_fallback_kernel<<<2048, 256, 0, st>>>(n, geom, ks, fb_list, fb_count, fb_cap, WG, WR, sMat, status);
+    surr_table_kernel<<<(int)blocks, 64 * SURR_WAVES, 0, st>>>(n, batch, geom, ks, Wb, bias, Bf, out, fb_list, fb_count,
+                                                               fb_cap);
+    surr_fallback_kernel<<<2048, 256, 0, st>>>(n, geom, ks, fb_list, fb_count, fb_cap, out, status);
 }
 
 // the network (MFMA path) at arbitrary (r, v): out[6 m + o], the raw outputs (real parts:
