@@ -56,9 +56,10 @@ def workload():
     return mesh, omegas
 
 
-def cpu_baseline(mesh, omegas, rows, threads):
-    """Oracle (C restatement of the reference) on the host cores: a sampled-row exact
-    assembly extrapolated to the full matrix, plus LAPACK LU and solves on a full matrix."""
+def cpu_baseline(mesh, omegas, nfreq, threads):
+    """The oracle (C restatement of the reference) on the host cores: nfreq complete omega-solves
+    (full exact assembly of S and D with OpenMP, LAPACK LU and solves through numpy's OpenBLAS,
+    pressure integration is negligible)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     import scipy.linalg as sl
@@ -66,30 +67,24 @@ def cpu_baseline(mesh, omegas, rows, threads):
     O.set_threads(threads)
     om = O.Mesh.from_mhx(mesh)
     n = mesh.nfaces
-    w = float(np.median(omegas))
-    k = w * w / 9.81
-    sel = np.linspace(0, n, 9).astype(int)
-    t0 = time.perf_counter()
-    done = 0
-    for a in sel[:-1]:
-        r = max(1, rows // 8)
-        O.assemble_rows(om, k, (int(a), int(a) + r), gf=O.GF_DELHOMMEAU)
-        done += r
-    t_rows = time.perf_counter() - t0
-    t_asm = t_rows * n / done
+    ws = omegas[np.linspace(0, len(omegas) - 1, nfreq + 2).round().astype(int)[1:-1]]
     rng = np.random.default_rng(0)
-    A = 0.5 * np.eye(n) + (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / n
-    B = rng.standard_normal((n, 7)) + 1j * rng.standard_normal((n, 7))
-    t0 = time.perf_counter()
-    lu = sl.lu_factor(A, check_finite=False)
-    sl.lu_solve(lu, B, check_finite=False)
-    A @ B
-    t_lu = time.perf_counter() - t0
-    per = t_asm + t_lu
+    rhs = rng.standard_normal((n, 7)) + 1j * rng.standard_normal((n, 7))
+    t_asm = t_lu = 0.0
+    for w in ws:
+        t0 = time.perf_counter()
+        S, D = O.assemble(om, w * w / 9.81, gf=O.GF_DELHOMMEAU, direct=True)
+        t1 = time.perf_counter()
+        lu = sl.lu_factor(D, check_finite=False)
+        sl.lu_solve(lu, S @ rhs, check_finite=False)
+        t2 = time.perf_counter()
+        t_asm += t1 - t0
+        t_lu += t2 - t1
+    per = (t_asm + t_lu) / len(ws)
     return {"value": 1.0 / per, "unit": "omega-solves/s", "cores": threads, "kind": "port",
-            "sample": f"exact assembly of {done} of {n} rows at omega={w:.3f} (oracle, {threads} threads, "
-                      f"{t_rows:.1f} s, extrapolated to {t_asm:.1f} s) + LAPACK zgetrf/zgetrs n={n} "
-                      f"({t_lu:.2f} s, numpy's OpenBLAS)"}
+            "sample": f"{len(ws)} complete omega-solves at omega = {np.round(ws, 3).tolist()} on {n} panels: "
+                      f"exact assembly (oracle, {threads} OpenMP threads, {t_asm:.2f} s) + LAPACK zgetrf/zgetrs "
+                      f"and S.B (numpy's OpenBLAS, {t_lu:.2f} s)"}
 
 
 def surrogate_flops_per_pair():
@@ -97,45 +92,78 @@ def surrogate_flops_per_pair():
     return 2.0 * (64 * 128 + 3 * 128 * 128 + 128 * 6)
 
 
+def profiled_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated on a copy
+    kernel) for the default workload, read from the committed summary profiles/<name>."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def lu_update_flops(n, nb=64):
+    """Algorithmic flops of the trailing updates of one blocked LU (complex multiply-add = 8 flops)."""
+    f = 0.0
+    for K in range(0, n, nb):
+        KE = min(n, K + nb)
+        if KE < n:
+            f += 8.0 * (n - KE) ** 2 * (KE - K)
+    return f
+
+
 def measure(ctx, run, ws, n, gf):
     """One timed call over the frequencies ws; returns rate, phase table and roofline of the
-    dominant kernel (HIP events on the library's stream)."""
+    dominant kernel (per-launch HIP events on the library's stream)."""
     ctx.reset_timing()
     t0 = time.perf_counter()
     A, B, Fd, Fk = run(ws)
     elapsed = time.perf_counter() - t0
     assert np.all(np.isfinite(A)) and np.all(np.isfinite(Fd))
-    phases = {p: ctx.phase_time(p) for p in ("wave_table", "wave", "lu", "solve", "forces", "rankine")}
+    phases = {p: ctx.phase_time(p) for p in ("wave_table", "wave", "lu", "lu_update", "solve", "forces", "rankine")}
     counts, flops = ctx.exact_work()
     tw, cw = phases["wave_table"]
+    tu, cu = phases["lu_update"]
     tl, cl = phases["lu"]
     nw = len(ws)
-    if gf == "Surrogate" and tw >= tl:
+    # the surrogate path factors in FP32 (mixed-precision solver), the others in FP64
+    fp32 = gf == "Surrogate"
+    lu_peak = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
+    fu = lu_update_flops(n) * nw
+    traffic = None
+    if fp32 and n == 4096 and nw == 64 and cu == 63:   # the profiled configuration
+        traffic = profiled_traffic("lu_update_4096x64_traffic.md")
+    rl_update = {"bound": "mfma", "kernel": "zgemm_mfma_kernel<%s> (LU trailing update)" % ("cplxf" if fp32 else "cplx"),
+                 "achieved": fu / (tu * 1e-3) / 1e12 if tu else None, "peak": lu_peak, "unit": "TFLOP/s",
+                 "traffic": traffic, "avg_launch_ms": tu / cu if cu else None, "launches": cu,
+                 "algorithmic_flops_per_launch": fu / cu if cu else None}
+    if rl_update["achieved"]:
+        rl_update["frac"] = rl_update["achieved"] / lu_peak
+    if gf == "Surrogate":
         pairs = n * (n + 1) / 2 * nw
         ach = pairs * surrogate_flops_per_pair() / (tw * 1e-3) / 1e12
-        roof = {"bound": "mfma", "kernel": "surr_table_kernel (+ exact fallback)", "achieved": ach,
-                "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / BF16_PEAK_TFLOPS, "traffic": None,
-                "avg_launch_ms": tw / cw, "launches": cw, "frequencies_per_launch": nw / cw}
-    elif gf == "ExactGuevelDelhommeau" and tw >= tl:
+        rl_table = {"bound": "mfma", "kernel": "surr_table_kernel (+ exact fallback)", "achieved": ach,
+                    "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / BF16_PEAK_TFLOPS, "traffic": None,
+                    "avg_launch_ms": tw / cw, "launches": cw, "frequencies_per_launch": nw / cw}
+    elif gf == "ExactGuevelDelhommeau":
         ach = flops / (tw * 1e-3) / 1e12
-        roof = {"bound": "valu", "kernel": "delhommeau_table_kernel", "achieved": ach,
-                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
-                "avg_launch_ms": tw / cw, "launches": cw, "frequencies_per_launch": nw / cw,
-                "work_per_frequency": {"integrand_evals": int(counts[0] // nw), "series_steps": int(counts[1] // nw),
-                                       "cf_steps": int(counts[2] // nw)}}
+        rl_table = {"bound": "valu", "kernel": "delhommeau_table_kernel", "achieved": ach,
+                    "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                    "avg_launch_ms": tw / cw, "launches": cw, "frequencies_per_launch": nw / cw,
+                    "work_per_frequency": {"integrand_evals": int(counts[0] // nw), "series_steps": int(counts[1] // nw),
+                                           "cf_steps": int(counts[2] // nw)}}
     else:
-        # the surrogate path factors in FP32 (mixed-precision solver), the others in FP64
-        fp32 = gf == "Surrogate"
-        peak = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
-        fl = 8.0 / 3.0 * n ** 3 * nw
-        ach = fl / (tl * 1e-3) / 1e12
-        roof = {"bound": "mfma", "kernel": "getrf_batched (zgemm_mfma_kernel trailing updates, %s)" %
-                ("FP32 factor of the mixed-precision solver" if fp32 else "FP64"), "achieved": ach,
-                "peak": peak, "unit": "TFLOP/s", "frac": ach / peak, "traffic": None,
-                "avg_launch_ms": tl / cl, "launches": cl}
+        rl_table = None
+    # dominant kernel by time
+    if rl_table is not None and tw >= tu:
+        roof, other = rl_table, rl_update
+    else:
+        roof, other = rl_update, rl_table
     lu_tflops = 8.0 / 3.0 * n ** 3 * nw / (tl * 1e-3) / 1e12 if tl else None
     per_w = {k: v[0] / nw for k, v in phases.items()}
-    return nw / elapsed, elapsed, per_w, roof, lu_tflops, (A, B, Fd, Fk)
+    return nw / elapsed, elapsed, per_w, roof, lu_tflops, (A, B, Fd, Fk), other
 
 
 def main():
@@ -145,7 +173,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=64, help="untimed omega-solves (the first W timed frequencies)")
     ap.add_argument("--gf", default="Surrogate", choices=["Surrogate", "ExactGuevelDelhommeau", "Wu"])
     ap.add_argument("--schedule", default="batched", choices=["batched", "per_omega"])
-    ap.add_argument("--cpu-rows", type=int, default=32)
+    ap.add_argument("--cpu-freqs", type=int, default=2, help="complete omega-solves timed on the host for cpu_baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-omega comparison run")
     args = ap.parse_args()
@@ -186,7 +214,7 @@ def main():
     warm = steps_w[:max(args.warmup, 1)]
     run_with(mode)(warm)   # body upload, Rankine part, buffers: outside the timed region
     log(f"bench: timing {args.steps} omega-solves ({args.schedule})")
-    value, elapsed, phases, roof, lu_tflops, res = measure(ctx, run_with(mode), steps_w, n, args.gf)
+    value, elapsed, phases, roof, lu_tflops, res, roof2 = measure(ctx, run_with(mode), steps_w, n, args.gf)
     solved, steps, fallbacks, resid = ctx.solver_stats()
     solver = {"mixed_precision_systems": solved, "refinement_steps": steps, "fp64_fallbacks": fallbacks,
               "max_relative_residual": resid}
@@ -199,7 +227,7 @@ def main():
         exact_form = mhx.DirectBEM("ExactGuevelDelhommeau")
         run_plain = lambda ws: mhx.api._body_coeffs(body, names, ws, [0.0], exact_form.gf, True,
                                                     _lib.MODE_PER_OMEGA, 1025.0, 9.81)
-        pv, pel, pph, proof, plu, pres = measure(ctx, run_plain, steps_w[sub], n, "ExactGuevelDelhommeau")
+        pv, pel, pph, proof, plu, pres, _ = measure(ctx, run_plain, steps_w[sub], n, "ExactGuevelDelhommeau")
         errs = {}
         pairs = (("added_mass", res[0][:, :, sub], pres[0], lambda x, q: x[:, :, q]),
                  ("radiation_damping", res[1][:, :, sub], pres[1], lambda x, q: x[:, :, q]),
@@ -213,7 +241,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("bench: cpu baseline")
-        cpu = cpu_baseline(mesh, omegas, args.cpu_rows, min(16, os.cpu_count() or 1))
+        cpu = cpu_baseline(mesh, omegas, args.cpu_freqs, min(16, os.cpu_count() or 1))
 
     out = {
         "metric": METRIC, "value": value, "unit": "omega-solves/s", "n_gpus": 1, "steps": args.steps,
@@ -230,6 +258,7 @@ def main():
         "plain_gpu_per_omega": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "roofline": roof,
+        "roofline_next_kernel": roof2,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -124,7 +124,9 @@ int mhx_set_lu_mode(mhx_ctx* ctx, int mode);
 int mhx_solver_stats(mhx_ctx* ctx, int64_t counts[3], double* max_residual);
 
 /* per-phase timing (HIP events on the context's stream, enabled by mhx_options.timing):
- * phase: "rankine", "wave_table" (exact wave term), "wave" (S, D formation), "lu", "solve", "forces";
+ * phase: "rankine", "wave_table" (wave term: exact table, or the fused surrogate pair kernel),
+ * "wave" (S, D formation from the table or the Wu fit), "lu" (factorization), "lu_update" (each
+ * trailing-update GEMM launch inside "lu", timed separately), "solve", "forces";
  * returns total ms and launch count since reset */
 int mhx_phase_time(mhx_ctx* ctx, const char* phase, double* total_ms, int64_t* count);
 void mhx_reset_timing(mhx_ctx* ctx);

@@ -120,6 +120,12 @@ struct Phase {
     }
 };
 
+// per-launch events of the trailing-update GEMMs -> phase "lu_update" (one count per launch)
+void take_launches(mhx_ctx* c, mhx::LaunchTimer& t) {
+    for (auto& e : t.ev) c->pending.emplace_back("lu_update", e);
+    t.ev.clear();
+}
+
 void flush_timing(mhx_ctx* c) {
     for (auto& p : c->pending) {
         float ms = 0.f;
@@ -295,7 +301,10 @@ int solve_fp64(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
     HCHK(hipMemsetAsync(b->info, 0, sizeof(int) * batch, ctx->stream));
     {
         Phase ph(ctx, "lu");
-        int rc = mhx::getrf_batched(ctx->stream, n, b->D, n, b->nn(), batch, b->ipiv, n, b->wperm, b->info);
+        mhx::LaunchTimer lt;
+        int rc = mhx::getrf_batched(ctx->stream, n, b->D, n, b->nn(), batch, b->ipiv, n, b->wperm, b->info,
+                                    ctx->timing ? &lt : nullptr);
+        take_launches(ctx, lt);
         if (rc) return fail(ctx, MHX_EDIM, "LU: order above 4096 is not supported");
     }
     HCHK(hipGetLastError());
@@ -339,7 +348,10 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
         Phase ph(ctx, "lu");
         if (!b->df_ready) mhx::convert_to_f32(st, b->nn() * batch, b->D, b->Df);
         b->df_ready = false;
-        rc = mhx::getrf_batched(st, n, b->Df, n, b->nn(), batch, b->ipiv, n, b->wperm, b->info);
+        mhx::LaunchTimer lt;
+        rc = mhx::getrf_batched(st, n, b->Df, n, b->nn(), batch, b->ipiv, n, b->wperm, b->info,
+                                ctx->timing ? &lt : nullptr);
+        take_launches(ctx, lt);
         if (rc) return fail(ctx, MHX_EDIM, "LU: order above 4096 is not supported");
     }
     HCHK(hipGetLastError());

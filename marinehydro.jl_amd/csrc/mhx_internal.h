@@ -2,6 +2,9 @@
 #pragma once
 #include "mhx_device.h"
 
+#include <utility>
+#include <vector>
+
 namespace mhx {
 
 // ---- assembly (mhx_assembly.hip)
@@ -39,10 +42,14 @@ void launch_forces(hipStream_t st, int n, const PanelGeom* geom, const double* d
                    const double* betas, int nb, cplx* F, cplx* FK);
 
 // ---- dense LU and solves (mhx_lu.hip)
+// optional per-launch timing of the trailing-update GEMMs: event pairs recorded on the stream
+struct LaunchTimer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+};
 void zgemm(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cplx* A, int lda, int64_t sA, const cplx* B,
            int ldb, int64_t sB, cplx* C, int ldc, int64_t sC, int batch);
 int getrf_batched(hipStream_t st, int n, cplx* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
-                  int* work_perm, int* info);
+                  int* work_perm, int* info, LaunchTimer* update_timer = nullptr);
 // complex elements of getrs work per matrix (sW must be at least this; zero it when allocated).
 // getrs_batched keeps the diagonal-block inverses and the row permutation in work / perm: a later
 // call on the same factors with prepared = true reuses them.  *epoch: the work buffer's sweep
@@ -54,7 +61,7 @@ int getrs_batched(hipStream_t st, int trans, int n, int R, const cplx* LU, int l
                   int batch, int* epoch, bool prepared = false);
 // FP32 factorization (mixed-precision solver); the solves read FP32 factors and compute in FP64
 int getrf_batched(hipStream_t st, int n, cplxf* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
-                  int* work_perm, int* info);
+                  int* work_perm, int* info, LaunchTimer* update_timer = nullptr);
 int getrs_batched(hipStream_t st, int trans, int n, int R, const cplxf* LU, int lda, int64_t sA, const int* ipiv,
                   int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
                   int batch, int* epoch, bool prepared = false);

@@ -481,7 +481,7 @@ void zgemm(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cp
 // ---------------------------------------------------------------------------
 template <class C>
 static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
-                   int* work_perm, int* info) {
+                   int* work_perm, int* info, LaunchTimer* timer) {
     if (n <= 0) return 0;
     const int rows_per = (n + LEAF_THREADS - 1) / LEAF_THREADS;
     if (rows_per > 4) return -1;  // orders above 4096 need a larger register tile
@@ -511,21 +511,31 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
         }
         if (KE < n) {
             trsm_panel_kernel<C><<<dim3((n - KE + TRSM_COLS - 1) / TRSM_COLS, 1, batch), 256, 0, st>>>(mb, n, K, KE);
+            hipEvent_t e0 = nullptr, e1 = nullptr;
+            if (timer) {
+                hipEventCreate(&e0);
+                hipEventCreate(&e1);
+                hipEventRecord(e0, st);
+            }
             gemm_t<C>(st, true, true, n - KE, n - KE, KE - K, A + KE + (int64_t)lda * K, lda, strideA,
                       A + K + (int64_t)lda * KE, lda, strideA, A + KE + (int64_t)lda * KE, lda, strideA, batch);
+            if (timer) {
+                hipEventRecord(e1, st);
+                timer->ev.emplace_back(e0, e1);
+            }
         }
     }
     return 0;
 }
 
 int getrf_batched(hipStream_t st, int n, cplx* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
-                  int* work_perm, int* info) {
-    return getrf_t(st, n, A, lda, strideA, batch, ipiv, ipiv_stride, work_perm, info);
+                  int* work_perm, int* info, LaunchTimer* update_timer) {
+    return getrf_t(st, n, A, lda, strideA, batch, ipiv, ipiv_stride, work_perm, info, update_timer);
 }
 
 int getrf_batched(hipStream_t st, int n, cplxf* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
-                  int* work_perm, int* info) {
-    return getrf_t(st, n, A, lda, strideA, batch, ipiv, ipiv_stride, work_perm, info);
+                  int* work_perm, int* info, LaunchTimer* update_timer) {
+    return getrf_t(st, n, A, lda, strideA, batch, ipiv, ipiv_stride, work_perm, info, update_timer);
 }
 
 // ---------------------------------------------------------------------------
