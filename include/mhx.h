@@ -137,6 +137,28 @@ void mhx_reset_timing(mhx_ctx* ctx);
  * used for roofline reporting) */
 int mhx_exact_work(mhx_ctx* ctx, int64_t counts[3], double* flops);
 
+/* Tangent (forward-mode) derivative of hydrodynamic_coefficients (problems_and_results.jl:464-495)
+ * along a direction of the inputs: the mesh fields (vertices nv x 3, centers / normals nf x 3,
+ * areas nf; NULL = zero), the dof fields ddofs (same layout as dofs; NULL = zero) and the
+ * frequencies domegas (nw; NULL = zero).  Returns the coefficients (as mhx_hydro_coeffs, one LU per
+ * frequency in FP64) and their directional derivatives dA, dB (ndof, ndof, nw), dFd, dFk
+ * (ndof, nw, nb complex).  It is the derivative the reference computes with ForwardDiff/Enzyme
+ * through its rules for the panel integrals and the solve (ad_rules.jl:79-414), with the mesh
+ * tangent seeded as fd_mesh_rules! does (ad_rules_geometry.jl:174-230); gf: MHX_GF_WU or
+ * MHX_GF_DELHOMMEAU.  The radius derivative of a body scaled about the origin is the direction
+ * (vertices / r, centers / r, normals 0, 2 areas / r). */
+typedef struct {
+    const double* vertices;
+    const double* centers;
+    const double* normals;
+    const double* areas;
+} mhx_mesh_tangent;
+int mhx_hydro_coeffs_jvp(mhx_ctx* ctx, const mhx_mesh* mesh, const mhx_mesh_tangent* dmesh, int64_t ndof,
+                         const double* dofs, const double* ddofs, int64_t nw, const double* omegas,
+                         const double* domegas, int64_t nb, const double* betas, int gf, int direct, double* A_out,
+                         double* B_out, double* Fd_out, double* Fk_out, double* dA_out, double* dB_out,
+                         double* dFd_out, double* dFk_out);
+
 /* Learned surrogate of the exact Guevel-Delhommeau wave term (no counterpart in the reference; it
  * replaces the exact term of src/green_functions/exact_Guevel_Delhommeau.jl:15-59 inside assembly).
  * The network maps (r, v) = (k R_h, k |z + zeta|) through 64 Fourier features and 4 x 128 SiLU

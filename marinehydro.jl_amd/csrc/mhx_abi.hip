@@ -12,6 +12,7 @@
 
 #include "../../include/mhx.h"
 #include "mhx_surrogate.h"
+#include "mhx_tangent.h"
 
 
 
@@ -731,6 +732,168 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
         }
     }
     flush_timing(ctx);
+    return rc;
+}
+
+// Directional derivative of the coefficients (forward mode): per frequency, the primal omega-solve
+// (FP64 LU) and its tangent from the tangent assembly, the tangent right-hand sides and the
+// same LU factors.
+int mhx_hydro_coeffs_jvp(mhx_ctx* ctx, const mhx_mesh* mesh, const mhx_mesh_tangent* dmesh, int64_t ndof,
+                         const double* dofs, const double* ddofs, int64_t nw, const double* omegas,
+                         const double* domegas, int64_t nb, const double* betas, int gf, int direct, double* A_out,
+                         double* B_out, double* Fd_out, double* Fk_out, double* dA_out, double* dB_out,
+                         double* dFd_out, double* dFk_out) {
+    if (!ctx) return MHX_EINVAL;
+    if (gf != MHX_GF_WU && gf != MHX_GF_DELHOMMEAU)
+        return fail(ctx, MHX_EINVAL, "coeffs_jvp: the tangent is available for GFWu and ExactGuevelDelhommeau");
+    if (nw < 0 || nb < 0 || (nw && !omegas) || (nb && !betas)) return fail(ctx, MHX_EINVAL, "coeffs_jvp: bad frequency/heading arrays");
+    if ((ndof && (!A_out || !B_out || !dA_out || !dB_out)) || (nb && ndof && (!Fd_out || !Fk_out || !dFd_out || !dFk_out)))
+        return fail(ctx, MHX_EINVAL, "coeffs_jvp: null output");
+    for (int64_t w = 0; w < nw; w++)
+        if (!(omegas[w] > 0) || !std::isfinite(omegas[w])) return fail(ctx, MHX_EINVAL, "omega must be positive and finite");
+    const int64_t R64 = ndof + nb;
+    if (R64 <= 0) return fail(ctx, MHX_EDIM, "coeffs_jvp: no radiating dof and no heading");
+    if (R64 > 16) return fail(ctx, MHX_EDIM, "coeffs_jvp: at most 16 right-hand sides (dofs + headings)");
+    if (mesh && mesh->nf > 4096) return fail(ctx, MHX_EDIM, "coeffs_jvp: more than 4096 panels is not supported");
+    hipSetDevice(ctx->device);
+    mhx_body* b = nullptr;
+    int rc = body_upload(ctx, mesh, ndof, dofs, &b);
+    if (rc) return rc;
+    const int n = b->n, R = (int)R64;
+    direct = direct ? 1 : 0;
+    hipStream_t st = ctx->stream;
+    double *dv = nullptr, *ddv = nullptr, *dc = nullptr, *ddc = nullptr, *dn = nullptr, *ddn = nullptr, *da = nullptr,
+           *dda = nullptr, *dr = nullptr, *d_ddofs = nullptr, *d_betas = nullptr;
+    int64_t* df = nullptr;
+    mhx::PanelGeomD* gd = nullptr;
+    cplx *dS = nullptr, *dD = nullptr, *dBv = nullptr, *t1 = nullptr, *t2 = nullptr, *t3 = nullptr, *dphi = nullptr,
+         *dF = nullptr, *dFK = nullptr;
+    auto cleanup = [&]() {
+        dfree(dv); dfree(ddv); dfree(dc); dfree(ddc); dfree(dn); dfree(ddn); dfree(da); dfree(dda); dfree(dr);
+        dfree(d_ddofs); dfree(d_betas); dfree(df); dfree(gd); dfree(dS); dfree(dD); dfree(dBv); dfree(t1); dfree(t2);
+        dfree(t3); dfree(dphi); dfree(dF); dfree(dFK);
+        mhx_body_destroy(b);
+    };
+    const size_t nn = (size_t)n * n, nR = (size_t)n * R, nv = mesh->nv;
+    const bool tab = gf == MHX_GF_DELHOMMEAU;
+    if ((rc = ensure_work(ctx, b, R, (int)nb, tab, 1)) || (rc = ensure_rankine(ctx, b, direct)) ||
+        (rc = dalloc(ctx, &dv, 3 * nv)) || (rc = dalloc(ctx, &ddv, 3 * nv)) || (rc = dalloc(ctx, &df, 4 * (size_t)n)) ||
+        (rc = dalloc(ctx, &dc, 3 * (size_t)n)) || (rc = dalloc(ctx, &ddc, 3 * (size_t)n)) || (rc = dalloc(ctx, &dn, 3 * (size_t)n)) ||
+        (rc = dalloc(ctx, &ddn, 3 * (size_t)n)) || (rc = dalloc(ctx, &da, n)) || (rc = dalloc(ctx, &dda, n)) ||
+        (rc = dalloc(ctx, &dr, n)) || (rc = dalloc(ctx, &d_ddofs, 3 * (size_t)n * (ndof ? ndof : 1))) ||
+        (rc = dalloc(ctx, &d_betas, (size_t)(nb ? nb : 1))) || (rc = dalloc(ctx, &gd, n)) || (rc = dalloc(ctx, &dS, nn)) ||
+        (rc = dalloc(ctx, &dD, nn)) || (rc = dalloc(ctx, &dBv, nR)) || (rc = dalloc(ctx, &t1, nR)) ||
+        (rc = dalloc(ctx, &t2, nR)) || (rc = dalloc(ctx, &t3, nR)) || (rc = dalloc(ctx, &dphi, nR)) ||
+        (rc = dalloc(ctx, &dF, (size_t)(ndof ? ndof : 1) * R)) || (rc = dalloc(ctx, &dFK, (size_t)(ndof ? ndof : 1) * (nb ? nb : 1)))) {
+        cleanup();
+        return rc;
+    }
+    // mesh and tangent fields on the device (a null tangent array is zero)
+    auto up = [&](double* dst, const double* src, size_t cnt) {
+        if (src) hipMemcpyAsync(dst, src, sizeof(double) * cnt, hipMemcpyHostToDevice, st);
+        else hipMemsetAsync(dst, 0, sizeof(double) * cnt, st);
+    };
+    up(dv, mesh->vertices, 3 * nv);
+    up(dc, mesh->centers, 3 * (size_t)n);
+    up(dn, mesh->normals, 3 * (size_t)n);
+    up(da, mesh->areas, n);
+    up(dr, mesh->radii, n);
+    up(ddv, dmesh ? dmesh->vertices : nullptr, 3 * nv);
+    up(ddc, dmesh ? dmesh->centers : nullptr, 3 * (size_t)n);
+    up(ddn, dmesh ? dmesh->normals : nullptr, 3 * (size_t)n);
+    up(dda, dmesh ? dmesh->areas : nullptr, n);
+    if (ndof) up(d_ddofs, ddofs, 3 * (size_t)n * ndof);
+    if (nb) {
+        hipMemcpyAsync(d_betas, betas, sizeof(double) * nb, hipMemcpyHostToDevice, st);
+        hipMemcpyAsync(b->betas, betas, sizeof(double) * nb, hipMemcpyHostToDevice, st);
+    }
+    hipMemcpyAsync(df, mesh->faces, sizeof(int64_t) * 4 * n, hipMemcpyHostToDevice, st);
+    // device errors inside the sweep end it with rc set, so that the buffers are released below
+    auto ok = [&](hipError_t e, const char* what) {
+        if (e == hipSuccess) return true;
+        rc = fail(ctx, MHX_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+        return false;
+    };
+    mhx::launch_tangent_geometry(st, n, dv, ddv, df, dc, ddc, dn, ddn, da, dda, dr, gd);
+    if (!ok(hipGetLastError(), "tangent geometry")) {
+        cleanup();
+        return rc;
+    }
+    std::vector<cplx> F((size_t)ndof * R), FK((size_t)ndof * (nb ? nb : 1)), hdF((size_t)ndof * R),
+        hdFK((size_t)ndof * (nb ? nb : 1));
+    const double g = ctx->g;
+    for (int64_t iw = 0; iw < nw && !rc; iw++) {
+        const double w = omegas[iw], dw = domegas ? domegas[iw] : 0.0;
+        const double k = w * w / g, dk = 2.0 * w * dw / g;  // compute_wavenumber (src/waves.jl:133-134)
+        if ((rc = set_freqs(ctx, b, omegas + iw, 1))) break;
+        mhx::launch_bc(st, n, b->geom, b->dofs, ndof, b->betas, (int)nb, b->d_ws, b->d_ks, 1, g, b->B, b->nR());
+        mhx::launch_tangent_bc(st, n, gd, b->dofs, ndof ? d_ddofs : nullptr, ndof, d_betas, (int)nb, w, dw, k, dk, g, dBv);
+        if ((rc = assemble_into(ctx, b, 1, gf, direct, 1)) || (rc = check_status(ctx, b, gf))) break;
+        mhx::launch_tangent_pairs(st, n, gd, k, dk, direct, tab, b->WG, b->WR, dS, dD);
+        if ((rc = solve_fp64(ctx, b, R, direct, 1))) {
+            if (rc == MHX_ESINGULAR) fail(ctx, rc, ctx->err);
+            break;
+        }
+        cplx* W2 = b->work + 16 * b->nR();
+        const int64_t cnt = (int64_t)n * R;
+        {
+            Phase ph(ctx, "tangent");
+            if (direct) {
+                // dphi = D^{-1} (dS B + S dB - dD phi)
+                mhx::launch_matmul_nr(st, n, R, dS, 0, b->B, 0, t1, 0, b->work, 0, 1);
+                mhx::launch_matmul_nr(st, n, R, b->S, 0, dBv, 0, t2, 0, b->work, 0, 1);
+                mhx::launch_matmul_nr(st, n, R, dD, 0, b->Y, 0, t3, 0, b->work, 0, 1);
+                mhx::launch_axpby(st, cnt, t1, 1.0, t2, dphi);
+                mhx::launch_axpby(st, cnt, dphi, -1.0, t3, dphi);
+                mhx::getrs_batched(st, 0, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, dphi, 0, W2, 0, 1, &b->trs_epoch,
+                                   true);
+            } else {
+                // sigma in B; dsigma = D^{-1} (dB - dD sigma); dphi = dS sigma + S dsigma
+                mhx::launch_matmul_nr(st, n, R, dD, 0, b->B, 0, t3, 0, b->work, 0, 1);
+                mhx::launch_axpby(st, cnt, dBv, -1.0, t3, t2);
+                mhx::getrs_batched(st, 0, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, t2, 0, W2, 0, 1, &b->trs_epoch,
+                                   true);
+                mhx::launch_matmul_nr(st, n, R, dS, 0, b->B, 0, t1, 0, b->work, 0, 1);
+                mhx::launch_matmul_nr(st, n, R, b->S, 0, t2, 0, t3, 0, b->work, 0, 1);
+                mhx::launch_axpby(st, cnt, t1, 1.0, t3, dphi);
+            }
+        }
+        if (ndof > 0) {
+            mhx::launch_forces(st, n, b->geom, b->dofs, ndof, b->Y, b->nR(), R, ctx->rho, b->d_ws, b->d_ks, 1, g, b->betas,
+                               (int)nb, b->F, b->FK);
+            mhx::launch_tangent_forces(st, n, gd, b->dofs, d_ddofs, ndof, b->Y, dphi, R, ctx->rho, w, dw, k, dk, g, d_betas,
+                                       (int)nb, dF, dFK);
+        }
+        if (!ok(hipGetLastError(), "tangent launch") ||
+            !ok(hipMemcpyAsync(F.data(), b->F, sizeof(cplx) * ndof * R, hipMemcpyDeviceToHost, st), "copy F") ||
+            !ok(hipMemcpyAsync(hdF.data(), dF, sizeof(cplx) * ndof * R, hipMemcpyDeviceToHost, st), "copy dF") ||
+            (nb && !ok(hipMemcpyAsync(FK.data(), b->FK, sizeof(cplx) * ndof * nb, hipMemcpyDeviceToHost, st), "copy FK")) ||
+            (nb && !ok(hipMemcpyAsync(hdFK.data(), dFK, sizeof(cplx) * ndof * nb, hipMemcpyDeviceToHost, st), "copy dFK")) ||
+            !ok(hipStreamSynchronize(st), "sync"))
+            break;
+        pack(F, FK, ndof, R, (int)nb, iw, nw, w, A_out, B_out, Fd_out, Fk_out);
+        // tangents: A = Re F / w^2, B = Im F / w
+        for (int r = 0; r < R; r++)
+            for (int d = 0; d < ndof; d++) {
+                const cplx f = F[d + (size_t)ndof * r], tf = hdF[d + (size_t)ndof * r];
+                if (r < ndof) {
+                    dA_out[d + (size_t)ndof * (r + (size_t)ndof * iw)] = tf.x / (w * w) - 2.0 * f.x * dw / (w * w * w);
+                    dB_out[d + (size_t)ndof * (r + (size_t)ndof * iw)] = tf.y / w - f.y * dw / (w * w);
+                } else {
+                    const size_t o = d + (size_t)ndof * (iw + (size_t)nw * (r - ndof));
+                    dFd_out[2 * o] = tf.x;
+                    dFd_out[2 * o + 1] = tf.y;
+                }
+            }
+        for (int bb = 0; bb < (int)nb; bb++)
+            for (int d = 0; d < ndof; d++) {
+                const size_t o = d + (size_t)ndof * (iw + (size_t)nw * bb);
+                dFk_out[2 * o] = hdFK[d + (size_t)ndof * bb].x;
+                dFk_out[2 * o + 1] = hdFK[d + (size_t)ndof * bb].y;
+            }
+    }
+    flush_timing(ctx);
+    cleanup();
     return rc;
 }
 

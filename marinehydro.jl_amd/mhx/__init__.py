@@ -11,5 +11,6 @@ from .api import (SETTINGS, DiffractionProblem, DirectBEM, ExactGuevelDelhommeau
                   compute_wavenumber, context, default_greens_functions, diffraction_force, excitation_force,
                   froude_krylov_force, hydrodynamic_coefficients, is_direct, lu_solve, radiation_damping,
                   rotational_dofs, set_g, set_rho, solve, solve_linear, translational_dofs,
-                  SurrogateGuevelDelhommeau, train_surrogate, load_surrogate)
+                  SurrogateGuevelDelhommeau, train_surrogate, load_surrogate,
+                  hydrodynamic_coefficients_jvp, mesh_scaling_tangent)
 from .meshes import Mesh  # noqa: F401

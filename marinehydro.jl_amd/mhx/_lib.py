@@ -34,7 +34,7 @@ EXPORTED = ["mhx_create", "mhx_destroy", "mhx_last_error", "mhx_set_constants", 
             "mhx_lu_solve", "mhx_hydro_coeffs", "mhx_body_create", "mhx_body_destroy", "mhx_body_coeffs",
             "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work", "mhx_set_batch", "mhx_surrogate_train",
             "mhx_surrogate_save", "mhx_surrogate_load", "mhx_surrogate_params", "mhx_surrogate_eval",
-            "mhx_surrogate_loss_grad", "mhx_set_lu_mode", "mhx_solver_stats"]
+            "mhx_surrogate_loss_grad", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_hydro_coeffs_jvp"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int64)
@@ -48,6 +48,10 @@ class Options(C.Structure):
 class CMesh(C.Structure):
     _fields_ = [("nv", C.c_int64), ("nf", C.c_int64), ("vertices", _dp), ("faces", _ip), ("centers", _dp),
                 ("normals", _dp), ("areas", _dp), ("radii", _dp)]
+
+
+class CMeshTangent(C.Structure):
+    _fields_ = [("vertices", _dp), ("centers", _dp), ("normals", _dp), ("areas", _dp)]
 
 
 class MHXError(RuntimeError):
@@ -85,6 +89,8 @@ def lib():
         L.mhx_lu_solve.argtypes = [vp, C.c_int64, C.c_int64, _dp, C.c_int, _dp]
         L.mhx_hydro_coeffs.argtypes = [vp, C.POINTER(CMesh), C.c_int64, _dp, C.c_int64, _dp, C.c_int64, _dp,
                                        C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
+        L.mhx_hydro_coeffs_jvp.argtypes = [vp, C.POINTER(CMesh), C.POINTER(CMeshTangent), C.c_int64, _dp, _dp,
+                                           C.c_int64, _dp, _dp, C.c_int64, _dp, C.c_int, C.c_int] + [_dp] * 8
         L.mhx_body_create.argtypes = [vp, C.POINTER(CMesh), C.c_int64, _dp, C.POINTER(vp)]
         L.mhx_body_destroy.argtypes = [vp]
         L.mhx_body_destroy.restype = None

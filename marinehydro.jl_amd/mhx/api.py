@@ -187,6 +187,7 @@ class FloatingBody:
             raise TypeError("mesh must be an mhx.meshes.Mesh")
         self.mesh = mesh
         self.body_name = body_name
+        self._rotation = {}  # rigid rotational dofs: name -> axis (their motion follows the centers)
         if isinstance(dofs, dict):
             items = dofs.items()
         else:
@@ -197,6 +198,7 @@ class FloatingBody:
                     items.append((name, translational_dofs(mesh, name)))
                 else:
                     items.append((name, rotational_dofs(mesh, name, rotation_center)))
+                    self._rotation[name] = np.eye(3)[("Roll", "Pitch", "Yaw").index(name)]
         self.dofs = {}
         for name, d in items:
             d = np.asfortranarray(d, dtype=np.float64)
@@ -385,6 +387,104 @@ def hydrodynamic_coefficients(floatingbody, parameters, formulation=None, schedu
     else:
         out.update(diffraction_force=np.zeros(0), Froude_Krylov_force=np.zeros(0), excitation_force=np.zeros(0))
     return out
+
+
+def mesh_scaling_tangent(mesh, center=(0.0, 0.0, 0.0), scale=1.0):
+    """Tangent of the mesh fields along a uniform scaling about `center`: for a body of size r
+    (scale = r) built as r times a unit body, d/dr of (vertices, centers, normals, areas) is
+    ((V - c) / r, (C - c) / r, 0, 2 A / r).  This is what the reference's fd_mesh_rules! seeds for
+    the notebook's mesh_sphere(radius=r) (ad_rules_geometry.jl:174-230), up to its difference
+    step."""
+    c = np.asarray(center, dtype=np.float64)[None, :]
+    return {"vertices": (mesh.vertices - c) / scale, "centers": (mesh.centers - c) / scale,
+            "normals": np.zeros_like(mesh.normals), "areas": 2.0 * mesh.areas / scale}
+
+
+def hydrodynamic_coefficients_jvp(floatingbody, parameters, tangent, formulation=None):
+    """Forward-mode derivative of hydrodynamic_coefficients (src/problems_and_results.jl:464-495)
+    along `tangent`, a dict with any of: "vertices" (nv, 3), "centers", "normals" (nf, 3),
+    "areas" (nf,), "dofs" {name: (nf, 3)}, "wave_frequencies" (nw,).  Missing entries are zero;
+    the rigid rotational dofs of a body built from dof names follow the centers' tangent.
+    Returns (coefficients, tangents), two dicts with the layout of hydrodynamic_coefficients.
+    The reference obtains the same numbers with ForwardDiff or Enzyme through its AD rules
+    (ad_rules.jl:79-414); here the tangent assembly, right-hand sides and forces are device
+    kernels and the tangent solve reuses each frequency's FP64 LU."""
+    formulation = DirectBEM() if formulation is None else formulation
+    if formulation.gf not in (_lib.GF_WU, _lib.GF_DELHOMMEAU):
+        raise ValueError("derivatives are available for GFWu and ExactGuevelDelhommeau")
+    body = floatingbody
+    mesh = body.mesh
+    ws = np.ascontiguousarray(np.atleast_1d(np.asarray(parameters["wave_frequencies"], dtype=np.float64)))
+    inf = list(parameters.get("influenced_dofs", body.dof_names))
+    rad = list(parameters.get("radiating_dofs", []))
+    betas = np.ascontiguousarray(parameters.get("wave_directions", []), dtype=np.float64)
+    for nm in inf + rad:
+        if nm not in body.dofs:
+            raise ValueError("each influenced_dofs Symbol must be a key of floatingbody.dofs")
+    names = inf + [r for r in rad if r not in inf]
+    nd, nw, nb = len(names), ws.size, betas.size
+
+    def field(key, shape):
+        a = tangent.get(key)
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != shape:
+            raise DimensionMismatch(f"tangent {key} must have shape {shape}")
+        return a
+
+    dV = field("vertices", mesh.vertices.shape)
+    dC = field("centers", mesh.centers.shape)
+    dN = field("normals", mesh.normals.shape)
+    dA = field("areas", mesh.areas.shape)
+    dW = field("wave_frequencies", ws.shape)
+    tdofs = dict(tangent.get("dofs", {}))
+    rot = getattr(body, "_rotation", {})
+    dd = []
+    for nm in names:
+        if nm in tdofs:
+            d = np.asarray(tdofs[nm], dtype=np.float64)
+            if d.shape != (mesh.nfaces, 3):
+                raise DimensionMismatch(f"tangent dof {nm} must have shape ({mesh.nfaces}, 3)")
+        elif nm in rot and dC is not None:
+            d = np.cross(rot[nm][None, :], dC)
+        else:
+            d = np.zeros((mesh.nfaces, 3))
+        dd.append(np.asarray(d).ravel(order="F"))
+    ddofs = np.ascontiguousarray(np.concatenate(dd)) if nd else None
+    dofs = body._dof_stack(names) if nd else None
+    ctx = context()
+    ctx.set_constants(SETTINGS.rho, SETTINGS.g)
+    A = np.zeros((nd, nd, nw), order="F")
+    B = np.zeros((nd, nd, nw), order="F")
+    Fd = np.zeros((nd, nw, max(nb, 1)), dtype=np.complex128, order="F")
+    Fk = np.zeros((nd, nw, max(nb, 1)), dtype=np.complex128, order="F")
+    tA, tB, tFd, tFk = (np.zeros_like(x) for x in (A, B, Fd, Fk))
+    cm = _lib.cmesh(mesh)
+    tm = _lib.CMeshTangent(_lib.ptr(dV), _lib.ptr(dC), _lib.ptr(dN), _lib.ptr(dA))
+    ctx.check(_lib.lib().mhx_hydro_coeffs_jvp(
+        ctx.h, _lib.C.byref(cm), _lib.C.byref(tm), nd, _lib.ptr(dofs), _lib.ptr(ddofs), nw, _lib.ptr(ws),
+        _lib.ptr(dW), nb, _lib.ptr(betas) if nb else None, formulation.gf, int(formulation.direct),
+        *(_lib.ptr(x) for x in (A, B, Fd, Fk, tA, tB, tFd, tFk))))
+    ii = [names.index(nm) for nm in inf]
+    rr = [names.index(nm) for nm in rad]
+
+    def label(A, B, Fd, Fk):
+        out = {}
+        if rad:
+            out["added_mass"] = A[np.ix_(ii, rr)][..., None]
+            out["radiation_damping"] = B[np.ix_(ii, rr)][..., None]
+        else:
+            out["added_mass"] = out["radiation_damping"] = np.zeros(0)
+        if nb:
+            fd, fk = Fd[ii][:, :, :nb, None], Fk[ii][:, :, :nb, None]
+            out.update(diffraction_force=fd, Froude_Krylov_force=fk, excitation_force=fd + fk)
+        else:
+            out.update(diffraction_force=np.zeros(0), Froude_Krylov_force=np.zeros(0),
+                       excitation_force=np.zeros(0))
+        return out
+
+    return label(A, B, Fd, Fk), label(tA, tB, tFd, tFk)
 
 
 def assemble_matrices(green_functions, mesh, wavenumber, direct=True, include_identity=True, device=0):

@@ -325,3 +325,36 @@ def test_planted_mistake_is_caught(oracle):
 def meshes_flip(m):
     from mhx.meshes import Mesh
     return Mesh(m.vertices, m.faces[:, ::-1], m.centers, -m.normals, m.areas, m.radii)
+
+
+def test_notebook_derivatives_are_radius_scaling(oracle):
+    """mh_tutorial.ipynb cells 5, 9, 13: the notebook's ForwardDiff/Enzyme derivatives with respect
+    to the sphere radius (mesh tangent seeded by central differences of the mesher) and to omega
+    are reproduced by central differences of the oracle on the uniformly scaled sphere.  This pins
+    what the derivative tests compare against: d/dr is the uniform-scaling direction."""
+    from mhx import meshes
+    nb = REF["notebook_sphere"]
+    d = nb["derivatives_direct_Wu"]
+    base = meshes.sphere(1.0, 6, 6, immersed=True)
+
+    def coeffs(r, w, gf=oracle.GF_WU, direct=True):
+        om = oracle.Mesh.from_mhx(base.scaled(r))
+        res = oracle.hydro_coeffs(om, [oracle.rigid_dof(om, 2)], [w], [0.0], rho=1000.0, g=9.81, gf=gf, direct=direct)
+        return res["added_mass"][0, 0, 0], res["excitation_force"][0, 0, 0].real
+
+    h = 1e-4
+
+    def fd(f, x):  # fourth-order central difference
+        return (8 * (np.array(f(x + h)) - np.array(f(x - h))) - (np.array(f(x + 2 * h)) - np.array(f(x - 2 * h)))) / (12 * h)
+
+    dr = fd(lambda r: coeffs(r, 1.03), 1.0)
+    dw = fd(lambda w: coeffs(1.0, w), 1.03)
+    assert dr[0] == pytest.approx(d["dA_dr"], abs=2e-6)
+    assert dr[1] == pytest.approx(d["dReFex_dr"], abs=2e-5)
+    assert dw[0] == pytest.approx(d["dA_domega"], abs=2e-6)
+    assert dw[1] == pytest.approx(d["dReFex_domega"], abs=2e-6)
+    for key, (gf, direct) in {"indirect_Wu": (oracle.GF_WU, False), "direct_Delhommeau": (oracle.GF_DELHOMMEAU, True),
+                              "indirect_Delhommeau": (oracle.GF_DELHOMMEAU, False)}.items():
+        dr = fd(lambda r: coeffs(r, 1.03, gf, direct), 1.0)
+        dw = fd(lambda w: coeffs(1.0, w, gf, direct), 1.03)
+        np.testing.assert_allclose([dr[0], dw[0], dr[1]], nb["derivatives_table"][key], rtol=0, atol=1e-4)
