@@ -159,6 +159,27 @@ int mhx_hydro_coeffs_jvp(mhx_ctx* ctx, const mhx_mesh* mesh, const mhx_mesh_tang
                          double* B_out, double* Fd_out, double* Fk_out, double* dA_out, double* dB_out,
                          double* dFd_out, double* dFk_out);
 
+/* Reverse-mode gradient of the coefficients: for the real functional
+ *   J = sum Abar .* A + sum Bbar .* B + Re sum conj(Fdbar) .* Fd + Re sum conj(Fkbar) .* Fk
+ * (cotangents in the layouts of the outputs, Fd/Fk complex interleaved; NULL = zero), returns
+ * the coefficients (FP64 LU per frequency) and dJ/d(input) for every non-NULL field of grad:
+ * vertices (nv x 3), centers / normals (nf x 3), areas (nf), dofs (the dofs layout) and omegas
+ * (nw).  This is what the reference's Enzyme / Zygote reverse mode returns through its rules
+ * (ad_rules.jl:79-414): the transposed solve reuses the LU (z = A^H \ ybar, Abar = -z y^H) and
+ * the assembly is pulled back pair by pair.  gf: MHX_GF_WU or MHX_GF_DELHOMMEAU. */
+typedef struct {
+    double* vertices;
+    double* centers;
+    double* normals;
+    double* areas;
+    double* dofs;
+    double* omegas;
+} mhx_mesh_grad;
+int mhx_hydro_coeffs_vjp(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const double* dofs, int64_t nw,
+                         const double* omegas, int64_t nb, const double* betas, int gf, int direct, const double* A_bar,
+                         const double* B_bar, const double* Fd_bar, const double* Fk_bar, double* A_out, double* B_out,
+                         double* Fd_out, double* Fk_out, const mhx_mesh_grad* grad);
+
 /* Learned surrogate of the exact Guevel-Delhommeau wave term (no counterpart in the reference; it
  * replaces the exact term of src/green_functions/exact_Guevel_Delhommeau.jl:15-59 inside assembly).
  * The network maps (r, v) = (k R_h, k |z + zeta|) through 64 Fourier features and 4 x 128 SiLU

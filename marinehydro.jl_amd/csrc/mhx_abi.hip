@@ -897,6 +897,177 @@ int mhx_hydro_coeffs_jvp(mhx_ctx* ctx, const mhx_mesh* mesh, const mhx_mesh_tang
     return rc;
 }
 
+// Gradient of J = <Abar, A> + <Bbar, B> + Re<Fdbar, Fd> + Re<Fkbar, Fk> (reverse mode): per
+// frequency the primal omega-solve (FP64 LU), the pressure pullback, the transposed solve on the
+// same LU factors, and the right-hand-side and pair pullbacks accumulated into the gradients.
+int mhx_hydro_coeffs_vjp(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const double* dofs, int64_t nw,
+                         const double* omegas, int64_t nb, const double* betas, int gf, int direct, const double* A_bar,
+                         const double* B_bar, const double* Fd_bar, const double* Fk_bar, double* A_out, double* B_out,
+                         double* Fd_out, double* Fk_out, const mhx_mesh_grad* grad) {
+    if (!ctx) return MHX_EINVAL;
+    if (gf != MHX_GF_WU && gf != MHX_GF_DELHOMMEAU)
+        return fail(ctx, MHX_EINVAL, "coeffs_vjp: the adjoint is available for GFWu and ExactGuevelDelhommeau");
+    if (!grad) return fail(ctx, MHX_EINVAL, "coeffs_vjp: null gradient struct");
+    if (nw < 0 || nb < 0 || (nw && !omegas) || (nb && !betas)) return fail(ctx, MHX_EINVAL, "coeffs_vjp: bad frequency/heading arrays");
+    if ((ndof && (!A_out || !B_out)) || (nb && ndof && (!Fd_out || !Fk_out)))
+        return fail(ctx, MHX_EINVAL, "coeffs_vjp: null output");
+    for (int64_t w = 0; w < nw; w++)
+        if (!(omegas[w] > 0) || !std::isfinite(omegas[w])) return fail(ctx, MHX_EINVAL, "omega must be positive and finite");
+    const int64_t R64 = ndof + nb;
+    if (R64 <= 0) return fail(ctx, MHX_EDIM, "coeffs_vjp: no radiating dof and no heading");
+    if (R64 > 16) return fail(ctx, MHX_EDIM, "coeffs_vjp: at most 16 right-hand sides (dofs + headings)");
+    if (mesh && mesh->nf > 4096) return fail(ctx, MHX_EDIM, "coeffs_vjp: more than 4096 panels is not supported");
+    hipSetDevice(ctx->device);
+    mhx_body* b = nullptr;
+    int rc = body_upload(ctx, mesh, ndof, dofs, &b);
+    if (rc) return rc;
+    const int n = b->n, R = (int)R64;
+    direct = direct ? 1 : 0;
+    hipStream_t st = ctx->stream;
+    const size_t nv = mesh->nv, nR = (size_t)n * R;
+    const bool tab = gf == MHX_GF_DELHOMMEAU;
+    double *dv = nullptr, *dc = nullptr, *dn = nullptr, *da = nullptr, *dr = nullptr;
+    int64_t* df = nullptr;
+    double *gV = nullptr, *gC = nullptr, *gN = nullptr, *gA = nullptr, *gD = nullptr, *wk = nullptr;
+    cplx *Fb = nullptr, *FKb = nullptr, *phib = nullptr, *t1 = nullptr, *t2 = nullptr;
+    auto cleanup = [&]() {
+        dfree(dv); dfree(dc); dfree(dn); dfree(da); dfree(dr); dfree(df); dfree(gV); dfree(gC); dfree(gN); dfree(gA);
+        dfree(gD); dfree(wk); dfree(Fb); dfree(FKb); dfree(phib); dfree(t1); dfree(t2);
+        mhx_body_destroy(b);
+    };
+    const size_t nd1 = ndof ? ndof : 1, nb1 = nb ? nb : 1;
+    if ((rc = ensure_work(ctx, b, R, (int)nb, tab, 1)) || (rc = ensure_rankine(ctx, b, direct)) ||
+        (rc = dalloc(ctx, &dv, 3 * nv)) || (rc = dalloc(ctx, &df, 4 * (size_t)n)) || (rc = dalloc(ctx, &dc, 3 * (size_t)n)) ||
+        (rc = dalloc(ctx, &dn, 3 * (size_t)n)) || (rc = dalloc(ctx, &da, n)) || (rc = dalloc(ctx, &dr, n)) ||
+        (rc = dalloc(ctx, &gV, 3 * nv)) || (rc = dalloc(ctx, &gC, 3 * (size_t)n)) || (rc = dalloc(ctx, &gN, 3 * (size_t)n)) ||
+        (rc = dalloc(ctx, &gA, n)) || (rc = dalloc(ctx, &gD, 3 * (size_t)n * nd1)) || (rc = dalloc(ctx, &wk, 2)) ||
+        (rc = dalloc(ctx, &Fb, nd1 * R)) || (rc = dalloc(ctx, &FKb, nd1 * nb1)) || (rc = dalloc(ctx, &phib, nR)) ||
+        (rc = dalloc(ctx, &t1, nR)) || (rc = dalloc(ctx, &t2, nR))) {
+        cleanup();
+        return rc;
+    }
+    auto ok = [&](hipError_t e, const char* what) {
+        if (e == hipSuccess) return true;
+        rc = fail(ctx, MHX_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+        return false;
+    };
+    hipMemcpyAsync(dv, mesh->vertices, sizeof(double) * 3 * nv, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(df, mesh->faces, sizeof(int64_t) * 4 * n, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dc, mesh->centers, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dn, mesh->normals, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(da, mesh->areas, sizeof(double) * n, hipMemcpyHostToDevice, st);
+    hipMemcpyAsync(dr, mesh->radii, sizeof(double) * n, hipMemcpyHostToDevice, st);
+    hipMemsetAsync(gV, 0, sizeof(double) * 3 * nv, st);
+    hipMemsetAsync(gC, 0, sizeof(double) * 3 * n, st);
+    hipMemsetAsync(gN, 0, sizeof(double) * 3 * n, st);
+    hipMemsetAsync(gA, 0, sizeof(double) * n, st);
+    hipMemsetAsync(gD, 0, sizeof(double) * 3 * n * nd1, st);
+    if (nb) hipMemcpyAsync(b->betas, betas, sizeof(double) * nb, hipMemcpyHostToDevice, st);
+    const mhx::AdjointGrad gr{gV, gC, gN, gA, gD, wk};
+    std::vector<cplx> F((size_t)ndof * R), FK((size_t)ndof * nb1), hFb(nd1 * R), hFKb(nd1 * nb1);
+    const double g = ctx->g;
+    int* errw = mhx::getrs_error_word(n, b->work + 16 * b->nR());
+    for (int64_t iw = 0; iw < nw && !rc; iw++) {
+        const double w = omegas[iw], k = w * w / g;  // compute_wavenumber (src/waves.jl:133-134)
+        if ((rc = set_freqs(ctx, b, omegas + iw, 1))) break;
+        mhx::launch_bc(st, n, b->geom, b->dofs, ndof, b->betas, (int)nb, b->d_ws, b->d_ks, 1, g, b->B, b->nR());
+        if ((rc = assemble_into(ctx, b, 1, gf, direct, 1)) || (rc = check_status(ctx, b, gf))) break;
+        if ((rc = solve_fp64(ctx, b, R, direct, 1))) {
+            if (rc == MHX_ESINGULAR) fail(ctx, rc, ctx->err);
+            break;
+        }
+        if (ndof > 0)
+            mhx::launch_forces(st, n, b->geom, b->dofs, ndof, b->Y, b->nR(), R, ctx->rho, b->d_ws, b->d_ks, 1, g, b->betas,
+                               (int)nb, b->F, b->FK);
+        if (!ok(hipMemcpyAsync(F.data(), b->F, sizeof(cplx) * ndof * R, hipMemcpyDeviceToHost, st), "copy F") ||
+            (nb && !ok(hipMemcpyAsync(FK.data(), b->FK, sizeof(cplx) * ndof * nb, hipMemcpyDeviceToHost, st), "copy FK")) ||
+            !ok(hipStreamSynchronize(st), "sync"))
+            break;
+        pack(F, FK, ndof, R, (int)nb, iw, nw, w, A_out, B_out, Fd_out, Fk_out);
+        // force cotangents from the packing (problems_and_results.jl:207-208): A = Re F / w^2,
+        // B = Im F / w, Fd = F (diffraction columns), Fk = FK
+        double wbar = 0.0;
+        for (int r = 0; r < R; r++)
+            for (int d = 0; d < ndof; d++) {
+                cplx fb = make_double2(0.0, 0.0);
+                const cplx f = F[d + (size_t)ndof * r];
+                if (r < ndof) {
+                    const size_t o = d + (size_t)ndof * (r + (size_t)ndof * iw);
+                    const double ab = A_bar ? A_bar[o] : 0.0, bb = B_bar ? B_bar[o] : 0.0;
+                    fb = make_double2(ab / (w * w), bb / w);
+                    wbar += -2.0 * ab * f.x / (w * w * w) - bb * f.y / (w * w);
+                } else if (Fd_bar) {
+                    const size_t o = d + (size_t)ndof * (iw + (size_t)nw * (r - ndof));
+                    fb = make_double2(Fd_bar[2 * o], Fd_bar[2 * o + 1]);
+                }
+                hFb[d + (size_t)ndof * r] = fb;
+            }
+        for (int bb = 0; bb < (int)nb; bb++)
+            for (int d = 0; d < ndof; d++) {
+                const size_t o = d + (size_t)ndof * (iw + (size_t)nw * bb);
+                hFKb[d + (size_t)ndof * bb] = Fk_bar ? make_double2(Fk_bar[2 * o], Fk_bar[2 * o + 1]) : make_double2(0.0, 0.0);
+            }
+        double hwk[2] = {0.0, 0.0};
+        cplx* W2 = b->work + 16 * b->nR();
+        const int64_t cnt = (int64_t)n * R;
+        hipMemcpyAsync(Fb, hFb.data(), sizeof(cplx) * nd1 * R, hipMemcpyHostToDevice, st);
+        hipMemcpyAsync(FKb, hFKb.data(), sizeof(cplx) * nd1 * nb1, hipMemcpyHostToDevice, st);
+        hipMemsetAsync(wk, 0, 2 * sizeof(double), st);
+        {
+            Phase ph(ctx, "adjoint");
+            mhx::launch_adjoint_forces(st, n, b->geom, b->dofs, ndof, b->Y, R, Fb, FKb, ctx->rho, w, k, g, b->betas,
+                                       (int)nb, phib, gr);
+            const cplx *U, *V, *V2;
+            cplx* Bbar;
+            if (direct) {
+                // lambda = D^{-H} phibar; Bbar = S^H lambda; Sbar = lambda B^H; Dbar = -lambda phi^H
+                mhx::getrs_batched(st, 2, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, phib, 0, W2, 0, 1, &b->trs_epoch,
+                                   true);
+                mhx::launch_matmul_h(st, n, R, b->S, phib, t1);
+                mhx::launch_scale(st, cnt, -1.0, phib, t2);
+                U = phib;
+                V = b->B;
+                V2 = b->Y;
+                Bbar = t1;
+            } else {
+                // sigmabar = S^H phibar; mu = D^{-H} sigmabar; Sbar = phibar sigma^H; Dbar = -mu sigma^H; Bbar = mu
+                mhx::launch_matmul_h(st, n, R, b->S, phib, t1);
+                mhx::getrs_batched(st, 2, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, t1, 0, W2, 0, 1, &b->trs_epoch,
+                                   true);
+                mhx::launch_scale(st, cnt, -1.0, t1, t2);
+                U = phib;
+                V = b->B;
+                V2 = b->B;
+                Bbar = t1;
+            }
+            mhx::launch_adjoint_bc(st, n, b->geom, b->dofs, ndof, b->betas, (int)nb, w, k, g, Bbar, gr);
+            mhx::launch_adjoint_pairs(st, n, dv, df, dc, dn, da, dr, k, direct, tab, b->WG, b->WR, U, V, t2, V2, R, gr);
+        }
+        int herr = 0;
+        if (!ok(hipGetLastError(), "adjoint launch") ||
+            !ok(hipMemcpyAsync(hwk, wk, 2 * sizeof(double), hipMemcpyDeviceToHost, st), "copy wk") ||
+            !ok(hipMemcpyAsync(&herr, errw, sizeof(int), hipMemcpyDeviceToHost, st), "copy err") ||
+            !ok(hipStreamSynchronize(st), "sync"))
+            break;
+        if (herr) {
+            rc = fail(ctx, MHX_EDEVICE, "adjoint solve: triangular sweep gave up waiting for a block");
+            break;
+        }
+        if (grad->omegas) grad->omegas[iw] = wbar + hwk[0] + hwk[1] * 2.0 * w / g;
+    }
+    if (!rc) {
+        const bool good = (!grad->vertices || ok(hipMemcpyAsync(grad->vertices, gV, sizeof(double) * 3 * nv, hipMemcpyDeviceToHost, st), "copy")) &&
+                          (!grad->centers || ok(hipMemcpyAsync(grad->centers, gC, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st), "copy")) &&
+                          (!grad->normals || ok(hipMemcpyAsync(grad->normals, gN, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st), "copy")) &&
+                          (!grad->areas || ok(hipMemcpyAsync(grad->areas, gA, sizeof(double) * n, hipMemcpyDeviceToHost, st), "copy")) &&
+                          (!grad->dofs || !ndof || ok(hipMemcpyAsync(grad->dofs, gD, sizeof(double) * 3 * n * ndof, hipMemcpyDeviceToHost, st), "copy"));
+        if (good) ok(hipStreamSynchronize(st), "sync");
+    }
+    flush_timing(ctx);
+    cleanup();
+    return rc;
+}
+
 int mhx_hydro_coeffs(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const double* dofs, int64_t nw,
                      const double* omegas, int64_t nb, const double* betas, int gf, int direct, int mode, double* A_out,
                      double* B_out, double* Fd_out, double* Fk_out) {

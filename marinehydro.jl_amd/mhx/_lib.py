@@ -34,7 +34,8 @@ EXPORTED = ["mhx_create", "mhx_destroy", "mhx_last_error", "mhx_set_constants", 
             "mhx_lu_solve", "mhx_hydro_coeffs", "mhx_body_create", "mhx_body_destroy", "mhx_body_coeffs",
             "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work", "mhx_set_batch", "mhx_surrogate_train",
             "mhx_surrogate_save", "mhx_surrogate_load", "mhx_surrogate_params", "mhx_surrogate_eval",
-            "mhx_surrogate_loss_grad", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_hydro_coeffs_jvp"]
+            "mhx_surrogate_loss_grad", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_hydro_coeffs_jvp",
+            "mhx_hydro_coeffs_vjp"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int64)
@@ -52,6 +53,10 @@ class CMesh(C.Structure):
 
 class CMeshTangent(C.Structure):
     _fields_ = [("vertices", _dp), ("centers", _dp), ("normals", _dp), ("areas", _dp)]
+
+
+class CMeshGrad(C.Structure):
+    _fields_ = [("vertices", _dp), ("centers", _dp), ("normals", _dp), ("areas", _dp), ("dofs", _dp), ("omegas", _dp)]
 
 
 class MHXError(RuntimeError):
@@ -91,6 +96,8 @@ def lib():
                                        C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
         L.mhx_hydro_coeffs_jvp.argtypes = [vp, C.POINTER(CMesh), C.POINTER(CMeshTangent), C.c_int64, _dp, _dp,
                                            C.c_int64, _dp, _dp, C.c_int64, _dp, C.c_int, C.c_int] + [_dp] * 8
+        L.mhx_hydro_coeffs_vjp.argtypes = [vp, C.POINTER(CMesh), C.c_int64, _dp, C.c_int64, _dp, C.c_int64, _dp,
+                                           C.c_int, C.c_int] + [_dp] * 8 + [C.POINTER(CMeshGrad)]
         L.mhx_body_create.argtypes = [vp, C.POINTER(CMesh), C.c_int64, _dp, C.POINTER(vp)]
         L.mhx_body_destroy.argtypes = [vp]
         L.mhx_body_destroy.restype = None

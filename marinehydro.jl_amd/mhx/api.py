@@ -487,6 +487,98 @@ def hydrodynamic_coefficients_jvp(floatingbody, parameters, tangent, formulation
     return label(A, B, Fd, Fk), label(tA, tB, tFd, tFk)
 
 
+def hydrodynamic_coefficients_vjp(floatingbody, parameters, cotangents, formulation=None):
+    """Reverse-mode gradient of J = sum over the outputs of hydrodynamic_coefficients of
+    <cotangent, output> (real outputs: elementwise product; complex forces: Re(conj(c) * F)),
+    with `cotangents` a dict keyed like the outputs ("added_mass", "radiation_damping",
+    "diffraction_force", "Froude_Krylov_force", "excitation_force"; missing = zero; an
+    excitation cotangent applies to both of its terms).  Returns (coefficients, gradients):
+    gradients holds dJ/d"vertices" (nv, 3), "centers", "normals" (nf, 3), "areas" (nf,),
+    "dofs" {name: (nf, 3)} and "wave_frequencies" (nw,).  For a body built from dof names the
+    rotational dofs follow the centers, and their gradient is folded into "centers".  The
+    reference's Enzyme / Zygote reverse mode gives the same numbers through its rules
+    (ad_rules.jl:79-414); here the pullbacks are device kernels and the transposed solve reuses
+    each frequency's FP64 LU."""
+    formulation = DirectBEM() if formulation is None else formulation
+    if formulation.gf not in (_lib.GF_WU, _lib.GF_DELHOMMEAU):
+        raise ValueError("derivatives are available for GFWu and ExactGuevelDelhommeau")
+    body = floatingbody
+    mesh = body.mesh
+    ws = np.ascontiguousarray(np.atleast_1d(np.asarray(parameters["wave_frequencies"], dtype=np.float64)))
+    inf = list(parameters.get("influenced_dofs", body.dof_names))
+    rad = list(parameters.get("radiating_dofs", []))
+    betas = np.ascontiguousarray(parameters.get("wave_directions", []), dtype=np.float64)
+    for nm in inf + rad:
+        if nm not in body.dofs:
+            raise ValueError("each influenced_dofs Symbol must be a key of floatingbody.dofs")
+    names = inf + [r for r in rad if r not in inf]
+    nd, nw, nb = len(names), ws.size, betas.size
+    ii = [names.index(nm) for nm in inf]
+    rr = [names.index(nm) for nm in rad]
+    Ab = np.zeros((nd, nd, nw), order="F")
+    Bb = np.zeros((nd, nd, nw), order="F")
+    Fdb = np.zeros((nd, nw, max(nb, 1)), dtype=np.complex128, order="F")
+    Fkb = np.zeros((nd, nw, max(nb, 1)), dtype=np.complex128, order="F")
+
+    def cot(key, shape, dtype):
+        c = cotangents.get(key)
+        if c is None:
+            return None
+        c = np.asarray(c, dtype=dtype)
+        if c.size != int(np.prod(shape)):
+            raise DimensionMismatch(f"cotangent {key} must have {int(np.prod(shape))} elements (shape {shape})")
+        return c.reshape(shape)
+
+    for key, dst in (("added_mass", Ab), ("radiation_damping", Bb)):
+        c = cot(key, (len(inf), len(rad), nw), np.float64)
+        if c is not None:
+            dst[np.ix_(ii, rr)] += c
+    for key, dsts in (("diffraction_force", (Fdb,)), ("Froude_Krylov_force", (Fkb,)),
+                      ("excitation_force", (Fdb, Fkb))):
+        c = cot(key, (len(inf), nw, nb), np.complex128)
+        if c is not None:
+            for dst in dsts:
+                dst[ii, :, :nb] += c
+    dofs = body._dof_stack(names) if nd else None
+    ctx = context()
+    ctx.set_constants(SETTINGS.rho, SETTINGS.g)
+    A = np.zeros((nd, nd, nw), order="F")
+    B = np.zeros((nd, nd, nw), order="F")
+    Fd = np.zeros((nd, nw, max(nb, 1)), dtype=np.complex128, order="F")
+    Fk = np.zeros((nd, nw, max(nb, 1)), dtype=np.complex128, order="F")
+    gV = np.zeros_like(mesh.vertices)
+    gC = np.zeros_like(mesh.centers)
+    gN = np.zeros_like(mesh.normals)
+    gA = np.zeros_like(mesh.areas)
+    gD = np.zeros(3 * mesh.nfaces * max(nd, 1))
+    gW = np.zeros(nw)
+    cg = _lib.CMeshGrad(*(_lib.ptr(x) for x in (gV, gC, gN, gA, gD, gW)))
+    cm = _lib.cmesh(mesh)
+    ctx.check(_lib.lib().mhx_hydro_coeffs_vjp(
+        ctx.h, _lib.C.byref(cm), nd, _lib.ptr(dofs), nw, _lib.ptr(ws), nb, _lib.ptr(betas) if nb else None,
+        formulation.gf, int(formulation.direct), _lib.ptr(Ab), _lib.ptr(Bb), _lib.ptr(Fdb), _lib.ptr(Fkb),
+        _lib.ptr(A), _lib.ptr(B), _lib.ptr(Fd), _lib.ptr(Fk), _lib.C.byref(cg)))
+    grads = {"vertices": gV, "centers": gC, "normals": gN, "areas": gA, "wave_frequencies": gW, "dofs": {}}
+    rot = getattr(body, "_rotation", {})
+    for q, nm in enumerate(names):
+        gd = gD[3 * mesh.nfaces * q:3 * mesh.nfaces * (q + 1)].reshape(mesh.nfaces, 3, order="F").copy()
+        grads["dofs"][nm] = gd
+        if nm in rot:  # dof = axis x (c - rotation centre): cbar += dofbar x axis
+            gC += np.cross(gd, rot[nm][None, :])
+    out = {}
+    if rad:
+        out["added_mass"] = A[np.ix_(ii, rr)][..., None]
+        out["radiation_damping"] = B[np.ix_(ii, rr)][..., None]
+    else:
+        out["added_mass"] = out["radiation_damping"] = np.zeros(0)
+    if nb:
+        fd, fk = Fd[ii][:, :, :nb, None], Fk[ii][:, :, :nb, None]
+        out.update(diffraction_force=fd, Froude_Krylov_force=fk, excitation_force=fd + fk)
+    else:
+        out.update(diffraction_force=np.zeros(0), Froude_Krylov_force=np.zeros(0), excitation_force=np.zeros(0))
+    return out, grads
+
+
 def assemble_matrices(green_functions, mesh, wavenumber, direct=True, include_identity=True, device=0):
     """S, D (or K for the indirect method) as n x n complex column-major arrays."""
     gf = _wave_code(green_functions)
