@@ -295,3 +295,52 @@ def test_vjp_rigid_rotation_folds_into_centers(mhx):
     lhs = float(np.sum(gr["vertices"] * t["vertices"]) + np.sum(gr["centers"] * t["centers"]))
     rhs = _dot_cotangent(cot, tg)
     assert lhs == pytest.approx(rhs, rel=1e-10)
+
+
+def test_adjoint_sensitivities_hemisphere_2048(mhx):
+    """Hemisphere of radius 1 m, 2,048 panels, 16 frequencies in 0.5-2.0 rad/s, FP64: gradients
+    of heave added mass, damping and Re(heave excitation) at beta = 0 with respect to the radius,
+    the frequencies and the vertex coordinates, against central differences (step 1e-6) of the
+    FP64 omega-solve (itself within 1e-9 of the oracle).  Vertex coordinates are checked along
+    smooth directions (an affine map of the body): moving one vertex of a quad alone bends the
+    panel, the centre leaves the panel's plane and the self-influence double-layer term jumps,
+    for the reference as here, so a single-coordinate difference has no derivative to match."""
+    print("adjoint hemisphere 2048 x 16", flush=True)
+    from mhx import meshes
+    mhx.set_rho(1025.0)
+    try:
+        m = meshes.sphere_with_count(1.0, 2048)
+        ws = np.linspace(0.5, 2.0, 16)
+        par = {"wave_frequencies": ws, "radiating_dofs": ["Heave"], "wave_directions": [0.0]}
+        form = mhx.DirectBEM()
+
+        def coeffs(mesh, w=ws):
+            body = mhx.FloatingBody(mesh, ["Heave"])
+            r = mhx.hydrodynamic_coefficients(body, dict(par, wave_frequencies=w), form)
+            return np.stack([r["added_mass"].ravel(), r["radiation_damping"].ravel(), r["excitation_force"].ravel().real])
+
+        h = 1e-6
+        fd_r = (coeffs(m.scaled(1 + h)) - coeffs(m.scaled(1 - h))) / (2 * h)
+        fd_w = (coeffs(m, ws + h) - coeffs(m, ws - h)) / (2 * h)
+        rng = np.random.default_rng(2048)
+        at, tdir = _direction(m, rng)
+        fd_v = (coeffs(at(h)) - coeffs(at(-h))) / (2 * h)
+        body = mhx.FloatingBody(m, ["Heave"])
+        sc = mhx.mesh_scaling_tangent(m)
+        for q, key in enumerate(("added_mass", "radiation_damping", "excitation_force")):
+            cot = {key: np.ones(16, dtype=complex if key == "excitation_force" else float)}
+            p, g = mhx.hydrodynamic_coefficients_vjp(body, par, cot, form)
+            dr = _dot_tangent(g, sc, [])
+            assert dr == pytest.approx(fd_r[q].sum(), rel=1e-6), key
+            np.testing.assert_allclose(g["wave_frequencies"], fd_w[q], rtol=1e-6, atol=1e-6 * np.max(np.abs(fd_w[q])))
+            dv = _dot_tangent(g, tdir, [])
+            assert dv == pytest.approx(fd_v[q].sum(), rel=1e-6), key
+        # the forward mode gives the per-frequency radius derivatives
+        _, tr = mhx.hydrodynamic_coefficients_jvp(body, par, sc, form)
+        got = np.stack([tr["added_mass"].ravel(), tr["radiation_damping"].ravel(), tr["excitation_force"].ravel().real])
+        for q in range(3):
+            err = np.max(np.abs(got[q] - fd_r[q])) / np.max(np.abs(fd_r[q]))
+            print(f"  output {q}: max |jvp - fd| / max |fd| = {err:.2e}", flush=True)
+            assert err < 1e-6
+    finally:
+        mhx.set_rho(1000.0)
