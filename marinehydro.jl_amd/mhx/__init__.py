@@ -12,5 +12,6 @@ from .api import (SETTINGS, DiffractionProblem, DirectBEM, ExactGuevelDelhommeau
                   froude_krylov_force, hydrodynamic_coefficients, is_direct, lu_solve, radiation_damping,
                   rotational_dofs, set_g, set_rho, solve, solve_linear, translational_dofs,
                   SurrogateGuevelDelhommeau, train_surrogate, load_surrogate,
-                  hydrodynamic_coefficients_jvp, hydrodynamic_coefficients_vjp, mesh_scaling_tangent)
+                  hydrodynamic_coefficients_jvp, hydrodynamic_coefficients_vjp, mesh_scaling_tangent,
+                  chain_mesh_gradient)
 from .meshes import Mesh  # noqa: F401

@@ -579,6 +579,16 @@ def hydrodynamic_coefficients_vjp(floatingbody, parameters, cotangents, formulat
     return out, grads
 
 
+def chain_mesh_gradient(mesh, grads):
+    """Field-wise gradient (vertices, centers, normals, areas held independent, as the
+    reference's StaticArraysMesh and its AD treat them) -> gradient with respect to the vertices
+    when the centres, normals and areas are the face properties of the vertices (Capytaine's
+    formulas, mhx.meshes.face_properties)."""
+    from .meshes import face_properties_vjp
+    return grads["vertices"] + face_properties_vjp(mesh.vertices, mesh.faces, grads["centers"], grads["normals"],
+                                                   grads["areas"])
+
+
 def assemble_matrices(green_functions, mesh, wavenumber, direct=True, include_identity=True, device=0):
     """S, D (or K for the indirect method) as n x n complex column-major arrays."""
     gf = _wave_code(green_functions)

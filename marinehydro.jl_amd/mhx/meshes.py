@@ -94,6 +94,74 @@ def face_properties(vertices, faces):
     return centers, normals, areas, radii
 
 
+def face_properties_vjp(vertices, faces, g_centers=None, g_normals=None, g_areas=None):
+    """Pull the cotangents of the face properties (centres, normals, areas of face_properties)
+    back to the vertices: returns dJ/dV (nv, 3) for J depending on V only through them.  The
+    radii only select a far-field rule (no derivative).  Reverse mode of the formulas above:
+    normalisation n = x/|x| gives xbar = (nbar - (nbar . n) n)/|x|, a cross product x = u x w
+    gives ubar = w x xbar and wbar = xbar x u, and the area-weighted centre
+    C = (a1 c1 + a2 c2)/(a1 + a2) gives c1bar = Cbar a1/A and a1bar = Cbar . (c1 - C)/A."""
+    V = np.asarray(vertices, dtype=np.float64)
+    F = np.asarray(faces) - 1
+    nf = F.shape[0]
+    gC = np.zeros((nf, 3)) if g_centers is None else np.asarray(g_centers, dtype=np.float64)
+    gN = np.zeros((nf, 3)) if g_normals is None else np.asarray(g_normals, dtype=np.float64)
+    gA = np.zeros(nf) if g_areas is None else np.asarray(g_areas, dtype=np.float64)
+    out = np.zeros_like(V)
+
+    def unit_bar(x, nbar):
+        nx = np.linalg.norm(x, axis=1)[:, None]
+        n = x / nx
+        return (nbar - np.sum(nbar * n, 1)[:, None] * n) / nx
+
+    def cross_bar(u, w, xbar):  # x = u x w
+        return np.cross(w, xbar), np.cross(xbar, u)
+
+    tri = F[:, 0] == F[:, 3]
+    t, q = F[tri], F[~tri]
+    if t.size:
+        e1, e2 = V[t[:, 1]] - V[t[:, 0]], V[t[:, 2]] - V[t[:, 0]]
+        cr = np.cross(e1, e2)
+        ta = np.linalg.norm(cr, axis=1)[:, None]
+        crb = unit_bar(cr, gN[tri]) + (0.5 * gA[tri])[:, None] * cr / ta
+        b1, b2 = cross_bar(e1, e2, crb)
+        gc = gC[tri] / 3.0
+        np.add.at(out, t[:, 0], gc - b1 - b2)
+        np.add.at(out, t[:, 1], gc + b1)
+        np.add.at(out, t[:, 2], gc + b2)
+    if q.size:
+        V0, V1, V2, V3 = (V[q[:, k]] for k in range(4))
+        d1, d2 = V2 - V0, V3 - V1
+        qb = unit_bar(np.cross(d1, d2), gN[~tri])
+        b1, b2 = cross_bar(d1, d2, qb)
+        np.add.at(out, q[:, 2], b1)
+        np.add.at(out, q[:, 0], -b1)
+        np.add.at(out, q[:, 3], b2)
+        np.add.at(out, q[:, 1], -b2)
+        x1 = np.cross(V1 - V0, V2 - V0)
+        x2 = np.cross(V3 - V0, V2 - V0)
+        a1 = 0.5 * np.linalg.norm(x1, axis=1)
+        a2 = 0.5 * np.linalg.norm(x2, axis=1)
+        A = a1 + a2
+        c1 = (V0 + V1 + V2) / 3.0
+        c2 = (V2 + V3 + V0) / 3.0
+        C = (a1[:, None] * c1 + a2[:, None] * c2) / A[:, None]
+        Cb = gC[~tri]
+        c1b = Cb * (a1 / A)[:, None]
+        c2b = Cb * (a2 / A)[:, None]
+        a1b = gA[~tri] + np.sum(Cb * (c1 - C), 1) / A
+        a2b = gA[~tri] + np.sum(Cb * (c2 - C), 1) / A
+        x1b = (0.5 * a1b / np.linalg.norm(x1, axis=1))[:, None] * x1
+        x2b = (0.5 * a2b / np.linalg.norm(x2, axis=1))[:, None] * x2
+        u1, w1 = cross_bar(V1 - V0, V2 - V0, x1b)
+        u2, w2 = cross_bar(V3 - V0, V2 - V0, x2b)
+        np.add.at(out, q[:, 1], u1 + c1b / 3.0)
+        np.add.at(out, q[:, 2], w1 + w2 + (c1b + c2b) / 3.0)
+        np.add.at(out, q[:, 3], u2 + c2b / 3.0)
+        np.add.at(out, q[:, 0], -(u1 + w1 + u2 + w2) + (c1b + c2b) / 3.0)
+    return out
+
+
 def revolution(profile_rz, nphi, keep=None):
     """Rotate a meridian profile [(r0, z0), ...] (bottom to top) about the z axis.
 

@@ -267,6 +267,9 @@ def test_vjp_notebook_derivatives(mhx):
     _, g = mhx.hydrodynamic_coefficients_vjp(body, par, {"added_mass": np.ones(1)})
     assert _dot_tangent(g, sc, []) == pytest.approx(d["dA_dr"], abs=2e-6)
     assert g["wave_frequencies"][0] == pytest.approx(d["dA_domega"], abs=2e-6)
+    # chained through the face properties, the radius direction is the vertices alone
+    gv = mhx.chain_mesh_gradient(m, g)
+    assert float(np.sum(gv * m.vertices)) == pytest.approx(d["dA_dr"], abs=2e-6)
     _, g = mhx.hydrodynamic_coefficients_vjp(body, par, {"excitation_force": np.ones(1, dtype=complex)})
     assert _dot_tangent(g, sc, []) == pytest.approx(d["dReFex_dr"], abs=2e-6)
     assert g["wave_frequencies"][0] == pytest.approx(d["dReFex_domega"], abs=2e-6)
