@@ -137,6 +137,29 @@ void mhx_reset_timing(mhx_ctx* ctx);
  * used for roofline reporting) */
 int mhx_exact_work(mhx_ctx* ctx, int64_t counts[3], double* flops);
 
+/* Multi-body sweep (several FloatingBodies of uneven panel counts, each with its own frequency
+ * list, common headings): the reference's solve(problems::Vector) / hydrodynamic_coefficients
+ * over a list of bodies (solve.jl:75-78, problems_and_results.jl:464-495).  The assembly of
+ * every (body, frequency) item runs as one persistent queue launch per stage (exact table,
+ * then S/D), the LU is batched per body, and the frequencies are processed in as few chunks as
+ * the device-memory budget allows.  Outputs per body as mhx_body_coeffs.  gf: MHX_GF_WU or
+ * MHX_GF_DELHOMMEAU; FP64 throughout.  flags: MHX_SWEEP_GRAPH captures the whole sweep into a
+ * HIP graph on the first call and replays it on later calls with the same bodies and frequency
+ * counts (the frequency values may change).  A body's buffers belong to the sweep until the
+ * next sweep call or mhx_body_destroy. */
+typedef struct {
+    mhx_body* body;
+    int64_t nw;
+    const double* omegas;
+    double* A_out;  /* (ndof, ndof, nw) */
+    double* B_out;
+    double* Fd_out; /* (ndof, nw, nb) complex interleaved */
+    double* Fk_out;
+} mhx_sweep_body;
+#define MHX_SWEEP_GRAPH 1
+int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* bodies, int64_t nbodies, int64_t nb, const double* betas,
+                     int gf, int direct, int flags);
+
 /* Tangent (forward-mode) derivative of hydrodynamic_coefficients (problems_and_results.jl:464-495)
  * along a direction of the inputs: the mesh fields (vertices nv x 3, centers / normals nf x 3,
  * areas nf; NULL = zero), the dof fields ddofs (same layout as dofs; NULL = zero) and the

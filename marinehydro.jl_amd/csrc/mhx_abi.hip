@@ -16,6 +16,8 @@
 
 
 
+struct SweepState;  // multi-body sweep: item queue, device outputs, captured graph
+
 struct PhaseAcc {
     double ms = 0.0;
     int64_t count = 0;
@@ -42,6 +44,9 @@ struct mhx_ctx {
     int lu_mode = MHX_LU_AUTO;
     int64_t refine_steps = 0, mixed_solves = 0, fp64_fallbacks = 0;
     double max_residual = 0.0;
+    // multi-body sweeps: the last sweep's state (kept for graph replay) and the tuner's choices
+    SweepState* sweep = nullptr;
+    std::map<std::string, int> tuned;
 };
 
 struct mhx_body {
@@ -463,6 +468,147 @@ int body_upload(mhx_ctx* ctx, const mhx_mesh* m, int64_t ndof, const double* dof
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// multi-body sweep
+// ---------------------------------------------------------------------------
+struct SweepState {
+    std::vector<mhx_body*> bodies;
+    std::vector<int64_t> nws;        // frequencies per body
+    std::vector<int> per_chunk;      // items per chunk per body
+    int K = 1;                       // chunks
+    int64_t nb = 0;
+    int gf = 0, direct = 0;
+    std::vector<double*> w_all, k_all;  // device, per body: the sweep's frequencies and wavenumbers
+    std::vector<cplx*> F_all, FK_all;   // device, per body: forces of every frequency
+    std::vector<mhx::SweepItem> h_items;
+    mhx::SweepItem* items = nullptr;    // device, all chunks
+    std::vector<int> chunk_first, chunk_count;
+    std::vector<int64_t> chunk_units, chunk_tunits;
+    unsigned long long* counters = nullptr;  // table-queue counters, one per chunk
+    unsigned long long* status = nullptr;    // quadrature status and work tally of the sweep
+    int* sticky = nullptr;                   // first zero pivot per body
+    int rows = 256;                          // tuner's choice of rows per combine workgroup
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void release() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        for (auto& p : w_all) dfree(p);
+        for (auto& p : k_all) dfree(p);
+        for (auto& p : F_all) dfree(p);
+        for (auto& p : FK_all) dfree(p);
+        dfree(items);
+        dfree(counters);
+        dfree(status);
+        dfree(sticky);
+    }
+};
+
+namespace {
+
+void drop_sweep(mhx_ctx* c) {
+    if (!c->sweep) return;
+    c->sweep->release();
+    delete c->sweep;
+    c->sweep = nullptr;
+}
+
+bool sweep_uses(mhx_ctx* c, const mhx_body* b) {
+    if (!c->sweep) return false;
+    for (auto* x : c->sweep->bodies)
+        if (x == b) return true;
+    return false;
+}
+
+// Everything of one sweep on the stream, with no host synchronisation and no memset nodes, so the
+// sequence can be captured into a graph: frequencies from the per-body arrays, right-hand sides,
+// one table launch and one combine launch per chunk for all bodies, then per body the batched
+// LU, the solves and the pressure integration into the per-body force arrays.
+void sweep_enqueue(mhx_ctx* ctx, SweepState& s, bool zero_flags) {
+    hipStream_t st = ctx->stream;
+    const bool exact = s.gf == MHX_GF_DELHOMMEAU;
+    const double g = ctx->g;
+    mhx::launch_zero(st, s.status, 4 * sizeof(unsigned long long));
+    mhx::launch_zero(st, s.sticky, s.bodies.size() * sizeof(int));
+    for (size_t ib = 0; ib < s.bodies.size(); ib++) {
+        mhx_body* b = s.bodies[ib];
+        if (zero_flags)
+            for (int q = 0; q < b->cap_batch; q++)
+                mhx::launch_zero(st, b->work + 16 * b->nR() + b->sW() * q, sizeof(cplx) * mhx::getrs_flag_words(b->n));
+    }
+    for (int c = 0; c < s.K; c++) {
+        for (size_t ib = 0; ib < s.bodies.size(); ib++) {
+            mhx_body* b = s.bodies[ib];
+            const int64_t w0 = (int64_t)c * s.per_chunk[ib];
+            const int bt = (int)std::max<int64_t>(0, std::min<int64_t>(s.per_chunk[ib], s.nws[ib] - w0));
+            if (!bt) continue;
+            hipMemcpyAsync(b->d_ws, s.w_all[ib] + w0, sizeof(double) * bt, hipMemcpyDeviceToDevice, st);
+            hipMemcpyAsync(b->d_ks, s.k_all[ib] + w0, sizeof(double) * bt, hipMemcpyDeviceToDevice, st);
+            mhx::launch_bc(st, b->n, b->geom, b->dofs, b->ndof, b->betas, (int)s.nb, b->d_ws, b->d_ks, bt, g, b->B,
+                           b->nR());
+        }
+        const mhx::SweepItem* it = s.items + s.chunk_first[c];
+        if (exact) {
+            mhx::launch_zero(st, s.counters + c, sizeof(unsigned long long));
+            mhx::launch_queue_table(st, it, s.chunk_count[c], s.chunk_tunits[c], s.counters + c, 1e-8, 1e-8, s.status);
+        }
+        mhx::launch_queue_combine(st, it, s.chunk_count[c], s.chunk_units[c], exact, s.direct, 1, s.rows);
+        for (size_t ib = 0; ib < s.bodies.size(); ib++) {
+            mhx_body* b = s.bodies[ib];
+            const int64_t w0 = (int64_t)c * s.per_chunk[ib];
+            const int bt = (int)std::max<int64_t>(0, std::min<int64_t>(s.per_chunk[ib], s.nws[ib] - w0));
+            if (!bt) continue;
+            const int n = b->n, R = b->ndof + (int)s.nb;
+            mhx::launch_zero(st, b->info, sizeof(int) * bt);
+            mhx::getrf_batched(st, n, b->D, n, b->nn(), bt, b->ipiv, n, b->wperm, b->info, nullptr);
+            mhx::launch_sticky_info(st, b->info, bt, s.sticky + ib);
+            cplx* W2 = b->work + 16 * b->nR();
+            if (s.direct) {
+                mhx::launch_matmul_nr(st, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), bt);
+                mhx::getrs_batched(st, 0, n, R, b->D, n, b->nn(), b->ipiv, n, b->perm, n, b->Y, b->nR(), W2, b->sW(), bt,
+                                   &b->trs_epoch);
+            } else {
+                mhx::getrs_batched(st, 0, n, R, b->D, n, b->nn(), b->ipiv, n, b->perm, n, b->B, b->nR(), W2, b->sW(), bt,
+                                   &b->trs_epoch);
+                mhx::launch_matmul_nr(st, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), bt);
+            }
+            if (b->ndof > 0)
+                mhx::launch_forces(st, n, b->geom, b->dofs, b->ndof, b->Y, b->nR(), R, ctx->rho, b->d_ws, b->d_ks, bt, g,
+                                   b->betas, (int)s.nb, s.F_all[ib] + w0 * b->ndof * R,
+                                   s.FK_all[ib] + w0 * b->ndof * (s.nb ? s.nb : 1));
+        }
+    }
+}
+
+// rows per combine workgroup: time both shapes once per workload signature on the first chunk
+// (the outputs are rewritten by the sweep itself) and keep the faster
+int sweep_tune(mhx_ctx* ctx, SweepState& s, const std::string& key) {
+    auto f = ctx->tuned.find(key);
+    if (f != ctx->tuned.end()) return f->second;
+    hipStream_t st = ctx->stream;
+    const bool exact = s.gf == MHX_GF_DELHOMMEAU;
+    hipEvent_t e[3];
+    for (auto& x : e) hipEventCreate(&x);
+    const int cand[2] = {128, 256};
+    float ms[2] = {0.f, 0.f};
+    for (int v = 0; v < 2; v++) {  // warm, then time
+        mhx::launch_queue_combine(st, s.items, s.chunk_count[0], s.chunk_units[0], exact, s.direct, 1, cand[v]);
+        hipEventRecord(e[0], st);
+        mhx::launch_queue_combine(st, s.items, s.chunk_count[0], s.chunk_units[0], exact, s.direct, 1, cand[v]);
+        hipEventRecord(e[1], st);
+        hipEventSynchronize(e[1]);
+        hipEventElapsedTime(&ms[v], e[0], e[1]);
+    }
+    for (auto& x : e) hipEventDestroy(x);
+    const int best = ms[0] < ms[1] ? cand[0] : cand[1];
+    ctx->tuned[key] = best;
+    return best;
+}
+
+}  // namespace
+
 extern "C" {
 
 int mhx_create(mhx_ctx** out, const mhx_options* opt) {
@@ -498,6 +644,7 @@ void mhx_destroy(mhx_ctx* c) {
     if (!c) return;
     flush_timing(c);
     if (c->stream) hipStreamSynchronize(c->stream);
+    drop_sweep(c);
     dfree(c->surr_params); dfree(c->surr_B); dfree(c->surr_bias); dfree(c->surr_W);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -521,6 +668,10 @@ int mhx_body_create(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const doub
 
 void mhx_body_destroy(mhx_body* b) {
     if (!b) return;
+    if (b->ctx && sweep_uses(b->ctx, b)) {
+        hipStreamSynchronize(b->ctx->stream);
+        drop_sweep(b->ctx);
+    }
     dfree(b->geom); dfree(b->dofs); dfree(b->SR); dfree(b->DR[0]); dfree(b->DR[1]);
     free_work(b);
     delete b;
@@ -1065,6 +1216,230 @@ int mhx_hydro_coeffs_vjp(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const
     }
     flush_timing(ctx);
     cleanup();
+    return rc;
+}
+
+// Several bodies of uneven panel counts, each with its own frequency list, in one sweep: the
+// assembly of every (body, frequency) item runs as one queue launch per stage, the LU is batched
+// per body, and with MHX_SWEEP_GRAPH the whole sequence is captured once and replayed by later
+// calls with the same bodies and frequency counts.
+int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, int64_t nb, const double* betas, int gf,
+                     int direct, int flags) {
+    if (!ctx) return MHX_EINVAL;
+    if (!in || nbodies <= 0) return fail(ctx, MHX_EINVAL, "sweep: no bodies");
+    if (gf != MHX_GF_WU && gf != MHX_GF_DELHOMMEAU)
+        return fail(ctx, MHX_EINVAL, "sweep: the multi-body sweep runs GFWu or ExactGuevelDelhommeau");
+    if (nb < 0 || (nb && !betas)) return fail(ctx, MHX_EINVAL, "sweep: bad heading array");
+    if (flags & ~MHX_SWEEP_GRAPH) return fail(ctx, MHX_EINVAL, "sweep: unknown flag");
+    direct = direct ? 1 : 0;
+    const bool table = gf == MHX_GF_DELHOMMEAU, graph = (flags & MHX_SWEEP_GRAPH) != 0;
+    for (int64_t ib = 0; ib < nbodies; ib++) {
+        const mhx_sweep_body& e = in[ib];
+        if (!e.body || e.body->ctx != ctx) return fail(ctx, MHX_EINVAL, "sweep: body is null or from another context");
+        for (int64_t jb = 0; jb < ib; jb++)
+            if (in[jb].body == e.body) return fail(ctx, MHX_EINVAL, "sweep: a body is listed twice");
+        const int64_t R = e.body->ndof + nb;
+        if (R <= 0 || R > 16) return fail(ctx, MHX_EDIM, "sweep: ndof + nbeta must be in 1..16 for every body");
+        if (e.body->n > 4096) return fail(ctx, MHX_EDIM, "sweep: more than 4096 panels is not supported");
+        if (e.nw < 0 || (e.nw && !e.omegas)) return fail(ctx, MHX_EINVAL, "sweep: bad frequency array");
+        for (int64_t w = 0; w < e.nw; w++)
+            if (!(e.omegas[w] > 0) || !std::isfinite(e.omegas[w]))
+                return fail(ctx, MHX_EINVAL, "omega must be positive and finite");
+        if (e.body->ndof && e.nw && (!e.A_out || !e.B_out || (nb && (!e.Fd_out || !e.Fk_out))))
+            return fail(ctx, MHX_EINVAL, "sweep: null output");
+    }
+    hipSetDevice(ctx->device);
+    hipStream_t st = ctx->stream;
+    // chunk plan: K chunks, each body contributing ceil(nw / K) items per chunk
+    double total = 0.0;
+    int64_t K = 1;
+    for (int64_t ib = 0; ib < nbodies; ib++) {
+        const double n = in[ib].body->n;
+        total += (n * n * sizeof(cplx) * (table ? 4 : 2) + 20.0 * n * 16 * sizeof(cplx)) * in[ib].nw;
+        K = std::max<int64_t>(K, (in[ib].nw + ctx->max_batch - 1) / ctx->max_batch);
+    }
+    K = std::max<int64_t>(K, (int64_t)std::ceil(total / ctx->batch_bytes));
+    std::vector<int> per(nbodies);
+    int rc;
+    for (int64_t ib = 0; ib < nbodies; ib++) {
+        mhx_body* b = in[ib].body;
+        per[ib] = (int)((in[ib].nw + K - 1) / K);
+        if ((rc = ensure_work(ctx, b, b->ndof + (int)nb, (int)nb, table, std::max(1, per[ib]))) ||
+            (rc = ensure_rankine(ctx, b, direct)))
+            return rc;
+        if (nb && (rc = hipMemcpyAsync(b->betas, betas, sizeof(double) * nb, hipMemcpyHostToDevice, st)) != hipSuccess)
+            return fail(ctx, MHX_EDEVICE, "sweep: heading upload");
+    }
+    // the item queue (pointers into each body's buffers)
+    std::vector<mhx::SweepItem> items;
+    std::vector<int> cfirst, ccount;
+    std::vector<int64_t> cunits, ctunits;
+    for (int64_t c = 0; c < K; c++) {
+        cfirst.push_back((int)items.size());
+        int64_t u = 0, tu = 0;
+        for (int64_t ib = 0; ib < nbodies; ib++) {
+            mhx_body* b = in[ib].body;
+            const int64_t w0 = c * per[ib];
+            const int64_t bt = std::max<int64_t>(0, std::min<int64_t>(per[ib], in[ib].nw - w0));
+            for (int64_t q = 0; q < bt; q++) {
+                mhx::SweepItem it{};
+                it.geom = b->geom;
+                it.SR = b->SR;
+                it.DR = b->DR[direct];
+                it.S = b->S + q * b->nn();
+                it.D = b->D + q * b->nn();
+                it.WG = table ? b->WG + q * b->nn() : nullptr;
+                it.WR = table ? b->WR + q * b->nn() : nullptr;
+                it.k = b->d_ks + q;
+                it.n = b->n;
+                it.unit0 = u;
+                it.tunit0 = tu;
+                u += mhx::sweep_combine_units(b->n);
+                tu += mhx::sweep_table_units(b->n);
+                items.push_back(it);
+            }
+        }
+        ccount.push_back((int)items.size() - cfirst.back());
+        cunits.push_back(u);
+        ctunits.push_back(tu);
+    }
+    std::vector<mhx_body*> bodies(nbodies);
+    std::vector<int64_t> nws(nbodies);
+    for (int64_t ib = 0; ib < nbodies; ib++) {
+        bodies[ib] = in[ib].body;
+        nws[ib] = in[ib].nw;
+    }
+    // graph mode keeps its state (and graph) in the context for replay; an eager sweep uses its own
+    // state and leaves a captured sweep in place
+    SweepState local;
+    SweepState* s = graph ? ctx->sweep : &local;
+    const bool reuse = graph && s && s->exec && s->bodies == bodies && s->nws == nws && s->nb == nb && s->gf == gf &&
+                       s->direct == direct && s->K == K && s->h_items.size() == items.size() &&
+                       std::memcmp(s->h_items.data(), items.data(), items.size() * sizeof(mhx::SweepItem)) == 0;
+    auto abandon = [&]() {
+        if (graph) drop_sweep(ctx);
+        else local.release();
+    };
+    if (!reuse) {
+        if (graph) {
+            drop_sweep(ctx);
+            s = ctx->sweep = new SweepState();
+        }
+        s->bodies = bodies;
+        s->nws = nws;
+        s->per_chunk = per;
+        s->K = (int)K;
+        s->nb = nb;
+        s->gf = gf;
+        s->direct = direct;
+        s->h_items = items;
+        s->chunk_first = cfirst;
+        s->chunk_count = ccount;
+        s->chunk_units = cunits;
+        s->chunk_tunits = ctunits;
+        s->w_all.assign(nbodies, nullptr);
+        s->k_all.assign(nbodies, nullptr);
+        s->F_all.assign(nbodies, nullptr);
+        s->FK_all.assign(nbodies, nullptr);
+        rc = MHX_OK;
+        for (int64_t ib = 0; ib < nbodies && !rc; ib++) {
+            const size_t nw1 = in[ib].nw ? in[ib].nw : 1, nd1 = bodies[ib]->ndof ? bodies[ib]->ndof : 1;
+            if ((rc = dalloc(ctx, &s->w_all[ib], nw1)) || (rc = dalloc(ctx, &s->k_all[ib], nw1)) ||
+                (rc = dalloc(ctx, &s->F_all[ib], nd1 * (bodies[ib]->ndof + nb) * nw1)) ||
+                (rc = dalloc(ctx, &s->FK_all[ib], nd1 * (nb ? nb : 1) * nw1)))
+                break;
+        }
+        if (rc || (rc = dalloc(ctx, &s->items, items.size())) || (rc = dalloc(ctx, &s->counters, (size_t)K)) ||
+            (rc = dalloc(ctx, &s->status, 4)) || (rc = dalloc(ctx, &s->sticky, (size_t)nbodies))) {
+            abandon();
+            return rc;
+        }
+        hipMemcpyAsync(s->items, items.data(), sizeof(mhx::SweepItem) * items.size(), hipMemcpyHostToDevice, st);
+    }
+    // this call's frequencies (outside any graph: a replay reads them from these arrays)
+    for (int64_t ib = 0; ib < nbodies; ib++) {
+        if (!in[ib].nw) continue;
+        std::vector<double> k(in[ib].nw);
+        for (int64_t w = 0; w < in[ib].nw; w++) k[w] = in[ib].omegas[w] * in[ib].omegas[w] / ctx->g;  // waves.jl:133-134
+        hipMemcpyAsync(s->w_all[ib], in[ib].omegas, sizeof(double) * in[ib].nw, hipMemcpyHostToDevice, st);
+        hipMemcpyAsync(s->k_all[ib], k.data(), sizeof(double) * in[ib].nw, hipMemcpyHostToDevice, st);
+        if (hipStreamSynchronize(st) != hipSuccess) return fail(ctx, MHX_EDEVICE, "sweep: frequency upload");
+    }
+    if (!reuse && !items.empty()) {
+        std::string key = std::string(table ? "exact" : "wu") + (direct ? "/d" : "/i");
+        for (int64_t ib = 0; ib < nbodies; ib++) key += "/" + std::to_string(bodies[ib]->n) + "x" + std::to_string(per[ib]);
+        // wavenumbers of chunk 0 in place for the tuner's trial launches
+        for (int64_t ib = 0; ib < nbodies; ib++)
+            if (per[ib] && in[ib].nw)
+                hipMemcpyAsync(bodies[ib]->d_ks, s->k_all[ib], sizeof(double) * std::min<int64_t>(per[ib], in[ib].nw),
+                               hipMemcpyDeviceToDevice, st);
+        s->rows = sweep_tune(ctx, *s, key);
+    }
+    hipError_t e = hipSuccess;
+    if (graph) {
+        if (!reuse) {
+            for (auto* b : bodies) b->trs_epoch = 0;  // the graph zeroes the flags first; epochs restart at 1
+            e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
+            if (e == hipSuccess) {
+                sweep_enqueue(ctx, *s, true);
+                e = hipStreamEndCapture(st, &s->graph);
+            }
+            if (e == hipSuccess) e = hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0);
+            if (e != hipSuccess) {
+                abandon();
+                return fail(ctx, MHX_EDEVICE, std::string("sweep: graph capture: ") + hipGetErrorString(e));
+            }
+        }
+        Phase ph(ctx, "sweep");
+        e = hipGraphLaunch(s->exec, st);
+    } else {
+        Phase ph(ctx, "sweep");
+        sweep_enqueue(ctx, *s, false);
+        e = hipGetLastError();
+    }
+    if (e != hipSuccess) {
+        abandon();
+        return fail(ctx, MHX_EDEVICE, std::string("sweep: launch: ") + hipGetErrorString(e));
+    }
+    // statuses, then the coefficients
+    unsigned long long stq[4] = {0, 0, 0, 0};
+    std::vector<int> sticky(nbodies, 0), herr(nbodies, 0);
+    hipMemcpyAsync(stq, s->status, sizeof stq, hipMemcpyDeviceToHost, st);
+    hipMemcpyAsync(sticky.data(), s->sticky, sizeof(int) * nbodies, hipMemcpyDeviceToHost, st);
+    for (int64_t ib = 0; ib < nbodies; ib++)
+        hipMemcpyAsync(&herr[ib], mhx::getrs_error_word(bodies[ib]->n, bodies[ib]->work + 16 * bodies[ib]->nR()),
+                       sizeof(int), hipMemcpyDeviceToHost, st);
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) {
+        abandon();
+        return fail(ctx, MHX_EDEVICE, std::string("sweep: ") + hipGetErrorString(e));
+    }
+    flush_timing(ctx);
+    for (int q = 0; q < 3; q++) ctx->quad_work[q] += stq[q + 1];
+    rc = MHX_OK;
+    if (stq[0]) rc = fail(ctx, MHX_EQUAD, "exact Delhommeau quadrature exceeded its interval budget");
+    for (int64_t ib = 0; ib < nbodies && !rc; ib++) {
+        if (herr[ib]) rc = fail(ctx, MHX_EDEVICE, "solve: triangular sweep gave up waiting for a block");
+        else if (sticky[ib]) rc = fail(ctx, MHX_ESINGULAR, "LU: exactly zero pivot at column " + std::to_string(sticky[ib]));
+    }
+    for (int64_t ib = 0; ib < nbodies && !rc; ib++) {
+        mhx_body* b = bodies[ib];
+        const int ndof = b->ndof, R = ndof + (int)nb, nw = (int)in[ib].nw;
+        if (!ndof || !nw) continue;
+        std::vector<cplx> F((size_t)ndof * R * nw), FK((size_t)ndof * (nb ? nb : 1) * nw);
+        hipMemcpyAsync(F.data(), s->F_all[ib], sizeof(cplx) * F.size(), hipMemcpyDeviceToHost, st);
+        if (nb) hipMemcpyAsync(FK.data(), s->FK_all[ib], sizeof(cplx) * ndof * nb * nw, hipMemcpyDeviceToHost, st);
+        if (hipStreamSynchronize(st) != hipSuccess) {
+            rc = fail(ctx, MHX_EDEVICE, "sweep: copy back");
+            break;
+        }
+        for (int q = 0; q < nw; q++) {
+            std::vector<cplx> Fq(F.begin() + (size_t)q * ndof * R, F.begin() + (size_t)(q + 1) * ndof * R);
+            std::vector<cplx> FKq(FK.begin() + (size_t)q * ndof * nb, FK.begin() + (size_t)(q + 1) * ndof * nb);
+            pack(Fq, FKq, ndof, R, (int)nb, q, nw, in[ib].omegas[q], in[ib].A_out, in[ib].B_out, in[ib].Fd_out,
+                 in[ib].Fk_out);
+        }
+    }
+    if (!graph || rc) abandon();
     return rc;
 }
 

@@ -12,6 +12,8 @@
 // is one coalesced 2 KB (real) or 4 KB (complex) segment.
 #include "mhx_internal.h"
 
+#include <algorithm>
+
 namespace mhx {
 
 #define TJ 8
@@ -243,6 +245,208 @@ void launch_combine_table(hipStream_t st, int n, const PanelGeom* geom, const do
                           const cplx* WR, cplx* S, cplx* D, int64_t sMat) {
     dim3 grid((n + 255) / 256, (n + TJ - 1) / TJ, batch);
     combine_table_kernel<<<grid, 256, 0, st>>>(n, geom, SR, DR, ks, direct, include_identity, WG, WR, S, D, sMat);
+}
+
+// ---------------------------------------------------------------------------
+// multi-body sweep: every (body, frequency) item of a sweep in one launch per stage, over a
+// queue of equal-cost units (combine: 256 rows x TJ columns; table: 16 x 16 pair blocks of the
+// upper triangle).  Bodies of different panel counts share the launch.
+// ---------------------------------------------------------------------------
+int64_t sweep_combine_units(int n) { return (int64_t)((n + 255) / 256) * ((n + TJ - 1) / TJ); }
+int64_t sweep_table_units(int n) {
+    const int64_t nb = (n + 15) / 16;
+    return nb * (nb + 1) / 2;
+}
+
+// the item holding unit u (items sorted by their first unit)
+__device__ __forceinline__ int find_item(const SweepItem* __restrict__ items, int nitems, int64_t u, bool table) {
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        const int64_t start = table ? items[mid].tunit0 : items[mid].unit0;
+        if (start <= u) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) queue_table_kernel(const SweepItem* __restrict__ items, int nitems,
+                                                          int64_t total_units, unsigned long long* __restrict__ counter,
+                                                          double rtol, double atol,
+                                                          unsigned long long* __restrict__ status) {
+    __shared__ unsigned long long s_unit;
+    QuadWork wk;
+    const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    for (;;) {
+        if (threadIdx.x == 0) s_unit = atomicAdd(counter, 1ull);
+        __syncthreads();
+        const unsigned long long u = s_unit;
+        __syncthreads();
+        if (u >= (unsigned long long)total_units) break;
+        const int it = find_item(items, nitems, (int64_t)u, true);
+        const SweepItem& item = items[it];
+        // local unit -> block (bi, bj), bi <= bj, column-major over the upper triangle
+        const int64_t local = (int64_t)u - item.tunit0;
+        int64_t bj = (int64_t)((sqrt(8.0 * (double)local + 1.0) - 1.0) * 0.5);
+        while (bj * (bj + 1) / 2 > local) bj--;
+        while ((bj + 1) * (bj + 2) / 2 <= local) bj++;
+        const int64_t bi = local - bj * (bj + 1) / 2;
+        const int n = item.n;
+        const int i = (int)bi * 16 + tx, j = (int)bj * 16 + ty;
+        if (i < n && j < n && i <= j) {
+            const PanelGeom& gi = item.geom[i];
+            const PanelGeom& gj = item.geom[j];
+            const double k = *item.k;
+            double hh, vv;
+            reduced_hv(gi.c[0], gi.c[1], gi.c[2], gj.c[0], gj.c[1], gj.c[2], k, hh, vv);
+            cplx Gr, G;
+            if (!hquad_pair(hh, vv, rtol, atol, G, Gr, wk)) atomicOr(status, 1ull);
+            G = cscale(4.0, G);
+            Gr = cscale(4.0, Gr);
+            item.WG[i + (int64_t)n * j] = G;
+            item.WR[i + (int64_t)n * j] = Gr;
+            if (i != j) {
+                item.WG[j + (int64_t)n * i] = G;
+                item.WR[j + (int64_t)n * i] = Gr;
+            }
+        }
+    }
+    unsigned long long e = wk.evals, se = wk.series, cf = wk.cf;
+    for (int off = 32; off > 0; off >>= 1) {
+        e += __shfl_xor(e, off);
+        se += __shfl_xor(se, off);
+        cf += __shfl_xor(cf, off);
+    }
+    if ((threadIdx.x & 63) == 0 && e) {
+        atomicAdd(status + 1, e);
+        atomicAdd(status + 2, se);
+        atomicAdd(status + 3, cf);
+    }
+}
+
+// RG = threads per workgroup (blockDim.x), each unit's 256 rows in 256 / RG passes; chosen per
+// device and workload by the sweep's tuner
+template <bool EXACT, int RG>
+__global__ void __launch_bounds__(RG) queue_combine_kernel(const SweepItem* __restrict__ items, int nitems,
+                                                           int64_t total_units, int direct, int include_identity) {
+    __shared__ PanelGeom sg[TJ];
+#pragma unroll 1
+    for (int64_t u = blockIdx.x; u < total_units; u += gridDim.x) {
+        const int it = find_item(items, nitems, u, false);
+        const SweepItem& item = items[it];
+        const int n = item.n;
+        const int nrb = (n + 255) / 256;  // units are 256 rows x TJ columns whatever RG is
+        const int64_t local = u - item.unit0;
+        const int rb = (int)(local % nrb), cb = (int)(local / nrb);
+        const int j0 = cb * TJ;
+        __syncthreads();  // the previous unit's columns are no longer read
+        {
+            const double* src = reinterpret_cast<const double*>(item.geom + j0);
+            double* dst = reinterpret_cast<double*>(sg);
+            const int cnt = min(TJ, n - j0) * 32;
+            for (int t = threadIdx.x; t < cnt; t += RG) dst[t] = src[t];
+        }
+        __syncthreads();
+        const double k = *item.k;
+        const int jn = min(TJ, n - j0);
+#pragma unroll 1
+        for (int r0 = 0; r0 < 256; r0 += RG) {
+            const int i = rb * 256 + r0 + threadIdx.x;
+            if (i >= n) break;
+            const PanelGeom& gi = item.geom[i];
+            const double xr[3] = {gi.c[0], gi.c[1], gi.c[2]}, nr[3] = {gi.nrm[0], gi.nrm[1], gi.nrm[2]};
+    #pragma unroll 1
+            for (int jj = 0; jj < jn; jj++) {
+                const PanelGeom& g = sg[jj];
+                const int j = j0 + jj;
+                const int64_t idx = i + (int64_t)n * j;
+                double hh, vv;
+                reduced_hv(xr[0], xr[1], xr[2], g.c[0], g.c[1], g.c[2], k, hh, vv);
+                cplx Sv, Dv;
+                if (EXACT) {
+                    combine_entry(k, hh, vv, xr, nr, g, direct, include_identity && i == j, item.WG[idx], item.WR[idx],
+                                  item.SR[idx], item.DR[idx], Sv, Dv);
+                } else {
+                    // as combine_wu_kernel: Wu's own dG/dz
+                    cplx G, Gh, Gv;
+                    wu_term(hh, vv, G, Gh, Gv);
+                    const double dx1 = direct ? g.c[0] - xr[0] : xr[0] - g.c[0];
+                    const double dx2 = direct ? g.c[1] - xr[1] : xr[1] - g.c[1];
+                    double dr1 = 0.0, dr2 = 0.0;
+                    if (fabs(hh) > 1e-6) {
+                        dr1 = k * k / hh * dx1;
+                        dr2 = k * k / hh * dx2;
+                    }
+                    const double n0 = direct ? g.nrm[0] : nr[0], n1 = direct ? g.nrm[1] : nr[1], n2 = direct ? g.nrm[2] : nr[2];
+                    const cplx g0 = cscale(k, cscale(dr1, Gh)), g1 = cscale(k, cscale(dr2, Gh)), g2 = cscale(k, cscale(k, Gv));
+                    const cplx sw = cscale(g.area, cscale(k, G));
+                    const cplx dw = cscale(g.area, cadd(cadd(cscale(n0, g0), cscale(n1, g1)), cscale(n2, g2)));
+                    double dr = item.DR[idx];
+                    if (include_identity && i == j) dr += 0.5;
+                    Sv = cmk(item.SR[idx] + kScale * sw.x, kScale * sw.y);
+                    Dv = cmk(dr + kScale * dw.x, kScale * dw.y);
+                }
+                item.S[idx] = Sv;
+                item.D[idx] = Dv;
+            }
+        }
+    }
+}
+
+static int device_cus() {
+    int dev = 0, cus = 0;
+    hipGetDevice(&dev);
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+    return cus;
+}
+
+void launch_queue_table(hipStream_t st, const SweepItem* items, int nitems, int64_t total_units,
+                        unsigned long long* counter, double rtol, double atol, unsigned long long* status) {
+    if (total_units <= 0) return;
+    const int64_t grid = std::min<int64_t>(total_units, (int64_t)device_cus() * 8);
+    queue_table_kernel<<<(unsigned)grid, 256, 0, st>>>(items, nitems, total_units, counter, rtol, atol, status);
+}
+
+void launch_queue_combine(hipStream_t st, const SweepItem* items, int nitems, int64_t total_units, bool exact,
+                          int direct, int include_identity, int rows_per_group) {
+    if (total_units <= 0) return;
+    const int64_t grid = std::min<int64_t>(total_units, (int64_t)device_cus() * 16);
+    if (rows_per_group == 128) {
+        if (exact)
+            queue_combine_kernel<true, 128><<<(unsigned)grid, 128, 0, st>>>(items, nitems, total_units, direct, include_identity);
+        else
+            queue_combine_kernel<false, 128><<<(unsigned)grid, 128, 0, st>>>(items, nitems, total_units, direct, include_identity);
+    } else {
+        if (exact)
+            queue_combine_kernel<true, 256><<<(unsigned)grid, 256, 0, st>>>(items, nitems, total_units, direct, include_identity);
+        else
+            queue_combine_kernel<false, 256><<<(unsigned)grid, 256, 0, st>>>(items, nitems, total_units, direct, include_identity);
+    }
+}
+
+__global__ void zero_words_kernel(unsigned* __restrict__ p, int64_t nwords) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t < nwords) p[t] = 0u;
+}
+
+void launch_zero(hipStream_t st, void* p, size_t nbytes) {
+    const int64_t nw = (int64_t)(nbytes / 4);
+    if (nw <= 0) return;
+    zero_words_kernel<<<(unsigned)((nw + 255) / 256), 256, 0, st>>>(reinterpret_cast<unsigned*>(p), nw);
+}
+
+__global__ void sticky_info_kernel(const int* __restrict__ info, int batch, int* __restrict__ sticky) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (sticky[0]) return;
+    for (int q = 0; q < batch; q++)
+        if (info[q]) {
+            sticky[0] = info[q];
+            return;
+        }
+}
+
+void launch_sticky_info(hipStream_t st, const int* info, int batch, int* sticky) {
+    sticky_info_kernel<<<1, 64, 0, st>>>(info, batch, sticky);
 }
 
 // ---------------------------------------------------------------------------

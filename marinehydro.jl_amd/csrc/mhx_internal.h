@@ -30,6 +30,34 @@ void launch_combine_table(hipStream_t st, int n, const PanelGeom* geom, const do
 
 // ---- right-hand sides, products, forces (mhx_assembly.hip)
 // B[:, r] for r < ndof: -i omega (n . dof_r);  B[:, ndof+b]: Airy diffraction BC for heading b
+// One (body, frequency) item of a multi-body sweep: the body's geometry and Rankine cache,
+// the item's output matrices and a device pointer to its wavenumber (read at run time, so a
+// replayed sweep picks up new frequencies).  unit0 / tunit0: the item's first unit in the combine
+// and table queues (prefix sums over the items).
+struct SweepItem {
+    const PanelGeom* geom;
+    const double* SR;
+    const double* DR;
+    cplx *S, *D, *WG, *WR;
+    const double* k;
+    int64_t unit0, tunit0;
+    int n, pad;
+};
+// combine units of an item (256 rows x 8 columns) and table units (16 x 16 blocks, upper triangle)
+int64_t sweep_combine_units(int n);
+int64_t sweep_table_units(int n);
+// exact (G, dG/dr) tables of every item in one launch; workgroups take units from *counter
+// (zeroed by the caller on the stream)
+void launch_queue_table(hipStream_t st, const SweepItem* items, int nitems, int64_t total_units,
+                        unsigned long long* counter, double rtol, double atol, unsigned long long* status);
+// S, D of every item in one launch (exact: from the tables, else the Wu fit)
+void launch_queue_combine(hipStream_t st, const SweepItem* items, int nitems, int64_t total_units, bool exact,
+                          int direct, int include_identity, int rows_per_group);
+// zero nbytes (a multiple of 4) with a kernel, so that the operation can sit in a captured graph
+void launch_zero(hipStream_t st, void* p, size_t nbytes);
+// sticky[0] = first non-zero of info[0..batch) if sticky[0] is still zero
+void launch_sticky_info(hipStream_t st, const int* info, int batch, int* sticky);
+
 void launch_bc(hipStream_t st, int n, const PanelGeom* geom, const double* dofs, int ndof, const double* betas, int nb,
                const double* ws, const double* ks, int batch, double g, cplx* B, int64_t sB);
 // Y = S X (n x n times n x R); split-K partials in work (16 * n * R complex per item, stride sW)
@@ -56,6 +84,8 @@ int getrf_batched(hipStream_t st, int n, cplx* A, int lda, int64_t strideA, int 
 // counter (start at 0 with zeroed work).  getrs_error_word: nonzero after a sweep gave up waiting.
 int64_t getrs_work(int n, int R);
 int* getrs_error_word(int n, cplx* work);
+// complex words at the start of an item's getrs work that hold its flags and error word
+int64_t getrs_flag_words(int n);
 int getrs_batched(hipStream_t st, int trans, int n, int R, const cplx* LU, int lda, int64_t sA, const int* ipiv,
                   int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
                   int batch, int* epoch, bool prepared = false);

@@ -832,6 +832,7 @@ static void trs_pass(hipStream_t st, const C* A, int lda, int64_t sA, int n, int
 // be zero when the work memory is first used; *epoch (host counter, one per work buffer) is
 // advanced by every sweep.
 static int64_t flag_words(int n) { return ((n + TRS_B - 1) / TRS_B + 1 + 3) / 4; }
+int64_t getrs_flag_words(int n) { return flag_words(n); }
 
 int64_t getrs_work(int n, int R) {
     const int64_t nblk = (n + TRS_B - 1) / TRS_B;

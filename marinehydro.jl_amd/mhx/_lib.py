@@ -35,7 +35,7 @@ EXPORTED = ["mhx_create", "mhx_destroy", "mhx_last_error", "mhx_set_constants", 
             "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work", "mhx_set_batch", "mhx_surrogate_train",
             "mhx_surrogate_save", "mhx_surrogate_load", "mhx_surrogate_params", "mhx_surrogate_eval",
             "mhx_surrogate_loss_grad", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_hydro_coeffs_jvp",
-            "mhx_hydro_coeffs_vjp"]
+            "mhx_hydro_coeffs_vjp", "mhx_sweep_coeffs"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int64)
@@ -57,6 +57,14 @@ class CMeshTangent(C.Structure):
 
 class CMeshGrad(C.Structure):
     _fields_ = [("vertices", _dp), ("centers", _dp), ("normals", _dp), ("areas", _dp), ("dofs", _dp), ("omegas", _dp)]
+
+
+class CSweepBody(C.Structure):
+    _fields_ = [("body", C.c_void_p), ("nw", C.c_int64), ("omegas", _dp), ("A_out", _dp), ("B_out", _dp),
+                ("Fd_out", _dp), ("Fk_out", _dp)]
+
+
+SWEEP_GRAPH = 1
 
 
 class MHXError(RuntimeError):
@@ -98,6 +106,8 @@ def lib():
                                            C.c_int64, _dp, _dp, C.c_int64, _dp, C.c_int, C.c_int] + [_dp] * 8
         L.mhx_hydro_coeffs_vjp.argtypes = [vp, C.POINTER(CMesh), C.c_int64, _dp, C.c_int64, _dp, C.c_int64, _dp,
                                            C.c_int, C.c_int] + [_dp] * 8 + [C.POINTER(CMeshGrad)]
+        L.mhx_sweep_coeffs.argtypes = [vp, C.POINTER(CSweepBody), C.c_int64, C.c_int64, _dp, C.c_int, C.c_int,
+                                       C.c_int]
         L.mhx_body_create.argtypes = [vp, C.POINTER(CMesh), C.c_int64, _dp, C.POINTER(vp)]
         L.mhx_body_destroy.argtypes = [vp]
         L.mhx_body_destroy.restype = None

@@ -298,20 +298,26 @@ def excitation_force(res):
 
 
 # --------------------------------------------------------------------------- solves
+def _ensure_handle(body, names, ctx):
+    """The body's device-resident copy (geometry, dofs, Rankine cache, work buffers) for this
+    list of dofs, created on first use."""
+    key = tuple(names)
+    if body._handle is None or body._handle_key != key:
+        _release(body)
+        h = _lib.C.c_void_p()
+        cm = _lib.cmesh(body.mesh)
+        dofs = body._dof_stack(names)
+        ctx.check(_lib.lib().mhx_body_create(ctx.h, _lib.C.byref(cm), len(names), _lib.ptr(dofs), _lib.C.byref(h)))
+        body._handle, body._handle_key, body._ctx = h, key, ctx
+    return body._handle
+
+
 def _body_coeffs(body, names, omegas, betas, gf, direct, mode, rho, g, device=0):
     ctx = context(device)
     _ensure_surrogate(gf, device)
     ctx.set_constants(rho, g)
     L = _lib.lib()
-    key = tuple(names)
-    if body._handle is None or body._handle_key != key:
-        _release(body)
-        h = _lib.C.c_void_p()
-        m = body.mesh
-        cm = _lib.cmesh(m)
-        dofs = body._dof_stack(names)
-        ctx.check(L.mhx_body_create(ctx.h, _lib.C.byref(cm), len(names), _lib.ptr(dofs), _lib.C.byref(h)))
-        body._handle, body._handle_key, body._ctx = h, key, ctx
+    _ensure_handle(body, names, ctx)
     nd = len(names)
     om = np.ascontiguousarray(omegas, dtype=np.float64)
     be = np.ascontiguousarray(betas, dtype=np.float64)
@@ -587,6 +593,68 @@ def chain_mesh_gradient(mesh, grads):
     from .meshes import face_properties_vjp
     return grads["vertices"] + face_properties_vjp(mesh.vertices, mesh.faces, grads["centers"], grads["normals"],
                                                    grads["areas"])
+
+
+def _label(A, B, Fd, Fk, ii, rr, nb):
+    out = {}
+    if rr:
+        out["added_mass"] = A[np.ix_(ii, rr)][..., None]
+        out["radiation_damping"] = B[np.ix_(ii, rr)][..., None]
+    else:
+        out["added_mass"] = out["radiation_damping"] = np.zeros(0)
+    if nb:
+        fd, fk = Fd[ii][:, :, :nb, None], Fk[ii][:, :, :nb, None]
+        out.update(diffraction_force=fd, Froude_Krylov_force=fk, excitation_force=fd + fk)
+    else:
+        out.update(diffraction_force=np.zeros(0), Froude_Krylov_force=np.zeros(0), excitation_force=np.zeros(0))
+    return out
+
+
+def hydrodynamic_coefficients_many(jobs, formulation=None, graph=False):
+    """hydrodynamic_coefficients over several bodies at once (the reference's
+    solve(problems::Vector), src/solve.jl:75-78, over every body's problems): `jobs` is a list
+    of (floatingbody, parameters) with each body's own wave_frequencies and dof lists and the
+    same wave_directions for all.  The bodies may have different panel counts; their
+    (body, frequency) items share one assembly queue launch per stage on the device, the LU is
+    batched per body.  graph=True captures the whole sweep into a HIP graph on the first call
+    and replays it when the same bodies and frequency counts come again.  Returns one result
+    dict per job, laid out as hydrodynamic_coefficients."""
+    formulation = DirectBEM() if formulation is None else formulation
+    if formulation.gf not in (_lib.GF_WU, _lib.GF_DELHOMMEAU):
+        raise ValueError("the multi-body sweep runs GFWu or ExactGuevelDelhommeau")
+    if not jobs:
+        return []
+    betas = None
+    ctx = context()
+    ctx.set_constants(SETTINGS.rho, SETTINGS.g)
+    arr = (_lib.CSweepBody * len(jobs))()
+    keep, meta = [], []
+    for q, (body, par) in enumerate(jobs):
+        ws = np.ascontiguousarray(np.atleast_1d(np.asarray(par["wave_frequencies"], dtype=np.float64)))
+        inf = list(par.get("influenced_dofs", body.dof_names))
+        rad = list(par.get("radiating_dofs", []))
+        be = np.ascontiguousarray(par.get("wave_directions", []), dtype=np.float64)
+        if betas is None:
+            betas = be
+        elif be.shape != betas.shape or np.any(be != betas):
+            raise ValueError("every body of a multi-body sweep takes the same wave_directions")
+        for nm in inf + rad:
+            if nm not in body.dofs:
+                raise ValueError("each influenced_dofs Symbol must be a key of floatingbody.dofs")
+        names = inf + [r for r in rad if r not in inf]
+        h = _ensure_handle(body, names, ctx)
+        nd, nw, nb = len(names), ws.size, betas.size
+        A = np.zeros((nd, nd, nw), order="F")
+        B = np.zeros((nd, nd, nw), order="F")
+        Fd = np.zeros((nd, nw, max(nb, 1)), dtype=np.complex128, order="F")
+        Fk = np.zeros((nd, nw, max(nb, 1)), dtype=np.complex128, order="F")
+        arr[q] = _lib.CSweepBody(h, nw, _lib.ptr(ws), _lib.ptr(A), _lib.ptr(B), _lib.ptr(Fd), _lib.ptr(Fk))
+        keep.append((ws, A, B, Fd, Fk))
+        meta.append(([names.index(nm) for nm in inf], [names.index(nm) for nm in rad]))
+    nb = betas.size
+    ctx.check(_lib.lib().mhx_sweep_coeffs(ctx.h, arr, len(jobs), nb, _lib.ptr(betas) if nb else None, formulation.gf,
+                                          int(formulation.direct), _lib.SWEEP_GRAPH if graph else 0))
+    return [_label(A, B, Fd, Fk, ii, rr, nb) for (_, A, B, Fd, Fk), (ii, rr) in zip(keep, meta)]
 
 
 def assemble_matrices(green_functions, mesh, wavenumber, direct=True, include_identity=True, device=0):
