@@ -369,7 +369,10 @@ __global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, 
     const C* A = Ab + sA * bz;
     const C* B = Bb + sB * bz;
     C* Cm = Cb + sC * bz;
-    __shared__ R Ar[16][64], Ai[16][64], Br[16][64], Bi[16][64];
+    // Row strides padded against LDS bank conflicts: the B tile is stored by lanes that run down
+    // k (16 rows) for 4 columns, which with a 64-word row stride put 16 lanes on one bank; 74
+    // spreads them, and 80 puts the four k-rows of an A read on distinct bank groups.
+    __shared__ R Ar[16][80], Ai[16][80], Br[16][74], Bi[16][74];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     const int row0 = blockIdx.x * 64, col0 = blockIdx.y * 64;
@@ -381,28 +384,34 @@ __global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, 
         for (int bj = 0; bj < 2; bj++)
 #pragma unroll
             for (int v = 0; v < 4; v++) t1[bi][bj][v] = t2[bi][bj][v] = t3[bi][bj][v] = 0;
-    for (int k0 = 0; k0 < kk; k0 += 16) {
-        // A tile: 64 rows x 16 k; thread loads 4 elements (rows contiguous)
+    // A tile (64 rows x 16 k, rows contiguous) and B tile (16 k x 64 cols, k contiguous): each
+    // thread holds 4 + 4 elements.  The next chunk's loads are issued before the current chunk's
+    // MFMAs, so their latency hides behind the compute.
+    C ra[4], rb[4];
+    auto load_chunk = [&](int k0) {
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            int idx = tid + 256 * e;
-            int r = idx & 63, k = idx >> 6;
-            C t = tmk<C>(0, 0);
-            if (row0 + r < m && k0 + k < kk) t = A[(row0 + r) + (int64_t)lda * (k0 + k)];
-            Ar[k][r] = SUB ? -t.x : t.x;
-            Ai[k][r] = SUB ? -t.y : t.y;
+            const int idx = tid + 256 * e;
+            const int r = idx & 63, ka = idx >> 6;
+            ra[e] = (row0 + r < m && k0 + ka < kk) ? A[(row0 + r) + (int64_t)lda * (k0 + ka)] : tmk<C>(0, 0);
+            const int kb = idx & 15, c = idx >> 4;
+            rb[e] = (col0 + c < nn && k0 + kb < kk) ? B[(k0 + kb) + (int64_t)ldb * (col0 + c)] : tmk<C>(0, 0);
         }
-        // B tile: 16 k x 64 cols; k contiguous in memory
+    };
+    load_chunk(0);
+    for (int k0 = 0; k0 < kk; k0 += 16) {
 #pragma unroll
         for (int e = 0; e < 4; e++) {
-            int idx = tid + 256 * e;
-            int k = idx & 15, c = idx >> 4;
-            C t = tmk<C>(0, 0);
-            if (col0 + c < nn && k0 + k < kk) t = B[(k0 + k) + (int64_t)ldb * (col0 + c)];
-            Br[k][c] = t.x;
-            Bi[k][c] = t.y;
+            const int idx = tid + 256 * e;
+            const int r = idx & 63, ka = idx >> 6;
+            Ar[ka][r] = SUB ? -ra[e].x : ra[e].x;
+            Ai[ka][r] = SUB ? -ra[e].y : ra[e].y;
+            const int kb = idx & 15, c = idx >> 4;
+            Br[kb][c] = rb[e].x;
+            Bi[kb][c] = rb[e].y;
         }
         __syncthreads();
+        if (k0 + 16 < kk) load_chunk(k0 + 16);
 #pragma unroll
         for (int ks = 0; ks < 16; ks += 4) {
             R ar[2], ai[2], br[2], bi_[2];
