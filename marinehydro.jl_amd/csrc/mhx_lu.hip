@@ -372,7 +372,8 @@ __global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, 
     // Row strides padded against LDS bank conflicts: the B tile is stored by lanes that run down
     // k (16 rows) for 4 columns, which with a 64-word row stride put 16 lanes on one bank; 74
     // spreads them, and 80 puts the four k-rows of an A read on distinct bank groups.
-    __shared__ R Ar[16][80], Ai[16][80], Br[16][74], Bi[16][74];
+    constexpr int KC = 16;  // K chunk (32 raises the registers and halves the occupancy: slower)
+    __shared__ R Ar[KC][80], Ai[KC][80], Br[KC][74], Bi[KC][74];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
     const int row0 = blockIdx.x * 64, col0 = blockIdx.y * 64;
@@ -387,33 +388,34 @@ __global__ void __launch_bounds__(256) zgemm_mfma_kernel(int m, int nn, int kk, 
     // A tile (64 rows x 16 k, rows contiguous) and B tile (16 k x 64 cols, k contiguous): each
     // thread holds 4 + 4 elements.  The next chunk's loads are issued before the current chunk's
     // MFMAs, so their latency hides behind the compute.
-    C ra[4], rb[4];
+    constexpr int EPT = KC / 4;  // elements of each tile per thread
+    C ra[EPT], rb[EPT];
     auto load_chunk = [&](int k0) {
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
+        for (int e = 0; e < EPT; e++) {
             const int idx = tid + 256 * e;
             const int r = idx & 63, ka = idx >> 6;
             ra[e] = (row0 + r < m && k0 + ka < kk) ? A[(row0 + r) + (int64_t)lda * (k0 + ka)] : tmk<C>(0, 0);
-            const int kb = idx & 15, c = idx >> 4;
+            const int kb = idx % KC, c = idx / KC;
             rb[e] = (col0 + c < nn && k0 + kb < kk) ? B[(k0 + kb) + (int64_t)ldb * (col0 + c)] : tmk<C>(0, 0);
         }
     };
     load_chunk(0);
-    for (int k0 = 0; k0 < kk; k0 += 16) {
+    for (int k0 = 0; k0 < kk; k0 += KC) {
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
+        for (int e = 0; e < EPT; e++) {
             const int idx = tid + 256 * e;
             const int r = idx & 63, ka = idx >> 6;
             Ar[ka][r] = SUB ? -ra[e].x : ra[e].x;
             Ai[ka][r] = SUB ? -ra[e].y : ra[e].y;
-            const int kb = idx & 15, c = idx >> 4;
+            const int kb = idx % KC, c = idx / KC;
             Br[kb][c] = rb[e].x;
             Bi[kb][c] = rb[e].y;
         }
         __syncthreads();
-        if (k0 + 16 < kk) load_chunk(k0 + 16);
+        if (k0 + KC < kk) load_chunk(k0 + KC);
 #pragma unroll
-        for (int ks = 0; ks < 16; ks += 4) {
+        for (int ks = 0; ks < KC; ks += 4) {
             R ar[2], ai[2], br[2], bi_[2];
 #pragma unroll
             for (int q = 0; q < 2; q++) {
