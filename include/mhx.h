@@ -100,6 +100,15 @@ int mhx_hydro_coeffs(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const dou
 /* device-resident body (FloatingBody with its rigid dofs, src/bodies.jl:9-51): geometry and the
  * wavenumber-independent Rankine part stay on the device across calls */
 int mhx_body_create(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const double* dofs, mhx_body** body);
+
+/* A body with a lid for irregular-frequency removal (the reference's FloatingBody(mesh, lid_mesh,
+ * ...): solve.jl:4-11, 22-28, bodies.jl:9-28): mesh = hull panels [0, nhull) followed by the lid
+ * panels, dofs over all panels with zero rows on the lid.  Lid panels take no incident-wave
+ * condition, and a panel in the free surface (|z| < 1e-8) has the identity jump 1 instead of
+ * 1/2 (matrix_assembly.jl:25-27); pressures integrate over the hull only (the dofs vanish on the
+ * lid).  Used with mhx_body_coeffs / mhx_sweep_coeffs. */
+int mhx_body_create_lid(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t nhull, int64_t ndof, const double* dofs,
+                        mhx_body** body);
 void mhx_body_destroy(mhx_body* body);
 int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* body, int64_t nw, const double* omegas, int64_t nb, const double* betas,
                     int gf, int direct, int mode, double* A_out, double* B_out, double* Fd_out, double* Fk_out);

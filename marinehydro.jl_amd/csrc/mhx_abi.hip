@@ -666,6 +666,24 @@ int mhx_body_create(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const doub
     return body_upload(ctx, mesh, ndof, dofs, body);
 }
 
+int mhx_body_create_lid(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t nhull, int64_t ndof, const double* dofs,
+                        mhx_body** body) {
+    if (!ctx || !body) return MHX_EINVAL;
+    *body = nullptr;
+    if (!mesh || nhull <= 0 || nhull > mesh->nf) return fail(ctx, MHX_EDIM, "body: nhull must be in 1..nf");
+    hipSetDevice(ctx->device);
+    int rc = body_upload(ctx, mesh, ndof, dofs, body);
+    if (rc) return rc;
+    mhx::launch_mark_lid(ctx->stream, (*body)->n, (int)nhull, (*body)->geom);
+    const hipError_t e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        mhx_body_destroy(*body);
+        *body = nullptr;
+        return fail(ctx, MHX_EDEVICE, std::string("body lid: ") + hipGetErrorString(e));
+    }
+    return MHX_OK;
+}
+
 void mhx_body_destroy(mhx_body* b) {
     if (!b) return;
     if (b->ctx && sweep_uses(b->ctx, b)) {

@@ -38,8 +38,23 @@ __global__ void panel_geometry_kernel(int n, const double* __restrict__ verts, c
     }
     g.radius = radii[j];
     g.area = areas[j];
-    g.pad[0] = g.pad[1] = g.pad[2] = g.pad[3] = 0.0;
+    g.pad[0] = 0.5;  // identity jump
+    g.pad[1] = 1.0;  // hull panel
+    g.pad[2] = g.pad[3] = 0.0;
     geom[j] = g;
+}
+
+// Lid panels (index >= nhull) take no incident-wave condition; with a lid, a panel in the free
+// surface (|z| < 1e-8) has the jump 1 instead of 1/2 (solve.jl:4-11, 22-28; matrix_assembly.jl:25-27).
+__global__ void mark_lid_kernel(int n, int nhull, PanelGeom* __restrict__ geom) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    geom[i].pad[1] = i < nhull ? 1.0 : 0.0;
+    geom[i].pad[0] = fabs(geom[i].c[2]) < 1e-8 ? 1.0 : 0.5;
+}
+
+void launch_mark_lid(hipStream_t st, int n, int nhull, PanelGeom* geom) {
+    mark_lid_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, nhull, geom);
 }
 
 void launch_panel_geometry(hipStream_t st, int n, const double* verts, const int64_t* faces, const double* centers,
@@ -138,7 +153,7 @@ __global__ void __launch_bounds__(256) combine_wu_kernel(int n, const PanelGeom*
         cplx d = cscale(g.area, dsum);
         const int64_t idx = i + (int64_t)n * j;
         double sr = SR[idx], dr = DR[idx];
-        if (include_identity && i == j) dr += 0.5;
+        if (include_identity && i == j) dr += gi.pad[0];
         S[idx] = cmk(sr + kScale * s.x, kScale * s.y);
         D[idx] = cmk(dr + kScale * d.x, kScale * d.y);
     }
@@ -382,7 +397,7 @@ __global__ void __launch_bounds__(RG) queue_combine_kernel(const SweepItem* __re
                     const cplx sw = cscale(g.area, cscale(k, G));
                     const cplx dw = cscale(g.area, cadd(cadd(cscale(n0, g0), cscale(n1, g1)), cscale(n2, g2)));
                     double dr = item.DR[idx];
-                    if (include_identity && i == j) dr += 0.5;
+                    if (include_identity && i == j) dr += g.pad[0];
                     Sv = cmk(item.SR[idx] + kScale * sw.x, kScale * sw.y);
                     Dv = cmk(dr + kScale * dw.x, kScale * dw.y);
                 }
@@ -478,7 +493,7 @@ __global__ void bc_kernel(int n, const PanelGeom* __restrict__ geom, const doubl
         const cplx v0 = cscale(cih, cscale(cb, amp)), v1 = cscale(cih, cscale(sb, amp));
         const cplx v2 = cscale(cih, cmk(amp.y, -amp.x));  // -i * amp
         const cplx dot = cadd(cadd(cscale(nv[0], v0), cscale(nv[1], v1)), cscale(nv[2], v2));
-        B[i + (int64_t)n * (ndof + b)] = cmk(-dot.x, -dot.y);
+        B[i + (int64_t)n * (ndof + b)] = cmk(-dot.x * gi.pad[1], -dot.y * gi.pad[1]);
     }
 }
 

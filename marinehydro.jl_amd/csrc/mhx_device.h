@@ -56,6 +56,8 @@ struct PanelGeom {
     double radius;
     double area;    // mesh area (far field and wave term)
     double nrm[3];  // mesh normal
+    // pad[0]: the identity jump of D's diagonal (1/2; 1 on a panel in the free surface of a body
+    // with a lid), pad[1]: 1 on hull panels, 0 on lid panels (no incident-wave condition there)
     double pad[4];
 };
 static_assert(sizeof(PanelGeom) == 32 * sizeof(double), "PanelGeom must be 256 bytes");
@@ -507,7 +509,7 @@ __device__ __forceinline__ void combine_entry(double k, double hh, double vv, co
     cplx g0 = cscale(k, cscale(dr1, Gh)), g1 = cscale(k, cscale(dr2, Gh)), g2 = cscale(k, cscale(k, Gv));
     cplx s = cscale(a2, cscale(k, G));
     cplx d = cadd(cadd(cscale(n0, cscale(a2, g0)), cscale(n1, cscale(a2, g1))), cscale(n2, cscale(a2, g2)));
-    if (diag) dr += 0.5;
+    if (diag) dr += g.pad[0];
     S = cmk(sr + kScaleD * s.x, kScaleD * s.y);
     D = cmk(dr + kScaleD * d.x, kScaleD * d.y);
 }

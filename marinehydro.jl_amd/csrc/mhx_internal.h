@@ -11,6 +11,7 @@ namespace mhx {
 void launch_panel_geometry(hipStream_t st, int n, const double* verts, const int64_t* faces, const double* centers,
                            const double* normals, const double* areas, const double* radii, PanelGeom* geom);
 // Rankine + free-surface image parts (real, wavenumber independent): SR and D_R (direct) or K_R (indirect)
+void launch_mark_lid(hipStream_t st, int n, int nhull, PanelGeom* geom);
 void launch_rankine(hipStream_t st, int n, const PanelGeom* geom, int direct, double* SR, double* DR);
 // Batched launches: blockIdx.z indexes a frequency; ks / ws are device arrays of
 // wavenumbers / frequencies; matrices of item z start at base + z * stride.

@@ -182,10 +182,13 @@ class FloatingBody:
     """FloatingBody(mesh, dofs, name) with dofs a dict name -> (n, 3) motion, or
     FloatingBody(mesh, ["Heave", ...], rotation_center, name) for rigid dofs."""
 
-    def __init__(self, mesh, dofs, rotation_center=(0.0, 0.0, 0.0), body_name="body"):
+    def __init__(self, mesh, dofs, rotation_center=(0.0, 0.0, 0.0), body_name="body", lid_mesh=None):
         if not isinstance(mesh, Mesh):
             raise TypeError("mesh must be an mhx.meshes.Mesh")
+        if lid_mesh is not None and not isinstance(lid_mesh, Mesh):
+            raise TypeError("lid_mesh must be an mhx.meshes.Mesh")
         self.mesh = mesh
+        self.lid_mesh = lid_mesh  # irregular-frequency removal (bodies.jl:9-28); dofs live on the hull
         self.body_name = body_name
         self._rotation = {}  # rigid rotational dofs: name -> axis (their motion follows the centers)
         if isinstance(dofs, dict):
@@ -305,9 +308,22 @@ def _ensure_handle(body, names, ctx):
     if body._handle is None or body._handle_key != key:
         _release(body)
         h = _lib.C.c_void_p()
-        cm = _lib.cmesh(body.mesh)
-        dofs = body._dof_stack(names)
-        ctx.check(_lib.lib().mhx_body_create(ctx.h, _lib.C.byref(cm), len(names), _lib.ptr(dofs), _lib.C.byref(h)))
+        lid = getattr(body, "lid_mesh", None)
+        if lid is None:
+            cm = _lib.cmesh(body.mesh)
+            dofs = body._dof_stack(names)
+            ctx.check(_lib.lib().mhx_body_create(ctx.h, _lib.C.byref(cm), len(names), _lib.ptr(dofs),
+                                                 _lib.C.byref(h)))
+        else:
+            # hull + lid (solve.jl:22-28), the dofs padded with zero rows on the lid (_pad_hull_bc)
+            from .meshes import combine
+            full = combine(body.mesh, lid)
+            nh, nl = body.mesh.nfaces, lid.nfaces
+            dofs = np.ascontiguousarray(np.concatenate(
+                [np.concatenate([body.dofs[nm], np.zeros((nl, 3))]).ravel(order="F") for nm in names]))
+            cm = _lib.cmesh(full)
+            ctx.check(_lib.lib().mhx_body_create_lid(ctx.h, _lib.C.byref(cm), nh, len(names), _lib.ptr(dofs),
+                                                     _lib.C.byref(h)))
         body._handle, body._handle_key, body._ctx = h, key, ctx
     return body._handle
 
@@ -418,6 +434,8 @@ def hydrodynamic_coefficients_jvp(floatingbody, parameters, tangent, formulation
     formulation = DirectBEM() if formulation is None else formulation
     if formulation.gf not in (_lib.GF_WU, _lib.GF_DELHOMMEAU):
         raise ValueError("derivatives are available for GFWu and ExactGuevelDelhommeau")
+    if getattr(floatingbody, "lid_mesh", None) is not None:
+        raise ValueError("derivatives of a body with a lid are not supported")
     body = floatingbody
     mesh = body.mesh
     ws = np.ascontiguousarray(np.atleast_1d(np.asarray(parameters["wave_frequencies"], dtype=np.float64)))
@@ -508,6 +526,8 @@ def hydrodynamic_coefficients_vjp(floatingbody, parameters, cotangents, formulat
     formulation = DirectBEM() if formulation is None else formulation
     if formulation.gf not in (_lib.GF_WU, _lib.GF_DELHOMMEAU):
         raise ValueError("derivatives are available for GFWu and ExactGuevelDelhommeau")
+    if getattr(floatingbody, "lid_mesh", None) is not None:
+        raise ValueError("derivatives of a body with a lid are not supported")
     body = floatingbody
     mesh = body.mesh
     ws = np.ascontiguousarray(np.atleast_1d(np.asarray(parameters["wave_frequencies"], dtype=np.float64)))

@@ -162,6 +162,41 @@ def face_properties_vjp(vertices, faces, g_centers=None, g_normals=None, g_areas
     return out
 
 
+def combine(mesh1, mesh2):
+    """mesh1 + mesh2 (src/meshes.jl:94-98): vertices concatenated, the second mesh's faces shifted
+    by the first's vertex count, face properties concatenated."""
+    nv = mesh1.vertices.shape[0]
+    return Mesh(np.concatenate([mesh1.vertices, mesh2.vertices]), np.concatenate([mesh1.faces, mesh2.faces + nv]),
+                np.concatenate([mesh1.centers, mesh2.centers]), np.concatenate([mesh1.normals, mesh2.normals]),
+                np.concatenate([mesh1.areas, mesh2.areas]), np.concatenate([mesh1.radii, mesh2.radii]))
+
+
+def disk_lid(radius, nrings, nsectors, z=0.0):
+    """A horizontal disk at height z (the lid of a vertical cylinder or sphere cut at z): nrings
+    rings x nsectors sectors, the centre ring triangles, normals pointing up (+z), as Capytaine's
+    generate_lid orients a lid."""
+    rs = np.linspace(0.0, radius, nrings + 1)
+    # profile from the rim inwards so that the revolution's outward orientation points up
+    prof = [(r, z) for r in rs[::-1]]
+    m = revolution(prof, nsectors)
+    if np.mean(m.normals[:, 2]) < 0:
+        m = Mesh(m.vertices, m.faces[:, ::-1])
+    return m
+
+
+def rect_lid(length, width, nx, ny, z=0.0):
+    """A horizontal rectangle centred on the z axis at height z, nx x ny quads, normals up."""
+    xs = np.linspace(-length / 2, length / 2, nx + 1)
+    ys = np.linspace(-width / 2, width / 2, ny + 1)
+    verts = [(x, y, z) for y in ys for x in xs]
+    faces = []
+    for b in range(ny):
+        for a in range(nx):
+            v0 = b * (nx + 1) + a + 1
+            faces.append([v0, v0 + 1, v0 + nx + 2, v0 + nx + 1])
+    return Mesh(np.array(verts, dtype=np.float64), np.array(faces, dtype=np.int64))
+
+
 def revolution(profile_rz, nphi, keep=None):
     """Rotate a meridian profile [(r0, z0), ...] (bottom to top) about the z axis.
 

@@ -673,8 +673,22 @@ int ora_assemble(const ora_mesh* m, double k, int gf, int direct, int include_id
     return ora_assemble_rows(m, k, gf, direct, include_identity, 0, m->nf, S, D);
 }
 
+/* identity jump of D's diagonal: 1/2 (_add_identity!, matrix_assembly.jl:234-245); with a lid,
+ * the comprehension path's rule: 1 for a panel in the free surface (matrix_assembly.jl:25-27) */
+static double identity_jump(const ora_mesh* m, int64_t i, int surface_rule) {
+    return (surface_rule && fabs(m->centers[3 * i + 2]) < 1e-8) ? 1.0 : 0.5;
+}
+
+static int assemble_rows_impl(const ora_mesh* m, double k, int gf, int direct, int include_identity, int surface_rule,
+                              int64_t i0, int64_t i1, double* S, double* D);
+
 int ora_assemble_rows(const ora_mesh* m, double k, int gf, int direct, int include_identity, int64_t i0,
                       int64_t i1, double* S, double* D) {
+    return assemble_rows_impl(m, k, gf, direct, include_identity, 0, i0, i1, S, D);
+}
+
+static int assemble_rows_impl(const ora_mesh* m, double k, int gf, int direct, int include_identity, int surface_rule,
+                              int64_t i0, int64_t i1, double* S, double* D) {
     int64_t n = m->nf, nr = i1 - i0;
     if (i0 < 0 || i1 > n || nr < 0) return -1;
     ora_geom* g = source_geoms(m);
@@ -686,7 +700,7 @@ int ora_assemble_rows(const ora_mesh* m, double k, int gf, int direct, int inclu
             double sr, dr;
             rankine_entry(m, &g[j], i, j, direct, &sr, &dr);
             cplx sR = sr, dR = dr;
-            if (include_identity && i == j) dR += 0.5;
+            if (include_identity && i == j) dR += identity_jump(m, i, surface_rule);
             cplx sw, dw;
             wave_entry(m, i, j, k, gf, direct, &sw, &dw);
             Sc[(i - i0) + nr * j] = sR + sw;
@@ -888,9 +902,30 @@ void ora_integrate_pressure(const ora_mesh* m, const double* dof, const double* 
 /* All dofs radiate and are influenced; one DiffractionProblem per heading.  */
 /* ------------------------------------------------------------------------- */
 
+static int hydro_coeffs_impl(const ora_mesh* m, int64_t nhull, int surface_rule, int64_t ndof, const double* dofs,
+                             int64_t nw, const double* omegas, int64_t nb, const double* betas, double rho, double g,
+                             int gf, int direct, int mode, double* A, double* B, double* Fdd, double* Fkd);
+
 int ora_hydro_coeffs(const ora_mesh* m, int64_t ndof, const double* dofs, int64_t nw, const double* omegas,
                      int64_t nb, const double* betas, double rho, double g, int gf, int direct, int mode, double* A,
                      double* B, double* Fdd, double* Fkd) {
+    return hydro_coeffs_impl(m, m->nf, 0, ndof, dofs, nw, omegas, nb, betas, rho, g, gf, direct, mode, A, B, Fdd, Fkd);
+}
+
+/* With a lid (solve.jl:4-11, 22-28; bodies.jl:9-28): m = hull panels [0, nhull) + lid panels,
+ * dofs zero on the lid; the hull's boundary conditions are padded with zeros on the lid
+ * (_pad_hull_bc), the assembly uses the free-surface jump rule, and the pressure is integrated
+ * over the hull (the dofs vanish on the lid). */
+int ora_hydro_coeffs_lid(const ora_mesh* m, int64_t nhull, int64_t ndof, const double* dofs, int64_t nw,
+                         const double* omegas, int64_t nb, const double* betas, double rho, double g, int gf,
+                         int direct, int mode, double* A, double* B, double* Fdd, double* Fkd) {
+    if (nhull <= 0 || nhull > m->nf) return -1;
+    return hydro_coeffs_impl(m, nhull, 1, ndof, dofs, nw, omegas, nb, betas, rho, g, gf, direct, mode, A, B, Fdd, Fkd);
+}
+
+static int hydro_coeffs_impl(const ora_mesh* m, int64_t nhull, int surface_rule, int64_t ndof, const double* dofs,
+                             int64_t nw, const double* omegas, int64_t nb, const double* betas, double rho, double g,
+                             int gf, int direct, int mode, double* A, double* B, double* Fdd, double* Fkd) {
     int64_t n = m->nf;
     int64_t R = ndof + nb;
     cplx* Fd = (cplx*)Fdd;
@@ -906,14 +941,17 @@ int ora_hydro_coeffs(const ora_mesh* m, int64_t ndof, const double* dofs, int64_
         double w = omegas[iw];
         double k = w * w / g; /* compute_wavenumber (waves.jl:133-134) */
         for (int64_t r = 0; r < ndof; r++) ora_radiation_bc(m, dofs + 3 * n * r, w, (double*)(bc + n * r));
-        for (int64_t ib = 0; ib < nb; ib++) ora_airy_bc(m, w, k, betas[ib], g, (double*)(bc + n * (ndof + ib)));
+        for (int64_t ib = 0; ib < nb; ib++) {
+            ora_airy_bc(m, w, k, betas[ib], g, (double*)(bc + n * (ndof + ib)));
+            for (int64_t i = nhull; i < n; i++) bc[i + n * (ndof + ib)] = 0.0; /* _pad_hull_bc */
+        }
         if (mode == 0) {
-            ora_assemble(m, k, gf, direct, 1, (double*)S, (double*)D);
+            assemble_rows_impl(m, k, gf, direct, 1, surface_rule, 0, n, (double*)S, (double*)D);
             int e = ora_solve(n, R, (double*)D, (double*)S, (double*)bc, direct, (double*)phi, (double*)src);
             if (e && !info) info = e;
         } else {
             for (int64_t r = 0; r < R; r++) {
-                ora_assemble(m, k, gf, direct, 1, (double*)S, (double*)D);
+                assemble_rows_impl(m, k, gf, direct, 1, surface_rule, 0, n, (double*)S, (double*)D);
                 int e = ora_solve(n, 1, (double*)D, (double*)S, (double*)(bc + n * r), direct,
                                   (double*)(phi + n * r), (double*)(src + n * r));
                 if (e && !info) info = e;

@@ -90,6 +90,9 @@ void ora_integrate_pressure(const ora_mesh* m, const double* dof, const double* 
 int ora_hydro_coeffs(const ora_mesh* m, int64_t ndof, const double* dofs, int64_t nw, const double* omegas,
                      int64_t nb, const double* betas, double rho, double g, int gf, int direct, int mode,
                      double* A, double* B, double* Fd, double* Fk);
+int ora_hydro_coeffs_lid(const ora_mesh* m, int64_t nhull, int64_t ndof, const double* dofs, int64_t nw,
+                         const double* omegas, int64_t nb, const double* betas, double rho, double g, int gf,
+                         int direct, int mode, double* A, double* B, double* Fdd, double* Fkd);
 
 #ifdef __cplusplus
 }

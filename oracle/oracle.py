@@ -57,7 +57,7 @@ def lib():
         L.ora_expE1.argtypes = [C.c_double, C.c_double, _dp]
         L.ora_set_threads.argtypes = [C.c_int]
         for name in ("ora_assemble", "ora_assemble_rows", "ora_assemble_rankine", "ora_assemble_wave",
-                     "ora_hydro_coeffs", "ora_zgetrf", "ora_zgetrs", "ora_solve"):
+                     "ora_hydro_coeffs", "ora_hydro_coeffs_lid", "ora_zgetrf", "ora_zgetrs", "ora_solve"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -251,8 +251,10 @@ def rigid_dof(mesh, which, rc=(0.0, 0.0, 0.0)):
     return dof
 
 
-def hydro_coeffs(mesh, dofs, omegas, betas, rho=1000.0, g=9.81, gf=GF_WU, direct=True, mode=0):
-    """dofs: list of (n,3) arrays.  Returns A, B (ndof, ndof, nw), Fd, Fk (ndof, nw, nb)."""
+def hydro_coeffs(mesh, dofs, omegas, betas, rho=1000.0, g=9.81, gf=GF_WU, direct=True, mode=0, nhull=None):
+    """dofs: list of (n,3) arrays.  Returns A, B (ndof, ndof, nw), Fd, Fk (ndof, nw, nb).
+    nhull: with a lid, the mesh is hull panels [0, nhull) followed by the lid panels and the dofs
+    are zero on the lid (solve.jl:4-11, 22-28)."""
     n = mesh.nf
     ndof = len(dofs)
     dd = np.zeros((n, 3, ndof), order="F")
@@ -264,8 +266,13 @@ def hydro_coeffs(mesh, dofs, omegas, betas, rho=1000.0, g=9.81, gf=GF_WU, direct
     A = np.zeros((ndof, ndof, nw), order="F"); B = np.zeros((ndof, ndof, nw), order="F")
     Fd = np.zeros((ndof, nw, max(nb, 1)), dtype=np.complex128, order="F")
     Fk = np.zeros((ndof, nw, max(nb, 1)), dtype=np.complex128, order="F")
-    info = lib().ora_hydro_coeffs(C.byref(mesh.s), C.c_int64(ndof), _p(dd), C.c_int64(nw), _p(om), C.c_int64(nb),
-                                  _p(be), C.c_double(rho), C.c_double(g), gf, int(direct), int(mode),
-                                  _p(A), _p(B), _p(Fd), _p(Fk))
+    if nhull is None:
+        info = lib().ora_hydro_coeffs(C.byref(mesh.s), C.c_int64(ndof), _p(dd), C.c_int64(nw), _p(om), C.c_int64(nb),
+                                      _p(be), C.c_double(rho), C.c_double(g), gf, int(direct), int(mode),
+                                      _p(A), _p(B), _p(Fd), _p(Fk))
+    else:
+        info = lib().ora_hydro_coeffs_lid(C.byref(mesh.s), C.c_int64(nhull), C.c_int64(ndof), _p(dd), C.c_int64(nw),
+                                          _p(om), C.c_int64(nb), _p(be), C.c_double(rho), C.c_double(g), gf,
+                                          int(direct), int(mode), _p(A), _p(B), _p(Fd), _p(Fk))
     return dict(added_mass=A, radiation_damping=B, diffraction_force=Fd[:, :, :nb],
                 Froude_Krylov_force=Fk[:, :, :nb], excitation_force=(Fd + Fk)[:, :, :nb], info=info)

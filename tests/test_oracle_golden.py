@@ -358,3 +358,50 @@ def test_notebook_derivatives_are_radius_scaling(oracle):
         dr = fd(lambda r: coeffs(r, 1.03, gf, direct), 1.0)
         dw = fd(lambda w: coeffs(1.0, w, gf, direct), 1.03)
         np.testing.assert_allclose([dr[0], dw[0], dr[1]], nb["derivatives_table"][key], rtol=0, atol=1e-4)
+
+
+def _lid_cylinder(oracle, zlid=-0.25):
+    from mhx import meshes
+    hull = meshes.cylinder(1.0, 2.0, 16, 8, 4)          # R = 1, draft 2: side rows of 0.25 m
+    lid = meshes.disk_lid(1.0, 4, 16, z=zlid)            # at the first vertex ring below the waterline
+    full = meshes.combine(hull, lid)
+    nh = hull.nfaces
+
+    def dofs(m, pad):
+        heave = np.zeros((m.nfaces, 3)); heave[:, 2] = 1.0
+        heave[pad:] = 0.0
+        return [heave]
+    return (oracle.Mesh.from_mhx(hull), dofs(hull, hull.nfaces), oracle.Mesh.from_mhx(full), dofs(full, nh), nh)
+
+
+def test_lid_removes_irregular_frequency(oracle):
+    """Lid-based irregular-frequency removal (solve.jl:4-11, 22-28; test/irregular_frequency_removal.jl):
+    a vertical cylinder R = 1, draft 2 has its first irregular frequency where J0(kR) = 0,
+    omega^2 = g k coth(k T), k = 2.405 (omega = 4.858).  Without the lid the heave damping and
+    excitation jump there (damping goes negative); with a lid they are smooth, and away from the
+    irregular frequency both agree to the discretisation level."""
+    oh, dh, of, df, nh = _lid_cylinder(oracle)
+    w_irr = math.sqrt(9.81 * 2.405 / math.tanh(2.405 * 2.0))
+    ws = np.linspace(w_irr - 0.4, w_irr + 0.4, 17)
+    r0 = oracle.hydro_coeffs(oh, dh, ws, [0.0], rho=1025.0)
+    r1 = oracle.hydro_coeffs(of, df, ws, [0.0], rho=1025.0, nhull=nh)
+    b0, b1 = r0["radiation_damping"][0, 0], r1["radiation_damping"][0, 0]
+    f0, f1 = np.abs(r0["excitation_force"][0, :, 0]), np.abs(r1["excitation_force"][0, :, 0])
+    assert b0.min() < 0 and b1.min() > 0                   # the spike drives the lid-less damping negative
+    jump = lambda y: np.max(np.abs(np.diff(y, 2)))           # second differences
+    assert jump(b1) < 0.05 * jump(b0) and jump(f1) < 0.05 * jump(f0)
+    assert np.all(np.diff(b1) < 0) and np.all(np.diff(f1) < 0)   # monotone through the irregular frequency
+    ws2 = [0.8, 1.5, 2.5]
+    r0 = oracle.hydro_coeffs(oh, dh, ws2, [0.0], rho=1025.0)
+    r1 = oracle.hydro_coeffs(of, df, ws2, [0.0], rho=1025.0, nhull=nh)
+    for key, tol in (("added_mass", 1e-4), ("radiation_damping", 3e-3), ("excitation_force", 1e-3)):
+        assert np.max(np.abs(r1[key] - r0[key])) <= tol * np.max(np.abs(r0[key])), key
+
+
+def test_lid_path_without_lid_panels_is_the_plain_path(oracle):
+    """nhull = n (no lid panel, no panel in the free surface): the lid entry reduces to the plain path."""
+    oh, dh, _, _, _ = _lid_cylinder(oracle)
+    a = oracle.hydro_coeffs(oh, dh, [1.1, 2.0], [0.0, 0.5], rho=1025.0)
+    b = oracle.hydro_coeffs(oh, dh, [1.1, 2.0], [0.0, 0.5], rho=1025.0, nhull=oh.nf)
+    for key in ("added_mass", "radiation_damping", "excitation_force"):
+        np.testing.assert_array_equal(a[key], b[key])
