@@ -13,5 +13,6 @@ from .api import (SETTINGS, DiffractionProblem, DirectBEM, ExactGuevelDelhommeau
                   rotational_dofs, set_g, set_rho, solve, solve_linear, translational_dofs,
                   SurrogateGuevelDelhommeau, train_surrogate, load_surrogate,
                   hydrodynamic_coefficients_jvp, hydrodynamic_coefficients_vjp, mesh_scaling_tangent,
-                  chain_mesh_gradient, hydrodynamic_coefficients_many)
+                  chain_mesh_gradient, hydrodynamic_coefficients_many,
+                  combine_floatingbodies, make_body_name_list_unique)
 from .meshes import Mesh  # noqa: F401

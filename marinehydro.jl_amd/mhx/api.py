@@ -219,6 +219,51 @@ class FloatingBody:
         return np.ascontiguousarray(np.concatenate([self.dofs[nm].ravel(order="F") for nm in names]))
 
 
+def make_body_name_list_unique(names):
+    """make_body_name_list_unique (src/bodies.jl:138-161): repeated names get _2, _3, ..."""
+    seen, out = {}, []
+    for s in names:
+        if s not in seen:
+            out.append(s)
+            seen[s] = 2
+        else:
+            count = seen[s]
+            new = f"{s}_{count}"
+            while new in seen or new in names:
+                count += 1
+                new = f"{s}_{count}"
+            out.append(new)
+            seen[s] = count + 1
+            seen[new] = 1
+    return out
+
+
+def combine_floatingbodies(bodies, new_body_name=None):
+    """combine_floatingbodies (src/bodies.jl:165-217): one body whose mesh is the bodies' meshes
+    in order, whose dofs are every body's dofs renamed "body__dof" and padded with zero rows on
+    the other bodies' panels, and whose lid is the combined lids.  The coupled radiation and
+    diffraction problem of the array is then one solve on the combined mesh."""
+    from .meshes import combine_meshes
+    bodies = list(bodies)
+    names = make_body_name_list_unique([b.body_name.replace(" ", "_") for b in bodies])
+    mesh = combine_meshes([b.mesh for b in bodies])
+    ntot = mesh.nfaces
+    dofs, nbf = {}, 0
+    for b, bn in zip(bodies, names):
+        for dn, d in b.dofs.items():
+            full = np.zeros((ntot, 3))
+            full[nbf:nbf + b.mesh.nfaces] = d
+            dofs[f"{bn}__{dn}"] = full
+        nbf += b.mesh.nfaces
+    lids = [b.lid_mesh for b in bodies if getattr(b, "lid_mesh", None) is not None]
+    lid = combine_meshes(lids) if lids else None
+    return FloatingBody(mesh, dofs, body_name=new_body_name or "+".join(names), lid_mesh=lid)
+
+
+FloatingBody.__add__ = lambda self, other: combine_floatingbodies(
+    [self] + (list(other) if isinstance(other, (list, tuple)) else [other]))
+
+
 # --------------------------------------------------------------------------- problems
 @dataclass
 class RadiationProblem:

@@ -162,13 +162,26 @@ def face_properties_vjp(vertices, faces, g_centers=None, g_normals=None, g_areas
     return out
 
 
+def combine_meshes(meshes):
+    """combine_meshes (src/meshes.jl:60-98): vertices concatenated, each mesh's faces shifted by
+    the vertex count before it, face properties concatenated."""
+    meshes = list(meshes)
+    offs = np.cumsum([0] + [m.vertices.shape[0] for m in meshes[:-1]])
+    return Mesh(np.concatenate([m.vertices for m in meshes]),
+                np.concatenate([m.faces + o for m, o in zip(meshes, offs)]),
+                np.concatenate([m.centers for m in meshes]), np.concatenate([m.normals for m in meshes]),
+                np.concatenate([m.areas for m in meshes]), np.concatenate([m.radii for m in meshes]))
+
+
 def combine(mesh1, mesh2):
-    """mesh1 + mesh2 (src/meshes.jl:94-98): vertices concatenated, the second mesh's faces shifted
-    by the first's vertex count, face properties concatenated."""
-    nv = mesh1.vertices.shape[0]
-    return Mesh(np.concatenate([mesh1.vertices, mesh2.vertices]), np.concatenate([mesh1.faces, mesh2.faces + nv]),
-                np.concatenate([mesh1.centers, mesh2.centers]), np.concatenate([mesh1.normals, mesh2.normals]),
-                np.concatenate([mesh1.areas, mesh2.areas]), np.concatenate([mesh1.radii, mesh2.radii]))
+    """mesh1 + mesh2 (src/meshes.jl:94-96)."""
+    return combine_meshes([mesh1, mesh2])
+
+
+def translated(mesh, shift):
+    """The mesh moved by a constant vector (every field that is a position moves)."""
+    s = np.asarray(shift, dtype=np.float64)[None, :]
+    return Mesh(mesh.vertices + s, mesh.faces, mesh.centers + s, mesh.normals, mesh.areas, mesh.radii)
 
 
 def disk_lid(radius, nrings, nsectors, z=0.0):
