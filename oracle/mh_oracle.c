@@ -558,11 +558,21 @@ static ora_geom* source_geoms(const ora_mesh* m) {
 }
 
 /* one Rankine + image entry (assemble_birk_rankine inner loop, matrix_assembly.jl:301-324) */
+/* fixed: the all_normals replacement of the panel normal (forward speed, matrix_assembly.jl:224-232),
+ * NULL for the panel normal */
+static void rankine_entry_n(const ora_mesh* m, const ora_geom* g, int64_t i, int64_t j, int direct,
+                            const double* fixed, double* s_out, double* d_out);
+
 static void rankine_entry(const ora_mesh* m, const ora_geom* g, int64_t i, int64_t j, int direct,
                           double* s_out, double* d_out) {
+    rankine_entry_n(m, g, i, j, direct, NULL, s_out, d_out);
+}
+
+static void rankine_entry_n(const ora_mesh* m, const ora_geom* g, int64_t i, int64_t j, int direct,
+                            const double* fixed, double* s_out, double* d_out) {
     const double* c1 = m->centers + 3 * i;
     const double* c2 = m->centers + 3 * j;
-    const double* nvec = direct ? m->normals + 3 * j : m->normals + 3 * i;
+    const double* nvec = fixed ? fixed : (direct ? m->normals + 3 * j : m->normals + 3 * i);
     double phi, gs[3];
     ora_vrankine_from_geom(c1, c2, m->radii[j], m->areas[j], g->T, g->qgc, g->lc, &phi, gs);
     double s = 0.0, d = 0.0;
@@ -602,11 +612,19 @@ static void reduced_hv(const double* x, const double* xi, double k, double* hh, 
 /* one wave-term entry: Wu (_wu_sd, wu.jl:112-127) or exact Delhommeau
  * (with_reduced_coordinates_derivative, abstract_greens_function.jl:140-185 +
  *  integral_gradient, exact_Guevel_Delhommeau.jl:61-69) */
+static void wave_entry_n(const ora_mesh* m, int64_t i, int64_t j, double k, int gf, int direct, const double* fixed,
+                         cplx* s_out, cplx* d_out);
+
 static void wave_entry(const ora_mesh* m, int64_t i, int64_t j, double k, int gf, int direct, cplx* s_out,
                        cplx* d_out) {
+    wave_entry_n(m, i, j, k, gf, direct, NULL, s_out, d_out);
+}
+
+static void wave_entry_n(const ora_mesh* m, int64_t i, int64_t j, double k, int gf, int direct, const double* fixed,
+                         cplx* s_out, cplx* d_out) {
     const double* x = m->centers + 3 * i;
     const double* xi = m->centers + 3 * j;
-    const double* nvec = direct ? m->normals + 3 * j : m->normals + 3 * i;
+    const double* nvec = fixed ? fixed : (direct ? m->normals + 3 * j : m->normals + 3 * i);
     double a2 = m->areas[j];
     double hh, vv;
     reduced_hv(x, xi, k, &hh, &vv);
@@ -680,15 +698,22 @@ static double identity_jump(const ora_mesh* m, int64_t i, int surface_rule) {
 }
 
 static int assemble_rows_impl(const ora_mesh* m, double k, int gf, int direct, int include_identity, int surface_rule,
-                              int64_t i0, int64_t i1, double* S, double* D);
+                              const double* fixed, int64_t i0, int64_t i1, double* S, double* D);
 
 int ora_assemble_rows(const ora_mesh* m, double k, int gf, int direct, int include_identity, int64_t i0,
                       int64_t i1, double* S, double* D) {
-    return assemble_rows_impl(m, k, gf, direct, include_identity, 0, i0, i1, S, D);
+    return assemble_rows_impl(m, k, gf, direct, include_identity, 0, NULL, i0, i1, S, D);
+}
+
+/* assemble_matrices(...; all_normals = fixed) (matrix_assembly.jl:183-245): the panel normal in
+ * D / K replaced by the fixed vector, the identity jump scaled by (fixed . n_i) */
+int ora_assemble_fixed_normal(const ora_mesh* m, double k, int gf, int direct, const double* fixed, double* S,
+                              double* D) {
+    return assemble_rows_impl(m, k, gf, direct, 1, 0, fixed, 0, m->nf, S, D);
 }
 
 static int assemble_rows_impl(const ora_mesh* m, double k, int gf, int direct, int include_identity, int surface_rule,
-                              int64_t i0, int64_t i1, double* S, double* D) {
+                              const double* fixed, int64_t i0, int64_t i1, double* S, double* D) {
     int64_t n = m->nf, nr = i1 - i0;
     if (i0 < 0 || i1 > n || nr < 0) return -1;
     ora_geom* g = source_geoms(m);
@@ -698,11 +723,15 @@ static int assemble_rows_impl(const ora_mesh* m, double k, int gf, int direct, i
     for (int64_t j = 0; j < n; j++) {
         for (int64_t i = i0; i < i1; i++) {
             double sr, dr;
-            rankine_entry(m, &g[j], i, j, direct, &sr, &dr);
+            rankine_entry_n(m, &g[j], i, j, direct, fixed, &sr, &dr);
             cplx sR = sr, dR = dr;
-            if (include_identity && i == j) dR += identity_jump(m, i, surface_rule);
+            if (include_identity && i == j) {
+                const double* ni = m->normals + 3 * i;
+                const double proj = fixed ? fixed[0] * ni[0] + fixed[1] * ni[1] + fixed[2] * ni[2] : 1.0;
+                dR += identity_jump(m, i, surface_rule) * proj;
+            }
             cplx sw, dw;
-            wave_entry(m, i, j, k, gf, direct, &sw, &dw);
+            wave_entry_n(m, i, j, k, gf, direct, fixed, &sw, &dw);
             Sc[(i - i0) + nr * j] = sR + sw;
             Dc[(i - i0) + nr * j] = dR + dw;
         }
@@ -946,12 +975,12 @@ static int hydro_coeffs_impl(const ora_mesh* m, int64_t nhull, int surface_rule,
             for (int64_t i = nhull; i < n; i++) bc[i + n * (ndof + ib)] = 0.0; /* _pad_hull_bc */
         }
         if (mode == 0) {
-            assemble_rows_impl(m, k, gf, direct, 1, surface_rule, 0, n, (double*)S, (double*)D);
+            assemble_rows_impl(m, k, gf, direct, 1, surface_rule, NULL, 0, n, (double*)S, (double*)D);
             int e = ora_solve(n, R, (double*)D, (double*)S, (double*)bc, direct, (double*)phi, (double*)src);
             if (e && !info) info = e;
         } else {
             for (int64_t r = 0; r < R; r++) {
-                assemble_rows_impl(m, k, gf, direct, 1, surface_rule, 0, n, (double*)S, (double*)D);
+                assemble_rows_impl(m, k, gf, direct, 1, surface_rule, NULL, 0, n, (double*)S, (double*)D);
                 int e = ora_solve(n, 1, (double*)D, (double*)S, (double*)(bc + n * r), direct,
                                   (double*)(phi + n * r), (double*)(src + n * r));
                 if (e && !info) info = e;
@@ -980,5 +1009,112 @@ static int hydro_coeffs_impl(const ora_mesh* m, int64_t nhull, int surface_rule,
         }
     }
     free(S); free(D); free(bc); free(phi); free(src); free(p);
+    return info;
+}
+
+
+/* ------------------------------------------------------------------------- */
+/* forward speed (IndirectBEM; waves.jl:136-151, 183-199, 259-282; solve.jl:35-55;        */
+/* problems_and_results.jl:202-208, 236-380)                                               */
+/* ------------------------------------------------------------------------- */
+
+/* compute_encountered_values (waves.jl:136-151) */
+void ora_encountered(double omega, double beta, double U, double g, double out[3]) {
+    double k = omega * omega / g;
+    if (U == 0) {
+        out[0] = omega; out[1] = k; out[2] = beta;
+        return;
+    }
+    double doppler = omega - k * U * cos(beta);
+    double we = fabs(doppler);
+    out[0] = we;
+    out[1] = we * we / g;
+    out[2] = doppler >= 0 ? beta : beta + PI;
+}
+
+/* Every (U, omega, beta) group: radiation problems of all dofs and the diffraction problem at
+ * heading beta share the encountered (omega_e, k_e).  dgrad: ndof x 3 gradient of motion
+ * (evaluate_gradient_of_motion, bodies.jl:55-66: Pitch (0,0,1), Yaw (0,-1,0), else 0).
+ * Outputs: A, B (ndof, ndof, nw, nU, nb); Fd, Fk (ndof, nw, nb, nU). */
+int ora_hydro_coeffs_fs(const ora_mesh* m, int64_t ndof, const double* dofs, const double* dgrad, int64_t nw,
+                        const double* omegas, int64_t nb, const double* betas, int64_t nU, const double* Us,
+                        double rho, double g, int gf, double* A, double* B, double* Fdd, double* Fkd) {
+    int64_t n = m->nf;
+    int64_t R = ndof + 1;
+    cplx* Fd = (cplx*)Fdd;
+    cplx* Fk = (cplx*)Fkd;
+    cplx* S = (cplx*)malloc(sizeof(cplx) * n * n);
+    cplx* K = (cplx*)malloc(sizeof(cplx) * n * n);
+    cplx* Kx = (cplx*)malloc(sizeof(cplx) * n * n);
+    cplx* bc = (cplx*)malloc(sizeof(cplx) * n * R);
+    cplx* phi = (cplx*)malloc(sizeof(cplx) * n * R);
+    cplx* src = (cplx*)malloc(sizeof(cplx) * n * R);
+    cplx* p = (cplx*)malloc(sizeof(cplx) * n);
+    const double ex[3] = {1.0, 0.0, 0.0};
+    int info = 0;
+    for (int64_t iu = 0; iu < nU; iu++)
+        for (int64_t iw = 0; iw < nw; iw++)
+            for (int64_t ib = 0; ib < nb; ib++) {
+                const double U = Us[iu], w = omegas[iw], beta = betas[ib];
+                const double k = w * w / g;
+                double enc[3];
+                ora_encountered(w, beta, U, g, enc);
+                const double we = enc[0], ke = enc[1], be = enc[2];
+                /* radiation_bc (waves.jl:183-199): -i omega_e (n . dof) + U (n . d dof / dx) */
+                for (int64_t r = 0; r < ndof; r++) {
+                    ora_radiation_bc(m, dofs + 3 * n * r, we, (double*)(bc + n * r));
+                    if (U != 0)
+                        for (int64_t i = 0; i < n; i++) {
+                            const double* nv = m->normals + 3 * i;
+                            const double* gr = dgrad + 3 * r;
+                            bc[i + n * r] += U * (nv[0] * gr[0] + nv[1] * gr[1] + nv[2] * gr[2]);
+                        }
+                }
+                /* _airy_bc (waves.jl:266-282): incident k and omega, phase along the encountered
+                 * heading, velocity along the incident heading */
+                for (int64_t i = 0; i < n; i++) {
+                    const double* c = m->centers + 3 * i;
+                    const double* nv = m->normals + 3 * i;
+                    double wbar = c[0] * cos(be) + c[1] * sin(be);
+                    double cih = exp(k * c[2]);
+                    cplx amp = g * k / w * cexp(I * (k * wbar));
+                    cplx v0 = amp * cos(beta) * cih, v1 = amp * sin(beta) * cih, v2 = (-1.0 * I) * amp * cih;
+                    bc[i + n * ndof] = -(v0 * nv[0] + v1 * nv[1] + v2 * nv[2]);
+                }
+                assemble_rows_impl(m, ke, gf, 0, 1, 0, NULL, 0, n, (double*)S, (double*)K);
+                int e = ora_solve(n, R, (double*)K, (double*)S, (double*)bc, 0, (double*)phi, (double*)src);
+                if (e && !info) info = e;
+                if (U != 0) /* _maybe_forward_speed_pressure (solve.jl:35-45): dphi/dx = K_x sigma */
+                    assemble_rows_impl(m, ke, gf, 0, 1, 0, ex, 0, n, (double*)S, (double*)Kx);
+                for (int64_t r = 0; r < R; r++) {
+                    for (int64_t i = 0; i < n; i++) {
+                        p[i] = I * rho * we * phi[i + n * r]; /* solve.jl:50 */
+                        if (U != 0) {
+                            cplx dphidx = 0;
+                            for (int64_t j = 0; j < n; j++) dphidx += Kx[i + n * j] * src[j + n * r];
+                            p[i] += rho * U * dphidx;
+                        }
+                    }
+                    for (int64_t d = 0; d < ndof; d++) {
+                        double F[2];
+                        ora_integrate_pressure(m, dofs + 3 * n * d, (double*)p, F);
+                        if (r < ndof) { /* problems_and_results.jl:202-208, 341-353 */
+                            const int64_t o = d + ndof * (r + ndof * (iw + nw * (iu + nU * ib)));
+                            A[o] = F[0] / (we * we);
+                            B[o] = F[1] / we;
+                        } else {
+                            Fd[d + ndof * (iw + nw * (ib + nb * iu))] = F[0] + I * F[1];
+                        }
+                    }
+                }
+                /* froude_krylov_force (waves.jl:294-303): incident omega and k, encountered heading */
+                ora_airy_pressure(m, w, k, be, g, rho, (double*)p);
+                for (int64_t d = 0; d < ndof; d++) {
+                    double F[2];
+                    ora_integrate_pressure(m, dofs + 3 * n * d, (double*)p, F);
+                    Fk[d + ndof * (iw + nw * (ib + nb * iu))] = F[0] + I * F[1];
+                }
+            }
+    free(S); free(K); free(Kx); free(bc); free(phi); free(src); free(p);
     return info;
 }

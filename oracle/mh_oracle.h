@@ -93,6 +93,12 @@ int ora_hydro_coeffs(const ora_mesh* m, int64_t ndof, const double* dofs, int64_
 int ora_hydro_coeffs_lid(const ora_mesh* m, int64_t nhull, int64_t ndof, const double* dofs, int64_t nw,
                          const double* omegas, int64_t nb, const double* betas, double rho, double g, int gf,
                          int direct, int mode, double* A, double* B, double* Fdd, double* Fkd);
+int ora_assemble_fixed_normal(const ora_mesh* m, double k, int gf, int direct, const double* fixed, double* S,
+                              double* D);
+void ora_encountered(double omega, double beta, double U, double g, double out[3]);
+int ora_hydro_coeffs_fs(const ora_mesh* m, int64_t ndof, const double* dofs, const double* dgrad, int64_t nw,
+                        const double* omegas, int64_t nb, const double* betas, int64_t nU, const double* Us,
+                        double rho, double g, int gf, double* A, double* B, double* Fdd, double* Fkd);
 
 #ifdef __cplusplus
 }

@@ -57,7 +57,8 @@ def lib():
         L.ora_expE1.argtypes = [C.c_double, C.c_double, _dp]
         L.ora_set_threads.argtypes = [C.c_int]
         for name in ("ora_assemble", "ora_assemble_rows", "ora_assemble_rankine", "ora_assemble_wave",
-                     "ora_hydro_coeffs", "ora_hydro_coeffs_lid", "ora_zgetrf", "ora_zgetrs", "ora_solve"):
+                     "ora_hydro_coeffs", "ora_hydro_coeffs_lid", "ora_zgetrf", "ora_zgetrs", "ora_solve",
+                     "ora_assemble_fixed_normal", "ora_hydro_coeffs_fs"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -183,6 +184,51 @@ def assemble(mesh, k, gf=GF_WU, direct=True, include_identity=True):
                             _p(S), _p(D))
     assert rc == 0
     return S, D
+
+
+def assemble_fixed_normal(mesh, k, fixed, gf=GF_WU, direct=False):
+    """assemble_matrices(...; all_normals=fixed) (matrix_assembly.jl:183-245)."""
+    n = mesh.nf
+    S = np.zeros((n, n), dtype=np.complex128, order="F")
+    D = np.zeros((n, n), dtype=np.complex128, order="F")
+    fx = np.ascontiguousarray(fixed, dtype=np.float64)
+    rc = lib().ora_assemble_fixed_normal(C.byref(mesh.s), C.c_double(k), gf, int(direct), _p(fx), _p(S), _p(D))
+    assert rc == 0
+    return S, D
+
+
+def encountered(omega, beta, U, g=9.81):
+    """compute_encountered_values (waves.jl:136-151): (omega_e, k_e, beta_e)."""
+    out = np.zeros(3)
+    lib().ora_encountered(C.c_double(omega), C.c_double(beta), C.c_double(U), C.c_double(g), _p(out))
+    return tuple(out)
+
+
+def gradient_of_motion(name):
+    """evaluate_gradient_of_motion (bodies.jl:55-66) as the constant per-panel vector."""
+    return {"Pitch": (0.0, 0.0, 1.0), "Yaw": (0.0, -1.0, 0.0)}.get(name, (0.0, 0.0, 0.0))
+
+
+def hydro_coeffs_fs(mesh, dofs, dgrads, omegas, betas, Us, rho=1000.0, g=9.81, gf=GF_WU):
+    """Forward speed, IndirectBEM.  dofs: list of (n,3); dgrads: list of 3-vectors.  Returns
+    A, B (ndof, ndof, nw, nU, nb), Fd, Fk (ndof, nw, nb, nU)."""
+    n = mesh.nf
+    ndof = len(dofs)
+    dd = np.zeros((n, 3, ndof), order="F")
+    for d, m in enumerate(dofs):
+        dd[:, :, d] = m
+    dg = np.ascontiguousarray(np.asarray(dgrads, dtype=np.float64).reshape(ndof, 3))
+    om = np.ascontiguousarray(omegas, dtype=np.float64)
+    be = np.ascontiguousarray(betas, dtype=np.float64)
+    us = np.ascontiguousarray(Us, dtype=np.float64)
+    nw, nb, nu = len(om), len(be), len(us)
+    A = np.zeros((ndof, ndof, nw, nu, nb), order="F"); B = np.zeros_like(A)
+    Fd = np.zeros((ndof, nw, nb, nu), dtype=np.complex128, order="F"); Fk = np.zeros_like(Fd)
+    info = lib().ora_hydro_coeffs_fs(C.byref(mesh.s), C.c_int64(ndof), _p(dd), _p(dg), C.c_int64(nw), _p(om),
+                                     C.c_int64(nb), _p(be), C.c_int64(nu), _p(us), C.c_double(rho), C.c_double(g),
+                                     gf, _p(A), _p(B), _p(Fd), _p(Fk))
+    return dict(added_mass=A, radiation_damping=B, diffraction_force=Fd, Froude_Krylov_force=Fk,
+                excitation_force=Fd + Fk, info=info)
 
 
 def assemble_rows(mesh, k, rows, gf=GF_WU, direct=True, include_identity=True):

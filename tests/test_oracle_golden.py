@@ -405,3 +405,49 @@ def test_lid_path_without_lid_panels_is_the_plain_path(oracle):
     b = oracle.hydro_coeffs(oh, dh, [1.1, 2.0], [0.0, 0.5], rho=1025.0, nhull=oh.nf)
     for key in ("added_mass", "radiation_damping", "excitation_force"):
         np.testing.assert_array_equal(a[key], b[key])
+
+
+def test_fixed_normal_assembly_combines_to_K(oracle):
+    """assemble_matrices(...; all_normals=e_c) (the forward-speed x-derivative operator): by
+    linearity sum_c n_i,c K_c[i, :] = K[i, :] for the indirect K, identity jump included, for
+    both wave terms -- a wrong reflection sign or identity scaling breaks it."""
+    from mhx import meshes
+    m = meshes.sphere(1.0, 8, 16)
+    om = oracle.Mesh.from_mhx(m)
+    for gf in (oracle.GF_WU, oracle.GF_DELHOMMEAU):
+        _, K = oracle.assemble(om, 0.7, gf=gf, direct=False)
+        Ks = [oracle.assemble_fixed_normal(om, 0.7, e, gf=gf)[1] for e in np.eye(3)]
+        comb = sum(m.normals[:, c][:, None] * Ks[c] for c in range(3))
+        assert np.max(np.abs(comb - K)) <= 1e-14 * np.max(np.abs(K))
+
+
+def test_encountered_values(oracle):
+    """compute_encountered_values (waves.jl:136-151): Doppler shift omega - k U cos(beta), its
+    modulus, and the heading turned by pi when it is negative."""
+    w, beta, U, g = 2.0, 0.3, 6.0, 9.81
+    k = w * w / g
+    dop = w - k * U * math.cos(beta)
+    we, ke, be = oracle.encountered(w, beta, U, g)
+    assert dop < 0 and we == pytest.approx(-dop, rel=1e-15) and ke == pytest.approx(dop * dop / g, rel=1e-14)
+    assert be == pytest.approx(beta + math.pi)
+    we, ke, be = oracle.encountered(1.0, 2.0, 0.5, g)
+    assert we == pytest.approx(1.0 - 0.5 / g * math.cos(2.0)) and be == 2.0
+    assert oracle.encountered(1.3, 0.4, 0.0, g) == (1.3, 1.3 ** 2 / g, 0.4)
+
+
+def test_forward_speed_at_rest_is_the_indirect_path(oracle):
+    """forward_speeds = [0]: every heading's radiation problems and the diffraction problem equal
+    the plain IndirectBEM results bit for bit."""
+    from mhx import meshes
+    m = meshes.sphere(1.0, 8, 16)
+    om = oracle.Mesh.from_mhx(m)
+    heave = oracle.rigid_dof(om, 2)
+    pitch = oracle.rigid_dof(om, 4)
+    r0 = oracle.hydro_coeffs(om, [heave, pitch], [0.8, 1.5], [0.0, 1.0], rho=1025.0, direct=False)
+    r1 = oracle.hydro_coeffs_fs(om, [heave, pitch], [oracle.gradient_of_motion("Heave"),
+                                                      oracle.gradient_of_motion("Pitch")],
+                                [0.8, 1.5], [0.0, 1.0], [0.0], rho=1025.0)
+    for ib in range(2):
+        np.testing.assert_array_equal(r1["added_mass"][:, :, :, 0, ib], r0["added_mass"])
+        np.testing.assert_array_equal(r1["radiation_damping"][:, :, :, 0, ib], r0["radiation_damping"])
+    np.testing.assert_array_equal(r1["excitation_force"][:, :, :, 0], r0["excitation_force"])
