@@ -87,6 +87,35 @@ def cpu_baseline(mesh, omegas, nfreq, threads):
                       f"and S.B (numpy's OpenBLAS, {t_lu:.2f} s)"}
 
 
+def cpu_baseline_fs(mesh, U, w, beta, threads):
+    """The oracle on the host cores for one moving (U, omega, beta) group of the forward-speed
+    workload: exact indirect assembly of S, K and of the x-derivative operator K_x at the
+    encountered wavenumber (OpenMP), LAPACK LU of K, sigma for 7 right-hand sides, phi = S sigma and
+    dphi/dx = K_x sigma (numpy's OpenBLAS)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    import scipy.linalg as sl
+    O.build()
+    O.set_threads(threads)
+    om = O.Mesh.from_mhx(mesh)
+    n = mesh.nfaces
+    we, ke, _ = O.encountered(w, beta, U, 9.81)
+    rng = np.random.default_rng(0)
+    rhs = rng.standard_normal((n, 7)) + 1j * rng.standard_normal((n, 7))
+    t0 = time.perf_counter()
+    S, K = O.assemble(om, ke, gf=O.GF_DELHOMMEAU, direct=False)
+    _, Kx = O.assemble_fixed_normal(om, ke, [1.0, 0.0, 0.0], gf=O.GF_DELHOMMEAU, direct=False)
+    t1 = time.perf_counter()
+    sigma = sl.lu_solve(sl.lu_factor(K, check_finite=False), rhs, check_finite=False)
+    S @ sigma
+    Kx @ sigma
+    t2 = time.perf_counter()
+    return {"value": 1.0 / (t2 - t0), "unit": "problem groups/s", "cores": threads, "kind": "port",
+            "sample": f"one moving group (U={U}, omega={w}, beta={beta:.4f}) on {n} panels: exact assembly of S, K "
+                      f"and K_x (oracle, {threads} OpenMP threads, {t1 - t0:.2f} s) + LAPACK zgetrf/zgetrs, "
+                      f"S.sigma and K_x.sigma (numpy's OpenBLAS, {t2 - t1:.2f} s)"}
+
+
 def surrogate_flops_per_pair():
     """Multiply-adds of one network evaluation (64 -> 4 x 128 -> 6), two flops each."""
     return 2.0 * (64 * 128 + 3 * 128 * 128 + 128 * 6)
@@ -114,20 +143,23 @@ def lu_update_flops(n, nb=64):
     return f
 
 
-def measure(ctx, run, ws, n, gf):
+def measure(ctx, run, ws, n, gf, nlu=None, units=None):
     """One timed call over the frequencies ws; returns rate, phase table and roofline of the
-    dominant kernel (per-launch HIP events on the library's stream)."""
+    dominant kernel (per-launch HIP events on the library's stream).  nlu: factorizations the call
+    makes (default one per frequency); units: what the rate counts (default the frequencies)."""
     ctx.reset_timing()
     t0 = time.perf_counter()
     A, B, Fd, Fk = run(ws)
     elapsed = time.perf_counter() - t0
     assert np.all(np.isfinite(A)) and np.all(np.isfinite(Fd))
-    phases = {p: ctx.phase_time(p) for p in ("wave_table", "wave", "lu", "lu_update", "solve", "forces", "rankine")}
+    phases = {p: ctx.phase_time(p) for p in ("wave_table", "wave", "lu", "lu_update", "solve", "forces", "rankine",
+                                             "forward_speed")}
     counts, flops = ctx.exact_work()
     tw, cw = phases["wave_table"]
     tu, cu = phases["lu_update"]
     tl, cl = phases["lu"]
-    nw = len(ws)
+    nw = len(ws) if nlu is None else nlu
+    units = len(ws) if units is None else units
     # the surrogate path factors in FP32 (mixed-precision solver), the others in FP64
     fp32 = gf == "Surrogate"
     lu_peak = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
@@ -163,7 +195,7 @@ def measure(ctx, run, ws, n, gf):
         roof, other = rl_update, rl_table
     lu_tflops = 8.0 / 3.0 * n ** 3 * nw / (tl * 1e-3) / 1e12 if tl else None
     per_w = {k: v[0] / nw for k, v in phases.items()}
-    return nw / elapsed, elapsed, per_w, roof, lu_tflops, (A, B, Fd, Fk), other
+    return units / elapsed, elapsed, per_w, roof, lu_tflops, (A, B, Fd, Fk), other
 
 
 def main():
@@ -176,6 +208,8 @@ def main():
     ap.add_argument("--cpu-freqs", type=int, default=2, help="complete omega-solves timed on the host for cpu_baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-omega comparison run")
+    ap.add_argument("--workload", default="headline", choices=["headline", "forward_speed"],
+                    help="forward_speed: the forward-speed grid on the same body (IndirectBEM, exact or Wu term)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -193,6 +227,8 @@ def main():
     mhx.set_g(9.81)
     names = ["Surge", "Sway", "Heave", "Roll", "Pitch", "Yaw"]
     body = mhx.FloatingBody(mesh, names)
+    if args.workload == "forward_speed":
+        return bench_forward_speed(args, mhx, _lib, ctx, body, names, mesh, omegas)
     form = mhx.DirectBEM(args.gf)
     idx = np.linspace(0, len(omegas) - 1, args.steps).round().astype(int)
     steps_w = omegas[idx]
@@ -257,6 +293,55 @@ def main():
         "solver": solver,
         "plain_gpu_per_omega": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "roofline": roof,
+        "roofline_next_kernel": roof2,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+FS_SPEEDS = [0.0, 2.0]
+FS_HEADINGS = [0.0, np.pi / 6, np.pi / 2, np.pi]
+
+
+def bench_forward_speed(args, mhx, _lib, ctx, body, names, mesh, omegas):
+    """The forward-speed grid: U in {0, 2} m/s, headings 0, pi/6, pi/2, pi, `steps` of the 64
+    frequencies, six rigid dofs, IndirectBEM.  A unit is one (U, omega, beta) problem group (six
+    radiation problems and one diffraction problem); at U = 0 every heading shares one solve per
+    frequency, each moving group has its own encountered frequency, assembly, LU and K_x."""
+    gf = "ExactGuevelDelhommeau" if args.gf == "Surrogate" else args.gf
+    form = mhx.IndirectBEM(gf)
+    n = mesh.nfaces
+    steps = min(args.steps, 16) if args.steps == 64 else args.steps
+    ws = omegas[np.linspace(0, len(omegas) - 1, steps).round().astype(int)]
+    nb, nu = len(FS_HEADINGS), len(FS_SPEEDS)
+    ctx.set_batch(max(steps, 1))
+
+    def run(w):
+        return mhx.api._body_coeffs_fs(body, names, w, FS_HEADINGS, FS_SPEEDS, names, form.gf, False, 1025.0, 9.81)
+
+    log(f"bench: forward speed, n={n}, warmup")
+    run(ws[:1])
+    groups = nu * len(ws) * nb
+    nlu = len(ws) * (1 + nb * sum(1 for u in FS_SPEEDS if u != 0))
+    log(f"bench: timing {groups} problem groups ({nlu} factorizations)")
+    value, elapsed, phases, roof, lu_tflops, res, roof2 = measure(ctx, run, ws, n, gf, nlu=nlu, units=groups)
+    log("per-factorization phase ms:", phases)
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: cpu baseline (one moving group)")
+        cpu = cpu_baseline_fs(mesh, 2.0, float(ws[len(ws) // 2]), np.pi / 6, min(16, os.cpu_count() or 1))
+    out = {
+        "metric": "forward-speed problem groups/s at 4k panels", "value": value, "unit": "problem groups/s",
+        "n_gpus": 1, "steps": groups, "warmup": 1, "ms_per_step": elapsed * 1e3 / groups, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"forward-speed grid, IndirectBEM, wave term {WAVE_DESC[gf]}: capped cylinder R=4 "
+                               f"draft=12, {n} panels, 6 dofs, U in {FS_SPEEDS} m/s, headings {len(FS_HEADINGS)}, "
+                               f"{len(ws)} of the 64 frequencies in 0.1-3.0 rad/s",
+                   "panels": n, "dofs": 6, "headings": nb, "speeds": nu, "frequencies": len(ws),
+                   "factorizations": nlu, "parallelism": "single GPU"},
+        "phases_ms_per_factorization": phases,
+        "lu_tflops": lu_tflops,
         "roofline": roof,
         "roofline_next_kernel": roof2,
         "cpu_baseline": cpu,

@@ -113,6 +113,23 @@ void mhx_body_destroy(mhx_body* body);
 int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* body, int64_t nw, const double* omegas, int64_t nb, const double* betas,
                     int gf, int direct, int mode, double* A_out, double* B_out, double* Fd_out, double* Fk_out);
 
+/* Forward speed (the reference's forward_speeds parameter: problems_and_results.jl:236-380,
+ * waves.jl:136-151, 183-199, 266-303, solve.jl:13-19, 35-55).  For every (U, omega, beta) the
+ * radiation problems of all dofs and the diffraction problem at heading beta are solved at the
+ * encountered frequency omega_e = |omega - k U cos beta| (k = omega^2/g) and heading beta_e (beta,
+ * or beta + pi when the Doppler shift is negative); radiation conditions
+ * -i omega_e (n . dof) + U (n . dgrad), dgrad the constant gradient of motion of a rigid dof
+ * (bodies.jl:55-66: ndof x 3, Pitch (0,0,1), Yaw (0,-1,0), else 0; NULL when every U is 0);
+ * the pressure i rho omega_e phi + rho U dphi/dx with dphi/dx = K_x sigma (the indirect operator
+ * with every panel normal replaced by e_x).  U = 0 is the plain problem (any formulation); a
+ * nonzero U needs direct = 0.  gf: MHX_GF_WU or MHX_GF_DELHOMMEAU (MHX_GF_SURROGATE only when
+ * every U is 0).  Outputs, column-major like the reference's arrays:
+ *   A_out, B_out   ndof x ndof x nw x nU x nb     Re F / omega_e^2, Im F / omega_e
+ *   Fd_out, Fk_out ndof x nw x nb x nU complex    (Froude-Krylov with the incident omega, k and beta_e) */
+int mhx_body_coeffs_fs(mhx_ctx* ctx, mhx_body* body, int64_t nw, const double* omegas, int64_t nb, const double* betas,
+                       int64_t nU, const double* speeds, const double* dgrad, int gf, int direct, double* A_out,
+                       double* B_out, double* Fd_out, double* Fk_out);
+
 /* frequency-chunk limits of MHX_MODE_BATCHED (defaults: 64 items, 60% of the device memory free at
  * mhx_create) */
 int mhx_set_batch(mhx_ctx* ctx, int64_t max_items, double max_bytes);
@@ -135,7 +152,8 @@ int mhx_solver_stats(mhx_ctx* ctx, int64_t counts[3], double* max_residual);
 /* per-phase timing (HIP events on the context's stream, enabled by mhx_options.timing):
  * phase: "rankine", "wave_table" (wave term: exact table, or the fused surrogate pair kernel),
  * "wave" (S, D formation from the table or the Wu fit), "lu" (factorization), "lu_update" (each
- * trailing-update GEMM launch inside "lu", timed separately), "solve", "forces";
+ * trailing-update GEMM launch inside "lu", timed separately), "solve", "forces", "forward_speed"
+ * (K_x, K_x sigma and the potential update of mhx_body_coeffs_fs);
  * returns total ms and launch count since reset */
 int mhx_phase_time(mhx_ctx* ctx, const char* phase, double* total_ms, int64_t* count);
 void mhx_reset_timing(mhx_ctx* ctx);

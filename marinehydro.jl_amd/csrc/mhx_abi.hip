@@ -57,6 +57,7 @@ struct mhx_body {
     double* SR = nullptr;
     double* DR[2] = {nullptr, nullptr};  // [direct] : K_R (indirect) or D_R (direct)
     bool have_rankine[2] = {false, false};
+    double* KRx = nullptr;  // Rankine part of the forward-speed x-derivative operator (built on first use)
     // per-frequency work, cap_batch items
     int cap_batch = 0, cap_R = 0, cap_nb = 0;
     bool cap_table = false;
@@ -690,7 +691,7 @@ void mhx_body_destroy(mhx_body* b) {
         hipStreamSynchronize(b->ctx->stream);
         drop_sweep(b->ctx);
     }
-    dfree(b->geom); dfree(b->dofs); dfree(b->SR); dfree(b->DR[0]); dfree(b->DR[1]);
+    dfree(b->geom); dfree(b->dofs); dfree(b->SR); dfree(b->DR[0]); dfree(b->DR[1]); dfree(b->KRx);
     free_work(b);
     delete b;
 }
@@ -900,6 +901,193 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
             pack(Fq, FKq, ndof, R, (int)nb, w0 + q, nw, omegas[w0 + q], A_out, B_out, Fd_out, Fk_out);
         }
     }
+    flush_timing(ctx);
+    return rc;
+}
+
+// Forward speed.  U = 0: the plain problems, through mhx_body_coeffs (one batched solve per
+// frequency chunk shared by every heading).  U != 0: per (U, beta), the frequencies of a chunk
+// batched, each item at its own encountered (omega_e, k_e): indirect assembly at k_e, FP64 LU,
+// then K_x at k_e written over the dead LU factors, dphi/dx = K_x sigma, and
+// phi <- phi - i (U / omega_e) dphi/dx so that the force kernel's i rho omega_e phi carries the
+// rho U dphi/dx term (solve.jl:35-55).
+int mhx_body_coeffs_fs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas, int64_t nb, const double* betas,
+                       int64_t nU, const double* Us, const double* dgrad, int gf, int direct, double* A_out,
+                       double* B_out, double* Fd_out, double* Fk_out) {
+    if (!ctx || !b) return MHX_EINVAL;
+    if (nw < 0 || nb < 1 || nU < 1 || (nw && !omegas) || !betas || !Us)
+        return fail(ctx, MHX_EINVAL, "forward speed: bad frequency, heading or speed arrays (at least one heading and one speed)");
+    if (gf != MHX_GF_WU && gf != MHX_GF_DELHOMMEAU && gf != MHX_GF_SURROGATE)
+        return fail(ctx, MHX_EINVAL, "unknown Green function");
+    bool moving = false;
+    for (int64_t u = 0; u < nU; u++) {
+        if (!std::isfinite(Us[u])) return fail(ctx, MHX_EINVAL, "forward speed must be finite");
+        if (Us[u] != 0) moving = true;
+    }
+    for (int64_t w = 0; w < nw; w++)
+        if (!(omegas[w] > 0) || !std::isfinite(omegas[w])) return fail(ctx, MHX_EINVAL, "omega must be positive and finite");
+    for (int64_t q = 0; q < nb; q++)
+        if (!std::isfinite(betas[q])) return fail(ctx, MHX_EINVAL, "wave direction must be finite");
+    const int ndof = b->ndof, n = b->n;
+    if (ndof == 0 || nw == 0) return MHX_OK;  // every output is empty
+    if (!A_out || !B_out || !Fd_out || !Fk_out) return fail(ctx, MHX_EINVAL, "coeffs: null output");
+    if (moving) {
+        if (direct)
+            return fail(ctx, MHX_EINVAL,
+                        "nonzero forward speed is not supported with the direct method; use IndirectBEM (direct = false)");
+        if (gf == MHX_GF_SURROGATE)
+            return fail(ctx, MHX_EINVAL, "nonzero forward speed: GFWu or ExactGuevelDelhommeau only");
+        if (!dgrad) return fail(ctx, MHX_EINVAL, "nonzero forward speed: null gradient of motion");
+    }
+    if (ndof + 1 > 16) return fail(ctx, MHX_EDIM, "forward speed: at most 15 dofs");
+    if (n > 4096) return fail(ctx, MHX_EDIM, "coeffs: more than 4096 panels is not supported");
+    hipSetDevice(ctx->device);
+    const size_t nd = ndof, nd2 = nd * nd;
+    auto aidx = [&](int d, int r, int64_t iw, int64_t iu, int64_t ib) {
+        return d + nd * (r + nd * (iw + (size_t)nw * (iu + (size_t)nU * ib)));
+    };
+    auto fidx = [&](int d, int64_t iw, int64_t ib, int64_t iu) {
+        return d + nd * (iw + (size_t)nw * (ib + (size_t)nb * iu));
+    };
+    int rc = MHX_OK;
+    // ---- U = 0: the plain problems, every heading sharing the solve ----
+    for (int64_t iu = 0; iu < nU && !rc; iu++) {
+        if (Us[iu] != 0) continue;
+        const int64_t cb = 16 - ndof;
+        std::vector<double> tA(nd2 * nw), tB(nd2 * nw), tFd(2 * nd * nw * cb), tFk(2 * nd * nw * cb);
+        for (int64_t b0 = 0; b0 < nb && !rc; b0 += cb) {
+            const int64_t bc = std::min<int64_t>(cb, nb - b0);
+            rc = mhx_body_coeffs(ctx, b, nw, omegas, bc, betas + b0, gf, direct, MHX_MODE_BATCHED, tA.data(), tB.data(),
+                                 tFd.data(), tFk.data());
+            if (rc) break;
+            for (int64_t q = 0; q < bc; q++)
+                for (int64_t iw = 0; iw < nw; iw++)
+                    for (int d = 0; d < ndof; d++) {
+                        const size_t src = d + nd * (iw + (size_t)nw * q), dst = fidx(d, iw, b0 + q, iu);
+                        Fd_out[2 * dst] = tFd[2 * src];
+                        Fd_out[2 * dst + 1] = tFd[2 * src + 1];
+                        Fk_out[2 * dst] = tFk[2 * src];
+                        Fk_out[2 * dst + 1] = tFk[2 * src + 1];
+                    }
+        }
+        if (rc) break;
+        for (int64_t ib = 0; ib < nb; ib++)
+            for (int64_t iw = 0; iw < nw; iw++)
+                for (int r = 0; r < ndof; r++)
+                    for (int d = 0; d < ndof; d++) {
+                        const size_t src = d + nd * (r + nd * iw);
+                        A_out[aidx(d, r, iw, iu, ib)] = tA[src];
+                        B_out[aidx(d, r, iw, iu, ib)] = tB[src];
+                    }
+    }
+    if (rc || !moving) return rc;
+    // ---- U != 0 ----
+    const int R = ndof + 1;
+    const bool table = gf == MHX_GF_DELHOMMEAU;
+    const double g = ctx->g;
+    int chunk = 1;
+    {
+        const double per = (double)n * n * sizeof(cplx) * (table ? 4 : 2) + 21.0 * n * 16 * sizeof(cplx);
+        chunk = (int)std::min<double>((double)std::min<int64_t>(nw, ctx->max_batch), std::max(1.0, ctx->batch_bytes / per));
+    }
+    if ((rc = ensure_work(ctx, b, R, 1, table, chunk)) || (rc = ensure_rankine(ctx, b, 0))) return rc;
+    hipStream_t st = ctx->stream;
+    if (!b->KRx) {
+        if ((rc = dalloc(ctx, &b->KRx, (size_t)b->nn()))) return rc;
+        Phase ph(ctx, "rankine");
+        mhx::launch_rankine_x(st, n, b->geom, b->KRx);
+    }
+    double *d_grad = nullptr, *d_inc = nullptr;
+    cplx* psi = nullptr;
+    auto cleanup = [&]() { dfree(d_grad); dfree(d_inc); dfree(psi); };
+    if ((rc = dalloc(ctx, &d_grad, 3 * nd)) || (rc = dalloc(ctx, &d_inc, 3 * (size_t)chunk)) ||
+        (rc = dalloc(ctx, &psi, (size_t)b->nR() * chunk))) {
+        cleanup();
+        return rc;
+    }
+    // every exit of the loops below goes through cleanup (HCHK returns from the lambda)
+    auto run = [&]() -> int {
+        HCHK(hipMemcpyAsync(d_grad, dgrad, sizeof(double) * 3 * nd, hipMemcpyHostToDevice, st));
+        std::vector<double> we(chunk), ke(chunk), inc(3 * (size_t)chunk);
+        std::vector<cplx> F((size_t)ndof * R * chunk), FK((size_t)ndof * chunk);
+        int e;
+        for (int64_t iu = 0; iu < nU; iu++) {
+            const double U = Us[iu];
+            if (U == 0) continue;
+            for (int64_t ib = 0; ib < nb; ib++) {
+                const double beta = betas[ib];
+                for (int64_t w0 = 0; w0 < nw; w0 += chunk) {
+                    const int bt = (int)std::min<int64_t>(chunk, nw - w0);
+                    // compute_encountered_values (waves.jl:136-151)
+                    for (int q = 0; q < bt; q++) {
+                        const double w = omegas[w0 + q], k = w * w / g;
+                        const double dop = w - k * U * cos(beta);
+                        we[q] = fabs(dop);
+                        ke[q] = we[q] * we[q] / g;
+                        inc[q] = w;
+                        inc[chunk + q] = k;
+                        inc[2 * chunk + q] = dop >= 0 ? beta : beta + MHX_PI;
+                        if (!(we[q] > 0)) return fail(ctx, MHX_EINVAL, "forward speed: zero encountered frequency");
+                    }
+                    if ((e = set_freqs(ctx, b, we.data(), bt))) return e;
+                    HCHK(hipMemcpyAsync(d_inc, inc.data(), sizeof(double) * 3 * chunk, hipMemcpyHostToDevice, st));
+                    for (int q = 0; q < bt; q++)
+                        mhx::launch_bc_fs(st, n, b->geom, b->dofs, ndof, d_grad, we[q], U, inc[chunk + q], inc[q], beta,
+                                          inc[2 * chunk + q], g, b->B + q * b->nR());
+                    if ((e = assemble_into(ctx, b, bt, gf, 0, 1)) || (e = check_status(ctx, b, gf))) return e;
+                    e = solve_fp64(ctx, b, R, 0, bt);  // sigma stays in b->B, phi = S sigma in b->Y
+                    if (e == MHX_ESINGULAR) return fail(ctx, MHX_ESINGULAR, ctx->err);
+                    if (e) return e;
+                    {
+                        Phase ph(ctx, "forward_speed");
+                        for (int q = 0; q < bt; q++)
+                            mhx::launch_kx_combine(st, n, b->geom, b->KRx, ke[q], table, table ? b->WR + q * b->nn() : nullptr,
+                                                   b->D + q * b->nn());
+                        mhx::launch_matmul_nr(st, n, R, b->D, b->nn(), b->B, b->nR(), psi, b->nR(), b->work, b->sW(), bt);
+                        for (int q = 0; q < bt; q++)
+                            mhx::launch_caxpy(st, (int64_t)n * R, 0.0, -U / we[q], psi + q * b->nR(), b->Y + q * b->nR());
+                    }
+                    {
+                        Phase ph(ctx, "forces");
+                        mhx::launch_forces(st, n, b->geom, b->dofs, ndof, b->Y, b->nR(), R, ctx->rho, b->d_ws, b->d_ks, bt, g,
+                                           nullptr, 0, b->F, b->FK);
+                        // froude_krylov_force (waves.jl:294-303): incident omega and k, encountered heading
+                        for (int q = 0; q < bt; q++)
+                            mhx::launch_forces(st, n, b->geom, b->dofs, ndof, b->Y, b->nR(), 0, ctx->rho, d_inc + q,
+                                               d_inc + chunk + q, 1, g, d_inc + 2 * chunk + q, 1, b->F, b->FK + q * ndof);
+                    }
+                    HCHK(hipGetLastError());
+                    HCHK(hipMemcpyAsync(F.data(), b->F, sizeof(cplx) * ndof * R * bt, hipMemcpyDeviceToHost, st));
+                    HCHK(hipMemcpyAsync(FK.data(), b->FK, sizeof(cplx) * ndof * bt, hipMemcpyDeviceToHost, st));
+                    HCHK(hipStreamSynchronize(st));
+                    // problems_and_results.jl:202-208, 341-380
+                    for (int q = 0; q < bt; q++) {
+                        const int64_t iw = w0 + q;
+                        for (int r = 0; r < R; r++)
+                            for (int d = 0; d < ndof; d++) {
+                                const cplx f = F[d + (size_t)ndof * (r + (size_t)R * q)];
+                                if (r < ndof) {
+                                    A_out[aidx(d, r, iw, iu, ib)] = f.x / (we[q] * we[q]);
+                                    B_out[aidx(d, r, iw, iu, ib)] = f.y / we[q];
+                                } else {
+                                    const size_t o = fidx(d, iw, ib, iu);
+                                    Fd_out[2 * o] = f.x;
+                                    Fd_out[2 * o + 1] = f.y;
+                                }
+                            }
+                        for (int d = 0; d < ndof; d++) {
+                            const size_t o = fidx(d, iw, ib, iu);
+                            Fk_out[2 * o] = FK[d + (size_t)ndof * q].x;
+                            Fk_out[2 * o + 1] = FK[d + (size_t)ndof * q].y;
+                        }
+                    }
+                }
+            }
+        }
+        return MHX_OK;
+    };
+    rc = run();
+    cleanup();
     flush_timing(ctx);
     return rc;
 }

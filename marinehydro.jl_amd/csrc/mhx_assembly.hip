@@ -113,6 +113,130 @@ void launch_rankine(hipStream_t st, int n, const PanelGeom* geom, int direct, do
     rankine_kernel<<<grid, 256, 0, st>>>(n, geom, direct, SR, DR);
 }
 
+// ---------------------------------------------------------------------------
+// forward speed: the indirect x-derivative operator K_x (assemble_matrices with all_normals =
+// [1, 0, 0], matrix_assembly.jl:183-245; used by solve.jl:35-45 for dphi/dx = K_x sigma)
+// ---------------------------------------------------------------------------
+// Rankine part of K_x (wavenumber independent, cached per body): the indirect gradient with
+// respect to the field point dotted with e_x, panel and reflected image.
+__global__ void __launch_bounds__(256) rankine_x_kernel(int n, const PanelGeom* __restrict__ geom,
+                                                        double* __restrict__ KRx) {
+    __shared__ PanelGeom sg[TJ];
+    const int j0 = blockIdx.y * TJ;
+    stage_columns(geom, n, j0, sg);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const PanelGeom& gi = geom[i];
+    const double c1[3] = {gi.c[0], gi.c[1], gi.c[2]};
+    const double c1r[3] = {gi.c[0], gi.c[1], -gi.c[2]};
+    const int jn = min(TJ, n - j0);
+#pragma unroll 1
+    for (int jj = 0; jj < jn; jj++) {
+        const PanelGeom& g = sg[jj];
+        double phi, gs[3], phir, gr[3];
+        vrankine(c1, g, phi, gs);
+        vrankine(c1r, g, phir, gr);
+        // indirect: e_x . (-grad) + e_x . reflect(-grad of the image) = -gs_x - gr_x
+        const double d = -gs[0] - gr[0];
+        KRx[i + (int64_t)n * (j0 + jj)] = kScale * d;
+    }
+}
+
+void launch_rankine_x(hipStream_t st, int n, const PanelGeom* geom, double* KRx) {
+    dim3 grid((n + 255) / 256, (n + TJ - 1) / TJ);
+    rankine_x_kernel<<<grid, 256, 0, st>>>(n, geom, KRx);
+}
+
+// K_x = KRx + wave part with the normal e_x (only the horizontal x component of the reduced
+// gradient survives) + the identity jump times (e_x . n_i) on the diagonal.  EXACT: dG/dr from
+// the (G, dG/dr) table of the same wavenumber, else the Wu fit.
+template <bool EXACT>
+__global__ void __launch_bounds__(256) kx_combine_kernel(int n, const PanelGeom* __restrict__ geom,
+                                                         const double* __restrict__ KRx, double k,
+                                                         const cplx* __restrict__ WR, cplx* __restrict__ Kx) {
+    __shared__ PanelGeom sg[TJ];
+    const int j0 = blockIdx.y * TJ;
+    stage_columns(geom, n, j0, sg);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const PanelGeom& gi = geom[i];
+    const int jn = min(TJ, n - j0);
+#pragma unroll 1
+    for (int jj = 0; jj < jn; jj++) {
+        const PanelGeom& g = sg[jj];
+        const int j = j0 + jj;
+        const int64_t idx = i + (int64_t)n * j;
+        double hh, vv;
+        reduced_hv(gi.c[0], gi.c[1], gi.c[2], g.c[0], g.c[1], g.c[2], k, hh, vv);
+        cplx Gh;
+        if (EXACT) {
+            Gh = WR[idx];
+        } else {
+            cplx G, Gv;
+            wu_term(hh, vv, G, Gh, Gv);
+        }
+        const double dx1 = gi.c[0] - g.c[0];  // indirect: Delta = x - xi
+        double dr1 = 0.0;
+        if (fabs(hh) > 1e-6) dr1 = k * k / hh * dx1;
+        const cplx dw = cscale(g.area, cscale(k, cscale(dr1, Gh)));
+        double dr = KRx[idx];
+        if (i == j) dr += gi.pad[0] * gi.nrm[0];
+        Kx[idx] = cmk(dr + kScale * dw.x, kScale * dw.y);
+    }
+}
+
+void launch_kx_combine(hipStream_t st, int n, const PanelGeom* geom, const double* KRx, double k, bool exact,
+                       const cplx* WR, cplx* Kx) {
+    dim3 grid((n + 255) / 256, (n + TJ - 1) / TJ);
+    if (exact) kx_combine_kernel<true><<<grid, 256, 0, st>>>(n, geom, KRx, k, WR, Kx);
+    else kx_combine_kernel<false><<<grid, 256, 0, st>>>(n, geom, KRx, k, WR, Kx);
+}
+
+// forward-speed right-hand sides of one (U, omega, beta) group: radiation (radiation_bc,
+// waves.jl:183-199) -i omega_e (n . dof) + U (n . dgrad), dgrad the constant gradient of motion of
+// a rigid dof (bodies.jl:55-66); diffraction (_airy_bc, waves.jl:266-282) with the incident k and
+// omega, the phase along the encountered heading and the velocity along the incident one.
+__global__ void bc_fs_kernel(int n, const PanelGeom* __restrict__ geom, const double* __restrict__ dofs, int ndof,
+                             const double* __restrict__ dgrad, double we, double U, double kinc, double winc,
+                             double beta, double beta_e, double g, cplx* __restrict__ B) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const PanelGeom& gi = geom[i];
+    const double* nv = gi.nrm;
+    for (int r = 0; r < ndof; r++) {
+        const double* dof = dofs + (int64_t)3 * n * r;
+        const double disp = nv[0] * dof[i] + nv[1] * dof[i + n] + nv[2] * dof[i + 2 * n];
+        const double* gr = dgrad + 3 * r;
+        const double gd = nv[0] * gr[0] + nv[1] * gr[1] + nv[2] * gr[2];
+        B[i + (int64_t)n * r] = cmk(U * gd * gi.pad[1], -we * disp);  // -i omega_e disp + U gd, hull panels only
+    }
+    const double wbar = gi.c[0] * cos(beta_e) + gi.c[1] * sin(beta_e);
+    const double cih = exp(kinc * gi.c[2]);
+    double s, c;
+    sincos(kinc * wbar, &s, &c);
+    const double am = g * kinc / winc;
+    const cplx amp = cmk(am * c, am * s);
+    const cplx v0 = cscale(cih, cscale(cos(beta), amp)), v1 = cscale(cih, cscale(sin(beta), amp));
+    const cplx v2 = cscale(cih, cmk(amp.y, -amp.x));  // -i * amp
+    const cplx dot = cadd(cadd(cscale(nv[0], v0), cscale(nv[1], v1)), cscale(nv[2], v2));
+    B[i + (int64_t)n * ndof] = cmk(-dot.x * gi.pad[1], -dot.y * gi.pad[1]);
+}
+
+void launch_bc_fs(hipStream_t st, int n, const PanelGeom* geom, const double* dofs, int ndof, const double* dgrad,
+                  double we, double U, double kinc, double winc, double beta, double beta_e, double g, cplx* B) {
+    bc_fs_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, geom, dofs, ndof, dgrad, we, U, kinc, winc, beta, beta_e, g, B);
+}
+
+// Y += c X over cnt complex elements
+__global__ void caxpy_kernel(int64_t cnt, double cr, double ci, const cplx* __restrict__ X, cplx* __restrict__ Y) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t < cnt) Y[t] = cmk(Y[t].x + cr * X[t].x - ci * X[t].y, Y[t].y + cr * X[t].y + ci * X[t].x);
+}
+
+void launch_caxpy(hipStream_t st, int64_t cnt, double cr, double ci, const cplx* X, cplx* Y) {
+    caxpy_kernel<<<(unsigned)((cnt + 255) / 256), 256, 0, st>>>(cnt, cr, ci, X, Y);
+}
+
 // _wu_sd (wu.jl:112-127) + the scale of _assemble_wu_centers_loop (matrix_assembly.jl:343-369)
 __global__ void __launch_bounds__(256) combine_wu_kernel(int n, const PanelGeom* __restrict__ geom,
                                                          const double* __restrict__ SR, const double* __restrict__ DR,

@@ -12,6 +12,14 @@ void launch_panel_geometry(hipStream_t st, int n, const double* verts, const int
                            const double* normals, const double* areas, const double* radii, PanelGeom* geom);
 // Rankine + free-surface image parts (real, wavenumber independent): SR and D_R (direct) or K_R (indirect)
 void launch_mark_lid(hipStream_t st, int n, int nhull, PanelGeom* geom);
+// forward speed: Rankine part of the x-derivative operator (once per body), the full K_x for a
+// wavenumber, the group's right-hand sides, Y += c X
+void launch_rankine_x(hipStream_t st, int n, const PanelGeom* geom, double* KRx);
+void launch_kx_combine(hipStream_t st, int n, const PanelGeom* geom, const double* KRx, double k, bool exact,
+                       const cplx* WR, cplx* Kx);
+void launch_bc_fs(hipStream_t st, int n, const PanelGeom* geom, const double* dofs, int ndof, const double* dgrad,
+                  double we, double U, double kinc, double winc, double beta, double beta_e, double g, cplx* B);
+void launch_caxpy(hipStream_t st, int64_t cnt, double cr, double ci, const cplx* X, cplx* Y);
 void launch_rankine(hipStream_t st, int n, const PanelGeom* geom, int direct, double* SR, double* DR);
 // Batched launches: blockIdx.z indexes a frequency; ks / ws are device arrays of
 // wavenumbers / frequencies; matrices of item z start at base + z * stride.

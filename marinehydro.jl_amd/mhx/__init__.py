@@ -14,5 +14,6 @@ from .api import (SETTINGS, DiffractionProblem, DirectBEM, ExactGuevelDelhommeau
                   SurrogateGuevelDelhommeau, train_surrogate, load_surrogate,
                   hydrodynamic_coefficients_jvp, hydrodynamic_coefficients_vjp, mesh_scaling_tangent,
                   chain_mesh_gradient, hydrodynamic_coefficients_many,
-                  combine_floatingbodies, make_body_name_list_unique)
+                  combine_floatingbodies, make_body_name_list_unique, compute_encountered_values,
+                  evaluate_gradient_of_motion)
 from .meshes import Mesh  # noqa: F401

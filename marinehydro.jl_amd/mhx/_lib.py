@@ -35,7 +35,7 @@ EXPORTED = ["mhx_create", "mhx_destroy", "mhx_last_error", "mhx_set_constants", 
             "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work", "mhx_set_batch", "mhx_surrogate_train",
             "mhx_surrogate_save", "mhx_surrogate_load", "mhx_surrogate_params", "mhx_surrogate_eval",
             "mhx_surrogate_loss_grad", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_hydro_coeffs_jvp",
-            "mhx_hydro_coeffs_vjp", "mhx_sweep_coeffs", "mhx_body_create_lid"]
+            "mhx_hydro_coeffs_vjp", "mhx_sweep_coeffs", "mhx_body_create_lid", "mhx_body_coeffs_fs"]
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int64)
@@ -114,6 +114,8 @@ def lib():
         L.mhx_body_destroy.restype = None
         L.mhx_body_coeffs.argtypes = [vp, vp, C.c_int64, _dp, C.c_int64, _dp, C.c_int, C.c_int, C.c_int,
                                       _dp, _dp, _dp, _dp]
+        L.mhx_body_coeffs_fs.argtypes = [vp, vp, C.c_int64, _dp, C.c_int64, _dp, C.c_int64, _dp, _dp, C.c_int, C.c_int,
+                                         _dp, _dp, _dp, _dp]
         L.mhx_phase_time.argtypes = [vp, C.c_char_p, _dp, C.POINTER(C.c_int64)]
         L.mhx_reset_timing.argtypes = [vp]
         L.mhx_reset_timing.restype = None

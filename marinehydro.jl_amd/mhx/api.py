@@ -13,6 +13,8 @@ caller (or a test ported from the reference's test suite) reads the same:
   assemble_matrices(gfs, mesh, k; direct, include_identity) src/matrix_assembly.jl:164-208
   solve_linear(D, S, bc; direct)  (the reference's solve(D, S, bc))  src/matrix_assembly.jl:436-449
   SETTINGS, set_rho, set_g                                 src/constants.jl:10-29
+  compute_encountered_values, forward_speed= / forward_speeds  src/waves.jl:136-151,
+                                                           src/problems_and_results.jl:236-380
 
 Every computation runs in libmhx on the GPU; this module only validates
 arguments, keeps device contexts and packs results.  ArgumentError maps to
@@ -52,6 +54,35 @@ def compute_wavenumber(omega, g=None):
     """Deep-water dispersion k = omega^2 / g (src/waves.jl:133-134)."""
     g = SETTINGS.g if g is None else g
     return omega * omega / g
+
+
+def compute_encountered_values(omega, beta, forward_speed, g=None):
+    """(omega_e, k_e, beta_e) seen by a body moving at forward_speed along +x in waves of
+    frequency omega and heading beta (src/waves.jl:136-151): omega_e = |omega - k U cos beta|,
+    k_e = omega_e^2 / g, beta_e = beta, or beta + pi when the Doppler-shifted frequency is
+    negative."""
+    g = SETTINGS.g if g is None else g
+    k = compute_wavenumber(omega, g)
+    if forward_speed == 0:
+        return omega, k, beta
+    doppler = omega - k * forward_speed * np.cos(beta)
+    we = abs(doppler)
+    return we, compute_wavenumber(we, g), (beta if doppler >= 0 else beta + np.pi)
+
+
+def evaluate_gradient_of_motion(mesh, rigid_dof_name):
+    """d(dof)/dx on every panel (nf x 3) of a rigid dof, used by the forward-speed radiation
+    condition (src/bodies.jl:55-66): Pitch (0, 0, 1), Yaw (0, -1, 0), the others 0."""
+    out = np.zeros((mesh.nfaces, 3))
+    out[:] = _motion_gradient(rigid_dof_name)
+    return out
+
+
+def _motion_gradient(name):
+    """The constant row of evaluate_gradient_of_motion for a rigid dof name."""
+    if name not in RIGID_DOFS:
+        raise ValueError("forward speed problems are only developed for rigid body dofs")
+    return {"Pitch": (0.0, 0.0, 1.0), "Yaw": (0.0, -1.0, 0.0)}.get(name, (0.0, 0.0, 0.0))
 
 
 # --------------------------------------------------------------------------- Green functions
@@ -273,6 +304,8 @@ class RadiationProblem:
     influenced_dofs: list = None
     rho: float = None
     g: float = None
+    beta: float = None
+    forward_speed: float = 0.0
 
     def __post_init__(self):
         inf = list(self.influenced_dofs) if self.influenced_dofs is not None else self.floatingbody.dof_names
@@ -289,6 +322,7 @@ class RadiationProblem:
         self.rho = SETTINGS.rho if self.rho is None else float(self.rho)
         self.g = SETTINGS.g if self.g is None else float(self.g)
         self.wavenumber = compute_wavenumber(self.omega, self.g)
+        _encounter(self)
 
 
 @dataclass
@@ -299,6 +333,7 @@ class DiffractionProblem:
     influenced_dofs: list = None
     rho: float = None
     g: float = None
+    forward_speed: float = 0.0
 
     def __post_init__(self):
         inf = list(self.influenced_dofs) if self.influenced_dofs is not None else self.floatingbody.dof_names
@@ -308,6 +343,23 @@ class DiffractionProblem:
         self.rho = SETTINGS.rho if self.rho is None else float(self.rho)
         self.g = SETTINGS.g if self.g is None else float(self.g)
         self.wavenumber = compute_wavenumber(self.omega, self.g)
+        _encounter(self)
+
+
+def _encounter(problem):
+    """encountered_omega / _wavenumber / _beta of a problem (problems_and_results.jl:40-53,
+    118-133): the incident values at rest, compute_encountered_values otherwise (which needs a
+    heading)."""
+    U = float(problem.forward_speed)
+    problem.forward_speed = U
+    if U == 0:
+        problem.encountered_omega, problem.encountered_wavenumber = problem.omega, problem.wavenumber
+        problem.encountered_beta = problem.beta
+        return
+    if problem.beta is None:
+        raise ValueError("a problem with nonzero forward speed needs a wave direction beta")
+    problem.encountered_omega, problem.encountered_wavenumber, problem.encountered_beta = \
+        compute_encountered_values(problem.omega, problem.beta, U, problem.g)
 
 
 @dataclass
@@ -323,13 +375,19 @@ class DiffractionResult:
     fk: dict = field(default_factory=dict)
 
 
+def _result_omega(res):
+    """problems_and_results.jl:202-203: the encountered frequency of a moving body."""
+    p = res.problem
+    return p.omega if p.forward_speed == 0 else p.encountered_omega
+
+
 def added_mass(res):
-    w = res.problem.omega
+    w = _result_omega(res)
     return {k: v.real / w ** 2 for k, v in res.forces.items()}
 
 
 def radiation_damping(res):
-    w = res.problem.omega
+    w = _result_omega(res)
     return {k: v.imag / w for k, v in res.forces.items()}
 
 
@@ -383,6 +441,12 @@ def _body_coeffs(body, names, omegas, betas, gf, direct, mode, rho, g, device=0)
     om = np.ascontiguousarray(omegas, dtype=np.float64)
     be = np.ascontiguousarray(betas, dtype=np.float64)
     nw, nb = om.size, be.size
+    cb = 16 - nd  # right-hand sides of one solve: nd radiation + at most cb diffraction columns
+    if 0 < cb < nb:
+        parts = [_body_coeffs(body, names, om, be[i:i + cb], gf, direct, mode, rho, g, device)
+                 for i in range(0, nb, cb)]
+        return (parts[0][0], parts[0][1], np.concatenate([p[2] for p in parts], axis=2),
+                np.concatenate([p[3] for p in parts], axis=2))
     A = np.zeros((nd, nd, nw), order="F")
     B = np.zeros((nd, nd, nw), order="F")
     Fd = np.zeros((nd, nw, max(nb, 1)), dtype=np.complex128, order="F")
@@ -390,6 +454,37 @@ def _body_coeffs(body, names, omegas, betas, gf, direct, mode, rho, g, device=0)
     ctx.check(L.mhx_body_coeffs(ctx.h, body._handle, nw, _lib.ptr(om), nb, _lib.ptr(be) if nb else None, gf,
                                 int(direct), mode, _lib.ptr(A), _lib.ptr(B), _lib.ptr(Fd), _lib.ptr(Fk)))
     return A, B, Fd[:, :, :nb], Fk[:, :, :nb]
+
+
+def _body_coeffs_fs(body, names, omegas, betas, speeds, rad, gf, direct, rho, g, device=0):
+    """mhx_body_coeffs_fs: A, B (nd, nd, nw, nU, nb); Fd, Fk (nd, nw, nb, nU).  The gradient of
+    motion is needed for the radiating dofs only (rad) and only when some speed is nonzero."""
+    us = np.ascontiguousarray(speeds, dtype=np.float64)
+    nd = len(names)
+    dgrad = np.zeros((nd, 3))
+    if np.any(us != 0):
+        if direct:
+            raise ValueError("nonzero forward speed is not supported with the direct method; "
+                             "use IndirectBEM (direct = false)")
+        for i, nm in enumerate(names):
+            if nm in rad:
+                dgrad[i] = _motion_gradient(nm)
+    dgrad = np.ascontiguousarray(dgrad)
+    ctx = context(device)
+    _ensure_surrogate(gf, device)
+    ctx.set_constants(rho, g)
+    _ensure_handle(body, names, ctx)
+    om = np.ascontiguousarray(omegas, dtype=np.float64)
+    be = np.ascontiguousarray(betas, dtype=np.float64)
+    nw, nb, nu = om.size, be.size, us.size
+    A = np.zeros((nd, nd, nw, nu, nb), order="F")
+    B = np.zeros((nd, nd, nw, nu, nb), order="F")
+    Fd = np.zeros((nd, nw, nb, nu), dtype=np.complex128, order="F")
+    Fk = np.zeros((nd, nw, nb, nu), dtype=np.complex128, order="F")
+    ctx.check(_lib.lib().mhx_body_coeffs_fs(ctx.h, body._handle, nw, _lib.ptr(om), nb, _lib.ptr(be), nu, _lib.ptr(us),
+                                            _lib.ptr(dgrad), gf, int(direct), _lib.ptr(A), _lib.ptr(B), _lib.ptr(Fd),
+                                            _lib.ptr(Fk)))
+    return A, B, Fd, Fk
 
 
 def _release(body):
@@ -403,6 +498,20 @@ def solve(problem, formulation=None):
     formulation = DirectBEM() if formulation is None else formulation
     body = problem.floatingbody
     names = list(problem.influenced_dofs)
+    if problem.forward_speed != 0:
+        # one (U, omega, beta) group of mhx_body_coeffs_fs (solve.jl:59-65 with _problem_omega_k)
+        rad = [problem.radiating_dof] if isinstance(problem, RadiationProblem) else []
+        if rad and rad[0] not in names:
+            names = names + rad
+        A, B, Fd, Fk = _body_coeffs_fs(body, names, [problem.omega], [problem.beta], [problem.forward_speed], rad,
+                                       formulation.gf, formulation.direct, problem.rho, problem.g)
+        ninf = len(problem.influenced_dofs)
+        if rad:
+            r, w = names.index(rad[0]), problem.encountered_omega
+            return RadiationResult(problem, {nm: complex(A[i, r, 0, 0, 0] * w * w, B[i, r, 0, 0, 0] * w)
+                                             for i, nm in enumerate(names[:ninf])})
+        return DiffractionResult(problem, {nm: complex(Fd[i, 0, 0, 0]) for i, nm in enumerate(names)},
+                                 {nm: complex(Fk[i, 0, 0, 0]) for i, nm in enumerate(names)})
     if isinstance(problem, RadiationProblem):
         if problem.radiating_dof not in names:
             names = names + [problem.radiating_dof]
@@ -435,6 +544,8 @@ def hydrodynamic_coefficients(floatingbody, parameters, formulation=None, schedu
         if nm not in floatingbody.dofs:
             raise ValueError("each influenced_dofs Symbol must be a key of floatingbody.dofs")
     names = inf + [r for r in rad if r not in inf]
+    if "forward_speeds" in parameters:
+        return _coefficients_fs(floatingbody, parameters, formulation, ws, inf, rad, betas, names)
     mode = {"per_omega": _lib.MODE_PER_OMEGA, "per_problem": _lib.MODE_PER_PROBLEM,
             "batched": _lib.MODE_BATCHED}[schedule]
     A, B, Fd, Fk = _body_coeffs(floatingbody, names, ws, betas, formulation.gf, formulation.direct, mode,
@@ -453,6 +564,29 @@ def hydrodynamic_coefficients(floatingbody, parameters, formulation=None, schedu
         out.update(diffraction_force=fd, Froude_Krylov_force=fk, excitation_force=fd + fk)
     else:
         out.update(diffraction_force=np.zeros(0), Froude_Krylov_force=np.zeros(0), excitation_force=np.zeros(0))
+    return out
+
+
+def _coefficients_fs(body, parameters, formulation, ws, inf, rad, betas, names):
+    """hydrodynamic_coefficients with forward_speeds (problems_and_results.jl:236-380): radiation
+    problems over (beta, dof, omega, U), diffraction over (beta, omega, U).  Layouts as the
+    reference: added_mass / radiation_damping (n_inf, n_rad, n_omega, n_U, n_beta); forces
+    (n_inf, n_omega, n_beta, n_U)."""
+    us = np.atleast_1d(np.asarray(parameters["forward_speeds"], dtype=np.float64))
+    if not betas:
+        raise ValueError("forward_speeds needs wave_directions (the encountered frequency depends on them)")
+    A, B, Fd, Fk = _body_coeffs_fs(body, names, ws, betas, us, rad, formulation.gf, formulation.direct,
+                                   SETTINGS.rho, SETTINGS.g)
+    ii = [names.index(nm) for nm in inf]
+    rr = [names.index(nm) for nm in rad]
+    out = {}
+    if rad:
+        out["added_mass"] = A[np.ix_(ii, rr)]
+        out["radiation_damping"] = B[np.ix_(ii, rr)]
+    else:
+        out["added_mass"] = out["radiation_damping"] = np.zeros(0)
+    fd, fk = Fd[ii], Fk[ii]
+    out.update(diffraction_force=fd, Froude_Krylov_force=fk, excitation_force=fd + fk)
     return out
 
 

@@ -5,6 +5,12 @@
  * cites the reference file:line it restates.  Built by oracle/Makefile into
  * oracle/liboracle.so (gcc, no -ffast-math: signed zeros and atan2 branches
  * must behave exactly as in the reference).
+ *
+ * Parity unpinned: the two forward-speed terms of ora_hydro_coeffs_fs (U n.dgrad
+ * in the radiation condition, rho U K_x sigma in the pressure) are restatements
+ * of waves.jl:183-199 and solve.jl:35-45 with no golden vector or exact identity
+ * to check them against; the rest of that function is checked through the
+ * encountered values, the fixed-normal operator and the U = 0 reduction.
  */
 #include "mh_oracle.h"
 
