@@ -61,6 +61,86 @@ class Mesh:
         """Uniform scaling of every field, as the reference's radius-derivative tests do."""
         return Mesh(self.vertices * s, self.faces, self.centers * s, self.normals, self.areas * s * s, self.radii * s)
 
+    # ---- Capytaine-compatible I/O (src/meshes.jl:122-143): 0-based faces, Capytaine field names ----
+    CAPYTAINE_FIELDS = ("vertices", "faces", "faces_centers", "faces_normals", "faces_areas", "faces_radiuses")
+
+    @classmethod
+    def from_capytaine(cls, mesh):
+        """Mesh(PyObject) / StaticArraysMesh(PyObject) (src/meshes.jl:122-143): a Capytaine mesh, or
+        any object or dict with its fields (vertices, 0-based faces, faces_centers, faces_normals,
+        faces_areas, faces_radiuses), taken as given; only the faces move to 1-based indices."""
+        get = mesh.__getitem__ if isinstance(mesh, dict) else (lambda k: getattr(mesh, k))
+        f = np.asarray(get("faces"), dtype=np.int64)
+        if f.ndim != 2 or f.shape[1] != 4:
+            raise ValueError("faces must be (nf, 4): Capytaine stores triangles with the last index repeated")
+        if f.size and f.min() < 0:
+            raise ValueError("Capytaine faces are 0-based and non-negative")
+        return cls(np.asarray(get("vertices"), dtype=np.float64), f + 1, get("faces_centers"), get("faces_normals"),
+                   get("faces_areas"), get("faces_radiuses"))
+
+    def to_capytaine(self):
+        """The fields under Capytaine's names, faces 0-based (what Mesh(PyObject) reads)."""
+        return {"vertices": self.vertices.copy(), "faces": self.faces - 1, "faces_centers": self.centers.copy(),
+                "faces_normals": self.normals.copy(), "faces_areas": self.areas.copy(),
+                "faces_radiuses": self.radii.copy(), "nb_vertices": self.nvertices, "nb_faces": self.nfaces}
+
+    def save(self, path):
+        """Write the Capytaine fields to an .npz (numeric arrays only; 0-based faces), e.g. for a mesh
+        exported from Capytaine elsewhere with np.savez(path, **{k: getattr(m, k) for k in
+        Mesh.CAPYTAINE_FIELDS})."""
+        d = self.to_capytaine()
+        np.savez(path, **{k: d[k] for k in self.CAPYTAINE_FIELDS})
+
+    @classmethod
+    def load(cls, path):
+        """Read an .npz written by save() or by the np.savez line above (no pickled objects)."""
+        with np.load(path, allow_pickle=False) as z:
+            missing = [k for k in ("vertices", "faces") if k not in z.files]
+            if missing:
+                raise ValueError(f"{path}: missing {missing}")
+            d = {k: z[k] for k in z.files}
+        if all(k in d for k in cls.CAPYTAINE_FIELDS):
+            return cls.from_capytaine(d)
+        # vertices and faces only: face properties computed here (Capytaine's rules)
+        return cls(d["vertices"], np.asarray(d["faces"], dtype=np.int64) + 1)
+
+    # ---- Capytaine mesh methods used by the reference's tests and paper scripts ----
+    def __add__(self, other):
+        """mesh1 + mesh2 (src/meshes.jl:94-96, Capytaine's join): vertices and faces concatenated."""
+        return combine(self, other)
+
+    def translated(self, shift):
+        return translated(self, shift)
+
+    def translated_x(self, d):
+        return translated(self, (d, 0.0, 0.0))
+
+    def translated_y(self, d):
+        return translated(self, (0.0, d, 0.0))
+
+    def translated_z(self, d):
+        return translated(self, (0.0, 0.0, d))
+
+    def immersed_part(self, tol=1e-12):
+        """The faces at or below the free surface z = 0, unused vertices dropped (kept in order),
+        face properties recomputed.  Vertices within tol * (mesh size) of z = 0 are put on it, as a
+        clip at the plane does.  A face with vertices on both sides of z = 0 would have to be cut;
+        that is not done here (ValueError): mesh with a vertex ring on the waterline."""
+        V = self.vertices.copy()
+        scale = max(1.0, float(np.max(np.abs(V))) if V.size else 1.0)
+        V[np.abs(V[:, 2]) <= tol * scale, 2] = 0.0
+        F = self.faces - 1
+        above = V[F, 2] > 0.0
+        below = V[F, 2] < 0.0
+        if np.any(above.any(axis=1) & below.any(axis=1)):
+            raise ValueError("immersed_part: a face crosses z = 0 (clipping faces is not supported)")
+        keep = ~above.any(axis=1) & below.any(axis=1)
+        F = F[keep]
+        used = np.unique(F)
+        remap = np.full(V.shape[0], -1, dtype=np.int64)
+        remap[used] = np.arange(used.size)
+        return Mesh(V[used], remap[F] + 1)
+
 
 def face_properties(vertices, faces):
     """Capytaine's compute_faces_properties on 1-based faces."""
@@ -260,6 +340,14 @@ def sphere(radius=1.0, ntheta=10, nphi=10, immersed=True):
         prof = prof[: ntheta // 2 + 1]
         prof[-1, 1] = 0.0
     return revolution(prof, nphi)
+
+
+def mesh_sphere(radius=1.0, center=(0.0, 0.0, 0.0), resolution=(10, 10)):
+    """Capytaine's mesh_sphere(radius=, center=, resolution=(ntheta, nphi)) (the mesher of the
+    reference's tests and notebooks): the full UV sphere; .immersed_part() keeps the wetted half."""
+    ntheta, nphi = resolution
+    m = sphere(radius, ntheta, nphi, immersed=False)
+    return translated(m, center) if any(c != 0.0 for c in center) else m
 
 
 def hemisphere(radius=1.0, nlat=32, nlon=64):

@@ -172,10 +172,21 @@ def test_argument_errors(mhx):
     body = mhx.FloatingBody(sphere18(), ["Heave"])
     with pytest.raises(ValueError):
         mhx.hydrodynamic_coefficients(body, dict(wave_frequencies=[-1.0], radiating_dofs=["Heave"]))
-    names = ["Heave"] * 1
+    # more headings than one solve holds (1 dof + 16 headings): split into heading chunks, each
+    # heading's forces equal those of a call with that heading alone
+    betas = list(np.linspace(0, 3, 16))
+    many = mhx.hydrodynamic_coefficients(body, dict(wave_frequencies=[1.0], radiating_dofs=["Heave"],
+                                                    wave_directions=betas))
+    for q in (0, 14, 15):
+        one = mhx.hydrodynamic_coefficients(body, dict(wave_frequencies=[1.0], radiating_dofs=["Heave"],
+                                                       wave_directions=[betas[q]]))
+        np.testing.assert_allclose(many["diffraction_force"][:, :, q], one["diffraction_force"][:, :, 0], rtol=1e-12)
+    # 16 dofs leave no room for a heading column: the solver's limit is reported
+    m = sphere18()
+    wide = mhx.FloatingBody(m, {f"d{i}": np.tile(np.eye(3)[i % 3] * (1 + i), (m.nfaces, 1)) for i in range(16)})
     with pytest.raises(mhx.DimensionMismatch):
-        mhx.hydrodynamic_coefficients(body, dict(wave_frequencies=[1.0], radiating_dofs=names,
-                                                 wave_directions=list(np.linspace(0, 3, 16))))
+        mhx.hydrodynamic_coefficients(wide, dict(wave_frequencies=[1.0], radiating_dofs=wide.dof_names,
+                                                 wave_directions=[0.0]))
 
 
 @pytest.mark.parametrize("gf", ["Wu", "ExactGuevelDelhommeau"])

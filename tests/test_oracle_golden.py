@@ -451,3 +451,58 @@ def test_forward_speed_at_rest_is_the_indirect_path(oracle):
         np.testing.assert_array_equal(r1["added_mass"][:, :, :, 0, ib], r0["added_mass"])
         np.testing.assert_array_equal(r1["radiation_damping"][:, :, :, 0, ib], r0["radiation_damping"])
     np.testing.assert_array_equal(r1["excitation_force"][:, :, :, 0], r0["excitation_force"])
+
+
+# ---------------------------------------------------------------------------- paper: two-sphere array
+PAPER = json.load(open(os.path.join(HERE, "golden", "paper_array_sensitivities.json")))
+
+
+def _paper_pair(r, dx):
+    """paper/meshGradients_pair.jl:9-15: mesh_sphere(r, resolution=(6,6)).immersed_part() and the
+    same moved by r + dx along x, joined."""
+    from mhx import meshes
+    s = meshes.mesh_sphere(r, resolution=(6, 6)).immersed_part()
+    return s + s.translated_x(r + dx), s.nfaces
+
+
+def _paper_coeffs(O, r, dx, w):
+    """A, B of the heave of each sphere on its own panels (paper/ArraySensitivities.jl:16-50):
+    GFWu, direct, rho = 1000, k = omega^2 / 9.81."""
+    m, n1 = _paper_pair(r, dx)
+    d1 = np.zeros((m.nfaces, 3)); d1[:n1, 2] = 1.0
+    d2 = np.zeros((m.nfaces, 3)); d2[n1:, 2] = 1.0
+    res = O.hydro_coeffs(O.Mesh.from_mhx(m), [d1, d2], [w], [], rho=1000.0, g=9.81, gf=O.GF_WU, direct=True)
+    return np.array([res["added_mass"][:, :, 0], res["radiation_damping"][:, :, 0]])
+
+
+def test_paper_array_sensitivities(oracle):
+    """The reference's paper data (paper/Plots/11_heuristics_dx_DELhommeau.csv,
+    11_heuristics_dr_DELhommeau.csv, added_mass_/damping_data_dimensionless.csv; 336 numbers
+    computed by the reference with Zygote through its AD rules): the oracle's coefficients of the
+    two-sphere array, differentiated by a fourth-order central difference (h = 1e-4, inside the
+    far-field rule's branch), agree to 1e-6 (relative to max(|value|, 1)).  This pins the oracle on
+    coupled bodies at separations 1.5-8 radii and kr up to 9.5."""
+    h = 1e-4
+
+    def d4(f):
+        return (8 * (f(h) - f(-h)) - (f(2 * h) - f(-2 * h))) / (12 * h)
+
+    def check(got, want):
+        assert abs(got - want) <= 1e-6 * max(abs(want), 1.0), (got, want)
+
+    g = PAPER["dA11_ddx"]
+    for dx, a, b in zip(g["dx"], g["dA11"], g["dB11"]):
+        d = d4(lambda t: _paper_coeffs(oracle, 1.0, dx + t, g["omega"]))
+        check(d[0, 0, 0], a)
+        check(d[1, 0, 0], b)
+    g = PAPER["dA11_dr"]
+    for dx, a, b in zip(g["dx"], g["dA11"], g["dB11"]):
+        d = d4(lambda t: _paper_coeffs(oracle, 1.0 + t, dx, g["omega"]))
+        check(d[0, 0, 0], a)
+        check(d[1, 0, 0], b)
+    g = PAPER["dA21_dr"]
+    for dxr, kr, a, b in zip(g["dx_r_ratio"], g["kr"], g["dA21"], g["dB21"]):
+        w = math.sqrt(kr * 9.8)  # the script's g = 9.8 for omega (paper/ArraySensitivities.jl:54, 82)
+        d = d4(lambda t: _paper_coeffs(oracle, 1.0 + t, dxr, w))
+        check(d[0, 1, 0], a)
+        check(d[1, 1, 0], b)
