@@ -208,6 +208,8 @@ def main():
     ap.add_argument("--cpu-freqs", type=int, default=2, help="complete omega-solves timed on the host for cpu_baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain per-omega comparison run")
+    ap.add_argument("--lu-mode", default="auto", choices=["auto", "fp64", "mixed", "mixed_bf16"],
+                    help="dense solver precision (auto: mixed FP32 factor for the surrogate, FP64 otherwise)")
     ap.add_argument("--workload", default="headline", choices=["headline", "forward_speed"],
                     help="forward_speed: the forward-speed grid on the same body (IndirectBEM, exact or Wu term)")
     args = ap.parse_args()
@@ -227,6 +229,8 @@ def main():
     mhx.set_g(9.81)
     names = ["Surge", "Sway", "Heave", "Roll", "Pitch", "Yaw"]
     body = mhx.FloatingBody(mesh, names)
+    ctx.set_lu_mode({"auto": _lib.LU_AUTO, "fp64": _lib.LU_FP64, "mixed": _lib.LU_MIXED,
+                     "mixed_bf16": _lib.LU_MIXED_BF16}[args.lu_mode])
     if args.workload == "forward_speed":
         return bench_forward_speed(args, mhx, _lib, ctx, body, names, mesh, omegas)
     form = mhx.DirectBEM(args.gf)
@@ -263,6 +267,7 @@ def main():
         exact_form = mhx.DirectBEM("ExactGuevelDelhommeau")
         run_plain = lambda ws: mhx.api._body_coeffs(body, names, ws, [0.0], exact_form.gf, True,
                                                     _lib.MODE_PER_OMEGA, 1025.0, 9.81)
+        ctx.set_lu_mode(_lib.LU_FP64)  # the plain path: FP64 LU, as the reference
         pv, pel, pph, proof, plu, pres, _ = measure(ctx, run_plain, steps_w[sub], n, "ExactGuevelDelhommeau")
         errs = {}
         pairs = (("added_mass", res[0][:, :, sub], pres[0], lambda x, q: x[:, :, q]),

@@ -146,6 +146,8 @@ int mhx_set_batch(mhx_ctx* ctx, int64_t max_items, double max_bytes);
 #define MHX_LU_AUTO 0
 #define MHX_LU_FP64 1
 #define MHX_LU_MIXED 2
+#define MHX_LU_MIXED_BF16 3  /* as MHX_LU_MIXED, the FP32 factor's trailing updates on bf16 MFMA with each
+                                operand split into bf16 hi + lo (products to ~2^-16), up to 8 refinement steps */
 int mhx_set_lu_mode(mhx_ctx* ctx, int mode);
 int mhx_solver_stats(mhx_ctx* ctx, int64_t counts[3], double* max_residual);
 

@@ -70,10 +70,10 @@ struct mhx_body {
     int64_t fb_cap = 0;
     // mixed-precision solver: FP32 factors, right-hand-side copy, residuals, norms
     cplxf* Df = nullptr;
+    unsigned short* Ps = nullptr;  // split-bf16 panels of the FP32 factorization (MHX_LU_MIXED_BF16)
     cplx *Rr = nullptr, *Tr = nullptr;
     double* norms = nullptr;
     bool df_ready = false;  // Df already holds the FP32 copy of D (written by the fused surrogate assembly)
-    int trs_epoch = 0;  // sweep counter of the triangular-solve flags in work (zeroed with work)
     int64_t nn() const { return (int64_t)n * n; }
     int64_t nR() const { return (int64_t)n * cap_R; }
     int64_t sW() const { return 16 * nR() + mhx::getrs_work(n, cap_R); }  // split-K partials + getrs work
@@ -176,9 +176,8 @@ void free_work(mhx_body* b) {
     dfree(b->S); dfree(b->D); dfree(b->WG); dfree(b->WR); dfree(b->B); dfree(b->Y); dfree(b->work);
     dfree(b->F); dfree(b->FK); dfree(b->betas); dfree(b->d_ws); dfree(b->d_ks); dfree(b->ipiv); dfree(b->perm);
     dfree(b->wperm); dfree(b->info); dfree(b->status); dfree(b->fb_list); dfree(b->fb_count);
-    dfree(b->Df); dfree(b->Rr); dfree(b->Tr); dfree(b->norms);
+    dfree(b->Df); dfree(b->Ps); dfree(b->Rr); dfree(b->Tr); dfree(b->norms);
     b->fb_cap = 0;
-    b->trs_epoch = 0;
     b->cap_batch = b->cap_R = b->cap_nb = 0;
     b->cap_table = false;
 }
@@ -207,14 +206,14 @@ int ensure_work(mhx_ctx* ctx, mhx_body* b, int R, int nb, bool table, int batch 
         free_work(b);
         return rc;
     }
-    HCHK(hipMemsetAsync(b->work, 0, sizeof(cplx) * (size_t)b->sW() * B, ctx->stream));  // sweep flags
-    b->trs_epoch = 0;
     return MHX_OK;
 }
 
 bool use_mixed(const mhx_ctx* ctx, int gf) {
-    return ctx->lu_mode == MHX_LU_MIXED || (ctx->lu_mode == MHX_LU_AUTO && gf == MHX_GF_SURROGATE);
+    return ctx->lu_mode == MHX_LU_MIXED || ctx->lu_mode == MHX_LU_MIXED_BF16 ||
+           (ctx->lu_mode == MHX_LU_AUTO && gf == MHX_GF_SURROGATE);
 }
+bool use_split(const mhx_ctx* ctx) { return ctx->lu_mode == MHX_LU_MIXED_BF16; }
 
 // wave assembly of `batch` items (wavenumbers already in b->d_ks) into b->S, b->D
 int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int include_identity) {
@@ -284,15 +283,8 @@ int check_status(mhx_ctx* ctx, mhx_body* b, int gf) {
 // factor the `batch` matrices b->D in place and solve for the R columns of b->B; phi into b->Y
 int check_info(mhx_ctx* ctx, mhx_body* b, int batch, bool* singular) {
     std::vector<int> info(batch, 0);
-    int herr = 0;
-    int* errw = mhx::getrs_error_word(b->n, b->work + 16 * b->nR());
     HCHK(hipMemcpyAsync(info.data(), b->info, sizeof(int) * batch, hipMemcpyDeviceToHost, ctx->stream));
-    HCHK(hipMemcpyAsync(&herr, errw, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     HCHK(hipStreamSynchronize(ctx->stream));
-    if (herr) {
-        HCHK(hipMemsetAsync(errw, 0, sizeof(int), ctx->stream));
-        return fail(ctx, MHX_EDEVICE, "solve: triangular sweep gave up waiting for a block");
-    }
     *singular = false;
     for (int q = 0; q < batch; q++)
         if (info[q]) {
@@ -321,10 +313,10 @@ int solve_fp64(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
         if (direct) {
             mhx::launch_matmul_nr(ctx->stream, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), batch);
             mhx::getrs_batched(ctx->stream, 0, n, R, b->D, n, b->nn(), b->ipiv, n, b->perm, n, b->Y, b->nR(), W2,
-                               b->sW(), batch, &b->trs_epoch);
+                               b->sW(), batch);
         } else {
             mhx::getrs_batched(ctx->stream, 0, n, R, b->D, n, b->nn(), b->ipiv, n, b->perm, n, b->B, b->nR(), W2,
-                               b->sW(), batch, &b->trs_epoch);
+                               b->sW(), batch);
             mhx::launch_matmul_nr(ctx->stream, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), batch);
         }
     }
@@ -340,7 +332,8 @@ int solve_fp64(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
 // residual |r| / |rhs| <= kRefineTol on every item.  Items that do not converge in
 // kRefineMax steps (or whose FP32 factor is singular) are redone with the FP64 LU.
 constexpr double kRefineTol = 1e-12;
-constexpr int kRefineMax = 4;
+constexpr int kRefineMax = 4;       // FP32 trailing updates
+constexpr int kRefineMaxSplit = 8;  // split-bf16 trailing updates (factors to ~2^-16)
 
 int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
     const int n = b->n;
@@ -350,6 +343,9 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
     if ((rc = dalloc(ctx, &b->Df, (size_t)b->nn() * b->cap_batch)) || (rc = dalloc(ctx, &b->Rr, (size_t)b->nR() * b->cap_batch)) ||
         (rc = dalloc(ctx, &b->Tr, (size_t)b->nR() * b->cap_batch)) || (rc = dalloc(ctx, &b->norms, 2 * (size_t)b->cap_batch)))
         return rc;
+    const bool split = use_split(ctx);
+    if (split && (rc = dalloc(ctx, &b->Ps, (size_t)mhx::split_work(n) * b->cap_batch))) return rc;
+    const int refine_max = split ? kRefineMaxSplit : kRefineMax;
     HCHK(hipMemsetAsync(b->info, 0, sizeof(int) * batch, st));
     {
         Phase ph(ctx, "lu");
@@ -357,7 +353,7 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
         b->df_ready = false;
         mhx::LaunchTimer lt;
         rc = mhx::getrf_batched(st, n, b->Df, n, b->nn(), batch, b->ipiv, n, b->wperm, b->info,
-                                ctx->timing ? &lt : nullptr);
+                                ctx->timing ? &lt : nullptr, split ? b->Ps : nullptr);
         take_launches(ctx, lt);
         if (rc) return fail(ctx, MHX_EDIM, "LU: order above 4096 is not supported");
     }
@@ -372,10 +368,9 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
         if (direct)
             mhx::launch_matmul_nr(st, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), batch);
         HCHK(hipMemcpyAsync(b->Rr, X, sizeof(cplx) * nRb, hipMemcpyDeviceToDevice, st));
-        mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, X, b->nR(), W2, b->sW(), batch,
-                           &b->trs_epoch);
+        mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, X, b->nR(), W2, b->sW(), batch);
         std::vector<double> nr(2 * batch);
-        for (int it = 0; it <= kRefineMax; it++) {
+        for (int it = 0; it <= refine_max; it++) {
             mhx::launch_matmul_nr(st, n, R, b->D, b->nn(), X, b->nR(), b->Tr, b->nR(), b->work, b->sW(), batch);
             HCHK(hipMemsetAsync(b->norms, 0, sizeof(double) * 2 * batch, st));
             mhx::launch_residual(st, (int64_t)n * R, b->Rr, b->nR(), b->Tr, b->nR(), b->norms, batch);
@@ -391,9 +386,8 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
                 ctx->max_residual = std::max(ctx->max_residual, worst);
                 break;
             }
-            if (it == kRefineMax) break;
-            mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, b->Tr, b->nR(), W2, b->sW(), batch,
-                               &b->trs_epoch, true);
+            if (it == refine_max) break;
+            mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, b->Tr, b->nR(), W2, b->sW(), batch, true);
             mhx::launch_add(st, (int64_t)n * R, b->Tr, b->nR(), X, b->nR(), batch);
             ctx->refine_steps += batch;
         }
@@ -527,18 +521,12 @@ bool sweep_uses(mhx_ctx* c, const mhx_body* b) {
 // sequence can be captured into a graph: frequencies from the per-body arrays, right-hand sides,
 // one table launch and one combine launch per chunk for all bodies, then per body the batched
 // LU, the solves and the pressure integration into the per-body force arrays.
-void sweep_enqueue(mhx_ctx* ctx, SweepState& s, bool zero_flags) {
+void sweep_enqueue(mhx_ctx* ctx, SweepState& s) {
     hipStream_t st = ctx->stream;
     const bool exact = s.gf == MHX_GF_DELHOMMEAU;
     const double g = ctx->g;
     mhx::launch_zero(st, s.status, 4 * sizeof(unsigned long long));
     mhx::launch_zero(st, s.sticky, s.bodies.size() * sizeof(int));
-    for (size_t ib = 0; ib < s.bodies.size(); ib++) {
-        mhx_body* b = s.bodies[ib];
-        if (zero_flags)
-            for (int q = 0; q < b->cap_batch; q++)
-                mhx::launch_zero(st, b->work + 16 * b->nR() + b->sW() * q, sizeof(cplx) * mhx::getrs_flag_words(b->n));
-    }
     for (int c = 0; c < s.K; c++) {
         for (size_t ib = 0; ib < s.bodies.size(); ib++) {
             mhx_body* b = s.bodies[ib];
@@ -568,11 +556,9 @@ void sweep_enqueue(mhx_ctx* ctx, SweepState& s, bool zero_flags) {
             cplx* W2 = b->work + 16 * b->nR();
             if (s.direct) {
                 mhx::launch_matmul_nr(st, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), bt);
-                mhx::getrs_batched(st, 0, n, R, b->D, n, b->nn(), b->ipiv, n, b->perm, n, b->Y, b->nR(), W2, b->sW(), bt,
-                                   &b->trs_epoch);
+                mhx::getrs_batched(st, 0, n, R, b->D, n, b->nn(), b->ipiv, n, b->perm, n, b->Y, b->nR(), W2, b->sW(), bt);
             } else {
-                mhx::getrs_batched(st, 0, n, R, b->D, n, b->nn(), b->ipiv, n, b->perm, n, b->B, b->nR(), W2, b->sW(), bt,
-                                   &b->trs_epoch);
+                mhx::getrs_batched(st, 0, n, R, b->D, n, b->nn(), b->ipiv, n, b->perm, n, b->B, b->nR(), W2, b->sW(), bt);
                 mhx::launch_matmul_nr(st, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), bt);
             }
             if (b->ndof > 0)
@@ -750,26 +736,22 @@ int mhx_lu_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* A, int tra
     hipMemcpyAsync(dA, A, sizeof(cplx) * n * n, hipMemcpyHostToDevice, st);
     hipMemcpyAsync(dB, Bh, sizeof(cplx) * n * nrhs, hipMemcpyHostToDevice, st);
     hipMemsetAsync(info, 0, sizeof(int), st);
-    hipMemsetAsync(dW, 0, sizeof(cplx) * (size_t)mhx::getrs_work((int)n, (int)nrhs), st);  // sweep flags
-    int epoch = 0;
     {
         Phase ph(ctx, "lu");
         mhx::getrf_batched(st, (int)n, dA, (int)n, n * n, 1, ipiv, n, wperm, info);
     }
     {
         Phase ph(ctx, "solve");
-        mhx::getrs_batched(st, trans, (int)n, (int)nrhs, dA, (int)n, 0, ipiv, n, perm, n, dB, 0, dW, 0, 1, &epoch);
+        mhx::getrs_batched(st, trans, (int)n, (int)nrhs, dA, (int)n, 0, ipiv, n, perm, n, dB, 0, dW, 0, 1);
     }
-    int hinfo = 0, herr = 0;
+    int hinfo = 0;
     hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st);
-    hipMemcpyAsync(&herr, mhx::getrs_error_word((int)n, dW), sizeof(int), hipMemcpyDeviceToHost, st);
     hipMemcpyAsync(Bh, dB, sizeof(cplx) * n * nrhs, hipMemcpyDeviceToHost, st);
     hipError_t e = hipStreamSynchronize(st);
     if (e == hipSuccess) e = hipGetLastError();
     flush_timing(ctx);
     cleanup();
     if (e != hipSuccess) return fail(ctx, MHX_EDEVICE, std::string("lu_solve: ") + hipGetErrorString(e));
-    if (herr) return fail(ctx, MHX_EDEVICE, "lu_solve: triangular sweep gave up waiting for a block");
     if (hinfo) return fail(ctx, MHX_ESINGULAR, "LU: exactly zero pivot at column " + std::to_string(hinfo));
     return MHX_OK;
 }
@@ -1202,13 +1184,13 @@ int mhx_hydro_coeffs_jvp(mhx_ctx* ctx, const mhx_mesh* mesh, const mhx_mesh_tang
                 mhx::launch_matmul_nr(st, n, R, dD, 0, b->Y, 0, t3, 0, b->work, 0, 1);
                 mhx::launch_axpby(st, cnt, t1, 1.0, t2, dphi);
                 mhx::launch_axpby(st, cnt, dphi, -1.0, t3, dphi);
-                mhx::getrs_batched(st, 0, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, dphi, 0, W2, 0, 1, &b->trs_epoch,
+                mhx::getrs_batched(st, 0, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, dphi, 0, W2, 0, 1,
                                    true);
             } else {
                 // sigma in B; dsigma = D^{-1} (dB - dD sigma); dphi = dS sigma + S dsigma
                 mhx::launch_matmul_nr(st, n, R, dD, 0, b->B, 0, t3, 0, b->work, 0, 1);
                 mhx::launch_axpby(st, cnt, dBv, -1.0, t3, t2);
-                mhx::getrs_batched(st, 0, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, t2, 0, W2, 0, 1, &b->trs_epoch,
+                mhx::getrs_batched(st, 0, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, t2, 0, W2, 0, 1,
                                    true);
                 mhx::launch_matmul_nr(st, n, R, dS, 0, b->B, 0, t1, 0, b->work, 0, 1);
                 mhx::launch_matmul_nr(st, n, R, b->S, 0, t2, 0, t3, 0, b->work, 0, 1);
@@ -1323,7 +1305,6 @@ int mhx_hydro_coeffs_vjp(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const
     const mhx::AdjointGrad gr{gV, gC, gN, gA, gD, wk};
     std::vector<cplx> F((size_t)ndof * R), FK((size_t)ndof * nb1), hFb(nd1 * R), hFKb(nd1 * nb1);
     const double g = ctx->g;
-    int* errw = mhx::getrs_error_word(n, b->work + 16 * b->nR());
     for (int64_t iw = 0; iw < nw && !rc; iw++) {
         const double w = omegas[iw], k = w * w / g;  // compute_wavenumber (src/waves.jl:133-134)
         if ((rc = set_freqs(ctx, b, omegas + iw, 1))) break;
@@ -1378,7 +1359,7 @@ int mhx_hydro_coeffs_vjp(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const
             cplx* Bbar;
             if (direct) {
                 // lambda = D^{-H} phibar; Bbar = S^H lambda; Sbar = lambda B^H; Dbar = -lambda phi^H
-                mhx::getrs_batched(st, 2, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, phib, 0, W2, 0, 1, &b->trs_epoch,
+                mhx::getrs_batched(st, 2, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, phib, 0, W2, 0, 1,
                                    true);
                 mhx::launch_matmul_h(st, n, R, b->S, phib, t1);
                 mhx::launch_scale(st, cnt, -1.0, phib, t2);
@@ -1389,7 +1370,7 @@ int mhx_hydro_coeffs_vjp(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const
             } else {
                 // sigmabar = S^H phibar; mu = D^{-H} sigmabar; Sbar = phibar sigma^H; Dbar = -mu sigma^H; Bbar = mu
                 mhx::launch_matmul_h(st, n, R, b->S, phib, t1);
-                mhx::getrs_batched(st, 2, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, t1, 0, W2, 0, 1, &b->trs_epoch,
+                mhx::getrs_batched(st, 2, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, t1, 0, W2, 0, 1,
                                    true);
                 mhx::launch_scale(st, cnt, -1.0, t1, t2);
                 U = phib;
@@ -1400,16 +1381,10 @@ int mhx_hydro_coeffs_vjp(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const
             mhx::launch_adjoint_bc(st, n, b->geom, b->dofs, ndof, b->betas, (int)nb, w, k, g, Bbar, gr);
             mhx::launch_adjoint_pairs(st, n, dv, df, dc, dn, da, dr, k, direct, tab, b->WG, b->WR, U, V, t2, V2, R, gr);
         }
-        int herr = 0;
         if (!ok(hipGetLastError(), "adjoint launch") ||
             !ok(hipMemcpyAsync(hwk, wk, 2 * sizeof(double), hipMemcpyDeviceToHost, st), "copy wk") ||
-            !ok(hipMemcpyAsync(&herr, errw, sizeof(int), hipMemcpyDeviceToHost, st), "copy err") ||
             !ok(hipStreamSynchronize(st), "sync"))
             break;
-        if (herr) {
-            rc = fail(ctx, MHX_EDEVICE, "adjoint solve: triangular sweep gave up waiting for a block");
-            break;
-        }
         if (grad->omegas) grad->omegas[iw] = wbar + hwk[0] + hwk[1] * 2.0 * w / g;
     }
     if (!rc) {
@@ -1584,10 +1559,9 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
     hipError_t e = hipSuccess;
     if (graph) {
         if (!reuse) {
-            for (auto* b : bodies) b->trs_epoch = 0;  // the graph zeroes the flags first; epochs restart at 1
             e = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed);
             if (e == hipSuccess) {
-                sweep_enqueue(ctx, *s, true);
+                sweep_enqueue(ctx, *s);
                 e = hipStreamEndCapture(st, &s->graph);
             }
             if (e == hipSuccess) e = hipGraphInstantiate(&s->exec, s->graph, nullptr, nullptr, 0);
@@ -1600,7 +1574,7 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
         e = hipGraphLaunch(s->exec, st);
     } else {
         Phase ph(ctx, "sweep");
-        sweep_enqueue(ctx, *s, false);
+        sweep_enqueue(ctx, *s);
         e = hipGetLastError();
     }
     if (e != hipSuccess) {
@@ -1609,12 +1583,9 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
     }
     // statuses, then the coefficients
     unsigned long long stq[4] = {0, 0, 0, 0};
-    std::vector<int> sticky(nbodies, 0), herr(nbodies, 0);
+    std::vector<int> sticky(nbodies, 0);
     hipMemcpyAsync(stq, s->status, sizeof stq, hipMemcpyDeviceToHost, st);
     hipMemcpyAsync(sticky.data(), s->sticky, sizeof(int) * nbodies, hipMemcpyDeviceToHost, st);
-    for (int64_t ib = 0; ib < nbodies; ib++)
-        hipMemcpyAsync(&herr[ib], mhx::getrs_error_word(bodies[ib]->n, bodies[ib]->work + 16 * bodies[ib]->nR()),
-                       sizeof(int), hipMemcpyDeviceToHost, st);
     if ((e = hipStreamSynchronize(st)) != hipSuccess) {
         abandon();
         return fail(ctx, MHX_EDEVICE, std::string("sweep: ") + hipGetErrorString(e));
@@ -1624,8 +1595,7 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
     rc = MHX_OK;
     if (stq[0]) rc = fail(ctx, MHX_EQUAD, "exact Delhommeau quadrature exceeded its interval budget");
     for (int64_t ib = 0; ib < nbodies && !rc; ib++) {
-        if (herr[ib]) rc = fail(ctx, MHX_EDEVICE, "solve: triangular sweep gave up waiting for a block");
-        else if (sticky[ib]) rc = fail(ctx, MHX_ESINGULAR, "LU: exactly zero pivot at column " + std::to_string(sticky[ib]));
+        if (sticky[ib]) rc = fail(ctx, MHX_ESINGULAR, "LU: exactly zero pivot at column " + std::to_string(sticky[ib]));
     }
     for (int64_t ib = 0; ib < nbodies && !rc; ib++) {
         mhx_body* b = bodies[ib];
@@ -1681,8 +1651,8 @@ void mhx_reset_timing(mhx_ctx* ctx) {
 
 int mhx_set_lu_mode(mhx_ctx* ctx, int mode) {
     if (!ctx) return MHX_EINVAL;
-    if (mode != MHX_LU_AUTO && mode != MHX_LU_FP64 && mode != MHX_LU_MIXED)
-        return fail(ctx, MHX_EINVAL, "lu mode must be MHX_LU_AUTO, MHX_LU_FP64 or MHX_LU_MIXED");
+    if (mode != MHX_LU_AUTO && mode != MHX_LU_FP64 && mode != MHX_LU_MIXED && mode != MHX_LU_MIXED_BF16)
+        return fail(ctx, MHX_EINVAL, "lu mode must be MHX_LU_AUTO, MHX_LU_FP64, MHX_LU_MIXED or MHX_LU_MIXED_BF16");
     ctx->lu_mode = mode;
     return MHX_OK;
 }

@@ -87,23 +87,22 @@ void zgemm(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cp
            int ldb, int64_t sB, cplx* C, int ldc, int64_t sC, int batch);
 int getrf_batched(hipStream_t st, int n, cplx* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
                   int* work_perm, int* info, LaunchTimer* update_timer = nullptr);
-// complex elements of getrs work per matrix (sW must be at least this; zero it when allocated).
-// getrs_batched keeps the diagonal-block inverses and the row permutation in work / perm: a later
-// call on the same factors with prepared = true reuses them.  *epoch: the work buffer's sweep
-// counter (start at 0 with zeroed work).  getrs_error_word: nonzero after a sweep gave up waiting.
+// complex elements of getrs work per matrix (sW must be at least this).  getrs_batched keeps the
+// diagonal-block inverses and the row permutation in work / perm: a later call on the same
+// factors with prepared = true reuses them.
 int64_t getrs_work(int n, int R);
-int* getrs_error_word(int n, cplx* work);
-// complex words at the start of an item's getrs work that hold its flags and error word
-int64_t getrs_flag_words(int n);
 int getrs_batched(hipStream_t st, int trans, int n, int R, const cplx* LU, int lda, int64_t sA, const int* ipiv,
                   int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
-                  int batch, int* epoch, bool prepared = false);
+                  int batch, bool prepared = false);
 // FP32 factorization (mixed-precision solver); the solves read FP32 factors and compute in FP64
+// split: workspace of split_work(n) bf16 per matrix for the split-bf16 trailing updates (nullptr:
+// FP32 MFMA updates)
 int getrf_batched(hipStream_t st, int n, cplxf* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
-                  int* work_perm, int* info, LaunchTimer* update_timer = nullptr);
+                  int* work_perm, int* info, LaunchTimer* update_timer = nullptr, unsigned short* split = nullptr);
+int64_t split_work(int n);
 int getrs_batched(hipStream_t st, int trans, int n, int R, const cplxf* LU, int lda, int64_t sA, const int* ipiv,
                   int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
-                  int batch, int* epoch, bool prepared = false);
+                  int batch, bool prepared = false);
 void convert_to_f32(hipStream_t st, int64_t count, const cplx* src, cplxf* dst);
 // T <- Bv - T (n R elements per item) and per-item squared norms |T|^2, |Bv|^2 added into norms[2z], norms[2z+1]
 void launch_residual(hipStream_t st, int64_t cnt, const cplx* Bv, int64_t sB, cplx* T, int64_t sT, double* norms,

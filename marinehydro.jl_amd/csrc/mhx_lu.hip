@@ -22,6 +22,8 @@
 // iterative refinement); the triangular solves always run in FP64 arithmetic.
 #include "mhx_internal.h"
 
+#include <type_traits>
+
 namespace mhx {
 
 template <class C> struct Cx;
@@ -487,12 +489,211 @@ void zgemm(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cp
 }
 
 // ---------------------------------------------------------------------------
+// Split-bf16 trailing update of the FP32 factor (mixed-precision solver).  Each panel element
+// x is stored as two bf16 numbers, hi = bf16(x) and lo = bf16(x - hi) (round to nearest even),
+// and a product x y is taken as hi*hi + hi*lo + lo*hi: bf16 products are exact in FP32, so the
+// error per product is about 2^-16 relative (the dropped lo*lo and the rounding of lo), against
+// 2^-24 for FP32; the solver's FP64 refinement removes the difference.  The products run on
+// v_mfma_f32_32x32x16_bf16 with FP32 accumulation: Re = Br Ar - Bi Ai and Im = Bi Ar + Br Ai
+// (six MFMAs each).  The MFMA's M dimension is the column of C and N its row, so that a
+// register of the 32x32 accumulator holds 32 consecutive rows of one column (coalesced
+// read-modify-write of the column-major C).
+//
+// Split arrays of one matrix (bf16, padded to SPLIT_TILE rows/cols, k-contiguous, NB k slots):
+//   [0 .. 4)  L21 panel rows:    re hi, re lo, im hi, im lo   each [mpad][NB]
+//   [4 .. 8)  U12 panel columns: re hi, re lo, im hi, im lo   each [mpad][NB]
+// ---------------------------------------------------------------------------
+constexpr int SPLIT_TILE = 128;
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+
+static inline int split_pad(int n) { return (n + SPLIT_TILE - 1) / SPLIT_TILE * SPLIT_TILE; }
+int64_t split_work(int n) { return (int64_t)8 * split_pad(n) * NB; }
+
+__device__ __forceinline__ unsigned short bf16_rne(float f) {
+    unsigned int u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float((unsigned int)h << 16); }
+
+// thread (r, kb): k slots 8 kb .. 8 kb + 7 of row r of L21 (rows [KE, n), columns [K, KE)) and of
+// column r of U12 (rows [K, KE), columns [KE, n)); rows past n - KE and k past KE - K are zero
+__global__ void split_panels_kernel(MatBatch<cplxf> mb, int n, int K, int KE, int mpad, unsigned short* __restrict__ Sb,
+                                    int64_t sS) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = t >> 3, kb = t & 7;
+    if (r >= mpad) return;
+    const cplxf* A = mb.A + mb.stride * blockIdx.z;
+    unsigned short* S = Sb + sS * blockIdx.z;
+    const int m = n - KE, kk = KE - K, lda = mb.lda;
+    unsigned short h[2][4][8];  // [panel][array][k]
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        const int k = 8 * kb + e;
+        cplxf a = make_float2(0.f, 0.f), b = make_float2(0.f, 0.f);
+        if (r < m && k < kk) {
+            a = A[(KE + r) + (int64_t)lda * (K + k)];
+            b = A[(K + k) + (int64_t)lda * (KE + r)];
+        }
+        const float v[2][2] = {{a.x, a.y}, {b.x, b.y}};
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const unsigned short hi = bf16_rne(v[p][c]);
+                h[p][2 * c][e] = hi;
+                h[p][2 * c + 1][e] = bf16_rne(v[p][c] - bf16_val(hi));
+            }
+    }
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            uint4 w;
+            w.x = h[p][q][0] | ((unsigned)h[p][q][1] << 16);
+            w.y = h[p][q][2] | ((unsigned)h[p][q][3] << 16);
+            w.z = h[p][q][4] | ((unsigned)h[p][q][5] << 16);
+            w.w = h[p][q][6] | ((unsigned)h[p][q][7] << 16);
+            *reinterpret_cast<uint4*>(S + ((int64_t)(4 * p + q) * mpad + r) * NB + 8 * kb) = w;
+        }
+}
+
+__device__ __forceinline__ bf16x8_t bf16x8_neg(bf16x8_t v) {
+    uint4 u = __builtin_bit_cast(uint4, v);
+    u.x ^= 0x80008000u;
+    u.y ^= 0x80008000u;
+    u.z ^= 0x80008000u;
+    u.w ^= 0x80008000u;
+    return __builtin_bit_cast(bf16x8_t, u);
+}
+
+// C[0:m, 0:m] -= L21 U12 from the split arrays.  Workgroup tile 128 x 128 of C, four waves of
+// 64 x 64 (2 x 2 accumulator tiles of 32 x 32 per product sign class); k in chunks of 32 staged
+// through LDS (rows padded to 80 B so the eight lanes of a 16-byte read phase hit distinct banks).
+constexpr int SPLIT_KC = 32;
+constexpr int SPLIT_LDS_ROW = SPLIT_KC / 8 + 1;  // bf16x8 units per row (4 used + 1 pad)
+
+__global__ void __launch_bounds__(256) zgemm_split_kernel(int m, int mpad, const unsigned short* __restrict__ Sb,
+                                                          int64_t sS, cplxf* __restrict__ Cb, int ldc, int64_t sC) {
+    __shared__ bf16x8_t La[4][SPLIT_TILE][SPLIT_LDS_ROW];  // L21 rows   (MFMA B operand, C rows)
+    __shared__ bf16x8_t Lb[4][SPLIT_TILE][SPLIT_LDS_ROW];  // U12 columns (MFMA A operand, C columns)
+    const unsigned short* S = Sb + sS * blockIdx.z;
+    cplxf* Cm = Cb + sC * blockIdx.z;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, l32 = lane & 31;
+    const int row0 = blockIdx.x * SPLIT_TILE, col0 = blockIdx.y * SPLIT_TILE;
+    const int wr = (wave & 1) * 64, wc = (wave >> 1) * 64;
+    f32x16_t acc_re[2][2], acc_im[2][2];  // [column tile][row tile]
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) acc_re[a][b][v] = acc_im[a][b][v] = 0.f;
+    const bf16x8_t* Sa = reinterpret_cast<const bf16x8_t*>(S);                              // arrays 0..3
+    const bf16x8_t* Sbv = reinterpret_cast<const bf16x8_t*>(S + (int64_t)4 * mpad * NB);  // arrays 4..7
+    constexpr int UPR = NB / 8;  // bf16x8 units per row in global
+    // stage: 4 arrays x 128 rows x 4 units for each panel = 2048 units each, 8 + 8 per thread; the
+    // next chunk is loaded into registers before the current chunk's MFMAs
+    bf16x8_t pa[8], pb[8];
+    auto load_chunk = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int idx = tid + 256 * e;
+            const int q = idx >> 9, r = (idx >> 2) & 127, u = idx & 3;
+            pa[e] = Sa[((int64_t)q * mpad + row0 + r) * UPR + k0 / 8 + u];
+            pb[e] = Sbv[((int64_t)q * mpad + col0 + r) * UPR + k0 / 8 + u];
+        }
+    };
+    load_chunk(0);
+    // the tile's C values, loaded now so that their latency hides behind the k loop:
+    // accumulator register v of lane (l32 + 32 h) is C row l32 (N index), column 8 (v / 4) + 4 h + v % 4
+    // (M index) of its 32 x 32 tile
+    cplxf old[2][2][16];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int r = row0 + wr + 32 * b + l32;
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int c = col0 + wc + 32 * a + 8 * (v >> 2) + 4 * h + (v & 3);
+                old[a][b][v] = (r < m && c < m) ? Cm[r + (int64_t)ldc * c] : make_float2(0.f, 0.f);
+            }
+        }
+    for (int k0 = 0; k0 < NB; k0 += SPLIT_KC) {
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int idx = tid + 256 * e;
+            const int q = idx >> 9, r = (idx >> 2) & 127, u = idx & 3;
+            La[q][r][u] = pa[e];
+            Lb[q][r][u] = pb[e];
+        }
+        __syncthreads();
+        if (k0 + SPLIT_KC < NB) load_chunk(k0 + SPLIT_KC);
+#pragma unroll
+        for (int ks = 0; ks < SPLIT_KC / 16; ks++) {
+            const int u = 2 * ks + h;
+            bf16x8_t brh[2], brl[2], bih[2], bil[2], nbih[2], nbil[2], arh[2], arl[2], aih[2], ail[2];
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                const int c = wc + 32 * t + l32, r = wr + 32 * t + l32;
+                brh[t] = Lb[0][c][u];
+                brl[t] = Lb[1][c][u];
+                bih[t] = Lb[2][c][u];
+                bil[t] = Lb[3][c][u];
+                nbih[t] = bf16x8_neg(bih[t]);
+                nbil[t] = bf16x8_neg(bil[t]);
+                arh[t] = La[0][r][u];
+                arl[t] = La[1][r][u];
+                aih[t] = La[2][r][u];
+                ail[t] = La[3][r][u];
+            }
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) {
+                    f32x16_t re = acc_re[a][b], im = acc_im[a][b];
+                    re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh[a], arh[b], re, 0, 0, 0);
+                    re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh[a], arl[b], re, 0, 0, 0);
+                    re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brl[a], arh[b], re, 0, 0, 0);
+                    re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbih[a], aih[b], re, 0, 0, 0);
+                    re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbih[a], ail[b], re, 0, 0, 0);
+                    re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbil[a], aih[b], re, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bih[a], arh[b], im, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bih[a], arl[b], im, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bil[a], arh[b], im, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh[a], aih[b], im, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh[a], ail[b], im, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brl[a], aih[b], im, 0, 0, 0);
+                    acc_re[a][b] = re;
+                    acc_im[a][b] = im;
+                }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const int r = row0 + wr + 32 * b + l32;
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int c = col0 + wc + 32 * a + 8 * (v >> 2) + 4 * h + (v & 3);
+                if (r < m && c < m)
+                    Cm[r + (int64_t)ldc * c] =
+                        make_float2(old[a][b][v].x - acc_re[a][b][v], old[a][b][v].y - acc_im[a][b][v]);
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
 // getrf driver.  ipiv: 0-based global row indices, int32, per matrix stride.
 // work_perm: int workspace of (1 + 4*NB) per matrix.  info: int per matrix (0 / first zero pivot, 1-based).
 // ---------------------------------------------------------------------------
 template <class C>
 static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
-                   int* work_perm, int* info, LaunchTimer* timer) {
+                   int* work_perm, int* info, LaunchTimer* timer, unsigned short* split = nullptr) {
     if (n <= 0) return 0;
     const int rows_per = (n + LEAF_THREADS - 1) / LEAF_THREADS;
     if (rows_per > 4) return -1;  // orders above 4096 need a larger register tile
@@ -528,8 +729,19 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
                 hipEventCreate(&e1);
                 hipEventRecord(e0, st);
             }
-            gemm_t<C>(st, true, true, n - KE, n - KE, KE - K, A + KE + (int64_t)lda * K, lda, strideA,
-                      A + K + (int64_t)lda * KE, lda, strideA, A + KE + (int64_t)lda * KE, lda, strideA, batch);
+            if constexpr (std::is_same<C, cplxf>::value) {
+                if (split) {
+                    const int m = n - KE, mpad = split_pad(m);
+                    const int64_t sS = split_work(n);
+                    split_panels_kernel<<<dim3((mpad * 8 + 255) / 256, 1, batch), 256, 0, st>>>(mb, n, K, KE, mpad,
+                                                                                              split, sS);
+                    zgemm_split_kernel<<<dim3(mpad / SPLIT_TILE, mpad / SPLIT_TILE, batch), 256, 0, st>>>(
+                        m, mpad, split, sS, A + KE + (int64_t)lda * KE, lda, strideA);
+                }
+            }
+            if (!split || !std::is_same<C, cplxf>::value)
+                gemm_t<C>(st, true, true, n - KE, n - KE, KE - K, A + KE + (int64_t)lda * K, lda, strideA,
+                          A + K + (int64_t)lda * KE, lda, strideA, A + KE + (int64_t)lda * KE, lda, strideA, batch);
             if (timer) {
                 hipEventRecord(e1, st);
                 timer->ev.emplace_back(e0, e1);
@@ -545,8 +757,8 @@ int getrf_batched(hipStream_t st, int n, cplx* A, int lda, int64_t strideA, int 
 }
 
 int getrf_batched(hipStream_t st, int n, cplxf* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
-                  int* work_perm, int* info, LaunchTimer* update_timer) {
-    return getrf_t(st, n, A, lda, strideA, batch, ipiv, ipiv_stride, work_perm, info, update_timer);
+                  int* work_perm, int* info, LaunchTimer* update_timer, unsigned short* split) {
+    return getrf_t(st, n, A, lda, strideA, batch, ipiv, ipiv_stride, work_perm, info, update_timer, split);
 }
 
 // ---------------------------------------------------------------------------
@@ -640,35 +852,10 @@ __global__ void __launch_bounds__(TRS_B) invert_diag_kernel(const C* __restrict_
 }
 
 // ---------------------------------------------------------------------------
-// Sync-free blocked triangular solve: one persistent launch per pass.
+// Blocked triangular solve, one launch per TRS_B-row block step (trs_step_kernel below):
 //   forward (lower=1): L (unit lower), or U^H when conj;  backward (lower=0): U, or L^H when conj.
-// Task (step, item) solves the TRS_B-row block blk = step (forward) or nblk-1-step (backward):
-//   acc = y_blk - sum over the blocks solved before it of op(T)[blk, c] x_c,  x_blk = op(Inv_blk) acc.
-// Tasks are dealt to workgroups in increasing step order and every workgroup runs its tasks in
-// that order; a task waits on the per-(item, block) flag of each dependency, which its producer
-// sets to this pass's epoch after its x stores have completed.  Flags and x are accessed only
-// with device-coherent (agent-scope) atomic loads and stores, so no cache-wide fence is needed:
-// the consumer issues its x loads after it has seen the flag, the producer raises the flag after
-// s_waitcnt vmcnt(0) on its x stores.  The smallest unfinished task therefore never waits on
-// anything unfinished, so the pass completes whenever the grid fits on the device at once (it is
-// sized to half of that).  A wait that exceeds TRS_SPIN_LIMIT polls (tens of milliseconds, far
-// beyond a pass) sets *err and gives up, and every later wait of the launch then gives up at
-// once (the result is invalid and the caller reports an error): no launch can hang.
+// Block blk's x is op(Inv_blk) times its y after the updates by every block solved before it.
 // ---------------------------------------------------------------------------
-#define TRS_SPIN_LIMIT (1 << 17)
-// s_waitcnt vmcnt(0) (gfx9-family encoding: expcnt and lgkmcnt left at their maxima)
-constexpr int kWaitVmcnt0 = 0x0F70;
-
-__device__ __forceinline__ cplx ld_agent(const cplx* p) {
-    const double* q = reinterpret_cast<const double*>(p);
-    return cmk(__hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
-               __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-}
-__device__ __forceinline__ void st_agent(cplx* p, cplx v) {
-    double* q = reinterpret_cast<double*>(p);
-    __hip_atomic_store(q, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(q + 1, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // element (a, q) of the dependency tile (a: index contiguous in memory), e = 0..15 per thread
 template <class C>
@@ -688,162 +875,157 @@ __device__ __forceinline__ void trs_load_tile(const C* __restrict__ A, int lda, 
     }
 }
 
+// One block step of the blocked triangular solve X = op(T)^{-1} Y, all items of the batch
+// (blockIdx.z), in sweep order (lower: block rows upwards from 0, upper: downwards from the last).
+//   first = 1: x_blk = op(Inv_blk) y_blk (one workgroup per item; the first block of the sweep);
+//   first = 0: workgroup j updates block row rb = blk +/- (j + 1):  y_rb -= op(T)[rb, blk] x_blk,
+//              and the workgroup of the next block of the sweep (j = 0), whose y is then final,
+//              forms x_rb = op(Inv_rb) y_rb.
+// The launches' order carries the dependencies (x_blk and y_rb were written by earlier launches),
+// so there are no waits inside a launch.  Each element of y is reduced in the order of the block
+// steps, tile by tile, in FP64.
 template <int RB, class C>
-__global__ void __launch_bounds__(256) trs_sweep_kernel(const C* __restrict__ Ab, int lda, int64_t sA, int n, int R,
-                                                        int lower, int conj, const cplx* __restrict__ Invb,
-                                                        int64_t sInv, const cplx* __restrict__ Yb, int64_t sY,
-                                                        cplx* __restrict__ Xb, int64_t sX, int* __restrict__ flags,
-                                                        int64_t sF, int epoch, int batch, int* __restrict__ err) {
+__global__ void __launch_bounds__(256) trs_step_kernel(const C* __restrict__ Ab, int lda, int64_t sA, int n, int R,
+                                                       int lower, int conj, int blk, int first,
+                                                       const cplx* __restrict__ Invb, int64_t sInv,
+                                                       cplx* __restrict__ Yb, int64_t sY, cplx* __restrict__ Xb,
+                                                       int64_t sX) {
     constexpr int CPT = RB / 4;  // right-hand-side columns per thread
     constexpr int TE = TRS_B * TRS_B / 256;
-    constexpr int XE = RB * TRS_B / 256;
-    __shared__ C tileT[TRS_B][TRS_B + 1];  // tileT[s][i] = op(T)[b0 + i][c0 + s]
+    // the factor tile (tileT[s][i] = op(T)[r0 + i][b0 + s]) and, once the update is done, the
+    // diagonal-block inverse (invs[s][i] = op(Inv)[i][s]) share one LDS buffer
+    constexpr size_t kTile = sizeof(C) * TRS_B * (TRS_B + 1), kInv = sizeof(cplx) * TRS_B * TRS_B;
+    __shared__ __align__(16) unsigned char smem[kTile > kInv ? kTile : kInv];
     __shared__ cplx xs[TRS_B][RB];
+    C(*tileT)[TRS_B + 1] = reinterpret_cast<C(*)[TRS_B + 1]>(smem);
+    cplx(*invs)[TRS_B] = reinterpret_cast<cplx(*)[TRS_B]>(smem);
     const int tid = threadIdx.x, i = tid & (TRS_B - 1), rq = tid / TRS_B;
-    const int nblk = (n + TRS_B - 1) / TRS_B;
-    const int ntask = nblk * batch;
-    for (int task = blockIdx.x; task < ntask; task += gridDim.x) {
-        const int step = task / batch, item = task - step * batch;
-        const int blk = lower ? step : nblk - 1 - step;
-        const int b0 = blk * TRS_B, nbk = min(TRS_B, n - b0);
-        const C* A = Ab + sA * item;
-        const cplx* Y = Yb + sY * item;
-        cplx* X = Xb + sX * item;
-        int* fl = flags + sF * item;
-        cplx acc[CPT];
+    const int item = blockIdx.z, j = blockIdx.x;
+    const int b0 = blk * TRS_B, nbk = min(TRS_B, n - b0);
+    const C* A = Ab + sA * item;
+    cplx* Y = Yb + sY * item;
+    cplx* X = Xb + sX * item;
+    const int rb = first ? blk : (lower ? blk + 1 + j : blk - 1 - j);
+    const int r0 = rb * TRS_B, nrb = min(TRS_B, n - r0);
+    const bool diag = first || j == 0;  // this workgroup forms x of block rb
+    // the inverse of block rb, loaded at once (its latency hides behind the update)
+    cplx ireg[TE];
+    if (diag) {
+        const cplx* Iv = Invb + sInv * item + (int64_t)rb * TRS_B * TRS_B;
 #pragma unroll
-        for (int cc = 0; cc < CPT; cc++) {
-            const int col = rq * CPT + cc;
-            acc[cc] = (i < nbk && col < R) ? Y[(b0 + i) + (int64_t)n * col] : cmk(0.0, 0.0);
+        for (int e = 0; e < TE; e++) ireg[e] = Iv[tid + 256 * e];
+    }
+    // x_rb = op(Inv_rb) xs (xs holds the final y of block rb); stored to X
+    auto solve_diag = [&]() {
+#pragma unroll
+        for (int e = 0; e < TE; e++) {
+            const int idx = tid + 256 * e;
+            const int a = idx & (TRS_B - 1), q = idx / TRS_B;  // element (a, q) of Inv_rb
+            const cplx v = ireg[e];
+            if (!conj) invs[q][a] = v;
+            else invs[a][q] = cmk(v.x, -v.y);
         }
-        // the factor tiles do not depend on the flags: the next one is loaded while this one is used
-        C reg[TE];
-        if (step > 0) {
-            const int cb = lower ? 0 : nblk - 1;
-            trs_load_tile(A, lda, conj, b0, nbk, cb * TRS_B, min(TRS_B, n - cb * TRS_B), tid, reg);
-        }
-        for (int d = 0; d < step; d++) {
-            const int cb = lower ? d : nblk - 1 - d;  // dependency blocks in the order they are produced
-            const int c0 = cb * TRS_B, ncb = min(TRS_B, n - c0);
-            if (tid == 0) {
-                int spins = 0;
-                while (__hip_atomic_load(fl + cb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-                    // give up after TRS_SPIN_LIMIT polls, or at once when another wait already gave up
-                    if ((++spins & 63) == 1 &&
-                        (spins > TRS_SPIN_LIMIT || __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))) {
-                        atomicOr(err, 1);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(2);
-                }
-            }
-            __syncthreads();  // flag seen; every thread is done with the previous tile
-#pragma unroll
-            for (int e = 0; e < TE; e++) {
-                const int idx = tid + 256 * e;
-                const int a = idx & (TRS_B - 1), q = idx / TRS_B;
-                C w = reg[e];
-                if (!conj) {
-                    tileT[q][a] = w;
-                } else {
-                    w.y = -w.y;
-                    tileT[a][q] = w;
-                }
-            }
-            cplx xv[XE];
-#pragma unroll
-            for (int e = 0; e < XE; e++) {
-                const int idx = tid + 256 * e;
-                const int s = idx % TRS_B, r = idx / TRS_B;
-                xv[e] = (s < ncb && r < R) ? ld_agent(X + (c0 + s) + (int64_t)n * r) : cmk(0.0, 0.0);
-            }
-#pragma unroll
-            for (int e = 0; e < XE; e++) {
-                const int idx = tid + 256 * e;
-                xs[idx % TRS_B][idx / TRS_B] = xv[e];
-            }
-            if (d + 1 < step) {
-                const int nb2 = lower ? d + 1 : nblk - 2 - d;
-                trs_load_tile(A, lda, conj, b0, nbk, nb2 * TRS_B, min(TRS_B, n - nb2 * TRS_B), tid, reg);
-            }
-            __syncthreads();
-#pragma unroll 8
-            for (int s = 0; s < TRS_B; s++) {
-                const cplx t = to_d(tileT[s][i]);
-#pragma unroll
-                for (int cc = 0; cc < CPT; cc++) acc[cc] = csub(acc[cc], cmul(t, xs[s][rq * CPT + cc]));
-            }
-        }
-        __syncthreads();  // all threads done with xs of the last dependency
-        // x_blk = op(Inv_blk) acc
-#pragma unroll
-        for (int cc = 0; cc < CPT; cc++) xs[i][rq * CPT + cc] = acc[cc];
         __syncthreads();
-        const cplx* Inv = Invb + sInv * item + (int64_t)blk * TRS_B * TRS_B;
-        cplx x[CPT];
+        if (i < nrb) {
+            cplx x[CPT];
 #pragma unroll
-        for (int cc = 0; cc < CPT; cc++) x[cc] = cmk(0.0, 0.0);
-        if (i < nbk) {
+            for (int cc = 0; cc < CPT; cc++) x[cc] = cmk(0.0, 0.0);
 #pragma unroll 4
-            for (int s = 0; s < nbk; s++) {
-                cplx v;
-                if (!conj) v = Inv[i + TRS_B * s];
-                else {
-                    const cplx w = Inv[s + TRS_B * i];
-                    v = cmk(w.x, -w.y);
-                }
+            for (int s = 0; s < nrb; s++) {
+                const cplx v = invs[s][i];
 #pragma unroll
                 for (int cc = 0; cc < CPT; cc++) x[cc] = cadd(x[cc], cmul(v, xs[s][rq * CPT + cc]));
             }
 #pragma unroll
             for (int cc = 0; cc < CPT; cc++) {
                 const int col = rq * CPT + cc;
-                if (col < R) st_agent(X + (b0 + i) + (int64_t)n * col, x[cc]);
+                if (col < R) X[(r0 + i) + (int64_t)n * col] = x[cc];
             }
         }
-        // publish: the x stores above are device-coherent (agent-scope atomics); wait until every
-        // thread's stores have completed, then raise the flag
-        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
-        __syncthreads();  // also: every thread is done with tileT and xs before the next task
-        if (tid == 0) __hip_atomic_store(fl + blk, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    };
+    if (first) {
+#pragma unroll
+        for (int cc = 0; cc < CPT; cc++) {
+            const int col = rq * CPT + cc;
+            xs[i][col] = (i < nrb && col < R) ? Y[(r0 + i) + (int64_t)n * col] : cmk(0.0, 0.0);
+        }
+        solve_diag();
+        return;
+    }
+    C reg[TE];
+    trs_load_tile(A, lda, conj, r0, nrb, b0, nbk, tid, reg);
+#pragma unroll
+    for (int cc = 0; cc < CPT; cc++) {
+        const int col = rq * CPT + cc;
+        xs[i][col] = (i < nbk && col < R) ? X[(b0 + i) + (int64_t)n * col] : cmk(0.0, 0.0);
+    }
+    cplx acc[CPT];
+#pragma unroll
+    for (int cc = 0; cc < CPT; cc++) {
+        const int col = rq * CPT + cc;
+        acc[cc] = (i < nrb && col < R) ? Y[(r0 + i) + (int64_t)n * col] : cmk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int e = 0; e < TE; e++) {
+        const int idx = tid + 256 * e;
+        const int a = idx & (TRS_B - 1), q = idx / TRS_B;
+        C w = reg[e];
+        if (!conj) {
+            tileT[q][a] = w;
+        } else {
+            w.y = -w.y;
+            tileT[a][q] = w;
+        }
+    }
+    __syncthreads();
+#pragma unroll 8
+    for (int s = 0; s < TRS_B; s++) {
+        const cplx t = to_d(tileT[s][i]);
+#pragma unroll
+        for (int cc = 0; cc < CPT; cc++) acc[cc] = csub(acc[cc], cmul(t, xs[s][rq * CPT + cc]));
+    }
+    if (diag) {
+        __syncthreads();  // every thread is done with tileT and with xs of block blk
+#pragma unroll
+        for (int cc = 0; cc < CPT; cc++) xs[i][rq * CPT + cc] = acc[cc];
+        solve_diag();
+    } else if (i < nrb) {
+#pragma unroll
+        for (int cc = 0; cc < CPT; cc++) {
+            const int col = rq * CPT + cc;
+            if (col < R) Y[(r0 + i) + (int64_t)n * col] = acc[cc];
+        }
     }
 }
 
-// persistent grid: at most half of the workgroups the device can hold at once
 template <int RB, class C>
-static int trs_grid(int ntask) {
-    int dev = 0, cus = 0, occ = 0;
-    hipGetDevice(&dev);
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, trs_sweep_kernel<RB, C>, 256, 0) != hipSuccess || occ <= 0)
-        occ = 1;
-    return std::max(1, std::min(ntask, std::max(1, cus * occ / 2)));
+static void trs_pass_t(hipStream_t st, const C* A, int lda, int64_t sA, int n, int R, int lower, int conj,
+                       const cplx* Inv, int64_t sInv, cplx* Y, int64_t sY, cplx* X, int64_t sX, int batch) {
+    const int nblk = (n + TRS_B - 1) / TRS_B;
+    const int blk0 = lower ? 0 : nblk - 1;
+    trs_step_kernel<RB, C><<<dim3(1, 1, batch), 256, 0, st>>>(A, lda, sA, n, R, lower, conj, blk0, 1, Inv, sInv, Y, sY,
+                                                               X, sX);
+    for (int step = 0; step + 1 < nblk; step++) {
+        const int blk = lower ? step : nblk - 1 - step;
+        trs_step_kernel<RB, C><<<dim3(nblk - 1 - step, 1, batch), 256, 0, st>>>(A, lda, sA, n, R, lower, conj, blk, 0,
+                                                                                 Inv, sInv, Y, sY, X, sX);
+    }
 }
 
+// X = op(T)^{-1} Y; Y is overwritten (block rows are reduced in place)
 template <class C>
 static void trs_pass(hipStream_t st, const C* A, int lda, int64_t sA, int n, int R, int lower, int conj,
-                     const cplx* Inv, int64_t sInv, const cplx* Y, int64_t sY, cplx* X, int64_t sX, int* flags,
-                     int64_t sF, int epoch, int batch, int* err) {
-    const int ntask = ((n + TRS_B - 1) / TRS_B) * batch;
-    if (R <= 4)
-        trs_sweep_kernel<4, C><<<trs_grid<4, C>(ntask), 256, 0, st>>>(A, lda, sA, n, R, lower, conj, Inv, sInv, Y, sY,
-                                                                      X, sX, flags, sF, epoch, batch, err);
-    else if (R <= 8)
-        trs_sweep_kernel<8, C><<<trs_grid<8, C>(ntask), 256, 0, st>>>(A, lda, sA, n, R, lower, conj, Inv, sInv, Y, sY,
-                                                                      X, sX, flags, sF, epoch, batch, err);
-    else
-        trs_sweep_kernel<16, C><<<trs_grid<16, C>(ntask), 256, 0, st>>>(A, lda, sA, n, R, lower, conj, Inv, sInv, Y,
-                                                                        sY, X, sX, flags, sF, epoch, batch, err);
+                     const cplx* Inv, int64_t sInv, cplx* Y, int64_t sY, cplx* X, int64_t sX, int batch) {
+    if (R <= 4) trs_pass_t<4, C>(st, A, lda, sA, n, R, lower, conj, Inv, sInv, Y, sY, X, sX, batch);
+    else if (R <= 8) trs_pass_t<8, C>(st, A, lda, sA, n, R, lower, conj, Inv, sInv, Y, sY, X, sX, batch);
+    else trs_pass_t<16, C>(st, A, lda, sA, n, R, lower, conj, Inv, sInv, Y, sY, X, sX, batch);
 }
 
 // B (n x R, leading dimension n) is overwritten with the solution.  work per matrix (stride sW):
-// the wait flags of the triangular sweeps plus one error word (ceil(n/64) + 1 ints, rounded up to
-// whole complex elements; first, so that their place does not depend on R), W1, W2 (n R complex
-// each) and the diagonal-block inverses of L and U (2 ceil(n/64) 64^2 complex).  The flags must
-// be zero when the work memory is first used; *epoch (host counter, one per work buffer) is
-// advanced by every sweep.
+// a reserved header of ceil(n/64) + 1 ints (rounded up to whole complex elements; first, so that
+// the rest does not move with R); W1, W2 (n R complex each); the
+// diagonal-block inverses of L and U (2 ceil(n/64) 64^2 complex).
 static int64_t flag_words(int n) { return ((n + TRS_B - 1) / TRS_B + 1 + 3) / 4; }
-int64_t getrs_flag_words(int n) { return flag_words(n); }
 
 int64_t getrs_work(int n, int R) {
     const int64_t nblk = (n + TRS_B - 1) / TRS_B;
@@ -853,17 +1035,14 @@ int64_t getrs_work(int n, int R) {
 template <class C>
 static int getrs_t(hipStream_t st, int trans, int n, int R, const C* LU, int lda, int64_t sA, const int* ipiv,
                    int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
-                   int batch, bool prepared, int* epoch) {
+                   int batch, bool prepared) {
     if (R > MAXR) return -1;
     if (n <= 0 || R <= 0) return 0;
     const int nblk = (n + TRS_B - 1) / TRS_B;
-    int* flags = reinterpret_cast<int*>(work);
-    int* err = flags + nblk;  // error word of item 0
     cplx* W1 = work + flag_words(n);
     cplx* W2 = W1 + (int64_t)n * R;
     cplx* Linv = W2 + (int64_t)n * R;
     cplx* Uinv = Linv + (int64_t)nblk * TRS_B * TRS_B;
-    const int64_t sF = sW * (int64_t)(sizeof(cplx) / sizeof(int));
     if (!prepared) {
         invert_diag_kernel<C><<<dim3(nblk, 2, batch), TRS_B, 0, st>>>(LU, lda, sA, n, Linv, Uinv, sW);
         ipiv_to_perm_kernel<<<dim3(1, 1, batch), 256, 0, st>>>(ipiv, ipiv_stride, n, perm, perm_stride);
@@ -871,13 +1050,13 @@ static int getrs_t(hipStream_t st, int trans, int n, int R, const C* LU, int lda
     dim3 gp((n + 255) / 256, 1, batch);
     if (trans == 0) {
         permute_rows_kernel<<<gp, 256, 0, st>>>(n, R, perm, perm_stride, B, sB, W1, sW, 0);
-        trs_pass(st, LU, lda, sA, n, R, 1, 0, Linv, sW, W1, sW, W2, sW, flags, sF, ++*epoch, batch, err);  // L
-        trs_pass(st, LU, lda, sA, n, R, 0, 0, Uinv, sW, W2, sW, B, sB, flags, sF, ++*epoch, batch, err);   // U
+        trs_pass(st, LU, lda, sA, n, R, 1, 0, Linv, sW, W1, sW, W2, sW, batch);  // L
+        trs_pass(st, LU, lda, sA, n, R, 0, 0, Uinv, sW, W2, sW, B, sB, batch);   // U
     } else {
         for (int b = 0; b < batch; b++)
             hipMemcpyAsync(W1 + sW * b, B + sB * b, sizeof(cplx) * (size_t)n * R, hipMemcpyDeviceToDevice, st);
-        trs_pass(st, LU, lda, sA, n, R, 1, 1, Uinv, sW, W1, sW, W2, sW, flags, sF, ++*epoch, batch, err);  // U^H
-        trs_pass(st, LU, lda, sA, n, R, 0, 1, Linv, sW, W2, sW, W1, sW, flags, sF, ++*epoch, batch, err);  // L^H
+        trs_pass(st, LU, lda, sA, n, R, 1, 1, Uinv, sW, W1, sW, W2, sW, batch);  // U^H
+        trs_pass(st, LU, lda, sA, n, R, 0, 1, Linv, sW, W2, sW, W1, sW, batch);  // L^H
         permute_rows_kernel<<<gp, 256, 0, st>>>(n, R, perm, perm_stride, W1, sW, B, sB, 1);
     }
     return 0;
@@ -885,20 +1064,17 @@ static int getrs_t(hipStream_t st, int trans, int n, int R, const C* LU, int lda
 
 int getrs_batched(hipStream_t st, int trans, int n, int R, const cplx* LU, int lda, int64_t sA, const int* ipiv,
                   int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
-                  int batch, int* epoch, bool prepared) {
-    return getrs_t(st, trans, n, R, LU, lda, sA, ipiv, ipiv_stride, perm, perm_stride, B, sB, work, sW, batch, prepared,
-                   epoch);
+                  int batch, bool prepared) {
+    return getrs_t(st, trans, n, R, LU, lda, sA, ipiv, ipiv_stride, perm, perm_stride, B, sB, work, sW, batch, prepared);
 }
 
 
 int getrs_batched(hipStream_t st, int trans, int n, int R, const cplxf* LU, int lda, int64_t sA, const int* ipiv,
                   int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
-                  int batch, int* epoch, bool prepared) {
-    return getrs_t(st, trans, n, R, LU, lda, sA, ipiv, ipiv_stride, perm, perm_stride, B, sB, work, sW, batch, prepared,
-                   epoch);
+                  int batch, bool prepared) {
+    return getrs_t(st, trans, n, R, LU, lda, sA, ipiv, ipiv_stride, perm, perm_stride, B, sB, work, sW, batch, prepared);
 }
 
-int* getrs_error_word(int n, cplx* work) { return reinterpret_cast<int*>(work) + (n + TRS_B - 1) / TRS_B; }
 
 
 // ---------------------------------------------------------------------------

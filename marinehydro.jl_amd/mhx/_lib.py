@@ -29,6 +29,7 @@ MODE_BATCHED = 2
 LU_AUTO = 0
 LU_FP64 = 1
 LU_MIXED = 2
+LU_MIXED_BF16 = 3
 
 EXPORTED = ["mhx_create", "mhx_destroy", "mhx_last_error", "mhx_set_constants", "mhx_assemble", "mhx_solve",
             "mhx_lu_solve", "mhx_hydro_coeffs", "mhx_body_create", "mhx_body_destroy", "mhx_body_coeffs",
@@ -191,7 +192,8 @@ class Context:
         lib().mhx_reset_timing(self.h)
 
     def set_lu_mode(self, mode):
-        """LU_AUTO (mixed precision for the surrogate wave term), LU_FP64 or LU_MIXED."""
+        """LU_AUTO (mixed precision for the surrogate wave term), LU_FP64, LU_MIXED or LU_MIXED_BF16
+        (the FP32 factor's trailing updates on split-bf16 MFMA)."""
         self.check(lib().mhx_set_lu_mode(self.h, int(mode)))
 
     def solver_stats(self):

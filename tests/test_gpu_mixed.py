@@ -1,5 +1,6 @@
-"""Mixed-precision dense solver (FP32 LU on FP32 MFMA, FP64 residuals and iterative
-refinement, FP64 fallback) against LAPACK and the oracle, through the C ABI."""
+"""Mixed-precision dense solver (FP32 LU with trailing updates on FP32 MFMA or on split-bf16
+MFMA, FP64 residuals and iterative refinement, FP64 fallback) against LAPACK and the oracle,
+through the C ABI."""
 import math
 
 import numpy as np
@@ -18,17 +19,19 @@ def mhx():
     return M
 
 
-@pytest.fixture
-def mixed(mhx):
+@pytest.fixture(params=["fp32", "bf16"])
+def mixed(mhx, request):
+    """The mixed-precision solver with FP32 trailing updates (LU_MIXED) and with split-bf16 ones
+    (LU_MIXED_BF16)."""
     from mhx import _lib
     ctx = mhx.context()
-    ctx.set_lu_mode(_lib.LU_MIXED)
+    ctx.set_lu_mode(_lib.LU_MIXED if request.param == "fp32" else _lib.LU_MIXED_BF16)
     ctx.reset_timing()
     yield ctx
     ctx.set_lu_mode(_lib.LU_AUTO)
 
 
-@pytest.mark.parametrize("n", [5, 64, 130, 300])
+@pytest.mark.parametrize("n", [5, 64, 130, 300, 1000, 1100])
 @pytest.mark.parametrize("direct", [True, False])
 def test_mixed_solve_matches_lapack(mhx, mixed, n, direct):
     """Diagonally dominated complex systems (BEM-like): refined solution within 1e-12 of LAPACK."""
