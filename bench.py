@@ -41,7 +41,8 @@ BF16_PEAK_TFLOPS = 2500.0
 
 WAVE_DESC = {"Surrogate": "learned surrogate (Wu + BF16 MFMA residual network, exact FP64 fallback)",
              "ExactGuevelDelhommeau": "exact FP64 Guevel-Delhommeau", "Wu": "FP64 Liang-Wu-Noblesse fit"}
-DTYPE = {"Surrogate": "f64 (wave-term residual network: bf16 MFMA, f32 accumulate)",
+DTYPE = {"Surrogate": "f64 (wave-term residual network: bf16 MFMA, f32 accumulate; dense solve: FP32 LU with "
+                      "split-bf16 MFMA trailing updates, FP64 residuals and refinement to 1e-12)",
          "ExactGuevelDelhommeau": "f64", "Wu": "f64"}
 
 
@@ -143,10 +144,32 @@ def lu_update_flops(n, nb=64):
     return f
 
 
-def measure(ctx, run, ws, n, gf, nlu=None, units=None):
+def lu_update_bytes(n, nb=64, tile=128):
+    """Algorithmic HBM bytes of the split-bf16 trailing updates of one blocked LU: the trailing
+    matrix read and written once per step (FP32 complex, 16 B per element) and the step's
+    split panels (8 bf16 arrays of mpad x nb) read once."""
+    b = 0.0
+    for K in range(0, n, nb):
+        KE = min(n, K + nb)
+        if KE < n:
+            m = n - KE
+            mpad = (m + tile - 1) // tile * tile
+            b += 16.0 * m * m + 8.0 * mpad * nb * 2
+    return b
+
+
+def lu_kind(mode, gf):
+    """Trailing-update kernel the dense solver uses: 'split' (bf16 hi/lo MFMA), 'fp32' or 'fp64'."""
+    if mode == "mixed_bf16" or (mode == "auto" and gf == "Surrogate"):
+        return "split"
+    return "fp32" if mode == "mixed" else "fp64"
+
+
+def measure(ctx, run, ws, n, gf, nlu=None, units=None, kind="fp64"):
     """One timed call over the frequencies ws; returns rate, phase table and roofline of the
     dominant kernel (per-launch HIP events on the library's stream).  nlu: factorizations the call
-    makes (default one per frequency); units: what the rate counts (default the frequencies)."""
+    makes (default one per frequency); units: what the rate counts (default the frequencies);
+    kind: the trailing-update kernel (lu_kind)."""
     ctx.reset_timing()
     t0 = time.perf_counter()
     A, B, Fd, Fk = run(ws)
@@ -160,25 +183,40 @@ def measure(ctx, run, ws, n, gf, nlu=None, units=None):
     tl, cl = phases["lu"]
     nw = len(ws) if nlu is None else nlu
     units = len(ws) if units is None else units
-    # the surrogate path factors in FP32 (mixed-precision solver), the others in FP64
-    fp32 = gf == "Surrogate"
-    lu_peak = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
     fu = lu_update_flops(n) * nw
-    traffic = None
-    if fp32 and n == 4096 and nw == 64 and cu == 63:   # the profiled configuration
-        traffic = profiled_traffic("lu_update_4096x64_traffic.md")
-    rl_update = {"bound": "mfma", "kernel": "zgemm_mfma_kernel<%s> (LU trailing update)" % ("cplxf" if fp32 else "cplx"),
-                 "achieved": fu / (tu * 1e-3) / 1e12 if tu else None, "peak": lu_peak, "unit": "TFLOP/s",
-                 "traffic": traffic, "avg_launch_ms": tu / cu if cu else None, "launches": cu,
-                 "algorithmic_flops_per_launch": fu / cu if cu else None}
-    if rl_update["achieved"]:
-        rl_update["frac"] = rl_update["achieved"] / lu_peak
+    if kind == "split":
+        # the split-bf16 update moves the trailing matrix once per step: HBM-bound
+        bu = lu_update_bytes(n) * nw
+        traffic = None
+        if n == 4096 and nw == 64 and cu == 63:   # the profiled configuration
+            traffic = profiled_traffic("lu_update_split_4096x64_traffic.md")
+        rl_update = {"bound": "hbm", "kernel": "zgemm_split_kernel (LU trailing update, split-bf16 MFMA)",
+                     "achieved": bu / (tu * 1e-3) / 1e9 if tu else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "traffic": traffic, "avg_launch_ms": tu / cu if cu else None, "launches": cu,
+                     "algorithmic_bytes_per_launch": bu / cu if cu else None,
+                     "complex_tflops": fu / (tu * 1e-3) / 1e12 if tu else None}
+        if rl_update["achieved"]:
+            rl_update["frac"] = rl_update["achieved"] / HBM_PEAK_GBS
+    else:
+        fp32 = kind == "fp32"
+        lu_peak = FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS
+        rl_update = {"bound": "mfma",
+                     "kernel": "zgemm_mfma_kernel<%s> (LU trailing update)" % ("cplxf" if fp32 else "cplx"),
+                     "achieved": fu / (tu * 1e-3) / 1e12 if tu else None, "peak": lu_peak, "unit": "TFLOP/s",
+                     "traffic": None, "avg_launch_ms": tu / cu if cu else None, "launches": cu,
+                     "algorithmic_flops_per_launch": fu / cu if cu else None}
+        if rl_update["achieved"]:
+            rl_update["frac"] = rl_update["achieved"] / lu_peak
     if gf == "Surrogate":
         pairs = n * (n + 1) / 2 * nw
         ach = pairs * surrogate_flops_per_pair() / (tw * 1e-3) / 1e12
+        traffic = None
+        if n == 4096 and nw == 64 and cw == 1:   # the profiled configuration
+            traffic = profiled_traffic("surrogate_table_4096x64_traffic.md")
         rl_table = {"bound": "mfma", "kernel": "surr_table_kernel (+ exact fallback)", "achieved": ach,
-                    "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / BF16_PEAK_TFLOPS, "traffic": None,
-                    "avg_launch_ms": tw / cw, "launches": cw, "frequencies_per_launch": nw / cw}
+                    "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / BF16_PEAK_TFLOPS, "traffic": traffic,
+                    "avg_launch_ms": tw / cw, "launches": cw, "frequencies_per_launch": nw / cw,
+                    "algorithmic_flops_per_launch": pairs * surrogate_flops_per_pair() / cw}
     elif gf == "ExactGuevelDelhommeau":
         ach = flops / (tw * 1e-3) / 1e12
         rl_table = {"bound": "valu", "kernel": "delhommeau_table_kernel", "achieved": ach,
@@ -254,7 +292,8 @@ def main():
     warm = steps_w[:max(args.warmup, 1)]
     run_with(mode)(warm)   # body upload, Rankine part, buffers: outside the timed region
     log(f"bench: timing {args.steps} omega-solves ({args.schedule})")
-    value, elapsed, phases, roof, lu_tflops, res, roof2 = measure(ctx, run_with(mode), steps_w, n, args.gf)
+    value, elapsed, phases, roof, lu_tflops, res, roof2 = measure(ctx, run_with(mode), steps_w, n, args.gf,
+                                                                  kind=lu_kind(args.lu_mode, args.gf))
     solved, steps, fallbacks, resid = ctx.solver_stats()
     solver = {"mixed_precision_systems": solved, "refinement_steps": steps, "fp64_fallbacks": fallbacks,
               "max_relative_residual": resid}
@@ -269,6 +308,8 @@ def main():
                                                     _lib.MODE_PER_OMEGA, 1025.0, 9.81)
         ctx.set_lu_mode(_lib.LU_FP64)  # the plain path: FP64 LU, as the reference
         pv, pel, pph, proof, plu, pres, _ = measure(ctx, run_plain, steps_w[sub], n, "ExactGuevelDelhommeau")
+        ctx.set_lu_mode({"auto": _lib.LU_AUTO, "fp64": _lib.LU_FP64, "mixed": _lib.LU_MIXED,
+                         "mixed_bf16": _lib.LU_MIXED_BF16}[args.lu_mode])
         errs = {}
         pairs = (("added_mass", res[0][:, :, sub], pres[0], lambda x, q: x[:, :, q]),
                  ("radiation_damping", res[1][:, :, sub], pres[1], lambda x, q: x[:, :, q]),
@@ -330,7 +371,8 @@ def bench_forward_speed(args, mhx, _lib, ctx, body, names, mesh, omegas):
     groups = nu * len(ws) * nb
     nlu = len(ws) * (1 + nb * sum(1 for u in FS_SPEEDS if u != 0))
     log(f"bench: timing {groups} problem groups ({nlu} factorizations)")
-    value, elapsed, phases, roof, lu_tflops, res, roof2 = measure(ctx, run, ws, n, gf, nlu=nlu, units=groups)
+    value, elapsed, phases, roof, lu_tflops, res, roof2 = measure(ctx, run, ws, n, gf, nlu=nlu, units=groups,
+                                                                  kind=lu_kind(args.lu_mode, gf))
     log("per-factorization phase ms:", phases)
     cpu = None
     if not args.no_cpu_baseline:

@@ -140,14 +140,15 @@ int mhx_set_batch(mhx_ctx* ctx, int64_t max_items, double max_bytes);
  *                 factors in FP64 arithmetic, FP64 residuals against the FP64 matrix and iterative
  *                 refinement to |b - A x| <= 1e-12 |b| per system; an item that does not reach it in
  *                 4 steps (or has a singular FP32 factor) is re-solved with the FP64 LU;
- *   MHX_LU_AUTO   (default) mixed for MHX_GF_SURROGATE, FP64 otherwise.
+ *   MHX_LU_MIXED_BF16  as MHX_LU_MIXED with the FP32 factor's trailing updates on bf16 MFMA, each
+ *                 operand split into bf16 hi + lo (products to ~2^-16), up to 8 refinement steps;
+ *   MHX_LU_AUTO   (default) MHX_LU_MIXED_BF16 for MHX_GF_SURROGATE, FP64 otherwise.
  * mhx_solver_stats: counts[0] systems solved mixed, counts[1] refinement steps, counts[2] FP64
  * fallbacks; max_residual: largest accepted relative residual (since mhx_reset_timing). */
 #define MHX_LU_AUTO 0
 #define MHX_LU_FP64 1
 #define MHX_LU_MIXED 2
-#define MHX_LU_MIXED_BF16 3  /* as MHX_LU_MIXED, the FP32 factor's trailing updates on bf16 MFMA with each
-                                operand split into bf16 hi + lo (products to ~2^-16), up to 8 refinement steps */
+#define MHX_LU_MIXED_BF16 3
 int mhx_set_lu_mode(mhx_ctx* ctx, int mode);
 int mhx_solver_stats(mhx_ctx* ctx, int64_t counts[3], double* max_residual);
 

