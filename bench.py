@@ -134,27 +134,41 @@ def profiled_traffic(name):
     return None
 
 
-def lu_update_flops(n, nb=64):
+def lu_update_steps(n, nb=64, two_level=False, super_=4):
+    """(K, KE) of the LU's timed trailing updates: one per 64-column panel, or (split updates) one
+    per group of up to 4 panels that leaves a trailing matrix."""
+    steps, K = [], 0
+    while K < n:
+        S = 1
+        if two_level:
+            while S < super_ and K + (S + 1) * nb < n:
+                S += 1
+        if S > 1:
+            steps.append((K, K + S * nb))
+            K += S * nb
+        else:
+            KE = min(n, K + nb)
+            if KE < n:
+                steps.append((K, KE))
+            K = KE
+    return steps
+
+
+def lu_update_flops(n, nb=64, two_level=False):
     """Algorithmic flops of the trailing updates of one blocked LU (complex multiply-add = 8 flops)."""
-    f = 0.0
-    for K in range(0, n, nb):
-        KE = min(n, K + nb)
-        if KE < n:
-            f += 8.0 * (n - KE) ** 2 * (KE - K)
-    return f
+    return sum(8.0 * (n - KE) ** 2 * (KE - K) for K, KE in lu_update_steps(n, nb, two_level))
 
 
 def lu_update_bytes(n, nb=64, tile=128):
-    """Algorithmic HBM bytes of the split-bf16 trailing updates of one blocked LU: the trailing
-    matrix read and written once per step (FP32 complex, 16 B per element) and the step's
-    split panels (8 bf16 arrays of mpad x nb) read once."""
+    """Algorithmic HBM bytes of the split-bf16 trailing updates of one blocked LU (multi-panel
+    steps): the trailing matrix read and written once per update (FP32 complex, 16 B per
+    element) and the update's split panels (8 bf16 arrays of mpad x ks) read once."""
     b = 0.0
-    for K in range(0, n, nb):
-        KE = min(n, K + nb)
-        if KE < n:
-            m = n - KE
-            mpad = (m + tile - 1) // tile * tile
-            b += 16.0 * m * m + 8.0 * mpad * nb * 2
+    for K, KE in lu_update_steps(n, nb, True):
+        m = n - KE
+        mpad = (m + tile - 1) // tile * tile
+        ks = (KE - K + nb - 1) // nb * nb
+        b += 16.0 * m * m + 8.0 * mpad * ks * 2
     return b
 
 
@@ -183,12 +197,12 @@ def measure(ctx, run, ws, n, gf, nlu=None, units=None, kind="fp64"):
     tl, cl = phases["lu"]
     nw = len(ws) if nlu is None else nlu
     units = len(ws) if units is None else units
-    fu = lu_update_flops(n) * nw
+    fu = lu_update_flops(n, two_level=kind == "split") * nw
     if kind == "split":
         # the split-bf16 update moves the trailing matrix once per step: HBM-bound
         bu = lu_update_bytes(n) * nw
         traffic = None
-        if n == 4096 and nw == 64 and cu == 63:   # the profiled configuration
+        if n == 4096 and nw == 64 and cu == len(lu_update_steps(n, two_level=True)):  # the profiled configuration
             traffic = profiled_traffic("lu_update_split_4096x64_traffic.md")
         rl_update = {"bound": "hbm", "kernel": "zgemm_split_kernel (LU trailing update, split-bf16 MFMA)",
                      "achieved": bu / (tu * 1e-3) / 1e9 if tu else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",

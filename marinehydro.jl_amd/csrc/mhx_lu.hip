@@ -228,12 +228,21 @@ __global__ void __launch_bounds__(256) panel_update_kernel(MatBatch<C> mb, int n
     C l[LW];
 #pragma unroll
     for (int s = 0; s < LW; s++) l[s] = A[r + (int64_t)lda * (c0 + s)];
-    for (int jj = 0; jj < nj; jj++) {
-        C* p = A + r + (int64_t)lda * (j0 + jj);
-        C a = *p;
+    // 16 columns per round trip: their loads are issued before any of their stores (element by
+    // element, each load would wait for the previous store through the same pointer)
+    constexpr int JC = 16;
+    for (int jb = 0; jb < nj; jb += JC) {
+        C a[JC];
 #pragma unroll
-        for (int s = 0; s < LW; s++) a = tsub(a, tmul(l[s], U[s][jj]));
-        *p = a;
+        for (int q = 0; q < JC; q++)
+            if (jb + q < nj) a[q] = A[r + (int64_t)lda * (j0 + jb + q)];
+#pragma unroll
+        for (int q = 0; q < JC; q++)
+            if (jb + q < nj) {
+#pragma unroll
+                for (int s = 0; s < LW; s++) a[q] = tsub(a[q], tmul(l[s], U[s][jb + q]));
+                A[r + (int64_t)lda * (j0 + jb + q)] = a[q];
+            }
     }
 }
 
@@ -539,16 +548,18 @@ void zgemm(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cp
 // register of the 32x32 accumulator holds 32 consecutive rows of one column (coalesced
 // read-modify-write of the column-major C).
 //
-// Split arrays of one matrix (bf16, padded to SPLIT_TILE rows/cols, k-contiguous, NB k slots):
-//   [0 .. 4)  L21 panel rows:    re hi, re lo, im hi, im lo   each [mpad][NB]
-//   [4 .. 8)  U12 panel columns: re hi, re lo, im hi, im lo   each [mpad][NB]
+// Split arrays of one matrix (bf16, padded to SPLIT_TILE rows/cols, k-contiguous, ks = 64 S k
+// slots for the S <= SPLIT_SUPER panels of one update):
+//   [0 .. 4)  L21 rows:    re hi, re lo, im hi, im lo   each [mpad][ks]
+//   [4 .. 8)  U12 columns: re hi, re lo, im hi, im lo   each [mpad][ks]
 // ---------------------------------------------------------------------------
 constexpr int SPLIT_TILE = 128;
+constexpr int SPLIT_SUPER = 4;  // panels per trailing update on the split path (k = 256)
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
 static inline int split_pad(int n) { return (n + SPLIT_TILE - 1) / SPLIT_TILE * SPLIT_TILE; }
-int64_t split_work(int n) { return (int64_t)8 * split_pad(n) * NB; }
+int64_t split_work(int n) { return (int64_t)8 * split_pad(n) * SPLIT_SUPER * NB; }
 
 __device__ __forceinline__ unsigned short bf16_rne(float f) {
     unsigned int u = __float_as_uint(f);
@@ -558,11 +569,13 @@ __device__ __forceinline__ unsigned short bf16_rne(float f) {
 __device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float((unsigned int)h << 16); }
 
 // thread (r, kb): k slots 8 kb .. 8 kb + 7 of row r of L21 (rows [KE, n), columns [K, KE)) and of
-// column r of U12 (rows [K, KE), columns [KE, n)); rows past n - KE and k past KE - K are zero
-__global__ void split_panels_kernel(MatBatch<cplxf> mb, int n, int K, int KE, int mpad, unsigned short* __restrict__ Sb,
-                                    int64_t sS) {
+// column r of U12 (rows [K, KE), columns [KE, n)); rows past n - KE and k past KE - K are zero;
+// ks = 64 S k slots per row
+__global__ void split_panels_kernel(MatBatch<cplxf> mb, int n, int K, int KE, int mpad, int ks,
+                                    unsigned short* __restrict__ Sb, int64_t sS) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = t >> 3, kb = t & 7;
+    const int kbs = ks / 8;
+    const int r = t / kbs, kb = t - r * kbs;
     if (r >= mpad) return;
     const cplxf* A = mb.A + mb.stride * blockIdx.z;
     unsigned short* S = Sb + sS * blockIdx.z;
@@ -595,7 +608,7 @@ __global__ void split_panels_kernel(MatBatch<cplxf> mb, int n, int K, int KE, in
             w.y = h[p][q][2] | ((unsigned)h[p][q][3] << 16);
             w.z = h[p][q][4] | ((unsigned)h[p][q][5] << 16);
             w.w = h[p][q][6] | ((unsigned)h[p][q][7] << 16);
-            *reinterpret_cast<uint4*>(S + ((int64_t)(4 * p + q) * mpad + r) * NB + 8 * kb) = w;
+            *reinterpret_cast<uint4*>(S + ((int64_t)(4 * p + q) * mpad + r) * ks + 8 * kb) = w;
         }
 }
 
@@ -614,7 +627,7 @@ __device__ __forceinline__ bf16x8_t bf16x8_neg(bf16x8_t v) {
 constexpr int SPLIT_KC = 32;
 constexpr int SPLIT_LDS_ROW = SPLIT_KC / 8 + 1;  // bf16x8 units per row (4 used + 1 pad)
 
-__global__ void __launch_bounds__(256) zgemm_split_kernel(int m, int mpad, const unsigned short* __restrict__ Sb,
+__global__ void __launch_bounds__(256) zgemm_split_kernel(int m, int mpad, int ks, const unsigned short* __restrict__ Sb,
                                                           int64_t sS, cplxf* __restrict__ Cb, int ldc, int64_t sC) {
     __shared__ bf16x8_t La[4][SPLIT_TILE][SPLIT_LDS_ROW];  // L21 rows   (MFMA B operand, C rows)
     __shared__ bf16x8_t Lb[4][SPLIT_TILE][SPLIT_LDS_ROW];  // U12 columns (MFMA A operand, C columns)
@@ -631,8 +644,8 @@ __global__ void __launch_bounds__(256) zgemm_split_kernel(int m, int mpad, const
 #pragma unroll
             for (int v = 0; v < 16; v++) acc_re[a][b][v] = acc_im[a][b][v] = 0.f;
     const bf16x8_t* Sa = reinterpret_cast<const bf16x8_t*>(S);                              // arrays 0..3
-    const bf16x8_t* Sbv = reinterpret_cast<const bf16x8_t*>(S + (int64_t)4 * mpad * NB);  // arrays 4..7
-    constexpr int UPR = NB / 8;  // bf16x8 units per row in global
+    const bf16x8_t* Sbv = reinterpret_cast<const bf16x8_t*>(S + (int64_t)4 * mpad * ks);  // arrays 4..7
+    const int UPR = ks / 8;  // bf16x8 units per row in global
     // stage: 4 arrays x 128 rows x 4 units for each panel = 2048 units each, 8 + 8 per thread; the
     // next chunk is loaded into registers before the current chunk's MFMAs
     bf16x8_t pa[8], pb[8];
@@ -661,7 +674,7 @@ __global__ void __launch_bounds__(256) zgemm_split_kernel(int m, int mpad, const
                 old[a][b][v] = (r < m && c < m) ? Cm[r + (int64_t)ldc * c] : make_float2(0.f, 0.f);
             }
         }
-    for (int k0 = 0; k0 < NB; k0 += SPLIT_KC) {
+    for (int k0 = 0; k0 < ks; k0 += SPLIT_KC) {
 #pragma unroll
         for (int e = 0; e < 8; e++) {
             const int idx = tid + 256 * e;
@@ -670,7 +683,7 @@ __global__ void __launch_bounds__(256) zgemm_split_kernel(int m, int mpad, const
             Lb[q][r][u] = pb[e];
         }
         __syncthreads();
-        if (k0 + SPLIT_KC < NB) load_chunk(k0 + SPLIT_KC);
+        if (k0 + SPLIT_KC < ks) load_chunk(k0 + SPLIT_KC);
 #pragma unroll
         for (int ks = 0; ks < SPLIT_KC / 16; ks++) {
             const int u = 2 * ks + h;
@@ -739,8 +752,10 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
     if (rows_per > 4) return -1;  // orders above 4096 need a larger register tile
     MatBatch<C> mb{A, strideA, lda};
     const int64_t pstride = 1 + 4 * NB;
-    for (int K = 0; K < n; K += NB) {
-        const int KE = min(n, K + NB);
+    constexpr bool kF32 = std::is_same<C, cplxf>::value;
+    const bool use_split = kF32 && split != nullptr;
+    // leaves of 4 columns over [K, KE), each followed by the rank-4 update of the panel's rest
+    auto factor_panel = [&](int K, int KE) {
         for (int c0 = K; c0 < KE; c0 += LW) {
             const int m_rows = n - c0;
             const int rp = (m_rows + LEAF_THREADS - 1) / LEAF_THREADS;
@@ -756,43 +771,87 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
                 panel_update_kernel<C><<<g2, 256, 0, st>>>(mb, n, c0, KE);
             }
         }
+    };
+    // the panel's row interchanges applied to every column outside it
+    auto permute_others = [&](int K, int KE) {
         if (n - (KE - K) > 0) {
             build_perm_kernel<<<dim3(1, 1, batch), 64, 0, st>>>(ipiv, ipiv_stride, K, KE, work_perm, pstride);
             const int nout = n - (KE - K);
             apply_perm_kernel<C><<<dim3((nout + 1) / 2, 1, batch), 256, 0, st>>>(mb, n, K, KE, work_perm, pstride);
         }
-        if (KE < n) {
-            if constexpr (std::is_same<C, cplxf>::value)  // KE < n: a full panel
-                trsm_cols_kernel<<<dim3((n - KE + 255) / 256, 1, batch), 256, 0, st>>>(mb, n, K);
-            else
-                trsm_panel_kernel<C><<<dim3((n - KE + TRSM_COLS - 1) / TRSM_COLS, 1, batch), 256, 0, st>>>(mb, n, K, KE);
-            constexpr bool kF32 = std::is_same<C, cplxf>::value;
-            const bool use_split = kF32 && split != nullptr;
-            const int m = n - KE, mpad = split_pad(m);
-            const int64_t sS = split_work(n);
-            if constexpr (kF32)
-                if (use_split)
-                    split_panels_kernel<<<dim3((mpad * 8 + 255) / 256, 1, batch), 256, 0, st>>>(mb, n, K, KE, mpad,
-                                                                                              split, sS);
-            // the timed launch: the GEMM alone
-            hipEvent_t e0 = nullptr, e1 = nullptr;
-            if (timer) {
-                hipEventCreate(&e0);
-                hipEventCreate(&e1);
-                hipEventRecord(e0, st);
-            }
-            if constexpr (kF32)
-                if (use_split)
-                    zgemm_split_kernel<<<dim3(mpad / SPLIT_TILE, mpad / SPLIT_TILE, batch), 256, 0, st>>>(
-                        m, mpad, split, sS, A + KE + (int64_t)lda * KE, lda, strideA);
-            if (!use_split)
-                gemm_t<C>(st, true, true, m, m, KE - K, A + KE + (int64_t)lda * K, lda, strideA,
-                          A + K + (int64_t)lda * KE, lda, strideA, A + KE + (int64_t)lda * KE, lda, strideA, batch);
-            if (timer) {
-                hipEventRecord(e1, st);
-                timer->ev.emplace_back(e0, e1);
-            }
+    };
+    // U12 = L11^{-1} A12 for rows [K, KE = K + NB) and columns [KE, n)
+    auto trsm = [&](int K, int KE) {
+        if constexpr (kF32)
+            trsm_cols_kernel<<<dim3((n - KE + 255) / 256, 1, batch), 256, 0, st>>>(mb, n, K);
+        else
+            trsm_panel_kernel<C><<<dim3((n - KE + TRSM_COLS - 1) / TRSM_COLS, 1, batch), 256, 0, st>>>(mb, n, K, KE);
+    };
+    // A[KE:, KE:] -= A[KE:, K:KE] A[K:KE, KE:] (the timed launch: the GEMM alone)
+    auto update = [&](int K, int KE) {
+        const int m = n - KE, mpad = split_pad(m), ks = (KE - K + NB - 1) / NB * NB;
+        const int64_t sS = split_work(n);
+        if constexpr (kF32)
+            if (use_split)
+                split_panels_kernel<<<dim3((mpad * (ks / 8) + 255) / 256, 1, batch), 256, 0, st>>>(mb, n, K, KE, mpad,
+                                                                                                  ks, split, sS);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timer) {
+            hipEventCreate(&e0);
+            hipEventCreate(&e1);
+            hipEventRecord(e0, st);
         }
+        if constexpr (kF32)
+            if (use_split)
+                zgemm_split_kernel<<<dim3(mpad / SPLIT_TILE, mpad / SPLIT_TILE, batch), 256, 0, st>>>(
+                    m, mpad, ks, split, sS, A + KE + (int64_t)lda * KE, lda, strideA);
+        if (!use_split)
+            gemm_t<C>(st, true, true, m, m, KE - K, A + KE + (int64_t)lda * K, lda, strideA,
+                      A + K + (int64_t)lda * KE, lda, strideA, A + KE + (int64_t)lda * KE, lda, strideA, batch);
+        if (timer) {
+            hipEventRecord(e1, st);
+            timer->ev.emplace_back(e0, e1);
+        }
+    };
+    for (int K = 0; K < n;) {
+        // panels of this step: S full panels with a trailing matrix left after them (split path)
+        int S = 1;
+        if (use_split)
+            while (S < SPLIT_SUPER && K + (S + 1) * NB < n) S++;
+        if (S == 1) {  // one panel
+            const int KE = min(n, K + NB);
+            factor_panel(K, KE);
+            permute_others(K, KE);
+            if (KE < n) {
+                trsm(K, KE);
+                update(K, KE);
+            }
+            K = KE;
+            continue;
+        }
+        // Multi-level step (split updates): panels P_j = [K + 64 j, K + 64 (j + 1)), j < S.  The
+        // rest of the matrix receives their S rank-64 updates in one k = 64 S pass, read and
+        // written once.  Panel j, left-looking over the panels before it:
+        //   its columns (rows >= Kj) -= L21(P_<j) U12(P_<j)[:, P_j];  factor it; its interchanges
+        //   (also in the earlier panels' columns);  its U rows right of it -= L21(P_<j)[P_j rows]
+        //   U12(P_<j)[:, right of P_j];  U12(P_j) = L_jj^{-1} (those rows).
+        // Then A[KS:, KS:] -= A[KS:, K:KS] A[K:KS, KS:].  The same operations as S one-panel
+        // steps; only the trailing matrix's FP32 rounding differs (one accumulation instead of S).
+        const int KS = K + S * NB;
+        for (int j = 0; j < S; j++) {
+            const int Kj = K + j * NB, Kq = Kj + NB;
+            if (j > 0)
+                gemm_t<C>(st, true, true, n - Kj, NB, Kj - K, A + Kj + (int64_t)lda * K, lda, strideA,
+                          A + K + (int64_t)lda * Kj, lda, strideA, A + Kj + (int64_t)lda * Kj, lda, strideA, batch);
+            factor_panel(Kj, Kq);
+            permute_others(Kj, Kq);
+            if (j > 0)
+                gemm_t<C>(st, true, true, NB, n - Kq, Kj - K, A + Kj + (int64_t)lda * K, lda, strideA,
+                          A + K + (int64_t)lda * Kq, lda, strideA, A + Kj + (int64_t)lda * Kq, lda, strideA, batch);
+            trsm(Kj, Kq);
+        }
+        update(K, KS);
+        K = KS;
     }
     return 0;
 }
