@@ -79,11 +79,11 @@ __global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch<C> mb
     int* piv = ipiv + ipiv_stride * b;
     const int lda = mb.lda;
     const int tid = threadIdx.x;
-    __shared__ R red_v[LEAF_THREADS / 64];
-    __shared__ int red_r[LEAF_THREADS / 64];
-    __shared__ C xrow[2][LW];
-    __shared__ C prow[LW];
-    __shared__ int s_piv;
+    // per column parity (the LDS of column c is reused at c + 2, after the two barriers of c + 1)
+    __shared__ R red_v[2][LEAF_THREADS / 64];
+    __shared__ int red_r[2][LEAF_THREADS / 64];
+    __shared__ C xrow[2][LW];  // row col before the interchange
+    __shared__ C prow[2][LW];  // the pivot row
     __shared__ int s_pivs[LW];
     __shared__ C L11[LW][LW];
 
@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch<C> mb
 #pragma unroll
     for (int c = 0; c < LW; c++) {
         if (c >= wcols) break;
-        const int col = c0 + c;
+        const int col = c0 + c, par = c & 1;
         // --- pivot search: max |Re|+|Im|, smallest row on ties
         R bv = -1;
         int br = 0x7fffffff;
@@ -113,46 +113,39 @@ __global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch<C> mb
             int orow = __shfl_xor(br, off);
             if (ov > bv || (ov == bv && orow < br)) { bv = ov; br = orow; }
         }
-        if ((tid & 63) == 0) { red_v[tid >> 6] = bv; red_r[tid >> 6] = br; }
+        if ((tid & 63) == 0) { red_v[par][tid >> 6] = bv; red_r[par][tid >> 6] = br; }
         __syncthreads();
+        // every thread combines the per-wave candidates in wave order (the same rule as above)
+        R v = red_v[par][0];
+        int p = red_r[par][0];
+#pragma unroll
+        for (int w = 1; w < LEAF_THREADS / 64; w++)
+            if (red_v[par][w] > v || (red_v[par][w] == v && red_r[par][w] < p)) { v = red_v[par][w]; p = red_r[par][w]; }
         if (tid == 0) {
-            R v = red_v[0];
-            int r = red_r[0];
-            for (int w = 1; w < LEAF_THREADS / 64; w++)
-                if (red_v[w] > v || (red_v[w] == v && red_r[w] < r)) { v = red_v[w]; r = red_r[w]; }
-            s_piv = r;
-            piv[col] = r;
-            s_pivs[c] = r;
+            piv[col] = p;
+            s_pivs[c] = p;
         }
-        __syncthreads();
-        const int p = s_piv;
-        // --- swap rows col <-> p inside the leaf
+        // --- interchange rows col <-> p inside the leaf, and the pivot row for the update
         const int tc = (col - c0) % LEAF_THREADS, qc = (col - c0) / LEAF_THREADS;
         const int tp = (p - c0) % LEAF_THREADS, qp = (p - c0) / LEAF_THREADS;
+#pragma unroll
+        for (int q = 0; q < ROWS; q++) {
+            if (tid == tc && q == qc)
+                for (int cc = 0; cc < LW; cc++) xrow[par][cc] = x[q][cc];
+            if (tid == tp && q == qp)
+                for (int cc = 0; cc < LW; cc++) prow[par][cc] = x[q][cc];
+        }
+        __syncthreads();
         if (p != col) {
 #pragma unroll
             for (int q = 0; q < ROWS; q++) {
                 if (tid == tc && q == qc)
-                    for (int cc = 0; cc < LW; cc++) xrow[0][cc] = x[q][cc];
+                    for (int cc = 0; cc < LW; cc++) x[q][cc] = prow[par][cc];
                 if (tid == tp && q == qp)
-                    for (int cc = 0; cc < LW; cc++) xrow[1][cc] = x[q][cc];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int q = 0; q < ROWS; q++) {
-                if (tid == tc && q == qc)
-                    for (int cc = 0; cc < LW; cc++) x[q][cc] = xrow[1][cc];
-                if (tid == tp && q == qp)
-                    for (int cc = 0; cc < LW; cc++) x[q][cc] = xrow[0][cc];
+                    for (int cc = 0; cc < LW; cc++) x[q][cc] = xrow[par][cc];
             }
         }
-        // --- broadcast the pivot row
-#pragma unroll
-        for (int q = 0; q < ROWS; q++)
-            if (tid == tc && q == qc)
-                for (int cc = 0; cc < LW; cc++) prow[cc] = x[q][cc];
-        __syncthreads();
-        const C u = prow[c];
+        const C u = prow[par][c];
         if (u.x != 0 || u.y != 0) {
             const C inv = trecip(u);
 #pragma unroll
@@ -162,13 +155,12 @@ __global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch<C> mb
                     x[q][c] = tmul(x[q][c], inv);
 #pragma unroll
                     for (int cc = 0; cc < LW; cc++)
-                        if (cc > c) x[q][cc] = tsub(x[q][cc], tmul(x[q][c], prow[cc]));
+                        if (cc > c) x[q][cc] = tsub(x[q][cc], tmul(x[q][c], prow[par][cc]));
                 }
             }
         } else if (tid == 0) {
             atomicCAS(info + b, 0, col + 1);
         }
-        __syncthreads();
     }
     // store back
 #pragma unroll
@@ -185,27 +177,58 @@ __global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch<C> mb
     __syncthreads();
     // other outer-panel columns: apply the leaf's interchanges, then solve U rows
     const int nother = (KE - K) - wcols;
+    // The at most 2 LW rows a column touches (c0 + c and the pivot rows) are loaded together, the
+    // interchanges and the substitution done on the register copies (an entry stands for its row;
+    // a row listed twice keeps both copies equal), and all of them stored back.
+    int idx[2 * LW];
+#pragma unroll
+    for (int c = 0; c < LW; c++) {
+        idx[c] = c0 + min(c, wcols - 1);
+        idx[LW + c] = c < wcols ? s_pivs[c] : c0;
+    }
     for (int t = tid; t < nother; t += LEAF_THREADS) {
         int j = K + t;
         if (j >= c0) j += wcols;
         C* colj = A + (int64_t)lda * j;
-        for (int c = 0; c < wcols; c++) {
-            int r0 = c0 + c, r1 = s_pivs[c];
+        C v[2 * LW];
+#pragma unroll
+        for (int k = 0; k < 2 * LW; k++) v[k] = colj[idx[k]];
+        auto get = [&](int r) {
+            C w = v[0];
+#pragma unroll
+            for (int k = 2 * LW - 1; k >= 0; k--)
+                if (idx[k] == r) w = v[k];
+            return w;
+        };
+        auto set = [&](int r, C w) {
+#pragma unroll
+            for (int k = 0; k < 2 * LW; k++)
+                if (idx[k] == r) v[k] = w;
+        };
+#pragma unroll
+        for (int c = 0; c < LW; c++) {
+            if (c >= wcols) break;
+            const int r0 = c0 + c, r1 = s_pivs[c];
             if (r1 != r0) {
-                C t0 = colj[r0];
-                colj[r0] = colj[r1];
-                colj[r1] = t0;
+                const C t0 = get(r0), t1 = get(r1);
+                set(r0, t1);
+                set(r1, t0);
             }
         }
         if (j > c0) {  // U12 rows c0..c0+wcols-1: unit-lower forward substitution
             C u[LW];
-            for (int c = 0; c < wcols; c++) {
-                C s = colj[c0 + c];
+#pragma unroll
+            for (int c = 0; c < LW; c++) {
+                if (c >= wcols) break;
+                C s = get(c0 + c);
+#pragma unroll
                 for (int s2 = 0; s2 < c; s2++) s = tsub(s, tmul(L11[c][s2], u[s2]));
                 u[c] = s;
-                colj[c0 + c] = s;
+                set(c0 + c, s);
             }
         }
+#pragma unroll
+        for (int k = 0; k < 2 * LW; k++) colj[idx[k]] = v[k];
     }
 }
 
