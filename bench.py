@@ -321,7 +321,7 @@ def main():
         run_plain = lambda ws: mhx.api._body_coeffs(body, names, ws, [0.0], exact_form.gf, True,
                                                     _lib.MODE_PER_OMEGA, 1025.0, 9.81)
         ctx.set_lu_mode(_lib.LU_FP64)  # the plain path: FP64 LU, as the reference
-        pv, pel, pph, proof, plu, pres, _ = measure(ctx, run_plain, steps_w[sub], n, "ExactGuevelDelhommeau")
+        pv, pel, pph, proof, plu, pres, proof2 = measure(ctx, run_plain, steps_w[sub], n, "ExactGuevelDelhommeau")
         ctx.set_lu_mode({"auto": _lib.LU_AUTO, "fp64": _lib.LU_FP64, "mixed": _lib.LU_MIXED,
                          "mixed_bf16": _lib.LU_MIXED_BF16}[args.lu_mode])
         errs = {}
@@ -332,7 +332,9 @@ def main():
             errs[name] = max(float(np.linalg.norm(at(a, q) - at(b, q)) / np.linalg.norm(at(b, q))) for q in range(kp))
         log("max relative error vs exact FP64 per omega:", errs)
         plain = {"value": pv, "steps": kp, "omegas": steps_w[sub].tolist(), "phases_ms_per_omega": pph,
-                 "lu_tflops": plu, "max_rel_err_vs_exact": errs}
+                 "lu_tflops": plu, "max_rel_err_vs_exact": errs,
+                 # FP64 rates of the plain path's two main kernels against the MI355X peak
+                 "roofline": proof, "roofline_next_kernel": proof2}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -654,10 +654,21 @@ __global__ void __launch_bounds__(256) zgemm_split_kernel(int m, int mpad, int k
                                                           int64_t sS, cplxf* __restrict__ Cb, int ldc, int64_t sC) {
     __shared__ bf16x8_t La[4][SPLIT_TILE][SPLIT_LDS_ROW];  // L21 rows   (MFMA B operand, C rows)
     __shared__ bf16x8_t Lb[4][SPLIT_TILE][SPLIT_LDS_ROW];  // U12 columns (MFMA A operand, C columns)
-    const unsigned short* S = Sb + sS * blockIdx.z;
-    cplxf* Cm = Cb + sC * blockIdx.z;
+    // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin by linear id, so the
+    // id is remapped to give each XCD a contiguous run of tiles (row tile fastest: the tiles an XCD
+    // holds at once share their column panel and walk down the same column-major C columns).
+    // Grouping 8 tile columns instead cuts the panel re-reads (7.8 against 10.4 GB per launch of
+    // 64 matrices of order 4096) but is slower (3.26 against 3.03 ms).
+    int L = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+    {
+        const int T = gridDim.x * gridDim.y * gridDim.z, q = T / 8;
+        if (L < 8 * q) L = (L % 8) * q + L / 8;
+    }
+    const int tx = L % gridDim.x, ty = (L / gridDim.x) % gridDim.y, tz = L / (gridDim.x * gridDim.y);
+    const unsigned short* S = Sb + sS * tz;
+    cplxf* Cm = Cb + sC * tz;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, l32 = lane & 31;
-    const int row0 = blockIdx.x * SPLIT_TILE, col0 = blockIdx.y * SPLIT_TILE;
+    const int row0 = tx * SPLIT_TILE, col0 = ty * SPLIT_TILE;
     const int wr = (wave & 1) * 64, wc = (wave >> 1) * 64;
     f32x16_t acc_re[2][2], acc_im[2][2];  // [column tile][row tile]
 #pragma unroll
