@@ -122,8 +122,9 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* body, int64_t nw, const double* omeg
  * (bodies.jl:55-66: ndof x 3, Pitch (0,0,1), Yaw (0,-1,0), else 0; NULL when every U is 0);
  * the pressure i rho omega_e phi + rho U dphi/dx with dphi/dx = K_x sigma (the indirect operator
  * with every panel normal replaced by e_x).  U = 0 is the plain problem (any formulation); a
- * nonzero U needs direct = 0.  gf: MHX_GF_WU or MHX_GF_DELHOMMEAU (MHX_GF_SURROGATE only when
- * every U is 0).  Outputs, column-major like the reference's arrays:
+ * nonzero U needs direct = 0.  gf: any; with MHX_GF_SURROGATE, K_x takes dG/dr from the same
+ * surrogate table as S and D.  The dense solves follow mhx_set_lu_mode.  Outputs, column-major
+ * like the reference's arrays:
  *   A_out, B_out   ndof x ndof x nw x nU x nb     Re F / omega_e^2, Im F / omega_e
  *   Fd_out, Fk_out ndof x nw x nb x nU complex    (Froude-Krylov with the incident omega, k and beta_e) */
 int mhx_body_coeffs_fs(mhx_ctx* ctx, mhx_body* body, int64_t nw, const double* omegas, int64_t nb, const double* betas,

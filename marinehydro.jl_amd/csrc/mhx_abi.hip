@@ -218,8 +218,20 @@ bool use_split(const mhx_ctx* ctx, int gf) {
     return ctx->lu_mode == MHX_LU_MIXED_BF16 || (ctx->lu_mode == MHX_LU_AUTO && gf == MHX_GF_SURROGATE);
 }
 
-// wave assembly of `batch` items (wavenumbers already in b->d_ks) into b->S, b->D
-int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int include_identity) {
+// device bytes one batch item of a solve holds (the chunk size of the batched schedules)
+double item_bytes(const mhx_ctx* ctx, int n, int gf, bool table, int rhs_cols) {
+    double per = (double)n * n * sizeof(cplx) * (table ? 4 : 2) + (double)rhs_cols * n * 16 * sizeof(cplx);
+    if (gf == MHX_GF_SURROGATE) per += 8.0 * (double)mhx::surr_fallback_capacity(n, 1);  // fallback list
+    if (use_mixed(ctx, gf)) per += (double)n * n * sizeof(cplxf) + 2.0 * n * 16 * sizeof(cplx);  // FP32 factors
+    if (use_split(ctx, gf)) per += 2.0 * mhx::split_work(n);  // split-bf16 panels
+    return per;
+}
+
+// wave assembly of `batch` items (wavenumbers already in b->d_ks) into b->S, b->D.  keep_table:
+// the surrogate writes its (G, dG/dr) table into b->WG, b->WR (allocated by the caller) and S, D
+// are combined from it, as for the exact term, so that later kernels can read dG/dr.
+int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int include_identity,
+                  bool keep_table = false) {
     b->df_ready = false;
     if (gf == MHX_GF_WU) {
         Phase ph(ctx, "wave");
@@ -241,12 +253,17 @@ int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int 
         mhx::SurrOut out;
         out.SR = b->SR;
         out.DR = b->DR[direct];
-        out.S = b->S;
-        out.D = b->D;
         out.direct = direct;
         out.include_identity = include_identity;
         out.sMat = b->nn();
-        if (use_mixed(ctx, gf)) {
+        if (keep_table) {
+            out.WG = b->WG;
+            out.WR = b->WR;
+        } else {
+            out.S = b->S;
+            out.D = b->D;
+        }
+        if (!keep_table && use_mixed(ctx, gf)) {
             int rc = dalloc(ctx, &b->Df, (size_t)b->nn() * b->cap_batch);
             if (rc) return rc;
             out.Df = b->Df;
@@ -258,6 +275,11 @@ int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int 
                                    ctx->surr_B, out, b->fb_list, b->fb_count, b->fb_cap, b->status);
         }
         b->df_ready = out.Df != nullptr;
+        if (keep_table) {
+            Phase ph(ctx, "wave");
+            mhx::launch_combine_table(ctx->stream, b->n, b->geom, b->SR, b->DR[direct], b->d_ks, batch, direct,
+                                      include_identity, b->WG, b->WR, b->S, b->D, b->nn());
+        }
     } else {
         HCHK(hipMemsetAsync(b->status, 0, 4 * sizeof(unsigned long long), ctx->stream));
         {
@@ -835,10 +857,7 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
     const bool table = gf == MHX_GF_DELHOMMEAU;  // (G, dG/dr) table of the exact term
     int chunk = 1;
     if (mode == MHX_MODE_BATCHED && nw > 1) {
-        double per = (double)n * n * sizeof(cplx) * (table ? 4 : 2) + 20.0 * n * 16 * sizeof(cplx);
-        if (gf == MHX_GF_SURROGATE) per += 8.0 * (double)mhx::surr_fallback_capacity((int)n, 1);  // fallback list
-        if (use_mixed(ctx, gf)) per += (double)n * n * sizeof(cplxf) + 2.0 * n * 16 * sizeof(cplx);  // FP32 factors
-        if (use_split(ctx, gf)) per += 2.0 * mhx::split_work((int)n);  // split-bf16 panels
+        const double per = item_bytes(ctx, n, gf, table, 20);
         chunk = (int)std::min<double>((double)std::min<int64_t>(nw, ctx->max_batch), std::max(1.0, ctx->batch_bytes / per));
     }
     int rc;
@@ -893,8 +912,9 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
 
 // Forward speed.  U = 0: the plain problems, through mhx_body_coeffs (one batched solve per
 // frequency chunk shared by every heading).  U != 0: per (U, beta), the frequencies of a chunk
-// batched, each item at its own encountered (omega_e, k_e): indirect assembly at k_e, FP64 LU,
-// then K_x at k_e written over the dead LU factors, dphi/dx = K_x sigma, and
+// batched, each item at its own encountered (omega_e, k_e): indirect assembly at k_e (the
+// surrogate keeps its (G, dG/dr) table, as the exact term does), LU in the precision
+// mhx_set_lu_mode picks for the Green function, then K_x at k_e written over the dead LU factors, dphi/dx = K_x sigma, and
 // phi <- phi - i (U / omega_e) dphi/dx so that the force kernel's i rho omega_e phi carries the
 // rho U dphi/dx term (solve.jl:35-55).
 int mhx_body_coeffs_fs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas, int64_t nb, const double* betas,
@@ -921,8 +941,6 @@ int mhx_body_coeffs_fs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omeg
         if (direct)
             return fail(ctx, MHX_EINVAL,
                         "nonzero forward speed is not supported with the direct method; use IndirectBEM (direct = false)");
-        if (gf == MHX_GF_SURROGATE)
-            return fail(ctx, MHX_EINVAL, "nonzero forward speed: GFWu or ExactGuevelDelhommeau only");
         if (!dgrad) return fail(ctx, MHX_EINVAL, "nonzero forward speed: null gradient of motion");
     }
     if (ndof + 1 > 16) return fail(ctx, MHX_EDIM, "forward speed: at most 15 dofs");
@@ -969,11 +987,11 @@ int mhx_body_coeffs_fs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omeg
     if (rc || !moving) return rc;
     // ---- U != 0 ----
     const int R = ndof + 1;
-    const bool table = gf == MHX_GF_DELHOMMEAU;
+    const bool table = gf != MHX_GF_WU;  // K_x reads dG/dr of the exact term (or the surrogate's)
     const double g = ctx->g;
     int chunk = 1;
     {
-        const double per = (double)n * n * sizeof(cplx) * (table ? 4 : 2) + 21.0 * n * 16 * sizeof(cplx);
+        const double per = item_bytes(ctx, n, gf, table, 21);
         chunk = (int)std::min<double>((double)std::min<int64_t>(nw, ctx->max_batch), std::max(1.0, ctx->batch_bytes / per));
     }
     if ((rc = ensure_work(ctx, b, R, 1, table, chunk)) || (rc = ensure_rankine(ctx, b, 0))) return rc;
@@ -1020,10 +1038,9 @@ int mhx_body_coeffs_fs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omeg
                     for (int q = 0; q < bt; q++)
                         mhx::launch_bc_fs(st, n, b->geom, b->dofs, ndof, d_grad, we[q], U, inc[chunk + q], inc[q], beta,
                                           inc[2 * chunk + q], g, b->B + q * b->nR());
-                    if ((e = assemble_into(ctx, b, bt, gf, 0, 1)) || (e = check_status(ctx, b, gf))) return e;
-                    e = solve_fp64(ctx, b, R, 0, bt);  // sigma stays in b->B, phi = S sigma in b->Y
-                    if (e == MHX_ESINGULAR) return fail(ctx, MHX_ESINGULAR, ctx->err);
-                    if (e) return e;
+                    if ((e = assemble_into(ctx, b, bt, gf, 0, 1, true)) || (e = check_status(ctx, b, gf))) return e;
+                    // sigma stays in b->B, phi = S sigma in b->Y
+                    if ((e = factor_and_solve(ctx, b, R, 0, bt, gf))) return e;
                     {
                         Phase ph(ctx, "forward_speed");
                         for (int q = 0; q < bt; q++)

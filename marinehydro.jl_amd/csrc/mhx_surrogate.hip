@@ -367,6 +367,13 @@ __device__ __forceinline__ int surr_slot_input(int l, int kb, int h, int e) {
 }
 
 // FP32 parameters -> packed BF16 units (layout above) and FP32 biases [l][128] + output[8]
+// Inference works on y = -log2(e) z instead of the pre-activation z: the hidden layers'
+// biases, and the first layer's weights, carry the factor -log2(e), the activation is
+// h' = y / (1 + 2^y) = -log2(e) SiLU(z) (one v_exp_f32 with no scaling multiply), and the
+// next layer's weights would undo the factor (-1 / log2(e)) while also applying their own
+// (-log2(e)): the middle layers' weights are unchanged and only the output layer's carry
+// -1 / log2(e).
+constexpr float kLog2e = 1.4426950408889634f;
 __global__ void surr_pack_kernel(const float* __restrict__ params, unsigned short* __restrict__ Wb,
                                  float* __restrict__ bias) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -381,14 +388,14 @@ __global__ void surr_pack_kernel(const float* __restrict__ params, unsigned shor
         const int in = surr_slot_input(l, kb, h, e);
         float v = 0.f;
         if (row < OUT_DIM_D(l)) v = params[OFF_W_D(l) + (int64_t)row * IN_DIM_D(l) + in];
-        Wb[t] = f2bf(v);
+        Wb[t] = f2bf(v * (l == 0 ? -kLog2e : l == 4 ? -1.0f / kLog2e : 1.0f));
     }
     if (t < 4 * 128 + 8) {
         const int l = t < 512 ? t / 128 : 4;
         const int r = t - 128 * l;
         float v = 0.f;
         if (r < OUT_DIM_D(l)) v = params[OFF_W_D(l) + (int64_t)OUT_DIM_D(l) * IN_DIM_D(l) + r];
-        bias[t] = v;
+        bias[t] = l < 4 ? -kLog2e * v : v;
     }
 }
 
@@ -406,7 +413,8 @@ __device__ __forceinline__ void surr_load_lds(SurrLds& L, const unsigned short* 
     __syncthreads();
 }
 
-__device__ __forceinline__ float silu_fast(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+// -log2(e) SiLU(z) from y = -log2(e) z (2^y overflows to inf for large y: h' -> 0)
+__device__ __forceinline__ float silu_log2(float y) { return y * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(y)); }
 
 // one hidden layer on the two column tiles: act (B operands, KB input blocks) -> act (8 blocks).
 // Output neuron tiles are done one at a time so that only 32 accumulators are live.
@@ -441,8 +449,8 @@ __device__ __forceinline__ void surr_hidden(const SurrLds& L, int layer, int lan
 #pragma unroll
             for (int e = 0; e < 8; e++) {
                 const int q = 4 * (2 * b + (e >> 2)) + (e & 3);
-                v0[e] = (__bf16)silu_fast(acc0[q]);
-                v1[e] = (__bf16)silu_fast(acc1[q]);
+                v0[e] = (__bf16)silu_log2(acc0[q]);
+                v1[e] = (__bf16)silu_log2(acc1[q]);
             }
             nxt[0][2 * t + b] = v0;
             nxt[1][2 * t + b] = v1;

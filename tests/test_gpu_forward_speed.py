@@ -131,3 +131,31 @@ def test_single_problems_match_the_grid(mhx):
     ref = np.array([grid["diffraction_force"][:, 0, 0, 0], grid["Froude_Krylov_force"][:, 0, 0, 0]])
     rel, elem, ok = close(got, ref, tol=1e-11)
     assert ok, (rel, elem)
+
+
+def test_forward_speed_surrogate_within_1e3_of_exact(mhx):
+    """The learned wave term with forward speed: sphere R = 1 (64 panels) and cylinder R = 1, draft 2
+    (768 panels), six dofs, speeds 0, 1.5 and 8 m/s, three frequencies, headings 0 and pi/3.  S, D
+    and K_x come from the surrogate's (G, dG/dr) table, the solves are the default mixed precision
+    of the surrogate (split-bf16 LU refined to 1e-12): every coefficient block within 1e-3
+    (normwise) of the exact FP64 path; the solves ran mixed, with no FP64 fallback."""
+    from mhx import meshes
+    mhx.load_surrogate()
+    ctx = mhx.context()
+    for name, mesh in (("sphere64", meshes.sphere(1.0, 8, 16)), ("cylinder768", meshes.cylinder(1.0, 2.0, 32, 16, 8))):
+        print(f"forward speed surrogate {name}", flush=True)
+        body = mhx.FloatingBody(mesh, DOFS, rotation_center=(0.0, 0.0, -0.5))
+        par = {"wave_frequencies": [0.6, 1.2, 2.0], "radiating_dofs": DOFS, "wave_directions": [0.0, math.pi / 3],
+               "forward_speeds": [0.0, 1.5, 8.0]}
+        ex = mhx.hydrodynamic_coefficients(body, par, mhx.IndirectBEM("ExactGuevelDelhommeau"))
+        ctx.reset_timing()
+        su = mhx.hydrodynamic_coefficients(body, par, mhx.IndirectBEM("Surrogate"))
+        mixed, _, fallbacks, _ = ctx.solver_stats()
+        assert mixed > 0 and fallbacks == 0, (mixed, fallbacks)
+        for key in KEYS:
+            for iu in range(3):
+                r = ex[key][..., iu, :] if key in ("added_mass", "radiation_damping") else ex[key][..., iu]
+                s = su[key][..., iu, :] if key in ("added_mass", "radiation_damping") else su[key][..., iu]
+                err = np.linalg.norm(s - r) / np.linalg.norm(r)
+                print(f"  {key} U-index {iu}: {err:.2e}", flush=True)
+                assert err <= 1e-3, (name, key, iu, err)
