@@ -202,7 +202,9 @@ def measure(ctx, run, ws, n, gf, nlu=None, units=None, kind="fp64"):
         # the split-bf16 update moves the trailing matrix once per step: HBM-bound
         bu = lu_update_bytes(n) * nw
         traffic = None
-        if n == 4096 and nw == 64 and cu == len(lu_update_steps(n, two_level=True)):  # the profiled configuration
+        # the library factors a batch of >= 8 items of order >= 512 as two halves on two streams
+        halves = 2 if nw >= 8 and n >= 512 else 1
+        if n == 4096 and nw == 64 and cu == halves * len(lu_update_steps(n, two_level=True)):  # the profiled configuration
             traffic = profiled_traffic("lu_update_split_4096x64_traffic.md")
         rl_update = {"bound": "hbm", "kernel": "zgemm_split_kernel (LU trailing update, split-bf16 MFMA)",
                      "achieved": bu / (tu * 1e-3) / 1e9 if tu else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -26,6 +26,10 @@ struct PhaseAcc {
 struct mhx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // second stream (default priority) for the other half of a batched factorization, with the
+    // events that fork it from and join it back to `stream`
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     double rho = 1000.0, g = 9.81;
     int timing = 0;
     std::string err;
@@ -360,6 +364,29 @@ constexpr double kRefineTol = 1e-12;
 constexpr int kRefineMax = 4;       // FP32 trailing updates
 constexpr int kRefineMaxSplit = 8;  // split-bf16 trailing updates (factors to ~2^-16)
 
+// FP32 LU of the batch's b->Df in place.  A batch of 8 or more items of order >= 512 is factored
+// as two halves on two streams: the panel phases (launches of one workgroup per item,
+// latency-bound) of one half can then run beside the trailing updates of the other, on compute
+// units the panels leave idle.  The halves share nothing but the device.
+int getrf_f32_two_streams(mhx_ctx* ctx, mhx_body* b, int batch, mhx::LaunchTimer* lt, unsigned short* split) {
+    const int n = b->n;
+    const int h = (batch >= 8 && n >= 512) ? batch / 2 : 0;  // items on the side stream
+    const int m = batch - h;
+    int rc = 0;
+    if (h) {
+        HCHK(hipEventRecord(ctx->fork, ctx->stream));
+        HCHK(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+        rc = mhx::getrf_batched(ctx->side, n, b->Df + (size_t)m * b->nn(), n, b->nn(), h, b->ipiv + (size_t)m * n, n,
+                                b->wperm + (size_t)m * mhx::kPermWork, b->info + m, lt,
+                                split ? split + (size_t)m * mhx::split_work(n) : nullptr);
+        HCHK(hipEventRecord(ctx->join, ctx->side));
+    }
+    if (!rc)
+        rc = mhx::getrf_batched(ctx->stream, n, b->Df, n, b->nn(), m, b->ipiv, n, b->wperm, b->info, lt, split);
+    if (h) HCHK(hipStreamWaitEvent(ctx->stream, ctx->join, 0));
+    return rc;
+}
+
 int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf) {
     const int n = b->n;
     hipStream_t st = ctx->stream;
@@ -377,8 +404,7 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf)
         if (!b->df_ready) mhx::convert_to_f32(st, b->nn() * batch, b->D, b->Df);
         b->df_ready = false;
         mhx::LaunchTimer lt;
-        rc = mhx::getrf_batched(st, n, b->Df, n, b->nn(), batch, b->ipiv, n, b->wperm, b->info,
-                                ctx->timing ? &lt : nullptr, split ? b->Ps : nullptr);
+        rc = getrf_f32_two_streams(ctx, b, batch, ctx->timing ? &lt : nullptr, split ? b->Ps : nullptr);
         take_launches(ctx, lt);
         if (rc) return fail(ctx, MHX_EDIM, "LU: order above 4096 is not supported");
     }
@@ -642,7 +668,14 @@ int mhx_create(mhx_ctx** out, const mhx_options* opt) {
         delete c;
         return MHX_EDEVICE;
     }
-    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
+        if (c->side) hipStreamDestroy(c->side);
+        if (c->stream) hipStreamDestroy(c->stream);
+        if (c->fork) hipEventDestroy(c->fork);
+        if (c->join) hipEventDestroy(c->join);
         delete c;
         return MHX_EDEVICE;
     }
@@ -658,7 +691,11 @@ void mhx_destroy(mhx_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     drop_sweep(c);
     dfree(c->surr_params); dfree(c->surr_B); dfree(c->surr_bias); dfree(c->surr_W);
+    if (c->side) hipStreamSynchronize(c->side);
     if (c->stream) hipStreamDestroy(c->stream);
+    if (c->side) hipStreamDestroy(c->side);
+    if (c->fork) hipEventDestroy(c->fork);
+    if (c->join) hipEventDestroy(c->join);
     delete c;
 }
 

@@ -34,6 +34,19 @@ __host__ __device__ __forceinline__ void surr_inputs(double r, double v, float& 
     u2 = (float)((log(vv) - lv0) / (lv1 - lv0));
 }
 
+// the same mapping at inference, in FP32 with base-2 logarithms (the ratios of logarithms do not
+// depend on the base): u differs from the FP64 mapping by a few 1e-8, far below the BF16
+// rounding of the Fourier features it feeds
+__device__ __forceinline__ void surr_inputs_f32(double r, double v, float& u1, float& u2) {
+    const float l2r0 = -9.965784284662087f;            // log2(1e-3)
+    const float inv_r = 1.0f / (6.6438706166529995f + 9.965784284662087f);   // 1 / (log2(100 + 1e-3) - log2(1e-3))
+    const float l2v0 = -10.965784284662087f;           // log2(5e-4)
+    const float inv_v = 1.0f / (5.643856189774724f + 10.965784284662087f);  // 1 / (log2(50) - log2(5e-4))
+    const float vv = fmaxf((float)v, 5e-4f);
+    u1 = (__builtin_amdgcn_logf((float)r + 1e-3f) - l2r0) * inv_r;
+    u2 = (__builtin_amdgcn_logf(vv) - l2v0) * inv_v;
+}
+
 __device__ __forceinline__ bool surr_in_box(double r, double v) {
     return r <= SURR_RMAX && v <= SURR_VMAX && v >= SURR_VMIN;
 }
@@ -570,27 +583,27 @@ __global__ void __launch_bounds__(64 * SURR_WAVES) surr_table_kernel(
         int bi, bj;
         tri_tile(unit - (int64_t)item * per_item, bi, bj);
         const double k = ks[item];
-        // the lane's two points: pairs c and 32 + c of the unit
-        float u1[2], u2[2];
+        // the lane's own pair 32 h + c; column tile pt of the network holds pairs 32 pt + c, so a
+        // lane takes the other tile's inputs from its partner lane (lane ^ 32)
         double hh_me = 0.0, vv_me = 0.0;
-        bool valid_me = false;
-#pragma unroll
-        for (int pt = 0; pt < 2; pt++) {
-            const int p = 32 * pt + c;
+        bool valid_me;
+        float u1[2], u2[2];
+        {
+            const int p = 32 * h + c;
             const int i = 8 * bi + (p & 7), j = 8 * bj + (p >> 3);
-            double hh = 0.0, vv = 0.0;
-            const bool valid = i < n && j < n && i <= j;
-            if (valid) {
+            valid_me = i < n && j < n && i <= j;
+            if (valid_me) {
                 const PanelGeom& gi = geom[i];
                 const PanelGeom& gj = geom[j];
-                reduced_hv(gi.c[0], gi.c[1], gi.c[2], gj.c[0], gj.c[1], gj.c[2], k, hh, vv);
+                reduced_hv(gi.c[0], gi.c[1], gi.c[2], gj.c[0], gj.c[1], gj.c[2], k, hh_me, vv_me);
             }
-            surr_inputs(hh, -vv, u1[pt], u2[pt]);
-            if (pt == h) {
-                hh_me = hh;
-                vv_me = vv;
-                valid_me = valid;
-            }
+            float a1, a2;
+            surr_inputs_f32(hh_me, -vv_me, a1, a2);
+            const float b1 = __shfl_xor(a1, 32), b2 = __shfl_xor(a2, 32);
+            u1[0] = h ? b1 : a1;
+            u1[1] = h ? a1 : b1;
+            u2[0] = h ? b2 : a2;
+            u2[1] = h ? a2 : b2;
         }
         f32x16 out[2];
         surr_mlp64(L, lane, u1, u2, out);
@@ -692,7 +705,7 @@ __global__ void __launch_bounds__(64 * SURR_WAVES) surr_eval_kernel(int M, const
             const int m = 64 * unit + 32 * pt + c;
             double r = 0.0, v = 0.005;
             if (m < M) { r = rv[2 * m]; v = rv[2 * m + 1]; }
-            surr_inputs(r, v, u1[pt], u2[pt]);
+            surr_inputs_f32(r, v, u1[pt], u2[pt]);
         }
         f32x16 o[2];
         surr_mlp64(L, lane, u1, u2, o);
