@@ -7,9 +7,9 @@
 //
 // Blocking (right-looking):
 //   outer panels of NB=64 columns; each outer panel is factored in leaves of
-//   LW=4 columns held in registers of one 1024-thread workgroup (the pivot
-//   search is a block reduction), each leaf followed by a multi-workgroup
-//   rank-4 update of the rest of the outer panel;
+//   W columns (4 for FP64, 8 for FP32) held in registers of one 1024-thread
+//   workgroup (the pivot search is a block reduction), each leaf followed by a
+//   multi-workgroup rank-W update of the rest of the outer panel;
 //   then the 64 row interchanges are applied to the other columns, the
 //   64-row U block is solved (unit-lower TRSM), and the trailing matrix is
 //   updated by a complex GEMM on FP64 MFMA (v_mfma_f64_16x16x4f64).
@@ -53,7 +53,6 @@ template <class C> __device__ __forceinline__ C trecip(C u) {
 }
 template <class C> __device__ __forceinline__ cplx to_d(C v) { return cmk((double)v.x, (double)v.y); }
 
-#define LW 4
 #define NB 64
 #define LEAF_THREADS 1024
 
@@ -65,11 +64,11 @@ struct MatBatch {
 };
 
 // ---------------------------------------------------------------------------
-// leaf: factor columns [c0, c0+LW) over rows [c0, n); swap the pivot rows in
-// the other columns of the outer panel [K, KE); solve the 4-row U block of the
+// leaf: factor columns [c0, c0+W) over rows [c0, n); swap the pivot rows in
+// the other columns of the outer panel [K, KE); solve the W-row U block of the
 // outer panel columns right of the leaf.
 // ---------------------------------------------------------------------------
-template <int ROWS, class C>
+template <int W, int ROWS, class C>
 __global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch<C> mb, int n, int c0, int K, int KE,
                                                                   int* __restrict__ ipiv, int64_t ipiv_stride,
                                                                   int* __restrict__ info) {
@@ -82,21 +81,21 @@ __global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch<C> mb
     // per column parity (the LDS of column c is reused at c + 2, after the two barriers of c + 1)
     __shared__ R red_v[2][LEAF_THREADS / 64];
     __shared__ int red_r[2][LEAF_THREADS / 64];
-    __shared__ C xrow[2][LW];  // row col before the interchange
-    __shared__ C prow[2][LW];  // the pivot row
-    __shared__ int s_pivs[LW];
-    __shared__ C L11[LW][LW];
+    __shared__ C xrow[2][W];  // row col before the interchange
+    __shared__ C prow[2][W];  // the pivot row
+    __shared__ int s_pivs[W];
+    __shared__ C L11[W][W];
 
-    const int wcols = min(LW, n - c0);
-    C x[ROWS][LW];
+    const int wcols = min(W, n - c0);
+    C x[ROWS][W];
 #pragma unroll
     for (int q = 0; q < ROWS; q++) {
         int r = c0 + tid + LEAF_THREADS * q;
 #pragma unroll
-        for (int c = 0; c < LW; c++) x[q][c] = (r < n && c < wcols) ? A[r + (int64_t)lda * (c0 + c)] : tmk<C>(0, 0);
+        for (int c = 0; c < W; c++) x[q][c] = (r < n && c < wcols) ? A[r + (int64_t)lda * (c0 + c)] : tmk<C>(0, 0);
     }
 #pragma unroll
-    for (int c = 0; c < LW; c++) {
+    for (int c = 0; c < W; c++) {
         if (c >= wcols) break;
         const int col = c0 + c, par = c & 1;
         // --- pivot search: max |Re|+|Im|, smallest row on ties
@@ -131,18 +130,18 @@ __global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch<C> mb
 #pragma unroll
         for (int q = 0; q < ROWS; q++) {
             if (tid == tc && q == qc)
-                for (int cc = 0; cc < LW; cc++) xrow[par][cc] = x[q][cc];
+                for (int cc = 0; cc < W; cc++) xrow[par][cc] = x[q][cc];
             if (tid == tp && q == qp)
-                for (int cc = 0; cc < LW; cc++) prow[par][cc] = x[q][cc];
+                for (int cc = 0; cc < W; cc++) prow[par][cc] = x[q][cc];
         }
         __syncthreads();
         if (p != col) {
 #pragma unroll
             for (int q = 0; q < ROWS; q++) {
                 if (tid == tc && q == qc)
-                    for (int cc = 0; cc < LW; cc++) x[q][cc] = prow[par][cc];
+                    for (int cc = 0; cc < W; cc++) x[q][cc] = prow[par][cc];
                 if (tid == tp && q == qp)
-                    for (int cc = 0; cc < LW; cc++) x[q][cc] = xrow[par][cc];
+                    for (int cc = 0; cc < W; cc++) x[q][cc] = xrow[par][cc];
             }
         }
         const C u = prow[par][c];
@@ -154,7 +153,7 @@ __global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch<C> mb
                 if (r > col && r < n) {
                     x[q][c] = tmul(x[q][c], inv);
 #pragma unroll
-                    for (int cc = 0; cc < LW; cc++)
+                    for (int cc = 0; cc < W; cc++)
                         if (cc > c) x[q][cc] = tsub(x[q][cc], tmul(x[q][c], prow[par][cc]));
                 }
             }
@@ -168,45 +167,49 @@ __global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch<C> mb
         int r = c0 + tid + LEAF_THREADS * q;
         if (r < n)
 #pragma unroll
-            for (int c = 0; c < LW; c++)
+            for (int c = 0; c < W; c++)
                 if (c < wcols) A[r + (int64_t)lda * (c0 + c)] = x[q][c];
     }
     // L11 (unit lower 4x4 of this leaf) to LDS
     if (tid < wcols)
-        for (int c = 0; c < LW; c++) L11[tid][c] = x[0][c];
+        for (int c = 0; c < W; c++) L11[tid][c] = x[0][c];
     __syncthreads();
     // other outer-panel columns: apply the leaf's interchanges, then solve U rows
     const int nother = (KE - K) - wcols;
-    // The at most 2 LW rows a column touches (c0 + c and the pivot rows) are loaded together, the
+    // The at most 2 W rows a column touches (c0 + c and the pivot rows) are loaded together, the
     // interchanges and the substitution done on the register copies (an entry stands for its row;
     // a row listed twice keeps both copies equal), and all of them stored back.
-    int idx[2 * LW];
+    // (the rows are the same for every thread: scalar registers)
+    int idx[2 * W];
 #pragma unroll
-    for (int c = 0; c < LW; c++) {
+    for (int c = 0; c < W; c++) {
         idx[c] = c0 + min(c, wcols - 1);
-        idx[LW + c] = c < wcols ? s_pivs[c] : c0;
+        idx[W + c] = __builtin_amdgcn_readfirstlane(c < wcols ? s_pivs[c] : c0);
     }
     for (int t = tid; t < nother; t += LEAF_THREADS) {
+        // L11 is read from LDS inside the loop: hoisted out of it, the W x W block would occupy
+        // 2 W^2 registers for the whole loop (it runs once per thread: nother < LEAF_THREADS)
+        asm volatile("" ::: "memory");
         int j = K + t;
         if (j >= c0) j += wcols;
         C* colj = A + (int64_t)lda * j;
-        C v[2 * LW];
+        C v[2 * W];
 #pragma unroll
-        for (int k = 0; k < 2 * LW; k++) v[k] = colj[idx[k]];
+        for (int k = 0; k < 2 * W; k++) v[k] = colj[idx[k]];
         auto get = [&](int r) {
             C w = v[0];
 #pragma unroll
-            for (int k = 2 * LW - 1; k >= 0; k--)
+            for (int k = 2 * W - 1; k >= 0; k--)
                 if (idx[k] == r) w = v[k];
             return w;
         };
         auto set = [&](int r, C w) {
 #pragma unroll
-            for (int k = 0; k < 2 * LW; k++)
+            for (int k = 0; k < 2 * W; k++)
                 if (idx[k] == r) v[k] = w;
         };
 #pragma unroll
-        for (int c = 0; c < LW; c++) {
+        for (int c = 0; c < W; c++) {
             if (c >= wcols) break;
             const int r0 = c0 + c, r1 = s_pivs[c];
             if (r1 != r0) {
@@ -216,9 +219,9 @@ __global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch<C> mb
             }
         }
         if (j > c0) {  // U12 rows c0..c0+wcols-1: unit-lower forward substitution
-            C u[LW];
+            C u[W];
 #pragma unroll
-            for (int c = 0; c < LW; c++) {
+            for (int c = 0; c < W; c++) {
                 if (c >= wcols) break;
                 C s = get(c0 + c);
 #pragma unroll
@@ -228,29 +231,29 @@ __global__ void __launch_bounds__(LEAF_THREADS) getf2_leaf_kernel(MatBatch<C> mb
             }
         }
 #pragma unroll
-        for (int k = 0; k < 2 * LW; k++) colj[idx[k]] = v[k];
+        for (int k = 0; k < 2 * W; k++) colj[idx[k]] = v[k];
     }
 }
 
-// rank-LW update of the outer panel: A[r, j] -= sum_s L[r, c0+s] U[c0+s, j],
-// rows r in [c0+LW, n), columns j in [c0+LW, KE)
-template <class C>
+// rank-W update of the outer panel: A[r, j] -= sum_s L[r, c0+s] U[c0+s, j],
+// rows r in [c0+W, n), columns j in [c0+W, KE)
+template <int W, class C>
 __global__ void __launch_bounds__(256) panel_update_kernel(MatBatch<C> mb, int n, int c0, int KE) {
     const int b = blockIdx.z;
     C* A = mb.A + mb.stride * b;
     const int lda = mb.lda;
-    __shared__ C U[LW][NB];
-    const int j0 = c0 + LW, nj = KE - j0;
-    for (int t = threadIdx.x; t < LW * nj; t += blockDim.x) {
+    __shared__ C U[W][NB];
+    const int j0 = c0 + W, nj = KE - j0;
+    for (int t = threadIdx.x; t < W * nj; t += blockDim.x) {
         int s = t / nj, jj = t % nj;
         U[s][jj] = A[(c0 + s) + (int64_t)lda * (j0 + jj)];
     }
     __syncthreads();
-    const int r = c0 + LW + blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = c0 + W + blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= n) return;
-    C l[LW];
+    C l[W];
 #pragma unroll
-    for (int s = 0; s < LW; s++) l[s] = A[r + (int64_t)lda * (c0 + s)];
+    for (int s = 0; s < W; s++) l[s] = A[r + (int64_t)lda * (c0 + s)];
     // 16 columns per round trip: their loads are issued before any of their stores (element by
     // element, each load would wait for the previous store through the same pointer)
     constexpr int JC = 16;
@@ -263,7 +266,7 @@ __global__ void __launch_bounds__(256) panel_update_kernel(MatBatch<C> mb, int n
         for (int q = 0; q < JC; q++)
             if (jb + q < nj) {
 #pragma unroll
-                for (int s = 0; s < LW; s++) a[q] = tsub(a[q], tmul(l[s], U[s][jb + q]));
+                for (int s = 0; s < W; s++) a[q] = tsub(a[q], tmul(l[s], U[s][jb + q]));
                 A[r + (int64_t)lda * (j0 + jb + q)] = a[q];
             }
     }
@@ -788,21 +791,23 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
     const int64_t pstride = 1 + 4 * NB;
     constexpr bool kF32 = std::is_same<C, cplxf>::value;
     const bool use_split = kF32 && split != nullptr;
-    // leaves of 4 columns over [K, KE), each followed by the rank-4 update of the panel's rest
+    // leaves of W columns over [K, KE), each followed by the rank-W update of the panel's rest
+    // (W = 8 for FP32: half the latency-bound leaf launches; FP64 keeps 4, its registers)
+    constexpr int W = kF32 ? 8 : 4;
     auto factor_panel = [&](int K, int KE) {
-        for (int c0 = K; c0 < KE; c0 += LW) {
+        for (int c0 = K; c0 < KE; c0 += W) {
             const int m_rows = n - c0;
             const int rp = (m_rows + LEAF_THREADS - 1) / LEAF_THREADS;
             dim3 g1(1, 1, batch);
             if (rp <= 1)
-                getf2_leaf_kernel<1, C><<<g1, LEAF_THREADS, 0, st>>>(mb, n, c0, K, KE, ipiv, ipiv_stride, info);
+                getf2_leaf_kernel<W, 1, C><<<g1, LEAF_THREADS, 0, st>>>(mb, n, c0, K, KE, ipiv, ipiv_stride, info);
             else if (rp <= 2)
-                getf2_leaf_kernel<2, C><<<g1, LEAF_THREADS, 0, st>>>(mb, n, c0, K, KE, ipiv, ipiv_stride, info);
+                getf2_leaf_kernel<W, 2, C><<<g1, LEAF_THREADS, 0, st>>>(mb, n, c0, K, KE, ipiv, ipiv_stride, info);
             else
-                getf2_leaf_kernel<4, C><<<g1, LEAF_THREADS, 0, st>>>(mb, n, c0, K, KE, ipiv, ipiv_stride, info);
-            if (c0 + LW < KE && c0 + LW < n) {
-                dim3 g2((n - c0 - LW + 255) / 256, 1, batch);
-                panel_update_kernel<C><<<g2, 256, 0, st>>>(mb, n, c0, KE);
+                getf2_leaf_kernel<W, 4, C><<<g1, LEAF_THREADS, 0, st>>>(mb, n, c0, K, KE, ipiv, ipiv_stride, info);
+            if (c0 + W < KE && c0 + W < n) {
+                dim3 g2((n - c0 - W + 255) / 256, 1, batch);
+                panel_update_kernel<W, C><<<g2, 256, 0, st>>>(mb, n, c0, KE);
             }
         }
     };
