@@ -76,3 +76,22 @@ def test_mixed_coefficients_match_oracle(mhx, oracle, mixed, gf, direct):
     _coeff_parity(mhx, oracle, sphere64(), [0.5, 1.03, 2.2], [0.0, math.pi / 3], gf, direct, mode="batched")
     solved, steps, fallbacks, resid = mixed.solver_stats()
     assert solved == 3 and fallbacks == 0 and resid <= 1e-12
+
+
+def test_two_stream_batch_matches_fp64(mhx, mixed):
+    """A batch of 9 frequencies on a 768-panel cylinder (order >= 512, 8 or more items): the FP32
+    factorization runs as two halves (4 and 5 matrices) on two streams.  Every coefficient matches
+    the FP64 LU path to 1e-9, every system refined to 1e-12, none fell back."""
+    print("mixed two-stream batch", flush=True)
+    from mhx import _lib, meshes
+    body = mhx.FloatingBody(meshes.cylinder(1.0, 2.0, 32, 16, 8), ["Surge", "Heave", "Pitch"])
+    par = dict(wave_frequencies=list(np.linspace(0.3, 2.7, 9)), radiating_dofs=["Surge", "Heave", "Pitch"],
+               wave_directions=[0.0, 1.0])
+    got = mhx.hydrodynamic_coefficients(body, par, mhx.DirectBEM("Wu"), schedule="batched")
+    solved, steps, fallbacks, resid = mixed.solver_stats()
+    assert solved == 9 and fallbacks == 0 and resid <= 1e-12
+    mixed.set_lu_mode(_lib.LU_FP64)
+    ref = mhx.hydrodynamic_coefficients(body, par, mhx.DirectBEM("Wu"), schedule="batched")
+    for key in ("added_mass", "radiation_damping", "excitation_force"):
+        err = np.linalg.norm(got[key] - ref[key]) / np.linalg.norm(ref[key])
+        assert err <= 1e-9, (key, err)
