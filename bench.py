@@ -5,19 +5,24 @@ draft 12 m, 4,096 panels (64 sectors x (40 side + 24 cap rows)), 64
 frequencies uniform in 0.1-3.0 rad/s, 6 rigid dofs about the origin, heading
 beta = 0, rho = 1025, g = 9.81, direct method.
 
-A step is one omega-solve: wave-term assembly (the wavenumber-independent
-Rankine part is cached per body), one LU, all right-hand sides (6 radiation +
-1 diffraction) and pressure integration into A, B and F_ex.  K steps use K
-frequencies spread over the 64-frequency sweep (default: all 64, one batch).  The mesh, dofs, the cached
-Rankine part and the network weights are resident on the device before the
-timed region.
+A step is one sweep of all 64 frequencies, each an omega-solve: wave-term
+assembly (the wavenumber-independent Rankine part is cached per body), one LU,
+all right-hand sides (6 radiation + 1 diffraction) and pressure integration
+into A, B and F_ex; value = 64 K / elapsed for K timed steps.  The mesh, dofs,
+the cached Rankine part, the network weights and the batch buffers (allocated
+by the warmup sweep at the same batch size) are resident on the device before
+the timed region.
 
 The headline path is the learned surrogate of the exact Guevel-Delhommeau term
 (Wu fit + BF16-MFMA network residual, exact FP64 fallback near the free
-surface) with all K frequencies through the batched schedule.  It is compared
-with the plain GPU path (exact FP64 wave term, one frequency at a time) on a
-spread subset of the same frequencies, and its coefficients are checked
-against that exact path (relative error reported).
+surface) with the 64 frequencies of a sweep as one batch.  The plain GPU path
+(FP64 throughout) is reported four ways on 4 of the frequencies: exact term one
+frequency at a time (the denominator of speedup_vs_plain), the Wu term one
+frequency at a time, and both with the reference's schedule (assembly and LU
+per problem).  The headline path's coefficients are checked against the exact
+path at those frequencies (relative error reported).  The CPU baseline is the
+oracle on the host cores in two labelled modes (best CPU; reference-faithful,
+sampled and extrapolated).
 
 Prints one JSON line (last line of stdout).
 """
@@ -57,21 +62,43 @@ def workload():
     return mesh, omegas
 
 
-def cpu_baseline(mesh, omegas, nfreq, threads):
-    """The oracle (C restatement of the reference) on the host cores: nfreq complete omega-solves
-    (full exact assembly of S and D with OpenMP, LAPACK LU and solves through numpy's OpenBLAS,
-    pressure integration is negligible)."""
+def host_cpu():
+    """(logical CPUs the OS reports, CPU model) of the host."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return os.cpu_count() or 1, model
+
+
+def cpu_baseline(mesh, omegas, nfreq, threads, sample_rows=32):
+    """The oracle (C restatement of the reference) on the host cores, in the two modes of the
+    CPU-baseline plan, both for complete omega-solves of the bench body (6 dofs, beta = 0):
+      best CPU: nfreq complete omega-solves, exact assembly of S and D once per frequency on
+        `threads` OpenMP threads, LAPACK LU and multi-RHS solve through numpy's OpenBLAS (timed in full);
+      reference-faithful: the reference's schedule (solve.jl:75-78: assembly and LU per
+        problem, 7 problems per frequency) with single-threaded assembly as the reference's serial
+        loops; the single-thread assembly is timed on `sample_rows` seeded rows and scaled to n
+        rows (the oracle evaluates every entry on its own, so the cost is linear in the rows), the
+        LU and one-column solve are timed in full.
+    Returns (best, faithful)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     import scipy.linalg as sl
     O.build()
-    O.set_threads(threads)
+    nproc, model = host_cpu()
     om = O.Mesh.from_mhx(mesh)
     n = mesh.nfaces
     ws = omegas[np.linspace(0, len(omegas) - 1, nfreq + 2).round().astype(int)[1:-1]]
     rng = np.random.default_rng(0)
     rhs = rng.standard_normal((n, 7)) + 1j * rng.standard_normal((n, 7))
+    O.set_threads(threads)
     t_asm = t_lu = 0.0
+    D = S = None
     for w in ws:
         t0 = time.perf_counter()
         S, D = O.assemble(om, w * w / 9.81, gf=O.GF_DELHOMMEAU, direct=True)
@@ -82,10 +109,34 @@ def cpu_baseline(mesh, omegas, nfreq, threads):
         t_asm += t1 - t0
         t_lu += t2 - t1
     per = (t_asm + t_lu) / len(ws)
-    return {"value": 1.0 / per, "unit": "omega-solves/s", "cores": threads, "kind": "port",
+    host = f"host: {nproc} logical CPUs ({model}); {threads} threads used (the GPU box's CPU share)"
+    best = {"value": 1.0 / per, "unit": "omega-solves/s", "cores": threads, "kind": "port", "mode": "best CPU",
             "sample": f"{len(ws)} complete omega-solves at omega = {np.round(ws, 3).tolist()} on {n} panels: "
-                      f"exact assembly (oracle, {threads} OpenMP threads, {t_asm:.2f} s) + LAPACK zgetrf/zgetrs "
-                      f"and S.B (numpy's OpenBLAS, {t_lu:.2f} s)"}
+                      f"exact assembly of S and D once per frequency (oracle, {threads} OpenMP threads, "
+                      f"{t_asm:.2f} s) + LAPACK zgetrf/zgetrs and S.B for 7 right-hand sides (numpy's OpenBLAS, "
+                      f"{t_lu:.2f} s); {host}"}
+    # reference-faithful: single-thread assembly on seeded rows, per-problem LU
+    O.set_threads(1)
+    w = float(ws[0])
+    rows = np.sort(np.random.default_rng(1).choice(n, size=sample_rows, replace=False))
+    t0 = time.perf_counter()
+    for i in rows:
+        O.assemble_rows(om, w * w / 9.81, (int(i), int(i) + 1), gf=O.GF_DELHOMMEAU, direct=True)
+    t_rows = time.perf_counter() - t0
+    t_full_asm = t_rows * n / sample_rows
+    t0 = time.perf_counter()
+    lu = sl.lu_factor(D, check_finite=False)
+    sl.lu_solve(lu, S @ rhs[:, :1], check_finite=False)
+    t_lu1 = time.perf_counter() - t0
+    per_f = 7 * (t_full_asm + t_lu1)
+    faithful = {"value": 1.0 / per_f, "unit": "omega-solves/s", "cores": 1, "kind": "port",
+                "mode": "reference-faithful (sampled, extrapolated)",
+                "sample": f"single-thread exact assembly of {sample_rows} seeded rows at omega = {w:.3f} "
+                          f"({t_rows:.2f} s) scaled to {n} rows ({t_full_asm:.1f} s per assembly), LAPACK "
+                          f"zgetrf/zgetrs + S.b for one right-hand side ({t_lu1:.2f} s, numpy's OpenBLAS, "
+                          f"{threads} threads), 7 problems per frequency each with its own assembly and LU "
+                          f"(solve.jl:75-78); {host}"}
+    return best, faithful
 
 
 def cpu_baseline_fs(mesh, U, w, beta, threads):
@@ -179,14 +230,15 @@ def lu_kind(mode, gf):
     return "fp32" if mode == "mixed" else "fp64"
 
 
-def measure(ctx, run, ws, n, gf, nlu=None, units=None, kind="fp64"):
-    """One timed call over the frequencies ws; returns rate, phase table and roofline of the
-    dominant kernel (per-launch HIP events on the library's stream).  nlu: factorizations the call
-    makes (default one per frequency); units: what the rate counts (default the frequencies);
+def measure(ctx, run, ws, n, gf, nlu=None, units=None, kind="fp64", calls=1):
+    """`calls` timed calls of run(ws); returns rate, phase table and roofline of the dominant
+    kernel (per-launch HIP events on the library's stream).  nlu: factorizations per call
+    (default one per frequency); units: what the rate counts per call (default the frequencies);
     kind: the trailing-update kernel (lu_kind)."""
     ctx.reset_timing()
     t0 = time.perf_counter()
-    A, B, Fd, Fk = run(ws)
+    for _ in range(calls):
+        A, B, Fd, Fk = run(ws)
     elapsed = time.perf_counter() - t0
     assert np.all(np.isfinite(A)) and np.all(np.isfinite(Fd))
     phases = {p: ctx.phase_time(p) for p in ("wave_table", "wave", "lu", "lu_update", "solve", "forces", "rankine",
@@ -195,20 +247,21 @@ def measure(ctx, run, ws, n, gf, nlu=None, units=None, kind="fp64"):
     tw, cw = phases["wave_table"]
     tu, cu = phases["lu_update"]
     tl, cl = phases["lu"]
-    nw = len(ws) if nlu is None else nlu
-    units = len(ws) if units is None else units
+    nw1 = len(ws) if nlu is None else nlu       # factorizations per call
+    nw = nw1 * calls
+    units = (len(ws) if units is None else units) * calls
     fu = lu_update_flops(n, two_level=kind == "split") * nw
     if kind == "split":
         # the split-bf16 update moves the trailing matrix once per step: HBM-bound
         bu = lu_update_bytes(n) * nw
         traffic = None
-        # the library factors a batch of >= 8 items of order >= 512 as two halves on two streams
-        halves = 2 if nw >= 8 and n >= 512 else 1
-        if n == 4096 and nw == 64 and cu == halves * len(lu_update_steps(n, two_level=True)):  # the profiled configuration
+        items = nw / cu * len(lu_update_steps(n, two_level=True)) if cu else 0
+        if n == 4096 and abs(items - 32) < 1e-9:  # the profiled configuration: 32 matrices per launch
             traffic = profiled_traffic("lu_update_split_4096x64_traffic.md")
         rl_update = {"bound": "hbm", "kernel": "zgemm_split_kernel (LU trailing update, split-bf16 MFMA)",
                      "achieved": bu / (tu * 1e-3) / 1e9 if tu else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "traffic": traffic, "avg_launch_ms": tu / cu if cu else None, "launches": cu,
+                     "matrices_per_launch": items,
                      "algorithmic_bytes_per_launch": bu / cu if cu else None,
                      "complex_tflops": fu / (tu * 1e-3) / 1e12 if tu else None}
         if rl_update["achieved"]:
@@ -227,7 +280,7 @@ def measure(ctx, run, ws, n, gf, nlu=None, units=None, kind="fp64"):
         pairs = n * (n + 1) / 2 * nw
         ach = pairs * surrogate_flops_per_pair() / (tw * 1e-3) / 1e12
         traffic = None
-        if n == 4096 and nw == 64 and cw == 1:   # the profiled configuration
+        if n == 4096 and cw and abs(nw / cw - 64) < 1e-9:   # the profiled configuration: 64 frequencies per launch
             traffic = profiled_traffic("surrogate_table_4096x64_traffic.md")
         rl_table = {"bound": "mfma", "kernel": "surr_table_kernel (+ exact fallback)", "achieved": ach,
                     "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / BF16_PEAK_TFLOPS, "traffic": traffic,
@@ -255,13 +308,14 @@ def measure(ctx, run, ws, n, gf, nlu=None, units=None, kind="fp64"):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=64, help="omega-solves timed (frequencies of the sweep)")
-    ap.add_argument("--warmup", type=int, default=64, help="untimed omega-solves (the first W timed frequencies)")
+    ap.add_argument("--steps", type=int, default=10,
+                    help="timed steps; a step is one sweep of all 64 frequencies (64 omega-solves)")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed sweeps before the timed ones")
     ap.add_argument("--gf", default="Surrogate", choices=["Surrogate", "ExactGuevelDelhommeau", "Wu"])
     ap.add_argument("--schedule", default="batched", choices=["batched", "per_omega"])
     ap.add_argument("--cpu-freqs", type=int, default=2, help="complete omega-solves timed on the host for cpu_baseline")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain per-omega comparison run")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU comparison runs")
     ap.add_argument("--lu-mode", default="auto", choices=["auto", "fp64", "mixed", "mixed_bf16"],
                     help="dense solver precision (auto: mixed FP32 factor for the surrogate, FP64 otherwise)")
     ap.add_argument("--workload", default="headline", choices=["headline", "forward_speed"],
@@ -275,6 +329,7 @@ def main():
     import mhx
     from mhx import _lib
 
+    modes = {"auto": _lib.LU_AUTO, "fp64": _lib.LU_FP64, "mixed": _lib.LU_MIXED, "mixed_bf16": _lib.LU_MIXED_BF16}
     mesh, omegas = workload()
     n = mesh.nfaces
     ctx = _lib.Context(rho=1025.0, g=9.81, device=0, timing=True)
@@ -283,18 +338,16 @@ def main():
     mhx.set_g(9.81)
     names = ["Surge", "Sway", "Heave", "Roll", "Pitch", "Yaw"]
     body = mhx.FloatingBody(mesh, names)
-    ctx.set_lu_mode({"auto": _lib.LU_AUTO, "fp64": _lib.LU_FP64, "mixed": _lib.LU_MIXED,
-                     "mixed_bf16": _lib.LU_MIXED_BF16}[args.lu_mode])
+    ctx.set_lu_mode(modes[args.lu_mode])
     if args.workload == "forward_speed":
         return bench_forward_speed(args, mhx, _lib, ctx, body, names, mesh, omegas)
     form = mhx.DirectBEM(args.gf)
-    idx = np.linspace(0, len(omegas) - 1, args.steps).round().astype(int)
-    steps_w = omegas[idx]
+    nw = len(omegas)
     mode = {"batched": _lib.MODE_BATCHED, "per_omega": _lib.MODE_PER_OMEGA}[args.schedule]
-    ctx.set_batch(max(args.steps, 1))
+    ctx.set_batch(nw)
 
-    def run_with(m):
-        return lambda ws: mhx.api._body_coeffs(body, names, ws, [0.0], form.gf, form.direct, m, 1025.0, 9.81)
+    def run_with(b, gf, direct, m):
+        return lambda ws: mhx.api._body_coeffs(b, names, ws, [0.0], gf, direct, m, 1025.0, 9.81)
 
     if args.gf == "Surrogate":
         t0 = time.perf_counter()
@@ -304,62 +357,75 @@ def main():
         else:
             val = mhx.train_surrogate()
             log(f"bench: trained surrogate on the device ({time.perf_counter() - t0:.1f} s, val mse {val:.2e})")
-    log(f"bench: n={n}, warmup over {args.warmup} frequencies")
-    warm = steps_w[:max(args.warmup, 1)]
-    run_with(mode)(warm)   # body upload, Rankine part, buffers: outside the timed region
-    log(f"bench: timing {args.steps} omega-solves ({args.schedule})")
-    value, elapsed, phases, roof, lu_tflops, res, roof2 = measure(ctx, run_with(mode), steps_w, n, args.gf,
-                                                                  kind=lu_kind(args.lu_mode, args.gf))
+    run = run_with(body, form.gf, form.direct, mode)
+    log(f"bench: n={n}, {max(args.warmup, 1)} warmup sweep(s) of {nw} frequencies")
+    for _ in range(max(args.warmup, 1)):
+        run(omegas)   # body upload, Rankine part, batch buffers at the timed size: outside the timed region
+    log(f"bench: timing {args.steps} sweeps of {nw} omega-solves ({args.schedule})")
+    value, elapsed, phases, roof, lu_tflops, res, roof2 = measure(ctx, run, omegas, n, args.gf,
+                                                                  kind=lu_kind(args.lu_mode, args.gf),
+                                                                  calls=args.steps)
     solved, steps, fallbacks, resid = ctx.solver_stats()
     solver = {"mixed_precision_systems": solved, "refinement_steps": steps, "fp64_fallbacks": fallbacks,
               "max_relative_residual": resid}
-    log("per-frequency phase ms:", phases, "solver:", solver)
+    log(f"bench: {value:.1f} omega-solves/s; per-frequency phase ms:", phases, "solver:", solver)
     plain = None
-    if not args.no_plain and (mode != _lib.MODE_PER_OMEGA or args.gf != "ExactGuevelDelhommeau"):
-        kp = min(4, args.steps)
-        sub = np.linspace(0, args.steps - 1, kp).round().astype(int)
-        log(f"bench: plain GPU path (exact FP64, per omega) at omega = {np.round(steps_w[sub], 3).tolist()}")
-        exact_form = mhx.DirectBEM("ExactGuevelDelhommeau")
-        run_plain = lambda ws: mhx.api._body_coeffs(body, names, ws, [0.0], exact_form.gf, True,
-                                                    _lib.MODE_PER_OMEGA, 1025.0, 9.81)
-        ctx.set_lu_mode(_lib.LU_FP64)  # the plain path: FP64 LU, as the reference
-        pv, pel, pph, proof, plu, pres, proof2 = measure(ctx, run_plain, steps_w[sub], n, "ExactGuevelDelhommeau")
-        ctx.set_lu_mode({"auto": _lib.LU_AUTO, "fp64": _lib.LU_FP64, "mixed": _lib.LU_MIXED,
-                         "mixed_bf16": _lib.LU_MIXED_BF16}[args.lu_mode])
-        errs = {}
-        pairs = (("added_mass", res[0][:, :, sub], pres[0], lambda x, q: x[:, :, q]),
-                 ("radiation_damping", res[1][:, :, sub], pres[1], lambda x, q: x[:, :, q]),
-                 ("excitation_force", (res[2] + res[3])[:, sub], pres[2] + pres[3], lambda x, q: x[:, q]))
-        for name, a, b, at in pairs:
-            errs[name] = max(float(np.linalg.norm(at(a, q) - at(b, q)) / np.linalg.norm(at(b, q))) for q in range(kp))
-        log("max relative error vs exact FP64 per omega:", errs)
-        plain = {"value": pv, "steps": kp, "omegas": steps_w[sub].tolist(), "phases_ms_per_omega": pph,
-                 "lu_tflops": plu, "max_rel_err_vs_exact": errs,
-                 # FP64 rates of the plain path's two main kernels against the MI355X peak
-                 "roofline": proof, "roofline_next_kernel": proof2}
+    if not args.no_plain:
+        # the plain GPU path on its own body handle (buffers for one frequency), FP64 LU as the reference
+        pbody = mhx.FloatingBody(mesh, names)
+        sub = np.linspace(0, nw - 1, 4).round().astype(int)
+        ws4 = omegas[sub]
+        ctx.set_lu_mode(_lib.LU_FP64)
+        plain = {}
+        for label, gf, m, ws in (("exact_per_omega", "ExactGuevelDelhommeau", _lib.MODE_PER_OMEGA, ws4),
+                                 ("wu_per_omega", "Wu", _lib.MODE_PER_OMEGA, ws4),
+                                 ("exact_reference_semantics", "ExactGuevelDelhommeau", _lib.MODE_PER_PROBLEM, ws4[1:2]),
+                                 ("wu_reference_semantics", "Wu", _lib.MODE_PER_PROBLEM, ws4[1:2])):
+            f = mhx.DirectBEM(gf)
+            prun = run_with(pbody, f.gf, True, m)
+            prun(ws[:1])  # buffers and the wave table of this Green function allocated
+            log(f"bench: plain GPU {label} at omega = {np.round(ws, 3).tolist()}")
+            pv, pel, pph, proof, plu, pres, proof2 = measure(ctx, prun, ws, n, gf, nlu=len(ws) * (7 if m == _lib.MODE_PER_PROBLEM else 1))
+            entry = {"value": pv, "unit": "omega-solves/s", "omegas": np.round(ws, 4).tolist(),
+                     "schedule": "assembly + FP64 LU per problem (7 per frequency, solve.jl:75-78)"
+                     if m == _lib.MODE_PER_PROBLEM else "one assembly + FP64 LU + 7 right-hand sides per frequency",
+                     "phases_ms_per_factorization": pph, "lu_tflops": plu}
+            if label == "exact_per_omega":
+                errs = {}
+                pairs = (("added_mass", lambda r, q: r[0][:, :, q]), ("radiation_damping", lambda r, q: r[1][:, :, q]),
+                         ("excitation_force", lambda r, q: (r[2] + r[3])[:, q]))
+                for name, at in pairs:
+                    errs[name] = max(float(np.linalg.norm(at(res, sub[q]) - at(pres, q)) / np.linalg.norm(at(pres, q)))
+                                     for q in range(len(sub)))
+                log("max relative error of the headline path vs exact FP64 per omega:", errs)
+                entry.update(max_rel_err_vs_exact=errs, roofline=proof, roofline_next_kernel=proof2)
+            plain[label] = entry
+        ctx.set_lu_mode(modes[args.lu_mode])
 
-    cpu = None
+    cpu = cpu_faithful = None
     if not args.no_cpu_baseline:
-        log("bench: cpu baseline")
-        cpu = cpu_baseline(mesh, omegas, args.cpu_freqs, min(16, os.cpu_count() or 1))
+        log("bench: cpu baseline (oracle, both modes)")
+        cpu, cpu_faithful = cpu_baseline(mesh, omegas, args.cpu_freqs, min(16, os.cpu_count() or 1))
 
+    denom = plain["exact_per_omega"]["value"] if plain else None
     out = {
         "metric": METRIC, "value": value, "unit": "omega-solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": DTYPE[args.gf], "data": "synthetic",
         "config": {"workload": f"{args.schedule} GPU path, wave term {WAVE_DESC[args.gf]}, direct method: "
-                               f"capped cylinder R=4 draft=12, {n} panels, 6 dofs, beta=0, "
-                               f"{args.steps} of the 64 frequencies in 0.1-3.0 rad/s",
-                   "panels": n, "dofs": 6, "headings": 1, "schedule": args.schedule,
+                               f"capped cylinder R=4 draft=12, {n} panels, 6 dofs, beta=0, all {nw} frequencies "
+                               f"in 0.1-3.0 rad/s per step (one step = one {nw}-frequency sweep)",
+                   "panels": n, "dofs": 6, "headings": 1, "frequencies_per_step": nw, "schedule": args.schedule,
                    "parallelism": "single GPU"},
         "phases_ms_per_omega": phases,
         "lu_tflops": lu_tflops,
         "solver": solver,
-        "plain_gpu_per_omega": plain,
-        "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "plain_gpu": plain,
+        "speedup_vs_plain": {"value": value / denom, "denominator": "plain_gpu.exact_per_omega"} if denom else None,
         "roofline": roof,
         "roofline_next_kernel": roof2,
         "cpu_baseline": cpu,
+        "cpu_baseline_reference_faithful": cpu_faithful,
     }
     print(json.dumps(out), flush=True)
 
@@ -369,14 +435,14 @@ FS_HEADINGS = [0.0, np.pi / 6, np.pi / 2, np.pi]
 
 
 def bench_forward_speed(args, mhx, _lib, ctx, body, names, mesh, omegas):
-    """The forward-speed grid: U in {0, 2} m/s, headings 0, pi/6, pi/2, pi, `steps` of the 64
+    """The forward-speed grid: U in {0, 2} m/s, headings 0, pi/6, pi/2, pi, 16 of the 64
     frequencies, six rigid dofs, IndirectBEM.  A unit is one (U, omega, beta) problem group (six
     radiation problems and one diffraction problem); at U = 0 every heading shares one solve per
     frequency, each moving group has its own encountered frequency, assembly, LU and K_x."""
     gf = "ExactGuevelDelhommeau" if args.gf == "Surrogate" else args.gf
     form = mhx.IndirectBEM(gf)
     n = mesh.nfaces
-    steps = min(args.steps, 16) if args.steps == 64 else args.steps
+    steps = 16
     ws = omegas[np.linspace(0, len(omegas) - 1, steps).round().astype(int)]
     nb, nu = len(FS_HEADINGS), len(FS_SPEEDS)
     ctx.set_batch(max(steps, 1))

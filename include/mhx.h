@@ -89,6 +89,14 @@ int mhx_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* D, const doub
  * X = A \ B (trans = 0) or A^H \ B (trans = 2); B overwritten by X.  Returns MHX_ESINGULAR on a zero pivot. */
 int mhx_lu_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* A, int trans, double* B);
 
+/* A batch of independent systems of one order (LinearSolve LUFactorization on each matrix, as the
+ * reference's solve(problems::Vector) issues one per problem: src/matrix_assembly.jl:411-416,
+ * src/solve.jl:75-78): X_q = A_q \ B_q (trans = 0) or A_q^H \ B_q (trans = 2) for q < batch.
+ * A: batch consecutive n x n column-major complex matrices; B: batch consecutive n x nrhs blocks,
+ * overwritten by the solutions.  The precision follows mhx_set_lu_mode (MHX_LU_AUTO: FP64; the
+ * mixed modes solve trans = 0 only and fall back to FP64 per the mixed-solver rules). */
+int mhx_lu_solve_batched(mhx_ctx* ctx, int64_t n, int64_t nrhs, int64_t batch, const double* A, int trans, double* B);
+
 /* hydrodynamic_coefficients(body, (wave_frequencies, radiating_dofs=all, wave_directions), formulation)
  * (src/problems_and_results.jl:464-495) with every dof radiating and influenced:
  *   A_out, B_out  ndof x ndof x nw          (influenced, radiating, omega)

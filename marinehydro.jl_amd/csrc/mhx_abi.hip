@@ -3,6 +3,7 @@
 // (reference: src/solve.jl:47-78, src/problems_and_results.jl:464-495).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -78,12 +79,18 @@ struct mhx_body {
     cplx *Rr = nullptr, *Tr = nullptr;
     double* norms = nullptr;
     bool df_ready = false;  // Df already holds the FP32 copy of D (written by the fused surrogate assembly)
+    // generation of the per-frequency buffers above: a new value whenever any of them is freed or
+    // reallocated, so that a captured sweep graph holding their addresses is not replayed
+    uint64_t gen = 0;
     int64_t nn() const { return (int64_t)n * n; }
     int64_t nR() const { return (int64_t)n * cap_R; }
     int64_t sW() const { return 16 * nR() + mhx::getrs_work(n, cap_R); }  // split-K partials + getrs work
 };
 
 namespace {
+
+std::atomic<uint64_t> g_buffer_gen{0};
+void bump_gen(mhx_body* b) { b->gen = ++g_buffer_gen; }
 
 int fail(mhx_ctx* c, int code, const std::string& msg) {
     if (c) c->err = msg;
@@ -181,13 +188,15 @@ void free_work(mhx_body* b) {
     dfree(b->F); dfree(b->FK); dfree(b->betas); dfree(b->d_ws); dfree(b->d_ks); dfree(b->ipiv); dfree(b->perm);
     dfree(b->wperm); dfree(b->info); dfree(b->status); dfree(b->fb_list); dfree(b->fb_count);
     dfree(b->Df); dfree(b->Ps); dfree(b->Rr); dfree(b->Tr); dfree(b->norms);
+    bump_gen(b);
     b->fb_cap = 0;
     b->cap_batch = b->cap_R = b->cap_nb = 0;
     b->cap_table = false;
 }
 
 // work buffers for `batch` frequency items with up to R right-hand sides and nb headings
-int ensure_work(mhx_ctx* ctx, mhx_body* b, int R, int nb, bool table, int batch = 1) {
+// (with_S = false: no S matrices, for a temporary body that only solves systems)
+int ensure_work(mhx_ctx* ctx, mhx_body* b, int R, int nb, bool table, int batch = 1, bool with_S = true) {
     if (batch <= b->cap_batch && R <= b->cap_R && nb <= b->cap_nb && (!table || b->cap_table)) return MHX_OK;
     const int nbt = batch > b->cap_batch ? batch : b->cap_batch;
     const int cr = R > 16 ? R : 16, cnb = nb > 8 ? nb : 8;
@@ -199,7 +208,7 @@ int ensure_work(mhx_ctx* ctx, mhx_body* b, int R, int nb, bool table, int batch 
     b->cap_table = tab;
     const size_t n = b->n, nn = n * n, B = nbt;
     int rc;
-    if ((rc = dalloc(ctx, &b->S, nn * B)) || (rc = dalloc(ctx, &b->D, nn * B))) return rc;
+    if ((with_S && (rc = dalloc(ctx, &b->S, nn * B))) || (rc = dalloc(ctx, &b->D, nn * B))) return rc;
     if (tab && ((rc = dalloc(ctx, &b->WG, nn * B)) || (rc = dalloc(ctx, &b->WR, nn * B)))) return rc;
     if ((rc = dalloc(ctx, &b->B, n * cr * B)) || (rc = dalloc(ctx, &b->Y, n * cr * B)) ||
         (rc = dalloc(ctx, &b->work, (size_t)b->sW() * B)) || (rc = dalloc(ctx, &b->F, (size_t)(b->ndof ? b->ndof : 1) * cr * B)) ||
@@ -247,6 +256,7 @@ int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int 
         if (b->fb_cap < cap) {
             int rc;
             dfree(b->fb_list);
+            bump_gen(b);
             b->fb_list = nullptr;
             b->fb_cap = 0;
             if (!b->fb_count && (rc = dalloc(ctx, &b->fb_count, 1))) return rc;
@@ -323,8 +333,10 @@ int check_info(mhx_ctx* ctx, mhx_body* b, int batch, bool* singular) {
     return MHX_OK;
 }
 
-// FP64 LU of D (in place) and the solves (rhs in b->B; solution potentials in b->Y)
-int solve_fp64(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
+// FP64 LU of D (in place) and the solves (rhs in b->B; solution potentials in b->Y).
+// form_phi = false (indirect only): the system's solution stays in b->B and S is not used;
+// trans = 2 then solves with D^H.
+int solve_fp64(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, bool form_phi = true, int trans = 0) {
     const int n = b->n;
     HCHK(hipMemsetAsync(b->info, 0, sizeof(int) * batch, ctx->stream));
     {
@@ -344,9 +356,11 @@ int solve_fp64(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch) {
             mhx::getrs_batched(ctx->stream, 0, n, R, b->D, n, b->nn(), b->ipiv, n, b->perm, n, b->Y, b->nR(), W2,
                                b->sW(), batch);
         } else {
-            mhx::getrs_batched(ctx->stream, 0, n, R, b->D, n, b->nn(), b->ipiv, n, b->perm, n, b->B, b->nR(), W2,
+            mhx::getrs_batched(ctx->stream, trans, n, R, b->D, n, b->nn(), b->ipiv, n, b->perm, n, b->B, b->nR(), W2,
                                b->sW(), batch);
-            mhx::launch_matmul_nr(ctx->stream, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), batch);
+            if (form_phi)
+                mhx::launch_matmul_nr(ctx->stream, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(),
+                                      batch);
         }
     }
     HCHK(hipGetLastError());
@@ -387,7 +401,7 @@ int getrf_f32_two_streams(mhx_ctx* ctx, mhx_body* b, int batch, mhx::LaunchTimer
     return rc;
 }
 
-int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf) {
+int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf, bool form_phi = true) {
     const int n = b->n;
     hipStream_t st = ctx->stream;
     int rc;
@@ -442,7 +456,7 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf)
             mhx::launch_add(st, (int64_t)n * R, b->Tr, b->nR(), X, b->nR(), batch);
             ctx->refine_steps += batch;
         }
-        if (converged && !direct)
+        if (converged && !direct && form_phi)
             mhx::launch_matmul_nr(st, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), batch);
         HCHK(hipGetLastError());
     }
@@ -453,7 +467,7 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf)
     // FP64 fallback (D is intact): restore the right-hand side and factor in FP64
     ctx->fp64_fallbacks += batch;
     if (!singular && !direct) HCHK(hipMemcpyAsync(b->B, b->Rr, sizeof(cplx) * nRb, hipMemcpyDeviceToDevice, st));
-    return solve_fp64(ctx, b, R, direct, batch);
+    return solve_fp64(ctx, b, R, direct, batch, form_phi);
 }
 
 int factor_and_solve(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch = 1, int gf = MHX_GF_WU) {
@@ -524,6 +538,8 @@ struct SweepState {
     int K = 1;                       // chunks
     int64_t nb = 0;
     int gf = 0, direct = 0;
+    double rho = 0.0, g = 0.0;       // constants the captured kernels carry by value
+    std::vector<uint64_t> gens;      // buffer generation of each body at capture
     std::vector<double*> w_all, k_all;  // device, per body: the sweep's frequencies and wavenumbers
     std::vector<cplx*> F_all, FK_all;   // device, per body: forces of every frequency
     std::vector<mhx::SweepItem> h_items;
@@ -816,6 +832,45 @@ int mhx_lu_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* A, int tra
     if (e != hipSuccess) return fail(ctx, MHX_EDEVICE, std::string("lu_solve: ") + hipGetErrorString(e));
     if (hinfo) return fail(ctx, MHX_ESINGULAR, "LU: exactly zero pivot at column " + std::to_string(hinfo));
     return MHX_OK;
+}
+
+int mhx_lu_solve_batched(mhx_ctx* ctx, int64_t n, int64_t nrhs, int64_t batch, const double* A, int trans, double* Bh) {
+    if (!ctx) return MHX_EINVAL;
+    if (n <= 0 || nrhs <= 0 || batch <= 0) return fail(ctx, MHX_EDIM, "lu_solve_batched: empty system or batch");
+    if (nrhs > 16) return fail(ctx, MHX_EDIM, "lu_solve_batched: at most 16 right-hand sides per system");
+    if (n > 4096) return fail(ctx, MHX_EDIM, "lu_solve_batched: order above 4096 is not supported");
+    if (batch > (1 << 16)) return fail(ctx, MHX_EDIM, "lu_solve_batched: batch too large");
+    if (!A || !Bh) return fail(ctx, MHX_EINVAL, "lu_solve_batched: null buffer");
+    if (trans != 0 && trans != 2) return fail(ctx, MHX_EINVAL, "lu_solve_batched: trans must be 0 or 2");
+    hipSetDevice(ctx->device);
+    mhx_body b;
+    b.ctx = ctx;
+    b.n = (int)n;
+    b.ndof = 0;
+    int rc = ensure_work(ctx, &b, (int)nrhs, 0, false, (int)batch, false);
+    if (!rc) {
+        hipStream_t st = ctx->stream;
+        const size_t nn = (size_t)n * n, nR = (size_t)n * nrhs;
+        for (int64_t q = 0; q < batch && !rc; q++) {
+            if (hipMemcpyAsync(b.D + q * b.nn(), A + 2 * nn * q, sizeof(cplx) * nn, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(b.B + q * b.nR(), Bh + 2 * nR * q, sizeof(cplx) * nR, hipMemcpyHostToDevice, st) != hipSuccess)
+                rc = fail(ctx, MHX_EDEVICE, "lu_solve_batched: upload");
+        }
+        // the mixed-precision solvers take A x = B only; A^H x = B is solved in FP64
+        const bool mixed = trans == 0 && (ctx->lu_mode == MHX_LU_MIXED || ctx->lu_mode == MHX_LU_MIXED_BF16);
+        if (!rc)
+            rc = mixed ? solve_mixed(ctx, &b, (int)nrhs, 0, (int)batch,
+                                     ctx->lu_mode == MHX_LU_MIXED_BF16 ? MHX_GF_SURROGATE : MHX_GF_WU, false)
+                       : solve_fp64(ctx, &b, (int)nrhs, 0, (int)batch, false, trans);
+        if (rc == MHX_ESINGULAR) rc = fail(ctx, MHX_ESINGULAR, ctx->err);
+        for (int64_t q = 0; q < batch && !rc; q++)
+            if (hipMemcpyAsync(Bh + 2 * nR * q, b.B + q * b.nR(), sizeof(cplx) * nR, hipMemcpyDeviceToHost, st) != hipSuccess)
+                rc = fail(ctx, MHX_EDEVICE, "lu_solve_batched: copy back");
+        if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(ctx, MHX_EDEVICE, "lu_solve_batched: sync");
+    }
+    flush_timing(ctx);
+    free_work(&b);
+    return rc;
 }
 
 int mhx_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* D, const double* S, const double* bc, int direct,
@@ -1544,15 +1599,18 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
     }
     std::vector<mhx_body*> bodies(nbodies);
     std::vector<int64_t> nws(nbodies);
+    std::vector<uint64_t> gens(nbodies);
     for (int64_t ib = 0; ib < nbodies; ib++) {
         bodies[ib] = in[ib].body;
         nws[ib] = in[ib].nw;
+        gens[ib] = in[ib].body->gen;
     }
     // graph mode keeps its state (and graph) in the context for replay; an eager sweep uses its own
     // state and leaves a captured sweep in place
     SweepState local;
     SweepState* s = graph ? ctx->sweep : &local;
-    const bool reuse = graph && s && s->exec && s->bodies == bodies && s->nws == nws && s->nb == nb && s->gf == gf &&
+    const bool reuse = graph && s && s->exec && s->bodies == bodies && s->nws == nws && s->gens == gens &&
+                       s->rho == ctx->rho && s->g == ctx->g && s->nb == nb && s->gf == gf &&
                        s->direct == direct && s->K == K && s->h_items.size() == items.size() &&
                        std::memcmp(s->h_items.data(), items.data(), items.size() * sizeof(mhx::SweepItem)) == 0;
     auto abandon = [&]() {
@@ -1566,6 +1624,9 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
         }
         s->bodies = bodies;
         s->nws = nws;
+        s->gens = gens;
+        s->rho = ctx->rho;
+        s->g = ctx->g;
         s->per_chunk = per;
         s->K = (int)K;
         s->nb = nb;

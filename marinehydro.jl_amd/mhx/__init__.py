@@ -9,7 +9,8 @@ from ._lib import DimensionMismatch, MHXError, SingularException  # noqa: F401
 from .api import (SETTINGS, DiffractionProblem, DirectBEM, ExactGuevelDelhommeau, FloatingBody, GFWu,  # noqa: F401
                   IndirectBEM, RadiationProblem, Rankine, RankineReflected, added_mass, assemble_matrices,
                   compute_wavenumber, context, default_greens_functions, diffraction_force, excitation_force,
-                  froude_krylov_force, hydrodynamic_coefficients, is_direct, lu_solve, radiation_damping,
+                  froude_krylov_force, hydrodynamic_coefficients, is_direct, lu_solve, lu_solve_batched,
+                  radiation_damping,
                   rotational_dofs, set_g, set_rho, solve, solve_linear, translational_dofs,
                   SurrogateGuevelDelhommeau, train_surrogate, load_surrogate,
                   hydrodynamic_coefficients_jvp, hydrodynamic_coefficients_vjp, mesh_scaling_tangent,

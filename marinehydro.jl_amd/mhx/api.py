@@ -903,3 +903,27 @@ def lu_solve(A, B, trans=0, device=0):
     ctx = context(device)
     ctx.check(_lib.lib().mhx_lu_solve(ctx.h, n, B2.shape[1], _lib.ptr(A), int(trans), _lib.ptr(B2)))
     return B2[:, 0] if vec else B2
+
+
+def lu_solve_batched(A, B, trans=0, device=0):
+    """X_q = A_q \\ B_q (trans=0) or A_q^H \\ B_q (trans=2) for a batch of systems of one order,
+    through the device's batched LU: A (batch, n, n), B (batch, n, R) or (batch, n).  The
+    precision follows the context's LU mode (context().set_lu_mode)."""
+    A = np.asarray(A, dtype=np.complex128)
+    B = np.asarray(B, dtype=np.complex128)
+    if A.ndim != 3 or A.shape[1] != A.shape[2]:
+        raise DimensionMismatch("A must be (batch, n, n)")
+    vec = B.ndim == 2
+    B3 = B[:, :, None] if vec else B
+    nb, n = A.shape[0], A.shape[1]
+    if B3.shape[0] != nb or B3.shape[1] != n:
+        raise DimensionMismatch("B must be (batch, n, R) with the batch and order of A")
+    R = B3.shape[2]
+    Ac = np.empty((nb, n, n), dtype=np.complex128)
+    Ac[...] = np.transpose(A, (0, 2, 1))   # each matrix column-major in its slot
+    Bc = np.empty((nb, R, n), dtype=np.complex128)
+    Bc[...] = np.transpose(B3, (0, 2, 1))
+    ctx = context(device)
+    ctx.check(_lib.lib().mhx_lu_solve_batched(ctx.h, n, R, nb, _lib.ptr(Ac), int(trans), _lib.ptr(Bc)))
+    X = np.transpose(Bc, (0, 2, 1))
+    return X[:, :, 0] if vec else X

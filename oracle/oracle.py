@@ -56,6 +56,10 @@ def lib():
         L.ora_expint_e1.argtypes = [C.c_double, C.c_double, _dp]
         L.ora_expE1.argtypes = [C.c_double, C.c_double, _dp]
         L.ora_set_threads.argtypes = [C.c_int]
+        L.ora_radiation_bc.argtypes = [C.c_void_p, _dp, C.c_double, _dp]
+        L.ora_airy_bc.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, _dp]
+        L.ora_airy_pressure.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, _dp]
+        L.ora_integrate_pressure.argtypes = [C.c_void_p, _dp, _dp, _dp]
         for name in ("ora_assemble", "ora_assemble_rows", "ora_assemble_rankine", "ora_assemble_wave",
                      "ora_hydro_coeffs", "ora_hydro_coeffs_lid", "ora_zgetrf", "ora_zgetrs", "ora_solve",
                      "ora_assemble_fixed_normal", "ora_hydro_coeffs_fs"):
@@ -322,3 +326,61 @@ def hydro_coeffs(mesh, dofs, omegas, betas, rho=1000.0, g=9.81, gf=GF_WU, direct
                                           int(direct), int(mode), _p(A), _p(B), _p(Fd), _p(Fk))
     return dict(added_mass=A, radiation_damping=B, diffraction_force=Fd[:, :, :nb],
                 Froude_Krylov_force=Fk[:, :, :nb], excitation_force=(Fd + Fk)[:, :, :nb], info=info)
+
+
+def hydro_coeffs_lapack(mesh, dofs, omegas, betas, rho=1000.0, g=9.81, gf=GF_WU, direct=True):
+    """hydro_coeffs (mode 0, no lid) with the dense solve done by LAPACK (numpy's OpenBLAS,
+    multi-threaded) instead of the oracle's own single-threaded LU, for orders where that LU is
+    slow (n >= 1024).  The same steps as hydro_coeffs_impl in mh_oracle.c, one frequency at a time:
+    ora_assemble, ora_radiation_bc / ora_airy_bc, phi = D^-1 (S bc) (direct) or phi = S (D^-1 bc)
+    (indirect) (matrix_assembly.jl:436-449), p = i rho omega phi (solve.jl:50),
+    ora_integrate_pressure, A = Re F / omega^2, B = Im F / omega (problems_and_results.jl:207-208),
+    Froude-Krylov from ora_airy_pressure (waves.jl:294-303)."""
+    import scipy.linalg as sl
+    n = mesh.nf
+    ndof = len(dofs)
+    dd = [np.asfortranarray(np.asarray(m, dtype=np.float64).reshape(n, 3)) for m in dofs]  # n x 3 column-major
+    om = np.asarray(omegas, dtype=np.float64)
+    be = np.asarray(betas, dtype=np.float64)
+    nw, nb = len(om), len(be)
+    R = ndof + nb
+    A = np.zeros((ndof, ndof, nw), order="F"); B = np.zeros((ndof, ndof, nw), order="F")
+    Fd = np.zeros((ndof, nw, nb), dtype=np.complex128, order="F")
+    Fk = np.zeros((ndof, nw, nb), dtype=np.complex128, order="F")
+    L = lib()
+    ms = C.byref(mesh.s)
+    F2 = np.zeros(2)
+    for iw, w in enumerate(om):
+        k = w * w / g  # compute_wavenumber (waves.jl:133-134)
+        bc = np.zeros((n, R), dtype=np.complex128, order="F")
+        col = np.zeros(n, dtype=np.complex128)
+        for r in range(ndof):
+            L.ora_radiation_bc(ms, _p(dd[r]), C.c_double(w), _p(col))
+            bc[:, r] = col
+        for ib in range(nb):
+            L.ora_airy_bc(ms, C.c_double(w), C.c_double(k), C.c_double(be[ib]), C.c_double(g), _p(col))
+            bc[:, ndof + ib] = col
+        S, D = assemble(mesh, k, gf=gf, direct=direct)
+        lu = sl.lu_factor(D, check_finite=False)
+        if direct:
+            phi = sl.lu_solve(lu, S @ bc, check_finite=False)
+        else:
+            phi = S @ sl.lu_solve(lu, bc, check_finite=False)
+        for r in range(R):
+            p = np.ascontiguousarray(1j * rho * w * phi[:, r])  # solve.jl:50
+            for d in range(ndof):
+                L.ora_integrate_pressure(ms, _p(dd[d]), _p(p), _p(F2))
+                if r < ndof:
+                    A[d, r, iw] = F2[0] / (w * w)
+                    B[d, r, iw] = F2[1] / w
+                else:
+                    Fd[d, iw, r - ndof] = F2[0] + 1j * F2[1]
+        for ib in range(nb):
+            p = np.zeros(n, dtype=np.complex128)
+            L.ora_airy_pressure(ms, C.c_double(w), C.c_double(k), C.c_double(be[ib]), C.c_double(g), C.c_double(rho),
+                                _p(p))
+            for d in range(ndof):
+                L.ora_integrate_pressure(ms, _p(dd[d]), _p(p), _p(F2))
+                Fk[d, iw, ib] = F2[0] + 1j * F2[1]
+    return dict(added_mass=A, radiation_damping=B, diffraction_force=Fd, Froude_Krylov_force=Fk,
+                excitation_force=Fd + Fk)

@@ -120,3 +120,31 @@ def test_config3_seeded_uneven_workload(mhx):
         _check(res[q], ref, 1e-10)
         _check(res_g[q], res[q], 1e-12)
         _check(res_g2[q], res[q], 1e-12)
+
+
+def test_graph_replay_after_constant_and_buffer_changes(mhx):
+    """A captured sweep is not replayed with stale state: (i) rho and g changed between two graph
+    calls (the force and boundary-condition kernels carry them by value) give the eager sweep's
+    coefficients at the new constants; (ii) a single-body call with 10 headings between two graph
+    calls reallocates that body's work buffers (more headings than the buffers hold), and the next
+    graph call equals the eager sweep."""
+    print("multibody graph: constants and buffer generations", flush=True)
+    jobs = _small_jobs(mhx)
+    form = mhx.DirectBEM()
+    try:
+        mhx.hydrodynamic_coefficients_many(jobs, form, graph=True)
+        mhx.set_rho(1000.0)
+        mhx.set_g(9.80665)
+        got = mhx.hydrodynamic_coefficients_many(jobs, form, graph=True)
+        ref = mhx.hydrodynamic_coefficients_many(jobs, form)
+        for g, r in zip(got, ref):
+            _check(g, r, 1e-12)
+        body, par = jobs[2]
+        mhx.hydrodynamic_coefficients(body, dict(par, wave_directions=list(np.linspace(0.0, 3.0, 10))), form)
+        got = mhx.hydrodynamic_coefficients_many(jobs, form, graph=True)
+        ref = mhx.hydrodynamic_coefficients_many(jobs, form)
+        for g, r in zip(got, ref):
+            _check(g, r, 1e-12)
+    finally:
+        mhx.set_rho(1025.0)
+        mhx.set_g(9.81)

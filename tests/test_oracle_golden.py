@@ -249,6 +249,21 @@ def test_reference_semantics_equal_per_omega(oracle):
         np.testing.assert_allclose(a[key], b[key], rtol=1e-13, atol=1e-9)
 
 
+@pytest.mark.parametrize("gf", [0, 1])
+@pytest.mark.parametrize("direct", [True, False])
+def test_lapack_orchestration_equals_oracle(oracle, gf, direct):
+    """hydro_coeffs_lapack (the oracle's assembly, BCs and pressure integration with LAPACK for the
+    dense solve, used for orders >= 1024) equals the C oracle's own hydro_coeffs to rounding."""
+    from mhx import meshes
+    m = meshes.sphere(1.0, 8, 16, immersed=True)
+    om = oracle.Mesh.from_mhx(m)
+    dofs = [oracle.rigid_dof(om, d) for d in range(6)]
+    a = oracle.hydro_coeffs(om, dofs, [0.5, 1.3], [0.0, 0.7], rho=1025.0, gf=gf, direct=direct)
+    b = oracle.hydro_coeffs_lapack(om, dofs, [0.5, 1.3], [0.0, 0.7], rho=1025.0, gf=gf, direct=direct)
+    for key in ("added_mass", "radiation_damping", "diffraction_force", "Froude_Krylov_force"):
+        assert np.linalg.norm(a[key] - b[key]) <= 1e-13 * np.linalg.norm(a[key]), key
+
+
 def test_hulme_hemisphere(oracle):
     """test/consistency_with_analytical_solutions.jl:54-79: Wu on a finer sphere, rtol 0.1."""
     from mhx import meshes
