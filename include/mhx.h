@@ -89,6 +89,18 @@ int mhx_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* D, const doub
  * X = A \ B (trans = 0) or A^H \ B (trans = 2); B overwritten by X.  Returns MHX_ESINGULAR on a zero pivot. */
 int mhx_lu_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* A, int trans, double* B);
 
+/* No counterpart in the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): mhx_lu_solve on device
+ * buffers with no allocation.  A (n x n, column-major complex, device memory of the context's
+ * device) is overwritten by its LU factors (LAPACK pivoting), B (n x nrhs, device) by
+ * X = A \ B (trans = 0) or A^H \ B (trans = 2).  The pivots and the solve workspace live in a
+ * static device array of MHX_NOALLOC_SCRATCH_BYTES per GPU (16 MiB: orders up to 4096 with up to
+ * 16 right-hand sides); a call whose workspace exceeds it returns MHX_EDIM.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (several contexts or host threads) race.
+ * Synchronous; MHX_ESINGULAR on an exactly zero pivot. */
+#define MHX_NOALLOC_SCRATCH_BYTES (16ll << 20)
+int mhx_lu_solve_noalloc(mhx_ctx* ctx, int64_t n, int64_t nrhs, double* A, int trans, double* B);
+
 /* A batch of independent systems of one order (LinearSolve LUFactorization on each matrix, as the
  * reference's solve(problems::Vector) issues one per problem: src/matrix_assembly.jl:411-416,
  * src/solve.jl:75-78): X_q = A_q \ B_q (trans = 0) or A_q^H \ B_q (trans = 2) for q < batch.

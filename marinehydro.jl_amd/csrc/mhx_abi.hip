@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -52,6 +53,9 @@ struct mhx_ctx {
     // multi-body sweeps: the last sweep's state (kept for graph replay) and the tuner's choices
     SweepState* sweep = nullptr;
     std::map<std::string, int> tuned;
+    // this device's copy of the static scratch array of mhx_lu_solve_noalloc (null if its address
+    // could not be taken)
+    unsigned char* scratch = nullptr;
 };
 
 struct mhx_body {
@@ -529,6 +533,34 @@ int body_upload(mhx_ctx* ctx, const mhx_mesh* m, int64_t ndof, const double* dof
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// scratch of mhx_lu_solve_noalloc: a file-scope device array, one copy per GPU (each GPU holds
+// its own copy at its own address until the process ends).  Host code never passes the array's
+// name to a kernel or copy: it uses the address hipGetSymbolAddress returns on each device.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[MHX_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::map<int, unsigned char*> g_scratch;  // device ordinal -> address of `scratch` on it
+
+// once per device, with that device current (called by mhx_create)
+unsigned char* scratch_address(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch.find(device);
+    if (it != g_scratch.end()) return it->second;
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) {
+        (void)hipGetLastError();
+        p = nullptr;
+    }
+    g_scratch[device] = (unsigned char*)p;
+    return (unsigned char*)p;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // multi-body sweep
 // ---------------------------------------------------------------------------
 struct SweepState {
@@ -697,6 +729,7 @@ int mhx_create(mhx_ctx** out, const mhx_options* opt) {
     }
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) == hipSuccess && free_b > 0) c->batch_bytes = 0.6 * (double)free_b;
+    c->scratch = scratch_address(c->device);
     *out = c;
     return MHX_OK;
 }
@@ -830,6 +863,58 @@ int mhx_lu_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* A, int tra
     flush_timing(ctx);
     cleanup();
     if (e != hipSuccess) return fail(ctx, MHX_EDEVICE, std::string("lu_solve: ") + hipGetErrorString(e));
+    if (hinfo) return fail(ctx, MHX_ESINGULAR, "LU: exactly zero pivot at column " + std::to_string(hinfo));
+    return MHX_OK;
+}
+
+// Layout of the scratch for one system of order n with R right-hand sides: ipiv, perm (n ints
+// each), the interchange workspace, info, then (256-byte aligned) the solve workspace.
+static int64_t noalloc_ints(int64_t n) { return 2 * n + mhx::kPermWork + 1; }
+static int64_t noalloc_bytes(int64_t n, int64_t R) {
+    const int64_t head = (noalloc_ints(n) * (int64_t)sizeof(int) + 255) / 256 * 256;
+    return head + mhx::getrs_work((int)n, (int)R) * (int64_t)sizeof(cplx);
+}
+
+int mhx_lu_solve_noalloc(mhx_ctx* ctx, int64_t n, int64_t nrhs, double* A, int trans, double* B) {
+    if (!ctx) return MHX_EINVAL;
+    if (n <= 0 || nrhs <= 0) return fail(ctx, MHX_EDIM, "lu_solve_noalloc: empty system");
+    if (nrhs > 16) return fail(ctx, MHX_EDIM, "lu_solve_noalloc: at most 16 right-hand sides per call");
+    if (n > 4096) return fail(ctx, MHX_EDIM, "lu_solve_noalloc: order above 4096 is not supported");
+    if (!A || !B) return fail(ctx, MHX_EINVAL, "lu_solve_noalloc: null buffer");
+    if (trans != 0 && trans != 2) return fail(ctx, MHX_EINVAL, "lu_solve_noalloc: trans must be 0 or 2");
+    const int64_t need = noalloc_bytes(n, nrhs);
+    if (need > (int64_t)MHX_NOALLOC_SCRATCH_BYTES)
+        return fail(ctx, MHX_EDIM, "lu_solve_noalloc: the workspace of order " + std::to_string(n) + " with " +
+                                       std::to_string(nrhs) + " right-hand sides (" + std::to_string(need) +
+                                       " bytes) exceeds the " + std::to_string((long long)MHX_NOALLOC_SCRATCH_BYTES) +
+                                       "-byte scratch");
+    if (!ctx->scratch) return fail(ctx, MHX_EDEVICE, "lu_solve_noalloc: scratch address unavailable on this device");
+    hipSetDevice(ctx->device);
+    for (const void* p : {(const void*)A, (const void*)B}) {
+        hipPointerAttribute_t at;
+        if (hipPointerGetAttributes(&at, p) != hipSuccess ||
+            (at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged) || at.device != ctx->device) {
+            (void)hipGetLastError();
+            return fail(ctx, MHX_EINVAL, "lu_solve_noalloc: A and B must be device memory of the context's device");
+        }
+    }
+    hipStream_t st = ctx->stream;
+    int* ipiv = reinterpret_cast<int*>(ctx->scratch);
+    int* perm = ipiv + n;
+    int* wperm = perm + n;
+    int* info = wperm + mhx::kPermWork;
+    cplx* W = reinterpret_cast<cplx*>(ctx->scratch + (noalloc_ints(n) * (int64_t)sizeof(int) + 255) / 256 * 256);
+    cplx* dA = reinterpret_cast<cplx*>(A);
+    cplx* dB = reinterpret_cast<cplx*>(B);
+    mhx::launch_zero(st, info, sizeof(int));  // a fill kernel, not a memset node
+    if (mhx::getrf_batched(st, (int)n, dA, (int)n, n * n, 1, ipiv, n, wperm, info, nullptr))
+        return fail(ctx, MHX_EDIM, "lu_solve_noalloc: order not supported by the factorization");
+    mhx::getrs_batched(st, trans, (int)n, (int)nrhs, dA, (int)n, 0, ipiv, n, perm, n, dB, 0, W, 0, 1);
+    int hinfo = 0;
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return fail(ctx, MHX_EDEVICE, std::string("lu_solve_noalloc: ") + hipGetErrorString(e));
     if (hinfo) return fail(ctx, MHX_ESINGULAR, "LU: exactly zero pivot at column " + std::to_string(hinfo));
     return MHX_OK;
 }

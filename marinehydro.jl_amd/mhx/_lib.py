@@ -26,13 +26,14 @@ GF_SURROGATE = 2
 MODE_PER_OMEGA = 0
 MODE_PER_PROBLEM = 1
 MODE_BATCHED = 2
+NOALLOC_SCRATCH_BYTES = 16 << 20  # MHX_NOALLOC_SCRATCH_BYTES
 LU_AUTO = 0
 LU_FP64 = 1
 LU_MIXED = 2
 LU_MIXED_BF16 = 3
 
 EXPORTED = ["mhx_create", "mhx_destroy", "mhx_last_error", "mhx_set_constants", "mhx_assemble", "mhx_solve",
-            "mhx_lu_solve", "mhx_lu_solve_batched", "mhx_hydro_coeffs", "mhx_body_create", "mhx_body_destroy", "mhx_body_coeffs",
+            "mhx_lu_solve", "mhx_lu_solve_batched", "mhx_lu_solve_noalloc", "mhx_hydro_coeffs", "mhx_body_create", "mhx_body_destroy", "mhx_body_coeffs",
             "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work", "mhx_set_batch", "mhx_surrogate_train",
             "mhx_surrogate_save", "mhx_surrogate_load", "mhx_surrogate_params", "mhx_surrogate_eval",
             "mhx_surrogate_loss_grad", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_hydro_coeffs_jvp",
@@ -102,6 +103,7 @@ def lib():
         L.mhx_solve.argtypes = [vp, C.c_int64, C.c_int64, _dp, _dp, _dp, C.c_int, _dp, _dp]
         L.mhx_lu_solve.argtypes = [vp, C.c_int64, C.c_int64, _dp, C.c_int, _dp]
         L.mhx_lu_solve_batched.argtypes = [vp, C.c_int64, C.c_int64, C.c_int64, _dp, C.c_int, _dp]
+        L.mhx_lu_solve_noalloc.argtypes = [vp, C.c_int64, C.c_int64, C.c_void_p, C.c_int, C.c_void_p]
         L.mhx_hydro_coeffs.argtypes = [vp, C.POINTER(CMesh), C.c_int64, _dp, C.c_int64, _dp, C.c_int64, _dp,
                                        C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp]
         L.mhx_hydro_coeffs_jvp.argtypes = [vp, C.POINTER(CMesh), C.POINTER(CMeshTangent), C.c_int64, _dp, _dp,
