@@ -318,6 +318,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU comparison runs")
     ap.add_argument("--lu-mode", default="auto", choices=["auto", "fp64", "mixed", "mixed_bf16"],
                     help="dense solver precision (auto: mixed FP32 factor for the surrogate, FP64 otherwise)")
+    ap.add_argument("--freqs", type=int, default=64,
+                    help="frequencies per step, spread over the 64 of the workload (fewer only for profiling)")
     ap.add_argument("--workload", default="headline", choices=["headline", "forward_speed"],
                     help="forward_speed: the forward-speed grid on the same body (IndirectBEM, exact or Wu term)")
     args = ap.parse_args()
@@ -342,6 +344,8 @@ def main():
     if args.workload == "forward_speed":
         return bench_forward_speed(args, mhx, _lib, ctx, body, names, mesh, omegas)
     form = mhx.DirectBEM(args.gf)
+    if args.freqs != len(omegas):
+        omegas = omegas[np.linspace(0, len(omegas) - 1, max(1, args.freqs)).round().astype(int)]
     nw = len(omegas)
     mode = {"batched": _lib.MODE_BATCHED, "per_omega": _lib.MODE_PER_OMEGA}[args.schedule]
     ctx.set_batch(nw)

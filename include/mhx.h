@@ -94,8 +94,10 @@ int mhx_lu_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* A, int tra
  * buffers with no allocation.  A (n x n, column-major complex, device memory of the context's
  * device) is overwritten by its LU factors (LAPACK pivoting), B (n x nrhs, device) by
  * X = A \ B (trans = 0) or A^H \ B (trans = 2).  The pivots and the solve workspace live in a
- * static device array of MHX_NOALLOC_SCRATCH_BYTES per GPU (16 MiB: orders up to 4096 with up to
- * 16 right-hand sides); a call whose workspace exceeds it returns MHX_EDIM.  Every call on a GPU
+ * static device array of MHX_NOALLOC_SCRATCH_BYTES per GPU (16 MiB: ipiv and the triangular-solve
+ * workspace, 2 n R + 128 ceil(n / 64) 64 complex values, e.g. any order up to 4096 with 16
+ * right-hand sides, 6000 with 16); a call whose workspace exceeds it returns MHX_EDIM.  Pivoting
+ * is always MHX_PIVOT_ALWAYS (the original matrix is not kept).  Every call on a GPU
  * shares that GPU's array: calls that overlap on one GPU (several contexts or host threads) race.
  * Synchronous; MHX_ESINGULAR on an exactly zero pivot. */
 #define MHX_NOALLOC_SCRATCH_BYTES (16ll << 20)
@@ -172,6 +174,21 @@ int mhx_set_batch(mhx_ctx* ctx, int64_t max_items, double max_bytes);
 #define MHX_LU_MIXED_BF16 3
 int mhx_set_lu_mode(mhx_ctx* ctx, int mode);
 int mhx_solver_stats(mhx_ctx* ctx, int64_t counts[3], double* max_residual);
+
+/* Pivoting of the dense LU (LAPACK's partial pivoting either way, src/matrix_assembly.jl:411-416):
+ *   MHX_PIVOT_VERIFIED (default)  factor with the diagonal pivots while checking LAPACK's rule on
+ *                 every candidate (the diagonal is a largest |Re| + |Im| of its column after the
+ *                 earlier eliminations, first index on ties); a matrix that breaks it anywhere, or
+ *                 meets an exactly zero pivot, is factored again from its original values with row
+ *                 interchanges.  The influence matrices of the BEM keep their diagonal pivots, so
+ *                 the panel work needs no pivot search;
+ *   MHX_PIVOT_ALWAYS  row interchanges with the pivot search on every column.
+ * mhx_pivot_stats: counts[0] factorizations that kept the verified diagonal pivots, counts[1]
+ * factorizations with the pivot search (since mhx_reset_timing). */
+#define MHX_PIVOT_VERIFIED 0
+#define MHX_PIVOT_ALWAYS 1
+int mhx_set_pivoting(mhx_ctx* ctx, int mode);
+int mhx_pivot_stats(mhx_ctx* ctx, int64_t counts[2]);
 
 /* per-phase timing (HIP events on the context's stream, enabled by mhx_options.timing):
  * phase: "rankine", "wave_table" (wave term: exact table, or the fused surrogate pair kernel),

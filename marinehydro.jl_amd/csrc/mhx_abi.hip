@@ -46,8 +46,10 @@ struct mhx_ctx {
     unsigned short* surr_W = nullptr;
     double surr_val_loss = -1.0;
     int64_t surr_steps = 0;
-    // dense solver: precision policy and mixed-precision statistics
+    // dense solver: precision and pivoting policies, mixed-precision statistics
     int lu_mode = MHX_LU_AUTO;
+    int pivoting = MHX_PIVOT_VERIFIED;
+    int64_t lu_diagonal = 0, lu_interchanged = 0;  // factorizations that kept / did not keep the diagonal pivots
     int64_t refine_steps = 0, mixed_solves = 0, fp64_fallbacks = 0;
     double max_residual = 0.0;
     // multi-body sweeps: the last sweep's state (kept for graph replay) and the tuner's choices
@@ -72,6 +74,7 @@ struct mhx_body {
     bool cap_table = false;
     cplx *S = nullptr, *D = nullptr, *WG = nullptr, *WR = nullptr;
     cplx *B = nullptr, *Y = nullptr, *work = nullptr, *F = nullptr, *FK = nullptr;
+    cplx* D0 = nullptr;  // copy of D kept for the pivoted refactorization of a flagged item (FP64 path)
     double *betas = nullptr, *d_ws = nullptr, *d_ks = nullptr;
     int *ipiv = nullptr, *perm = nullptr, *wperm = nullptr, *info = nullptr;
     unsigned long long* status = nullptr;
@@ -191,7 +194,7 @@ void free_work(mhx_body* b) {
     dfree(b->S); dfree(b->D); dfree(b->WG); dfree(b->WR); dfree(b->B); dfree(b->Y); dfree(b->work);
     dfree(b->F); dfree(b->FK); dfree(b->betas); dfree(b->d_ws); dfree(b->d_ks); dfree(b->ipiv); dfree(b->perm);
     dfree(b->wperm); dfree(b->info); dfree(b->status); dfree(b->fb_list); dfree(b->fb_count);
-    dfree(b->Df); dfree(b->Ps); dfree(b->Rr); dfree(b->Tr); dfree(b->norms);
+    dfree(b->Df); dfree(b->Ps); dfree(b->Rr); dfree(b->Tr); dfree(b->norms); dfree(b->D0);
     bump_gen(b);
     b->fb_cap = 0;
     b->cap_batch = b->cap_R = b->cap_nb = 0;
@@ -240,6 +243,7 @@ double item_bytes(const mhx_ctx* ctx, int n, int gf, bool table, int rhs_cols) {
     double per = (double)n * n * sizeof(cplx) * (table ? 4 : 2) + (double)rhs_cols * n * 16 * sizeof(cplx);
     if (gf == MHX_GF_SURROGATE) per += 8.0 * (double)mhx::surr_fallback_capacity(n, 1);  // fallback list
     if (use_mixed(ctx, gf)) per += (double)n * n * sizeof(cplxf) + 2.0 * n * 16 * sizeof(cplx);  // FP32 factors
+    else if (ctx->pivoting == MHX_PIVOT_VERIFIED) per += (double)n * n * sizeof(cplx);  // copy of D
     if (use_split(ctx, gf)) per += 2.0 * mhx::split_work(n);  // split-bf16 panels
     return per;
 }
@@ -337,6 +341,49 @@ int check_info(mhx_ctx* ctx, mhx_body* b, int batch, bool* singular) {
     return MHX_OK;
 }
 
+// Per-item flags of a verified factorization (nonzero: LAPACK's rule failed somewhere, the item
+// must be factored again with row interchanges); the flag of item q is wperm[kPermWork q].
+int read_pivot_flags(mhx_ctx* ctx, const int* wperm, int batch, std::vector<int>& flags) {
+    flags.assign(batch, 0);
+    HCHK(hipMemcpy2DAsync(flags.data(), sizeof(int), wperm, sizeof(int) * mhx::kPermWork, sizeof(int), batch,
+                          hipMemcpyDeviceToHost, ctx->stream));
+    HCHK(hipStreamSynchronize(ctx->stream));
+    return MHX_OK;
+}
+
+// FP64 LU of D (in place) with the context's pivoting.  Verified diagonal pivots: D is copied to
+// D0 first, and every item whose factorization breaks LAPACK's rule is copied back and factored
+// again with row interchanges (so the pivots are LAPACK's either way).
+int getrf_fp64(mhx_ctx* ctx, mhx_body* b, int batch, mhx::LaunchTimer* lt) {
+    const int n = b->n;
+    hipStream_t st = ctx->stream;
+    if (ctx->pivoting != MHX_PIVOT_VERIFIED) {
+        if (mhx::getrf_batched(st, n, b->D, n, b->nn(), batch, b->ipiv, n, b->wperm, b->info, lt))
+            return fail(ctx, MHX_EDIM, "LU: order above 16384 is not supported");
+        ctx->lu_interchanged += batch;
+        return MHX_OK;
+    }
+    int rc;
+    if ((rc = dalloc(ctx, &b->D0, (size_t)b->nn() * b->cap_batch))) return rc;
+    HCHK(hipMemcpyAsync(b->D0, b->D, sizeof(cplx) * b->nn() * batch, hipMemcpyDeviceToDevice, st));
+    if (mhx::getrf_batched(st, n, b->D, n, b->nn(), batch, b->ipiv, n, b->wperm, b->info, lt, mhx::kPivotVerified,
+                           b->work, b->sW()))
+        return fail(ctx, MHX_EDIM, "LU: order above 16384 is not supported");
+    std::vector<int> fl;
+    if ((rc = read_pivot_flags(ctx, b->wperm, batch, fl))) return rc;
+    for (int q = 0; q < batch; q++) {
+        if (!fl[q]) {
+            ctx->lu_diagonal++;
+            continue;
+        }
+        ctx->lu_interchanged++;
+        HCHK(hipMemcpyAsync(b->D + q * b->nn(), b->D0 + q * b->nn(), sizeof(cplx) * b->nn(), hipMemcpyDeviceToDevice, st));
+        mhx::getrf_batched(st, n, b->D + q * b->nn(), n, b->nn(), 1, b->ipiv + (size_t)q * n, n,
+                           b->wperm + (size_t)q * mhx::kPermWork, b->info + q, lt);
+    }
+    return MHX_OK;
+}
+
 // FP64 LU of D (in place) and the solves (rhs in b->B; solution potentials in b->Y).
 // form_phi = false (indirect only): the system's solution stays in b->B and S is not used;
 // trans = 2 then solves with D^H.
@@ -346,10 +393,9 @@ int solve_fp64(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, bool for
     {
         Phase ph(ctx, "lu");
         mhx::LaunchTimer lt;
-        int rc = mhx::getrf_batched(ctx->stream, n, b->D, n, b->nn(), batch, b->ipiv, n, b->wperm, b->info,
-                                    ctx->timing ? &lt : nullptr);
+        int rc = getrf_fp64(ctx, b, batch, ctx->timing ? &lt : nullptr);
         take_launches(ctx, lt);
-        if (rc) return fail(ctx, MHX_EDIM, "LU: order above 4096 is not supported");
+        if (rc) return rc;
     }
     HCHK(hipGetLastError());
     {
@@ -386,7 +432,8 @@ constexpr int kRefineMaxSplit = 8;  // split-bf16 trailing updates (factors to ~
 // as two halves on two streams: the panel phases (launches of one workgroup per item,
 // latency-bound) of one half can then run beside the trailing updates of the other, on compute
 // units the panels leave idle.  The halves share nothing but the device.
-int getrf_f32_two_streams(mhx_ctx* ctx, mhx_body* b, int batch, mhx::LaunchTimer* lt, unsigned short* split) {
+int getrf_f32_two_streams(mhx_ctx* ctx, mhx_body* b, int batch, mhx::LaunchTimer* lt, unsigned short* split,
+                          int pivot) {
     const int n = b->n;
     const int h = (batch >= 8 && n >= 512) ? batch / 2 : 0;  // items on the side stream
     const int m = batch - h;
@@ -396,11 +443,13 @@ int getrf_f32_two_streams(mhx_ctx* ctx, mhx_body* b, int batch, mhx::LaunchTimer
         HCHK(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
         rc = mhx::getrf_batched(ctx->side, n, b->Df + (size_t)m * b->nn(), n, b->nn(), h, b->ipiv + (size_t)m * n, n,
                                 b->wperm + (size_t)m * mhx::kPermWork, b->info + m, lt,
-                                split ? split + (size_t)m * mhx::split_work(n) : nullptr);
+                                split ? split + (size_t)m * mhx::split_work(n) : nullptr, pivot,
+                                reinterpret_cast<cplxf*>(b->work + (size_t)m * b->sW()), 2 * b->sW());
         HCHK(hipEventRecord(ctx->join, ctx->side));
     }
     if (!rc)
-        rc = mhx::getrf_batched(ctx->stream, n, b->Df, n, b->nn(), m, b->ipiv, n, b->wperm, b->info, lt, split);
+        rc = mhx::getrf_batched(ctx->stream, n, b->Df, n, b->nn(), m, b->ipiv, n, b->wperm, b->info, lt, split, pivot,
+                                reinterpret_cast<cplxf*>(b->work), 2 * b->sW());
     if (h) HCHK(hipStreamWaitEvent(ctx->stream, ctx->join, 0));
     return rc;
 }
@@ -422,9 +471,33 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf,
         if (!b->df_ready) mhx::convert_to_f32(st, b->nn() * batch, b->D, b->Df);
         b->df_ready = false;
         mhx::LaunchTimer lt;
-        rc = getrf_f32_two_streams(ctx, b, batch, ctx->timing ? &lt : nullptr, split ? b->Ps : nullptr);
+        const bool verified = ctx->pivoting == MHX_PIVOT_VERIFIED;
+        rc = getrf_f32_two_streams(ctx, b, batch, ctx->timing ? &lt : nullptr, split ? b->Ps : nullptr,
+                                   verified ? mhx::kPivotVerified : mhx::kPivotStrict);
+        if (rc) {
+            take_launches(ctx, lt);
+            return fail(ctx, MHX_EDIM, "LU: order above 16384 is not supported");
+        }
+        // verified diagonal pivots: items that broke LAPACK's rule are converted again from the
+        // FP64 D (intact) and factored with row interchanges
+        if (verified) {
+            std::vector<int> fl;
+            if ((rc = read_pivot_flags(ctx, b->wperm, batch, fl))) return rc;
+            for (int q = 0; q < batch; q++) {
+                if (!fl[q]) {
+                    ctx->lu_diagonal++;
+                    continue;
+                }
+                ctx->lu_interchanged++;
+                mhx::convert_to_f32(st, b->nn(), b->D + q * b->nn(), b->Df + q * b->nn());
+                mhx::getrf_batched(st, n, b->Df + q * b->nn(), n, b->nn(), 1, b->ipiv + (size_t)q * n, n,
+                                   b->wperm + (size_t)q * mhx::kPermWork, b->info + q, ctx->timing ? &lt : nullptr,
+                                   split ? b->Ps + (size_t)q * mhx::split_work(n) : nullptr);
+            }
+        } else {
+            ctx->lu_interchanged += batch;
+        }
         take_launches(ctx, lt);
-        if (rc) return fail(ctx, MHX_EDIM, "LU: order above 4096 is not supported");
     }
     HCHK(hipGetLastError());
     bool singular = false;
@@ -581,6 +654,9 @@ struct SweepState {
     unsigned long long* counters = nullptr;  // table-queue counters, one per chunk
     unsigned long long* status = nullptr;    // quadrature status and work tally of the sweep
     int* sticky = nullptr;                   // first zero pivot per body
+    bool verified = false;                   // LU with verified diagonal pivots
+    int* pflags = nullptr;                   // their per-item flags: body ib, frequency w at flag_off[ib] + w
+    std::vector<int64_t> flag_off;
     int rows = 256;                          // tuner's choice of rows per combine workgroup
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -597,6 +673,7 @@ struct SweepState {
         dfree(counters);
         dfree(status);
         dfree(sticky);
+        dfree(pflags);
     }
 };
 
@@ -650,7 +727,16 @@ void sweep_enqueue(mhx_ctx* ctx, SweepState& s) {
             if (!bt) continue;
             const int n = b->n, R = b->ndof + (int)s.nb;
             mhx::launch_zero(st, b->info, sizeof(int) * bt);
-            mhx::getrf_batched(st, n, b->D, n, b->nn(), bt, b->ipiv, n, b->wperm, b->info, nullptr);
+            if (s.verified) {
+                // the copy of D for a pivoted redo of flagged items (after the sweep), then the
+                // factorization with verified diagonal pivots and its flags
+                mhx::launch_copy(st, b->D, b->D0, sizeof(cplx) * b->nn() * bt);
+                mhx::getrf_batched(st, n, b->D, n, b->nn(), bt, b->ipiv, n, b->wperm, b->info, nullptr,
+                                   mhx::kPivotVerified, b->work, b->sW());
+                mhx::launch_gather_pivot_flags(st, b->wperm, bt, s.pflags + s.flag_off[ib] + w0);
+            } else {
+                mhx::getrf_batched(st, n, b->D, n, b->nn(), bt, b->ipiv, n, b->wperm, b->info, nullptr);
+            }
             mhx::launch_sticky_info(st, b->info, bt, s.sticky + ib);
             cplx* W2 = b->work + 16 * b->nR();
             if (s.direct) {
@@ -825,46 +911,17 @@ int mhx_assemble(mhx_ctx* ctx, const mhx_mesh* mesh, double k, int gf, int direc
     return rc;
 }
 
+static int lu_solve_host(mhx_ctx* ctx, int64_t n, int64_t nrhs, int64_t batch, const double* A, int trans,
+                         double* Bh, bool fp64_only);
+
 int mhx_lu_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* A, int trans, double* Bh) {
     if (!ctx) return MHX_EINVAL;
     if (n <= 0 || nrhs <= 0) return fail(ctx, MHX_EDIM, "lu_solve: empty system");
     if (nrhs > 16) return fail(ctx, MHX_EDIM, "lu_solve: at most 16 right-hand sides per call");
-    if (n > 4096) return fail(ctx, MHX_EDIM, "lu_solve: order above 4096 is not supported");
+    if (n > mhx::kMaxOrder) return fail(ctx, MHX_EDIM, "lu_solve: order above 16384 is not supported");
     if (!A || !Bh) return fail(ctx, MHX_EINVAL, "lu_solve: null buffer");
     if (trans != 0 && trans != 2) return fail(ctx, MHX_EINVAL, "lu_solve: trans must be 0 or 2");
-    hipSetDevice(ctx->device);
-    cplx *dA = nullptr, *dB = nullptr, *dW = nullptr;
-    int *ipiv = nullptr, *perm = nullptr, *wperm = nullptr, *info = nullptr;
-    int rc;
-    auto cleanup = [&]() { dfree(dA); dfree(dB); dfree(dW); dfree(ipiv); dfree(perm); dfree(wperm); dfree(info); };
-    if ((rc = dalloc(ctx, &dA, (size_t)n * n)) || (rc = dalloc(ctx, &dB, (size_t)n * nrhs)) ||
-        (rc = dalloc(ctx, &dW, (size_t)mhx::getrs_work((int)n, (int)nrhs))) || (rc = dalloc(ctx, &ipiv, n)) || (rc = dalloc(ctx, &perm, n)) ||
-        (rc = dalloc(ctx, &wperm, (size_t)mhx::kPermWork)) || (rc = dalloc(ctx, &info, 1))) {
-        cleanup();
-        return rc;
-    }
-    hipStream_t st = ctx->stream;
-    hipMemcpyAsync(dA, A, sizeof(cplx) * n * n, hipMemcpyHostToDevice, st);
-    hipMemcpyAsync(dB, Bh, sizeof(cplx) * n * nrhs, hipMemcpyHostToDevice, st);
-    hipMemsetAsync(info, 0, sizeof(int), st);
-    {
-        Phase ph(ctx, "lu");
-        mhx::getrf_batched(st, (int)n, dA, (int)n, n * n, 1, ipiv, n, wperm, info);
-    }
-    {
-        Phase ph(ctx, "solve");
-        mhx::getrs_batched(st, trans, (int)n, (int)nrhs, dA, (int)n, 0, ipiv, n, perm, n, dB, 0, dW, 0, 1);
-    }
-    int hinfo = 0;
-    hipMemcpyAsync(&hinfo, info, sizeof(int), hipMemcpyDeviceToHost, st);
-    hipMemcpyAsync(Bh, dB, sizeof(cplx) * n * nrhs, hipMemcpyDeviceToHost, st);
-    hipError_t e = hipStreamSynchronize(st);
-    if (e == hipSuccess) e = hipGetLastError();
-    flush_timing(ctx);
-    cleanup();
-    if (e != hipSuccess) return fail(ctx, MHX_EDEVICE, std::string("lu_solve: ") + hipGetErrorString(e));
-    if (hinfo) return fail(ctx, MHX_ESINGULAR, "LU: exactly zero pivot at column " + std::to_string(hinfo));
-    return MHX_OK;
+    return lu_solve_host(ctx, n, nrhs, 1, A, trans, Bh, true);
 }
 
 // Layout of the scratch for one system of order n with R right-hand sides: ipiv, perm (n ints
@@ -879,7 +936,7 @@ int mhx_lu_solve_noalloc(mhx_ctx* ctx, int64_t n, int64_t nrhs, double* A, int t
     if (!ctx) return MHX_EINVAL;
     if (n <= 0 || nrhs <= 0) return fail(ctx, MHX_EDIM, "lu_solve_noalloc: empty system");
     if (nrhs > 16) return fail(ctx, MHX_EDIM, "lu_solve_noalloc: at most 16 right-hand sides per call");
-    if (n > 4096) return fail(ctx, MHX_EDIM, "lu_solve_noalloc: order above 4096 is not supported");
+    if (n > mhx::kMaxOrder) return fail(ctx, MHX_EDIM, "lu_solve_noalloc: order above 16384 is not supported");
     if (!A || !B) return fail(ctx, MHX_EINVAL, "lu_solve_noalloc: null buffer");
     if (trans != 0 && trans != 2) return fail(ctx, MHX_EINVAL, "lu_solve_noalloc: trans must be 0 or 2");
     const int64_t need = noalloc_bytes(n, nrhs);
@@ -923,10 +980,17 @@ int mhx_lu_solve_batched(mhx_ctx* ctx, int64_t n, int64_t nrhs, int64_t batch, c
     if (!ctx) return MHX_EINVAL;
     if (n <= 0 || nrhs <= 0 || batch <= 0) return fail(ctx, MHX_EDIM, "lu_solve_batched: empty system or batch");
     if (nrhs > 16) return fail(ctx, MHX_EDIM, "lu_solve_batched: at most 16 right-hand sides per system");
-    if (n > 4096) return fail(ctx, MHX_EDIM, "lu_solve_batched: order above 4096 is not supported");
+    if (n > mhx::kMaxOrder) return fail(ctx, MHX_EDIM, "lu_solve_batched: order above 16384 is not supported");
     if (batch > (1 << 16)) return fail(ctx, MHX_EDIM, "lu_solve_batched: batch too large");
     if (!A || !Bh) return fail(ctx, MHX_EINVAL, "lu_solve_batched: null buffer");
     if (trans != 0 && trans != 2) return fail(ctx, MHX_EINVAL, "lu_solve_batched: trans must be 0 or 2");
+    return lu_solve_host(ctx, n, nrhs, batch, A, trans, Bh, false);
+}
+
+// systems from host buffers through a temporary body: FP64 (fp64_only, or the context's LU mode
+// says so, or trans = 2) or the mixed-precision solver
+static int lu_solve_host(mhx_ctx* ctx, int64_t n, int64_t nrhs, int64_t batch, const double* A, int trans,
+                         double* Bh, bool fp64_only) {
     hipSetDevice(ctx->device);
     mhx_body b;
     b.ctx = ctx;
@@ -942,7 +1006,8 @@ int mhx_lu_solve_batched(mhx_ctx* ctx, int64_t n, int64_t nrhs, int64_t batch, c
                 rc = fail(ctx, MHX_EDEVICE, "lu_solve_batched: upload");
         }
         // the mixed-precision solvers take A x = B only; A^H x = B is solved in FP64
-        const bool mixed = trans == 0 && (ctx->lu_mode == MHX_LU_MIXED || ctx->lu_mode == MHX_LU_MIXED_BF16);
+        const bool mixed =
+            !fp64_only && trans == 0 && (ctx->lu_mode == MHX_LU_MIXED || ctx->lu_mode == MHX_LU_MIXED_BF16);
         if (!rc)
             rc = mixed ? solve_mixed(ctx, &b, (int)nrhs, 0, (int)batch,
                                      ctx->lu_mode == MHX_LU_MIXED_BF16 ? MHX_GF_SURROGATE : MHX_GF_WU, false)
@@ -963,7 +1028,7 @@ int mhx_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* D, const doub
     if (!ctx) return MHX_EINVAL;
     if (n <= 0 || nrhs <= 0) return fail(ctx, MHX_EDIM, "solve: empty system");
     if (nrhs > 16) return fail(ctx, MHX_EDIM, "solve: at most 16 right-hand sides per call");
-    if (n > 4096) return fail(ctx, MHX_EDIM, "solve: order above 4096 is not supported");
+    if (n > mhx::kMaxOrder) return fail(ctx, MHX_EDIM, "solve: order above 16384 is not supported");
     if (!D || !S || !bc || !phi_out) return fail(ctx, MHX_EINVAL, "solve: null buffer");
     hipSetDevice(ctx->device);
     mhx_body b;
@@ -1024,7 +1089,7 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
     const int R = ndof + (int)nb;
     if (R <= 0) return fail(ctx, MHX_EDIM, "coeffs: no radiating dof and no heading");
     if (R > 16) return fail(ctx, MHX_EDIM, "coeffs: ndof + nbeta must be at most 16");
-    if (n > 4096) return fail(ctx, MHX_EDIM, "coeffs: more than 4096 panels is not supported");
+    if (n > mhx::kMaxOrder) return fail(ctx, MHX_EDIM, "coeffs: more than 16384 panels is not supported");
     if ((ndof && (!A_out || !B_out)) || (nb && ndof && (!Fd_out || !Fk_out)))
         return fail(ctx, MHX_EINVAL, "coeffs: null output");
     for (int64_t w = 0; w < nw; w++)
@@ -1121,7 +1186,7 @@ int mhx_body_coeffs_fs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omeg
         if (!dgrad) return fail(ctx, MHX_EINVAL, "nonzero forward speed: null gradient of motion");
     }
     if (ndof + 1 > 16) return fail(ctx, MHX_EDIM, "forward speed: at most 15 dofs");
-    if (n > 4096) return fail(ctx, MHX_EDIM, "coeffs: more than 4096 panels is not supported");
+    if (n > mhx::kMaxOrder) return fail(ctx, MHX_EDIM, "coeffs: more than 16384 panels is not supported");
     hipSetDevice(ctx->device);
     const size_t nd = ndof, nd2 = nd * nd;
     auto aidx = [&](int d, int r, int64_t iw, int64_t iu, int64_t ib) {
@@ -1291,7 +1356,7 @@ int mhx_hydro_coeffs_jvp(mhx_ctx* ctx, const mhx_mesh* mesh, const mhx_mesh_tang
     const int64_t R64 = ndof + nb;
     if (R64 <= 0) return fail(ctx, MHX_EDIM, "coeffs_jvp: no radiating dof and no heading");
     if (R64 > 16) return fail(ctx, MHX_EDIM, "coeffs_jvp: at most 16 right-hand sides (dofs + headings)");
-    if (mesh && mesh->nf > 4096) return fail(ctx, MHX_EDIM, "coeffs_jvp: more than 4096 panels is not supported");
+    if (mesh && mesh->nf > mhx::kMaxOrder) return fail(ctx, MHX_EDIM, "coeffs_jvp: more than 16384 panels is not supported");
     hipSetDevice(ctx->device);
     mhx_body* b = nullptr;
     int rc = body_upload(ctx, mesh, ndof, dofs, &b);
@@ -1453,7 +1518,7 @@ int mhx_hydro_coeffs_vjp(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const
     const int64_t R64 = ndof + nb;
     if (R64 <= 0) return fail(ctx, MHX_EDIM, "coeffs_vjp: no radiating dof and no heading");
     if (R64 > 16) return fail(ctx, MHX_EDIM, "coeffs_vjp: at most 16 right-hand sides (dofs + headings)");
-    if (mesh && mesh->nf > 4096) return fail(ctx, MHX_EDIM, "coeffs_vjp: more than 4096 panels is not supported");
+    if (mesh && mesh->nf > mhx::kMaxOrder) return fail(ctx, MHX_EDIM, "coeffs_vjp: more than 16384 panels is not supported");
     hipSetDevice(ctx->device);
     mhx_body* b = nullptr;
     int rc = body_upload(ctx, mesh, ndof, dofs, &b);
@@ -1612,6 +1677,7 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
     if (flags & ~MHX_SWEEP_GRAPH) return fail(ctx, MHX_EINVAL, "sweep: unknown flag");
     direct = direct ? 1 : 0;
     const bool table = gf == MHX_GF_DELHOMMEAU, graph = (flags & MHX_SWEEP_GRAPH) != 0;
+    const bool verified = ctx->pivoting == MHX_PIVOT_VERIFIED;
     for (int64_t ib = 0; ib < nbodies; ib++) {
         const mhx_sweep_body& e = in[ib];
         if (!e.body || e.body->ctx != ctx) return fail(ctx, MHX_EINVAL, "sweep: body is null or from another context");
@@ -1619,7 +1685,7 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
             if (in[jb].body == e.body) return fail(ctx, MHX_EINVAL, "sweep: a body is listed twice");
         const int64_t R = e.body->ndof + nb;
         if (R <= 0 || R > 16) return fail(ctx, MHX_EDIM, "sweep: ndof + nbeta must be in 1..16 for every body");
-        if (e.body->n > 4096) return fail(ctx, MHX_EDIM, "sweep: more than 4096 panels is not supported");
+        if (e.body->n > mhx::kMaxOrder) return fail(ctx, MHX_EDIM, "sweep: more than 16384 panels is not supported");
         if (e.nw < 0 || (e.nw && !e.omegas)) return fail(ctx, MHX_EINVAL, "sweep: bad frequency array");
         for (int64_t w = 0; w < e.nw; w++)
             if (!(e.omegas[w] > 0) || !std::isfinite(e.omegas[w]))
@@ -1634,7 +1700,8 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
     int64_t K = 1;
     for (int64_t ib = 0; ib < nbodies; ib++) {
         const double n = in[ib].body->n;
-        total += (n * n * sizeof(cplx) * (table ? 4 : 2) + 20.0 * n * 16 * sizeof(cplx)) * in[ib].nw;
+        total += (n * n * sizeof(cplx) * ((table ? 4 : 2) + (verified ? 1 : 0)) + 20.0 * n * 16 * sizeof(cplx)) *
+                 in[ib].nw;
         K = std::max<int64_t>(K, (in[ib].nw + ctx->max_batch - 1) / ctx->max_batch);
     }
     K = std::max<int64_t>(K, (int64_t)std::ceil(total / ctx->batch_bytes));
@@ -1644,7 +1711,7 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
         mhx_body* b = in[ib].body;
         per[ib] = (int)((in[ib].nw + K - 1) / K);
         if ((rc = ensure_work(ctx, b, b->ndof + (int)nb, (int)nb, table, std::max(1, per[ib]))) ||
-            (rc = ensure_rankine(ctx, b, direct)))
+            (rc = ensure_rankine(ctx, b, direct)) || (verified && (rc = dalloc(ctx, &b->D0, (size_t)b->nn() * b->cap_batch))))
             return rc;
         if (nb && (rc = hipMemcpyAsync(b->betas, betas, sizeof(double) * nb, hipMemcpyHostToDevice, st)) != hipSuccess)
             return fail(ctx, MHX_EDEVICE, "sweep: heading upload");
@@ -1695,7 +1762,7 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
     SweepState local;
     SweepState* s = graph ? ctx->sweep : &local;
     const bool reuse = graph && s && s->exec && s->bodies == bodies && s->nws == nws && s->gens == gens &&
-                       s->rho == ctx->rho && s->g == ctx->g && s->nb == nb && s->gf == gf &&
+                       s->rho == ctx->rho && s->g == ctx->g && s->verified == verified && s->nb == nb && s->gf == gf &&
                        s->direct == direct && s->K == K && s->h_items.size() == items.size() &&
                        std::memcmp(s->h_items.data(), items.data(), items.size() * sizeof(mhx::SweepItem)) == 0;
     auto abandon = [&]() {
@@ -1712,6 +1779,13 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
         s->gens = gens;
         s->rho = ctx->rho;
         s->g = ctx->g;
+        s->verified = verified;
+        s->flag_off.assign(nbodies, 0);
+        int64_t nflags = 0;
+        for (int64_t ib = 0; ib < nbodies; ib++) {
+            s->flag_off[ib] = nflags;
+            nflags += in[ib].nw;
+        }
         s->per_chunk = per;
         s->K = (int)K;
         s->nb = nb;
@@ -1735,7 +1809,8 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
                 break;
         }
         if (rc || (rc = dalloc(ctx, &s->items, items.size())) || (rc = dalloc(ctx, &s->counters, (size_t)K)) ||
-            (rc = dalloc(ctx, &s->status, 4)) || (rc = dalloc(ctx, &s->sticky, (size_t)nbodies))) {
+            (rc = dalloc(ctx, &s->status, 4)) || (rc = dalloc(ctx, &s->sticky, (size_t)nbodies)) ||
+            (verified && (rc = dalloc(ctx, &s->pflags, (size_t)std::max<int64_t>(1, nflags))))) {
             abandon();
             return rc;
         }
@@ -1819,6 +1894,47 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
                  in[ib].Fk_out);
         }
     }
+    // verified diagonal pivots: every (body, frequency) whose factorization broke LAPACK's rule is
+    // solved again on its own with row interchanges and its coefficients replaced
+    if (!rc && s->verified) {
+        int64_t nflags = 0;
+        for (int64_t ib = 0; ib < nbodies; ib++) nflags += in[ib].nw;
+        std::vector<int> fl((size_t)std::max<int64_t>(1, nflags), 0);
+        if (nflags && (hipMemcpyAsync(fl.data(), s->pflags, sizeof(int) * nflags, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                       hipStreamSynchronize(st) != hipSuccess))
+            rc = fail(ctx, MHX_EDEVICE, "sweep: pivot flags");
+        const int saved = ctx->pivoting;
+        for (int64_t ib = 0; ib < nbodies && !rc; ib++) {
+            mhx_body* b = bodies[ib];
+            const int ndof = b->ndof, nw = (int)in[ib].nw;
+            for (int w = 0; w < nw && !rc; w++) {
+                if (!fl[s->flag_off[ib] + w]) {
+                    ctx->lu_diagonal++;
+                    continue;
+                }
+                if (!ndof) continue;
+                const size_t nd = ndof, nb1 = nb ? nb : 1;
+                std::vector<double> A1(nd * nd), B1(nd * nd), Fd1(2 * nd * nb1), Fk1(2 * nd * nb1);
+                ctx->pivoting = MHX_PIVOT_ALWAYS;
+                rc = mhx_body_coeffs(ctx, b, 1, in[ib].omegas + w, nb, betas, gf, direct, MHX_MODE_PER_OMEGA, A1.data(),
+                                     B1.data(), Fd1.data(), Fk1.data());
+                ctx->pivoting = saved;
+                if (rc) break;
+                for (size_t e = 0; e < nd * nd; e++) {
+                    in[ib].A_out[e + nd * nd * w] = A1[e];
+                    in[ib].B_out[e + nd * nd * w] = B1[e];
+                }
+                for (int64_t bb = 0; bb < nb; bb++)
+                    for (size_t d = 0; d < nd; d++) {
+                        const size_t o = d + nd * (w + (size_t)nw * bb), o1 = d + nd * bb;
+                        in[ib].Fd_out[2 * o] = Fd1[2 * o1];
+                        in[ib].Fd_out[2 * o + 1] = Fd1[2 * o1 + 1];
+                        in[ib].Fk_out[2 * o] = Fk1[2 * o1];
+                        in[ib].Fk_out[2 * o + 1] = Fk1[2 * o1 + 1];
+                    }
+            }
+        }
+    }
     if (!graph || rc) abandon();
     return rc;
 }
@@ -1850,6 +1966,7 @@ void mhx_reset_timing(mhx_ctx* ctx) {
     ctx->phases.clear();
     ctx->quad_work[0] = ctx->quad_work[1] = ctx->quad_work[2] = 0;
     ctx->refine_steps = ctx->mixed_solves = ctx->fp64_fallbacks = 0;
+    ctx->lu_diagonal = ctx->lu_interchanged = 0;
     ctx->max_residual = 0.0;
 }
 
@@ -1858,6 +1975,21 @@ int mhx_set_lu_mode(mhx_ctx* ctx, int mode) {
     if (mode != MHX_LU_AUTO && mode != MHX_LU_FP64 && mode != MHX_LU_MIXED && mode != MHX_LU_MIXED_BF16)
         return fail(ctx, MHX_EINVAL, "lu mode must be MHX_LU_AUTO, MHX_LU_FP64, MHX_LU_MIXED or MHX_LU_MIXED_BF16");
     ctx->lu_mode = mode;
+    return MHX_OK;
+}
+
+int mhx_set_pivoting(mhx_ctx* ctx, int mode) {
+    if (!ctx) return MHX_EINVAL;
+    if (mode != MHX_PIVOT_VERIFIED && mode != MHX_PIVOT_ALWAYS)
+        return fail(ctx, MHX_EINVAL, "pivoting must be MHX_PIVOT_VERIFIED or MHX_PIVOT_ALWAYS");
+    ctx->pivoting = mode;
+    return MHX_OK;
+}
+
+int mhx_pivot_stats(mhx_ctx* ctx, int64_t counts[2]) {
+    if (!ctx || !counts) return MHX_EINVAL;
+    counts[0] = ctx->lu_diagonal;
+    counts[1] = ctx->lu_interchanged;
     return MHX_OK;
 }
 

@@ -85,8 +85,18 @@ struct LaunchTimer {
 };
 void zgemm(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cplx* A, int lda, int64_t sA, const cplx* B,
            int ldb, int64_t sB, cplx* C, int ldc, int64_t sC, int batch);
+// Pivoting of getrf_batched: kPivotStrict (row interchanges, LAPACK's partial pivoting) or
+// kPivotVerified (diagonal pivots with LAPACK's rule checked on every candidate: a matrix that
+// breaks it is flagged in work_perm[kPermWork * b] (nonzero) and must be factored again, from its
+// original values, with kPivotStrict; info is then not set by the verified pass).  The verified
+// pass needs inv: 2 x 64 x 64 elements per matrix (stride inv_stride) for the inverses of each
+// panel's diagonal triangles.
+constexpr int kPivotStrict = 0;
+constexpr int kPivotVerified = 1;
+constexpr int kMaxOrder = 16384;  // largest order of the factorization (leaf register tiles)
 int getrf_batched(hipStream_t st, int n, cplx* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
-                  int* work_perm, int* info, LaunchTimer* update_timer = nullptr);
+                  int* work_perm, int* info, LaunchTimer* update_timer = nullptr, int pivot = kPivotStrict,
+                  cplx* inv = nullptr, int64_t inv_stride = 0);
 // complex elements of getrs work per matrix (sW must be at least this).  getrs_batched keeps the
 // diagonal-block inverses and the row permutation in work / perm: a later call on the same
 // factors with prepared = true reuses them.
@@ -98,7 +108,8 @@ int getrs_batched(hipStream_t st, int trans, int n, int R, const cplx* LU, int l
 // split: workspace of split_work(n) bf16 per matrix for the split-bf16 trailing updates (nullptr:
 // FP32 MFMA updates)
 int getrf_batched(hipStream_t st, int n, cplxf* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
-                  int* work_perm, int* info, LaunchTimer* update_timer = nullptr, unsigned short* split = nullptr);
+                  int* work_perm, int* info, LaunchTimer* update_timer = nullptr, unsigned short* split = nullptr,
+                  int pivot = kPivotStrict, cplxf* inv = nullptr, int64_t inv_stride = 0);
 int64_t split_work(int n);
 int getrs_batched(hipStream_t st, int trans, int n, int R, const cplxf* LU, int lda, int64_t sA, const int* ipiv,
                   int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
@@ -109,5 +120,9 @@ void launch_residual(hipStream_t st, int64_t cnt, const cplx* Bv, int64_t sB, cp
                      int batch);
 void launch_add(hipStream_t st, int64_t cnt, const cplx* D, int64_t sD, cplx* X, int64_t sX, int batch);
 constexpr int kPermWork = 1 + 4 * 64;  // ints per matrix for getrf_batched's work_perm
+// device-to-device copy by a kernel (bytes a multiple of 16; graph-safe)
+void launch_copy(hipStream_t st, const void* src, void* dst, size_t bytes);
+// out[q] = the verified-pivot flag of item q of a batch factored with kPivotVerified
+void launch_gather_pivot_flags(hipStream_t st, const int* wperm, int batch, int* out);
 
 }  // namespace mhx

@@ -406,13 +406,20 @@ __global__ void __launch_bounds__(256) trsm_cols_kernel(MatBatch<cplxf> mb, int 
 }
 
 // ---------------------------------------------------------------------------
-// C[m x nn] (+)= (-)A[m x kk] * B[kk x nn], complex, column-major, on MFMA
-// (v_mfma_f64_16x16x4f64 for cplx, v_mfma_f32_16x16x4f32 for cplxf).
-// 256 threads (4 waves in 2x2), 64x64 tile, K chunks of 16 staged in LDS as
-// separate real / imaginary planes.  Lane layout of both instructions:
-//   A: lane l holds A[l%16][l/16], B: B[l/16][l%16];
-//   C: 4 values per lane, f64: C[4v + l/16][l%16], f32: C[4(l/16) + v][l%16].
+// Verified diagonal pivoting.  For the influence matrices of the BEM (1/2 I plus small
+// off-diagonal couplings) LAPACK's partial pivoting keeps the diagonal pivot in every column:
+// after the earlier eliminations the diagonal is a largest |Re| + |Im| of the column's candidates
+// (and the first on ties).  These kernels factor with the diagonal pivots and check exactly that
+// rule on every candidate of every column; a matrix that breaks it anywhere, or meets an exactly
+// zero pivot, is flagged and factored again by the caller with row interchanges (the leaf kernels
+// above).  The pivot sequence is therefore LAPACK's in both cases, and a panel of 64 columns
+// costs one workgroup per matrix for its diagonal block plus one multi-workgroup launch for its
+// L21 rows and U12 columns.
+// Flag of matrix b: flags[flag_stride * b] (0: the rule held so far; set to 1 on a failure;
+// cleared by the diagonal kernel of the first panel).
 // ---------------------------------------------------------------------------
+template <class C> __device__ __forceinline__ typename Cx<C>::R tabs1(C a) { return fabs(a.x) + fabs(a.y); }
+
 typedef double double4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
@@ -432,6 +439,263 @@ template <> struct Mfma16<float> {
     static __device__ __forceinline__ int row(int v, int lk) { return 4 * lk + v; }
 };
 
+// value of lane `src` (uniform) of a wave
+__device__ __forceinline__ double rlane(double v, int src) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), src),
+                            __builtin_amdgcn_readlane(__double2loint(v), src));
+}
+__device__ __forceinline__ float rlane(float v, int src) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+template <class C> __device__ __forceinline__ C rlane(C v, int src) { return tmk<C>(rlane(v.x, src), rlane(v.y, src)); }
+
+// 1 / u as conj(u) / |u|^2 (one real division) where |u|^2 is far from overflow and underflow,
+// else Smith's algorithm
+template <class C> __device__ __forceinline__ C crecip(C u) {
+    typedef typename Cx<C>::R R;
+    const R d = u.x * u.x + u.y * u.y;
+    const R lo = sizeof(R) == 8 ? R(1e-280) : R(1e-35), hi = sizeof(R) == 8 ? R(1e280) : R(1e35);
+    if (d > lo && d < hi) {
+        const R sc = R(1) / d;
+        return tmk<C>(u.x * sc, -u.y * sc);
+    }
+    return trecip(u);
+}
+
+// Diagonal block A[K:K+nb, K:K+nb] (nb <= 64; padded with the identity to 64), one workgroup of
+// 512 threads per matrix.  LU with the diagonal pivots in four slabs of 16 columns: wave s holds
+// columns 16 s .. 16 s + 15 (lane = row) and factors them alone, wave-synchronously (row k
+// broadcast with readlane: no barrier per column), checking LAPACK's rule on the candidates of
+// each column; its multipliers go through LDS (one barrier per slab) to waves s + 1 .. 3, which
+// apply the slab's 16 rank-1 updates to their columns in column order.  Then waves 0-3 form
+// L11^{-1} and waves 4-7 U11^{-1} (16 right-hand-side columns of the identity per wave,
+// wave-synchronous substitution), stored in inv (L11^{-1} at 0, U11^{-1} at 64 x 64, column-major)
+// for the L21 / U12 products of panel_inv_gemm_kernel.  Rows below the block are checked there.
+template <class C>
+__global__ void __launch_bounds__(512) diag_lu_kernel(MatBatch<C> mb, int n, int K, int* __restrict__ ipiv,
+                                                      int64_t ipiv_stride, int* __restrict__ flags, int64_t flag_stride,
+                                                      C* __restrict__ inv, int64_t inv_stride) {
+    typedef typename Cx<C>::R R;
+    const int b = blockIdx.z;
+    C* A = mb.A + mb.stride * b;
+    const int lda = mb.lda;
+    const int nb = min(NB, n - K);
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int* flag = flags + flag_stride * b;
+    __shared__ C Ls[2][NB][17];   // a slab's multipliers by row (two slabs in flight)
+    __shared__ C LU[NB][NB + 1];  // the factored block
+    if (K == 0 && tid == 0) *flag = 0;
+    C x[16];
+    bool bad = false;
+    if (w < 4) {
+#pragma unroll
+        for (int c = 0; c < 16; c++) {
+            const int col = 16 * w + c;
+            x[c] = (lane < nb && col < nb) ? A[(K + lane) + (int64_t)lda * (K + col)]
+                                           : tmk<C>(lane == col ? R(1) : R(0), R(0));
+        }
+    }
+    for (int s = 0; s < 4; s++) {
+        if (w == s) {
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) {
+                const int k = 16 * s + kk;
+                const C u = rlane(x[kk], k);
+                const bool zero = u.x == 0 && u.y == 0;
+                const C iv = zero ? tmk<C>(0, 0) : crecip(u);
+                const R au = tabs1(u);
+                bad = bad || zero;
+                if (lane > k) {
+                    bad = bad || tabs1(x[kk]) > au;
+                    x[kk] = tmul(x[kk], iv);
+                }
+#pragma unroll
+                for (int cc = kk + 1; cc < 16; cc++) {
+                    const C ukc = rlane(x[cc], k);
+                    if (lane > k) x[cc] = tsub(x[cc], tmul(x[kk], ukc));
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 16; c++) Ls[s & 1][lane][c] = x[c];
+        }
+        __syncthreads();
+        if (w > s && w < 4) {
+            C l[16];
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) l[kk] = Ls[s & 1][lane][kk];
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) {
+                const int k = 16 * s + kk;
+#pragma unroll
+                for (int c = 0; c < 16; c++) {
+                    const C ukc = rlane(x[c], k);
+                    if (lane > k) x[c] = tsub(x[c], tmul(l[kk], ukc));
+                }
+            }
+        }
+    }
+    if (w < 4) {
+#pragma unroll
+        for (int c = 0; c < 16; c++) {
+            const int col = 16 * w + c;
+            LU[lane][col] = x[c];
+            if (lane < nb && col < nb) A[(K + lane) + (int64_t)lda * (K + col)] = x[c];
+        }
+    }
+    if (tid < nb) ipiv[ipiv_stride * b + K + tid] = K + tid;
+    if (bad) *flag = 1;
+    __syncthreads();
+    // inverses: X = L^{-1} (waves 0-3) or U^{-1} (waves 4-7), right-hand-side columns 16 (w % 4) ..
+    const int wc = 16 * (w & 3);
+    C* out = inv + inv_stride * b + (w < 4 ? 0 : NB * NB);
+#pragma unroll
+    for (int c = 0; c < 16; c++) x[c] = tmk<C>(lane == wc + c ? R(1) : R(0), R(0));
+    if (w < 4) {
+        // forward: row k final, rows below subtract L[i][k] x_k (unit diagonal)
+        for (int k = 0; k < NB - 1; k++) {
+            const C lik = LU[lane][k];
+#pragma unroll
+            for (int c = 0; c < 16; c++) {
+                if (wc + c > k) continue;  // x[., c] is zero above its diagonal
+                const C xk = rlane(x[c], k);
+                if (lane > k) x[c] = tsub(x[c], tmul(lik, xk));
+            }
+        }
+    } else {
+        // backward: row k divided by U[k][k], rows above subtract U[i][k] x_k
+        for (int k = NB - 1; k >= 0; k--) {
+            const C iv = crecip(LU[k][k]);
+            const C uik = LU[lane][k];
+#pragma unroll
+            for (int c = 0; c < 16; c++) {
+                if (wc + c < k) continue;  // x[., c] is zero below its diagonal
+                if (lane == k) x[c] = tmul(x[c], iv);
+                const C xk = rlane(x[c], k);
+                if (lane < k) x[c] = tsub(x[c], tmul(uik, xk));
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 16; c++) out[lane + NB * (wc + c)] = x[c];
+}
+
+// L21 = A21 U11^{-1} (part 0: rows [K + nb, n) in 64-row tiles) and U12 = L11^{-1} A12 (part 1:
+// columns [K + nb, n) in 64-column tiles), both in place, from the inverses of diag_lu_kernel:
+// 64 x 64 x 64 complex products on MFMA (as zgemm_mfma_kernel: FP64 four real products per
+// complex product, FP32 three).  Part 0 also checks LAPACK's rule on the rows below the block:
+// the updated candidate of column k is l_ik u_kk, whose |Re| + |Im| must not exceed |u_kk|'s.
+template <class C>
+__global__ void __launch_bounds__(256) panel_inv_gemm_kernel(MatBatch<C> mb, int n, int K, const C* inv,
+                                                             int64_t inv_stride, int* flags, int64_t flag_stride) {
+    typedef typename Cx<C>::R R;
+    typedef Mfma16<R> M;
+    constexpr bool THREE = sizeof(R) == 4;
+    const int b = blockIdx.z, part = blockIdx.y;
+    C* Am = mb.A + mb.stride * b;
+    const int lda = mb.lda;
+    const int nb = min(NB, n - K), KE = K + nb;
+    const C* iv = inv + inv_stride * b;
+    // operands: X (64 rows x 64 k) and Y (64 k x 64 cols); the product written over X (part 0) or Y
+    const int r0 = KE + 64 * blockIdx.x;  // part 0: first row; part 1: first column
+    const int rows = part == 0 ? min(64, n - r0) : nb;
+    const int cols = part == 0 ? nb : min(64, n - r0);
+    const C* X = part == 0 ? Am + r0 + (int64_t)lda * K : iv;       // A21 tile or L^{-1}
+    const int ldx = part == 0 ? lda : NB;
+    const C* Y = part == 0 ? iv + NB * NB : Am + K + (int64_t)lda * r0;  // U^{-1} or A12 tile
+    const int ldy = part == 0 ? NB : lda;
+    C* Out = part == 0 ? Am + r0 + (int64_t)lda * K : Am + K + (int64_t)lda * r0;
+    constexpr int KC = 16;
+    __shared__ R Ar[KC][80], Ai[KC][80], Br[KC][74], Bi[KC][74];
+    __shared__ R au[NB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    if (part == 0 && tid < NB) au[tid] = tid < nb ? tabs1(Am[(K + tid) + (int64_t)lda * (K + tid)]) : R(0);
+    typename M::V t1[2][2], t2[2][2], t3[2][2];
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) t1[bi][bj][v] = t2[bi][bj][v] = t3[bi][bj][v] = 0;
+    for (int k0 = 0; k0 < NB; k0 += KC) {
+#pragma unroll
+        for (int e = 0; e < KC / 4; e++) {
+            const int idx = tid + 256 * e;
+            const int r = idx & 63, ka = idx >> 6;
+            const C xa = (r < rows && k0 + ka < nb) ? X[r + (int64_t)ldx * (k0 + ka)] : tmk<C>(0, 0);
+            Ar[ka][r] = xa.x;
+            Ai[ka][r] = xa.y;
+            const int kb = idx % KC, c = idx / KC;
+            const C yb = (c < cols && k0 + kb < nb) ? Y[(k0 + kb) + (int64_t)ldy * c] : tmk<C>(0, 0);
+            Br[kb][c] = yb.x;
+            Bi[kb][c] = yb.y;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ks = 0; ks < KC; ks += 4) {
+            R ar[2], ai[2], br[2], bi_[2];
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                ar[q] = Ar[ks + lk][wm + q * 16 + li];
+                ai[q] = Ai[ks + lk][wm + q * 16 + li];
+                br[q] = Br[ks + lk][wn + q * 16 + li];
+                bi_[q] = Bi[ks + lk][wn + q * 16 + li];
+            }
+#pragma unroll
+            for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+                for (int bj = 0; bj < 2; bj++) {
+                    if (THREE) {
+                        t1[bi][bj] = M::run(ar[bi], br[bj], t1[bi][bj]);
+                        t2[bi][bj] = M::run(ai[bi], bi_[bj], t2[bi][bj]);
+                        t3[bi][bj] = M::run(ar[bi] + ai[bi], br[bj] + bi_[bj], t3[bi][bj]);
+                    } else {
+                        t1[bi][bj] = M::run(ar[bi], br[bj], t1[bi][bj]);
+                        t1[bi][bj] = M::run(-ai[bi], bi_[bj], t1[bi][bj]);
+                        t3[bi][bj] = M::run(ar[bi], bi_[bj], t3[bi][bj]);
+                        t3[bi][bj] = M::run(ai[bi], br[bj], t3[bi][bj]);
+                    }
+                }
+        }
+        __syncthreads();
+    }
+    bool bad = false;
+#pragma unroll
+    for (int bi = 0; bi < 2; bi++)
+#pragma unroll
+        for (int bj = 0; bj < 2; bj++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int r = wm + bi * 16 + M::row(v, lk), c = wn + bj * 16 + li;
+                if (r < rows && c < cols) {
+                    R re, im;
+                    if (THREE) {
+                        re = t1[bi][bj][v] - t2[bi][bj][v];
+                        im = t3[bi][bj][v] - t1[bi][bj][v] - t2[bi][bj][v];
+                    } else {
+                        re = t1[bi][bj][v];
+                        im = t3[bi][bj][v];
+                    }
+                    const C val = tmk<C>(re, im);
+                    if (part == 0) {
+                        const C ukk = Am[(K + c) + (int64_t)lda * (K + c)];
+                        bad = bad || tabs1(tmul(val, ukk)) > au[c];
+                    }
+                    Out[r + (int64_t)(part == 0 ? lda : lda) * c] = val;
+                }
+            }
+    if (bad) flags[flag_stride * b] = 1;
+}
+
+// ---------------------------------------------------------------------------
+// C[m x nn] (+)= (-)A[m x kk] * B[kk x nn], complex, column-major, on MFMA
+// (v_mfma_f64_16x16x4f64 for cplx, v_mfma_f32_16x16x4f32 for cplxf).
+// 256 threads (4 waves in 2x2), 64x64 tile, K chunks of 16 staged in LDS as
+// separate real / imaginary planes.  Lane layout of both instructions:
+//   A: lane l holds A[l%16][l/16], B: B[l/16][l%16];
+//   C: 4 values per lane, f64: C[4v + l/16][l%16], f32: C[4(l/16) + v][l%16].
+// ---------------------------------------------------------------------------
 // Complex product per k-step: FP64 uses the four real products (Re = ar br - ai bi, Im = ar bi +
 // ai br), rounding as the reference's zgemm; FP32 (the mixed-precision factor) uses three,
 // T1 = ar br, T2 = ai bi, T3 = (ar + ai)(br + bi): Re = T1 - T2, Im = T3 - T1 - T2.
@@ -781,37 +1045,51 @@ __global__ void __launch_bounds__(256) zgemm_split_kernel(int m, int mpad, int k
 // getrf driver.  ipiv: 0-based global row indices, int32, per matrix stride.
 // work_perm: int workspace of (1 + 4*NB) per matrix.  info: int per matrix (0 / first zero pivot, 1-based).
 // ---------------------------------------------------------------------------
+// leaf of W columns over rows of ROWS x 1024 (the column count shrinks as the rows grow, to keep
+// the register tile at 16 (FP64) or 32 (FP32) elements per thread)
+template <int W, int ROWS, class C>
+static void leaf_t(hipStream_t st, MatBatch<C> mb, int n, int c0, int K, int KE, int* ipiv, int64_t ipiv_stride,
+                   int* info, int batch) {
+    getf2_leaf_kernel<W, ROWS, C><<<dim3(1, 1, batch), LEAF_THREADS, 0, st>>>(mb, n, c0, K, KE, ipiv, ipiv_stride, info);
+    if (c0 + W < KE && c0 + W < n)
+        panel_update_kernel<W, C><<<dim3((n - c0 - W + 255) / 256, 1, batch), 256, 0, st>>>(mb, n, c0, KE);
+}
+
+// One leaf at c0 of the panel [K, KE).  The leaf width is fixed for the whole panel by the rows
+// below K (every width divides the panel's 64 columns, so no leaf crosses its end); the rows per
+// thread of each leaf follow the rows below c0.
+template <class C>
+static int leaf_dispatch(hipStream_t st, MatBatch<C> mb, int n, int c0, int K, int KE, int* ipiv, int64_t ipiv_stride,
+                         int* info, int batch) {
+    constexpr bool kF32 = std::is_same<C, cplxf>::value;
+    constexpr int W = kF32 ? 8 : 4;  // rows per thread <= 4
+    const int rp_panel = (n - K + LEAF_THREADS - 1) / LEAF_THREADS;
+    const int rp = (n - c0 + LEAF_THREADS - 1) / LEAF_THREADS;
+    if (rp_panel > 8) { leaf_t<W / 4, 16, C>(st, mb, n, c0, K, KE, ipiv, ipiv_stride, info, batch); return W / 4; }
+    if (rp_panel > 4) { leaf_t<W / 2, 8, C>(st, mb, n, c0, K, KE, ipiv, ipiv_stride, info, batch); return W / 2; }
+    if (rp <= 1) { leaf_t<W, 1, C>(st, mb, n, c0, K, KE, ipiv, ipiv_stride, info, batch); return W; }
+    if (rp <= 2) { leaf_t<W, 2, C>(st, mb, n, c0, K, KE, ipiv, ipiv_stride, info, batch); return W; }
+    leaf_t<W, 4, C>(st, mb, n, c0, K, KE, ipiv, ipiv_stride, info, batch);
+    return W;
+}
+
 template <class C>
 static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
-                   int* work_perm, int* info, LaunchTimer* timer, unsigned short* split = nullptr) {
+                   int* work_perm, int* info, LaunchTimer* timer, unsigned short* split, int pivot, C* inv,
+                   int64_t inv_stride) {
     if (n <= 0) return 0;
-    const int rows_per = (n + LEAF_THREADS - 1) / LEAF_THREADS;
-    if (rows_per > 4) return -1;  // orders above 4096 need a larger register tile
+    if (n > kMaxOrder) return -1;
+    if (pivot == kPivotVerified && !inv) return -2;
     MatBatch<C> mb{A, strideA, lda};
-    const int64_t pstride = 1 + 4 * NB;
+    const int64_t pstride = kPermWork;
     constexpr bool kF32 = std::is_same<C, cplxf>::value;
     const bool use_split = kF32 && split != nullptr;
-    // leaves of W columns over [K, KE), each followed by the rank-W update of the panel's rest
-    // (W = 8 for FP32: half the latency-bound leaf launches; FP64 keeps 4, its registers)
-    constexpr int W = kF32 ? 8 : 4;
+    const bool verified = pivot == kPivotVerified;
+    // pivoted: leaves over [K, KE), each followed by the rank-W update of the panel's rest
     auto factor_panel = [&](int K, int KE) {
-        for (int c0 = K; c0 < KE; c0 += W) {
-            const int m_rows = n - c0;
-            const int rp = (m_rows + LEAF_THREADS - 1) / LEAF_THREADS;
-            dim3 g1(1, 1, batch);
-            if (rp <= 1)
-                getf2_leaf_kernel<W, 1, C><<<g1, LEAF_THREADS, 0, st>>>(mb, n, c0, K, KE, ipiv, ipiv_stride, info);
-            else if (rp <= 2)
-                getf2_leaf_kernel<W, 2, C><<<g1, LEAF_THREADS, 0, st>>>(mb, n, c0, K, KE, ipiv, ipiv_stride, info);
-            else
-                getf2_leaf_kernel<W, 4, C><<<g1, LEAF_THREADS, 0, st>>>(mb, n, c0, K, KE, ipiv, ipiv_stride, info);
-            if (c0 + W < KE && c0 + W < n) {
-                dim3 g2((n - c0 - W + 255) / 256, 1, batch);
-                panel_update_kernel<W, C><<<g2, 256, 0, st>>>(mb, n, c0, KE);
-            }
-        }
+        for (int c0 = K; c0 < KE;) c0 += leaf_dispatch<C>(st, mb, n, c0, K, KE, ipiv, ipiv_stride, info, batch);
     };
-    // the panel's row interchanges applied to every column outside it
+    // pivoted: the panel's row interchanges applied to every column outside it
     auto permute_others = [&](int K, int KE) {
         if (n - (KE - K) > 0) {
             build_perm_kernel<<<dim3(1, 1, batch), 64, 0, st>>>(ipiv, ipiv_stride, K, KE, work_perm, pstride);
@@ -819,12 +1097,22 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
             apply_perm_kernel<C><<<dim3((nout + 1) / 2, 1, batch), 256, 0, st>>>(mb, n, K, KE, work_perm, pstride);
         }
     };
-    // U12 = L11^{-1} A12 for rows [K, KE = K + NB) and columns [KE, n)
+    // pivoted: U12 = L11^{-1} A12 for rows [K, KE = K + NB) and columns [KE, n)
     auto trsm = [&](int K, int KE) {
         if constexpr (kF32)
             trsm_cols_kernel<<<dim3((n - KE + 255) / 256, 1, batch), 256, 0, st>>>(mb, n, K);
         else
             trsm_panel_kernel<C><<<dim3((n - KE + TRSM_COLS - 1) / TRSM_COLS, 1, batch), 256, 0, st>>>(mb, n, K, KE);
+    };
+    // verified diagonal pivots: the panel's diagonal block (and its triangular inverses), then its
+    // L21 rows and U12 columns as products with those inverses
+    auto factor_verified = [&](int K) {
+        diag_lu_kernel<C><<<dim3(1, 1, batch), 512, 0, st>>>(mb, n, K, ipiv, ipiv_stride, work_perm, pstride, inv,
+                                                             inv_stride);
+        const int m = n - min(n, K + NB);
+        if (m > 0)
+            panel_inv_gemm_kernel<C><<<dim3((m + 63) / 64, 2, batch), 256, 0, st>>>(mb, n, K, inv, inv_stride, work_perm,
+                                                                                     pstride);
     };
     // A[KE:, KE:] -= A[KE:, K:KE] A[K:KE, KE:] (the timed launch: the GEMM alone)
     auto update = [&](int K, int KE) {
@@ -859,12 +1147,14 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
             while (S < SPLIT_SUPER && K + (S + 1) * NB < n) S++;
         if (S == 1) {  // one panel
             const int KE = min(n, K + NB);
-            factor_panel(K, KE);
-            permute_others(K, KE);
-            if (KE < n) {
-                trsm(K, KE);
-                update(K, KE);
+            if (verified) {
+                factor_verified(K);
+            } else {
+                factor_panel(K, KE);
+                permute_others(K, KE);
+                if (KE < n) trsm(K, KE);
             }
+            if (KE < n) update(K, KE);
             K = KE;
             continue;
         }
@@ -876,12 +1166,21 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
         //   U12(P_<j)[:, right of P_j];  U12(P_j) = L_jj^{-1} (those rows).
         // Then A[KS:, KS:] -= A[KS:, K:KS] A[K:KS, KS:].  The same operations as S one-panel
         // steps; only the trailing matrix's FP32 rounding differs (one accumulation instead of S).
+        // With verified diagonal pivots there are no interchanges, so both left-looking products
+        // of panel j come before its factorization.
         const int KS = K + S * NB;
         for (int j = 0; j < S; j++) {
             const int Kj = K + j * NB, Kq = Kj + NB;
             if (j > 0)
                 gemm_t<C>(st, true, true, n - Kj, NB, Kj - K, A + Kj + (int64_t)lda * K, lda, strideA,
                           A + K + (int64_t)lda * Kj, lda, strideA, A + Kj + (int64_t)lda * Kj, lda, strideA, batch);
+            if (verified) {
+                if (j > 0)
+                    gemm_t<C>(st, true, true, NB, n - Kq, Kj - K, A + Kj + (int64_t)lda * K, lda, strideA,
+                              A + K + (int64_t)lda * Kq, lda, strideA, A + Kj + (int64_t)lda * Kq, lda, strideA, batch);
+                factor_verified(Kj);
+                continue;
+            }
             factor_panel(Kj, Kq);
             permute_others(Kj, Kq);
             if (j > 0)
@@ -896,13 +1195,16 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
 }
 
 int getrf_batched(hipStream_t st, int n, cplx* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
-                  int* work_perm, int* info, LaunchTimer* update_timer) {
-    return getrf_t(st, n, A, lda, strideA, batch, ipiv, ipiv_stride, work_perm, info, update_timer);
+                  int* work_perm, int* info, LaunchTimer* update_timer, int pivot, cplx* inv, int64_t inv_stride) {
+    return getrf_t(st, n, A, lda, strideA, batch, ipiv, ipiv_stride, work_perm, info, update_timer, nullptr, pivot, inv,
+                   inv_stride);
 }
 
 int getrf_batched(hipStream_t st, int n, cplxf* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
-                  int* work_perm, int* info, LaunchTimer* update_timer, unsigned short* split) {
-    return getrf_t(st, n, A, lda, strideA, batch, ipiv, ipiv_stride, work_perm, info, update_timer, split);
+                  int* work_perm, int* info, LaunchTimer* update_timer, unsigned short* split, int pivot, cplxf* inv,
+                  int64_t inv_stride) {
+    return getrf_t(st, n, A, lda, strideA, batch, ipiv, ipiv_stride, work_perm, info, update_timer, split, pivot, inv,
+                   inv_stride);
 }
 
 // ---------------------------------------------------------------------------
@@ -1277,6 +1579,33 @@ __global__ void axpy_kernel(int64_t cnt, const cplx* __restrict__ Db, int64_t sD
 void launch_add(hipStream_t st, int64_t cnt, const cplx* D, int64_t sD, cplx* X, int64_t sX, int batch) {
     const int64_t blocks = std::min<int64_t>((cnt + 255) / 256, 256);
     axpy_kernel<<<dim3((int)blocks, 1, batch), 256, 0, st>>>(cnt, D, sD, X, sX);
+}
+
+}  // namespace mhx
+
+namespace mhx {
+
+// device-to-device copy with a kernel (16-byte units), so that it can sit in a captured graph
+__global__ void copy16_kernel(int64_t count, const uint4* __restrict__ src, uint4* __restrict__ dst) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count; t += (int64_t)gridDim.x * blockDim.x)
+        dst[t] = src[t];
+}
+
+void launch_copy(hipStream_t st, const void* src, void* dst, size_t bytes) {
+    const int64_t count = (int64_t)(bytes / 16);
+    if (count <= 0) return;
+    const int64_t blocks = std::min<int64_t>((count + 255) / 256, 8192);
+    copy16_kernel<<<(int)blocks, 256, 0, st>>>(count, reinterpret_cast<const uint4*>(src), reinterpret_cast<uint4*>(dst));
+}
+
+// out[q] = the verified-pivot flag of item q (wperm[kPermWork q])
+__global__ void gather_flags_kernel(const int* __restrict__ wperm, int batch, int* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < batch) out[t] = wperm[(int64_t)kPermWork * t];
+}
+
+void launch_gather_pivot_flags(hipStream_t st, const int* wperm, int batch, int* out) {
+    if (batch > 0) gather_flags_kernel<<<(batch + 255) / 256, 256, 0, st>>>(wperm, batch, out);
 }
 
 }  // namespace mhx

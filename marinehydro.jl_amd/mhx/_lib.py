@@ -31,12 +31,15 @@ LU_AUTO = 0
 LU_FP64 = 1
 LU_MIXED = 2
 LU_MIXED_BF16 = 3
+PIVOT_VERIFIED = 0
+PIVOT_ALWAYS = 1
 
 EXPORTED = ["mhx_create", "mhx_destroy", "mhx_last_error", "mhx_set_constants", "mhx_assemble", "mhx_solve",
             "mhx_lu_solve", "mhx_lu_solve_batched", "mhx_lu_solve_noalloc", "mhx_hydro_coeffs", "mhx_body_create", "mhx_body_destroy", "mhx_body_coeffs",
             "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work", "mhx_set_batch", "mhx_surrogate_train",
             "mhx_surrogate_save", "mhx_surrogate_load", "mhx_surrogate_params", "mhx_surrogate_eval",
-            "mhx_surrogate_loss_grad", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_hydro_coeffs_jvp",
+            "mhx_surrogate_loss_grad", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_set_pivoting",
+            "mhx_pivot_stats", "mhx_hydro_coeffs_jvp",
             "mhx_hydro_coeffs_vjp", "mhx_sweep_coeffs", "mhx_body_create_lid", "mhx_body_coeffs_fs"]
 
 _dp = C.POINTER(C.c_double)
@@ -127,6 +130,8 @@ def lib():
         L.mhx_set_batch.argtypes = [vp, C.c_int64, C.c_double]
         L.mhx_set_lu_mode.argtypes = [vp, C.c_int]
         L.mhx_solver_stats.argtypes = [vp, _ip, _dp]
+        L.mhx_set_pivoting.argtypes = [vp, C.c_int]
+        L.mhx_pivot_stats.argtypes = [vp, _ip]
         L.mhx_surrogate_train.argtypes = [vp, C.c_int64, C.c_int64, C.c_int64, C.c_double, C.c_uint64, _dp]
         L.mhx_surrogate_save.argtypes = [vp, C.c_char_p]
         L.mhx_surrogate_load.argtypes = [vp, C.c_char_p]
@@ -205,6 +210,18 @@ class Context:
         r = C.c_double()
         self.check(lib().mhx_solver_stats(self.h, c.ctypes.data_as(_ip), C.byref(r)))
         return int(c[0]), int(c[1]), int(c[2]), r.value
+
+    def set_pivoting(self, mode):
+        """PIVOT_VERIFIED (default: diagonal pivots with LAPACK's rule checked on every candidate, a
+        matrix that breaks it factored again with row interchanges) or PIVOT_ALWAYS."""
+        self.check(lib().mhx_set_pivoting(self.h, int(mode)))
+
+    def pivot_stats(self):
+        """(factorizations that kept the verified diagonal pivots, factorizations with the pivot
+        search) since reset_timing."""
+        c = np.zeros(2, dtype=np.int64)
+        self.check(lib().mhx_pivot_stats(self.h, c.ctypes.data_as(_ip)))
+        return int(c[0]), int(c[1])
 
     # ---- learned surrogate of the exact wave term ----
     def surrogate_train(self, ngrid=2048, steps=8000, batch=4096, lr=2e-3, seed=20261203):

@@ -86,9 +86,12 @@ def test_noalloc_checks(env):
     hip, mhx, _lib, ctx = env
     print("noalloc argument checks", flush=True)
     one = hip.upload(np.zeros((1, 1)))
-    # orders above the factorization's limit are refused before any launch
+    # a workspace larger than the scratch (order 8000 with 16 right-hand sides: 20 MB) is refused
+    # before any launch, as is an order above the factorization's limit
     with pytest.raises(mhx.DimensionMismatch):
-        _call(env, one, one, 5000, 1, 0)
+        _call(env, one, one, 8000, 16, 0)
+    with pytest.raises(mhx.DimensionMismatch):
+        _call(env, one, one, 20000, 1, 0)
     hip.free(one)
     # host memory is refused
     h = np.zeros((4, 4), dtype=np.complex128)
