@@ -38,6 +38,7 @@ __device__ __forceinline__ C tmk(typename Cx<C>::R r, typename Cx<C>::R i) {
     return c;
 }
 template <class C> __device__ __forceinline__ C tsub(C a, C b) { return tmk<C>(a.x - b.x, a.y - b.y); }
+template <class C> __device__ __forceinline__ C tadd(C a, C b) { return tmk<C>(a.x + b.x, a.y + b.y); }
 template <class C> __device__ __forceinline__ C tmul(C a, C b) {
     return tmk<C>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -482,8 +483,11 @@ __global__ void __launch_bounds__(512) diag_lu_kernel(MatBatch<C> mb, int n, int
     const int nb = min(NB, n - K);
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     int* flag = flags + flag_stride * b;
-    __shared__ C Ls[2][NB][17];   // a slab's multipliers by row (two slabs in flight)
     __shared__ C LU[NB][NB + 1];  // the factored block
+    // the slab multipliers (LU phase) and then the triangle being inverted share this buffer
+    __shared__ __align__(16) unsigned char xbuf[sizeof(C) * NB * (NB + 1)];
+    __shared__ __align__(16) unsigned char tbuf[sizeof(C) * 48 * 17];
+    C(*Ls)[NB][17] = reinterpret_cast<C(*)[NB][17]>(xbuf);  // Ls[2][NB][17]
     if (K == 0 && tid == 0) *flag = 0;
     C x[16];
     bool bad = false;
@@ -545,38 +549,76 @@ __global__ void __launch_bounds__(512) diag_lu_kernel(MatBatch<C> mb, int n, int
     if (tid < nb) ipiv[ipiv_stride * b + K + tid] = K + tid;
     if (bad) *flag = 1;
     __syncthreads();
-    // inverses: X = L^{-1} (waves 0-3) or U^{-1} (waves 4-7), right-hand-side columns 16 (w % 4) ..
-    const int wc = 16 * (w & 3);
-    C* out = inv + inv_stride * b + (w < 4 ? 0 : NB * NB);
+    // inverses, one triangle at a time in X (the slab buffer's space): the four 16 x 16 diagonal
+    // blocks by substitution (wave w: block w % 4, right-hand-side columns 8 (w / 4) .. + 7; lane =
+    // row r (lane % 16) and column pair g (lane / 16)), then the off-diagonal blocks block row by
+    // block row: X_ij = -X_ii sum_k T_ik X_kj with T = L (k = j .. i - 1, rows downwards) or U
+    // (k = i + 1 .. j, rows upwards).
+    C(*X)[NB + 1] = reinterpret_cast<C(*)[NB + 1]>(xbuf);
+    C(*T)[17] = reinterpret_cast<C(*)[17]>(tbuf);
+    const int blk = w & 3, r16 = lane & 15, g = lane >> 4, cb = 8 * (w >> 2) + 2 * g;
+    for (int tri = 0; tri < 2; tri++) {
+        C y[2];
 #pragma unroll
-    for (int c = 0; c < 16; c++) x[c] = tmk<C>(lane == wc + c ? R(1) : R(0), R(0));
-    if (w < 4) {
-        // forward: row k final, rows below subtract L[i][k] x_k (unit diagonal)
-        for (int k = 0; k < NB - 1; k++) {
-            const C lik = LU[lane][k];
+        for (int c = 0; c < 2; c++) y[c] = tmk<C>(r16 == cb + c ? R(1) : R(0), R(0));
+        if (tri == 0) {
+            for (int k = 0; k < 15; k++) {
+                const C lrk = LU[16 * blk + r16][16 * blk + k];
 #pragma unroll
-            for (int c = 0; c < 16; c++) {
-                if (wc + c > k) continue;  // x[., c] is zero above its diagonal
-                const C xk = rlane(x[c], k);
-                if (lane > k) x[c] = tsub(x[c], tmul(lik, xk));
+                for (int c = 0; c < 2; c++) {
+                    const C yk = tmk<C>(__shfl(y[c].x, k + 16 * g), __shfl(y[c].y, k + 16 * g));
+                    if (r16 > k) y[c] = tsub(y[c], tmul(lrk, yk));
+                }
+            }
+        } else {
+            for (int k = 15; k >= 0; k--) {
+                const C iv = crecip(LU[16 * blk + k][16 * blk + k]);
+                const C urk = LU[16 * blk + r16][16 * blk + k];
+#pragma unroll
+                for (int c = 0; c < 2; c++) {
+                    if (r16 == k) y[c] = tmul(y[c], iv);
+                    const C yk = tmk<C>(__shfl(y[c].x, k + 16 * g), __shfl(y[c].y, k + 16 * g));
+                    if (r16 < k) y[c] = tsub(y[c], tmul(urk, yk));
+                }
             }
         }
-    } else {
-        // backward: row k divided by U[k][k], rows above subtract U[i][k] x_k
-        for (int k = NB - 1; k >= 0; k--) {
-            const C iv = crecip(LU[k][k]);
-            const C uik = LU[lane][k];
+        // the whole triangle buffer: zero, then the diagonal blocks
+        for (int e = tid; e < NB * NB; e += 512) X[e & 63][e >> 6] = tmk<C>(0, 0);
+        __syncthreads();
 #pragma unroll
-            for (int c = 0; c < 16; c++) {
-                if (wc + c < k) continue;  // x[., c] is zero below its diagonal
-                if (lane == k) x[c] = tmul(x[c], iv);
-                const C xk = rlane(x[c], k);
-                if (lane < k) x[c] = tsub(x[c], tmul(uik, xk));
+        for (int c = 0; c < 2; c++) X[16 * blk + r16][16 * blk + cb + c] = y[c];
+        __syncthreads();
+        for (int step = 0; step < 3; step++) {
+            const int i = tri == 0 ? step + 1 : 2 - step;  // block row
+            const int nbj = tri == 0 ? i : 3 - i;           // blocks j in this row
+            // T_j = sum_k LU_ik X_kj
+            for (int o = tid; o < nbj * 256; o += 512) {
+                const int jj = o >> 8, r = (o >> 4) & 15, c = o & 15;
+                const int j = tri == 0 ? jj : i + 1 + jj;
+                const int k0 = tri == 0 ? j : i + 1, k1 = tri == 0 ? i - 1 : j;
+                C acc = tmk<C>(0, 0);
+                for (int k = k0; k <= k1; k++)
+#pragma unroll 4
+                    for (int t = 0; t < 16; t++)
+                        acc = tadd(acc, tmul(LU[16 * i + r][16 * k + t], X[16 * k + t][16 * j + c]));
+                T[16 * jj + r][c] = acc;
             }
+            __syncthreads();
+            // X_ij = -X_ii T_j
+            for (int o = tid; o < nbj * 256; o += 512) {
+                const int jj = o >> 8, r = (o >> 4) & 15, c = o & 15;
+                const int j = tri == 0 ? jj : i + 1 + jj;
+                C acc = tmk<C>(0, 0);
+#pragma unroll 4
+                for (int t = 0; t < 16; t++) acc = tadd(acc, tmul(X[16 * i + r][16 * i + t], T[16 * jj + t][c]));
+                X[16 * i + r][16 * j + c] = tmk<C>(-acc.x, -acc.y);
+            }
+            __syncthreads();
         }
+        C* out = inv + inv_stride * b + (tri == 0 ? 0 : NB * NB);
+        for (int e = tid; e < NB * NB; e += 512) out[e] = X[e & 63][e >> 6];
+        __syncthreads();
     }
-#pragma unroll
-    for (int c = 0; c < 16; c++) out[lane + NB * (wc + c)] = x[c];
 }
 
 // L21 = A21 U11^{-1} (part 0: rows [K + nb, n) in 64-row tiles) and U12 = L11^{-1} A12 (part 1:
