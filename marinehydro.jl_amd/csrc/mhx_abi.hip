@@ -241,11 +241,27 @@ bool use_split(const mhx_ctx* ctx, int gf) {
 // device bytes one batch item of a solve holds (the chunk size of the batched schedules)
 double item_bytes(const mhx_ctx* ctx, int n, int gf, bool table, int rhs_cols) {
     double per = (double)n * n * sizeof(cplx) * (table ? 4 : 2) + (double)rhs_cols * n * 16 * sizeof(cplx);
-    if (gf == MHX_GF_SURROGATE) per += 8.0 * (double)mhx::surr_fallback_capacity(n, 1);  // fallback list
+    if (gf != MHX_GF_WU) per += 8.0 * (double)mhx::surr_fallback_capacity(n, 1);  // pair list
     if (use_mixed(ctx, gf)) per += (double)n * n * sizeof(cplxf) + 2.0 * n * 16 * sizeof(cplx);  // FP32 factors
     else if (ctx->pivoting == MHX_PIVOT_VERIFIED) per += (double)n * n * sizeof(cplx);  // copy of D
     if (use_split(ctx, gf)) per += 2.0 * mhx::split_work(n);  // split-bf16 panels
     return per;
+}
+
+// the body's pair list (one entry per unordered pair of each of `batch` items): the surrogate's
+// exact-fallback pairs, or the exact term's pairs left for adaptive refinement
+int ensure_pair_list(mhx_ctx* ctx, mhx_body* b, int batch) {
+    const int64_t cap = mhx::surr_fallback_capacity(b->n, batch);
+    if (b->fb_cap >= cap) return MHX_OK;
+    int rc;
+    dfree(b->fb_list);
+    bump_gen(b);
+    b->fb_list = nullptr;
+    b->fb_cap = 0;
+    if (!b->fb_count && (rc = dalloc(ctx, &b->fb_count, 1))) return rc;
+    if ((rc = dalloc(ctx, &b->fb_list, (size_t)cap))) return rc;
+    b->fb_cap = cap;
+    return MHX_OK;
 }
 
 // wave assembly of `batch` items (wavenumbers already in b->d_ks) into b->S, b->D.  keep_table:
@@ -260,17 +276,8 @@ int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int 
                                include_identity, b->S, b->D, b->nn());
     } else if (gf == MHX_GF_SURROGATE) {
         if (!ctx->surr_ready) return fail(ctx, MHX_EINVAL, "surrogate: no trained network (train or load one)");
-        const int64_t cap = mhx::surr_fallback_capacity(b->n, batch);
-        if (b->fb_cap < cap) {
-            int rc;
-            dfree(b->fb_list);
-            bump_gen(b);
-            b->fb_list = nullptr;
-            b->fb_cap = 0;
-            if (!b->fb_count && (rc = dalloc(ctx, &b->fb_count, 1))) return rc;
-            if ((rc = dalloc(ctx, &b->fb_list, (size_t)cap))) return rc;
-            b->fb_cap = cap;
-        }
+        int rc;
+        if ((rc = ensure_pair_list(ctx, b, batch))) return rc;
         // fused: the pair kernel writes S, D (and the FP32 copy of D for the mixed-precision solver)
         mhx::SurrOut out;
         out.SR = b->SR;
@@ -303,11 +310,13 @@ int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int 
                                       include_identity, b->WG, b->WR, b->S, b->D, b->nn());
         }
     } else {
+        int rc;
+        if ((rc = ensure_pair_list(ctx, b, batch))) return rc;
         HCHK(hipMemsetAsync(b->status, 0, 4 * sizeof(unsigned long long), ctx->stream));
         {
             Phase ph(ctx, "wave_table");
             mhx::launch_delhommeau_table(ctx->stream, b->n, b->geom, b->d_ks, batch, 1e-8, 1e-8, b->WG, b->WR, b->nn(),
-                                         b->status);
+                                         b->fb_list, b->fb_count, b->fb_cap, b->status);
         }
         Phase ph(ctx, "wave");
         mhx::launch_combine_table(ctx->stream, b->n, b->geom, b->SR, b->DR[direct], b->d_ks, batch, direct,

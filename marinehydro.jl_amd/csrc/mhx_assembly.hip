@@ -293,52 +293,152 @@ void launch_combine_wu(hipStream_t st, int n, const PanelGeom* geom, const doubl
 // Exact Guevel-Delhommeau dimensionless (G, dG/dr) per unordered panel pair.
 // The reduced coordinates are bitwise symmetric in (i, j), so one evaluation
 // serves both matrix entries.
-__global__ void __launch_bounds__(256) delhommeau_table_kernel(int n, const PanelGeom* __restrict__ geom,
-                                                               const double* __restrict__ ks, double rtol, double atol,
-                                                               cplx* __restrict__ WGb, cplx* __restrict__ WRb,
-                                                               int64_t sMat, unsigned long long* __restrict__ status) {
-    if (blockIdx.x > blockIdx.y) return;
-    const double k = ks[blockIdx.z];
-    cplx* WG = WGb + sMat * blockIdx.z;
-    cplx* WR = WRb + sMat * blockIdx.z;
-    const int i = blockIdx.x * 16 + threadIdx.x;
-    const int j = blockIdx.y * 16 + threadIdx.y;
-    QuadWork wk;
-    if (i < n && j < n && i <= j) {
-        const PanelGeom& gi = geom[i];
-        const PanelGeom& gj = geom[j];
-        double hh, vv;
-        reduced_hv(gi.c[0], gi.c[1], gi.c[2], gj.c[0], gj.c[1], gj.c[2], k, hh, vv);
-        cplx Gr, G;
-        if (!hquad_pair(hh, vv, rtol, atol, G, Gr, wk)) atomicOr(status, 1ull);
-        G = cscale(4.0, G);
-        Gr = cscale(4.0, Gr);
-        WG[i + (int64_t)n * j] = G;
-        WR[i + (int64_t)n * j] = Gr;
-        if (i != j) {
-            WG[j + (int64_t)n * i] = G;
-            WR[j + (int64_t)n * i] = Gr;
-        }
+static int device_cus();
+
+// Exact wave-term table, pass 1: every unordered pair gets the first GK(7,15) interval of both
+// quadratures (uniform work per lane); pairs whose G and dG/dr already meet the termination rule
+// are written, the others appended to a list (item << 40 | i << 20 | j) for pass 2.  Splitting
+// the adaptive refinement off keeps the lanes of a wave from waiting on their slowest pair.
+__device__ __forceinline__ void table_store(cplx* WG, cplx* WR, int n, int i, int j, cplx G, cplx Gr) {
+    G = cscale(4.0, G);
+    Gr = cscale(4.0, Gr);
+    WG[i + (int64_t)n * j] = G;
+    WR[i + (int64_t)n * j] = Gr;
+    if (i != j) {
+        WG[j + (int64_t)n * i] = G;
+        WR[j + (int64_t)n * i] = Gr;
     }
-    // one atomic per wave for the work tally
+}
+
+// one atomic per wave for the work tally; tid: the thread's linear index in its block
+__device__ __forceinline__ void tally_work(const QuadWork& wk, int tid, unsigned long long* status) {
     unsigned long long e = wk.evals, se = wk.series, cf = wk.cf;
     for (int off = 32; off > 0; off >>= 1) {
         e += __shfl_xor(e, off);
         se += __shfl_xor(se, off);
         cf += __shfl_xor(cf, off);
     }
-    const int lane = (threadIdx.y * 16 + threadIdx.x) & 63;
-    if (lane == 0 && e) {
+    if ((tid & 63) == 0 && e) {
         atomicAdd(status + 1, e);
         atomicAdd(status + 2, se);
         atomicAdd(status + 3, cf);
     }
 }
 
+__global__ void __launch_bounds__(256) delhommeau_table_kernel(int n, const PanelGeom* __restrict__ geom,
+                                                               const double* __restrict__ ks, double rtol, double atol,
+                                                               cplx* __restrict__ WGb, cplx* __restrict__ WRb,
+                                                               int64_t sMat, unsigned long long* __restrict__ rlist,
+                                                               unsigned long long* __restrict__ rcount, int64_t rcap,
+                                                               unsigned long long* __restrict__ status) {
+    if (blockIdx.x > blockIdx.y) return;
+    const int item = blockIdx.z;
+    const double k = ks[item];
+    const int tid = threadIdx.y * 16 + threadIdx.x;
+    __shared__ double cosn[15];
+    if (tid < 15) cosn[tid] = first_node_cos(tid);
+    __syncthreads();
+    const int i = blockIdx.x * 16 + threadIdx.x;
+    const int j = blockIdx.y * 16 + threadIdx.y;
+    QuadWork wk;
+    bool refine = false;
+    if (i < n && j < n && i <= j) {
+        const PanelGeom& gi = geom[i];
+        const PanelGeom& gj = geom[j];
+        double hh, vv;
+        reduced_hv(gi.c[0], gi.c[1], gi.c[2], gj.c[0], gj.c[1], gj.c[2], k, hh, vv);
+        cplx G, Gr;
+        double EG, ER;
+        cplx* WG = WGb + sMat * item;
+        cplx* WR = WRb + sMat * item;
+        if (hquad_first(cosn, hh, vv, rtol, atol, G, Gr, EG, ER, wk)) {
+            table_store(WG, WR, n, i, j, G, Gr);
+        } else {
+            refine = true;
+            // the first interval's estimates for pass 2, in the pair's own table slots (which pass
+            // 2 overwrites): G and dG/dr at (i, j), both errors at (j, i); a diagonal pair has
+            // one slot and is recomputed
+            if (i != j) {
+                WG[i + (int64_t)n * j] = G;
+                WR[i + (int64_t)n * j] = Gr;
+                WG[j + (int64_t)n * i] = cmk(EG, ER);
+            }
+        }
+    }
+    // wave-aggregated append to the refinement list
+    const unsigned long long mask = __ballot(refine);
+    if (mask) {
+        const int lane = tid & 63;
+        unsigned long long base = 0;
+        const int leader = __ffsll((long long)mask) - 1;
+        if (lane == leader) base = atomicAdd(rcount, (unsigned long long)__popcll(mask));
+        base = __shfl(base, leader);
+        if (refine) {
+            const unsigned long long slot = base + __popcll(mask & ((1ull << lane) - 1ull));
+            if (slot < (unsigned long long)rcap)
+                rlist[slot] = ((unsigned long long)item << 40) | ((unsigned long long)i << 20) | (unsigned)j;
+        }
+    }
+    tally_work(wk, tid, status);
+}
+
+// pass 2: the listed pairs, each the adaptive refinement on one 32-lane group from pass 1's first
+// interval (two pairs per wave; grid-stride: the count is only known on the device).  The nodes of an interval are
+// spread over the group's lanes, so its lanes work in step; only the two groups of a wave can
+// take different numbers of bisections.
+__global__ void __launch_bounds__(256) delhommeau_refine_kernel(int n, const PanelGeom* __restrict__ geom,
+                                                                const double* __restrict__ ks, double rtol, double atol,
+                                                                cplx* __restrict__ WGb, cplx* __restrict__ WRb,
+                                                                int64_t sMat, const unsigned long long* __restrict__ rlist,
+                                                                const unsigned long long* __restrict__ rcount,
+                                                                int64_t rcap, unsigned long long* __restrict__ status) {
+    __shared__ double cosn[15];
+    __shared__ QuadBoxes boxes[8];
+    if (threadIdx.x < 15) cosn[threadIdx.x] = first_node_cos(threadIdx.x);
+    __syncthreads();
+    const int64_t cnt = (int64_t)min(*rcount, (unsigned long long)rcap);
+    const int gl = threadIdx.x & 31, grp = threadIdx.x >> 5;
+    QuadBoxes& B = boxes[grp];
+    QuadWork wk;
+    for (int64_t t = blockIdx.x * 8LL + grp; t < cnt; t += (int64_t)gridDim.x * 8) {
+        const unsigned long long e = rlist[t];
+        const int item = (int)(e >> 40), i = (int)((e >> 20) & 0xFFFFF), j = (int)(e & 0xFFFFF);
+        const PanelGeom& gi = geom[i];
+        const PanelGeom& gj = geom[j];
+        double hh, vv;
+        reduced_hv(gi.c[0], gi.c[1], gi.c[2], gj.c[0], gj.c[1], gj.c[2], ks[item], hh, vv);
+        cplx* WG = WGb + sMat * item;
+        cplx* WR = WRb + sMat * item;
+        cplx IG, IR;
+        double EG, ER;
+        if (i != j) {
+            IG = WG[i + (int64_t)n * j];
+            IR = WR[i + (int64_t)n * j];
+            const cplx E = WG[j + (int64_t)n * i];
+            EG = E.x;
+            ER = E.y;
+        } else {
+            first_interval_group(gl, cosn, hh, vv, IG, EG, IR, ER, wk);
+        }
+        cplx G, Gr;
+        const bool ok = refine_pair_group(B, gl, hh, vv, rtol, atol, IG, EG, IR, ER, G, Gr, wk);
+        if (gl == 0) {
+            if (!ok) atomicOr(status, 1ull);
+            table_store(WG, WR, n, i, j, G, Gr);
+        }
+    }
+    tally_work(wk, threadIdx.x, status);
+}
+
 void launch_delhommeau_table(hipStream_t st, int n, const PanelGeom* geom, const double* ks, int batch, double rtol,
-                             double atol, cplx* WG, cplx* WR, int64_t sMat, unsigned long long* status) {
+                             double atol, cplx* WG, cplx* WR, int64_t sMat, unsigned long long* rlist,
+                             unsigned long long* rcount, int64_t rcap, unsigned long long* status) {
+    launch_zero(st, rcount, sizeof(unsigned long long));
     dim3 grid((n + 15) / 16, (n + 15) / 16, batch);
-    delhommeau_table_kernel<<<grid, dim3(16, 16), 0, st>>>(n, geom, ks, rtol, atol, WG, WR, sMat, status);
+    delhommeau_table_kernel<<<grid, dim3(16, 16), 0, st>>>(n, geom, ks, rtol, atol, WG, WR, sMat, rlist, rcount, rcap,
+                                                           status);
+    delhommeau_refine_kernel<<<device_cus() * 8, 256, 0, st>>>(n, geom, ks, rtol, atol, WG, WR, sMat, rlist, rcount,
+                                                               rcap, status);
 }
 
 // with_reduced_coordinates_derivative (abstract_greens_function.jl:140-185) and

@@ -301,15 +301,26 @@ struct QuadWork {
 #define E1_QUAL __constant__ static
 #include "mhx_e1_table.h"
 
-__device__ __noinline__ cplx expE1(cplx z, QuadWork& wk) {
+// exp(z) E1(z); ez = exp(z) (the caller's value: the integrands need it as well)
+__device__ __noinline__ cplx expE1(cplx z, cplx ez, QuadWork& wk) {
     if (e1_use_series(z)) {
         // E1 = -gamma - log z - sum_{n=1}^{N} a_n z^n, Horner, degree from the table at ceil|z|
         int ir = (int)ceil(hypot(z.x, z.y));
         ir = ir > 40 ? 40 : ir;
         const int N = E1_DEG[ir];
-        double pr = E1_A[N - 1], pi = 0.0;
-        for (int q = N - 2; q >= 0; q--) {
-            const double tr = E1_A[q] + (z.x * pr - z.y * pi);
+        // Horner from the largest degree among the wave's active lanes, with the coefficients
+        // above a lane's own degree taken as zero: the lane's polynomial stays zero until its
+        // leading term, so its value is bitwise that of Horner from its own degree, and the
+        // coefficient index is the same for every lane (a scalar load)
+        int Nw = 0;
+#pragma unroll
+        for (int bit = 6; bit >= 0; bit--)
+            if (__ballot(N >= Nw + (1 << bit))) Nw += 1 << bit;
+        Nw = __builtin_amdgcn_readfirstlane(Nw);
+        double pr = 0.0, pi = 0.0;
+        for (int q = Nw - 1; q >= 0; q--) {
+            const double a = q < N ? E1_A[q] : 0.0;
+            const double tr = a + (z.x * pr - z.y * pi);
             pi = z.x * pi + z.y * pr;
             pr = tr;
         }
@@ -317,7 +328,7 @@ __device__ __noinline__ cplx expE1(cplx z, QuadWork& wk) {
         const double sr = z.x * pr - z.y * pi, si = z.x * pi + z.y * pr;
         cplx L = clog_(z);
         cplx e1 = cmk(-MHX_EULER - L.x - sr, -L.y - si);
-        return cmul(cexp_(z), e1);
+        return cmul(ez, e1);
     }
     const double tiny = 1e-300;
     cplx f = cmk(z.x + 1.0, z.y);
@@ -347,7 +358,7 @@ __device__ __forceinline__ cplx del_integrand(double th, double r, double z, Qua
     cplx zeta = cmk(z, r * c);
     cplx ez = cexp_(zeta);
     wk.evals++;
-    cplx w = expE1(zeta, wk);
+    cplx w = expE1(zeta, ez, wk);
     cplx j = cmk(w.x / MHX_PI - ez.y, w.y / MHX_PI + ez.x);  // w/pi + i e^zeta
     if (!DERIV) return cmk(j.x, ez.x);
     cplx inv = cdiv(cmk(1.0, 0.0), cmk(MHX_PI * zeta.x, MHX_PI * zeta.y));
@@ -393,11 +404,62 @@ __device__ __forceinline__ void del_integrand_both(double th, double r, double z
     cplx zeta = cmk(z, r * c);
     cplx ez = cexp_(zeta);
     wk.evals++;
-    cplx w = expE1(zeta, wk);
+    cplx w = expE1(zeta, ez, wk);
     cplx j = cmk(w.x / MHX_PI - ez.y, w.y / MHX_PI + ez.x);
     fG = cmk(j.x, ez.x);
     cplx inv = cdiv(cmk(1.0, 0.0), cmk(MHX_PI * zeta.x, MHX_PI * zeta.y));
     fR = cmk(-c * (j.y - inv.y), -c * ez.y);
+}
+
+// del_integrand_both with cos(th) given (bitwise the same when c == cos(th))
+__device__ __forceinline__ void del_integrand_both_c(double c, double r, double z, QuadWork& wk, cplx& fG, cplx& fR) {
+    cplx zeta = cmk(z, r * c);
+    cplx ez = cexp_(zeta);
+    wk.evals++;
+    cplx w = expE1(zeta, ez, wk);
+    cplx j = cmk(w.x / MHX_PI - ez.y, w.y / MHX_PI + ez.x);
+    fG = cmk(j.x, ez.x);
+    cplx inv = cdiv(cmk(1.0, 0.0), cmk(MHX_PI * zeta.x, MHX_PI * zeta.y));
+    fR = cmk(-c * (j.y - inv.y), -c * ez.y);
+}
+
+// cosines of the 15 nodes of the first interval [0, pi/2]: cosn[7] at the centre, cosn[i] and
+// cosn[8 + i] at c - d x_i and c + d x_i (i < 7), computed as gk15_both computes them
+__device__ __forceinline__ double first_node_cos(int q) {
+    const double a = 0.0, b = MHX_PI / 2;
+    const double c = (a + b) * 0.5, d = (b - a) * 0.5;
+    if (q == 7) return cos(c);
+    const double dx = d * GK_X[q < 7 ? q : q - 8];
+    return cos(q < 7 ? c - dx : c + dx);
+}
+
+// gk15_both on [0, pi/2] from the node cosines
+__device__ __forceinline__ void gk15_both_first(const double* cosn, double r, double z, cplx& IG, double& EG, cplx& IR,
+                                                double& ER, QuadWork& wk) {
+    const double d = (MHX_PI / 2 - 0.0) * 0.5;
+    cplx g0, r0;
+    del_integrand_both_c(cosn[7], r, z, wk, g0, r0);
+    cplx IkG = cscale(GK_WK[7], g0), IgG = cscale(GK_WG[3], g0);
+    cplx IkR = cscale(GK_WK[7], r0), IgR = cscale(GK_WG[3], r0);
+#pragma unroll 1
+    for (int i = 0; i < 7; i++) {
+        cplx gm, rm, gp, rp;
+        del_integrand_both_c(cosn[i], r, z, wk, gm, rm);
+        del_integrand_both_c(cosn[8 + i], r, z, wk, gp, rp);
+        cplx fG = cadd(gm, gp), fR = cadd(rm, rp);
+        IkG = cadd(IkG, cscale(GK_WK[i], fG));
+        IkR = cadd(IkR, cscale(GK_WK[i], fR));
+        if (i & 1) {
+            IgG = cadd(IgG, cscale(GK_WG[i >> 1], fG));
+            IgR = cadd(IgR, cscale(GK_WG[i >> 1], fR));
+        }
+    }
+    IkG = cscale(d, IkG); IgG = cscale(d, IgG);
+    IkR = cscale(d, IkR); IgR = cscale(d, IgR);
+    EG = hypot(IkG.x - IgG.x, IkG.y - IgG.y);
+    ER = hypot(IkR.x - IgR.x, IkR.y - IgR.y);
+    IG = IkG;
+    IR = IkR;
 }
 
 // GK(7,15) of both integrands on [a, b] with shared nodes; same accumulation order as gk15.
@@ -466,6 +528,200 @@ __device__ bool hquad_from(double r, double z, double rtol, double atol, cplx I0
     for (int q = 0; q < nb; q++) s = cadd(s, bi[q]);
     out = s;
     return ok;
+}
+
+// The first interval [0, pi/2] of both quadratures (estimates G, Gr, errors EG, ER).  Returns true
+// when both already meet the termination rule of hquad_from (E <= max(rtol |I|, atol)): G, Gr are
+// then the final values, bitwise those of hquad_pair.
+__device__ __forceinline__ bool hquad_first(const double* cosn, double r, double z, double rtol, double atol, cplx& G,
+                                            cplx& Gr, double& EG, double& ER, QuadWork& wk) {
+    gk15_both_first(cosn, r, z, G, EG, Gr, ER, wk);
+    return !(EG > fmax(rtol * hypot(G.x, G.y), atol)) && !(ER > fmax(rtol * hypot(Gr.x, Gr.y), atol));
+}
+
+// ---- wave-cooperative adaptive quadrature (one pair per 32-lane group) ----
+// Group sums over the 32 lanes of a group (width 32) or over its 16-lane halves (width 16).
+template <int W> __device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+    for (int off = W / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, W);
+    return v;
+}
+template <int W> __device__ __forceinline__ cplx group_sum(cplx v) { return cmk(group_sum<W>(v.x), group_sum<W>(v.y)); }
+
+// Kronrod and Gauss weights of node q of the GK(7,15) ordering used here: q = 7 the centre,
+// q < 7 at c - d x_q, q = 8 + i at c + d x_i
+__device__ __forceinline__ void gk_weights(int q, double& wk, double& wg) {
+    const int i = q == 7 ? 7 : (q < 7 ? q : q - 8);
+    wk = GK_WK[i];
+    wg = i == 7 ? GK_WG[3] : ((i & 1) ? GK_WG[i >> 1] : 0.0);
+}
+
+// Interval list of one group in LDS
+struct QuadBoxes {
+    double a[MHX_MAXBOX], b[MHX_MAXBOX], e[MHX_MAXBOX];
+    cplx i[MHX_MAXBOX];
+};
+
+// One bisection of [a, b] on a group: the 15 nodes of [a, m] on lanes 0-14 and those of [m, b] on
+// lanes 16-30 (nodes as gk15 places them), the GK sums of each half by shuffles within its 16
+// lanes.  BOTH: both integrands from one E1 evaluation per node (del_integrand_both); otherwise
+// the one of DERIV.  Results (I1, E1 of [a, m]; I2, E2 of [m, b]) are the same on every lane.
+template <int MODE>  // 0: G, 1: dG/dr, 2: both
+__device__ __forceinline__ void group_bisect(int gl, double a, double b, double r, double z, cplx* I1, double* E1,
+                                             cplx* I2, double* E2, QuadWork& wk) {
+    const int half = gl >> 4, q = gl & 15;
+    double wkq = 0.0, wgq = 0.0;
+    if (q < 15) gk_weights(q, wkq, wgq);
+    const double xq = q == 7 ? 0.0 : GK_X[q < 7 ? q : (q < 15 ? q - 8 : 0)];
+    const double w = (b - a) / 2, m = a + w;
+    const double ha = half ? m : a, hb = half ? b : m;
+    const double c = (ha + hb) * 0.5, d = (hb - ha) * 0.5;
+    const double th = q == 7 ? c : (q < 7 ? c - d * xq : c + d * xq);
+    constexpr int NI = MODE == 2 ? 2 : 1;
+    cplx f[NI];
+#pragma unroll
+    for (int t = 0; t < NI; t++) f[t] = cmk(0.0, 0.0);
+    if (q < 15) {
+        if (MODE == 2) del_integrand_both(th, r, z, wk, f[0], f[1]);
+        else f[0] = del_integrand<MODE == 1>(th, r, z, wk);
+    }
+#pragma unroll
+    for (int t = 0; t < NI; t++) {
+        const cplx Ik = cscale(d, group_sum<16>(cscale(wkq, f[t]))), Ig = cscale(d, group_sum<16>(cscale(wgq, f[t])));
+        const double Eh = hypot(Ik.x - Ig.x, Ik.y - Ig.y);
+        I1[t] = cmk(__shfl(Ik.x, 0, 32), __shfl(Ik.y, 0, 32));
+        I2[t] = cmk(__shfl(Ik.x, 16, 32), __shfl(Ik.y, 16, 32));
+        E1[t] = __shfl(Eh, 0, 32);
+        E2[t] = __shfl(Eh, 16, 32);
+    }
+}
+
+// hquad_from on one group, continued from a list of nb0 intervals that lane 0 has written to B
+// (totals It, Et): the same rule (pop the interval of largest error, first index on ties, the
+// last interval moved into its place; bisect; stop when E <= max(rtol |I|, atol); re-sum in list
+// order), each bisection by group_bisect.  gl: lane in the group; every lane returns the same
+// values.
+template <bool DERIV>
+__device__ bool hquad_group_loop(QuadBoxes& B, int gl, double r, double z, double rtol, double atol, int nb0, cplx It,
+                                 double Et, cplx& out, QuadWork& wk) {
+    int nb = nb0;
+    bool ok = true;
+    __asm__ volatile("" ::: "memory");
+    while (Et > fmax(rtol * hypot(It.x, It.y), atol)) {
+        if (nb >= MHX_MAXBOX - 1) { ok = false; break; }
+        // interval of largest error (first index on ties)
+        double bv = -1.0;
+        int bi = 0x7fffffff;
+        for (int t = gl; t < nb; t += 32)
+            if (B.e[t] > bv) { bv = B.e[t]; bi = t; }
+#pragma unroll
+        for (int off = 16; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(bv, off, 32);
+            const int oi = __shfl_xor(bi, off, 32);
+            if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; }
+        }
+        const int im = bi < nb ? bi : 0;  // no finite maximum (NaN errors): the first interval
+        const double a = B.a[im], b = B.b[im], e = B.e[im];
+        const cplx iv = B.i[im];
+        const double la = B.a[nb - 1], lb = B.b[nb - 1], le = B.e[nb - 1];
+        const cplx li = B.i[nb - 1];
+        __asm__ volatile("" ::: "memory");
+        if (gl == 0) { B.a[im] = la; B.b[im] = lb; B.e[im] = le; B.i[im] = li; }
+        nb--;
+        It = csub(It, iv);
+        Et -= e;
+        cplx I1, I2;
+        double E1, E2;
+        group_bisect<DERIV ? 1 : 0>(gl, a, b, r, z, &I1, &E1, &I2, &E2, wk);
+        const double m = a + (b - a) / 2;
+        It = cadd(It, cadd(I1, I2));
+        Et += E1 + E2;
+        __asm__ volatile("" ::: "memory");
+        if (gl == 0) {
+            B.a[nb] = a; B.b[nb] = m; B.i[nb] = I1; B.e[nb] = E1;
+            B.a[nb + 1] = m; B.b[nb + 1] = b; B.i[nb + 1] = I2; B.e[nb + 1] = E2;
+        }
+        nb += 2;
+        __asm__ volatile("" ::: "memory");
+    }
+    cplx s = cmk(0.0, 0.0);
+    if (gl == 0)
+        for (int t = 0; t < nb; t++) s = cadd(s, B.i[t]);
+    out = cmk(__shfl(s.x, 0, 32), __shfl(s.y, 0, 32));
+    __asm__ volatile("" ::: "memory");
+    return ok;
+}
+
+// hquad_from on one group from the first interval's estimate (I0, E)
+template <bool DERIV>
+__device__ __forceinline__ bool hquad_from_group(QuadBoxes& B, int gl, double r, double z, double rtol, double atol,
+                                                 cplx I0, double E, cplx& out, QuadWork& wk) {
+    if (gl == 0) {
+        B.a[0] = 0.0; B.b[0] = MHX_PI / 2; B.i[0] = I0; B.e[0] = E;
+    }
+    return hquad_group_loop<DERIV>(B, gl, r, z, rtol, atol, 1, I0, E, out, wk);
+}
+
+// hquad_from's state after its first step on [0, pi/2] (interval 0 popped, its halves pushed):
+// the list {[0, m]: I1, E1; [m, pi/2]: I2, E2} and the totals as hquad_from forms them
+template <bool DERIV>
+__device__ __forceinline__ bool hquad_group_after_first(QuadBoxes& B, int gl, double r, double z, double rtol,
+                                                        double atol, cplx I0, double E, cplx I1, double E1, cplx I2,
+                                                        double E2, cplx& out, QuadWork& wk) {
+    const double a = 0.0, b = MHX_PI / 2, m = a + (b - a) / 2;
+    if (gl == 0) {
+        B.a[0] = a; B.b[0] = m; B.i[0] = I1; B.e[0] = E1;
+        B.a[1] = m; B.b[1] = b; B.i[1] = I2; B.e[1] = E2;
+    }
+    double Et = E;
+    Et -= E;
+    Et += E1 + E2;
+    return hquad_group_loop<DERIV>(B, gl, r, z, rtol, atol, 2, cadd(csub(I0, I0), cadd(I1, I2)), Et, out, wk);
+}
+
+// The refinement of one pair on a group, from the first interval's estimates of both quadratures
+// (IG, EG of G; IR, ER of dG/dr).  A quadrature that already meets the termination rule keeps
+// its estimate; when both need refinement, their common first bisection evaluates both
+// integrands at once.  The results are those of hquad_pair.
+__device__ __forceinline__ bool refine_pair_group(QuadBoxes& B, int gl, double r, double z, double rtol, double atol,
+                                                  cplx IG, double EG, cplx IR, double ER, cplx& G, cplx& Gr,
+                                                  QuadWork& wk) {
+    const bool needG = EG > fmax(rtol * hypot(IG.x, IG.y), atol);
+    const bool needR = ER > fmax(rtol * hypot(IR.x, IR.y), atol);
+    bool okG = true, okR = true;
+    G = IG;
+    Gr = IR;
+    if (needG && needR) {
+        cplx I1[2], I2[2];
+        double E1[2], E2[2];
+        group_bisect<2>(gl, 0.0, MHX_PI / 2, r, z, I1, E1, I2, E2, wk);
+        okR = hquad_group_after_first<true>(B, gl, r, z, rtol, atol, IR, ER, I1[1], E1[1], I2[1], E2[1], Gr, wk);
+        okG = hquad_group_after_first<false>(B, gl, r, z, rtol, atol, IG, EG, I1[0], E1[0], I2[0], E2[0], G, wk);
+    } else if (needR) {
+        okR = hquad_from_group<true>(B, gl, r, z, rtol, atol, IR, ER, Gr, wk);
+    } else if (needG) {
+        okG = hquad_from_group<false>(B, gl, r, z, rtol, atol, IG, EG, G, wk);
+    }
+    return okG && okR;
+}
+
+// The first interval [0, pi/2] of both quadratures on a group: the 15 nodes on lanes 0-14
+// (cosines from cosn)
+__device__ __forceinline__ void first_interval_group(int gl, const double* cosn, double r, double z, cplx& IG,
+                                                     double& EG, cplx& IR, double& ER, QuadWork& wk) {
+    const double d = (MHX_PI / 2 - 0.0) * 0.5;
+    cplx fG = cmk(0.0, 0.0), fR = cmk(0.0, 0.0);
+    double wkq = 0.0, wgq = 0.0;
+    if (gl < 15) {
+        gk_weights(gl, wkq, wgq);
+        del_integrand_both_c(cosn[gl], r, z, wk, fG, fR);
+    }
+    IG = cscale(d, group_sum<32>(cscale(wkq, fG)));
+    const cplx IgG = cscale(d, group_sum<32>(cscale(wgq, fG)));
+    IR = cscale(d, group_sum<32>(cscale(wkq, fR)));
+    const cplx IgR = cscale(d, group_sum<32>(cscale(wgq, fR)));
+    EG = hypot(IG.x - IgG.x, IG.y - IgG.y);
+    ER = hypot(IR.x - IgR.x, IR.y - IgR.y);
 }
 
 // G and dG/dr of the exact wave term (both without the factor 4)

@@ -27,11 +27,14 @@ void launch_rankine(hipStream_t st, int n, const PanelGeom* geom, int direct, do
 void launch_combine_wu(hipStream_t st, int n, const PanelGeom* geom, const double* SR, const double* DR,
                        const double* ks, int batch, int direct, int include_identity, cplx* S, cplx* D,
                        int64_t sMat);
-// exact Guevel-Delhommeau pair table (G, dG/dr) for i <= j mirrored to j > i.
+// exact Guevel-Delhommeau pair table (G, dG/dr) for i <= j mirrored to j > i, in two passes
+// (first interval for every pair; the adaptive refinement for the pairs listed in rlist, which
+// holds rcap >= batch n (n + 1) / 2 entries; *rcount is cleared here).
 // status[0]: nonzero if an interval budget was exhausted; status[1..3] += integrand
 // evaluations, power-series terms, continued-fraction steps (status is not cleared here)
 void launch_delhommeau_table(hipStream_t st, int n, const PanelGeom* geom, const double* ks, int batch, double rtol,
-                             double atol, cplx* WG, cplx* WR, int64_t sMat, unsigned long long* status);
+                             double atol, cplx* WG, cplx* WR, int64_t sMat, unsigned long long* rlist,
+                             unsigned long long* rcount, int64_t rcap, unsigned long long* status);
 // S, D from a pair table of (G, dG/dr) (Delhommeau or surrogate)
 void launch_combine_table(hipStream_t st, int n, const PanelGeom* geom, const double* SR, const double* DR,
                           const double* ks, int batch, int direct, int include_identity, const cplx* WG,
