@@ -273,7 +273,7 @@ int mhx_hydro_coeffs_vjp(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const
 
 /* Learned surrogate of the exact Guevel-Delhommeau wave term (no counterpart in the reference; it
  * replaces the exact term of src/green_functions/exact_Guevel_Delhommeau.jl:15-59 inside assembly).
- * The network maps (r, v) = (k R_h, k |z + zeta|) through 64 Fourier features and 4 x 128 SiLU
+ * The network maps (r, v) = (k R_h, k |z + zeta|) through 64 Fourier features and 4 x 128 ReLU
  * layers to the residual (exact - GFWu) of G, dG/dr, dG/dz: real parts as 100 x residual,
  * imaginary parts as 100 exp(min(v, 5)) x residual (they decay like exp(-v)).  It is trained on
  * the device on ngrid^2 log-spaced jittered samples generated by the exact FP64 kernel.

@@ -297,7 +297,7 @@ int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int 
             if (rc) return rc;
             out.Df = b->Df;
         }
-        HCHK(hipMemsetAsync(b->status, 0, 4 * sizeof(unsigned long long), ctx->stream));
+        mhx::launch_zero(ctx->stream, b->status, 4 * sizeof(unsigned long long));
         {
             Phase ph(ctx, "wave_table");
             mhx::launch_surr_table(ctx->stream, b->n, b->geom, b->d_ks, batch, ctx->surr_W, ctx->surr_bias,
@@ -312,7 +312,7 @@ int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int 
     } else {
         int rc;
         if ((rc = ensure_pair_list(ctx, b, batch))) return rc;
-        HCHK(hipMemsetAsync(b->status, 0, 4 * sizeof(unsigned long long), ctx->stream));
+        mhx::launch_zero(ctx->stream, b->status, 4 * sizeof(unsigned long long));
         {
             Phase ph(ctx, "wave_table");
             mhx::launch_delhommeau_table(ctx->stream, b->n, b->geom, b->d_ks, batch, 1e-8, 1e-8, b->WG, b->WR, b->nn(),
@@ -398,7 +398,7 @@ int getrf_fp64(mhx_ctx* ctx, mhx_body* b, int batch, mhx::LaunchTimer* lt) {
 // trans = 2 then solves with D^H.
 int solve_fp64(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, bool form_phi = true, int trans = 0) {
     const int n = b->n;
-    HCHK(hipMemsetAsync(b->info, 0, sizeof(int) * batch, ctx->stream));
+    mhx::launch_zero(ctx->stream, b->info, sizeof(int) * batch);
     {
         Phase ph(ctx, "lu");
         mhx::LaunchTimer lt;
@@ -474,7 +474,7 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf,
     const bool split = use_split(ctx, gf);
     if (split && (rc = dalloc(ctx, &b->Ps, (size_t)mhx::split_work(n) * b->cap_batch))) return rc;
     const int refine_max = split ? kRefineMaxSplit : kRefineMax;
-    HCHK(hipMemsetAsync(b->info, 0, sizeof(int) * batch, st));
+    mhx::launch_zero(st, b->info, sizeof(int) * batch);
     {
         Phase ph(ctx, "lu");
         if (!b->df_ready) mhx::convert_to_f32(st, b->nn() * batch, b->D, b->Df);
@@ -523,7 +523,7 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf,
         std::vector<double> nr(2 * batch);
         for (int it = 0; it <= refine_max; it++) {
             mhx::launch_matmul_nr(st, n, R, b->D, b->nn(), X, b->nR(), b->Tr, b->nR(), b->work, b->sW(), batch);
-            HCHK(hipMemsetAsync(b->norms, 0, sizeof(double) * 2 * batch, st));
+            mhx::launch_zero(st, b->norms, sizeof(double) * 2 * batch);
             mhx::launch_residual(st, (int64_t)n * R, b->Rr, b->nR(), b->Tr, b->nR(), b->norms, batch);
             HCHK(hipMemcpyAsync(nr.data(), b->norms, sizeof(double) * 2 * batch, hipMemcpyDeviceToHost, st));
             HCHK(hipStreamSynchronize(st));
@@ -1402,7 +1402,7 @@ int mhx_hydro_coeffs_jvp(mhx_ctx* ctx, const mhx_mesh* mesh, const mhx_mesh_tang
     // mesh and tangent fields on the device (a null tangent array is zero)
     auto up = [&](double* dst, const double* src, size_t cnt) {
         if (src) hipMemcpyAsync(dst, src, sizeof(double) * cnt, hipMemcpyHostToDevice, st);
-        else hipMemsetAsync(dst, 0, sizeof(double) * cnt, st);
+        else mhx::launch_zero(st, dst, sizeof(double) * cnt);
     };
     up(dv, mesh->vertices, 3 * nv);
     up(dc, mesh->centers, 3 * (size_t)n);
@@ -1568,11 +1568,11 @@ int mhx_hydro_coeffs_vjp(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const
     hipMemcpyAsync(dn, mesh->normals, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st);
     hipMemcpyAsync(da, mesh->areas, sizeof(double) * n, hipMemcpyHostToDevice, st);
     hipMemcpyAsync(dr, mesh->radii, sizeof(double) * n, hipMemcpyHostToDevice, st);
-    hipMemsetAsync(gV, 0, sizeof(double) * 3 * nv, st);
-    hipMemsetAsync(gC, 0, sizeof(double) * 3 * n, st);
-    hipMemsetAsync(gN, 0, sizeof(double) * 3 * n, st);
-    hipMemsetAsync(gA, 0, sizeof(double) * n, st);
-    hipMemsetAsync(gD, 0, sizeof(double) * 3 * n * nd1, st);
+    mhx::launch_zero(st, gV, sizeof(double) * 3 * nv);
+    mhx::launch_zero(st, gC, sizeof(double) * 3 * n);
+    mhx::launch_zero(st, gN, sizeof(double) * 3 * n);
+    mhx::launch_zero(st, gA, sizeof(double) * n);
+    mhx::launch_zero(st, gD, sizeof(double) * 3 * n * nd1);
     if (nb) hipMemcpyAsync(b->betas, betas, sizeof(double) * nb, hipMemcpyHostToDevice, st);
     const mhx::AdjointGrad gr{gV, gC, gN, gA, gD, wk};
     std::vector<cplx> F((size_t)ndof * R), FK((size_t)ndof * nb1), hFb(nd1 * R), hFKb(nd1 * nb1);
@@ -1622,7 +1622,7 @@ int mhx_hydro_coeffs_vjp(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const
         const int64_t cnt = (int64_t)n * R;
         hipMemcpyAsync(Fb, hFb.data(), sizeof(cplx) * nd1 * R, hipMemcpyHostToDevice, st);
         hipMemcpyAsync(FKb, hFKb.data(), sizeof(cplx) * nd1 * nb1, hipMemcpyHostToDevice, st);
-        hipMemsetAsync(wk, 0, 2 * sizeof(double), st);
+        mhx::launch_zero(st, wk, 2 * sizeof(double));
         {
             Phase ph(ctx, "adjoint");
             mhx::launch_adjoint_forces(st, n, b->geom, b->dofs, ndof, b->Y, R, Fb, FKb, ctx->rho, w, k, g, b->betas,
@@ -2199,7 +2199,8 @@ int mhx_surrogate_save(mhx_ctx* ctx, const char* path) {
     if (!f) return fail(ctx, MHX_EINVAL, std::string("surrogate_save: cannot open ") + path);
     char hdr[512];
     std::snprintf(hdr, sizeof hdr,
-                  "{\"format\": \"mhx-surrogate\", \"version\": 1, \"params\": %lld, \"fourier\": %d, \"outputs\": %d, "
+                  "{\"format\": \"mhx-surrogate\", \"version\": 2, \"activation\": \"relu\", \"params\": %lld, "
+                  "\"fourier\": %d, \"outputs\": %d, "
                   "\"scale\": %.1f, \"r_max\": %.1f, \"v_min\": %g, \"v_max\": %.1f, \"steps\": %lld, "
                   "\"val_mse\": %.6e}\n",
                   (long long)P, SURR_NF, SURR_NOUT, SURR_SCALE, SURR_RMAX, SURR_VMIN, SURR_VMAX,
@@ -2222,9 +2223,13 @@ int mhx_surrogate_load(mhx_ctx* ctx, const char* path) {
     bool ok = std::fgets(line, sizeof line, f) && std::strcmp(line, "MHXSURR1\n") == 0 && std::fgets(line, sizeof line, f);
     long long pc = -1, steps = 0;
     double vl = -1.0;
+    bool act_ok = false;
     if (ok) {
         const char* q = std::strstr(line, "\"params\": ");
         ok = q && std::sscanf(q + 10, "%lld", &pc) == 1 && pc == P;
+        // the hidden layers' activation the weights were trained with (files without the field
+        // hold a network with another activation)
+        act_ok = std::strstr(line, "\"activation\": \"relu\"") != nullptr;
         const char* q2 = std::strstr(line, "\"steps\": ");
         if (q2) std::sscanf(q2 + 9, "%lld", &steps);
         const char* q3 = std::strstr(line, "\"val_mse\": ");
@@ -2234,6 +2239,9 @@ int mhx_surrogate_load(mhx_ctx* ctx, const char* path) {
          std::fread(hp.data(), sizeof(float), hp.size(), f) == hp.size();
     std::fclose(f);
     if (!ok) return fail(ctx, MHX_EINVAL, std::string("surrogate_load: not an mhx surrogate file: ") + path);
+    if (!act_ok)
+        return fail(ctx, MHX_EINVAL, std::string("surrogate_load: the network in ") + path +
+                                         " was not trained with ReLU hidden layers; train it again");
     hipSetDevice(ctx->device);
     int rc;
     if ((rc = surr_alloc(ctx))) return rc;

@@ -3,14 +3,14 @@
 // The network maps the reduced coordinates (r, v) = (k R_h, k |z + zeta|) to six
 // outputs: 100 x the residual (exact - Liang-Wu-Noblesse fit) of Re/Im G, Re/Im dG/dr
 // and Re/Im dG/dz.  Architecture: 2 inputs -> 64 Fourier features
-// [sin(2 pi B u), cos(2 pi B u)] of u = (log-scaled r, log-scaled v) -> 4 x 128 SiLU
+// [sin(2 pi B u), cos(2 pi B u)] of u = (log-scaled r, log-scaled v) -> 4 x 128 ReLU
 // -> 6 linear outputs.  The fit it corrects is the reference's GFWu
 // (src/green_functions/wu.jl); the exact term is exact_Guevel_Delhommeau.jl:15-59.
 //
 //  * samples: ngrid^2 points on a log-spaced grid with seeded jitter, targets from the
 //    exact FP64 kernel (hquad_pair) and the Wu fit, on the device; imaginary residuals
 //    are scaled by exp(min(v, 5)) (the imaginary parts decay like exp(-v));
-//  * training: FP32 kernels (tiled GEMMs with fused bias/SiLU epilogues, split-K
+//  * training: FP32 kernels (tiled GEMMs with fused bias/ReLU epilogues, split-K
 //    weight gradients, MSE loss, Adam);
 //  * inference: BF16 weights and activations on v_mfma_f32_32x32x8bf16_1k with FP32
 //    accumulation, fused into the pair-table assembly; pairs outside the trained box
@@ -119,18 +119,16 @@ void launch_surr_features(hipStream_t st, int M, const int* idx, const float* U,
     surr_features_kernel<<<(M * SURR_NF + 255) / 256, 256, 0, st>>>(M, idx, U, Bf, F);
 }
 
-__device__ __forceinline__ float silu_f(float z) { return z / (1.0f + expf(-z)); }
-__device__ __forceinline__ float dsilu_f(float z) {
-    const float s = 1.0f / (1.0f + expf(-z));
-    return s * (1.0f + z * (1.0f - s));
-}
+// hidden-layer activation (ReLU) and its derivative (0 at z = 0)
+__device__ __forceinline__ float act_f(float z) { return fmaxf(z, 0.0f); }
+__device__ __forceinline__ float dact_f(float z) { return z > 0.0f ? 1.0f : 0.0f; }
 
 // C[M x N] = sum_k A(m, k) B(k, n).  A(m,k) = TA ? A[k*lda+m] : A[m*lda+k];
 // B(k,n) = TB ? B[n*ldb+k] : B[k*ldb+n].  Epilogues:
 //  EPI 0: C = acc (split-K partial: C + blockIdx.z * M*N, k range split over gridDim.z)
-//  EPI 1: Z = acc + bias[n], H = silu(Z)   (C = Z, C2 = H)
+//  EPI 1: Z = acc + bias[n], H = relu(Z)   (C = Z, C2 = H)
 //  EPI 2: Z = acc + bias[n]                 (C = Z)
-//  EPI 3: C = acc * silu'(Zp[m*N+n])        (backward through the activation)
+//  EPI 3: C = acc * relu'(Zp[m*N+n])        (backward through the activation)
 template <bool TA, bool TB, int EPI>
 __global__ void __launch_bounds__(256) sgemm_kernel(int M, int N, int K, const float* __restrict__ A, int lda,
                                                     const float* __restrict__ B, int ldb, float* __restrict__ C,
@@ -186,11 +184,11 @@ __global__ void __launch_bounds__(256) sgemm_kernel(int M, int N, int K, const f
             } else if (EPI == 1) {
                 const float z = acc[i][j] + bias[nn];
                 C[o] = z;
-                C2[o] = silu_f(z);
+                C2[o] = act_f(z);
             } else if (EPI == 2) {
                 C[o] = acc[i][j] + bias[nn];
             } else {
-                C[o] = acc[i][j] * dsilu_f(Zp[o]);
+                C[o] = acc[i][j] * dact_f(Zp[o]);
             }
         }
 }
@@ -308,7 +306,7 @@ void surr_forward_backward(hipStream_t st, SurrTrainWork& w, int M, const int* i
         sum_parts_kernel<<<(nW + 255) / 256, 256, 0, st>>>(nW, splits, w.part, gW);
         colsum_kernel<<<OUT_DIM[l], 256, 0, st>>>(M, OUT_DIM[l], w.dz[l], gb);
         if (l > 0) {
-            // dz[l-1] = (dz[l] W_l) * silu'(z[l-1])
+            // dz[l-1] = (dz[l] W_l) * relu'(z[l-1])
             const float* W = params + OFF_W[l];
             SgemmArgs d{M, IN_DIM[l], OUT_DIM[l], w.dz[l], OUT_DIM[l], W, IN_DIM[l], w.dz[l - 1], nullptr, nullptr,
                         w.z[l - 1]};
@@ -380,13 +378,6 @@ __device__ __forceinline__ int surr_slot_input(int l, int kb, int h, int e) {
 }
 
 // FP32 parameters -> packed BF16 units (layout above) and FP32 biases [l][128] + output[8]
-// Inference works on y = -log2(e) z instead of the pre-activation z: the hidden layers'
-// biases, and the first layer's weights, carry the factor -log2(e), the activation is
-// h' = y / (1 + 2^y) = -log2(e) SiLU(z) (one v_exp_f32 with no scaling multiply), and the
-// next layer's weights would undo the factor (-1 / log2(e)) while also applying their own
-// (-log2(e)): the middle layers' weights are unchanged and only the output layer's carry
-// -1 / log2(e).
-constexpr float kLog2e = 1.4426950408889634f;
 __global__ void surr_pack_kernel(const float* __restrict__ params, unsigned short* __restrict__ Wb,
                                  float* __restrict__ bias) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -401,14 +392,14 @@ __global__ void surr_pack_kernel(const float* __restrict__ params, unsigned shor
         const int in = surr_slot_input(l, kb, h, e);
         float v = 0.f;
         if (row < OUT_DIM_D(l)) v = params[OFF_W_D(l) + (int64_t)row * IN_DIM_D(l) + in];
-        Wb[t] = f2bf(v * (l == 0 ? -kLog2e : l == 4 ? -1.0f / kLog2e : 1.0f));
+        Wb[t] = f2bf(v);
     }
     if (t < 4 * 128 + 8) {
         const int l = t < 512 ? t / 128 : 4;
         const int r = t - 128 * l;
         float v = 0.f;
         if (r < OUT_DIM_D(l)) v = params[OFF_W_D(l) + (int64_t)OUT_DIM_D(l) * IN_DIM_D(l) + r];
-        bias[t] = l < 4 ? -kLog2e * v : v;
+        bias[t] = v;
     }
 }
 
@@ -426,49 +417,67 @@ __device__ __forceinline__ void surr_load_lds(SurrLds& L, const unsigned short* 
     __syncthreads();
 }
 
-// -log2(e) SiLU(z) from y = -log2(e) z (2^y overflows to inf for large y: h' -> 0)
-__device__ __forceinline__ float silu_log2(float y) { return y * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(y)); }
-
 // one hidden layer on the two column tiles: act (B operands, KB input blocks) -> act (8 blocks).
-// Output neuron tiles are done one at a time so that only 32 accumulators are live.
+// Software-pipelined over the four output neuron tiles: the MFMAs of tile t + 1 are issued
+// between the activation instructions of tile t, so the matrix cores and the VALU work together
+// within one wave.
+template <int KB>
+__device__ __forceinline__ void surr_mfma_tile(const bf16x8* W, const float* bias, int t, int lane, int h,
+                                               const bf16x8 act[2][8], f32x16& acc0, f32x16& acc1) {
+#pragma unroll
+    for (int g = 0; g < 4; g++) {
+        const float4 b = *reinterpret_cast<const float4*>(bias + 32 * t + 8 * g + 4 * h);
+        acc0[4 * g + 0] = acc1[4 * g + 0] = b.x;
+        acc0[4 * g + 1] = acc1[4 * g + 1] = b.y;
+        acc0[4 * g + 2] = acc1[4 * g + 2] = b.z;
+        acc0[4 * g + 3] = acc1[4 * g + 3] = b.w;
+    }
+#pragma unroll
+    for (int kb = 0; kb < KB; kb++) {
+        const bf16x8 a = W[(t * KB + kb) * 64 + lane];
+        acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, act[0][kb], acc0, 0, 0, 0);
+        acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, act[1][kb], acc1, 0, 0, 0);
+    }
+}
+
+// ReLU of neuron tile t's accumulators into the B operands of the next layer
+__device__ __forceinline__ void surr_act_tile(int t, const f32x16& acc0, const f32x16& acc1, bf16x8 nxt[2][8]) {
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+        bf16x8 v0, v1;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int q = 4 * (2 * b + (e >> 2)) + (e & 3);
+            v0[e] = (__bf16)fmaxf(acc0[q], 0.0f);
+            v1[e] = (__bf16)fmaxf(acc1[q], 0.0f);
+        }
+        nxt[0][2 * t + b] = v0;
+        nxt[1][2 * t + b] = v1;
+    }
+}
+
 template <int KB>
 __device__ __forceinline__ void surr_hidden(const SurrLds& L, int layer, int lane, bf16x8 act[2][8]) {
     const int h = lane >> 5;
     const bf16x8* W = L.w + sw_layer_off(layer);
     const float* bias = L.bias + 128 * layer;
     bf16x8 nxt[2][8];
-#pragma unroll
-    for (int t = 0; t < 4; t++) {
-        // keep the scheduler from hoisting the next tile's LDS loads (register pressure)
-        __builtin_amdgcn_sched_barrier(0);
-        f32x16 acc0, acc1;
-#pragma unroll
-        for (int g = 0; g < 4; g++) {
-            const float4 b = *reinterpret_cast<const float4*>(bias + 32 * t + 8 * g + 4 * h);
-            acc0[4 * g + 0] = acc1[4 * g + 0] = b.x;
-            acc0[4 * g + 1] = acc1[4 * g + 1] = b.y;
-            acc0[4 * g + 2] = acc1[4 * g + 2] = b.z;
-            acc0[4 * g + 3] = acc1[4 * g + 3] = b.w;
-        }
-#pragma unroll
-        for (int kb = 0; kb < KB; kb++) {
-            const bf16x8 a = W[(t * KB + kb) * 64 + lane];
-            acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, act[0][kb], acc0, 0, 0, 0);
-            acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, act[1][kb], acc1, 0, 0, 0);
-        }
-#pragma unroll
-        for (int b = 0; b < 2; b++) {
-            bf16x8 v0, v1;
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-                const int q = 4 * (2 * b + (e >> 2)) + (e & 3);
-                v0[e] = (__bf16)silu_log2(acc0[q]);
-                v1[e] = (__bf16)silu_log2(acc1[q]);
-            }
-            nxt[0][2 * t + b] = v0;
-            nxt[1][2 * t + b] = v1;
-        }
-    }
+    f32x16 a0, a1, b0, b1;
+    // scheduling regions {MFMAs of tile t + 1, activations of tile t}; the barriers keep a later
+    // region's LDS loads from being hoisted, which would raise the register pressure
+    __builtin_amdgcn_sched_barrier(0);
+    surr_mfma_tile<KB>(W, bias, 0, lane, h, act, a0, a1);
+    __builtin_amdgcn_sched_barrier(0);
+    surr_mfma_tile<KB>(W, bias, 1, lane, h, act, b0, b1);
+    surr_act_tile(0, a0, a1, nxt);
+    __builtin_amdgcn_sched_barrier(0);
+    surr_mfma_tile<KB>(W, bias, 2, lane, h, act, a0, a1);
+    surr_act_tile(1, b0, b1, nxt);
+    __builtin_amdgcn_sched_barrier(0);
+    surr_mfma_tile<KB>(W, bias, 3, lane, h, act, b0, b1);
+    surr_act_tile(2, a0, a1, nxt);
+    __builtin_amdgcn_sched_barrier(0);
+    surr_act_tile(3, b0, b1, nxt);
 #pragma unroll
     for (int kb = 0; kb < 8; kb++) {
         act[0][kb] = nxt[0][kb];
@@ -678,7 +687,7 @@ void launch_surr_table(hipStream_t st, int n, const PanelGeom* geom, const doubl
                        const unsigned short* Wb, const float* bias, const float* Bf, const SurrOut& out,
                        unsigned long long* fb_list, unsigned long long* fb_count, int64_t fb_cap,
                        unsigned long long* status) {
-    hipMemsetAsync(fb_count, 0, sizeof(unsigned long long), st);
+    launch_zero(st, fb_count, sizeof(unsigned long long));
     const int nb = (n + 7) / 8;
     const int64_t units = (int64_t)nb * (nb + 1) / 2 * batch;
     const int64_t blocks = std::min<int64_t>(device_cus(), (units + SURR_WAVES - 1) / SURR_WAVES);

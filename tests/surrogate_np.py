@@ -4,7 +4,7 @@ Layout of the flat parameter vector (as libmhx stores it): for each of the five
 layers, W (out x in, row-major) followed by b (out); layer widths
 64 -> 128 -> 128 -> 128 -> 128 -> 6.  Inputs u (m x 2) map to features
 [sin(2 pi u B^T), cos(2 pi u B^T)] with the 32 x 2 Fourier matrix B; hidden
-layers use SiLU; the loss is the mean of squared errors over m x 6 outputs.
+layers use ReLU; the loss is the mean of squared errors over m x 6 outputs.
 """
 import numpy as np
 
@@ -35,13 +35,12 @@ def inputs(r, v):
     return np.stack([u1, u2], 1)
 
 
-def silu(z):
-    return z / (1.0 + np.exp(-z))
+def relu(z):
+    return np.maximum(z, 0.0)
 
 
-def dsilu(z):
-    s = 1.0 / (1.0 + np.exp(-z))
-    return s * (1.0 + z * (1.0 - s))
+def drelu(z):
+    return (z > 0.0).astype(z.dtype)
 
 
 def forward(params, B, U, keep=False):
@@ -52,7 +51,7 @@ def forward(params, B, U, keep=False):
     for l, (W, b) in enumerate(layers):
         z = h @ W.T + b
         zs.append(z)
-        h = silu(z) if l < 4 else z
+        h = relu(z) if l < 4 else z
         hs.append(h)
     return (h, hs, zs) if keep else h
 
@@ -70,7 +69,7 @@ def loss_grad(params, B, U, Y):
         W, _ = layers[l]
         grads[l] = (dz.T @ hs[l], dz.sum(0))
         if l > 0:
-            dz = (dz @ W) * dsilu(zs[l - 1])
+            dz = (dz @ W) * drelu(zs[l - 1])
     g = np.concatenate([np.concatenate([gW.ravel(), gb]) for gW, gb in grads])
     return loss, g
 

@@ -50,6 +50,7 @@ def test_shipped_weights_format():
         hdr = json.loads(f.readline().decode())
         body = f.read()
     assert hdr["format"] == "mhx-surrogate" and hdr["params"] == snp.NPARAMS and hdr["outputs"] == 6
+    assert hdr["version"] == 2 and hdr["activation"] == "relu"
     assert len(body) == 4 * (64 + snp.NPARAMS)
     vals = np.frombuffer(body, dtype=np.float32)
     assert np.all(np.isfinite(vals))
