@@ -954,20 +954,21 @@ __device__ __forceinline__ bf16x8_t bf16x8_neg(bf16x8_t v) {
 }
 
 // C[0:m, 0:m] -= L21 U12 from the split arrays.  Workgroup tile 128 x 128 of C, four waves of
-// 64 x 64 (2 x 2 accumulator tiles of 32 x 32 per product sign class); k in chunks of 32 staged
-// through LDS (rows padded to 80 B so the eight lanes of a 16-byte read phase hit distinct banks).
-constexpr int SPLIT_KC = 32;
-constexpr int SPLIT_LDS_ROW = SPLIT_KC / 8 + 1;  // bf16x8 units per row (4 used + 1 pad)
+// 64 x 64 (2 x 2 accumulator tiles of 32 x 32, real and imaginary); k in chunks of 16 staged
+// through LDS (rows padded to 48 B: the eight lanes of a 16-byte read phase hit distinct banks),
+// the next chunk prefetched into registers during the current chunk's MFMAs.  Two workgroups per
+// CU (accumulators in AGPRs, operands loaded per column tile, C read in the epilogue): one
+// workgroup's C read-modify-write overlaps the other's MFMAs.
+constexpr int SPLIT_KC = 16;
+constexpr int SPLIT_LDS_ROW = SPLIT_KC / 8 + 1;  // bf16x8 units per row (2 used + 1 pad)
 
-__global__ void __launch_bounds__(256) zgemm_split_kernel(int m, int mpad, int ks, const unsigned short* __restrict__ Sb,
-                                                          int64_t sS, cplxf* __restrict__ Cb, int ldc, int64_t sC) {
+__global__ void __launch_bounds__(256, 2) zgemm_split_kernel(int m, int mpad, int ks, const unsigned short* __restrict__ Sb,
+                                                             int64_t sS, cplxf* __restrict__ Cb, int ldc, int64_t sC) {
     __shared__ bf16x8_t La[4][SPLIT_TILE][SPLIT_LDS_ROW];  // L21 rows   (MFMA B operand, C rows)
     __shared__ bf16x8_t Lb[4][SPLIT_TILE][SPLIT_LDS_ROW];  // U12 columns (MFMA A operand, C columns)
     // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin by linear id, so the
     // id is remapped to give each XCD a contiguous run of tiles (row tile fastest: the tiles an XCD
-    // holds at once share their column panel and walk down the same column-major C columns).
-    // Grouping 8 tile columns instead cuts the panel re-reads (7.8 against 10.4 GB per launch of
-    // 64 matrices of order 4096) but is slower (3.26 against 3.03 ms).
+    // holds at once share their column panel and walk down the same column-major C columns)
     int L = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     {
         const int T = gridDim.x * gridDim.y * gridDim.z, q = T / 8;
@@ -989,84 +990,88 @@ __global__ void __launch_bounds__(256) zgemm_split_kernel(int m, int mpad, int k
     const bf16x8_t* Sa = reinterpret_cast<const bf16x8_t*>(S);                              // arrays 0..3
     const bf16x8_t* Sbv = reinterpret_cast<const bf16x8_t*>(S + (int64_t)4 * mpad * ks);  // arrays 4..7
     const int UPR = ks / 8;  // bf16x8 units per row in global
-    // stage: 4 arrays x 128 rows x 4 units for each panel = 2048 units each, 8 + 8 per thread; the
-    // next chunk is loaded into registers before the current chunk's MFMAs
-    bf16x8_t pa[8], pb[8];
+    // one chunk: 4 arrays x 128 rows x 2 units for each panel = 1024 units each, 4 + 4 per thread
+    bf16x8_t pa[4], pb[4];
     auto load_chunk = [&](int k0) {
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
+        for (int e = 0; e < 4; e++) {
             const int idx = tid + 256 * e;
-            const int q = idx >> 9, r = (idx >> 2) & 127, u = idx & 3;
+            const int q = idx >> 8, r = (idx >> 1) & 127, u = idx & 1;
             pa[e] = Sa[((int64_t)q * mpad + row0 + r) * UPR + k0 / 8 + u];
             pb[e] = Sbv[((int64_t)q * mpad + col0 + r) * UPR + k0 / 8 + u];
         }
     };
     load_chunk(0);
-    // the tile's C values, loaded now so that their latency hides behind the k loop:
-    // accumulator register v of lane (l32 + 32 h) is C row l32 (N index), column 8 (v / 4) + 4 h + v % 4
-    // (M index) of its 32 x 32 tile
-    cplxf old[2][2][16];
-#pragma unroll
-    for (int a = 0; a < 2; a++)
-#pragma unroll
-        for (int b = 0; b < 2; b++) {
-            const int r = row0 + wr + 32 * b + l32;
-#pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const int c = col0 + wc + 32 * a + 8 * (v >> 2) + 4 * h + (v & 3);
-                old[a][b][v] = (r < m && c < m) ? Cm[r + (int64_t)ldc * c] : make_float2(0.f, 0.f);
-            }
-        }
     for (int k0 = 0; k0 < ks; k0 += SPLIT_KC) {
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
+        for (int e = 0; e < 4; e++) {
             const int idx = tid + 256 * e;
-            const int q = idx >> 9, r = (idx >> 2) & 127, u = idx & 3;
+            const int q = idx >> 8, r = (idx >> 1) & 127, u = idx & 1;
             La[q][r][u] = pa[e];
             Lb[q][r][u] = pb[e];
         }
         __syncthreads();
         if (k0 + SPLIT_KC < ks) load_chunk(k0 + SPLIT_KC);
+        bf16x8_t arh[2], arl[2], aih[2], ail[2];
 #pragma unroll
-        for (int ks = 0; ks < SPLIT_KC / 16; ks++) {
-            const int u = 2 * ks + h;
-            bf16x8_t brh[2], brl[2], bih[2], bil[2], nbih[2], nbil[2], arh[2], arl[2], aih[2], ail[2];
+        for (int t = 0; t < 2; t++) {
+            const int r = wr + 32 * t + l32;
+            arh[t] = La[0][r][h];
+            arl[t] = La[1][r][h];
+            aih[t] = La[2][r][h];
+            ail[t] = La[3][r][h];
+        }
 #pragma unroll
-            for (int t = 0; t < 2; t++) {
-                const int c = wc + 32 * t + l32, r = wr + 32 * t + l32;
-                brh[t] = Lb[0][c][u];
-                brl[t] = Lb[1][c][u];
-                bih[t] = Lb[2][c][u];
-                bil[t] = Lb[3][c][u];
-                nbih[t] = bf16x8_neg(bih[t]);
-                nbil[t] = bf16x8_neg(bil[t]);
-                arh[t] = La[0][r][u];
-                arl[t] = La[1][r][u];
-                aih[t] = La[2][r][u];
-                ail[t] = La[3][r][u];
+        for (int a = 0; a < 2; a++) {
+            const int c = wc + 32 * a + l32;
+            const bf16x8_t brh = Lb[0][c][h], brl = Lb[1][c][h], bih = Lb[2][c][h], bil = Lb[3][c][h];
+            const bf16x8_t nbih = bf16x8_neg(bih), nbil = bf16x8_neg(bil);
+#pragma unroll
+            for (int b = 0; b < 2; b++) {
+                f32x16_t re = acc_re[a][b], im = acc_im[a][b];
+                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh, arh[b], re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh, arl[b], re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brl, arh[b], re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbih, aih[b], re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbih, ail[b], re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbil, aih[b], re, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bih, arh[b], im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bih, arl[b], im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bil, arh[b], im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh, aih[b], im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh, ail[b], im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brl, aih[b], im, 0, 0, 0);
+                acc_re[a][b] = re;
+                acc_im[a][b] = im;
             }
-#pragma unroll
-            for (int a = 0; a < 2; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++) {
-                    f32x16_t re = acc_re[a][b], im = acc_im[a][b];
-                    re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh[a], arh[b], re, 0, 0, 0);
-                    re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh[a], arl[b], re, 0, 0, 0);
-                    re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brl[a], arh[b], re, 0, 0, 0);
-                    re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbih[a], aih[b], re, 0, 0, 0);
-                    re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbih[a], ail[b], re, 0, 0, 0);
-                    re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbil[a], aih[b], re, 0, 0, 0);
-                    im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bih[a], arh[b], im, 0, 0, 0);
-                    im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bih[a], arl[b], im, 0, 0, 0);
-                    im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bil[a], arh[b], im, 0, 0, 0);
-                    im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh[a], aih[b], im, 0, 0, 0);
-                    im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh[a], ail[b], im, 0, 0, 0);
-                    im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brl[a], aih[b], im, 0, 0, 0);
-                    acc_re[a][b] = re;
-                    acc_im[a][b] = im;
-                }
         }
         __syncthreads();
+    }
+    // epilogue: accumulator register v of lane (l32 + 32 h) is C row l32 (N index), column
+    // 8 (v / 4) + 4 h + v % 4 (M index) of its 32 x 32 tile; each access covers 32 consecutive
+    // rows of two columns.  A wave tile inside C (the common case) reads all 32 values of a
+    // column tile at once; edge tiles check every element.
+    if (row0 + wr + 64 <= m && col0 + wc + 64 <= m) {
+#pragma unroll
+        for (int a = 0; a < 2; a++) {
+            cplxf old[2][16];
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int r = row0 + wr + 32 * b + l32, c = col0 + wc + 32 * a + 8 * (v >> 2) + 4 * h + (v & 3);
+                    old[b][v] = Cm[r + (int64_t)ldc * c];
+                }
+#pragma unroll
+            for (int b = 0; b < 2; b++)
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const int r = row0 + wr + 32 * b + l32, c = col0 + wc + 32 * a + 8 * (v >> 2) + 4 * h + (v & 3);
+                    Cm[r + (int64_t)ldc * c] =
+                        make_float2(old[b][v].x - acc_re[a][b][v], old[b][v].y - acc_im[a][b][v]);
+                }
+        }
+        return;
     }
 #pragma unroll
     for (int a = 0; a < 2; a++)
@@ -1076,9 +1081,10 @@ __global__ void __launch_bounds__(256) zgemm_split_kernel(int m, int mpad, int k
 #pragma unroll
             for (int v = 0; v < 16; v++) {
                 const int c = col0 + wc + 32 * a + 8 * (v >> 2) + 4 * h + (v & 3);
-                if (r < m && c < m)
-                    Cm[r + (int64_t)ldc * c] =
-                        make_float2(old[a][b][v].x - acc_re[a][b][v], old[a][b][v].y - acc_im[a][b][v]);
+                if (r < m && c < m) {
+                    const cplxf o = Cm[r + (int64_t)ldc * c];
+                    Cm[r + (int64_t)ldc * c] = make_float2(o.x - acc_re[a][b][v], o.y - acc_im[a][b][v]);
+                }
             }
         }
 }
