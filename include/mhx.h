@@ -209,9 +209,14 @@ int mhx_exact_work(mhx_ctx* ctx, int64_t counts[3], double* flops);
  * list, common headings): the reference's solve(problems::Vector) / hydrodynamic_coefficients
  * over a list of bodies (solve.jl:75-78, problems_and_results.jl:464-495).  The assembly of
  * every (body, frequency) item runs as one persistent queue launch per stage (exact table,
- * then S/D), the LU is batched per body, and the frequencies are processed in as few chunks as
- * the device-memory budget allows.  Outputs per body as mhx_body_coeffs.  gf: MHX_GF_WU or
- * MHX_GF_DELHOMMEAU; FP64 throughout.  flags: MHX_SWEEP_GRAPH captures the whole sweep into a
+ * then S/D; with MHX_GF_SURROGATE one fused pair launch writing S, D and the FP32 copy of D), the
+ * LU is batched per body, and the frequencies are processed in as few chunks as the
+ * device-memory budget allows.  Outputs per body as mhx_body_coeffs.  gf: MHX_GF_WU,
+ * MHX_GF_DELHOMMEAU or MHX_GF_SURROGATE.  The dense solves follow mhx_set_lu_mode: FP64, or the
+ * FP32 (split-bf16) LU with one refinement step against the FP64 matrix on the device; a
+ * (body, frequency) whose residual then exceeds 1e-12 |b| is solved again in FP64 after the
+ * sweep, as is one whose factorization broke LAPACK's pivoting rule (with row interchanges).
+ * flags: MHX_SWEEP_GRAPH captures the whole sweep into a
  * HIP graph on the first call and replays it on later calls with the same bodies and frequency
  * counts (the frequency values may change).  A body's buffers belong to the sweep until the
  * next sweep call or mhx_body_destroy. */

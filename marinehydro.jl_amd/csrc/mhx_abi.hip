@@ -667,6 +667,11 @@ struct SweepState {
     int* pflags = nullptr;                   // their per-item flags: body ib, frequency w at flag_off[ib] + w
     std::vector<int64_t> flag_off;
     int rows = 256;                          // tuner's choice of rows per combine workgroup
+    bool mixed = false, split = false;       // FP32 LU (split-bf16 updates) with one refinement step
+    double* rnorms = nullptr;                // mixed: |r|^2, |b|^2 after the refinement, per (body, frequency)
+    std::vector<int64_t> chunk_sunits;       // surrogate units per chunk
+    unsigned long long *fb_list = nullptr, *fb_count = nullptr;  // surrogate: exact-fallback pairs
+    int64_t fb_cap = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void release() {
@@ -683,6 +688,9 @@ struct SweepState {
         dfree(status);
         dfree(sticky);
         dfree(pflags);
+        dfree(rnorms);
+        dfree(fb_list);
+        dfree(fb_count);
     }
 };
 
@@ -708,7 +716,7 @@ bool sweep_uses(mhx_ctx* c, const mhx_body* b) {
 // LU, the solves and the pressure integration into the per-body force arrays.
 void sweep_enqueue(mhx_ctx* ctx, SweepState& s) {
     hipStream_t st = ctx->stream;
-    const bool exact = s.gf == MHX_GF_DELHOMMEAU;
+    const bool exact = s.gf == MHX_GF_DELHOMMEAU, surr = s.gf == MHX_GF_SURROGATE;
     const double g = ctx->g;
     mhx::launch_zero(st, s.status, 4 * sizeof(unsigned long long));
     mhx::launch_zero(st, s.sticky, s.bodies.size() * sizeof(int));
@@ -728,7 +736,11 @@ void sweep_enqueue(mhx_ctx* ctx, SweepState& s) {
             mhx::launch_zero(st, s.counters + c, sizeof(unsigned long long));
             mhx::launch_queue_table(st, it, s.chunk_count[c], s.chunk_tunits[c], s.counters + c, 1e-8, 1e-8, s.status);
         }
-        mhx::launch_queue_combine(st, it, s.chunk_count[c], s.chunk_units[c], exact, s.direct, 1, s.rows);
+        if (surr)  // the fused pair kernel writes S, D and the FP32 copy of D
+            mhx::launch_queue_surr(st, it, s.chunk_count[c], s.chunk_sunits[c], s.direct, 1, ctx->surr_W,
+                                   ctx->surr_bias, ctx->surr_B, s.fb_list, s.fb_count, s.fb_cap, s.status);
+        else
+            mhx::launch_queue_combine(st, it, s.chunk_count[c], s.chunk_units[c], exact, s.direct, 1, s.rows);
         for (size_t ib = 0; ib < s.bodies.size(); ib++) {
             mhx_body* b = s.bodies[ib];
             const int64_t w0 = (int64_t)c * s.per_chunk[ib];
@@ -736,6 +748,39 @@ void sweep_enqueue(mhx_ctx* ctx, SweepState& s) {
             if (!bt) continue;
             const int n = b->n, R = b->ndof + (int)s.nb;
             mhx::launch_zero(st, b->info, sizeof(int) * bt);
+            if (s.mixed) {
+                // FP32 LU (verified diagonal pivots, their flags), the solves with the FP32 factors,
+                // one refinement step against the FP64 D and the residual norms for the host's check
+                if (!surr) mhx::convert_to_f32(st, b->nn() * bt, b->D, b->Df);
+                getrf_f32_two_streams(ctx, b, bt, nullptr, s.split ? b->Ps : nullptr,
+                                      s.verified ? mhx::kPivotVerified : mhx::kPivotStrict);
+                if (s.verified) mhx::launch_gather_pivot_flags(st, b->wperm, bt, s.pflags + s.flag_off[ib] + w0);
+                mhx::launch_sticky_info(st, b->info, bt, s.sticky + ib);
+                cplx* W2 = b->work + 16 * b->nR();
+                cplx* X = s.direct ? b->Y : b->B;
+                const int64_t nRb = b->nR() * bt;
+                if (s.direct)
+                    mhx::launch_matmul_nr(st, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), bt);
+                mhx::launch_copy(st, X, b->Rr, sizeof(cplx) * nRb);
+                mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, X, b->nR(), W2, b->sW(), bt);
+                mhx::launch_matmul_nr(st, n, R, b->D, b->nn(), X, b->nR(), b->Tr, b->nR(), b->work, b->sW(), bt);
+                mhx::launch_zero(st, b->norms, sizeof(double) * 2 * bt);
+                mhx::launch_residual(st, (int64_t)n * R, b->Rr, b->nR(), b->Tr, b->nR(), b->norms, bt);
+                mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, b->Tr, b->nR(), W2, b->sW(),
+                                   bt, true);
+                mhx::launch_add(st, (int64_t)n * R, b->Tr, b->nR(), X, b->nR(), bt);
+                double* rn = s.rnorms + 2 * (s.flag_off[ib] + w0);
+                mhx::launch_matmul_nr(st, n, R, b->D, b->nn(), X, b->nR(), b->Tr, b->nR(), b->work, b->sW(), bt);
+                mhx::launch_zero(st, rn, sizeof(double) * 2 * bt);
+                mhx::launch_residual(st, (int64_t)n * R, b->Rr, b->nR(), b->Tr, b->nR(), rn, bt);
+                if (!s.direct)
+                    mhx::launch_matmul_nr(st, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), bt);
+                if (b->ndof > 0)
+                    mhx::launch_forces(st, n, b->geom, b->dofs, b->ndof, b->Y, b->nR(), R, ctx->rho, b->d_ws, b->d_ks, bt,
+                                       g, b->betas, (int)s.nb, s.F_all[ib] + w0 * b->ndof * R,
+                                       s.FK_all[ib] + w0 * b->ndof * (s.nb ? s.nb : 1));
+                continue;
+            }
             if (s.verified) {
                 // the copy of D for a pivoted redo of flagged items (after the sweep), then the
                 // factorization with verified diagonal pivots and its flags
@@ -1680,13 +1725,16 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
                      int direct, int flags) {
     if (!ctx) return MHX_EINVAL;
     if (!in || nbodies <= 0) return fail(ctx, MHX_EINVAL, "sweep: no bodies");
-    if (gf != MHX_GF_WU && gf != MHX_GF_DELHOMMEAU)
-        return fail(ctx, MHX_EINVAL, "sweep: the multi-body sweep runs GFWu or ExactGuevelDelhommeau");
+    if (gf != MHX_GF_WU && gf != MHX_GF_DELHOMMEAU && gf != MHX_GF_SURROGATE)
+        return fail(ctx, MHX_EINVAL, "sweep: the multi-body sweep runs GFWu, ExactGuevelDelhommeau or the surrogate");
+    if (gf == MHX_GF_SURROGATE && !ctx->surr_ready)
+        return fail(ctx, MHX_EINVAL, "sweep: the surrogate needs a trained network (mhx_surrogate_train/load)");
     if (nb < 0 || (nb && !betas)) return fail(ctx, MHX_EINVAL, "sweep: bad heading array");
     if (flags & ~MHX_SWEEP_GRAPH) return fail(ctx, MHX_EINVAL, "sweep: unknown flag");
     direct = direct ? 1 : 0;
-    const bool table = gf == MHX_GF_DELHOMMEAU, graph = (flags & MHX_SWEEP_GRAPH) != 0;
+    const bool table = gf == MHX_GF_DELHOMMEAU, surr = gf == MHX_GF_SURROGATE, graph = (flags & MHX_SWEEP_GRAPH) != 0;
     const bool verified = ctx->pivoting == MHX_PIVOT_VERIFIED;
+    const bool mixed = use_mixed(ctx, gf), split = use_split(ctx, gf);
     for (int64_t ib = 0; ib < nbodies; ib++) {
         const mhx_sweep_body& e = in[ib];
         if (!e.body || e.body->ctx != ctx) return fail(ctx, MHX_EINVAL, "sweep: body is null or from another context");
@@ -1709,8 +1757,10 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
     int64_t K = 1;
     for (int64_t ib = 0; ib < nbodies; ib++) {
         const double n = in[ib].body->n;
-        total += (n * n * sizeof(cplx) * ((table ? 4 : 2) + (verified ? 1 : 0)) + 20.0 * n * 16 * sizeof(cplx)) *
-                 in[ib].nw;
+        double per = n * n * sizeof(cplx) * ((table ? 4 : 2) + (verified && !mixed ? 1 : 0)) + 20.0 * n * 16 * sizeof(cplx);
+        if (mixed) per += n * n * sizeof(cplxf) + 2.0 * n * 16 * sizeof(cplx);   // FP32 factors, Rr, Tr
+        if (surr) per += 8.0 * mhx::surr_fallback_capacity((int)n, 1);            // fallback pair list
+        total += per * in[ib].nw;
         K = std::max<int64_t>(K, (in[ib].nw + ctx->max_batch - 1) / ctx->max_batch);
     }
     K = std::max<int64_t>(K, (int64_t)std::ceil(total / ctx->batch_bytes));
@@ -1720,7 +1770,14 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
         mhx_body* b = in[ib].body;
         per[ib] = (int)((in[ib].nw + K - 1) / K);
         if ((rc = ensure_work(ctx, b, b->ndof + (int)nb, (int)nb, table, std::max(1, per[ib]))) ||
-            (rc = ensure_rankine(ctx, b, direct)) || (verified && (rc = dalloc(ctx, &b->D0, (size_t)b->nn() * b->cap_batch))))
+            (rc = ensure_rankine(ctx, b, direct)) ||
+            (verified && !mixed && (rc = dalloc(ctx, &b->D0, (size_t)b->nn() * b->cap_batch))))
+            return rc;
+        if (mixed && ((rc = dalloc(ctx, &b->Df, (size_t)b->nn() * b->cap_batch)) ||
+                      (rc = dalloc(ctx, &b->Rr, (size_t)b->nR() * b->cap_batch)) ||
+                      (rc = dalloc(ctx, &b->Tr, (size_t)b->nR() * b->cap_batch)) ||
+                      (rc = dalloc(ctx, &b->norms, 2 * (size_t)b->cap_batch)) ||
+                      (split && (rc = dalloc(ctx, &b->Ps, (size_t)mhx::split_work(b->n) * b->cap_batch)))))
             return rc;
         if (nb && (rc = hipMemcpyAsync(b->betas, betas, sizeof(double) * nb, hipMemcpyHostToDevice, st)) != hipSuccess)
             return fail(ctx, MHX_EDEVICE, "sweep: heading upload");
@@ -1728,10 +1785,11 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
     // the item queue (pointers into each body's buffers)
     std::vector<mhx::SweepItem> items;
     std::vector<int> cfirst, ccount;
-    std::vector<int64_t> cunits, ctunits;
+    std::vector<int64_t> cunits, ctunits, csunits;
+    int64_t fb_cap = 1;
     for (int64_t c = 0; c < K; c++) {
         cfirst.push_back((int)items.size());
-        int64_t u = 0, tu = 0;
+        int64_t u = 0, tu = 0, su = 0, pairs = 0;
         for (int64_t ib = 0; ib < nbodies; ib++) {
             mhx_body* b = in[ib].body;
             const int64_t w0 = c * per[ib];
@@ -1745,18 +1803,24 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
                 it.D = b->D + q * b->nn();
                 it.WG = table ? b->WG + q * b->nn() : nullptr;
                 it.WR = table ? b->WR + q * b->nn() : nullptr;
+                it.Df = surr && mixed ? b->Df + q * b->nn() : nullptr;
                 it.k = b->d_ks + q;
                 it.n = b->n;
                 it.unit0 = u;
                 it.tunit0 = tu;
+                it.sunit0 = su;
                 u += mhx::sweep_combine_units(b->n);
                 tu += mhx::sweep_table_units(b->n);
+                su += mhx::sweep_surr_units(b->n);
+                pairs += mhx::surr_fallback_capacity(b->n, 1);
                 items.push_back(it);
             }
         }
         ccount.push_back((int)items.size() - cfirst.back());
         cunits.push_back(u);
         ctunits.push_back(tu);
+        csunits.push_back(su);
+        fb_cap = std::max(fb_cap, pairs);
     }
     std::vector<mhx_body*> bodies(nbodies);
     std::vector<int64_t> nws(nbodies);
@@ -1771,7 +1835,8 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
     SweepState local;
     SweepState* s = graph ? ctx->sweep : &local;
     const bool reuse = graph && s && s->exec && s->bodies == bodies && s->nws == nws && s->gens == gens &&
-                       s->rho == ctx->rho && s->g == ctx->g && s->verified == verified && s->nb == nb && s->gf == gf &&
+                       s->rho == ctx->rho && s->g == ctx->g && s->verified == verified && s->mixed == mixed &&
+                       s->split == split && s->nb == nb && s->gf == gf &&
                        s->direct == direct && s->K == K && s->h_items.size() == items.size() &&
                        std::memcmp(s->h_items.data(), items.data(), items.size() * sizeof(mhx::SweepItem)) == 0;
     auto abandon = [&]() {
@@ -1789,6 +1854,8 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
         s->rho = ctx->rho;
         s->g = ctx->g;
         s->verified = verified;
+        s->mixed = mixed;
+        s->split = split;
         s->flag_off.assign(nbodies, 0);
         int64_t nflags = 0;
         for (int64_t ib = 0; ib < nbodies; ib++) {
@@ -1805,6 +1872,8 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
         s->chunk_count = ccount;
         s->chunk_units = cunits;
         s->chunk_tunits = ctunits;
+        s->chunk_sunits = csunits;
+        s->fb_cap = surr ? fb_cap : 0;
         s->w_all.assign(nbodies, nullptr);
         s->k_all.assign(nbodies, nullptr);
         s->F_all.assign(nbodies, nullptr);
@@ -1819,7 +1888,9 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
         }
         if (rc || (rc = dalloc(ctx, &s->items, items.size())) || (rc = dalloc(ctx, &s->counters, (size_t)K)) ||
             (rc = dalloc(ctx, &s->status, 4)) || (rc = dalloc(ctx, &s->sticky, (size_t)nbodies)) ||
-            (verified && (rc = dalloc(ctx, &s->pflags, (size_t)std::max<int64_t>(1, nflags))))) {
+            (verified && (rc = dalloc(ctx, &s->pflags, (size_t)std::max<int64_t>(1, nflags)))) ||
+            (mixed && (rc = dalloc(ctx, &s->rnorms, 2 * (size_t)std::max<int64_t>(1, nflags)))) ||
+            (surr && ((rc = dalloc(ctx, &s->fb_list, (size_t)fb_cap)) || (rc = dalloc(ctx, &s->fb_count, 1))))) {
             abandon();
             return rc;
         }
@@ -1834,7 +1905,7 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
         hipMemcpyAsync(s->k_all[ib], k.data(), sizeof(double) * in[ib].nw, hipMemcpyHostToDevice, st);
         if (hipStreamSynchronize(st) != hipSuccess) return fail(ctx, MHX_EDEVICE, "sweep: frequency upload");
     }
-    if (!reuse && !items.empty()) {
+    if (!reuse && !items.empty() && !surr) {
         std::string key = std::string(table ? "exact" : "wu") + (direct ? "/d" : "/i");
         for (int64_t ib = 0; ib < nbodies; ib++) key += "/" + std::to_string(bodies[ib]->n) + "x" + std::to_string(per[ib]);
         // wavenumbers of chunk 0 in place for the tuner's trial launches
@@ -1903,31 +1974,50 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
                  in[ib].Fk_out);
         }
     }
-    // verified diagonal pivots: every (body, frequency) whose factorization broke LAPACK's rule is
-    // solved again on its own with row interchanges and its coefficients replaced
-    if (!rc && s->verified) {
+    // every (body, frequency) whose factorization broke LAPACK's rule (verified diagonal pivots),
+    // or whose mixed-precision solution missed the residual tolerance after its refinement step,
+    // is solved again on its own (row interchanges; FP64 LU for a missed residual) and its
+    // coefficients replaced
+    if (!rc && (s->verified || s->mixed)) {
         int64_t nflags = 0;
         for (int64_t ib = 0; ib < nbodies; ib++) nflags += in[ib].nw;
         std::vector<int> fl((size_t)std::max<int64_t>(1, nflags), 0);
-        if (nflags && (hipMemcpyAsync(fl.data(), s->pflags, sizeof(int) * nflags, hipMemcpyDeviceToHost, st) != hipSuccess ||
+        std::vector<double> rn(2 * (size_t)std::max<int64_t>(1, nflags), 0.0);
+        if (nflags && ((s->verified && hipMemcpyAsync(fl.data(), s->pflags, sizeof(int) * nflags, hipMemcpyDeviceToHost,
+                                                      st) != hipSuccess) ||
+                       (s->mixed && hipMemcpyAsync(rn.data(), s->rnorms, sizeof(double) * 2 * nflags,
+                                                   hipMemcpyDeviceToHost, st) != hipSuccess) ||
                        hipStreamSynchronize(st) != hipSuccess))
-            rc = fail(ctx, MHX_EDEVICE, "sweep: pivot flags");
-        const int saved = ctx->pivoting;
+            rc = fail(ctx, MHX_EDEVICE, "sweep: pivot flags and residuals");
+        const int saved_piv = ctx->pivoting, saved_lu = ctx->lu_mode;
         for (int64_t ib = 0; ib < nbodies && !rc; ib++) {
             mhx_body* b = bodies[ib];
             const int ndof = b->ndof, nw = (int)in[ib].nw;
             for (int w = 0; w < nw && !rc; w++) {
-                if (!fl[s->flag_off[ib] + w]) {
-                    ctx->lu_diagonal++;
-                    continue;
+                const int64_t f = s->flag_off[ib] + w;
+                const bool pivot = s->verified && fl[f];
+                bool missed = false;
+                if (s->mixed) {
+                    const double rel = rn[2 * f + 1] > 0 ? std::sqrt(rn[2 * f] / rn[2 * f + 1]) : std::sqrt(rn[2 * f]);
+                    missed = !(rel <= kRefineTol);
+                    if (!missed) {
+                        ctx->mixed_solves++;
+                        ctx->refine_steps++;
+                        ctx->max_residual = std::max(ctx->max_residual, rel);
+                    }
                 }
+                if (s->verified && !pivot) ctx->lu_diagonal++;
+                if (!pivot && !missed) continue;
+                if (missed) ctx->fp64_fallbacks++;
                 if (!ndof) continue;
                 const size_t nd = ndof, nb1 = nb ? nb : 1;
                 std::vector<double> A1(nd * nd), B1(nd * nd), Fd1(2 * nd * nb1), Fk1(2 * nd * nb1);
-                ctx->pivoting = MHX_PIVOT_ALWAYS;
+                if (pivot) ctx->pivoting = MHX_PIVOT_ALWAYS;
+                if (missed) ctx->lu_mode = MHX_LU_FP64;
                 rc = mhx_body_coeffs(ctx, b, 1, in[ib].omegas + w, nb, betas, gf, direct, MHX_MODE_PER_OMEGA, A1.data(),
                                      B1.data(), Fd1.data(), Fk1.data());
-                ctx->pivoting = saved;
+                ctx->pivoting = saved_piv;
+                ctx->lu_mode = saved_lu;
                 if (rc) break;
                 for (size_t e = 0; e < nd * nd; e++) {
                     in[ib].A_out[e + nd * nd * w] = A1[e];

@@ -51,8 +51,9 @@ struct SweepItem {
     const double* SR;
     const double* DR;
     cplx *S, *D, *WG, *WR;
+    cplxf* Df;                    // FP32 copy of D (the surrogate's mixed-precision solve), or null
     const double* k;
-    int64_t unit0, tunit0;
+    int64_t unit0, tunit0, sunit0;  // first combine / exact-table / surrogate unit of the item
     int n, pad;
 };
 // combine units of an item (256 rows x 8 columns) and table units (16 x 16 blocks, upper triangle)

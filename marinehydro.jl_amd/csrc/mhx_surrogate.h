@@ -50,5 +50,13 @@ void launch_surr_table(hipStream_t st, int n, const PanelGeom* geom, const doubl
                        unsigned long long* status);
 void launch_surr_eval(hipStream_t st, int M, const double* rv, const unsigned short* Wb, const float* bias,
                       const float* Bf, float* out);
+// the fused pair kernel over a queue of items of several bodies (S, D and, when SweepItem::Df is
+// set, the FP32 copy of D), units of 8 x 8 pair blocks counted from SweepItem::sunit0; fallback
+// pairs (outside the trained box) get the exact term; fb_count is zeroed here
+int64_t sweep_surr_units(int n);
+void launch_queue_surr(hipStream_t st, const SweepItem* items, int nitems, int64_t total_units, int direct,
+                       int include_identity, const unsigned short* Wb, const float* bias, const float* Bf,
+                       unsigned long long* fb_list, unsigned long long* fb_count, int64_t fb_cap,
+                       unsigned long long* status);
 
 }  // namespace mhx

@@ -538,51 +538,147 @@ __device__ __forceinline__ void tri_tile(int64_t t, int& bi, int& bj) {
 
 #define SURR_WAVES 8
 
+// destination of one item's pair table, pointers offset to the item: the (G, dG/dr) table (S
+// null), or the finished S and D with their Rankine parts SR, DR (and the FP32 copy Df of D when
+// set)
+struct SurrDst {
+    cplx *WG, *WR;
+    const double *SR, *DR;
+    cplx *S, *D;
+    cplxf* Df;
+    int direct, include_identity;
+};
+
+__device__ __forceinline__ SurrDst surr_dst(const SurrOut& o, int item) {
+    SurrDst d;
+    d.WG = o.WG ? o.WG + o.sMat * item : nullptr;
+    d.WR = o.WR ? o.WR + o.sMat * item : nullptr;
+    d.SR = o.SR;
+    d.DR = o.DR;
+    d.S = o.S ? o.S + o.sMat * item : nullptr;
+    d.D = o.D ? o.D + o.sMat * item : nullptr;
+    d.Df = o.Df ? o.Df + o.sMat * item : nullptr;
+    d.direct = o.direct;
+    d.include_identity = o.include_identity;
+    return d;
+}
+
+__device__ __forceinline__ SurrDst surr_dst(const SweepItem& it, int direct, int include_identity) {
+    SurrDst d;
+    d.WG = d.WR = nullptr;
+    d.SR = it.SR;
+    d.DR = it.DR;
+    d.S = it.S;
+    d.D = it.D;
+    d.Df = it.Df;
+    d.direct = direct;
+    d.include_identity = include_identity;
+    return d;
+}
+
 // store the pair's wave term: the (G, dG/dr) table, or (fused) the S and D entries (i, j) and
 // (j, i) with their Rankine parts, and the FP32 copy of D for the mixed-precision solver
-__device__ __forceinline__ void surr_store_pair(const SurrOut& o, int n, int item, int i, int j, double k, double hh,
-                                                double vv, const PanelGeom* __restrict__ geom, cplx G, cplx Gr) {
+__device__ __forceinline__ void surr_store_pair(const SurrDst& o, int n, int i, int j, double k, double hh, double vv,
+                                                const PanelGeom* __restrict__ geom, cplx G, cplx Gr) {
     const int64_t ij = i + (int64_t)n * j, ji = j + (int64_t)n * i;
     if (!o.S) {
-        cplx* WG = o.WG + o.sMat * item;
-        cplx* WR = o.WR + o.sMat * item;
-        WG[ij] = G;
-        WR[ij] = Gr;
+        o.WG[ij] = G;
+        o.WR[ij] = Gr;
         if (i != j) {
-            WG[ji] = G;
-            WR[ji] = Gr;
+            o.WG[ji] = G;
+            o.WR[ji] = Gr;
         }
         return;
     }
     const PanelGeom& gi = geom[i];
     const PanelGeom& gj = geom[j];
-    cplx* S = o.S + o.sMat * item;
-    cplx* D = o.D + o.sMat * item;
-    cplxf* Df = o.Df ? o.Df + o.sMat * item : nullptr;
     cplx Sv, Dv;
     combine_entry(k, hh, vv, gi.c, gi.nrm, gj, o.direct, o.include_identity && i == j, G, Gr, o.SR[ij], o.DR[ij], Sv, Dv);
-    S[ij] = Sv;
-    D[ij] = Dv;
-    if (Df) Df[ij] = make_float2((float)Dv.x, (float)Dv.y);
+    o.S[ij] = Sv;
+    o.D[ij] = Dv;
+    if (o.Df) o.Df[ij] = make_float2((float)Dv.x, (float)Dv.y);
     if (i != j) {
         combine_entry(k, hh, vv, gj.c, gj.nrm, gi, o.direct, false, G, Gr, o.SR[ji], o.DR[ji], Sv, Dv);
-        S[ji] = Sv;
-        D[ji] = Dv;
-        if (Df) Df[ji] = make_float2((float)Dv.x, (float)Dv.y);
+        o.S[ji] = Sv;
+        o.D[ji] = Dv;
+        if (o.Df) o.Df[ji] = make_float2((float)Dv.x, (float)Dv.y);
     }
 }
 
-// Persistent pair-table kernel over all items of the batch.  One work unit = 8 x 8 block of
-// panel pairs (i in 8 bi.., j in 8 bj.., bi <= bj; pairs with i > j are skipped); each wave takes
-// units in a strided order.  Pairs inside the trained box get Wu + network; others are
-// appended to the fallback list as (item << 40 | i << 20 | j).
+// One work unit of a wave: the 8 x 8 block of panel pairs (i in 8 bi.., j in 8 bj.., bi <= bj;
+// pairs with i > j are skipped) of one item (wavenumber k).  Pairs inside the trained box get Wu +
+// network; the others are appended to the fallback list as (tag | i << 20 | j), tag = item << 40.
+__device__ __forceinline__ void surr_unit(const SurrLds& L, int lane, int n, int bi, int bj, double k,
+                                          const PanelGeom* __restrict__ geom, const SurrDst& dst,
+                                          unsigned long long tag, unsigned long long* __restrict__ fb_list,
+                                          unsigned long long* __restrict__ fb_count, int64_t fb_cap) {
+    const int h = lane >> 5, c = lane & 31;
+    // the lane's own pair 32 h + c; column tile pt of the network holds pairs 32 pt + c, so a
+    // lane takes the other tile's inputs from its partner lane (lane ^ 32)
+    double hh_me = 0.0, vv_me = 0.0;
+    bool valid_me;
+    float u1[2], u2[2];
+    {
+        const int p = 32 * h + c;
+        const int i = 8 * bi + (p & 7), j = 8 * bj + (p >> 3);
+        valid_me = i < n && j < n && i <= j;
+        if (valid_me) {
+            const PanelGeom& gi = geom[i];
+            const PanelGeom& gj = geom[j];
+            reduced_hv(gi.c[0], gi.c[1], gi.c[2], gj.c[0], gj.c[1], gj.c[2], k, hh_me, vv_me);
+        }
+        float a1, a2;
+        surr_inputs_f32(hh_me, -vv_me, a1, a2);
+        const float b1 = __shfl_xor(a1, 32), b2 = __shfl_xor(a2, 32);
+        u1[0] = h ? b1 : a1;
+        u1[1] = h ? a1 : b1;
+        u2[0] = h ? b2 : a2;
+        u2[1] = h ? a2 : b2;
+    }
+    f32x16 out[2];
+    surr_mlp64(L, lane, u1, u2, out);
+    // outputs 0..3 of this lane's pair (32 h + c): own column tile 0, or lane c's tile 1
+    float o[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const float other = __shfl_xor(out[1][q], 32);
+        o[q] = h == 0 ? out[0][q] : other;
+    }
+    const int p = 32 * h + c;
+    const int i = 8 * bi + (p & 7), j = 8 * bj + (p >> 3);
+    const bool net = valid_me && surr_in_box(hh_me, -vv_me);
+    const bool fb = valid_me && !net;
+    // wave-aggregated append to the fallback list
+    const unsigned long long mask = __ballot(fb);
+    if (mask) {
+        unsigned long long base = 0;
+        const int leader = __ffsll((long long)mask) - 1;
+        if (lane == leader) base = atomicAdd(fb_count, (unsigned long long)__popcll(mask));
+        base = __shfl(base, leader);
+        if (fb) {
+            const unsigned long long slot = base + __popcll(mask & ((1ull << lane) - 1ull));
+            if (slot < (unsigned long long)fb_cap) fb_list[slot] = tag | ((unsigned long long)i << 20) | (unsigned)j;
+        }
+    }
+    if (!net) return;
+    cplx W, Wh, Wv;
+    wu_term(hh_me > 0 ? hh_me : 1e-300, vv_me, W, Wh, Wv);
+    const double is = 1.0 / SURR_SCALE;
+    const double ii = is * exp(-fmin(-vv_me, SURR_IM_VC));
+    const cplx G = cmk(W.x + is * o[0], W.y + ii * o[1]);
+    const cplx Gr = cmk(Wh.x + is * o[2], Wh.y + ii * o[3]);
+    surr_store_pair(dst, n, i, j, k, hh_me, vv_me, geom, G, Gr);
+}
+
+// Persistent pair-table kernel over all items of the batch: each wave takes units in a strided
+// order.
 __global__ void __launch_bounds__(64 * SURR_WAVES) surr_table_kernel(
     int n, int batch, const PanelGeom* __restrict__ geom, const double* __restrict__ ks,
-    const unsigned short* __restrict__ Wb, const float* __restrict__ bias, const float* __restrict__ Bf, SurrOut dst,
+    const unsigned short* __restrict__ Wb, const float* __restrict__ bias, const float* __restrict__ Bf, SurrOut out,
     unsigned long long* __restrict__ fb_list, unsigned long long* __restrict__ fb_count, int64_t fb_cap) {
     __shared__ SurrLds L;
     surr_load_lds(L, Wb, bias, Bf);
-    const int lane = threadIdx.x & 63, h = lane >> 5, c = lane & 31;
+    const int lane = threadIdx.x & 63;
     const int nb = (n + 7) / 8;
     const int64_t per_item = (int64_t)nb * (nb + 1) / 2;
     const int64_t units = per_item * batch;
@@ -591,67 +687,50 @@ __global__ void __launch_bounds__(64 * SURR_WAVES) surr_table_kernel(
         const int item = (int)(unit / per_item);
         int bi, bj;
         tri_tile(unit - (int64_t)item * per_item, bi, bj);
-        const double k = ks[item];
-        // the lane's own pair 32 h + c; column tile pt of the network holds pairs 32 pt + c, so a
-        // lane takes the other tile's inputs from its partner lane (lane ^ 32)
-        double hh_me = 0.0, vv_me = 0.0;
-        bool valid_me;
-        float u1[2], u2[2];
-        {
-            const int p = 32 * h + c;
-            const int i = 8 * bi + (p & 7), j = 8 * bj + (p >> 3);
-            valid_me = i < n && j < n && i <= j;
-            if (valid_me) {
-                const PanelGeom& gi = geom[i];
-                const PanelGeom& gj = geom[j];
-                reduced_hv(gi.c[0], gi.c[1], gi.c[2], gj.c[0], gj.c[1], gj.c[2], k, hh_me, vv_me);
-            }
-            float a1, a2;
-            surr_inputs_f32(hh_me, -vv_me, a1, a2);
-            const float b1 = __shfl_xor(a1, 32), b2 = __shfl_xor(a2, 32);
-            u1[0] = h ? b1 : a1;
-            u1[1] = h ? a1 : b1;
-            u2[0] = h ? b2 : a2;
-            u2[1] = h ? a2 : b2;
+        surr_unit(L, lane, n, bi, bj, ks[item], geom, surr_dst(out, item), (unsigned long long)item << 40, fb_list,
+                  fb_count, fb_cap);
+    }
+}
+
+// the same over a queue of items of different bodies (SweepItem::sunit0: an item's first unit)
+__global__ void __launch_bounds__(64 * SURR_WAVES) surr_queue_kernel(
+    const SweepItem* __restrict__ items, int nitems, int64_t units, int direct, int include_identity,
+    const unsigned short* __restrict__ Wb, const float* __restrict__ bias, const float* __restrict__ Bf,
+    unsigned long long* __restrict__ fb_list, unsigned long long* __restrict__ fb_count, int64_t fb_cap) {
+    __shared__ SurrLds L;
+    surr_load_lds(L, Wb, bias, Bf);
+    const int lane = threadIdx.x & 63;
+    const int64_t stride = (int64_t)gridDim.x * SURR_WAVES;
+    for (int64_t unit = (int64_t)blockIdx.x * SURR_WAVES + (threadIdx.x >> 6); unit < units; unit += stride) {
+        int lo = 0, hi = nitems - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (items[mid].sunit0 <= unit) lo = mid;
+            else hi = mid - 1;
         }
-        f32x16 out[2];
-        surr_mlp64(L, lane, u1, u2, out);
-        // outputs 0..3 of this lane's pair (32 h + c): own column tile 0, or lane c's tile 1
-        float o[4];
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const float other = __shfl_xor(out[1][q], 32);
-            o[q] = h == 0 ? out[0][q] : other;
-        }
-        const int p = 32 * h + c;
-        const int i = 8 * bi + (p & 7), j = 8 * bj + (p >> 3);
-        const bool net = valid_me && surr_in_box(hh_me, -vv_me);
-        const bool fb = valid_me && !net;
-        // wave-aggregated append to the fallback list
-        const unsigned long long mask = __ballot(fb);
-        if (mask) {
-            unsigned long long base = 0;
-            const int leader = __ffsll((long long)mask) - 1;
-            if (lane == leader) base = atomicAdd(fb_count, (unsigned long long)__popcll(mask));
-            base = __shfl(base, leader);
-            if (fb) {
-                const unsigned long long slot = base + __popcll(mask & ((1ull << lane) - 1ull));
-                if (slot < (unsigned long long)fb_cap)
-                    fb_list[slot] = ((unsigned long long)item << 40) | ((unsigned long long)i << 20) | (unsigned)j;
-            }
-        }
-        if (!net) continue;
-        cplx W, Wh, Wv;
-        wu_term(hh_me > 0 ? hh_me : 1e-300, vv_me, W, Wh, Wv);
-        const double is = 1.0 / SURR_SCALE;
-        const double ii = is * exp(-fmin(-vv_me, SURR_IM_VC));
-        const cplx G = cmk(W.x + is * o[0], W.y + ii * o[1]);
-        const cplx Gr = cmk(Wh.x + is * o[2], Wh.y + ii * o[3]);
-        surr_store_pair(dst, n, item, i, j, k, hh_me, vv_me, geom, G, Gr);
+        const SweepItem& it = items[lo];
+        int bi, bj;
+        tri_tile(unit - it.sunit0, bi, bj);
+        surr_unit(L, lane, it.n, bi, bj, *it.k, it.geom, surr_dst(it, direct, include_identity),
+                  (unsigned long long)lo << 40, fb_list, fb_count, fb_cap);
     }
 }
 
 // exact term for the listed fallback pairs (grid-stride: the count is only known on the device)
+__device__ __forceinline__ void surr_exact_pair(const SurrDst& dst, int n, const PanelGeom* __restrict__ geom,
+                                                double k, int i, int j, unsigned long long* __restrict__ status) {
+    const PanelGeom& gi = geom[i];
+    const PanelGeom& gj = geom[j];
+    double hh, vv;
+    reduced_hv(gi.c[0], gi.c[1], gi.c[2], gj.c[0], gj.c[1], gj.c[2], k, hh, vv);
+    QuadWork wk;
+    cplx G, Gr;
+    if (!hquad_pair(hh, vv, 1e-8, 1e-8, G, Gr, wk)) atomicOr(status, 1ull);
+    G = cscale(4.0, G);
+    Gr = cscale(4.0, Gr);
+    surr_store_pair(dst, n, i, j, k, hh, vv, geom, G, Gr);
+}
+
 __global__ void surr_fallback_kernel(int n, const PanelGeom* __restrict__ geom, const double* __restrict__ ks,
                                      const unsigned long long* __restrict__ fb_list,
                                      const unsigned long long* __restrict__ fb_count, int64_t fb_cap, SurrOut out,
@@ -660,17 +739,20 @@ __global__ void surr_fallback_kernel(int n, const PanelGeom* __restrict__ geom, 
     for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < cnt; t += (int64_t)gridDim.x * blockDim.x) {
         const unsigned long long e = fb_list[t];
         const int item = (int)(e >> 40), i = (int)((e >> 20) & 0xFFFFF), j = (int)(e & 0xFFFFF);
-        const double k = ks[item];
-        const PanelGeom& gi = geom[i];
-        const PanelGeom& gj = geom[j];
-        double hh, vv;
-        reduced_hv(gi.c[0], gi.c[1], gi.c[2], gj.c[0], gj.c[1], gj.c[2], k, hh, vv);
-        QuadWork wk;
-        cplx G, Gr;
-        if (!hquad_pair(hh, vv, 1e-8, 1e-8, G, Gr, wk)) atomicOr(status, 1ull);
-        G = cscale(4.0, G);
-        Gr = cscale(4.0, Gr);
-        surr_store_pair(out, n, item, i, j, k, hh, vv, geom, G, Gr);
+        surr_exact_pair(surr_dst(out, item), n, geom, ks[item], i, j, status);
+    }
+}
+
+__global__ void surr_queue_fallback_kernel(const SweepItem* __restrict__ items, int direct, int include_identity,
+                                           const unsigned long long* __restrict__ fb_list,
+                                           const unsigned long long* __restrict__ fb_count, int64_t fb_cap,
+                                           unsigned long long* __restrict__ status) {
+    const int64_t cnt = (int64_t)min(*fb_count, (unsigned long long)fb_cap);
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < cnt; t += (int64_t)gridDim.x * blockDim.x) {
+        const unsigned long long e = fb_list[t];
+        const SweepItem& it = items[(int)(e >> 40)];
+        const int i = (int)((e >> 20) & 0xFFFFF), j = (int)(e & 0xFFFFF);
+        surr_exact_pair(surr_dst(it, direct, include_identity), it.n, it.geom, *it.k, i, j, status);
     }
 }
 
@@ -694,6 +776,23 @@ void launch_surr_table(hipStream_t st, int n, const PanelGeom* geom, const doubl
     surr_table_kernel<<<(int)blocks, 64 * SURR_WAVES, 0, st>>>(n, batch, geom, ks, Wb, bias, Bf, out, fb_list, fb_count,
                                                                fb_cap);
     surr_fallback_kernel<<<2048, 256, 0, st>>>(n, geom, ks, fb_list, fb_count, fb_cap, out, status);
+}
+
+int64_t sweep_surr_units(int n) {
+    const int64_t nb = (n + 7) / 8;
+    return nb * (nb + 1) / 2;
+}
+
+void launch_queue_surr(hipStream_t st, const SweepItem* items, int nitems, int64_t total_units, int direct,
+                       int include_identity, const unsigned short* Wb, const float* bias, const float* Bf,
+                       unsigned long long* fb_list, unsigned long long* fb_count, int64_t fb_cap,
+                       unsigned long long* status) {
+    launch_zero(st, fb_count, sizeof(unsigned long long));
+    if (total_units <= 0) return;
+    const int64_t blocks = std::min<int64_t>(device_cus(), (total_units + SURR_WAVES - 1) / SURR_WAVES);
+    surr_queue_kernel<<<(int)blocks, 64 * SURR_WAVES, 0, st>>>(items, nitems, total_units, direct, include_identity, Wb,
+                                                               bias, Bf, fb_list, fb_count, fb_cap);
+    surr_queue_fallback_kernel<<<2048, 256, 0, st>>>(items, direct, include_identity, fb_list, fb_count, fb_cap, status);
 }
 
 // the network (MFMA path) at arbitrary (r, v): out[6 m + o], the raw outputs (real parts:

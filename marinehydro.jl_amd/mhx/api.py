@@ -819,8 +819,9 @@ def hydrodynamic_coefficients_many(jobs, formulation=None, graph=False):
     and replays it when the same bodies and frequency counts come again.  Returns one result
     dict per job, laid out as hydrodynamic_coefficients."""
     formulation = DirectBEM() if formulation is None else formulation
-    if formulation.gf not in (_lib.GF_WU, _lib.GF_DELHOMMEAU):
-        raise ValueError("the multi-body sweep runs GFWu or ExactGuevelDelhommeau")
+    if formulation.gf not in (_lib.GF_WU, _lib.GF_DELHOMMEAU, _lib.GF_SURROGATE):
+        raise ValueError("the multi-body sweep runs GFWu, ExactGuevelDelhommeau or the surrogate")
+    _ensure_surrogate(formulation.gf)
     if not jobs:
         return []
     betas = None
