@@ -320,6 +320,8 @@ def main():
                     help="dense solver precision (auto: mixed FP32 factor for the surrogate, FP64 otherwise)")
     ap.add_argument("--freqs", type=int, default=64,
                     help="frequencies per step, spread over the 64 of the workload (fewer only for profiling)")
+    ap.add_argument("--activation", default="silu", choices=["silu", "relu"],
+                    help="surrogate network: the 4 x 128 SiLU network (the workload's), or its ReLU variant")
     ap.add_argument("--workload", default="headline", choices=["headline", "forward_speed"],
                     help="forward_speed: the forward-speed grid on the same body (IndirectBEM, exact or Wu term)")
     args = ap.parse_args()
@@ -355,11 +357,12 @@ def main():
 
     if args.gf == "Surrogate":
         t0 = time.perf_counter()
-        if os.path.exists(mhx.api.SURROGATE_WEIGHTS):
-            mhx.load_surrogate()
-            log(f"bench: loaded surrogate weights ({time.perf_counter() - t0:.2f} s)")
+        wpath = mhx.api.SURROGATE_WEIGHTS if args.activation == "silu" else mhx.api.SURROGATE_WEIGHTS_RELU
+        if os.path.exists(wpath):
+            mhx.load_surrogate(wpath)
+            log(f"bench: loaded surrogate weights {os.path.basename(wpath)} ({time.perf_counter() - t0:.2f} s)")
         else:
-            val = mhx.train_surrogate()
+            val = mhx.train_surrogate(activation=args.activation)
             log(f"bench: trained surrogate on the device ({time.perf_counter() - t0:.1f} s, val mse {val:.2e})")
     run = run_with(body, form.gf, form.direct, mode)
     log(f"bench: n={n}, {max(args.warmup, 1)} warmup sweep(s) of {nw} frequencies")
@@ -420,7 +423,9 @@ def main():
                                f"capped cylinder R=4 draft=12, {n} panels, 6 dofs, beta=0, all {nw} frequencies "
                                f"in 0.1-3.0 rad/s per step (one step = one {nw}-frequency sweep)",
                    "panels": n, "dofs": 6, "headings": 1, "frequencies_per_step": nw, "schedule": args.schedule,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU",
+                   **({"surrogate_network": "64 Fourier features -> 4 x 128 " +
+                       ("SiLU" if args.activation == "silu" else "ReLU") + " -> 6"} if args.gf == "Surrogate" else {})},
         "phases_ms_per_omega": phases,
         "lu_tflops": lu_tflops,
         "solver": solver,

@@ -278,8 +278,8 @@ int mhx_hydro_coeffs_vjp(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const
 
 /* Learned surrogate of the exact Guevel-Delhommeau wave term (no counterpart in the reference; it
  * replaces the exact term of src/green_functions/exact_Guevel_Delhommeau.jl:15-59 inside assembly).
- * The network maps (r, v) = (k R_h, k |z + zeta|) through 64 Fourier features and 4 x 128 ReLU
- * layers to the residual (exact - GFWu) of G, dG/dr, dG/dz: real parts as 100 x residual,
+ * The network maps (r, v) = (k R_h, k |z + zeta|) through 64 Fourier features and 4 x 128 hidden
+ * layers (SiLU by default, or ReLU) to the residual (exact - GFWu) of G, dG/dr, dG/dz: real parts as 100 x residual,
  * imaginary parts as 100 exp(min(v, 5)) x residual (they decay like exp(-v)).  It is trained on
  * the device on ngrid^2 log-spaced jittered samples generated by the exact FP64 kernel.
  *   train: Adam, cosine learning rate, minibatches of `batch`; val_loss = MSE on the held-out samples
@@ -287,7 +287,12 @@ int mhx_hydro_coeffs_vjp(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const
  *   params: copies the FP32 parameters (nparams floats) and the 32 x 2 Fourier matrix to the host
  *   eval: the BF16-MFMA inference network at m points rv[2m] = (r, v); out[6m] its raw outputs
  *   loss_grad: one FP32 forward/backward of the training kernels on m samples (U m x 2, Y m x 6)
- *              with the given parameters; grad_out has nparams floats */
+ *              with the given parameters; grad_out has nparams floats
+ *   set_activation: the hidden-layer activation of the next training and of loss_grad (a loaded
+ *              network of the other activation is dropped); load takes the file's activation;
+ *              get_activation: the context's current one */
+#define MHX_SURR_SILU 0
+#define MHX_SURR_RELU 1
 int mhx_surrogate_train(mhx_ctx* ctx, int64_t ngrid, int64_t steps, int64_t batch, double lr, uint64_t seed,
                         double* val_loss);
 int mhx_surrogate_save(mhx_ctx* ctx, const char* path);
@@ -296,6 +301,8 @@ int mhx_surrogate_params(mhx_ctx* ctx, float* params_out, float* fourier_out, in
 int mhx_surrogate_eval(mhx_ctx* ctx, int64_t m, const double* rv, float* out);
 int mhx_surrogate_loss_grad(mhx_ctx* ctx, int64_t m, const float* U, const float* Y, const float* params,
                             const float* fourier, float* grad_out, float* loss_out);
+int mhx_surrogate_set_activation(mhx_ctx* ctx, int activation);
+int mhx_surrogate_get_activation(mhx_ctx* ctx, int* activation);
 
 #ifdef __cplusplus
 }

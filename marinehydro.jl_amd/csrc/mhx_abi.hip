@@ -42,6 +42,7 @@ struct mhx_ctx {
     int64_t max_batch = 64;
     // learned wave-term surrogate
     bool surr_ready = false;
+    int surr_act = MHX_SURR_SILU;  // hidden-layer activation of the network (next training / loaded)
     float *surr_params = nullptr, *surr_B = nullptr, *surr_bias = nullptr;
     unsigned short* surr_W = nullptr;
     double surr_val_loss = -1.0;
@@ -301,7 +302,7 @@ int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int 
         {
             Phase ph(ctx, "wave_table");
             mhx::launch_surr_table(ctx->stream, b->n, b->geom, b->d_ks, batch, ctx->surr_W, ctx->surr_bias,
-                                   ctx->surr_B, out, b->fb_list, b->fb_count, b->fb_cap, b->status);
+                                   ctx->surr_B, out, b->fb_list, b->fb_count, b->fb_cap, b->status, ctx->surr_act);
         }
         b->df_ready = out.Df != nullptr;
         if (keep_table) {
@@ -738,7 +739,8 @@ void sweep_enqueue(mhx_ctx* ctx, SweepState& s) {
         }
         if (surr)  // the fused pair kernel writes S, D and the FP32 copy of D
             mhx::launch_queue_surr(st, it, s.chunk_count[c], s.chunk_sunits[c], s.direct, 1, ctx->surr_W,
-                                   ctx->surr_bias, ctx->surr_B, s.fb_list, s.fb_count, s.fb_cap, s.status);
+                                   ctx->surr_bias, ctx->surr_B, s.fb_list, s.fb_count, s.fb_cap, s.status,
+                                   ctx->surr_act);
         else
             mhx::launch_queue_combine(st, it, s.chunk_count[c], s.chunk_units[c], exact, s.direct, 1, s.rows);
         for (size_t ib = 0; ib < s.bodies.size(); ib++) {
@@ -2242,7 +2244,7 @@ int mhx_surrogate_train(mhx_ctx* ctx, int64_t ngrid, int64_t steps, int64_t batc
         Phase ph(ctx, "surrogate_train");
         for (int64_t step = 1; step <= steps; step++) {
             mhx::launch_batch_index(st, (int)batch, ns, seed * 1315423911ull + (uint64_t)step, t.idx);
-            mhx::surr_forward_backward(st, t.w, (int)batch, t.idx, U, Y, ctx->surr_params, t.grad, nullptr);
+            mhx::surr_forward_backward(st, t.w, (int)batch, t.idx, U, Y, ctx->surr_params, t.grad, nullptr, ctx->surr_act);
             const double frac = (double)(step - 1) / (double)steps;
             const float lrs = (float)(lr * 0.5 * (1.0 + std::cos(3.141592653589793 * frac)));
             mhx::surr_adam(st, P, ctx->surr_params, t.grad, t.m1, t.m2, lrs, (int)step);
@@ -2257,7 +2259,7 @@ int mhx_surrogate_train(mhx_ctx* ctx, int64_t ngrid, int64_t steps, int64_t batc
         const int m = (int)std::min<int64_t>(batch, nval - v0);
         if (m < 64) break;
         strided_index_kernel<<<(m + 255) / 256, 256, 0, st>>>(m, v0 * SURR_VAL_STRIDE, SURR_VAL_STRIDE, t.idx);
-        mhx::surr_forward_backward(st, t.w, m, t.idx, U, Y, ctx->surr_params, t.grad, t.loss);
+        mhx::surr_forward_backward(st, t.w, m, t.idx, U, Y, ctx->surr_params, t.grad, t.loss, ctx->surr_act);
         float l = 0.f;
         HCHK(hipMemcpyAsync(&l, t.loss, sizeof(float), hipMemcpyDeviceToHost, st));
         HCHK(hipStreamSynchronize(st));
@@ -2289,11 +2291,12 @@ int mhx_surrogate_save(mhx_ctx* ctx, const char* path) {
     if (!f) return fail(ctx, MHX_EINVAL, std::string("surrogate_save: cannot open ") + path);
     char hdr[512];
     std::snprintf(hdr, sizeof hdr,
-                  "{\"format\": \"mhx-surrogate\", \"version\": 2, \"activation\": \"relu\", \"params\": %lld, "
+                  "{\"format\": \"mhx-surrogate\", \"version\": 2, \"activation\": \"%s\", \"params\": %lld, "
                   "\"fourier\": %d, \"outputs\": %d, "
                   "\"scale\": %.1f, \"r_max\": %.1f, \"v_min\": %g, \"v_max\": %.1f, \"steps\": %lld, "
                   "\"val_mse\": %.6e}\n",
-                  (long long)P, SURR_NF, SURR_NOUT, SURR_SCALE, SURR_RMAX, SURR_VMIN, SURR_VMAX,
+                  ctx->surr_act == MHX_SURR_RELU ? "relu" : "silu", (long long)P, SURR_NF, SURR_NOUT, SURR_SCALE,
+                  SURR_RMAX, SURR_VMIN, SURR_VMAX,
                   (long long)ctx->surr_steps, ctx->surr_val_loss);
     std::fputs("MHXSURR1\n", f);
     std::fputs(hdr, f);
@@ -2313,13 +2316,20 @@ int mhx_surrogate_load(mhx_ctx* ctx, const char* path) {
     bool ok = std::fgets(line, sizeof line, f) && std::strcmp(line, "MHXSURR1\n") == 0 && std::fgets(line, sizeof line, f);
     long long pc = -1, steps = 0;
     double vl = -1.0;
-    bool act_ok = false;
+    int act = MHX_SURR_SILU;
+    bool act_ok = true;
     if (ok) {
         const char* q = std::strstr(line, "\"params\": ");
         ok = q && std::sscanf(q + 10, "%lld", &pc) == 1 && pc == P;
-        // the hidden layers' activation the weights were trained with (files without the field
-        // hold a network with another activation)
-        act_ok = std::strstr(line, "\"activation\": \"relu\"") != nullptr;
+        // the hidden layers' activation the weights were trained with (version-1 files, without
+        // the field, hold SiLU networks)
+        const char* qa = std::strstr(line, "\"activation\": \"");
+        if (qa) {
+            qa += 15;
+            if (std::strncmp(qa, "silu\"", 5) == 0) act = MHX_SURR_SILU;
+            else if (std::strncmp(qa, "relu\"", 5) == 0) act = MHX_SURR_RELU;
+            else act_ok = false;
+        }
         const char* q2 = std::strstr(line, "\"steps\": ");
         if (q2) std::sscanf(q2 + 9, "%lld", &steps);
         const char* q3 = std::strstr(line, "\"val_mse\": ");
@@ -2330,8 +2340,7 @@ int mhx_surrogate_load(mhx_ctx* ctx, const char* path) {
     std::fclose(f);
     if (!ok) return fail(ctx, MHX_EINVAL, std::string("surrogate_load: not an mhx surrogate file: ") + path);
     if (!act_ok)
-        return fail(ctx, MHX_EINVAL, std::string("surrogate_load: the network in ") + path +
-                                         " was not trained with ReLU hidden layers; train it again");
+        return fail(ctx, MHX_EINVAL, std::string("surrogate_load: unknown activation in ") + path);
     hipSetDevice(ctx->device);
     int rc;
     if ((rc = surr_alloc(ctx))) return rc;
@@ -2340,8 +2349,24 @@ int mhx_surrogate_load(mhx_ctx* ctx, const char* path) {
     mhx::launch_surr_pack(ctx->stream, ctx->surr_params, ctx->surr_W, ctx->surr_bias);
     HCHK(hipStreamSynchronize(ctx->stream));
     ctx->surr_ready = true;
+    ctx->surr_act = act;
     ctx->surr_steps = steps;
     ctx->surr_val_loss = vl;
+    return MHX_OK;
+}
+
+int mhx_surrogate_set_activation(mhx_ctx* ctx, int activation) {
+    if (!ctx) return MHX_EINVAL;
+    if (activation != MHX_SURR_SILU && activation != MHX_SURR_RELU)
+        return fail(ctx, MHX_EINVAL, "surrogate activation must be MHX_SURR_SILU or MHX_SURR_RELU");
+    if (activation != ctx->surr_act) ctx->surr_ready = false;  // a network of the other activation is dropped
+    ctx->surr_act = activation;
+    return MHX_OK;
+}
+
+int mhx_surrogate_get_activation(mhx_ctx* ctx, int* activation) {
+    if (!ctx || !activation) return MHX_EINVAL;
+    *activation = ctx->surr_act;
     return MHX_OK;
 }
 
@@ -2369,7 +2394,7 @@ int mhx_surrogate_eval(mhx_ctx* ctx, int64_t m, const double* rv, float* out) {
         return rc;
     }
     hipMemcpyAsync(drv, rv, sizeof(double) * 2 * m, hipMemcpyHostToDevice, ctx->stream);
-    mhx::launch_surr_eval(ctx->stream, (int)m, drv, ctx->surr_W, ctx->surr_bias, ctx->surr_B, dout);
+    mhx::launch_surr_eval(ctx->stream, (int)m, drv, ctx->surr_W, ctx->surr_bias, ctx->surr_B, dout, ctx->surr_act);
     hipMemcpyAsync(out, dout, sizeof(float) * 6 * m, hipMemcpyDeviceToHost, ctx->stream);
     hipError_t e = hipStreamSynchronize(ctx->stream);
     dfree(drv); dfree(dout);
@@ -2399,7 +2424,7 @@ int mhx_surrogate_loss_grad(mhx_ctx* ctx, int64_t m, const float* U, const float
     hipMemcpyAsync(dY, Yt, sizeof(float) * 6 * m, hipMemcpyHostToDevice, st);
     hipMemcpyAsync(dp, params, sizeof(float) * P, hipMemcpyHostToDevice, st);
     hipMemcpyAsync(dB, fourier, sizeof(float) * 2 * SURR_NF, hipMemcpyHostToDevice, st);
-    mhx::surr_forward_backward(st, t.w, (int)m, nullptr, dU, dY, dp, t.grad, t.loss);
+    mhx::surr_forward_backward(st, t.w, (int)m, nullptr, dU, dY, dp, t.grad, t.loss, ctx->surr_act);
     float l = 0.f;
     hipMemcpyAsync(&l, t.loss, sizeof(float), hipMemcpyDeviceToHost, st);
     hipMemcpyAsync(grad_out, t.grad, sizeof(float) * P, hipMemcpyDeviceToHost, st);

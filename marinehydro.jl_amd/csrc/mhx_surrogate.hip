@@ -3,14 +3,14 @@
 // The network maps the reduced coordinates (r, v) = (k R_h, k |z + zeta|) to six
 // outputs: 100 x the residual (exact - Liang-Wu-Noblesse fit) of Re/Im G, Re/Im dG/dr
 // and Re/Im dG/dz.  Architecture: 2 inputs -> 64 Fourier features
-// [sin(2 pi B u), cos(2 pi B u)] of u = (log-scaled r, log-scaled v) -> 4 x 128 ReLU
+// [sin(2 pi B u), cos(2 pi B u)] of u = (log-scaled r, log-scaled v) -> 4 x 128 SiLU (or ReLU)
 // -> 6 linear outputs.  The fit it corrects is the reference's GFWu
 // (src/green_functions/wu.jl); the exact term is exact_Guevel_Delhommeau.jl:15-59.
 //
 //  * samples: ngrid^2 points on a log-spaced grid with seeded jitter, targets from the
 //    exact FP64 kernel (hquad_pair) and the Wu fit, on the device; imaginary residuals
 //    are scaled by exp(min(v, 5)) (the imaginary parts decay like exp(-v));
-//  * training: FP32 kernels (tiled GEMMs with fused bias/ReLU epilogues, split-K
+//  * training: FP32 kernels (tiled GEMMs with fused bias/activation epilogues, split-K
 //    weight gradients, MSE loss, Adam);
 //  * inference: BF16 weights and activations on v_mfma_f32_32x32x8bf16_1k with FP32
 //    accumulation, fused into the pair-table assembly; pairs outside the trained box
@@ -119,17 +119,25 @@ void launch_surr_features(hipStream_t st, int M, const int* idx, const float* U,
     surr_features_kernel<<<(M * SURR_NF + 255) / 256, 256, 0, st>>>(M, idx, U, Bf, F);
 }
 
-// hidden-layer activation (ReLU) and its derivative (0 at z = 0)
-__device__ __forceinline__ float act_f(float z) { return fmaxf(z, 0.0f); }
-__device__ __forceinline__ float dact_f(float z) { return z > 0.0f ? 1.0f : 0.0f; }
+// hidden-layer activation of the network (SURR_ACT_SILU: z / (1 + e^-z); SURR_ACT_RELU: max(z, 0))
+// and its derivative (ReLU: 0 at z = 0)
+template <int ACT> __device__ __forceinline__ float act_f(float z) {
+    if (ACT == SURR_ACT_RELU) return fmaxf(z, 0.0f);
+    return z / (1.0f + expf(-z));
+}
+template <int ACT> __device__ __forceinline__ float dact_f(float z) {
+    if (ACT == SURR_ACT_RELU) return z > 0.0f ? 1.0f : 0.0f;
+    const float s = 1.0f / (1.0f + expf(-z));
+    return s * (1.0f + z * (1.0f - s));
+}
 
 // C[M x N] = sum_k A(m, k) B(k, n).  A(m,k) = TA ? A[k*lda+m] : A[m*lda+k];
 // B(k,n) = TB ? B[n*ldb+k] : B[k*ldb+n].  Epilogues:
 //  EPI 0: C = acc (split-K partial: C + blockIdx.z * M*N, k range split over gridDim.z)
-//  EPI 1: Z = acc + bias[n], H = relu(Z)   (C = Z, C2 = H)
+//  EPI 1: Z = acc + bias[n], H = act(Z)    (C = Z, C2 = H)
 //  EPI 2: Z = acc + bias[n]                 (C = Z)
-//  EPI 3: C = acc * relu'(Zp[m*N+n])        (backward through the activation)
-template <bool TA, bool TB, int EPI>
+//  EPI 3: C = acc * act'(Zp[m*N+n])         (backward through the activation)
+template <bool TA, bool TB, int EPI, int ACT>
 __global__ void __launch_bounds__(256) sgemm_kernel(int M, int N, int K, const float* __restrict__ A, int lda,
                                                     const float* __restrict__ B, int ldb, float* __restrict__ C,
                                                     float* __restrict__ C2, const float* __restrict__ bias,
@@ -184,11 +192,11 @@ __global__ void __launch_bounds__(256) sgemm_kernel(int M, int N, int K, const f
             } else if (EPI == 1) {
                 const float z = acc[i][j] + bias[nn];
                 C[o] = z;
-                C2[o] = act_f(z);
+                C2[o] = act_f<ACT>(z);
             } else if (EPI == 2) {
                 C[o] = acc[i][j] + bias[nn];
             } else {
-                C[o] = acc[i][j] * dact_f(Zp[o]);
+                C[o] = acc[i][j] * dact_f<ACT>(Zp[o]);
             }
         }
 }
@@ -266,9 +274,14 @@ struct SgemmArgs {
 };
 
 template <bool TA, bool TB, int EPI>
-static void sgemm(hipStream_t st, const SgemmArgs& a, int splits = 1) {
+static void sgemm(hipStream_t st, const SgemmArgs& a, int splits = 1, int act = SURR_ACT_SILU) {
     dim3 grid((a.N + 63) / 64, (a.M + 63) / 64, splits);
-    sgemm_kernel<TA, TB, EPI><<<grid, 256, 0, st>>>(a.M, a.N, a.K, a.A, a.lda, a.B, a.ldb, a.C, a.C2, a.bias, a.Zp);
+    if (act == SURR_ACT_RELU)
+        sgemm_kernel<TA, TB, EPI, SURR_ACT_RELU><<<grid, 256, 0, st>>>(a.M, a.N, a.K, a.A, a.lda, a.B, a.ldb, a.C, a.C2,
+                                                                       a.bias, a.Zp);
+    else
+        sgemm_kernel<TA, TB, EPI, SURR_ACT_SILU><<<grid, 256, 0, st>>>(a.M, a.N, a.K, a.A, a.lda, a.B, a.ldb, a.C, a.C2,
+                                                                       a.bias, a.Zp);
 }
 
 // parameter offsets inside the flat FP32 parameter vector
@@ -281,14 +294,14 @@ int64_t surr_param_count() { return OFF_W[4] + (int64_t)SURR_NOUT * 128 + SURR_N
 
 // one forward + backward pass on M rows (idx or identity), gradients into grad (flat, zeroed here)
 void surr_forward_backward(hipStream_t st, SurrTrainWork& w, int M, const int* idx, const float* U, const float* Y,
-                           const float* params, float* grad, float* loss) {
+                           const float* params, float* grad, float* loss, int act) {
     launch_surr_features(st, M, idx, U, w.Bf, w.act[0]);
     const float* hin = w.act[0];
     for (int l = 0; l < 5; l++) {
         const float* W = params + OFF_W[l];
         const float* b = W + (int64_t)OUT_DIM[l] * IN_DIM[l];
         SgemmArgs a{M, OUT_DIM[l], IN_DIM[l], hin, IN_DIM[l], W, IN_DIM[l], w.z[l], w.act[l + 1], b, nullptr};
-        if (l < 4) sgemm<false, true, 1>(st, a);
+        if (l < 4) sgemm<false, true, 1>(st, a, 1, act);
         else sgemm<false, true, 2>(st, a);
         hin = w.act[l + 1];
     }
@@ -306,11 +319,11 @@ void surr_forward_backward(hipStream_t st, SurrTrainWork& w, int M, const int* i
         sum_parts_kernel<<<(nW + 255) / 256, 256, 0, st>>>(nW, splits, w.part, gW);
         colsum_kernel<<<OUT_DIM[l], 256, 0, st>>>(M, OUT_DIM[l], w.dz[l], gb);
         if (l > 0) {
-            // dz[l-1] = (dz[l] W_l) * relu'(z[l-1])
+            // dz[l-1] = (dz[l] W_l) * act'(z[l-1])
             const float* W = params + OFF_W[l];
             SgemmArgs d{M, IN_DIM[l], OUT_DIM[l], w.dz[l], OUT_DIM[l], W, IN_DIM[l], w.dz[l - 1], nullptr, nullptr,
                         w.z[l - 1]};
-            sgemm<false, false, 3>(st, d);
+            sgemm<false, false, 3>(st, d, 1, act);
         }
     }
 }
@@ -378,6 +391,13 @@ __device__ __forceinline__ int surr_slot_input(int l, int kb, int h, int e) {
 }
 
 // FP32 parameters -> packed BF16 units (layout above) and FP32 biases [l][128] + output[8]
+// Inference works on y = -log2(e) z instead of the pre-activation z: the hidden layers'
+// biases, and the first layer's weights, carry the factor -log2(e); the activation gives
+// h' = -log2(e) act(z) from y (SiLU: y / (1 + 2^y), one v_exp_f32 with no scaling multiply;
+// ReLU: min(y, 0)), and the next layer's weights would undo the factor (-1 / log2(e)) while
+// also applying their own (-log2(e)): the middle layers' weights are unchanged and only the
+// output layer's carry -1 / log2(e).
+constexpr float kLog2e = 1.4426950408889634f;
 __global__ void surr_pack_kernel(const float* __restrict__ params, unsigned short* __restrict__ Wb,
                                  float* __restrict__ bias) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -392,14 +412,14 @@ __global__ void surr_pack_kernel(const float* __restrict__ params, unsigned shor
         const int in = surr_slot_input(l, kb, h, e);
         float v = 0.f;
         if (row < OUT_DIM_D(l)) v = params[OFF_W_D(l) + (int64_t)row * IN_DIM_D(l) + in];
-        Wb[t] = f2bf(v);
+        Wb[t] = f2bf(v * (l == 0 ? -kLog2e : l == 4 ? -1.0f / kLog2e : 1.0f));
     }
     if (t < 4 * 128 + 8) {
         const int l = t < 512 ? t / 128 : 4;
         const int r = t - 128 * l;
         float v = 0.f;
         if (r < OUT_DIM_D(l)) v = params[OFF_W_D(l) + (int64_t)OUT_DIM_D(l) * IN_DIM_D(l) + r];
-        bias[t] = v;
+        bias[t] = l < 4 ? -kLog2e * v : v;
     }
 }
 
@@ -440,7 +460,14 @@ __device__ __forceinline__ void surr_mfma_tile(const bf16x8* W, const float* bia
     }
 }
 
-// ReLU of neuron tile t's accumulators into the B operands of the next layer
+// -log2(e) act(z) from y = -log2(e) z (SiLU: 2^y overflows to inf for large y, h' -> 0)
+template <int ACT> __device__ __forceinline__ float act_log2(float y) {
+    if (ACT == SURR_ACT_RELU) return fminf(y, 0.0f);
+    return y * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(y));
+}
+
+// activation of neuron tile t's accumulators into the B operands of the next layer
+template <int ACT>
 __device__ __forceinline__ void surr_act_tile(int t, const f32x16& acc0, const f32x16& acc1, bf16x8 nxt[2][8]) {
 #pragma unroll
     for (int b = 0; b < 2; b++) {
@@ -448,15 +475,15 @@ __device__ __forceinline__ void surr_act_tile(int t, const f32x16& acc0, const f
 #pragma unroll
         for (int e = 0; e < 8; e++) {
             const int q = 4 * (2 * b + (e >> 2)) + (e & 3);
-            v0[e] = (__bf16)fmaxf(acc0[q], 0.0f);
-            v1[e] = (__bf16)fmaxf(acc1[q], 0.0f);
+            v0[e] = (__bf16)act_log2<ACT>(acc0[q]);
+            v1[e] = (__bf16)act_log2<ACT>(acc1[q]);
         }
         nxt[0][2 * t + b] = v0;
         nxt[1][2 * t + b] = v1;
     }
 }
 
-template <int KB>
+template <int KB, int ACT>
 __device__ __forceinline__ void surr_hidden(const SurrLds& L, int layer, int lane, bf16x8 act[2][8]) {
     const int h = lane >> 5;
     const bf16x8* W = L.w + sw_layer_off(layer);
@@ -469,15 +496,15 @@ __device__ __forceinline__ void surr_hidden(const SurrLds& L, int layer, int lan
     surr_mfma_tile<KB>(W, bias, 0, lane, h, act, a0, a1);
     __builtin_amdgcn_sched_barrier(0);
     surr_mfma_tile<KB>(W, bias, 1, lane, h, act, b0, b1);
-    surr_act_tile(0, a0, a1, nxt);
+    surr_act_tile<ACT>(0, a0, a1, nxt);
     __builtin_amdgcn_sched_barrier(0);
     surr_mfma_tile<KB>(W, bias, 2, lane, h, act, a0, a1);
-    surr_act_tile(1, b0, b1, nxt);
+    surr_act_tile<ACT>(1, b0, b1, nxt);
     __builtin_amdgcn_sched_barrier(0);
     surr_mfma_tile<KB>(W, bias, 3, lane, h, act, b0, b1);
-    surr_act_tile(2, a0, a1, nxt);
+    surr_act_tile<ACT>(2, a0, a1, nxt);
     __builtin_amdgcn_sched_barrier(0);
-    surr_act_tile(3, b0, b1, nxt);
+    surr_act_tile<ACT>(3, b0, b1, nxt);
 #pragma unroll
     for (int kb = 0; kb < 8; kb++) {
         act[0][kb] = nxt[0][kb];
@@ -487,6 +514,7 @@ __device__ __forceinline__ void surr_hidden(const SurrLds& L, int layer, int lan
 
 // the network on the lane's two points (u of column tile 0 and 1); out[pt] holds the
 // output rows: lanes 0-31 outputs 0..3 (registers 0..3), lanes 32-63 outputs 4, 5 (registers 0, 1)
+template <int ACT>
 __device__ __forceinline__ void surr_mlp64(const SurrLds& L, int lane, const float u1[2], const float u2[2],
                                            f32x16 out[2]) {
     const int h = lane >> 5;
@@ -503,10 +531,10 @@ __device__ __forceinline__ void surr_mlp64(const SurrLds& L, int lane, const flo
                 act[pt][kb][e] = (__bf16)__builtin_amdgcn_sinf(x);
                 act[pt][kb][4 + e] = (__bf16)__builtin_amdgcn_cosf(x);
             }
-    surr_hidden<4>(L, 0, lane, act);
-    surr_hidden<8>(L, 1, lane, act);
-    surr_hidden<8>(L, 2, lane, act);
-    surr_hidden<8>(L, 3, lane, act);
+    surr_hidden<4, ACT>(L, 0, lane, act);
+    surr_hidden<8, ACT>(L, 1, lane, act);
+    surr_hidden<8, ACT>(L, 2, lane, act);
+    surr_hidden<8, ACT>(L, 3, lane, act);
     const bf16x8* W = L.w + sw_layer_off(4);
 #pragma unroll
     for (int pt = 0; pt < 2; pt++)
@@ -608,6 +636,7 @@ __device__ __forceinline__ void surr_store_pair(const SurrDst& o, int n, int i, 
 // One work unit of a wave: the 8 x 8 block of panel pairs (i in 8 bi.., j in 8 bj.., bi <= bj;
 // pairs with i > j are skipped) of one item (wavenumber k).  Pairs inside the trained box get Wu +
 // network; the others are appended to the fallback list as (tag | i << 20 | j), tag = item << 40.
+template <int ACT>
 __device__ __forceinline__ void surr_unit(const SurrLds& L, int lane, int n, int bi, int bj, double k,
                                           const PanelGeom* __restrict__ geom, const SurrDst& dst,
                                           unsigned long long tag, unsigned long long* __restrict__ fb_list,
@@ -636,7 +665,7 @@ __device__ __forceinline__ void surr_unit(const SurrLds& L, int lane, int n, int
         u2[1] = h ? a2 : b2;
     }
     f32x16 out[2];
-    surr_mlp64(L, lane, u1, u2, out);
+    surr_mlp64<ACT>(L, lane, u1, u2, out);
     // outputs 0..3 of this lane's pair (32 h + c): own column tile 0, or lane c's tile 1
     float o[4];
 #pragma unroll
@@ -672,6 +701,7 @@ __device__ __forceinline__ void surr_unit(const SurrLds& L, int lane, int n, int
 
 // Persistent pair-table kernel over all items of the batch: each wave takes units in a strided
 // order.
+template <int ACT>
 __global__ void __launch_bounds__(64 * SURR_WAVES) surr_table_kernel(
     int n, int batch, const PanelGeom* __restrict__ geom, const double* __restrict__ ks,
     const unsigned short* __restrict__ Wb, const float* __restrict__ bias, const float* __restrict__ Bf, SurrOut out,
@@ -687,12 +717,13 @@ __global__ void __launch_bounds__(64 * SURR_WAVES) surr_table_kernel(
         const int item = (int)(unit / per_item);
         int bi, bj;
         tri_tile(unit - (int64_t)item * per_item, bi, bj);
-        surr_unit(L, lane, n, bi, bj, ks[item], geom, surr_dst(out, item), (unsigned long long)item << 40, fb_list,
+        surr_unit<ACT>(L, lane, n, bi, bj, ks[item], geom, surr_dst(out, item), (unsigned long long)item << 40, fb_list,
                   fb_count, fb_cap);
     }
 }
 
 // the same over a queue of items of different bodies (SweepItem::sunit0: an item's first unit)
+template <int ACT>
 __global__ void __launch_bounds__(64 * SURR_WAVES) surr_queue_kernel(
     const SweepItem* __restrict__ items, int nitems, int64_t units, int direct, int include_identity,
     const unsigned short* __restrict__ Wb, const float* __restrict__ bias, const float* __restrict__ Bf,
@@ -711,7 +742,7 @@ __global__ void __launch_bounds__(64 * SURR_WAVES) surr_queue_kernel(
         const SweepItem& it = items[lo];
         int bi, bj;
         tri_tile(unit - it.sunit0, bi, bj);
-        surr_unit(L, lane, it.n, bi, bj, *it.k, it.geom, surr_dst(it, direct, include_identity),
+        surr_unit<ACT>(L, lane, it.n, bi, bj, *it.k, it.geom, surr_dst(it, direct, include_identity),
                   (unsigned long long)lo << 40, fb_list, fb_count, fb_cap);
     }
 }
@@ -768,13 +799,17 @@ int64_t surr_fallback_capacity(int n, int batch) { return (int64_t)n * (n + 1) /
 void launch_surr_table(hipStream_t st, int n, const PanelGeom* geom, const double* ks, int batch,
                        const unsigned short* Wb, const float* bias, const float* Bf, const SurrOut& out,
                        unsigned long long* fb_list, unsigned long long* fb_count, int64_t fb_cap,
-                       unsigned long long* status) {
+                       unsigned long long* status, int act) {
     launch_zero(st, fb_count, sizeof(unsigned long long));
     const int nb = (n + 7) / 8;
     const int64_t units = (int64_t)nb * (nb + 1) / 2 * batch;
     const int64_t blocks = std::min<int64_t>(device_cus(), (units + SURR_WAVES - 1) / SURR_WAVES);
-    surr_table_kernel<<<(int)blocks, 64 * SURR_WAVES, 0, st>>>(n, batch, geom, ks, Wb, bias, Bf, out, fb_list, fb_count,
-                                                               fb_cap);
+    if (act == SURR_ACT_RELU)
+        surr_table_kernel<SURR_ACT_RELU><<<(int)blocks, 64 * SURR_WAVES, 0, st>>>(n, batch, geom, ks, Wb, bias, Bf, out,
+                                                                                 fb_list, fb_count, fb_cap);
+    else
+        surr_table_kernel<SURR_ACT_SILU><<<(int)blocks, 64 * SURR_WAVES, 0, st>>>(n, batch, geom, ks, Wb, bias, Bf, out,
+                                                                                 fb_list, fb_count, fb_cap);
     surr_fallback_kernel<<<2048, 256, 0, st>>>(n, geom, ks, fb_list, fb_count, fb_cap, out, status);
 }
 
@@ -786,17 +821,22 @@ int64_t sweep_surr_units(int n) {
 void launch_queue_surr(hipStream_t st, const SweepItem* items, int nitems, int64_t total_units, int direct,
                        int include_identity, const unsigned short* Wb, const float* bias, const float* Bf,
                        unsigned long long* fb_list, unsigned long long* fb_count, int64_t fb_cap,
-                       unsigned long long* status) {
+                       unsigned long long* status, int act) {
     launch_zero(st, fb_count, sizeof(unsigned long long));
     if (total_units <= 0) return;
     const int64_t blocks = std::min<int64_t>(device_cus(), (total_units + SURR_WAVES - 1) / SURR_WAVES);
-    surr_queue_kernel<<<(int)blocks, 64 * SURR_WAVES, 0, st>>>(items, nitems, total_units, direct, include_identity, Wb,
-                                                               bias, Bf, fb_list, fb_count, fb_cap);
+    if (act == SURR_ACT_RELU)
+        surr_queue_kernel<SURR_ACT_RELU><<<(int)blocks, 64 * SURR_WAVES, 0, st>>>(
+            items, nitems, total_units, direct, include_identity, Wb, bias, Bf, fb_list, fb_count, fb_cap);
+    else
+        surr_queue_kernel<SURR_ACT_SILU><<<(int)blocks, 64 * SURR_WAVES, 0, st>>>(
+            items, nitems, total_units, direct, include_identity, Wb, bias, Bf, fb_list, fb_count, fb_cap);
     surr_queue_fallback_kernel<<<2048, 256, 0, st>>>(items, direct, include_identity, fb_list, fb_count, fb_cap, status);
 }
 
 // the network (MFMA path) at arbitrary (r, v): out[6 m + o], the raw outputs (real parts:
 // SURR_SCALE x residual; imaginary parts: SURR_SCALE exp(min(v, SURR_IM_VC)) x residual)
+template <int ACT>
 __global__ void __launch_bounds__(64 * SURR_WAVES) surr_eval_kernel(int M, const double* __restrict__ rv,
                                                                     const unsigned short* __restrict__ Wb,
                                                                     const float* __restrict__ bias,
@@ -816,7 +856,7 @@ __global__ void __launch_bounds__(64 * SURR_WAVES) surr_eval_kernel(int M, const
             surr_inputs_f32(r, v, u1[pt], u2[pt]);
         }
         f32x16 o[2];
-        surr_mlp64(L, lane, u1, u2, o);
+        surr_mlp64<ACT>(L, lane, u1, u2, o);
 #pragma unroll
         for (int pt = 0; pt < 2; pt++) {
             const int m = 64 * unit + 32 * pt + c;
@@ -833,10 +873,13 @@ __global__ void __launch_bounds__(64 * SURR_WAVES) surr_eval_kernel(int M, const
 }
 
 void launch_surr_eval(hipStream_t st, int M, const double* rv, const unsigned short* Wb, const float* bias,
-                      const float* Bf, float* out) {
+                      const float* Bf, float* out, int act) {
     const int units = (M + 63) / 64;
     const int blocks = std::max(1, std::min(device_cus(), (units + SURR_WAVES - 1) / SURR_WAVES));
-    surr_eval_kernel<<<blocks, 64 * SURR_WAVES, 0, st>>>(M, rv, Wb, bias, Bf, out);
+    if (act == SURR_ACT_RELU)
+        surr_eval_kernel<SURR_ACT_RELU><<<blocks, 64 * SURR_WAVES, 0, st>>>(M, rv, Wb, bias, Bf, out);
+    else
+        surr_eval_kernel<SURR_ACT_SILU><<<blocks, 64 * SURR_WAVES, 0, st>>>(M, rv, Wb, bias, Bf, out);
 }
 
 }  // namespace mhx

@@ -26,6 +26,7 @@ GF_SURROGATE = 2
 MODE_PER_OMEGA = 0
 MODE_PER_PROBLEM = 1
 MODE_BATCHED = 2
+SURR_SILU, SURR_RELU = 0, 1  # MHX_SURR_SILU / MHX_SURR_RELU
 NOALLOC_SCRATCH_BYTES = 16 << 20  # MHX_NOALLOC_SCRATCH_BYTES
 LU_AUTO = 0
 LU_FP64 = 1
@@ -38,7 +39,7 @@ EXPORTED = ["mhx_create", "mhx_destroy", "mhx_last_error", "mhx_set_constants", 
             "mhx_lu_solve", "mhx_lu_solve_batched", "mhx_lu_solve_noalloc", "mhx_hydro_coeffs", "mhx_body_create", "mhx_body_destroy", "mhx_body_coeffs",
             "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work", "mhx_set_batch", "mhx_surrogate_train",
             "mhx_surrogate_save", "mhx_surrogate_load", "mhx_surrogate_params", "mhx_surrogate_eval",
-            "mhx_surrogate_loss_grad", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_set_pivoting",
+            "mhx_surrogate_loss_grad", "mhx_surrogate_set_activation", "mhx_surrogate_get_activation", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_set_pivoting",
             "mhx_pivot_stats", "mhx_hydro_coeffs_jvp",
             "mhx_hydro_coeffs_vjp", "mhx_sweep_coeffs", "mhx_body_create_lid", "mhx_body_coeffs_fs"]
 
@@ -138,6 +139,8 @@ def lib():
         L.mhx_surrogate_params.argtypes = [vp, _fp, _fp, _ip]
         L.mhx_surrogate_eval.argtypes = [vp, C.c_int64, _dp, _fp]
         L.mhx_surrogate_loss_grad.argtypes = [vp, C.c_int64, _fp, _fp, _fp, _fp, _fp, _fp]
+        L.mhx_surrogate_set_activation.argtypes = [vp, C.c_int]
+        L.mhx_surrogate_get_activation.argtypes = [vp, C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -224,9 +227,12 @@ class Context:
         return int(c[0]), int(c[1])
 
     # ---- learned surrogate of the exact wave term ----
-    def surrogate_train(self, ngrid=2048, steps=8000, batch=4096, lr=2e-3, seed=20261203):
-        """Generate ngrid^2 exact samples on the device and train the network; returns the
-        validation MSE (scaled units) on the held-out samples."""
+    def surrogate_train(self, ngrid=2048, steps=8000, batch=4096, lr=2e-3, seed=20261203, activation=None):
+        """Generate ngrid^2 exact samples on the device and train the network (hidden-layer
+        activation "silu" or "relu"; None keeps the context's); returns the validation MSE (scaled
+        units) on the held-out samples."""
+        if activation is not None:
+            self.set_surrogate_activation(activation)
         vl = C.c_double()
         self.check(lib().mhx_surrogate_train(self.h, int(ngrid), int(steps), int(batch), float(lr), int(seed),
                                              C.byref(vl)))
@@ -266,6 +272,16 @@ class Context:
         env = np.exp(-np.minimum(np.asarray(v, float).ravel(), 5.0))
         out[:, 1::2] *= env[:, None]
         return out
+
+    def set_surrogate_activation(self, activation):
+        """"silu" or "relu": the activation of the next training and of surrogate_loss_grad."""
+        code = {"silu": SURR_SILU, "relu": SURR_RELU}[activation]
+        self.check(lib().mhx_surrogate_set_activation(self.h, code))
+
+    def surrogate_activation(self):
+        a = C.c_int()
+        self.check(lib().mhx_surrogate_get_activation(self.h, C.byref(a)))
+        return {SURR_SILU: "silu", SURR_RELU: "relu"}[a.value]
 
     def surrogate_loss_grad(self, U, Y, params, fourier):
         """One FP32 forward/backward of the training kernels: (loss, gradient)."""

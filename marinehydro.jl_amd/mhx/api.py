@@ -165,7 +165,8 @@ def context(device=0):
 
 
 # --------------------------------------------------------------------------- surrogate
-SURROGATE_WEIGHTS = os.path.join(_lib.PKG_ROOT, "weights", "delhommeau_surrogate.mhxs")
+SURROGATE_WEIGHTS = os.path.join(_lib.PKG_ROOT, "weights", "delhommeau_surrogate.mhxs")        # 4 x 128 SiLU
+SURROGATE_WEIGHTS_RELU = os.path.join(_lib.PKG_ROOT, "weights", "delhommeau_surrogate_relu.mhxs")  # ReLU variant
 _surrogate_ready = set()
 
 

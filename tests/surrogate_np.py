@@ -4,7 +4,7 @@ Layout of the flat parameter vector (as libmhx stores it): for each of the five
 layers, W (out x in, row-major) followed by b (out); layer widths
 64 -> 128 -> 128 -> 128 -> 128 -> 6.  Inputs u (m x 2) map to features
 [sin(2 pi u B^T), cos(2 pi u B^T)] with the 32 x 2 Fourier matrix B; hidden
-layers use ReLU; the loss is the mean of squared errors over m x 6 outputs.
+layers use SiLU (or ReLU); the loss is the mean of squared errors over m x 6 outputs.
 """
 import numpy as np
 
@@ -35,6 +35,15 @@ def inputs(r, v):
     return np.stack([u1, u2], 1)
 
 
+def silu(z):
+    return z / (1.0 + np.exp(-z))
+
+
+def dsilu(z):
+    s = 1.0 / (1.0 + np.exp(-z))
+    return s * (1.0 + z * (1.0 - s))
+
+
 def relu(z):
     return np.maximum(z, 0.0)
 
@@ -43,7 +52,10 @@ def drelu(z):
     return (z > 0.0).astype(z.dtype)
 
 
-def forward(params, B, U, keep=False):
+ACT = {"silu": (silu, dsilu), "relu": (relu, drelu)}
+
+
+def forward(params, B, U, keep=False, act="silu"):
     layers = unpack(params)
     p = 2.0 * np.pi * (U @ B.T)
     h = np.concatenate([np.sin(p), np.cos(p)], 1)
@@ -51,14 +63,14 @@ def forward(params, B, U, keep=False):
     for l, (W, b) in enumerate(layers):
         z = h @ W.T + b
         zs.append(z)
-        h = relu(z) if l < 4 else z
+        h = ACT[act][0](z) if l < 4 else z
         hs.append(h)
     return (h, hs, zs) if keep else h
 
 
-def loss_grad(params, B, U, Y):
+def loss_grad(params, B, U, Y, act="silu"):
     """MSE loss and its gradient wrt the flat parameters (reverse mode by hand)."""
-    out, hs, zs = forward(params, B, U, keep=True)
+    out, hs, zs = forward(params, B, U, keep=True, act=act)
     m = U.shape[0]
     d = out - Y
     loss = float(np.mean(d * d))
@@ -69,7 +81,7 @@ def loss_grad(params, B, U, Y):
         W, _ = layers[l]
         grads[l] = (dz.T @ hs[l], dz.sum(0))
         if l > 0:
-            dz = (dz @ W) * drelu(zs[l - 1])
+            dz = (dz @ W) * ACT[act][1](zs[l - 1])
     g = np.concatenate([np.concatenate([gW.ravel(), gb]) for gW, gb in grads])
     return loss, g
 

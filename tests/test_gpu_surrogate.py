@@ -40,25 +40,41 @@ def trained(mhx):
     return mhx.context().surrogate_params()
 
 
-def test_loss_and_gradient_match_numpy(mhx):
-    print("surrogate loss/grad", flush=True)
+def _away_from_kinks(params, B, U, act, margin=1e-4):
+    """Rows whose hidden pre-activations all stay |z| > margin (for ReLU, whose derivative jumps at
+    0: FP32 and FP64 may put a pre-activation this close to 0 on different sides)."""
+    if act != "relu":
+        return np.ones(U.shape[0], dtype=bool)
+    _, _, zs = snp.forward(params, B, U.astype(np.float32).astype(np.float64), keep=True, act=act)
+    return np.all([np.min(np.abs(z), axis=1) > margin for z in zs[:4]], axis=0)
+
+
+@pytest.mark.parametrize("act", ["silu", "relu"])
+def test_loss_and_gradient_match_numpy(mhx, act):
+    print(f"surrogate loss/grad ({act})", flush=True)
     rng = np.random.default_rng(3)
     params, B = snp.init_params(rng)
     ctx = mhx.context()
     assert ctx.surrogate_param_count() == snp.NPARAMS
-    for m in (64, 333, 4096):
-        U = rng.uniform(0, 1, (m, 2))
-        Y = rng.standard_normal((m, 6))
-        p32, B32 = params.astype(np.float32), B.astype(np.float32)
-        loss, g = ctx.surrogate_loss_grad(U, Y, p32, B32)
-        lref, gref = snp.loss_grad(p32.astype(np.float64), B32.astype(np.float64), U.astype(np.float32).astype(
-            np.float64), Y.astype(np.float32).astype(np.float64))
-        assert loss == pytest.approx(lref, rel=1e-5)
-        offs = np.cumsum([0] + [o * i + o for i, o in zip(snp.IN, snp.OUT)])
-        for l in range(5):
-            a, b = offs[l], offs[l + 1]
-            rel = np.linalg.norm(g[a:b] - gref[a:b]) / np.linalg.norm(gref[a:b])
-            assert rel < 2e-4, (m, l, rel)
+    ctx.set_surrogate_activation(act)
+    try:
+        for m in (64, 333, 4096):
+            U = rng.uniform(0, 1, (m, 2))
+            Y = rng.standard_normal((m, 6))
+            p32, B32 = params.astype(np.float32), B.astype(np.float32)
+            keep = _away_from_kinks(p32.astype(np.float64), B32.astype(np.float64), U, act)
+            U, Y = U[keep], Y[keep]
+            loss, g = ctx.surrogate_loss_grad(U, Y, p32, B32)
+            lref, gref = snp.loss_grad(p32.astype(np.float64), B32.astype(np.float64), U.astype(np.float32).astype(
+                np.float64), Y.astype(np.float32).astype(np.float64), act=act)
+            assert loss == pytest.approx(lref, rel=1e-5)
+            offs = np.cumsum([0] + [o * i + o for i, o in zip(snp.IN, snp.OUT)])
+            for l in range(5):
+                a, b = offs[l], offs[l + 1]
+                rel = np.linalg.norm(g[a:b] - gref[a:b]) / np.linalg.norm(gref[a:b])
+                assert rel < 2e-4, (m, l, rel)
+    finally:
+        ctx.set_surrogate_activation("silu")
 
 
 def test_short_training_and_round_trip(mhx, tmp_path):
@@ -186,3 +202,42 @@ def test_coefficients_within_1e3_of_exact(mhx, trained):
         print(f"{key} sweep: surrogate {es_all:.2e}  Wu {ew_all:.2e}  gain {gains[key]:.1f}x", flush=True)
     assert gains["added_mass"] > 10 and gains["excitation_force"] > 10, gains
     assert gains["radiation_damping"] > 1, gains
+
+
+RELU_WEIGHTS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "marinehydro.jl_amd",
+                            "weights", "delhommeau_surrogate_relu.mhxs")
+
+
+@pytest.mark.skipif(not os.path.exists(RELU_WEIGHTS), reason="no ReLU weights file")
+def test_relu_network_inference_and_accuracy(mhx, oracle):
+    """The shipped ReLU variant of the network: the BF16-MFMA inference against the FP32 NumPy
+    network (same bounds as the SiLU network), and (Wu + network) against the oracle's exact term
+    at least ten times closer than Wu alone on G and dG/dr."""
+    print("surrogate relu variant", flush=True)
+    ctx = mhx.context()
+    mhx.load_surrogate(RELU_WEIGHTS)
+    try:
+        assert ctx.surrogate_activation() == "relu"
+        params, B = ctx.surrogate_params()
+        rng = np.random.default_rng(21)
+        r, v = _points(rng, 3000)
+        got = ctx.surrogate_eval(r, v)
+        ref = snp.forward(params.astype(np.float64), B.astype(np.float64), snp.inputs(r, v).astype(np.float32),
+                          act="relu")
+        err = np.abs(got - ref)
+        scale = np.max(np.abs(ref), axis=0)
+        top = scale.max()
+        assert np.all(err.max(0) <= 0.03 * scale + 0.005 * top), (err.max(0), scale)
+        r, v = r[:800], v[:800]
+        net = ctx.surrogate_residual(r, v)
+        e_net, e_wu = [], []
+        for q in range(len(r)):
+            G, Gr, _ = oracle.delhommeau_term(r[q], -v[q])
+            W, Wr, _ = oracle.wu_term(r[q], -v[q])
+            e_net.append([W.real + net[q, 0] - G.real, Wr.real + net[q, 2] - Gr.real])
+            e_wu.append([W.real - G.real, Wr.real - Gr.real])
+        e_net, e_wu = np.asarray(e_net), np.asarray(e_wu)
+        for c in range(2):
+            assert np.sqrt(np.mean(e_net[:, c] ** 2)) < 0.1 * np.sqrt(np.mean(e_wu[:, c] ** 2)), c
+    finally:
+        mhx.load_surrogate()
