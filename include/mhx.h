@@ -255,6 +255,23 @@ int mhx_hydro_coeffs_jvp(mhx_ctx* ctx, const mhx_mesh* mesh, const mhx_mesh_tang
                          double* B_out, double* Fd_out, double* Fk_out, double* dA_out, double* dB_out,
                          double* dFd_out, double* dFk_out);
 
+/* Frequency derivative of the coefficients of a body with a lid and/or at forward speed (the
+ * reference differentiates both in omega with ForwardDiff/Enzyme: test/solve_problem_differentiation.jl:94-98
+ * for a lid body, test/forward_speed_tests.jl:280-303 for a moving one).  mesh: hull panels
+ * [0, nhull) then the lid panels (nhull = nf: no lid), dofs over all panels with zero rows on
+ * the lid (as mhx_body_create_lid).  For every (U, omega, beta) as mhx_body_coeffs_fs (U = 0
+ * included; a nonzero U needs direct = 0 and dgrad), with the directional derivatives along
+ * domegas (nw; NULL = zero) of every output, through the tangent kernels: the encountered
+ * frequency's tangent, dS/dD (or dK) and dK_x along the wavenumber, the tangent right-hand sides
+ * and the tangent solve on the same FP64 LU.  Layouts as mhx_body_coeffs_fs:
+ *   A_out, B_out, dA_out, dB_out   ndof x ndof x nw x nU x nb
+ *   Fd_out, Fk_out, dFd_out, dFk_out  ndof x nw x nb x nU complex.  gf: MHX_GF_WU or MHX_GF_DELHOMMEAU. */
+int mhx_hydro_coeffs_fs_jvp(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t nhull, int64_t ndof, const double* dofs,
+                            int64_t nw, const double* omegas, const double* domegas, int64_t nb, const double* betas,
+                            int64_t nU, const double* speeds, const double* dgrad, int gf, int direct, double* A_out,
+                            double* B_out, double* Fd_out, double* Fk_out, double* dA_out, double* dB_out,
+                            double* dFd_out, double* dFk_out);
+
 /* Reverse-mode gradient of the coefficients: for the real functional
  *   J = sum Abar .* A + sum Bbar .* B + Re sum conj(Fdbar) .* Fd + Re sum conj(Fkbar) .* Fk
  * (cotangents in the layouts of the outputs, Fd/Fk complex interleaved; NULL = zero), returns

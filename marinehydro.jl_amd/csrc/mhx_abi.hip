@@ -1555,6 +1555,210 @@ int mhx_hydro_coeffs_jvp(mhx_ctx* ctx, const mhx_mesh* mesh, const mhx_mesh_tang
     return rc;
 }
 
+// Frequency tangent of the coefficients of a body with an optional lid (panels [nhull, nf)) at
+// any forward speeds (U = 0 included), one (U, beta, omega) group at a time: the encountered
+// frequency and wavenumber with their tangents, the tangent pairs, right-hand sides, solve and
+// pressure (for a moving body also dK_x sigma + K_x dsigma), then A = Re F / omega_e^2,
+// B = Im F / omega_e differentiated through omega_e.
+int mhx_hydro_coeffs_fs_jvp(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t nhull, int64_t ndof, const double* dofs,
+                            int64_t nw, const double* omegas, const double* domegas, int64_t nb, const double* betas,
+                            int64_t nU, const double* Us, const double* dgrad, int gf, int direct, double* A_out,
+                            double* B_out, double* Fd_out, double* Fk_out, double* dA_out, double* dB_out,
+                            double* dFd_out, double* dFk_out) {
+    if (!ctx) return MHX_EINVAL;
+    if (!mesh) return fail(ctx, MHX_EINVAL, "fs_jvp: null mesh");
+    if (gf != MHX_GF_WU && gf != MHX_GF_DELHOMMEAU)
+        return fail(ctx, MHX_EINVAL, "fs_jvp: the tangent is available for GFWu and ExactGuevelDelhommeau");
+    if (nhull <= 0 || nhull > mesh->nf) return fail(ctx, MHX_EDIM, "fs_jvp: nhull must be in 1..nf");
+    if (nw < 0 || nb < 1 || nU < 1 || (nw && !omegas) || !betas || !Us)
+        return fail(ctx, MHX_EINVAL, "fs_jvp: bad frequency, heading or speed arrays (at least one heading and one speed)");
+    if (ndof < 1 || ndof + 1 > 16) return fail(ctx, MHX_EDIM, "fs_jvp: 1..15 dofs");
+    bool moving = false;
+    for (int64_t u = 0; u < nU; u++) {
+        if (!std::isfinite(Us[u])) return fail(ctx, MHX_EINVAL, "forward speed must be finite");
+        moving = moving || Us[u] != 0;
+    }
+    if (moving && direct)
+        return fail(ctx, MHX_EINVAL, "nonzero forward speed is not supported with the direct method; use IndirectBEM (direct = false)");
+    if (moving && !dgrad) return fail(ctx, MHX_EINVAL, "nonzero forward speed: null gradient of motion");
+    for (int64_t w = 0; w < nw; w++)
+        if (!(omegas[w] > 0) || !std::isfinite(omegas[w])) return fail(ctx, MHX_EINVAL, "omega must be positive and finite");
+    if (nw && (!A_out || !B_out || !Fd_out || !Fk_out || !dA_out || !dB_out || !dFd_out || !dFk_out))
+        return fail(ctx, MHX_EINVAL, "fs_jvp: null output");
+    if (mesh->nf > mhx::kMaxOrder) return fail(ctx, MHX_EDIM, "fs_jvp: more than 16384 panels is not supported");
+    hipSetDevice(ctx->device);
+    mhx_body* b = nullptr;
+    int rc = body_upload(ctx, mesh, ndof, dofs, &b);
+    if (rc) return rc;
+    hipStream_t st = ctx->stream;
+    if (nhull < mesh->nf) mhx::launch_mark_lid(st, b->n, (int)nhull, b->geom);
+    const int n = b->n, R = (int)ndof + 1;
+    direct = direct ? 1 : 0;
+    const bool tab = gf == MHX_GF_DELHOMMEAU;
+    const size_t nn = (size_t)n * n, nR = (size_t)n * R, nv = mesh->nv;
+    double *dv = nullptr, *dc = nullptr, *dn = nullptr, *da = nullptr, *dr = nullptr, *d_grad = nullptr,
+           *d_inc = nullptr;
+    int64_t* df = nullptr;
+    mhx::PanelGeomD* gd = nullptr;
+    cplx *dS = nullptr, *dD = nullptr, *dBv = nullptr, *t1 = nullptr, *t2 = nullptr, *t3 = nullptr, *dphi = nullptr,
+         *dsig = nullptr, *Kx = nullptr, *psi = nullptr, *dpsi = nullptr, *dF = nullptr, *dFK = nullptr;
+    auto cleanup = [&]() {
+        dfree(dv); dfree(dc); dfree(dn); dfree(da); dfree(dr); dfree(d_grad); dfree(d_inc); dfree(df); dfree(gd);
+        dfree(dS); dfree(dD); dfree(dBv); dfree(t1); dfree(t2); dfree(t3); dfree(dphi); dfree(dsig); dfree(Kx);
+        dfree(psi); dfree(dpsi); dfree(dF); dfree(dFK);
+        mhx_body_destroy(b);
+    };
+    if ((rc = ensure_work(ctx, b, R, 1, tab, 1)) || (rc = ensure_rankine(ctx, b, direct)) ||
+        (rc = dalloc(ctx, &dv, 3 * nv)) || (rc = dalloc(ctx, &df, 4 * (size_t)n)) || (rc = dalloc(ctx, &dc, 3 * (size_t)n)) ||
+        (rc = dalloc(ctx, &dn, 3 * (size_t)n)) || (rc = dalloc(ctx, &da, n)) || (rc = dalloc(ctx, &dr, n)) ||
+        (rc = dalloc(ctx, &d_grad, 3 * (size_t)ndof)) || (rc = dalloc(ctx, &d_inc, 3)) || (rc = dalloc(ctx, &gd, n)) ||
+        (rc = dalloc(ctx, &dS, nn)) || (rc = dalloc(ctx, &dD, nn)) || (rc = dalloc(ctx, &dBv, nR)) ||
+        (rc = dalloc(ctx, &t1, nR)) || (rc = dalloc(ctx, &t2, nR)) || (rc = dalloc(ctx, &t3, nR)) ||
+        (rc = dalloc(ctx, &dphi, nR)) || (rc = dalloc(ctx, &dsig, nR)) || (rc = dalloc(ctx, &psi, nR)) ||
+        (rc = dalloc(ctx, &dpsi, nR)) || (rc = dalloc(ctx, &dF, (size_t)ndof * R)) || (rc = dalloc(ctx, &dFK, (size_t)ndof)) ||
+        (moving && ((rc = dalloc(ctx, &Kx, nn)) || (!b->KRx && (rc = dalloc(ctx, &b->KRx, nn)))))) {
+        cleanup();
+        return rc;
+    }
+    auto up = [&](double* dst, const double* src, size_t cnt) {
+        hipMemcpyAsync(dst, src, sizeof(double) * cnt, hipMemcpyHostToDevice, st);
+    };
+    up(dv, mesh->vertices, 3 * nv);
+    up(dc, mesh->centers, 3 * (size_t)n);
+    up(dn, mesh->normals, 3 * (size_t)n);
+    up(da, mesh->areas, n);
+    up(dr, mesh->radii, n);
+    hipMemcpyAsync(df, mesh->faces, sizeof(int64_t) * 4 * n, hipMemcpyHostToDevice, st);
+    if (dgrad) up(d_grad, dgrad, 3 * (size_t)ndof);
+    else mhx::launch_zero(st, d_grad, sizeof(double) * 3 * ndof);
+    // the mesh is fixed: geometry with zero tangents
+    mhx::launch_tangent_geometry(st, n, dv, nullptr, df, dc, nullptr, dn, nullptr, da, nullptr, dr, gd);
+    if (moving) mhx::launch_rankine_x(st, n, b->geom, b->KRx);
+    auto ok = [&](hipError_t e, const char* what) {
+        if (e == hipSuccess) return true;
+        rc = fail(ctx, MHX_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+        return false;
+    };
+    if (!ok(hipGetLastError(), "fs_jvp set-up")) {
+        cleanup();
+        return rc;
+    }
+    const double g = ctx->g;
+    const size_t nd = ndof;
+    std::vector<cplx> F(nd * R), FK(nd), hdF(nd * R), hdFK(nd);
+    auto aidx = [&](size_t d, size_t r, int64_t iw, int64_t iu, int64_t ib) {
+        return d + nd * (r + nd * (iw + (size_t)nw * (iu + (size_t)nU * ib)));
+    };
+    auto fidx = [&](size_t d, int64_t iw, int64_t ib, int64_t iu) { return d + nd * (iw + (size_t)nw * (ib + (size_t)nb * iu)); };
+    const int64_t cnt = (int64_t)n * R;
+    cplx* W2 = b->work + 16 * b->nR();
+    for (int64_t iu = 0; iu < nU && !rc; iu++) {
+        const double U = Us[iu];
+        for (int64_t ib = 0; ib < nb && !rc; ib++) {
+            const double beta = betas[ib];
+            for (int64_t iw = 0; iw < nw && !rc; iw++) {
+                // encountered frequency (waves.jl:136-151) and its tangent
+                const double w = omegas[iw], dw = domegas ? domegas[iw] : 0.0;
+                const double k = w * w / g, dk = 2.0 * w * dw / g;
+                const double dop = w - k * U * cos(beta), sgn = dop >= 0 ? 1.0 : -1.0;
+                const double we = fabs(dop), dwe = sgn * (dw - dk * U * cos(beta));
+                const double ke = we * we / g, dke = 2.0 * we * dwe / g;
+                const double beta_e = dop >= 0 ? beta : beta + MHX_PI;
+                if (!(we > 0)) {
+                    rc = fail(ctx, MHX_EINVAL, "forward speed: zero encountered frequency");
+                    break;
+                }
+                if ((rc = set_freqs(ctx, b, &we, 1))) break;
+                mhx::launch_bc_fs(st, n, b->geom, b->dofs, (int)ndof, d_grad, we, U, k, w, beta, beta_e, g, b->B);
+                // radiation columns: d(-i omega_e n.dof) (the U n.dgrad term is constant); the
+                // diffraction column along the incident (omega, k)
+                mhx::launch_tangent_bc(st, n, gd, b->dofs, nullptr, (int)ndof, nullptr, 0, we, dwe, ke, dke, g, dBv);
+                mhx::launch_tangent_bc_inc(st, n, (int)nhull, gd, w, dw, k, dk, beta, beta_e, g, dBv + (size_t)n * ndof);
+                if ((rc = assemble_into(ctx, b, 1, gf, direct, 1)) || (rc = check_status(ctx, b, gf))) break;
+                mhx::launch_tangent_pairs(st, n, gd, ke, dke, direct, tab, b->WG, b->WR, dS, dD);
+                if ((rc = solve_fp64(ctx, b, R, direct, 1))) {
+                    if (rc == MHX_ESINGULAR) fail(ctx, rc, ctx->err);
+                    break;
+                }
+                if (direct) {
+                    // dphi = D^{-1} (dS B + S dB - dD phi)
+                    mhx::launch_matmul_nr(st, n, R, dS, 0, b->B, 0, t1, 0, b->work, 0, 1);
+                    mhx::launch_matmul_nr(st, n, R, b->S, 0, dBv, 0, t2, 0, b->work, 0, 1);
+                    mhx::launch_matmul_nr(st, n, R, dD, 0, b->Y, 0, t3, 0, b->work, 0, 1);
+                    mhx::launch_axpby(st, cnt, t1, 1.0, t2, dphi);
+                    mhx::launch_axpby(st, cnt, dphi, -1.0, t3, dphi);
+                    mhx::getrs_batched(st, 0, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, dphi, 0, W2, 0, 1, true);
+                } else {
+                    // sigma in B: dsigma = K^{-1} (dB - dK sigma); dphi = dS sigma + S dsigma
+                    mhx::launch_matmul_nr(st, n, R, dD, 0, b->B, 0, t3, 0, b->work, 0, 1);
+                    mhx::launch_axpby(st, cnt, dBv, -1.0, t3, dsig);
+                    mhx::getrs_batched(st, 0, n, R, b->D, n, 0, b->ipiv, n, b->perm, n, dsig, 0, W2, 0, 1, true);
+                    mhx::launch_matmul_nr(st, n, R, dS, 0, b->B, 0, t1, 0, b->work, 0, 1);
+                    mhx::launch_matmul_nr(st, n, R, b->S, 0, dsig, 0, t3, 0, b->work, 0, 1);
+                    mhx::launch_axpby(st, cnt, t1, 1.0, t3, dphi);
+                }
+                if (U != 0) {
+                    // phi_eff = phi - i (U / omega_e) psi, psi = K_x sigma (solve.jl:35-45);
+                    // dpsi = dK_x sigma + K_x dsigma
+                    mhx::launch_kx_combine(st, n, b->geom, b->KRx, ke, tab, tab ? b->WR : nullptr, Kx);
+                    mhx::launch_tangent_kx(st, n, gd, ke, dke, tab, b->WG, b->WR, dS);
+                    mhx::launch_matmul_nr(st, n, R, Kx, 0, b->B, 0, psi, 0, b->work, 0, 1);
+                    mhx::launch_matmul_nr(st, n, R, dS, 0, b->B, 0, t1, 0, b->work, 0, 1);
+                    mhx::launch_matmul_nr(st, n, R, Kx, 0, dsig, 0, t2, 0, b->work, 0, 1);
+                    mhx::launch_axpby(st, cnt, t1, 1.0, t2, dpsi);
+                    // dphi_eff = dphi - i U (dpsi / omega_e - psi domega_e / omega_e^2)
+                    mhx::launch_caxpy(st, cnt, 0.0, -U / we, dpsi, dphi);
+                    mhx::launch_caxpy(st, cnt, 0.0, U * dwe / (we * we), psi, dphi);
+                    mhx::launch_caxpy(st, cnt, 0.0, -U / we, psi, b->Y);
+                }
+                // forces at omega_e; Froude-Krylov with the incident omega, k along beta_e
+                // (d_inc = {beta_e, omega, k})
+                const double hinc[3] = {beta_e, w, k};
+                hipMemcpyAsync(d_inc, hinc, sizeof hinc, hipMemcpyHostToDevice, st);
+                mhx::launch_forces(st, n, b->geom, b->dofs, (int)ndof, b->Y, b->nR(), R, ctx->rho, b->d_ws, b->d_ks, 1, g,
+                                   nullptr, 0, b->F, b->FK);
+                mhx::launch_tangent_forces(st, n, gd, b->dofs, nullptr, (int)ndof, b->Y, dphi, R, ctx->rho, we, dwe, ke,
+                                           dke, g, nullptr, 0, dF, dFK);
+                mhx::launch_forces(st, n, b->geom, b->dofs, (int)ndof, b->Y, b->nR(), 0, ctx->rho, d_inc + 1, d_inc + 2, 1, g,
+                                   d_inc, 1, b->F, b->FK);
+                mhx::launch_tangent_forces(st, n, gd, b->dofs, nullptr, (int)ndof, b->Y, dphi, 0, ctx->rho, w, dw, k, dk,
+                                           g, d_inc, 1, dF, dFK);
+                const bool copied =
+                    ok(hipGetLastError(), "fs_jvp launch") &&
+                    ok(hipMemcpyAsync(F.data(), b->F, sizeof(cplx) * nd * R, hipMemcpyDeviceToHost, st), "copy F") &&
+                    ok(hipMemcpyAsync(hdF.data(), dF, sizeof(cplx) * nd * R, hipMemcpyDeviceToHost, st), "copy dF") &&
+                    ok(hipMemcpyAsync(FK.data(), b->FK, sizeof(cplx) * nd, hipMemcpyDeviceToHost, st), "copy FK") &&
+                    ok(hipMemcpyAsync(hdFK.data(), dFK, sizeof(cplx) * nd, hipMemcpyDeviceToHost, st), "copy dFK") &&
+                    ok(hipStreamSynchronize(st), "sync");
+                if (!copied) break;
+                for (size_t r = 0; r < (size_t)R; r++)
+                    for (size_t d = 0; d < nd; d++) {
+                        const cplx f = F[d + nd * r], tf = hdF[d + nd * r];
+                        if (r < nd) {
+                            A_out[aidx(d, r, iw, iu, ib)] = f.x / (we * we);
+                            B_out[aidx(d, r, iw, iu, ib)] = f.y / we;
+                            dA_out[aidx(d, r, iw, iu, ib)] = tf.x / (we * we) - 2.0 * f.x * dwe / (we * we * we);
+                            dB_out[aidx(d, r, iw, iu, ib)] = tf.y / we - f.y * dwe / (we * we);
+                        } else {
+                            const size_t o = fidx(d, iw, ib, iu);
+                            Fd_out[2 * o] = f.x;
+                            Fd_out[2 * o + 1] = f.y;
+                            dFd_out[2 * o] = tf.x;
+                            dFd_out[2 * o + 1] = tf.y;
+                            Fk_out[2 * o] = FK[d].x;
+                            Fk_out[2 * o + 1] = FK[d].y;
+                            dFk_out[2 * o] = hdFK[d].x;
+                            dFk_out[2 * o + 1] = hdFK[d].y;
+                        }
+                    }
+            }
+        }
+    }
+    flush_timing(ctx);
+    cleanup();
+    return rc;
+}
+
 // Gradient of J = <Abar, A> + <Bbar, B> + Re<Fdbar, Fd> + Re<Fkbar, Fk> (reverse mode): per
 // frequency the primal omega-solve (FP64 LU), the pressure pullback, the transposed solve on the
 // same LU factors, and the right-hand-side and pair pullbacks accumulated into the gradients.

@@ -30,6 +30,13 @@ void launch_tangent_geometry(hipStream_t st, int n, const double* verts, const d
 // (G, dG/dr) table WG, WR of the primal assembly, else the Wu fit
 void launch_tangent_pairs(hipStream_t st, int n, const PanelGeomD* geom, double k, double dk, int direct, bool exact,
                           const cplx* WG, const cplx* WR, cplx* dS, cplx* dD);
+// dK_x (n x n): the forward-speed operator (indirect, field normals e_x) along k, mesh fixed
+void launch_tangent_kx(hipStream_t st, int n, const PanelGeomD* geom, double k, double dk, bool exact, const cplx* WG,
+                       const cplx* WR, cplx* dKx);
+// tangent of the forward-speed diffraction column (incident omega, k; phase heading beta_e,
+// velocity heading beta; zero on the lid panels [nhull, n)) along (omega, k)
+void launch_tangent_bc_inc(hipStream_t st, int n, int nhull, const PanelGeomD* geom, double w, double dw, double k,
+                           double dk, double beta, double beta_e, double g, cplx* dB);
 // tangent of the right-hand sides (ndof radiation columns, then nb diffraction columns)
 void launch_tangent_bc(hipStream_t st, int n, const PanelGeomD* geom, const double* dofs, const double* ddofs, int ndof,
                        const double* betas, int nb, double w, double dw, double k, double dk, double g, cplx* dB);

@@ -518,6 +518,51 @@ __global__ void __launch_bounds__(256) tangent_pair_kernel(int n, const PanelGeo
     dD[idx] = cmk(out[2].d[0], out[3].d[0]);
 }
 
+// dK_x of every pair: the indirect operator with every field-panel normal replaced by e_x (the
+// forward-speed operator of solve.jl:35-45, launch_kx_combine), tangent along k only (the mesh is
+// fixed); the identity term on the diagonal does not depend on k
+template <bool EXACT>
+__global__ void __launch_bounds__(256) tangent_kx_kernel(int n, const PanelGeomD* __restrict__ geom, double kv,
+                                                         double kd, const cplx* __restrict__ WG,
+                                                         const cplx* __restrict__ WR, cplx* __restrict__ dKx) {
+    const int j = blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const PanelGeomD& gi = geom[i];
+    const int64_t idx = i + (int64_t)n * j;
+    const cplx wg = EXACT ? WG[idx] : cmk(0.0, 0.0), wr = EXACT ? WR[idx] : cmk(0.0, 0.0);
+    const Dual ex[3] = {dmk(1.0, 0.0), dmk(0.0, 0.0), dmk(0.0, 0.0)};
+    Dual out[4];
+    pair_eval<1, EXACT>(gi.c, ex, geom[j], dmk(kv, kd), 0, wg, wr, out);
+    dKx[idx] = cmk(out[2].d[0], out[3].d[0]);
+}
+
+// tangent of the forward-speed diffraction condition (bc_fs_kernel: the incident wave of
+// wavenumber k and frequency omega, phase along the encountered heading beta_e, velocity along
+// the incident heading beta; hull panels [0, nhull) only) along (omega, k)
+__global__ void tangent_bc_inc_kernel(int n, int nhull, const PanelGeomD* __restrict__ geom, double wv, double wd,
+                                      double kv, double kd, double beta, double beta_e, double g,
+                                      cplx* __restrict__ dB) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (i >= nhull) {
+        dB[i] = cmk(0.0, 0.0);
+        return;
+    }
+    const PanelGeomD& gi = geom[i];
+    const Dual omega = dmk(wv, wd), k = dmk(kv, kd);
+    const Dual wbar = gi.c[0] * cos(beta_e) + gi.c[1] * sin(beta_e);
+    const Dual cih = dexp(k * gi.c[2]);
+    Dual sn, cs;
+    dsincos(k * wbar, sn, cs);
+    const Dual am = g * k / omega;
+    const CDualN<1> amp = cdmk(am * cs, am * sn);
+    const CDualN<1> v0 = cdscale(cih, cdscale(dmk(cos(beta)), amp)), v1 = cdscale(cih, cdscale(dmk(sin(beta)), amp));
+    const CDualN<1> v2 = cdscale(cih, cdmk(amp.y, -amp.x));  // -i * amp
+    const CDualN<1> dot = cdadd(cdadd(cdscale(gi.nrm[0], v0), cdscale(gi.nrm[1], v1)), cdscale(gi.nrm[2], v2));
+    dB[i] = cmk(-dot.x.d[0], -dot.y.d[0]);
+}
+
 // radiation_bc (waves.jl:183-211) and _airy_bc (waves.jl:266-282) tangents
 __global__ void tangent_bc_kernel(int n, const PanelGeomD* __restrict__ geom, const double* __restrict__ dofs,
                                   const double* __restrict__ ddofs, int ndof, const double* __restrict__ betas, int nb,
@@ -916,6 +961,18 @@ void launch_tangent_pairs(hipStream_t st, int n, const PanelGeomD* geom, double 
         tangent_pair_kernel<true><<<grid, 256, 0, st>>>(n, geom, k, dk, direct, WG, WR, dS, dD);
     else
         tangent_pair_kernel<false><<<grid, 256, 0, st>>>(n, geom, k, dk, direct, WG, WR, dS, dD);
+}
+
+void launch_tangent_kx(hipStream_t st, int n, const PanelGeomD* geom, double k, double dk, bool exact, const cplx* WG,
+                       const cplx* WR, cplx* dKx) {
+    dim3 grid((n + 255) / 256, n);
+    if (exact) tangent_kx_kernel<true><<<grid, 256, 0, st>>>(n, geom, k, dk, WG, WR, dKx);
+    else tangent_kx_kernel<false><<<grid, 256, 0, st>>>(n, geom, k, dk, WG, WR, dKx);
+}
+
+void launch_tangent_bc_inc(hipStream_t st, int n, int nhull, const PanelGeomD* geom, double w, double dw, double k,
+                           double dk, double beta, double beta_e, double g, cplx* dB) {
+    tangent_bc_inc_kernel<<<(n + 255) / 256, 256, 0, st>>>(n, nhull, geom, w, dw, k, dk, beta, beta_e, g, dB);
 }
 
 void launch_tangent_bc(hipStream_t st, int n, const PanelGeomD* geom, const double* dofs, const double* ddofs, int ndof,

@@ -39,7 +39,7 @@ EXPORTED = ["mhx_create", "mhx_destroy", "mhx_last_error", "mhx_set_constants", 
             "mhx_lu_solve", "mhx_lu_solve_batched", "mhx_lu_solve_noalloc", "mhx_hydro_coeffs", "mhx_body_create", "mhx_body_destroy", "mhx_body_coeffs",
             "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work", "mhx_set_batch", "mhx_surrogate_train",
             "mhx_surrogate_save", "mhx_surrogate_load", "mhx_surrogate_params", "mhx_surrogate_eval",
-            "mhx_surrogate_loss_grad", "mhx_surrogate_set_activation", "mhx_surrogate_get_activation", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_set_pivoting",
+            "mhx_surrogate_loss_grad", "mhx_surrogate_set_activation", "mhx_surrogate_get_activation", "mhx_hydro_coeffs_fs_jvp", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_set_pivoting",
             "mhx_pivot_stats", "mhx_hydro_coeffs_jvp",
             "mhx_hydro_coeffs_vjp", "mhx_sweep_coeffs", "mhx_body_create_lid", "mhx_body_coeffs_fs"]
 
@@ -139,6 +139,8 @@ def lib():
         L.mhx_surrogate_params.argtypes = [vp, _fp, _fp, _ip]
         L.mhx_surrogate_eval.argtypes = [vp, C.c_int64, _dp, _fp]
         L.mhx_surrogate_loss_grad.argtypes = [vp, C.c_int64, _fp, _fp, _fp, _fp, _fp, _fp]
+        L.mhx_hydro_coeffs_fs_jvp.argtypes = [vp, C.POINTER(CMesh), C.c_int64, C.c_int64, _dp, C.c_int64, _dp, _dp,
+                                               C.c_int64, _dp, C.c_int64, _dp, _dp, C.c_int, C.c_int] + [_dp] * 8
         L.mhx_surrogate_set_activation.argtypes = [vp, C.c_int]
         L.mhx_surrogate_get_activation.argtypes = [vp, C.POINTER(C.c_int)]
         _lib = L

@@ -614,8 +614,8 @@ def hydrodynamic_coefficients_jvp(floatingbody, parameters, tangent, formulation
     formulation = DirectBEM() if formulation is None else formulation
     if formulation.gf not in (_lib.GF_WU, _lib.GF_DELHOMMEAU):
         raise ValueError("derivatives are available for GFWu and ExactGuevelDelhommeau")
-    if getattr(floatingbody, "lid_mesh", None) is not None:
-        raise ValueError("derivatives of a body with a lid are not supported")
+    if getattr(floatingbody, "lid_mesh", None) is not None or "forward_speeds" in parameters:
+        return _coefficients_domega(floatingbody, parameters, tangent, formulation)
     body = floatingbody
     mesh = body.mesh
     ws = np.ascontiguousarray(np.atleast_1d(np.asarray(parameters["wave_frequencies"], dtype=np.float64)))
@@ -686,6 +686,80 @@ def hydrodynamic_coefficients_jvp(floatingbody, parameters, tangent, formulation
         else:
             out.update(diffraction_force=np.zeros(0), Froude_Krylov_force=np.zeros(0),
                        excitation_force=np.zeros(0))
+        return out
+
+    return label(A, B, Fd, Fk), label(tA, tB, tFd, tFk)
+
+
+def _coefficients_domega(body, parameters, tangent, formulation):
+    """Frequency tangent of hydrodynamic_coefficients for a body with a lid and/or forward speeds
+    (mhx_hydro_coeffs_fs_jvp): the reference differentiates both in omega (ForwardDiff/Enzyme,
+    test/solve_problem_differentiation.jl:94-98, test/forward_speed_tests.jl:280-303).  Only the
+    "wave_frequencies" entry of the tangent is taken; a mesh or dof tangent is an error here.
+    Layouts as hydrodynamic_coefficients (with forward_speeds as its forward-speed layout)."""
+    if set(tangent) - {"wave_frequencies"}:
+        raise ValueError("for a body with a lid or forward speeds only the wave_frequencies tangent is available")
+    ws = np.ascontiguousarray(np.atleast_1d(np.asarray(parameters["wave_frequencies"], dtype=np.float64)))
+    dW = tangent.get("wave_frequencies")
+    dW = None if dW is None else np.ascontiguousarray(np.broadcast_to(np.asarray(dW, dtype=np.float64), ws.shape))
+    inf = list(parameters.get("influenced_dofs", body.dof_names))
+    rad = list(parameters.get("radiating_dofs", []))
+    betas = list(parameters.get("wave_directions", []))
+    if not betas:
+        raise ValueError("the frequency tangent needs wave_directions")
+    fs = "forward_speeds" in parameters
+    us = np.ascontiguousarray(np.atleast_1d(np.asarray(parameters.get("forward_speeds", [0.0]), dtype=np.float64)))
+    for nm in inf + rad:
+        if nm not in body.dofs:
+            raise ValueError("each influenced_dofs Symbol must be a key of floatingbody.dofs")
+    names = inf + [r for r in rad if r not in inf]
+    nd, nw, nb, nu = len(names), ws.size, len(betas), us.size
+    dgrad = np.zeros((nd, 3))
+    if np.any(us != 0):
+        if formulation.direct:
+            raise ValueError("nonzero forward speed is not supported with the direct method; use IndirectBEM (direct = false)")
+        for i, nm in enumerate(names):
+            if nm in rad:
+                dgrad[i] = _motion_gradient(nm)
+    lid = getattr(body, "lid_mesh", None)
+    if lid is None:
+        mesh, nhull = body.mesh, body.mesh.nfaces
+        dofs = body._dof_stack(names)
+    else:
+        from .meshes import combine
+        mesh, nhull = combine(body.mesh, lid), body.mesh.nfaces
+        dofs = np.ascontiguousarray(np.concatenate(
+            [np.concatenate([body.dofs[nm], np.zeros((lid.nfaces, 3))]).ravel(order="F") for nm in names]))
+    be = np.ascontiguousarray(betas, dtype=np.float64)
+    dgrad = np.ascontiguousarray(dgrad)
+    ctx = context()
+    ctx.set_constants(SETTINGS.rho, SETTINGS.g)
+    A = np.zeros((nd, nd, nw, nu, nb), order="F")
+    B = np.zeros_like(A)
+    Fd = np.zeros((nd, nw, nb, nu), dtype=np.complex128, order="F")
+    Fk = np.zeros_like(Fd)
+    tA, tB, tFd, tFk = (np.zeros_like(x) for x in (A, B, Fd, Fk))
+    cm = _lib.cmesh(mesh)
+    ctx.check(_lib.lib().mhx_hydro_coeffs_fs_jvp(
+        ctx.h, _lib.C.byref(cm), nhull, nd, _lib.ptr(dofs), nw, _lib.ptr(ws), _lib.ptr(dW), nb, _lib.ptr(be), nu,
+        _lib.ptr(us), _lib.ptr(dgrad), formulation.gf, int(formulation.direct),
+        *(_lib.ptr(x) for x in (A, B, Fd, Fk, tA, tB, tFd, tFk))))
+    ii = [names.index(nm) for nm in inf]
+    rr = [names.index(nm) for nm in rad]
+
+    def label(A, B, Fd, Fk):
+        out = {}
+        if rad:
+            a, b = A[np.ix_(ii, rr)], B[np.ix_(ii, rr)]
+            if not fs:   # (n_inf, n_rad, n_omega, 1): the plain layout, one heading's radiation results
+                a, b = a[:, :, :, 0, 0][..., None], b[:, :, :, 0, 0][..., None]
+            out["added_mass"], out["radiation_damping"] = a, b
+        else:
+            out["added_mass"] = out["radiation_damping"] = np.zeros(0)
+        fd, fk = Fd[ii], Fk[ii]
+        if not fs:       # (n_inf, n_omega, n_beta, 1)
+            fd, fk = fd[..., 0][..., None], fk[..., 0][..., None]
+        out.update(diffraction_force=fd, Froude_Krylov_force=fk, excitation_force=fd + fk)
         return out
 
     return label(A, B, Fd, Fk), label(tA, tB, tFd, tFk)
