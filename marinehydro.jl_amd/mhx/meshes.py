@@ -122,24 +122,72 @@ class Mesh:
         return translated(self, (0.0, 0.0, d))
 
     def immersed_part(self, tol=1e-12):
-        """The faces at or below the free surface z = 0, unused vertices dropped (kept in order),
-        face properties recomputed.  Vertices within tol * (mesh size) of z = 0 are put on it, as a
-        clip at the plane does.  A face with vertices on both sides of z = 0 would have to be cut;
-        that is not done here (ValueError): mesh with a vertex ring on the waterline."""
+        """Capytaine's keep_immersed_part (the reference's meshes are the immersed parts of
+        Capytaine meshes: test/consistency_with_Capytaine.jl): the faces at or below the free
+        surface z = 0, with every face that crosses it cut at the waterline.  Vertices within
+        tol * (mesh size) of z = 0 are put on it.  A cut face keeps its part below z = 0 (polygon
+        clipping against the half-space z <= 0): a quadrangle with one vertex above becomes a
+        pentagon, split into a quadrangle and a triangle; with two above, a quadrangle; with
+        three, a triangle (triangles stored as (a, b, c, a)).  The points where an edge meets
+        z = 0 are shared by the faces on both sides of the edge.  Faces with no vertex strictly
+        below z = 0 are dropped; unused vertices are dropped (the others keep their order)."""
         V = self.vertices.copy()
         scale = max(1.0, float(np.max(np.abs(V))) if V.size else 1.0)
         V[np.abs(V[:, 2]) <= tol * scale, 2] = 0.0
-        F = self.faces - 1
-        above = V[F, 2] > 0.0
-        below = V[F, 2] < 0.0
-        if np.any(above.any(axis=1) & below.any(axis=1)):
-            raise ValueError("immersed_part: a face crosses z = 0 (clipping faces is not supported)")
-        keep = ~above.any(axis=1) & below.any(axis=1)
-        F = F[keep]
+        verts = [tuple(v) for v in V]
+        cut = {}   # (i, j) sorted original vertex pair -> new vertex (0-based)
+
+        def cross_point(i, j):
+            key = (min(i, j), max(i, j))
+            if key not in cut:
+                p, q = V[key[0]], V[key[1]]
+                t = p[2] / (p[2] - q[2])
+                x = p + t * (q - p)
+                x[2] = 0.0
+                cut[key] = len(verts)
+                verts.append(tuple(x))
+            return cut[key]
+
+        faces = []
+        for f in self.faces - 1:
+            poly = [int(f[0])]
+            for v in f[1:]:
+                if int(v) != poly[-1]:
+                    poly.append(int(v))
+            if len(poly) > 1 and poly[-1] == poly[0]:
+                poly.pop()                      # triangle stored as (a, b, c, a)
+            z = V[poly, 2]
+            if not np.any(z < 0.0):
+                continue                        # at or above the surface
+            if not np.any(z > 0.0):
+                faces.append(list(f))           # wholly immersed: unchanged
+                continue
+            out = []
+            for e in range(len(poly)):
+                i, j = poly[e], poly[(e + 1) % len(poly)]
+                if V[i, 2] <= 0.0:
+                    out.append(i)
+                if (V[i, 2] < 0.0 < V[j, 2]) or (V[j, 2] < 0.0 < V[i, 2]):
+                    out.append(cross_point(i, j))
+            if len(out) == 3:
+                faces.append([out[0], out[1], out[2], out[0]])
+            elif len(out) == 4:
+                faces.append(out)
+            elif len(out) == 5:
+                faces.append(out[:4])
+                faces.append([out[0], out[3], out[4], out[0]])
+            else:
+                raise ValueError(f"immersed_part: unexpected cut polygon with {len(out)} vertices")
+        if not faces:
+            return Mesh(np.zeros((0, 3)), np.zeros((0, 4), dtype=np.int64))
+        F = np.asarray(faces, dtype=np.int64)
+        Vn = np.asarray(verts, dtype=np.float64)
         used = np.unique(F)
-        remap = np.full(V.shape[0], -1, dtype=np.int64)
+        remap = np.full(Vn.shape[0], -1, dtype=np.int64)
         remap[used] = np.arange(used.size)
-        return Mesh(V[used], remap[F] + 1)
+        return Mesh(Vn[used], remap[F] + 1)
+
+    keep_immersed_part = immersed_part
 
 
 def face_properties(vertices, faces):
@@ -348,6 +396,71 @@ def mesh_sphere(radius=1.0, center=(0.0, 0.0, 0.0), resolution=(10, 10)):
     ntheta, nphi = resolution
     m = sphere(radius, ntheta, nphi, immersed=False)
     return translated(m, center) if any(c != 0.0 for c in center) else m
+
+
+def mesh_horizontal_cylinder(length=10.0, radius=1.0, center=(0.0, 0.0, 0.0), resolution=(2, 8, 10),
+                             faces_max_radius=None):
+    """Capytaine's mesh_horizontal_cylinder (the mesher of the reference's horizontal-cylinder
+    tests, test/consistency_with_Capytaine.jl:211-216, 313-326): a closed cylinder along the x
+    axis, resolution = (nr, ntheta, nx) panels along a radius of the end disks, around the
+    circumference and along the length.  With faces_max_radius the counts are the smallest that
+    keep every face's radius (largest centre-to-vertex distance) at or below it, taking square
+    faces of side sqrt(2) faces_max_radius: nr = ceil(R / s), ntheta = ceil(2 pi R / s),
+    nx = ceil(L / s).  The circumference starts at the +y side (theta measured from +y towards
+    -z), so that an even ntheta puts a vertex row on each side of the waterline plane through
+    the axis.  Normals point out of the body; each end disk's centre ring is triangles.
+    .immersed_part() keeps the wetted part."""
+    R, L = float(radius), float(length)
+    if faces_max_radius is not None:
+        s = np.sqrt(2.0) * float(faces_max_radius)
+        nr, nth, nx = (max(1, int(np.ceil(R / s))), max(3, int(np.ceil(2 * np.pi * R / s))),
+                       max(1, int(np.ceil(L / s))))
+    else:
+        nr, nth, nx = (int(r) for r in resolution)
+    th = np.arange(nth) * (2 * np.pi / nth)
+    ys, zs = R * np.cos(th), -R * np.sin(th)          # theta = 0 at +y, increasing towards -z
+    xs = np.linspace(-L / 2, L / 2, nx + 1)
+    verts, faces = [], []
+
+    def add(v):
+        verts.append(v)
+        return len(verts)
+
+    side = [[add((x, ys[t], zs[t])) for t in range(nth)] for x in xs]
+    for a in range(nx):
+        for t in range(nth):
+            t1 = (t + 1) % nth
+            # outward: (t1 - t) is along -theta x ... ordering fixed by the normal check below
+            faces.append([side[a][t], side[a][t1], side[a + 1][t1], side[a + 1][t]])
+    for end, x in ((0, xs[0]), (1, xs[-1])):
+        rings = [side[0 if end == 0 else nx]]
+        for k in range(nr - 1, 0, -1):
+            rr = R * k / nr
+            rings.append([add((x, rr * np.cos(t), -rr * np.sin(t))) for t in th])
+        c = add((x, 0.0, 0.0))
+        for k in range(len(rings)):
+            outer = rings[k]
+            inner = rings[k + 1] if k + 1 < len(rings) else None
+            for t in range(nth):
+                t1 = (t + 1) % nth
+                if inner is None:
+                    faces.append([c, outer[t], outer[t1], c])
+                else:
+                    faces.append([outer[t], outer[t1], inner[t1], inner[t]])
+    m = Mesh(np.asarray(verts, dtype=np.float64), np.asarray(faces, dtype=np.int64))
+    # orient every face outwards: lateral faces away from the axis, end faces along +-x
+    Fo = m.faces.copy()
+    cen = m.centers
+    radial = np.stack([np.zeros(len(cen)), cen[:, 1], cen[:, 2]], axis=1)
+    is_end = np.isclose(np.abs(cen[:, 0]), L / 2) & np.isclose(np.abs(m.normals[:, 0]), 1.0)
+    out_dir = np.where(is_end[:, None], np.stack([np.sign(cen[:, 0]), np.zeros(len(cen)), np.zeros(len(cen))], 1),
+                       radial)
+    flip = np.sum(m.normals * out_dir, axis=1) < 0
+    Fo[flip] = Fo[flip][:, ::-1]
+    # a reversed triangle (c, b, a, c) keeps its repeated index last-and-first
+    m = Mesh(m.vertices, Fo)
+    c0 = np.asarray(center, dtype=np.float64)
+    return translated(m, c0) if np.any(c0 != 0.0) else m
 
 
 def hemisphere(radius=1.0, nlat=32, nlon=64):

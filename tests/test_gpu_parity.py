@@ -200,3 +200,19 @@ def test_batched_schedule_matches_oracle(mhx, oracle, gf):
         _coeff_parity(mhx, oracle, sphere64(), [0.45, 1.7], [0.5], gf, False, mode="batched")
     finally:
         mhx.context().set_batch(64)
+
+
+@pytest.mark.parametrize("zc", [0.0, 0.37])
+@pytest.mark.parametrize("gf", ["Wu", "ExactGuevelDelhommeau"])
+def test_clipped_horizontal_cylinder(mhx, oracle, zc, gf):
+    """The reference's horizontal cylinder (mesh_horizontal_cylinder(radius=1.5, length=2.5,
+    faces_max_radius=0.5).keep_immersed_part(), test/consistency_with_Capytaine.jl:211-216), and the
+    same body raised by 0.37 m so that the waterline cuts faces (quadrangles split into quadrangles
+    and triangles): coefficients against the oracle on the same panels, 1e-9."""
+    from mhx import meshes
+    m = meshes.mesh_horizontal_cylinder(length=2.5, radius=1.5, center=(0.0, 0.0, zc),
+                                        faces_max_radius=0.5).immersed_part()
+    if zc:
+        assert np.any(m.faces[:, 0] == m.faces[:, 3])   # cut faces became triangles
+    print(f"clipped horizontal cylinder zc={zc} ({m.nfaces} panels) {gf}", flush=True)
+    _coeff_parity(mhx, oracle, m, [0.5, 1.0, 2.0], [0.0, math.pi / 4], gf, True)

@@ -76,8 +76,7 @@ def test_capytaine_object_fields_taken_as_given(tmp_path):
 
 def test_capytaine_meshers():
     """mesh_sphere(...).immersed_part() is the generator's notebook sphere (same vertices, faces and
-    face properties); translated_x moves positions only; + joins (combine_meshes); a face across
-    z = 0 is refused."""
+    face properties); translated_x moves positions only; + joins (combine_meshes)."""
     a = meshes.mesh_sphere(1.0, resolution=(6, 6)).immersed_part()
     b = meshes.sphere(1.0, 6, 6)
     for f in ("vertices", "faces", "centers", "normals", "areas", "radii"):
@@ -91,5 +90,80 @@ def test_capytaine_meshers():
     j = a + t
     assert j.nfaces == 2 * a.nfaces and j.nvertices == 2 * a.nvertices
     np.testing.assert_array_equal(j.faces[a.nfaces:], a.faces + a.nvertices)
-    with pytest.raises(ValueError, match="crosses"):
-        meshes.mesh_sphere(1.0, center=(0.0, 0.0, 0.1), resolution=(6, 6)).immersed_part()
+
+
+def _polygon_below(ys, zs):
+    """Area of the planar polygon (ys, zs) below z = 0 and its chord length on z = 0, in the
+    plane, by the shoelace formula on the polygon clipped against z <= 0."""
+    pts = []
+    n = len(ys)
+    for e in range(n):
+        p, q = (ys[e], zs[e]), (ys[(e + 1) % n], zs[(e + 1) % n])
+        if p[1] <= 0:
+            pts.append(p)
+        if (p[1] < 0 < q[1]) or (q[1] < 0 < p[1]):
+            t = p[1] / (p[1] - q[1])
+            pts.append((p[0] + t * (q[0] - p[0]), 0.0))
+    y = np.array([a for a, _ in pts]); z = np.array([b for _, b in pts])
+    area = 0.5 * abs(np.dot(y, np.roll(z, -1)) - np.dot(z, np.roll(y, -1)))
+    on = y[np.abs(z) == 0.0]
+    return area, (on.max() - on.min()) if on.size >= 2 else 0.0
+
+
+def test_horizontal_cylinder_is_a_closed_outward_prism():
+    """mesh_horizontal_cylinder: the closed polygonal prism (regular ntheta-gon cross-section):
+    sum n a = 0, volume (divergence theorem, sum z n_z a) = L x polygon area to rounding, every
+    face radius within faces_max_radius, counts from the square-face rule."""
+    R, L, rmax = 1.5, 2.5, 0.5
+    m = meshes.mesh_horizontal_cylinder(length=L, radius=R, faces_max_radius=rmax)
+    s = np.sqrt(2.0) * rmax
+    nr, nth, nx = int(np.ceil(R / s)), int(np.ceil(2 * np.pi * R / s)), int(np.ceil(L / s))
+    assert m.nfaces == nth * nx + 2 * nr * nth
+    assert np.max(m.radii) <= rmax
+    np.testing.assert_allclose((m.normals * m.areas[:, None]).sum(0), 0.0, atol=1e-12)
+    poly = 0.5 * nth * R * R * np.sin(2 * np.pi / nth)
+    assert np.sum(m.centers[:, 2] * m.normals[:, 2] * m.areas) == pytest.approx(L * poly, rel=1e-13)
+    np.testing.assert_allclose(m.centers.mean(axis=0), 0.0, atol=1e-12)
+    c = meshes.mesh_horizontal_cylinder(length=L, radius=R, center=(1.0, -2.0, -3.0), resolution=(2, 8, 3))
+    assert c.nfaces == 8 * 3 + 2 * 2 * 8
+    np.testing.assert_allclose(c.centers.mean(axis=0), (1.0, -2.0, -3.0), atol=1e-12)
+
+
+@pytest.mark.parametrize("zc", [0.0, 0.37, -0.9, 0.8])
+def test_waterline_clipping(zc):
+    """immersed_part cuts the faces that cross z = 0: nothing is left above z = 0; the cut surface
+    plus the waterplane closes (sum n a = -A_wp e_z, with A_wp = L x chord of the cross-section
+    polygon at z = 0); the submerged volume is L x the polygon's area below z = 0 (both computed
+    in the cross-section plane by the shoelace formula); the intersection points are shared by
+    the faces on either side of a cut edge (no duplicate vertices)."""
+    R, L, nth = 1.0, 3.0, 36
+    m = meshes.mesh_horizontal_cylinder(length=L, radius=R, center=(0.0, 0.0, zc), resolution=(3, nth, 5))
+    im = m.immersed_part()
+    assert np.max(im.vertices[:, 2]) <= 0.0
+    th = np.arange(nth) * (2 * np.pi / nth)
+    area, chord = _polygon_below(R * np.cos(th), zc - R * np.sin(th))
+    vol = np.sum(im.centers[:, 2] * im.normals[:, 2] * im.areas)
+    assert vol == pytest.approx(L * area, rel=1e-12)
+    na = (im.normals * im.areas[:, None]).sum(0)
+    np.testing.assert_allclose(na[:2], 0.0, atol=1e-12)
+    assert na[2] == pytest.approx(-L * chord, rel=1e-12, abs=1e-12)
+    assert len(np.unique(np.round(im.vertices, 12), axis=0)) == im.nvertices
+    assert im.nfaces > 0 and np.all(im.areas > 0)
+
+
+def test_clipping_one_quad():
+    """A unit square in the vertical plane x = 0 standing on one vertex, cut at the height of its
+    centre: the part below is the lower half of the diamond (area 1/2) as one quadrangle-turned-
+    triangle; with the cut lower, a small triangle; with one vertex above, a pentagon as a quad plus
+    a triangle."""
+    h = np.sqrt(0.5)
+    V = np.array([[0, 0, -h], [0, h, 0.0], [0, 0, h], [0, -h, 0.0]], dtype=float)   # diamond, centre z = 0
+    F = np.array([[1, 2, 3, 4]])
+    for dz, area, nf in ((0.0, 0.5, 1), (0.5 * h, 0.125, 1), (-0.5 * h, 1.0 - 0.125, 2)):
+        im = meshes.Mesh(V + [0, 0, dz], F).immersed_part()
+        assert im.nfaces == nf
+        assert im.areas.sum() == pytest.approx(area, rel=1e-13)
+    # a vertical square with its top edge above the surface: a quadrangle
+    sq = meshes.Mesh(np.array([[0, 0, -1], [0, 1, -1], [0, 1, 0.5], [0, 0, 0.5]], dtype=float), F).immersed_part()
+    assert sq.nfaces == 1 and sq.faces[0, 0] != sq.faces[0, 3]
+    assert sq.areas[0] == pytest.approx(1.0)
