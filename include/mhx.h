@@ -47,6 +47,9 @@ extern "C" {
 #define MHX_MODE_PER_PROBLEM 1 /* reference semantics: assembly + LU per radiation/diffraction problem */
 #define MHX_MODE_BATCHED 2     /* frequencies in chunks: one launch sequence (assembly, batched LU, solves,
                                   forces) per chunk; chunk size from mhx_set_batch */
+#define MHX_MODE_GRAPH 3       /* MHX_MODE_BATCHED as a one-body mhx_sweep_coeffs captured into a HIP
+                                  graph on the first call and replayed on the next calls with the same
+                                  body, frequency count and headings (the frequencies may change) */
 
 typedef struct mhx_ctx mhx_ctx;
 typedef struct mhx_body mhx_body;

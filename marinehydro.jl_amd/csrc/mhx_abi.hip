@@ -1139,8 +1139,15 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
     if (nw < 0 || nb < 0 || (nw && !omegas) || (nb && !betas)) return fail(ctx, MHX_EINVAL, "coeffs: bad frequency/heading arrays");
     if (gf != MHX_GF_WU && gf != MHX_GF_DELHOMMEAU && gf != MHX_GF_SURROGATE)
         return fail(ctx, MHX_EINVAL, "unknown Green function");
-    if (mode != MHX_MODE_PER_OMEGA && mode != MHX_MODE_PER_PROBLEM && mode != MHX_MODE_BATCHED)
+    if (mode != MHX_MODE_PER_OMEGA && mode != MHX_MODE_PER_PROBLEM && mode != MHX_MODE_BATCHED &&
+        mode != MHX_MODE_GRAPH)
         return fail(ctx, MHX_EINVAL, "unknown mode");
+    if (mode == MHX_MODE_GRAPH) {
+        // the batched schedule as a one-body sweep captured into a HIP graph (replayed while the
+        // body, frequency count, headings, constants and precision stay the same)
+        mhx_sweep_body sb{b, nw, omegas, A_out, B_out, Fd_out, Fk_out};
+        return mhx_sweep_coeffs(ctx, &sb, 1, nb, betas, gf, direct, MHX_SWEEP_GRAPH);
+    }
     const int ndof = b->ndof, n = b->n;
     const int R = ndof + (int)nb;
     if (R <= 0) return fail(ctx, MHX_EDIM, "coeffs: no radiating dof and no heading");

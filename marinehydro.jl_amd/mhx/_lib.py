@@ -26,6 +26,7 @@ GF_SURROGATE = 2
 MODE_PER_OMEGA = 0
 MODE_PER_PROBLEM = 1
 MODE_BATCHED = 2
+MODE_GRAPH = 3  # MHX_MODE_GRAPH: the batched schedule captured into a HIP graph and replayed
 SURR_SILU, SURR_RELU = 0, 1  # MHX_SURR_SILU / MHX_SURR_RELU
 NOALLOC_SCRATCH_BYTES = 16 << 20  # MHX_NOALLOC_SCRATCH_BYTES
 LU_AUTO = 0

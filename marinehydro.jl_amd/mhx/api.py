@@ -535,7 +535,8 @@ def hydrodynamic_coefficients(floatingbody, parameters, formulation=None, schedu
     (src/problems_and_results.jl:464-495).  Layouts as the reference:
     added_mass / radiation_damping (n_inf, n_rad, n_omega, 1); forces (n_inf, n_omega, n_beta, 1).
     schedule="per_problem" reproduces the reference's one assembly + LU per problem;
-    schedule="batched" processes the frequencies in chunks with a batched LU."""
+    schedule="batched" processes the frequencies in chunks with a batched LU; schedule="graph" is
+    the batched schedule captured into a HIP graph on the first call and replayed after."""
     formulation = DirectBEM() if formulation is None else formulation
     ws = np.atleast_1d(np.asarray(parameters["wave_frequencies"], dtype=np.float64))
     inf = list(parameters.get("influenced_dofs", floatingbody.dof_names))
@@ -548,7 +549,7 @@ def hydrodynamic_coefficients(floatingbody, parameters, formulation=None, schedu
     if "forward_speeds" in parameters:
         return _coefficients_fs(floatingbody, parameters, formulation, ws, inf, rad, betas, names)
     mode = {"per_omega": _lib.MODE_PER_OMEGA, "per_problem": _lib.MODE_PER_PROBLEM,
-            "batched": _lib.MODE_BATCHED}[schedule]
+            "batched": _lib.MODE_BATCHED, "graph": _lib.MODE_GRAPH}[schedule]
     A, B, Fd, Fk = _body_coeffs(floatingbody, names, ws, betas, formulation.gf, formulation.direct, mode,
                                 SETTINGS.rho, SETTINGS.g)
     ii = [names.index(nm) for nm in inf]
