@@ -327,7 +327,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
-              "(frequency items would shard across GPUs, see DESIGN.md)", file=sys.stderr)
+              "(frequency items would shard across GPUs, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 
     import mhx

@@ -59,7 +59,7 @@ def _split_top(s):
 
 
 def test_julia_glue_ccalls_match_the_header():
-    """Every ccall in INTEGRATION.md's Julia glue names a declared function, lists as many argument
+    """Every ccall in the Julia glue shipped in the integration notes names a declared function, lists as many argument
     types as the declaration has parameters, and passes that many arguments."""
     text = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     arity = declared_arity()

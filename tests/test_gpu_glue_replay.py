@@ -1,6 +1,6 @@
 """The Julia glue's libmhx calls replayed through ctypes with Julia's memory layouts.
 
-The glue hands libmhx the reference's own arrays: a StaticArraysMesh's fields
+The Julia glue that binds libmhx into the reference's package hands libmhx the reference's own arrays: a StaticArraysMesh's fields
 (contiguous xyz triples, 1-based faces), dofs packed as reduce(vcat, vec.(...)) of each dof's
 column-major n x 3 matrix (with zero rows over a lid's panels), column-major output arrays, and
 MHX_MODE_BATCHED as the mode.  Here the same buffers are built in NumPy and passed to the same

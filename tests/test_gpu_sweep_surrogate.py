@@ -102,3 +102,20 @@ def test_config3_surrogate_sweep_graph_against_exact(mhx):
                 worst = max(worst, rel)
                 assert rel <= 1e-3, (q, body.mesh.nfaces, par["wave_frequencies"][w], name, rel)
     print(f"largest normwise block error against the exact path {worst:.3e}", flush=True)
+
+
+@pytest.mark.parametrize("gf", ["Surrogate", "Wu"])
+def test_graph_schedule_equals_batched(mhx, gf):
+    """schedule="graph" (MHX_MODE_GRAPH: the batched schedule captured on the first call, replayed
+    on the next with new frequencies) equals schedule="batched" to 1e-10."""
+    from mhx import meshes
+    body = mhx.FloatingBody(meshes.capped_cylinder(2.0, 3.0, 32, 12, 8), ["Surge", "Heave", "Pitch"])
+    form = mhx.DirectBEM(gf)
+    for ws in ([0.4, 0.9, 1.3, 2.1, 2.6], [0.5, 1.0, 1.4, 2.2, 2.7]):
+        par = dict(wave_frequencies=ws, radiating_dofs=body.dof_names, wave_directions=[0.0, math.pi / 2])
+        print(f"graph schedule {gf} omegas {ws[0]}..", flush=True)
+        g = mhx.hydrodynamic_coefficients(body, par, form, schedule="graph")
+        b = mhx.hydrodynamic_coefficients(body, par, form, schedule="batched")
+        for key in ("added_mass", "radiation_damping", "diffraction_force", "Froude_Krylov_force"):
+            rel, elem, ok = close(g[key], b[key], tol=1e-10)
+            assert ok, (gf, key, rel, elem)
