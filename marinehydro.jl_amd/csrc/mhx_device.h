@@ -228,8 +228,10 @@ __constant__ static const double WU_DA[10] = {2.948, -24.53, 249.69, -754.85, -1
 __constant__ static const double WU_DB[10] = {1.11, 2.894, -76.765, 1565.35, -11336.19, 44270.15, -97014.11, 118879.26, -76209.82, 19923.28};
 __constant__ static const double WU_DC[6] = {14.19, -148.24, 847.8, -2318.58, 3168.35, -1590.27};
 
-// _wu_wave_term_and_derivs (wu.jl:28-37): G, dG/dh, dG/dv of the dimensionless wave term.
-__device__ __noinline__ void wu_term(double hh, double vv, cplx& G, cplx& Gh, cplx& Gv) {
+// _wu_wave_term_and_derivs (wu.jl:28-37): G, dG/dh, dG/dv of the dimensionless wave term.  Inlined
+// where the caller keeps the outputs in registers (the surrogate's pair kernel); the assembly
+// kernels call the out-of-line wu_term below.
+__device__ __forceinline__ void wu_term_inline(double hh, double vv, cplx& G, cplx& Gh, cplx& Gv) {
     double dd = sqrt(hh * hh + vv * vv);
     double alpha = -vv / dd, beta = hh / dd, rho = dd / (1 + dd);
     double ev = exp(vv);
@@ -263,6 +265,22 @@ __device__ __noinline__ void wu_term(double hh, double vv, cplx& G, cplx& Gh, cp
     G = cmk(-L0 - Wr, -Wi);
     Gh = cmk(-Ls - Whr, -Whi);
     Gv = cmk(G.x + 2 / dd, G.y);
+}
+
+__device__ __noinline__ void wu_term(double hh, double vv, cplx& G, cplx& Gh, cplx& Gv) {
+    wu_term_inline(hh, vv, G, Gh, Gv);
+}
+
+// G and dG/dh, returned in registers (an out-of-line call whose outputs do not pass through the
+// caller's stack)
+struct WuGh {
+    cplx G, Gh;
+};
+__device__ __noinline__ WuGh wu_term_gh(double hh, double vv) {
+    WuGh o;
+    cplx Gv;
+    wu_term_inline(hh, vv, o.G, o.Gh, Gv);
+    return o;
 }
 
 // reduced coordinates (wu.jl:7-18; abstract_greens_function.jl:117-138)

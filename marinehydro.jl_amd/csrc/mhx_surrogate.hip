@@ -564,6 +564,25 @@ __device__ __forceinline__ void tri_tile(int64_t t, int& bi, int& bj) {
     bi = (int)(t - (int64_t)b * (b + 1) / 2);
 }
 
+// unit index within an item -> (bi, bj): the units come in fours, the 2 x 2 blocks of one 16 x 16
+// super-tile (BI <= BJ, in triangular order; bit 0 of t selects bi, bit 1 bj), so that the four units
+// whose 64-byte column segments share 128-byte lines of the column-major outputs (the FP32 copy of D,
+// the Rankine parts read) run in the same workgroup, on one XCD's L2.  False for the units below the
+// diagonal (bi > bj) or past the matrix: the wave skips them.
+__device__ __forceinline__ bool quad_tile(int64_t t, int nb, int& bi, int& bj) {
+    int BI, BJ;
+    tri_tile(t >> 2, BI, BJ);
+    bi = 2 * BI + (int)(t & 1);
+    bj = 2 * BJ + (int)((t >> 1) & 1);
+    return bi <= bj && bj < nb;
+}
+
+// units per item of order n: 4 per 16 x 16 super-tile of the upper triangle (a multiple of 4)
+__host__ __device__ __forceinline__ int64_t surr_item_units(int n) {
+    const int64_t nsb = ((n + 7) / 8 + 1) / 2;
+    return 2 * nsb * (nsb + 1);
+}
+
 #define SURR_WAVES 8
 
 // destination of one item's pair table, pointers offset to the item: the (G, dG/dr) table (S
@@ -604,33 +623,43 @@ __device__ __forceinline__ SurrDst surr_dst(const SweepItem& it, int direct, int
     return d;
 }
 
-// store the pair's wave term: the (G, dG/dr) table, or (fused) the S and D entries (i, j) and
-// (j, i) with their Rankine parts, and the FP32 copy of D for the mixed-precision solver
-__device__ __forceinline__ void surr_store_pair(const SurrDst& o, int n, int i, int j, double k, double hh, double vv,
-                                                const PanelGeom* __restrict__ geom, cplx G, cplx Gr) {
-    const int64_t ij = i + (int64_t)n * j, ji = j + (int64_t)n * i;
+typedef double nt_double2 __attribute__((ext_vector_type(2)));
+typedef float nt_float2 __attribute__((ext_vector_type(2)));
+
+// non-temporal (streaming) stores of the outputs: written once and read by the next kernel, they
+// would otherwise evict the panel geometry and the Rankine lines that the units re-read from L2
+__device__ __forceinline__ void store_stream(cplx* p, cplx v) {
+    __builtin_nontemporal_store(nt_double2{v.x, v.y}, reinterpret_cast<nt_double2*>(p));
+}
+__device__ __forceinline__ void store_stream(cplxf* p, double re, double im) {
+    __builtin_nontemporal_store(nt_float2{(float)re, (float)im}, reinterpret_cast<nt_float2*>(p));
+}
+
+// store one entry (row r, column c) of the item's output from the pair's wave term: the (G, dG/dr)
+// table, or (fused) the S and D entries with their Rankine parts, and the FP32 copy of D for the
+// mixed-precision solver
+__device__ __forceinline__ void surr_store_entry(const SurrDst& o, int n, int r, int c, double k, double hh,
+                                                 double vv, const PanelGeom* __restrict__ geom, cplx G, cplx Gr) {
+    const int64_t rc = r + (int64_t)n * c;
     if (!o.S) {
-        o.WG[ij] = G;
-        o.WR[ij] = Gr;
-        if (i != j) {
-            o.WG[ji] = G;
-            o.WR[ji] = Gr;
-        }
+        store_stream(o.WG + rc, G);
+        store_stream(o.WR + rc, Gr);
         return;
     }
-    const PanelGeom& gi = geom[i];
-    const PanelGeom& gj = geom[j];
+    const PanelGeom& gr = geom[r];
     cplx Sv, Dv;
-    combine_entry(k, hh, vv, gi.c, gi.nrm, gj, o.direct, o.include_identity && i == j, G, Gr, o.SR[ij], o.DR[ij], Sv, Dv);
-    o.S[ij] = Sv;
-    o.D[ij] = Dv;
-    if (o.Df) o.Df[ij] = make_float2((float)Dv.x, (float)Dv.y);
-    if (i != j) {
-        combine_entry(k, hh, vv, gj.c, gj.nrm, gi, o.direct, false, G, Gr, o.SR[ji], o.DR[ji], Sv, Dv);
-        o.S[ji] = Sv;
-        o.D[ji] = Dv;
-        if (o.Df) o.Df[ji] = make_float2((float)Dv.x, (float)Dv.y);
-    }
+    combine_entry(k, hh, vv, gr.c, gr.nrm, geom[c], o.direct, o.include_identity && r == c, G, Gr, o.SR[rc],
+                  o.DR[rc], Sv, Dv);
+    store_stream(o.S + rc, Sv);
+    store_stream(o.D + rc, Dv);
+    if (o.Df) store_stream(o.Df + rc, Dv.x, Dv.y);
+}
+
+// both entries (i, j) and (j, i) of a pair from one thread (the exact fallback)
+__device__ __forceinline__ void surr_store_pair(const SurrDst& o, int n, int i, int j, double k, double hh, double vv,
+                                                const PanelGeom* __restrict__ geom, cplx G, cplx Gr) {
+    surr_store_entry(o, n, i, j, k, hh, vv, geom, G, Gr);
+    if (i != j) surr_store_entry(o, n, j, i, k, hh, vv, geom, G, Gr);
 }
 
 // One work unit of a wave: the 8 x 8 block of panel pairs (i in 8 bi.., j in 8 bj.., bi <= bj;
@@ -685,18 +714,34 @@ __device__ __forceinline__ void surr_unit(const SurrLds& L, int lane, int n, int
         if (lane == leader) base = atomicAdd(fb_count, (unsigned long long)__popcll(mask));
         base = __shfl(base, leader);
         if (fb) {
-            const unsigned long long slot = base + __popcll(mask & ((1ull << lane) - 1ull));
+            // fallback lanes below this one: mbcnt counts the mask's bits under the lane
+            const unsigned below = __builtin_amdgcn_mbcnt_hi((unsigned)(mask >> 32),
+                                                             __builtin_amdgcn_mbcnt_lo((unsigned)mask, 0u));
+            const unsigned long long slot = base + below;
             if (slot < (unsigned long long)fb_cap) fb_list[slot] = tag | ((unsigned long long)i << 20) | (unsigned)j;
         }
     }
-    if (!net) return;
-    cplx W, Wh, Wv;
-    wu_term(hh_me > 0 ? hh_me : 1e-300, vv_me, W, Wh, Wv);
-    const double is = 1.0 / SURR_SCALE;
-    const double ii = is * exp(-fmin(-vv_me, SURR_IM_VC));
-    const cplx G = cmk(W.x + is * o[0], W.y + ii * o[1]);
-    const cplx Gr = cmk(Wh.x + is * o[2], Wh.y + ii * o[3]);
-    surr_store_pair(dst, n, i, j, k, hh_me, vv_me, geom, G, Gr);
+    if (!__ballot(net)) return;
+    cplx G = cmk(0.0, 0.0), Gr = cmk(0.0, 0.0);
+    if (net) {
+        const WuGh w = wu_term_gh(hh_me > 0 ? hh_me : 1e-300, vv_me);
+        const double is = 1.0 / SURR_SCALE;
+        // the imaginary outputs' scale in FP32 (the network's outputs carry BF16 rounding; an FP64
+        // exp here costs constants that the compiler keeps in registers across the unit loop)
+        const double ii = is * (double)expf(-fminf((float)-vv_me, (float)SURR_IM_VC));
+        G = cmk(w.G.x + is * o[0], w.G.y + ii * o[1]);
+        Gr = cmk(w.Gh.x + is * o[2], w.Gh.y + ii * o[3]);
+        // entry (i, j): lanes 8m .. 8m + 7 write rows 8 bi .. 8 bi + 7 of column 8 bj + m
+        surr_store_entry(dst, n, i, j, k, hh_me, vv_me, geom, G, Gr);
+    }
+    // entry (j, i) from the transposed lane t, whose pair is (8 bi + lane / 8, 8 bj + lane % 8): lanes
+    // 8m .. 8m + 7 again write eight consecutive rows of one column (hh and vv are symmetric in the pair)
+    const int t = ((lane & 7) << 3) | (lane >> 3);
+    const double hh_t = __shfl(hh_me, t), vv_t = __shfl(vv_me, t);
+    const cplx G_t = cmk(__shfl(G.x, t), __shfl(G.y, t)), Gr_t = cmk(__shfl(Gr.x, t), __shfl(Gr.y, t));
+    const bool net_t = __shfl((int)net, t) != 0;
+    const int mi = 8 * bi + (lane >> 3), mj = 8 * bj + (lane & 7);
+    if (net_t && mi != mj) surr_store_entry(dst, n, mj, mi, k, hh_t, vv_t, geom, G_t, Gr_t);
 }
 
 // Persistent pair-table kernel over all items of the batch: each wave takes units in a strided
@@ -710,13 +755,15 @@ __global__ void __launch_bounds__(64 * SURR_WAVES) surr_table_kernel(
     surr_load_lds(L, Wb, bias, Bf);
     const int lane = threadIdx.x & 63;
     const int nb = (n + 7) / 8;
-    const int64_t per_item = (int64_t)nb * (nb + 1) / 2;
+    const int64_t per_item = surr_item_units(n);
     const int64_t units = per_item * batch;
     const int64_t stride = (int64_t)gridDim.x * SURR_WAVES;
-    for (int64_t unit = (int64_t)blockIdx.x * SURR_WAVES + (threadIdx.x >> 6); unit < units; unit += stride) {
+    // the wave index through readfirstlane: the unit, its item and tile live in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (int64_t unit = (int64_t)blockIdx.x * SURR_WAVES + wave; unit < units; unit += stride) {
         const int item = (int)(unit / per_item);
         int bi, bj;
-        tri_tile(unit - (int64_t)item * per_item, bi, bj);
+        if (!quad_tile(unit - (int64_t)item * per_item, nb, bi, bj)) continue;
         surr_unit<ACT>(L, lane, n, bi, bj, ks[item], geom, surr_dst(out, item), (unsigned long long)item << 40, fb_list,
                   fb_count, fb_cap);
     }
@@ -732,7 +779,9 @@ __global__ void __launch_bounds__(64 * SURR_WAVES) surr_queue_kernel(
     surr_load_lds(L, Wb, bias, Bf);
     const int lane = threadIdx.x & 63;
     const int64_t stride = (int64_t)gridDim.x * SURR_WAVES;
-    for (int64_t unit = (int64_t)blockIdx.x * SURR_WAVES + (threadIdx.x >> 6); unit < units; unit += stride) {
+    // the wave index through readfirstlane: the unit, its item and tile live in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (int64_t unit = (int64_t)blockIdx.x * SURR_WAVES + wave; unit < units; unit += stride) {
         int lo = 0, hi = nitems - 1;
         while (lo < hi) {
             const int mid = (lo + hi + 1) >> 1;
@@ -741,7 +790,7 @@ __global__ void __launch_bounds__(64 * SURR_WAVES) surr_queue_kernel(
         }
         const SweepItem& it = items[lo];
         int bi, bj;
-        tri_tile(unit - it.sunit0, bi, bj);
+        if (!quad_tile(unit - it.sunit0, (it.n + 7) / 8, bi, bj)) continue;
         surr_unit<ACT>(L, lane, it.n, bi, bj, *it.k, it.geom, surr_dst(it, direct, include_identity),
                   (unsigned long long)lo << 40, fb_list, fb_count, fb_cap);
     }
@@ -801,8 +850,7 @@ void launch_surr_table(hipStream_t st, int n, const PanelGeom* geom, const doubl
                        unsigned long long* fb_list, unsigned long long* fb_count, int64_t fb_cap,
                        unsigned long long* status, int act) {
     launch_zero(st, fb_count, sizeof(unsigned long long));
-    const int nb = (n + 7) / 8;
-    const int64_t units = (int64_t)nb * (nb + 1) / 2 * batch;
+    const int64_t units = surr_item_units(n) * batch;
     const int64_t blocks = std::min<int64_t>(device_cus(), (units + SURR_WAVES - 1) / SURR_WAVES);
     if (act == SURR_ACT_RELU)
         surr_table_kernel<SURR_ACT_RELU><<<(int)blocks, 64 * SURR_WAVES, 0, st>>>(n, batch, geom, ks, Wb, bias, Bf, out,
@@ -813,10 +861,7 @@ void launch_surr_table(hipStream_t st, int n, const PanelGeom* geom, const doubl
     surr_fallback_kernel<<<2048, 256, 0, st>>>(n, geom, ks, fb_list, fb_count, fb_cap, out, status);
 }
 
-int64_t sweep_surr_units(int n) {
-    const int64_t nb = (n + 7) / 8;
-    return nb * (nb + 1) / 2;
-}
+int64_t sweep_surr_units(int n) { return surr_item_units(n); }
 
 void launch_queue_surr(hipStream_t st, const SweepItem* items, int nitems, int64_t total_units, int direct,
                        int include_identity, const unsigned short* Wb, const float* bias, const float* Bf,
