@@ -318,6 +318,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain GPU comparison runs")
     ap.add_argument("--lu-mode", default="auto", choices=["auto", "fp64", "mixed", "mixed_bf16"],
                     help="dense solver precision (auto: mixed FP32 factor for the surrogate, FP64 otherwise)")
+    ap.add_argument("--lu-streams", type=int, default=None,
+                    help="streams a batched FP32 factorization is split over (1-4; default: the library's)")
     ap.add_argument("--freqs", type=int, default=64,
                     help="frequencies per step, spread over the 64 of the workload (fewer only for profiling)")
     ap.add_argument("--activation", default="silu", choices=["silu", "relu"],
@@ -343,6 +345,8 @@ def main():
     names = ["Surge", "Sway", "Heave", "Roll", "Pitch", "Yaw"]
     body = mhx.FloatingBody(mesh, names)
     ctx.set_lu_mode(modes[args.lu_mode])
+    if args.lu_streams:
+        ctx.set_lu_streams(args.lu_streams)
     if args.workload == "forward_speed":
         return bench_forward_speed(args, mhx, _lib, ctx, body, names, mesh, omegas)
     form = mhx.DirectBEM(args.gf)

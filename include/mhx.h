@@ -177,6 +177,12 @@ int mhx_set_batch(mhx_ctx* ctx, int64_t max_items, double max_bytes);
 #define MHX_LU_MIXED_BF16 3
 int mhx_set_lu_mode(mhx_ctx* ctx, int mode);
 int mhx_solver_stats(mhx_ctx* ctx, int64_t counts[3], double* max_residual);
+/* Streams a batched FP32 factorization (MHX_LU_MIXED, MHX_LU_MIXED_BF16) is split over, 1 .. 4
+ * (default 2): a batch of 8 or more systems of order >= 512 is factored in that many parts of at
+ * least 4 systems, on as many streams, so that one part's panel phases overlap the others'
+ * trailing updates.  Results do not depend on it.  No counterpart in the reference (its LU is one
+ * LAPACK call per system); a tuning control of this build's batched solver. */
+int mhx_set_lu_streams(mhx_ctx* ctx, int streams);
 
 /* Pivoting of the dense LU (LAPACK's partial pivoting either way, src/matrix_assembly.jl:411-416):
  *   MHX_PIVOT_VERIFIED (default)  factor with the diagonal pivots while checking LAPACK's rule on

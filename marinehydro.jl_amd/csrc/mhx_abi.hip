@@ -20,6 +20,8 @@
 
 struct SweepState;  // multi-body sweep: item queue, device outputs, captured graph
 
+constexpr int kMaxLuStreams = 4;
+
 struct PhaseAcc {
     double ms = 0.0;
     int64_t count = 0;
@@ -28,10 +30,12 @@ struct PhaseAcc {
 struct mhx_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    // second stream (default priority) for the other half of a batched factorization, with the
-    // events that fork it from and join it back to `stream`
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    // side streams (default priority) for the other parts of a batched factorization, with the
+    // events that fork them from and join them back to `stream`; lu_streams: streams a batched
+    // FP32 factorization is split over (1 .. kMaxLuStreams)
+    hipStream_t side[kMaxLuStreams - 1] = {};
+    hipEvent_t fork = nullptr, join[kMaxLuStreams - 1] = {};
+    int lu_streams = 2;
     double rho = 1000.0, g = 9.81;
     int timing = 0;
     std::string err;
@@ -439,28 +443,29 @@ constexpr int kRefineMax = 4;       // FP32 trailing updates
 constexpr int kRefineMaxSplit = 8;  // split-bf16 trailing updates (factors to ~2^-16)
 
 // FP32 LU of the batch's b->Df in place.  A batch of 8 or more items of order >= 512 is factored
-// as two halves on two streams: the panel phases (launches of one workgroup per item,
-// latency-bound) of one half can then run beside the trailing updates of the other, on compute
-// units the panels leave idle.  The halves share nothing but the device.
+// in up to ctx->lu_streams parts (at least 4 items each) on as many default-priority streams,
+// forked from and joined back to the library's stream by events: the panel phases of one part
+// (launches of one workgroup per item, latency-bound) then run beside the trailing updates of the
+// others, on compute units the panels leave idle.  The parts share nothing but the device.
 int getrf_f32_two_streams(mhx_ctx* ctx, mhx_body* b, int batch, mhx::LaunchTimer* lt, unsigned short* split,
                           int pivot) {
     const int n = b->n;
-    const int h = (batch >= 8 && n >= 512) ? batch / 2 : 0;  // items on the side stream
-    const int m = batch - h;
+    int parts = 1;
+    if (batch >= 8 && n >= 512) parts = std::max(1, std::min(ctx->lu_streams, batch / 4));
     int rc = 0;
-    if (h) {
-        HCHK(hipEventRecord(ctx->fork, ctx->stream));
-        HCHK(hipStreamWaitEvent(ctx->side, ctx->fork, 0));
-        rc = mhx::getrf_batched(ctx->side, n, b->Df + (size_t)m * b->nn(), n, b->nn(), h, b->ipiv + (size_t)m * n, n,
-                                b->wperm + (size_t)m * mhx::kPermWork, b->info + m, lt,
-                                split ? split + (size_t)m * mhx::split_work(n) : nullptr, pivot,
-                                reinterpret_cast<cplxf*>(b->work + (size_t)m * b->sW()), 2 * b->sW());
-        HCHK(hipEventRecord(ctx->join, ctx->side));
+    if (parts > 1) HCHK(hipEventRecord(ctx->fork, ctx->stream));
+    // part p: items [first, first + cnt); part 0 on the library's stream, the others on side[p - 1]
+    for (int p = parts - 1; p >= 0 && !rc; p--) {
+        const int first = (int)((int64_t)batch * p / parts), cnt = (int)((int64_t)batch * (p + 1) / parts) - first;
+        hipStream_t st = p == 0 ? ctx->stream : ctx->side[p - 1];
+        if (p > 0) HCHK(hipStreamWaitEvent(st, ctx->fork, 0));
+        rc = mhx::getrf_batched(st, n, b->Df + (size_t)first * b->nn(), n, b->nn(), cnt, b->ipiv + (size_t)first * n, n,
+                                b->wperm + (size_t)first * mhx::kPermWork, b->info + first, lt,
+                                split ? split + (size_t)first * mhx::split_work(n) : nullptr, pivot,
+                                reinterpret_cast<cplxf*>(b->work + (size_t)first * b->sW()), 2 * b->sW());
+        if (p > 0) HCHK(hipEventRecord(ctx->join[p - 1], st));
     }
-    if (!rc)
-        rc = mhx::getrf_batched(ctx->stream, n, b->Df, n, b->nn(), m, b->ipiv, n, b->wperm, b->info, lt, split, pivot,
-                                reinterpret_cast<cplxf*>(b->work), 2 * b->sW());
-    if (h) HCHK(hipStreamWaitEvent(ctx->stream, ctx->join, 0));
+    for (int p = 1; p < parts; p++) HCHK(hipStreamWaitEvent(ctx->stream, ctx->join[p - 1], 0));
     return rc;
 }
 
@@ -858,14 +863,19 @@ int mhx_create(mhx_ctx** out, const mhx_options* opt) {
         delete c;
         return MHX_EDEVICE;
     }
-    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->join, hipEventDisableTiming) != hipSuccess) {
-        if (c->side) hipStreamDestroy(c->side);
+    bool ok = hipSetDevice(c->device) == hipSuccess &&
+              hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) == hipSuccess;
+    for (int i = 0; ok && i < kMaxLuStreams - 1; i++)
+        ok = hipStreamCreateWithFlags(&c->side[i], hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&c->join[i], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        for (int i = 0; i < kMaxLuStreams - 1; i++) {
+            if (c->side[i]) hipStreamDestroy(c->side[i]);
+            if (c->join[i]) hipEventDestroy(c->join[i]);
+        }
         if (c->stream) hipStreamDestroy(c->stream);
         if (c->fork) hipEventDestroy(c->fork);
-        if (c->join) hipEventDestroy(c->join);
         delete c;
         return MHX_EDEVICE;
     }
@@ -882,11 +892,14 @@ void mhx_destroy(mhx_ctx* c) {
     if (c->stream) hipStreamSynchronize(c->stream);
     drop_sweep(c);
     dfree(c->surr_params); dfree(c->surr_B); dfree(c->surr_bias); dfree(c->surr_W);
-    if (c->side) hipStreamSynchronize(c->side);
+    for (int i = 0; i < kMaxLuStreams - 1; i++)
+        if (c->side[i]) hipStreamSynchronize(c->side[i]);
     if (c->stream) hipStreamDestroy(c->stream);
-    if (c->side) hipStreamDestroy(c->side);
+    for (int i = 0; i < kMaxLuStreams - 1; i++) {
+        if (c->side[i]) hipStreamDestroy(c->side[i]);
+        if (c->join[i]) hipEventDestroy(c->join[i]);
+    }
     if (c->fork) hipEventDestroy(c->fork);
-    if (c->join) hipEventDestroy(c->join);
     delete c;
 }
 
@@ -2287,6 +2300,13 @@ int mhx_set_lu_mode(mhx_ctx* ctx, int mode) {
     if (mode != MHX_LU_AUTO && mode != MHX_LU_FP64 && mode != MHX_LU_MIXED && mode != MHX_LU_MIXED_BF16)
         return fail(ctx, MHX_EINVAL, "lu mode must be MHX_LU_AUTO, MHX_LU_FP64, MHX_LU_MIXED or MHX_LU_MIXED_BF16");
     ctx->lu_mode = mode;
+    return MHX_OK;
+}
+
+int mhx_set_lu_streams(mhx_ctx* ctx, int streams) {
+    if (!ctx) return MHX_EINVAL;
+    if (streams < 1 || streams > kMaxLuStreams) return fail(ctx, MHX_EINVAL, "lu streams must be 1 .. 4");
+    ctx->lu_streams = streams;
     return MHX_OK;
 }
 

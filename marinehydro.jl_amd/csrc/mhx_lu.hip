@@ -880,13 +880,18 @@ void zgemm(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cp
 // register of the 32x32 accumulator holds 32 consecutive rows of one column (coalesced
 // read-modify-write of the column-major C).
 //
-// Split arrays of one matrix (bf16, padded to SPLIT_TILE rows/cols, k-contiguous, ks = 64 S k
-// slots for the S <= SPLIT_SUPER panels of one update):
-//   [0 .. 4)  L21 rows:    re hi, re lo, im hi, im lo   each [mpad][ks]
-//   [4 .. 8)  U12 columns: re hi, re lo, im hi, im lo   each [mpad][ks]
+// Split arrays of one matrix (bf16, padded to SPLIT_TILE rows/cols, ks = 64 S k slots for the
+// S <= SPLIT_SUPER panels of one update), chunk-major: k in chunks of 16, and within a chunk the
+// 16 slots of a row contiguous (32 bytes), rows after each other.  Array q, row r, slot k is bf16
+// number ((q * (ks / 16) + k / 16) * mpad + r) * 16 + k % 16.  A wave's load of one chunk of 32
+// rows is then 1 KB contiguous (with rows k-contiguous over all of ks, the same load touched 32
+// cache lines and used a quarter of each: 22 % slower updates).
+//   q = 0 .. 3  L21 rows:    re hi, re lo, im hi, im lo
+//   q = 4 .. 7  U12 columns: re hi, re lo, im hi, im lo
 // ---------------------------------------------------------------------------
 constexpr int SPLIT_TILE = 128;
 constexpr int SPLIT_SUPER = 4;  // panels per trailing update on the split path (k = 256)
+constexpr int SPLIT_KC = 16;    // k chunk of the layout and of the update's LDS stages
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
 
@@ -900,48 +905,47 @@ __device__ __forceinline__ unsigned short bf16_rne(float f) {
 }
 __device__ __forceinline__ float bf16_val(unsigned short h) { return __uint_as_float((unsigned int)h << 16); }
 
-// thread (r, kb): k slots 8 kb .. 8 kb + 7 of row r of L21 (rows [KE, n), columns [K, KE)) and of
-// column r of U12 (rows [K, KE), columns [KE, n)); rows past n - KE and k past KE - K are zero;
-// ks = 64 S k slots per row
+// thread (kc, r), r fastest: k slots 16 kc .. 16 kc + 15 of row r of L21 (rows [KE, n), columns
+// [K, KE); consecutive threads read consecutive rows of a column) and of column r of U12 (rows
+// [K, KE), columns [KE, n)); rows past n - KE and k past KE - K are zero
 __global__ void split_panels_kernel(MatBatch<cplxf> mb, int n, int K, int KE, int mpad, int ks,
                                     unsigned short* __restrict__ Sb, int64_t sS) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int kbs = ks / 8;
-    const int r = t / kbs, kb = t - r * kbs;
-    if (r >= mpad) return;
+    const int nkc = ks / SPLIT_KC;
+    const int kc = t / mpad, r = t - kc * mpad;
+    if (kc >= nkc) return;
     const cplxf* A = mb.A + mb.stride * blockIdx.z;
     unsigned short* S = Sb + sS * blockIdx.z;
     const int m = n - KE, kk = KE - K, lda = mb.lda;
-    unsigned short h[2][4][8];  // [panel][array][k]
 #pragma unroll
-    for (int e = 0; e < 8; e++) {
-        const int k = 8 * kb + e;
-        cplxf a = make_float2(0.f, 0.f), b = make_float2(0.f, 0.f);
-        if (r < m && k < kk) {
-            a = A[(KE + r) + (int64_t)lda * (K + k)];
-            b = A[(K + k) + (int64_t)lda * (KE + r)];
-        }
-        const float v[2][2] = {{a.x, a.y}, {b.x, b.y}};
+    for (int p = 0; p < 2; p++) {
+        unsigned int w[4][SPLIT_KC / 2];  // [array][slot pair]
 #pragma unroll
-        for (int p = 0; p < 2; p++)
+        for (int e = 0; e < SPLIT_KC; e++) {
+            const int k = SPLIT_KC * kc + e;
+            cplxf a = make_float2(0.f, 0.f);
+            if (r < m && k < kk) a = p == 0 ? A[(KE + r) + (int64_t)lda * (K + k)] : A[(K + k) + (int64_t)lda * (KE + r)];
+            const float v[2] = {a.x, a.y};
 #pragma unroll
             for (int c = 0; c < 2; c++) {
-                const unsigned short hi = bf16_rne(v[p][c]);
-                h[p][2 * c][e] = hi;
-                h[p][2 * c + 1][e] = bf16_rne(v[p][c] - bf16_val(hi));
+                const unsigned short hi = bf16_rne(v[c]);
+                const unsigned short lo = bf16_rne(v[c] - bf16_val(hi));
+                if (e & 1) {
+                    w[2 * c][e / 2] |= (unsigned)hi << 16;
+                    w[2 * c + 1][e / 2] |= (unsigned)lo << 16;
+                } else {
+                    w[2 * c][e / 2] = hi;
+                    w[2 * c + 1][e / 2] = lo;
+                }
             }
-    }
-#pragma unroll
-    for (int p = 0; p < 2; p++)
+        }
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            uint4 w;
-            w.x = h[p][q][0] | ((unsigned)h[p][q][1] << 16);
-            w.y = h[p][q][2] | ((unsigned)h[p][q][3] << 16);
-            w.z = h[p][q][4] | ((unsigned)h[p][q][5] << 16);
-            w.w = h[p][q][6] | ((unsigned)h[p][q][7] << 16);
-            *reinterpret_cast<uint4*>(S + ((int64_t)(4 * p + q) * mpad + r) * ks + 8 * kb) = w;
+            uint4* dst = reinterpret_cast<uint4*>(S + (((int64_t)(4 * p + q) * nkc + kc) * mpad + r) * SPLIT_KC);
+            dst[0] = make_uint4(w[q][0], w[q][1], w[q][2], w[q][3]);
+            dst[1] = make_uint4(w[q][4], w[q][5], w[q][6], w[q][7]);
         }
+    }
 }
 
 __device__ __forceinline__ bf16x8_t bf16x8_neg(bf16x8_t v) {
@@ -956,10 +960,10 @@ __device__ __forceinline__ bf16x8_t bf16x8_neg(bf16x8_t v) {
 // C[0:m, 0:m] -= L21 U12 from the split arrays.  Workgroup tile 128 x 128 of C, four waves of
 // 64 x 64 (2 x 2 accumulator tiles of 32 x 32, real and imaginary); k in chunks of 16 staged
 // through LDS (rows padded to 48 B: the eight lanes of a 16-byte read phase hit distinct banks),
-// the next chunk prefetched into registers during the current chunk's MFMAs.  Two workgroups per
-// CU (accumulators in AGPRs, operands loaded per column tile, C read in the epilogue): one
+// the next chunk prefetched into registers during the current chunk's MFMAs: each thread loads
+// one 16-byte unit per array and panel, a chunk of 128 rows of one array being 4 KB contiguous.
+// Two workgroups per CU (operands loaded per column tile, C read in the epilogue): one
 // workgroup's C read-modify-write overlaps the other's MFMAs.
-constexpr int SPLIT_KC = 16;
 constexpr int SPLIT_LDS_ROW = SPLIT_KC / 8 + 1;  // bf16x8 units per row (2 used + 1 pad)
 
 __global__ void __launch_bounds__(256, 2) zgemm_split_kernel(int m, int mpad, int ks, const unsigned short* __restrict__ Sb,
@@ -987,28 +991,26 @@ __global__ void __launch_bounds__(256, 2) zgemm_split_kernel(int m, int mpad, in
         for (int b = 0; b < 2; b++)
 #pragma unroll
             for (int v = 0; v < 16; v++) acc_re[a][b][v] = acc_im[a][b][v] = 0.f;
+    const int nkc = ks / SPLIT_KC;
     const bf16x8_t* Sa = reinterpret_cast<const bf16x8_t*>(S);                              // arrays 0..3
     const bf16x8_t* Sbv = reinterpret_cast<const bf16x8_t*>(S + (int64_t)4 * mpad * ks);  // arrays 4..7
-    const int UPR = ks / 8;  // bf16x8 units per row in global
-    // one chunk: 4 arrays x 128 rows x 2 units for each panel = 1024 units each, 4 + 4 per thread
+    // one chunk: 4 arrays x 128 rows x 2 units for each panel; thread tid takes unit tid of each
+    // array's 256 (row tid / 2, half tid % 2)
     bf16x8_t pa[4], pb[4];
     auto load_chunk = [&](int k0) {
+        const int kc = k0 / SPLIT_KC;
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int idx = tid + 256 * e;
-            const int q = idx >> 8, r = (idx >> 1) & 127, u = idx & 1;
-            pa[e] = Sa[((int64_t)q * mpad + row0 + r) * UPR + k0 / 8 + u];
-            pb[e] = Sbv[((int64_t)q * mpad + col0 + r) * UPR + k0 / 8 + u];
+        for (int q = 0; q < 4; q++) {
+            pa[q] = Sa[(((int64_t)q * nkc + kc) * mpad + row0) * 2 + tid];
+            pb[q] = Sbv[(((int64_t)q * nkc + kc) * mpad + col0) * 2 + tid];
         }
     };
     load_chunk(0);
     for (int k0 = 0; k0 < ks; k0 += SPLIT_KC) {
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-            const int idx = tid + 256 * e;
-            const int q = idx >> 8, r = (idx >> 1) & 127, u = idx & 1;
-            La[q][r][u] = pa[e];
-            Lb[q][r][u] = pb[e];
+        for (int q = 0; q < 4; q++) {
+            La[q][tid >> 1][tid & 1] = pa[q];
+            Lb[q][tid >> 1][tid & 1] = pb[q];
         }
         __syncthreads();
         if (k0 + SPLIT_KC < ks) load_chunk(k0 + SPLIT_KC);
@@ -1168,7 +1170,7 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
         const int64_t sS = split_work(n);
         if constexpr (kF32)
             if (use_split)
-                split_panels_kernel<<<dim3((mpad * (ks / 8) + 255) / 256, 1, batch), 256, 0, st>>>(mb, n, K, KE, mpad,
+                split_panels_kernel<<<dim3((mpad * (ks / SPLIT_KC) + 255) / 256, 1, batch), 256, 0, st>>>(mb, n, K, KE, mpad,
                                                                                                   ks, split, sS);
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timer) {

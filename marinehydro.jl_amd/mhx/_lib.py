@@ -40,7 +40,7 @@ EXPORTED = ["mhx_create", "mhx_destroy", "mhx_last_error", "mhx_set_constants", 
             "mhx_lu_solve", "mhx_lu_solve_batched", "mhx_lu_solve_noalloc", "mhx_hydro_coeffs", "mhx_body_create", "mhx_body_destroy", "mhx_body_coeffs",
             "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work", "mhx_set_batch", "mhx_surrogate_train",
             "mhx_surrogate_save", "mhx_surrogate_load", "mhx_surrogate_params", "mhx_surrogate_eval",
-            "mhx_surrogate_loss_grad", "mhx_surrogate_set_activation", "mhx_surrogate_get_activation", "mhx_hydro_coeffs_fs_jvp", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_set_pivoting",
+            "mhx_surrogate_loss_grad", "mhx_surrogate_set_activation", "mhx_surrogate_get_activation", "mhx_hydro_coeffs_fs_jvp", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_set_pivoting", "mhx_set_lu_streams",
             "mhx_pivot_stats", "mhx_hydro_coeffs_jvp",
             "mhx_hydro_coeffs_vjp", "mhx_sweep_coeffs", "mhx_body_create_lid", "mhx_body_coeffs_fs"]
 
@@ -131,6 +131,7 @@ def lib():
         L.mhx_exact_work.argtypes = [vp, _ip, _dp]
         L.mhx_set_batch.argtypes = [vp, C.c_int64, C.c_double]
         L.mhx_set_lu_mode.argtypes = [vp, C.c_int]
+        L.mhx_set_lu_streams.argtypes = [vp, C.c_int]
         L.mhx_solver_stats.argtypes = [vp, _ip, _dp]
         L.mhx_set_pivoting.argtypes = [vp, C.c_int]
         L.mhx_pivot_stats.argtypes = [vp, _ip]
@@ -209,6 +210,10 @@ class Context:
         """LU_AUTO (mixed precision for the surrogate wave term), LU_FP64, LU_MIXED or LU_MIXED_BF16
         (the FP32 factor's trailing updates on split-bf16 MFMA)."""
         self.check(lib().mhx_set_lu_mode(self.h, int(mode)))
+
+    def set_lu_streams(self, streams):
+        """Streams (1 .. 4) a batched FP32 factorization is split over; results do not depend on it."""
+        self.check(lib().mhx_set_lu_streams(self.h, int(streams)))
 
     def solver_stats(self):
         """(systems solved in mixed precision, refinement steps, FP64 fallbacks, max residual)."""
