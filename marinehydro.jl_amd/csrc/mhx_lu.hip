@@ -1092,6 +1092,183 @@ __global__ void __launch_bounds__(256, 2) zgemm_split_kernel(int m, int mpad, in
 }
 
 // ---------------------------------------------------------------------------
+// Split-bf16 products of FP32 complex operands read straight from the matrix (the split path's
+// panel-side products: the left-looking updates inside a super-panel and the L21 / U12 products
+// with the diagonal-block inverses).  Each operand element is split into bf16 hi + lo while it is
+// staged into LDS, and the products run as in zgemm_split_kernel (hi*hi + hi*lo + lo*hi on
+// v_mfma_f32_32x32x16_bf16, FP32 accumulation): 1,024 flops per cycle and SIMD against 64 for the
+// FP32 MFMA of zgemm_mfma_kernel<cplxf>, whose three-product form left these small-k products
+// compute-bound.  Their precision is the trailing updates' (products to ~2^-16).
+// Tile 64 x 64 per workgroup of 256 threads: wave w owns the 32 x 32 block of rows 32 (w & 1) and
+// columns 32 (w >> 1); k in chunks of 16, the next chunk's FP32 values loaded into registers
+// during the current chunk's MFMAs.
+// ---------------------------------------------------------------------------
+struct FsplitLds {
+    // [array re hi, re lo, im hi, im lo][row or column][2 units of 8 k + pad] as 8-byte halves
+    __align__(16) uint2 a[4][64][6];
+    __align__(16) uint2 b[4][64][6];
+};
+
+__device__ __forceinline__ uint2 split_pack4(float x0, float x1, float x2, float x3, bool lo) {
+    const float v[4] = {x0, x1, x2, x3};
+    unsigned short h[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const unsigned short hi = bf16_rne(v[e]);
+        h[e] = lo ? bf16_rne(v[e] - bf16_val(hi)) : hi;
+    }
+    return make_uint2(h[0] | ((unsigned)h[1] << 16), h[2] | ((unsigned)h[3] << 16));
+}
+
+// acc = A[0:rows, 0:kk] B[0:kk, 0:cols] for the 64 x 64 tile whose first row of A / column of B
+// the pointers address (rows, cols <= 64); acc register v of lane (l32, h) of wave w is element
+// (32 (w & 1) + l32, 32 (w >> 1) + 8 (v / 4) + 4 h + v % 4)
+__device__ __forceinline__ void fsplit_tile64(const cplxf* __restrict__ A, int lda, const cplxf* __restrict__ B,
+                                              int ldb, int rows, int cols, int kk, FsplitLds& L, f32x16_t& acc_re,
+                                              f32x16_t& acc_im) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, l32 = lane & 31;
+    const int wr = 32 * (wave & 1), wc = 32 * (wave >> 1);
+#pragma unroll
+    for (int v = 0; v < 16; v++) acc_re[v] = acc_im[v] = 0.f;
+    // staging: A element (row ar, k 4 akq + e), B element (k 4 bkq + e, column bc), e = 0..3
+    const int ar = tid & 63, akq = tid >> 6, bc = tid >> 2, bkq = tid & 3;
+    cplxf ra[4], rb[4];
+    auto load_chunk = [&](int k0) {
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            const int ka = k0 + 4 * akq + e, kb = k0 + 4 * bkq + e;
+            ra[e] = (ar < rows && ka < kk) ? A[ar + (int64_t)lda * ka] : make_float2(0.f, 0.f);
+            rb[e] = (bc < cols && kb < kk) ? B[kb + (int64_t)ldb * bc] : make_float2(0.f, 0.f);
+        }
+    };
+    auto store_chunk = [&]() {
+        const int ua = 2 * (akq >> 1) + (akq & 1), ub = 2 * (bkq >> 1) + (bkq & 1);  // 8-byte slot in the row
+#pragma unroll
+        for (int c = 0; c < 2; c++)
+#pragma unroll
+            for (int lo = 0; lo < 2; lo++) {
+                L.a[2 * c + lo][ar][ua] = c == 0 ? split_pack4(ra[0].x, ra[1].x, ra[2].x, ra[3].x, lo)
+                                                 : split_pack4(ra[0].y, ra[1].y, ra[2].y, ra[3].y, lo);
+                L.b[2 * c + lo][bc][ub] = c == 0 ? split_pack4(rb[0].x, rb[1].x, rb[2].x, rb[3].x, lo)
+                                                 : split_pack4(rb[0].y, rb[1].y, rb[2].y, rb[3].y, lo);
+            }
+    };
+    load_chunk(0);
+    for (int k0 = 0; k0 < kk; k0 += 16) {
+        store_chunk();
+        __syncthreads();
+        if (k0 + 16 < kk) load_chunk(k0 + 16);
+        const bf16x8_t arh = *reinterpret_cast<const bf16x8_t*>(&L.a[0][wr + l32][2 * h]);
+        const bf16x8_t arl = *reinterpret_cast<const bf16x8_t*>(&L.a[1][wr + l32][2 * h]);
+        const bf16x8_t aih = *reinterpret_cast<const bf16x8_t*>(&L.a[2][wr + l32][2 * h]);
+        const bf16x8_t ail = *reinterpret_cast<const bf16x8_t*>(&L.a[3][wr + l32][2 * h]);
+        const bf16x8_t brh = *reinterpret_cast<const bf16x8_t*>(&L.b[0][wc + l32][2 * h]);
+        const bf16x8_t brl = *reinterpret_cast<const bf16x8_t*>(&L.b[1][wc + l32][2 * h]);
+        const bf16x8_t bih = *reinterpret_cast<const bf16x8_t*>(&L.b[2][wc + l32][2 * h]);
+        const bf16x8_t bil = *reinterpret_cast<const bf16x8_t*>(&L.b[3][wc + l32][2 * h]);
+        const bf16x8_t nbih = bf16x8_neg(bih), nbil = bf16x8_neg(bil);
+        acc_re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh, arh, acc_re, 0, 0, 0);
+        acc_im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bih, arh, acc_im, 0, 0, 0);
+        acc_re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh, arl, acc_re, 0, 0, 0);
+        acc_im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bih, arl, acc_im, 0, 0, 0);
+        acc_re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brl, arh, acc_re, 0, 0, 0);
+        acc_im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bil, arh, acc_im, 0, 0, 0);
+        acc_re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbih, aih, acc_re, 0, 0, 0);
+        acc_im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh, aih, acc_im, 0, 0, 0);
+        acc_re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbih, ail, acc_re, 0, 0, 0);
+        acc_im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh, ail, acc_im, 0, 0, 0);
+        acc_re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbil, aih, acc_re, 0, 0, 0);
+        acc_im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brl, aih, acc_im, 0, 0, 0);
+        __syncthreads();
+    }
+}
+
+// C[m x nn] (+)= (-)A[m x kk] B[kk x nn], FP32 complex column-major, split-bf16 products
+template <bool SUB, bool ACCUM>
+__global__ void __launch_bounds__(256) zgemm_fsplit_kernel(int m, int nn, int kk, const cplxf* __restrict__ Ab, int lda,
+                                                           int64_t sA, const cplxf* __restrict__ Bb, int ldb, int64_t sB,
+                                                           cplxf* __restrict__ Cb, int ldc, int64_t sC) {
+    __shared__ FsplitLds L;
+    const int bz = blockIdx.z;
+    const int row0 = blockIdx.x * 64, col0 = blockIdx.y * 64;
+    f32x16_t acc_re, acc_im;
+    fsplit_tile64(Ab + sA * bz + row0, lda, Bb + sB * bz + (int64_t)ldb * col0, ldb, min(64, m - row0),
+                  min(64, nn - col0), kk, L, acc_re, acc_im);
+    cplxf* Cm = Cb + sC * bz;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5, l32 = lane & 31;
+    const int r = row0 + 32 * (wave & 1) + l32;
+    if (r >= m) return;
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int c = col0 + 32 * (wave >> 1) + 8 * (v >> 2) + 4 * h + (v & 3);
+        if (c < nn) {
+            float re = SUB ? -acc_re[v] : acc_re[v], im = SUB ? -acc_im[v] : acc_im[v];
+            if (ACCUM) {
+                const cplxf old = Cm[r + (int64_t)ldc * c];
+                re += old.x;
+                im += old.y;
+            }
+            Cm[r + (int64_t)ldc * c] = make_float2(re, im);
+        }
+    }
+}
+
+static void gemm_fsplit(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cplxf* A, int lda,
+                        int64_t sA, const cplxf* B, int ldb, int64_t sB, cplxf* Cm, int ldc, int64_t sC, int batch) {
+    if (m <= 0 || nn <= 0) return;
+    dim3 grid((m + 63) / 64, (nn + 63) / 64, batch);
+    if (sub && accum)
+        zgemm_fsplit_kernel<true, true><<<grid, 256, 0, st>>>(m, nn, kk, A, lda, sA, B, ldb, sB, Cm, ldc, sC);
+    else if (!sub && accum)
+        zgemm_fsplit_kernel<false, true><<<grid, 256, 0, st>>>(m, nn, kk, A, lda, sA, B, ldb, sB, Cm, ldc, sC);
+    else
+        zgemm_fsplit_kernel<false, false><<<grid, 256, 0, st>>>(m, nn, kk, A, lda, sA, B, ldb, sB, Cm, ldc, sC);
+}
+
+// panel_inv_gemm_kernel on split-bf16 products: L21 = A21 U11^{-1} (part 0, with the check of
+// LAPACK's rule on the rows below the block) and U12 = L11^{-1} A12 (part 1), in place
+__global__ void __launch_bounds__(256) panel_inv_fsplit_kernel(MatBatch<cplxf> mb, int n, int K, const cplxf* inv,
+                                                               int64_t inv_stride, int* flags, int64_t flag_stride) {
+    __shared__ FsplitLds L;
+    __shared__ float au[NB];
+    const int b = blockIdx.z, part = blockIdx.y;
+    cplxf* Am = mb.A + mb.stride * b;
+    const int lda = mb.lda;
+    const int nb = min(NB, n - K), KE = K + nb;
+    const cplxf* iv = inv + inv_stride * b;
+    const int r0 = KE + 64 * blockIdx.x;  // part 0: first row; part 1: first column
+    const int rows = part == 0 ? min(64, n - r0) : nb;
+    const int cols = part == 0 ? nb : min(64, n - r0);
+    const cplxf* X = part == 0 ? Am + r0 + (int64_t)lda * K : iv;
+    const int ldx = part == 0 ? lda : NB;
+    const cplxf* Y = part == 0 ? iv + NB * NB : Am + K + (int64_t)lda * r0;
+    const int ldy = part == 0 ? NB : lda;
+    cplxf* Out = part == 0 ? Am + r0 + (int64_t)lda * K : Am + K + (int64_t)lda * r0;
+    if (part == 0 && threadIdx.x < NB)
+        au[threadIdx.x] = threadIdx.x < nb ? tabs1(Am[(K + threadIdx.x) + (int64_t)lda * (K + threadIdx.x)]) : 0.f;
+    f32x16_t acc_re, acc_im;
+    fsplit_tile64(X, ldx, Y, ldy, rows, cols, nb, L, acc_re, acc_im);  // its barriers order the au writes
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5, l32 = lane & 31;
+    const int r = 32 * (wave & 1) + l32;
+    bool bad = false;
+    if (r < rows) {
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int c = 32 * (wave >> 1) + 8 * (v >> 2) + 4 * h + (v & 3);
+            if (c < cols) {
+                const cplxf val = make_float2(acc_re[v], acc_im[v]);
+                if (part == 0) {
+                    const cplxf ukk = Am[(K + c) + (int64_t)lda * (K + c)];
+                    bad = bad || tabs1(tmul(val, ukk)) > au[c];
+                }
+                Out[r + (int64_t)lda * c] = val;
+            }
+        }
+    }
+    if (bad) flags[flag_stride * b] = 1;
+}
+
+// ---------------------------------------------------------------------------
 // getrf driver.  ipiv: 0-based global row indices, int32, per matrix stride.
 // work_perm: int workspace of (1 + 4*NB) per matrix.  info: int per matrix (0 / first zero pivot, 1-based).
 // ---------------------------------------------------------------------------
@@ -1160,9 +1337,28 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
         diag_lu_kernel<C><<<dim3(1, 1, batch), 512, 0, st>>>(mb, n, K, ipiv, ipiv_stride, work_perm, pstride, inv,
                                                              inv_stride);
         const int m = n - min(n, K + NB);
-        if (m > 0)
-            panel_inv_gemm_kernel<C><<<dim3((m + 63) / 64, 2, batch), 256, 0, st>>>(mb, n, K, inv, inv_stride, work_perm,
-                                                                                     pstride);
+        if (m <= 0) return;
+        if constexpr (kF32)
+            if (use_split) {
+                panel_inv_fsplit_kernel<<<dim3((m + 63) / 64, 2, batch), 256, 0, st>>>(mb, n, K, inv, inv_stride,
+                                                                                     work_perm, pstride);
+                return;
+            }
+        panel_inv_gemm_kernel<C><<<dim3((m + 63) / 64, 2, batch), 256, 0, st>>>(mb, n, K, inv, inv_stride, work_perm,
+                                                                                 pstride);
+    };
+    // the left-looking products inside a super-panel: A[r0:r0+m, c0:c0+nn] -= A[r0:, K:K+kk] A[K:K+kk, c0:]
+    // (split-bf16 products on the split path, like its trailing updates)
+    auto narrow = [&](int r0, int c0, int m, int nn, int K, int kk) {
+        const C* Ap = A + r0 + (int64_t)lda * K;
+        const C* Bp = A + K + (int64_t)lda * c0;
+        C* Cp = A + r0 + (int64_t)lda * c0;
+        if constexpr (kF32)
+            if (use_split) {
+                gemm_fsplit(st, true, true, m, nn, kk, Ap, lda, strideA, Bp, lda, strideA, Cp, lda, strideA, batch);
+                return;
+            }
+        gemm_t<C>(st, true, true, m, nn, kk, Ap, lda, strideA, Bp, lda, strideA, Cp, lda, strideA, batch);
     };
     // A[KE:, KE:] -= A[KE:, K:KE] A[K:KE, KE:] (the timed launch: the GEMM alone)
     auto update = [&](int K, int KE) {
@@ -1221,21 +1417,15 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
         const int KS = K + S * NB;
         for (int j = 0; j < S; j++) {
             const int Kj = K + j * NB, Kq = Kj + NB;
-            if (j > 0)
-                gemm_t<C>(st, true, true, n - Kj, NB, Kj - K, A + Kj + (int64_t)lda * K, lda, strideA,
-                          A + K + (int64_t)lda * Kj, lda, strideA, A + Kj + (int64_t)lda * Kj, lda, strideA, batch);
+            if (j > 0) narrow(Kj, Kj, n - Kj, NB, K, Kj - K);
             if (verified) {
-                if (j > 0)
-                    gemm_t<C>(st, true, true, NB, n - Kq, Kj - K, A + Kj + (int64_t)lda * K, lda, strideA,
-                              A + K + (int64_t)lda * Kq, lda, strideA, A + Kj + (int64_t)lda * Kq, lda, strideA, batch);
+                if (j > 0) narrow(Kj, Kq, NB, n - Kq, K, Kj - K);
                 factor_verified(Kj);
                 continue;
             }
             factor_panel(Kj, Kq);
             permute_others(Kj, Kq);
-            if (j > 0)
-                gemm_t<C>(st, true, true, NB, n - Kq, Kj - K, A + Kj + (int64_t)lda * K, lda, strideA,
-                          A + K + (int64_t)lda * Kq, lda, strideA, A + Kj + (int64_t)lda * Kq, lda, strideA, batch);
+            if (j > 0) narrow(Kj, Kq, NB, n - Kq, K, Kj - K);
             trsm(Kj, Kq);
         }
         update(K, KS);
