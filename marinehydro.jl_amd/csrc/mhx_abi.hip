@@ -448,7 +448,7 @@ constexpr int kRefineMaxSplit = 8;  // split-bf16 trailing updates (factors to ~
 // (launches of one workgroup per item, latency-bound) then run beside the trailing updates of the
 // others, on compute units the panels leave idle.  The parts share nothing but the device.
 int getrf_f32_two_streams(mhx_ctx* ctx, mhx_body* b, int batch, mhx::LaunchTimer* lt, unsigned short* split,
-                          int pivot) {
+                          int pivot, cplx* inv64 = nullptr) {
     const int n = b->n;
     int parts = 1;
     if (batch >= 8 && n >= 512) parts = std::max(1, std::min(ctx->lu_streams, batch / 4));
@@ -462,7 +462,8 @@ int getrf_f32_two_streams(mhx_ctx* ctx, mhx_body* b, int batch, mhx::LaunchTimer
         rc = mhx::getrf_batched(st, n, b->Df + (size_t)first * b->nn(), n, b->nn(), cnt, b->ipiv + (size_t)first * n, n,
                                 b->wperm + (size_t)first * mhx::kPermWork, b->info + first, lt,
                                 split ? split + (size_t)first * mhx::split_work(n) : nullptr, pivot,
-                                reinterpret_cast<cplxf*>(b->work + (size_t)first * b->sW()), 2 * b->sW());
+                                reinterpret_cast<cplxf*>(b->work + (size_t)first * b->sW()), 2 * b->sW(),
+                                inv64 ? inv64 + (size_t)first * b->sW() : nullptr, b->sW());
         if (p > 0) HCHK(hipEventRecord(ctx->join[p - 1], st));
     }
     for (int p = 1; p < parts; p++) HCHK(hipStreamWaitEvent(ctx->stream, ctx->join[p - 1], 0));
@@ -480,6 +481,7 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf,
     const bool split = use_split(ctx, gf);
     if (split && (rc = dalloc(ctx, &b->Ps, (size_t)mhx::split_work(n) * b->cap_batch))) return rc;
     const int refine_max = split ? kRefineMaxSplit : kRefineMax;
+    bool inverses_ready = true;  // the verified factorization wrote the solves' diagonal-block inverses
     mhx::launch_zero(st, b->info, sizeof(int) * batch);
     {
         Phase ph(ctx, "lu");
@@ -487,8 +489,11 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf,
         b->df_ready = false;
         mhx::LaunchTimer lt;
         const bool verified = ctx->pivoting == MHX_PIVOT_VERIFIED;
+        // with verified pivots the factorization also leaves the FP64 diagonal-block inverses in
+        // the solves' tables (work after the split-K partials, for R right-hand sides)
         rc = getrf_f32_two_streams(ctx, b, batch, ctx->timing ? &lt : nullptr, split ? b->Ps : nullptr,
-                                   verified ? mhx::kPivotVerified : mhx::kPivotStrict);
+                                   verified ? mhx::kPivotVerified : mhx::kPivotStrict,
+                                   verified ? mhx::getrs_inverses(b->work + 16 * b->nR(), n, R) : nullptr);
         if (rc) {
             take_launches(ctx, lt);
             return fail(ctx, MHX_EDIM, "LU: order above 16384 is not supported");
@@ -504,6 +509,7 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf,
                     continue;
                 }
                 ctx->lu_interchanged++;
+                inverses_ready = false;  // the refactored item's blocks differ from the verified pass
                 mhx::convert_to_f32(st, b->nn(), b->D + q * b->nn(), b->Df + q * b->nn());
                 mhx::getrf_batched(st, n, b->Df + q * b->nn(), n, b->nn(), 1, b->ipiv + (size_t)q * n, n,
                                    b->wperm + (size_t)q * mhx::kPermWork, b->info + q, ctx->timing ? &lt : nullptr,
@@ -511,6 +517,7 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf,
             }
         } else {
             ctx->lu_interchanged += batch;
+            inverses_ready = false;
         }
         take_launches(ctx, lt);
     }
@@ -525,7 +532,8 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf,
         if (direct)
             mhx::launch_matmul_nr(st, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), batch);
         HCHK(hipMemcpyAsync(b->Rr, X, sizeof(cplx) * nRb, hipMemcpyDeviceToDevice, st));
-        mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, X, b->nR(), W2, b->sW(), batch);
+        mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, X, b->nR(), W2, b->sW(), batch,
+                           inverses_ready ? mhx::kGetrsInverses : mhx::kGetrsFresh);
         std::vector<double> nr(2 * batch);
         for (int it = 0; it <= refine_max; it++) {
             mhx::launch_matmul_nr(st, n, R, b->D, b->nn(), X, b->nR(), b->Tr, b->nR(), b->work, b->sW(), batch);
@@ -760,7 +768,8 @@ void sweep_enqueue(mhx_ctx* ctx, SweepState& s) {
                 // one refinement step against the FP64 D and the residual norms for the host's check
                 if (!surr) mhx::convert_to_f32(st, b->nn() * bt, b->D, b->Df);
                 getrf_f32_two_streams(ctx, b, bt, nullptr, s.split ? b->Ps : nullptr,
-                                      s.verified ? mhx::kPivotVerified : mhx::kPivotStrict);
+                                      s.verified ? mhx::kPivotVerified : mhx::kPivotStrict,
+                                      s.verified ? mhx::getrs_inverses(b->work + 16 * b->nR(), n, R) : nullptr);
                 if (s.verified) mhx::launch_gather_pivot_flags(st, b->wperm, bt, s.pflags + s.flag_off[ib] + w0);
                 mhx::launch_sticky_info(st, b->info, bt, s.sticky + ib);
                 cplx* W2 = b->work + 16 * b->nR();
@@ -769,7 +778,10 @@ void sweep_enqueue(mhx_ctx* ctx, SweepState& s) {
                 if (s.direct)
                     mhx::launch_matmul_nr(st, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), bt);
                 mhx::launch_copy(st, X, b->Rr, sizeof(cplx) * nRb);
-                mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, X, b->nR(), W2, b->sW(), bt);
+                // (a flagged item's results are replaced by the pivoted redo after the sweep; its
+                // verified factors and their inverses are consistent with each other meanwhile)
+                mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, X, b->nR(), W2, b->sW(), bt,
+                                   s.verified ? mhx::kGetrsInverses : mhx::kGetrsFresh);
                 mhx::launch_matmul_nr(st, n, R, b->D, b->nn(), X, b->nR(), b->Tr, b->nR(), b->work, b->sW(), bt);
                 mhx::launch_zero(st, b->norms, sizeof(double) * 2 * bt);
                 mhx::launch_residual(st, (int64_t)n * R, b->Rr, b->nR(), b->Tr, b->nR(), b->norms, bt);

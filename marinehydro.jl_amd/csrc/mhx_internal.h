@@ -98,26 +98,35 @@ void zgemm(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cp
 constexpr int kPivotStrict = 0;
 constexpr int kPivotVerified = 1;
 constexpr int kMaxOrder = 16384;  // largest order of the factorization (leaf register tiles)
+// inv64 (verified pass, optional): where the FP64 copies of the diagonal blocks' triangular
+// inverses go, in the layout of getrs_batched's inverse tables (getrs_inverses(work, n, R) of the
+// getrs work of each matrix, stride inv64_stride), so that the solves need not form them again.
 int getrf_batched(hipStream_t st, int n, cplx* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
                   int* work_perm, int* info, LaunchTimer* update_timer = nullptr, int pivot = kPivotStrict,
-                  cplx* inv = nullptr, int64_t inv_stride = 0);
+                  cplx* inv = nullptr, int64_t inv_stride = 0, cplx* inv64 = nullptr, int64_t inv64_stride = 0);
 // complex elements of getrs work per matrix (sW must be at least this).  getrs_batched keeps the
-// diagonal-block inverses and the row permutation in work / perm: a later call on the same
-// factors with prepared = true reuses them.
+// diagonal-block inverses and the row permutation in work / perm.  prepared: kGetrsFresh forms
+// both; kGetrsPrepared reuses both (a later call on the same factors); kGetrsInverses reuses the
+// inverses written by the verified factorization (inv64) and forms the permutation.
+constexpr int kGetrsFresh = 0;
+constexpr int kGetrsPrepared = 1;
+constexpr int kGetrsInverses = 2;
 int64_t getrs_work(int n, int R);
+cplx* getrs_inverses(cplx* work, int n, int R);  // the inverse tables inside one matrix's getrs work
 int getrs_batched(hipStream_t st, int trans, int n, int R, const cplx* LU, int lda, int64_t sA, const int* ipiv,
                   int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
-                  int batch, bool prepared = false);
+                  int batch, int prepared = kGetrsFresh);
 // FP32 factorization (mixed-precision solver); the solves read FP32 factors and compute in FP64
 // split: workspace of split_work(n) bf16 per matrix for the split-bf16 trailing updates (nullptr:
 // FP32 MFMA updates)
 int getrf_batched(hipStream_t st, int n, cplxf* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
                   int* work_perm, int* info, LaunchTimer* update_timer = nullptr, unsigned short* split = nullptr,
-                  int pivot = kPivotStrict, cplxf* inv = nullptr, int64_t inv_stride = 0);
+                  int pivot = kPivotStrict, cplxf* inv = nullptr, int64_t inv_stride = 0, cplx* inv64 = nullptr,
+                  int64_t inv64_stride = 0);
 int64_t split_work(int n);
 int getrs_batched(hipStream_t st, int trans, int n, int R, const cplxf* LU, int lda, int64_t sA, const int* ipiv,
                   int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
-                  int batch, bool prepared = false);
+                  int batch, int prepared = kGetrsFresh);
 void convert_to_f32(hipStream_t st, int64_t count, const cplx* src, cplxf* dst);
 // T <- Bv - T (n R elements per item) and per-item squared norms |T|^2, |Bv|^2 added into norms[2z], norms[2z+1]
 void launch_residual(hipStream_t st, int64_t cnt, const cplx* Bv, int64_t sB, cplx* T, int64_t sT, double* norms,

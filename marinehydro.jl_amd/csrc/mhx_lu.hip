@@ -475,7 +475,8 @@ template <class C> __device__ __forceinline__ C crecip(C u) {
 template <class C>
 __global__ void __launch_bounds__(512) diag_lu_kernel(MatBatch<C> mb, int n, int K, int* __restrict__ ipiv,
                                                       int64_t ipiv_stride, int* __restrict__ flags, int64_t flag_stride,
-                                                      C* __restrict__ inv, int64_t inv_stride) {
+                                                      C* __restrict__ inv, int64_t inv_stride, cplx* __restrict__ inv64,
+                                                      int64_t inv64_stride, int64_t inv64_u) {
     typedef typename Cx<C>::R R;
     const int b = blockIdx.z;
     C* A = mb.A + mb.stride * b;
@@ -617,6 +618,11 @@ __global__ void __launch_bounds__(512) diag_lu_kernel(MatBatch<C> mb, int n, int
         }
         C* out = inv + inv_stride * b + (tri == 0 ? 0 : NB * NB);
         for (int e = tid; e < NB * NB; e += 512) out[e] = X[e & 63][e >> 6];
+        // the same inverse in FP64, as block K / 64 of the triangular solves' inverse tables
+        if (inv64) {
+            cplx* o64 = inv64 + inv64_stride * b + (int64_t)(K / NB) * NB * NB + (tri == 0 ? 0 : inv64_u);
+            for (int e = tid; e < NB * NB; e += 512) o64[e] = to_d(X[e & 63][e >> 6]);
+        }
         __syncthreads();
     }
 }
@@ -1303,7 +1309,7 @@ static int leaf_dispatch(hipStream_t st, MatBatch<C> mb, int n, int c0, int K, i
 template <class C>
 static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
                    int* work_perm, int* info, LaunchTimer* timer, unsigned short* split, int pivot, C* inv,
-                   int64_t inv_stride) {
+                   int64_t inv_stride, cplx* inv64, int64_t inv64_stride) {
     if (n <= 0) return 0;
     if (n > kMaxOrder) return -1;
     if (pivot == kPivotVerified && !inv) return -2;
@@ -1335,7 +1341,8 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
     // L21 rows and U12 columns as products with those inverses
     auto factor_verified = [&](int K) {
         diag_lu_kernel<C><<<dim3(1, 1, batch), 512, 0, st>>>(mb, n, K, ipiv, ipiv_stride, work_perm, pstride, inv,
-                                                             inv_stride);
+                                                             inv_stride, inv64, inv64_stride,
+                                                             (int64_t)((n + NB - 1) / NB) * NB * NB);
         const int m = n - min(n, K + NB);
         if (m <= 0) return;
         if constexpr (kF32)
@@ -1435,16 +1442,17 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
 }
 
 int getrf_batched(hipStream_t st, int n, cplx* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
-                  int* work_perm, int* info, LaunchTimer* update_timer, int pivot, cplx* inv, int64_t inv_stride) {
+                  int* work_perm, int* info, LaunchTimer* update_timer, int pivot, cplx* inv, int64_t inv_stride,
+                  cplx* inv64, int64_t inv64_stride) {
     return getrf_t(st, n, A, lda, strideA, batch, ipiv, ipiv_stride, work_perm, info, update_timer, nullptr, pivot, inv,
-                   inv_stride);
+                   inv_stride, inv64, inv64_stride);
 }
 
 int getrf_batched(hipStream_t st, int n, cplxf* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
                   int* work_perm, int* info, LaunchTimer* update_timer, unsigned short* split, int pivot, cplxf* inv,
-                  int64_t inv_stride) {
+                  int64_t inv_stride, cplx* inv64, int64_t inv64_stride) {
     return getrf_t(st, n, A, lda, strideA, batch, ipiv, ipiv_stride, work_perm, info, update_timer, split, pivot, inv,
-                   inv_stride);
+                   inv_stride, inv64, inv64_stride);
 }
 
 // ---------------------------------------------------------------------------
@@ -1718,10 +1726,12 @@ int64_t getrs_work(int n, int R) {
     return flag_words(n) + 2 * (int64_t)n * R + 2 * nblk * TRS_B * TRS_B;
 }
 
+cplx* getrs_inverses(cplx* work, int n, int R) { return work + flag_words(n) + 2 * (int64_t)n * R; }
+
 template <class C>
 static int getrs_t(hipStream_t st, int trans, int n, int R, const C* LU, int lda, int64_t sA, const int* ipiv,
                    int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
-                   int batch, bool prepared) {
+                   int batch, int prepared) {
     if (R > MAXR) return -1;
     if (n <= 0 || R <= 0) return 0;
     const int nblk = (n + TRS_B - 1) / TRS_B;
@@ -1729,10 +1739,10 @@ static int getrs_t(hipStream_t st, int trans, int n, int R, const C* LU, int lda
     cplx* W2 = W1 + (int64_t)n * R;
     cplx* Linv = W2 + (int64_t)n * R;
     cplx* Uinv = Linv + (int64_t)nblk * TRS_B * TRS_B;
-    if (!prepared) {
+    if (prepared == kGetrsFresh)
         invert_diag_kernel<C><<<dim3(nblk, 2, batch), TRS_B, 0, st>>>(LU, lda, sA, n, Linv, Uinv, sW);
+    if (prepared != kGetrsPrepared)
         ipiv_to_perm_kernel<<<dim3(1, 1, batch), 256, 0, st>>>(ipiv, ipiv_stride, n, perm, perm_stride);
-    }
     dim3 gp((n + 255) / 256, 1, batch);
     if (trans == 0) {
         permute_rows_kernel<<<gp, 256, 0, st>>>(n, R, perm, perm_stride, B, sB, W1, sW, 0);
@@ -1750,14 +1760,14 @@ static int getrs_t(hipStream_t st, int trans, int n, int R, const C* LU, int lda
 
 int getrs_batched(hipStream_t st, int trans, int n, int R, const cplx* LU, int lda, int64_t sA, const int* ipiv,
                   int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
-                  int batch, bool prepared) {
+                  int batch, int prepared) {
     return getrs_t(st, trans, n, R, LU, lda, sA, ipiv, ipiv_stride, perm, perm_stride, B, sB, work, sW, batch, prepared);
 }
 
 
 int getrs_batched(hipStream_t st, int trans, int n, int R, const cplxf* LU, int lda, int64_t sA, const int* ipiv,
                   int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
-                  int batch, bool prepared) {
+                  int batch, int prepared) {
     return getrs_t(st, trans, n, R, LU, lda, sA, ipiv, ipiv_stride, perm, perm_stride, B, sB, work, sW, batch, prepared);
 }
 
