@@ -442,32 +442,43 @@ constexpr double kRefineTol = 1e-12;
 constexpr int kRefineMax = 4;       // FP32 trailing updates
 constexpr int kRefineMaxSplit = 8;  // split-bf16 trailing updates (factors to ~2^-16)
 
-// FP32 LU of the batch's b->Df in place.  A batch of 8 or more items of order >= 512 is factored
-// in up to ctx->lu_streams parts (at least 4 items each) on as many default-priority streams,
-// forked from and joined back to the library's stream by events: the panel phases of one part
-// (launches of one workgroup per item, latency-bound) then run beside the trailing updates of the
-// others, on compute units the panels leave idle.  The parts share nothing but the device.
-int getrf_f32_two_streams(mhx_ctx* ctx, mhx_body* b, int batch, mhx::LaunchTimer* lt, unsigned short* split,
-                          int pivot, cplx* inv64 = nullptr) {
-    const int n = b->n;
-    int parts = 1;
-    if (batch >= 8 && n >= 512) parts = std::max(1, std::min(ctx->lu_streams, batch / 4));
-    int rc = 0;
+// Parts a batched FP32 factorization and its solves are split into, one stream each: a batch of 8
+// or more items of order >= 512 in up to ctx->lu_streams parts of at least 4 items.
+int lu_parts(const mhx_ctx* ctx, int n, int batch) {
+    return (batch >= 8 && n >= 512) ? std::max(1, std::min(ctx->lu_streams, batch / 4)) : 1;
+}
+
+// fn(stream, first, count) for the items [first, first + count) of each of `parts` parts of a batch:
+// part 0 on the library's stream, part p on side[p - 1] (default priority), forked from and joined
+// back to the library's stream by events.  The parts share nothing but the device.
+template <class F>
+int on_parts(mhx_ctx* ctx, int parts, int batch, F fn) {
     if (parts > 1) HCHK(hipEventRecord(ctx->fork, ctx->stream));
-    // part p: items [first, first + cnt); part 0 on the library's stream, the others on side[p - 1]
+    int rc = 0;
     for (int p = parts - 1; p >= 0 && !rc; p--) {
         const int first = (int)((int64_t)batch * p / parts), cnt = (int)((int64_t)batch * (p + 1) / parts) - first;
         hipStream_t st = p == 0 ? ctx->stream : ctx->side[p - 1];
         if (p > 0) HCHK(hipStreamWaitEvent(st, ctx->fork, 0));
-        rc = mhx::getrf_batched(st, n, b->Df + (size_t)first * b->nn(), n, b->nn(), cnt, b->ipiv + (size_t)first * n, n,
-                                b->wperm + (size_t)first * mhx::kPermWork, b->info + first, lt,
-                                split ? split + (size_t)first * mhx::split_work(n) : nullptr, pivot,
-                                reinterpret_cast<cplxf*>(b->work + (size_t)first * b->sW()), 2 * b->sW(),
-                                inv64 ? inv64 + (size_t)first * b->sW() : nullptr, b->sW());
+        rc = fn(st, first, cnt);
         if (p > 0) HCHK(hipEventRecord(ctx->join[p - 1], st));
     }
     for (int p = 1; p < parts; p++) HCHK(hipStreamWaitEvent(ctx->stream, ctx->join[p - 1], 0));
     return rc;
+}
+
+// FP32 LU of the batch's b->Df in place, in lu_parts parts on their streams: the panel phases of
+// one part (launches of one workgroup per item, latency-bound) run beside the trailing updates of
+// the others, on compute units the panels leave idle.
+int getrf_f32_two_streams(mhx_ctx* ctx, mhx_body* b, int batch, mhx::LaunchTimer* lt, unsigned short* split,
+                          int pivot, cplx* inv64 = nullptr) {
+    const int n = b->n;
+    return on_parts(ctx, lu_parts(ctx, n, batch), batch, [&](hipStream_t st, int first, int cnt) {
+        return mhx::getrf_batched(st, n, b->Df + (size_t)first * b->nn(), n, b->nn(), cnt, b->ipiv + (size_t)first * n,
+                                  n, b->wperm + (size_t)first * mhx::kPermWork, b->info + first, lt,
+                                  split ? split + (size_t)first * mhx::split_work(n) : nullptr, pivot,
+                                  reinterpret_cast<cplxf*>(b->work + (size_t)first * b->sW()), 2 * b->sW(),
+                                  inv64 ? inv64 + (size_t)first * b->sW() : nullptr, b->sW());
+    });
 }
 
 int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf, bool form_phi = true) {
@@ -552,7 +563,8 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf,
                 break;
             }
             if (it == refine_max) break;
-            mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, b->Tr, b->nR(), W2, b->sW(), batch, true);
+            mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, b->Tr, b->nR(), W2, b->sW(), batch,
+                               mhx::kGetrsPrepared);
             mhx::launch_add(st, (int64_t)n * R, b->Tr, b->nR(), X, b->nR(), batch);
             ctx->refine_steps += batch;
         }

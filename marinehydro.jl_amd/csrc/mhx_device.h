@@ -283,6 +283,69 @@ __device__ __noinline__ WuGh wu_term_gh(double hh, double vv) {
     return o;
 }
 
+// The same G and dG/dh with the special functions in FP32 (the learned wave term's base: its
+// network residual is accurate to ~1e-5 of G, and FP32 exp, log, Bessel and Struve values move G
+// by ~1e-7).  The polynomial fits A, B, C, D, dA, dB, dC stay in FP64: their alternating
+// coefficients (|c| up to 2e5) cancel to O(1), which FP32 Horner would leave wrong in the third digit.
+__device__ __forceinline__ float struve_h0f(float xx) {
+    if (xx <= 3.0f) {
+        const float yy = (xx / 3.0f) * (xx / 3.0f);
+        const float h = 1.909859164f +
+                        (-1.909855001f + (0.687514637f + (-0.126164557f + (0.013828813f + -0.000876918f * yy) * yy) * yy) * yy) * yy;
+        return h * (xx / 3.0f);
+    }
+    const float yy = (3.0f / xx) * (3.0f / xx);
+    const float c1 = 2.0f * (0.99999906f + (4.77228920f + (3.85542044f + 0.32303607f * yy) * yy) * yy);
+    const float c2 = (float)MHX_PI * xx * (1.0f + (4.88331068f + (4.28957333f + 0.52120508f * yy) * yy) * yy);
+    return c1 / c2 + y0f(xx);
+}
+__device__ __forceinline__ float struve_h1f(float xx) {
+    if (xx <= 3.0f) {
+        const float yy = (xx / 3.0f) * (xx / 3.0f);
+        return (1.909859286f +
+                (-1.145914713f + (0.294656958f + (-0.042070508f + (0.003785727f + -0.000207183f * yy) * yy) * yy) * yy) * yy) * yy;
+    }
+    const float yy = (3.0f / xx) * (3.0f / xx);
+    const float c1 = 2.0f * (1.00000004f + (3.92205313f + (2.64893033f + 0.27450895f * yy) * yy) * yy);
+    const float c2 = (float)MHX_PI * (1.0f + (3.81095112f + (2.26216956f + 0.10885141f * yy) * yy) * yy);
+    return c1 / c2 + y1f(xx);
+}
+__device__ __noinline__ WuGh wu_term_gh_f32sf(double hh, double vv) {
+    const double dd = sqrt(hh * hh + vv * vv), idd = 1.0 / dd;
+    const double alpha = -vv * idd, beta = hh * idd, rho = dd / (1 + dd);
+    const float hf = (float)hh, vf = (float)vv, ddf = (float)dd;
+    const double ev = (double)expf(vf);
+    const double omr3 = (1.0 - rho) * (1.0 - rho) * (1.0 - rho);
+    const double dd2 = dd * dd, dd3 = dd2 * dd, iden = 1.0 / (1.0 + dd3);
+    const double A = horner_wu<9>(WU_A, rho), B = horner_wu<9>(WU_B, rho), C = horner_wu<7>(WU_C, rho),
+                 D = horner_wu<9>(WU_D, rho);
+    double RR = (1.0 - beta) * A;
+    RR -= beta * B;
+    RR -= alpha * C / (1.0 + 6.0 * alpha * rho * (1.0 - rho));
+    RR += beta * (1.0 - beta) * D;
+    const double Lp = rho * omr3 * RR;
+    double PP = (double)logf(0.5f * (ddf - vf)) + 0.5772156649 - 2.0 * dd2;
+    PP = ev * PP + dd2 - vv;
+    const double L0 = 2.0 * PP * iden + 2.0 * Lp;
+    const double H0 = struve_h0f(hf), J0 = j0f(hf);
+    const double Wr = 2 * MHX_PI * H0 * ev, Wi = 2 * MHX_PI * (-J0) * ev;
+    const double dA = horner_wu<9>(WU_DA, rho), dB = horner_wu<9>(WU_DB, rho), dC = horner_wu<5>(WU_DC, rho);
+    double RS = beta * dA;
+    RS -= (1.0 - alpha) * dB;
+    RS += beta * (1.0 - beta) * rho * (1.0 - 2.0 * rho) * dC;
+    const double Lsp = rho * omr3 * RS;
+    double PS = (beta + hh) / (dd - vv);
+    PS = PS - 2.0 * beta + 2.0 * ev * dd - hh;
+    const double QS = (double)expf(-ddf) * (1.0 - beta) * (1.0 + dd * iden);
+    const double Ls = 2.0 * PS * iden - 4.0 * QS + 2.0 * Lsp;
+    const double H1 = struve_h1f(hf), J1 = j1f(hf);
+    const double Whr = 2 * MHX_PI * (2 / MHX_PI - H1) * ev, Whi = 2 * MHX_PI * J1 * ev;
+    WuGh o;
+    o.G = cmk(-L0 - Wr, -Wi);
+    o.Gh = cmk(-Ls - Whr, -Whi);
+    return o;
+}
+
 // reduced coordinates (wu.jl:7-18; abstract_greens_function.jl:117-138)
 __device__ __forceinline__ void reduced_hv(double x0, double x1, double x2, double y0_, double y1_, double y2_,
                                            double k, double& hh, double& vv) {

@@ -1231,10 +1231,22 @@ static void gemm_fsplit(hipStream_t st, bool sub, bool accum, int m, int nn, int
         zgemm_fsplit_kernel<false, false><<<grid, 256, 0, st>>>(m, nn, kk, A, lda, sA, B, ldb, sB, Cm, ldc, sC);
 }
 
+// Where panel_inv_fsplit_kernel also writes its L21 rows and U12 columns as the split arrays of
+// the coming trailing update (S null: not written): the update of A[KS:, KS:] by the panels of the
+// super-panel starting at column K0, nkc k chunks of 16, rows / columns padded to mpad.  Only rows
+// (L21) and columns (U12) >= KS go there; L21 and U12 of a panel are final once this kernel has
+// run, so the update needs no separate split pass over them.
+struct SplitDst {
+    unsigned short* S;
+    int64_t sS;
+    int KS, K0, nkc, mpad;
+};
+
 // panel_inv_gemm_kernel on split-bf16 products: L21 = A21 U11^{-1} (part 0, with the check of
 // LAPACK's rule on the rows below the block) and U12 = L11^{-1} A12 (part 1), in place
 __global__ void __launch_bounds__(256) panel_inv_fsplit_kernel(MatBatch<cplxf> mb, int n, int K, const cplxf* inv,
-                                                               int64_t inv_stride, int* flags, int64_t flag_stride) {
+                                                               int64_t inv_stride, int* flags, int64_t flag_stride,
+                                                               SplitDst sd) {
     __shared__ FsplitLds L;
     __shared__ float au[NB];
     const int b = blockIdx.z, part = blockIdx.y;
@@ -1272,6 +1284,69 @@ __global__ void __launch_bounds__(256) panel_inv_fsplit_kernel(MatBatch<cplxf> m
         }
     }
     if (bad) flags[flag_stride * b] = 1;
+    if (!sd.S || r >= rows) return;
+    unsigned short* Sb = sd.S + sd.sS * b;
+    if (part == 0) {
+        // row r0 + r of L21: k = K - K0 + c, four consecutive k per accumulator group (8-byte stores)
+        const int rr = r0 + r - sd.KS;
+        if (rr < 0) return;
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+            const int c0 = 32 * (wave >> 1) + 8 * g + 4 * h;
+            if (c0 >= cols) continue;
+            const int k = K - sd.K0 + c0;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint2 w = q < 2 ? split_pack4(acc_re[4 * g], acc_re[4 * g + 1], acc_re[4 * g + 2], acc_re[4 * g + 3], q & 1)
+                                      : split_pack4(acc_im[4 * g], acc_im[4 * g + 1], acc_im[4 * g + 2], acc_im[4 * g + 3], q & 1);
+                *reinterpret_cast<uint2*>(Sb + (((int64_t)q * sd.nkc + k / 16) * sd.mpad + rr) * 16 + k % 16) = w;
+            }
+        }
+    } else {
+        // row r of U12 (k = K - K0 + r), columns r0 + c
+        const int k = K - sd.K0 + r;
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int c = 32 * (wave >> 1) + 8 * (v >> 2) + 4 * h + (v & 3);
+            const int cc = r0 + c - sd.KS;
+            if (c >= cols || cc < 0) continue;
+            const float re = acc_re[v], im = acc_im[v];
+            const unsigned short rh = bf16_rne(re), ih = bf16_rne(im);
+            const unsigned short part4[4] = {rh, bf16_rne(re - bf16_val(rh)), ih, bf16_rne(im - bf16_val(ih))};
+#pragma unroll
+            for (int q = 0; q < 4; q++)
+                Sb[(((int64_t)(4 + q) * sd.nkc + k / 16) * sd.mpad + cc) * 16 + k % 16] = part4[q];
+        }
+    }
+}
+
+// The two left-looking products of panel j of a super-panel in one launch (they write disjoint
+// blocks and read only the earlier panels): product 0, its columns A[Kj:n, Kj:Kq] -= A[Kj:n, K:Kj]
+// A[K:Kj, Kj:Kq] (row tile blockIdx.x); product 1, its U rows A[Kj:Kq, Kq:n] -= A[Kj:Kq, K:Kj]
+// A[K:Kj, Kq:n] (column tile blockIdx.x).  blockIdx.y selects the product.
+__global__ void __launch_bounds__(256) narrow_pair_fsplit_kernel(MatBatch<cplxf> mb, int n, int K, int Kj) {
+    __shared__ FsplitLds L;
+    const int b = blockIdx.z, lda = mb.lda, Kq = Kj + NB, kk = Kj - K;
+    cplxf* A = mb.A + mb.stride * b;
+    const int prod = blockIdx.y;
+    const int m = prod == 0 ? n - Kj : NB, nn = prod == 0 ? NB : n - Kq;
+    const int row0 = Kj + (prod == 0 ? 64 * blockIdx.x : 0), col0 = prod == 0 ? Kj : Kq + 64 * blockIdx.x;
+    const int rows = min(64, Kj + m - row0), cols = min(64, (prod == 0 ? Kq : n) - col0);
+    if (rows <= 0 || cols <= 0) return;
+    f32x16_t acc_re, acc_im;
+    fsplit_tile64(A + row0 + (int64_t)lda * K, lda, A + K + (int64_t)lda * col0, lda, rows, cols, kk, L, acc_re, acc_im);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, h = lane >> 5, l32 = lane & 31;
+    const int r = 32 * (wave & 1) + l32;
+    if (r >= rows) return;
+#pragma unroll
+    for (int v = 0; v < 16; v++) {
+        const int c = 32 * (wave >> 1) + 8 * (v >> 2) + 4 * h + (v & 3);
+        if (c < cols) {
+            cplxf* p = A + (row0 + r) + (int64_t)lda * (col0 + c);
+            const cplxf o = *p;
+            *p = make_float2(o.x - acc_re[v], o.y - acc_im[v]);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1339,7 +1414,7 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
     };
     // verified diagonal pivots: the panel's diagonal block (and its triangular inverses), then its
     // L21 rows and U12 columns as products with those inverses
-    auto factor_verified = [&](int K) {
+    auto factor_verified = [&](int K, SplitDst sd) {
         diag_lu_kernel<C><<<dim3(1, 1, batch), 512, 0, st>>>(mb, n, K, ipiv, ipiv_stride, work_perm, pstride, inv,
                                                              inv_stride, inv64, inv64_stride,
                                                              (int64_t)((n + NB - 1) / NB) * NB * NB);
@@ -1348,7 +1423,7 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
         if constexpr (kF32)
             if (use_split) {
                 panel_inv_fsplit_kernel<<<dim3((m + 63) / 64, 2, batch), 256, 0, st>>>(mb, n, K, inv, inv_stride,
-                                                                                     work_perm, pstride);
+                                                                                     work_perm, pstride, sd);
                 return;
             }
         panel_inv_gemm_kernel<C><<<dim3((m + 63) / 64, 2, batch), 256, 0, st>>>(mb, n, K, inv, inv_stride, work_perm,
@@ -1368,11 +1443,23 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
         gemm_t<C>(st, true, true, m, nn, kk, Ap, lda, strideA, Bp, lda, strideA, Cp, lda, strideA, batch);
     };
     // A[KE:, KE:] -= A[KE:, K:KE] A[K:KE, KE:] (the timed launch: the GEMM alone)
+    // the split arrays of the update of A[KS:, KS:] by the panels from K0 on (written by the
+    // verified path's panel products; none otherwise)
+    auto split_dst = [&](int K0, int KS) {
+        SplitDst sd{nullptr, 0, KS, K0, 0, 0};
+        if (use_split && verified && KS < n) {
+            sd.S = split;
+            sd.sS = split_work(n);
+            sd.nkc = (KS - K0) / SPLIT_KC;
+            sd.mpad = split_pad(n - KS);
+        }
+        return sd;
+    };
     auto update = [&](int K, int KE) {
         const int m = n - KE, mpad = split_pad(m), ks = (KE - K + NB - 1) / NB * NB;
         const int64_t sS = split_work(n);
         if constexpr (kF32)
-            if (use_split)
+            if (use_split && !verified)
                 split_panels_kernel<<<dim3((mpad * (ks / SPLIT_KC) + 255) / 256, 1, batch), 256, 0, st>>>(mb, n, K, KE, mpad,
                                                                                                   ks, split, sS);
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1401,7 +1488,7 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
         if (S == 1) {  // one panel
             const int KE = min(n, K + NB);
             if (verified) {
-                factor_verified(K);
+                factor_verified(K, split_dst(K, KE));
             } else {
                 factor_panel(K, KE);
                 permute_others(K, KE);
@@ -1424,12 +1511,25 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
         const int KS = K + S * NB;
         for (int j = 0; j < S; j++) {
             const int Kj = K + j * NB, Kq = Kj + NB;
-            if (j > 0) narrow(Kj, Kj, n - Kj, NB, K, Kj - K);
             if (verified) {
-                if (j > 0) narrow(Kj, Kq, NB, n - Kq, K, Kj - K);
-                factor_verified(Kj);
+                if (j > 0) {
+                    if constexpr (kF32) {
+                        if (use_split) {
+                            const int tiles = max((n - Kj + 63) / 64, (n - Kq + 63) / 64);
+                            narrow_pair_fsplit_kernel<<<dim3(tiles, 2, batch), 256, 0, st>>>(mb, n, K, Kj);
+                        } else {
+                            narrow(Kj, Kj, n - Kj, NB, K, Kj - K);
+                            narrow(Kj, Kq, NB, n - Kq, K, Kj - K);
+                        }
+                    } else {
+                        narrow(Kj, Kj, n - Kj, NB, K, Kj - K);
+                        narrow(Kj, Kq, NB, n - Kq, K, Kj - K);
+                    }
+                }
+                factor_verified(Kj, split_dst(K, KS));
                 continue;
             }
+            if (j > 0) narrow(Kj, Kj, n - Kj, NB, K, Kj - K);
             factor_panel(Kj, Kq);
             permute_others(Kj, Kq);
             if (j > 0) narrow(Kj, Kq, NB, n - Kq, K, Kj - K);
@@ -1467,13 +1567,22 @@ __global__ void ipiv_to_perm_kernel(const int* __restrict__ ipiv, int64_t ipiv_s
     const int b = blockIdx.z;
     const int* piv = ipiv + ipiv_stride * b;
     int* idx = perm + perm_stride * b;
+    // the interchanges are applied in order by one thread; the pivots reach it through LDS in
+    // chunks (read one by one from global memory, a batch of order-4096 matrices took 265 us)
+    constexpr int CH = 2048;
+    __shared__ int ps[CH];
     for (int i = threadIdx.x; i < n; i += blockDim.x) idx[i] = i;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        for (int k = 0; k < n; k++) {
-            int p = piv[k];
-            if (p != k) { int t = idx[k]; idx[k] = idx[p]; idx[p] = t; }
-        }
+    for (int k0 = 0; k0 < n; k0 += CH) {
+        const int cnt = min(CH, n - k0);
+        __syncthreads();  // the previous chunk's interchanges (and the identity) are done
+        for (int i = threadIdx.x; i < cnt; i += blockDim.x) ps[i] = piv[k0 + i];
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int k = 0; k < cnt; k++) {
+                const int p = ps[k], kk = k0 + k;
+                if (p != kk) { int t = idx[kk]; idx[kk] = idx[p]; idx[p] = t; }
+            }
+    }
 }
 
 // gather (inverse=0): Y[i] = X[idx[i]];  scatter (inverse=1): Y[idx[i]] = X[i]

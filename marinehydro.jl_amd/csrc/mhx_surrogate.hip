@@ -724,7 +724,7 @@ __device__ __forceinline__ void surr_unit(const SurrLds& L, int lane, int n, int
     if (!__ballot(net)) return;
     cplx G = cmk(0.0, 0.0), Gr = cmk(0.0, 0.0);
     if (net) {
-        const WuGh w = wu_term_gh(hh_me > 0 ? hh_me : 1e-300, vv_me);
+        const WuGh w = wu_term_gh_f32sf(hh_me > 0 ? hh_me : 1e-300, vv_me);
         const double is = 1.0 / SURR_SCALE;
         // the imaginary outputs' scale in FP32 (the network's outputs carry BF16 rounding; an FP64
         // exp here costs constants that the compiler keeps in registers across the unit loop)
