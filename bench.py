@@ -256,8 +256,9 @@ def measure(ctx, run, ws, n, gf, nlu=None, units=None, kind="fp64", calls=1):
         bu = lu_update_bytes(n) * nw
         traffic = None
         items = nw / cu * len(lu_update_steps(n, two_level=True)) if cu else 0
-        if n == 4096 and abs(items - 32) < 1e-9:  # the profiled configuration: 32 matrices per launch
-            traffic = profiled_traffic("lu_update_split_4096x64_traffic.md")
+        if n == 4096 and items:  # profiled at 32 matrices per launch; the traffic scales with the matrices
+            t32 = profiled_traffic("lu_update_split_4096x64_traffic.md")
+            traffic = t32 / 32 * items if t32 else None
         rl_update = {"bound": "hbm", "kernel": "zgemm_split_kernel (LU trailing update, split-bf16 MFMA)",
                      "achieved": bu / (tu * 1e-3) / 1e9 if tu else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "traffic": traffic, "avg_launch_ms": tu / cu if cu else None, "launches": cu,
@@ -396,6 +397,10 @@ def main():
         solved, steps, fallbacks, resid = ctx.solver_stats()
     solver = {"mixed_precision_systems": solved, "refinement_steps": steps, "fp64_fallbacks": fallbacks,
               "max_relative_residual": resid}
+    lu_streams, lu_panels = ctx.lu_config()
+    if lu_streams:
+        solver["lu_config"] = {"streams": lu_streams, "panels_per_split_update": lu_panels,
+                               "chosen_by": "explicit --lu-streams" if args.lu_streams else "the library's LU tuner"}
     log(f"bench: {value:.1f} omega-solves/s; per-frequency phase ms:", phases, "solver:", solver)
     plain = None
     if not args.no_plain:

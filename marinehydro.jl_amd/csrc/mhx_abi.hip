@@ -36,6 +36,13 @@ struct mhx_ctx {
     hipStream_t side[kMaxLuStreams - 1] = {};
     hipEvent_t fork = nullptr, join[kMaxLuStreams - 1] = {};
     int lu_streams = 2;
+    // panels per split-bf16 trailing update of a batched FP32 factorization (1 .. 4)
+    int lu_super = mhx::kSplitSuperMax;
+    // autotuning of the batched FP32 factorization's streams and super-panel width (per order,
+    // batch, update kind and pivoting, on the first factorization of that signature); streams set
+    // by mhx_set_lu_streams are kept
+    bool autotune = true, lu_streams_fixed = false;
+    int last_streams = 0, last_super = 0;  // the configuration of the last batched FP32 factorization
     double rho = 1000.0, g = 9.81;
     int timing = 0;
     std::string err;
@@ -477,8 +484,67 @@ int getrf_f32_two_streams(mhx_ctx* ctx, mhx_body* b, int batch, mhx::LaunchTimer
                                   n, b->wperm + (size_t)first * mhx::kPermWork, b->info + first, lt,
                                   split ? split + (size_t)first * mhx::split_work(n) : nullptr, pivot,
                                   reinterpret_cast<cplxf*>(b->work + (size_t)first * b->sW()), 2 * b->sW(),
-                                  inv64 ? inv64 + (size_t)first * b->sW() : nullptr, b->sW());
+                                  inv64 ? inv64 + (size_t)first * b->sW() : nullptr, b->sW(), ctx->lu_super);
     });
+}
+
+// The LU tuner: on the first batched FP32 factorization of a signature (order, batch, update kind,
+// pivoting) it factors the batch's own matrices (FP32 copies made again from the intact FP64 D
+// before every trial) with 1-3 streams and 2-4 panels per split update, timed with events on the
+// library's stream, and keeps the fastest for every later call of that signature.  Results do not
+// depend on the choice beyond the FP32 rounding of the factors (the refinement takes the solutions
+// to the same tolerance).  Returns whether Df still holds the FP32 copy of D.
+bool tune_lu(mhx_ctx* ctx, mhx_body* b, int batch, unsigned short* split, int pivot, cplx* inv64) {
+    const int n = b->n;
+    if (batch < 8 || n < 512) return true;
+    const std::string key = "lu:" + std::to_string(n) + ":" + std::to_string(batch) + ":" + (split ? "s" : "f") + ":" +
+                            std::to_string(pivot) + (ctx->lu_streams_fixed ? ":fixed" + std::to_string(ctx->lu_streams) : "");
+    auto f = ctx->tuned.find(key);
+    if (f != ctx->tuned.end()) {
+        if (!ctx->lu_streams_fixed) ctx->lu_streams = f->second / 16;
+        ctx->lu_super = f->second % 16;
+        return true;
+    }
+    if (!ctx->autotune) return true;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return true;
+    if (hipEventCreate(&e1) != hipSuccess) {
+        hipEventDestroy(e0);
+        return true;
+    }
+    const int keep_streams = ctx->lu_streams;
+    int best = -1;
+    float best_ms = 0.f;
+    for (int streams = 1; streams <= 3; streams++) {
+        if (ctx->lu_streams_fixed && streams != keep_streams) continue;
+        if (streams > 1 && batch / streams < 4) continue;
+        for (int sp = split ? 2 : mhx::kSplitSuperMax; sp <= mhx::kSplitSuperMax; sp++) {
+            ctx->lu_streams = streams;
+            ctx->lu_super = sp;
+            mhx::convert_to_f32(ctx->stream, b->nn() * batch, b->D, b->Df);
+            hipEventRecord(e0, ctx->stream);
+            getrf_f32_two_streams(ctx, b, batch, nullptr, split, pivot, inv64);
+            hipEventRecord(e1, ctx->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) continue;
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            if (best < 0 || ms < best_ms) {
+                best = 16 * streams + sp;
+                best_ms = ms;
+            }
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    if (best < 0) {
+        ctx->lu_streams = keep_streams;
+        ctx->lu_super = mhx::kSplitSuperMax;
+        return false;
+    }
+    ctx->tuned[key] = best;
+    ctx->lu_streams = ctx->lu_streams_fixed ? keep_streams : best / 16;
+    ctx->lu_super = best % 16;
+    return false;
 }
 
 int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf, bool form_phi = true) {
@@ -495,16 +561,21 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf,
     bool inverses_ready = true;  // the verified factorization wrote the solves' diagonal-block inverses
     mhx::launch_zero(st, b->info, sizeof(int) * batch);
     {
+        const bool verified = ctx->pivoting == MHX_PIVOT_VERIFIED;
+        cplx* inv64 = verified ? mhx::getrs_inverses(b->work + 16 * b->nR(), n, R) : nullptr;
+        if (!tune_lu(ctx, b, batch, split ? b->Ps : nullptr, verified ? mhx::kPivotVerified : mhx::kPivotStrict,
+                     inv64))
+            b->df_ready = false;  // the tuner's trials factored Df
+        ctx->last_streams = lu_parts(ctx, n, batch);
+        ctx->last_super = split ? ctx->lu_super : 1;
         Phase ph(ctx, "lu");
         if (!b->df_ready) mhx::convert_to_f32(st, b->nn() * batch, b->D, b->Df);
         b->df_ready = false;
         mhx::LaunchTimer lt;
-        const bool verified = ctx->pivoting == MHX_PIVOT_VERIFIED;
         // with verified pivots the factorization also leaves the FP64 diagonal-block inverses in
         // the solves' tables (work after the split-K partials, for R right-hand sides)
         rc = getrf_f32_two_streams(ctx, b, batch, ctx->timing ? &lt : nullptr, split ? b->Ps : nullptr,
-                                   verified ? mhx::kPivotVerified : mhx::kPivotStrict,
-                                   verified ? mhx::getrs_inverses(b->work + 16 * b->nR(), n, R) : nullptr);
+                                   verified ? mhx::kPivotVerified : mhx::kPivotStrict, inv64);
         if (rc) {
             take_launches(ctx, lt);
             return fail(ctx, MHX_EDIM, "LU: order above 16384 is not supported");
@@ -2331,6 +2402,20 @@ int mhx_set_lu_streams(mhx_ctx* ctx, int streams) {
     if (!ctx) return MHX_EINVAL;
     if (streams < 1 || streams > kMaxLuStreams) return fail(ctx, MHX_EINVAL, "lu streams must be 1 .. 4");
     ctx->lu_streams = streams;
+    ctx->lu_streams_fixed = true;
+    return MHX_OK;
+}
+
+int mhx_set_autotune(mhx_ctx* ctx, int on) {
+    if (!ctx) return MHX_EINVAL;
+    ctx->autotune = on != 0;
+    return MHX_OK;
+}
+
+int mhx_lu_config(mhx_ctx* ctx, int* streams, int* super_panels) {
+    if (!ctx || !streams || !super_panels) return MHX_EINVAL;
+    *streams = ctx->last_streams;
+    *super_panels = ctx->last_super;
     return MHX_OK;
 }
 

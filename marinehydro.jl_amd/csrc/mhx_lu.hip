@@ -896,7 +896,7 @@ void zgemm(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cp
 //   q = 4 .. 7  U12 columns: re hi, re lo, im hi, im lo
 // ---------------------------------------------------------------------------
 constexpr int SPLIT_TILE = 128;
-constexpr int SPLIT_SUPER = 4;  // panels per trailing update on the split path (k = 256)
+constexpr int SPLIT_SUPER = kSplitSuperMax;  // most panels per trailing update on the split path (k <= 256)
 constexpr int SPLIT_KC = 16;    // k chunk of the layout and of the update's LDS stages
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
@@ -1384,7 +1384,7 @@ static int leaf_dispatch(hipStream_t st, MatBatch<C> mb, int n, int c0, int K, i
 template <class C>
 static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
                    int* work_perm, int* info, LaunchTimer* timer, unsigned short* split, int pivot, C* inv,
-                   int64_t inv_stride, cplx* inv64, int64_t inv64_stride) {
+                   int64_t inv_stride, cplx* inv64, int64_t inv64_stride, int super_panels) {
     if (n <= 0) return 0;
     if (n > kMaxOrder) return -1;
     if (pivot == kPivotVerified && !inv) return -2;
@@ -1483,8 +1483,9 @@ static int getrf_t(hipStream_t st, int n, C* A, int lda, int64_t strideA, int ba
     for (int K = 0; K < n;) {
         // panels of this step: S full panels with a trailing matrix left after them (split path)
         int S = 1;
+        const int smax = super_panels < 1 ? 1 : (super_panels > SPLIT_SUPER ? SPLIT_SUPER : super_panels);
         if (use_split)
-            while (S < SPLIT_SUPER && K + (S + 1) * NB < n) S++;
+            while (S < smax && K + (S + 1) * NB < n) S++;
         if (S == 1) {  // one panel
             const int KE = min(n, K + NB);
             if (verified) {
@@ -1545,14 +1546,14 @@ int getrf_batched(hipStream_t st, int n, cplx* A, int lda, int64_t strideA, int 
                   int* work_perm, int* info, LaunchTimer* update_timer, int pivot, cplx* inv, int64_t inv_stride,
                   cplx* inv64, int64_t inv64_stride) {
     return getrf_t(st, n, A, lda, strideA, batch, ipiv, ipiv_stride, work_perm, info, update_timer, nullptr, pivot, inv,
-                   inv_stride, inv64, inv64_stride);
+                   inv_stride, inv64, inv64_stride, 1);
 }
 
 int getrf_batched(hipStream_t st, int n, cplxf* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
                   int* work_perm, int* info, LaunchTimer* update_timer, unsigned short* split, int pivot, cplxf* inv,
-                  int64_t inv_stride, cplx* inv64, int64_t inv64_stride) {
+                  int64_t inv_stride, cplx* inv64, int64_t inv64_stride, int super_panels) {
     return getrf_t(st, n, A, lda, strideA, batch, ipiv, ipiv_stride, work_perm, info, update_timer, split, pivot, inv,
-                   inv_stride, inv64, inv64_stride);
+                   inv_stride, inv64, inv64_stride, super_panels);
 }
 
 // ---------------------------------------------------------------------------

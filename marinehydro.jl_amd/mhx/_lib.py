@@ -40,7 +40,7 @@ EXPORTED = ["mhx_create", "mhx_destroy", "mhx_last_error", "mhx_set_constants", 
             "mhx_lu_solve", "mhx_lu_solve_batched", "mhx_lu_solve_noalloc", "mhx_hydro_coeffs", "mhx_body_create", "mhx_body_destroy", "mhx_body_coeffs",
             "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work", "mhx_set_batch", "mhx_surrogate_train",
             "mhx_surrogate_save", "mhx_surrogate_load", "mhx_surrogate_params", "mhx_surrogate_eval",
-            "mhx_surrogate_loss_grad", "mhx_surrogate_set_activation", "mhx_surrogate_get_activation", "mhx_hydro_coeffs_fs_jvp", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_set_pivoting", "mhx_set_lu_streams",
+            "mhx_surrogate_loss_grad", "mhx_surrogate_set_activation", "mhx_surrogate_get_activation", "mhx_hydro_coeffs_fs_jvp", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_set_pivoting", "mhx_set_lu_streams", "mhx_set_autotune", "mhx_lu_config",
             "mhx_pivot_stats", "mhx_hydro_coeffs_jvp",
             "mhx_hydro_coeffs_vjp", "mhx_sweep_coeffs", "mhx_body_create_lid", "mhx_body_coeffs_fs"]
 
@@ -132,6 +132,8 @@ def lib():
         L.mhx_set_batch.argtypes = [vp, C.c_int64, C.c_double]
         L.mhx_set_lu_mode.argtypes = [vp, C.c_int]
         L.mhx_set_lu_streams.argtypes = [vp, C.c_int]
+        L.mhx_set_autotune.argtypes = [vp, C.c_int]
+        L.mhx_lu_config.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.mhx_solver_stats.argtypes = [vp, _ip, _dp]
         L.mhx_set_pivoting.argtypes = [vp, C.c_int]
         L.mhx_pivot_stats.argtypes = [vp, _ip]
@@ -214,6 +216,16 @@ class Context:
     def set_lu_streams(self, streams):
         """Streams (1 .. 4) a batched FP32 factorization is split over; results do not depend on it."""
         self.check(lib().mhx_set_lu_streams(self.h, int(streams)))
+
+    def set_autotune(self, on):
+        """Autotuning of the batched FP32 factorization's streams and panels per split update."""
+        self.check(lib().mhx_set_autotune(self.h, int(bool(on))))
+
+    def lu_config(self):
+        """(streams, panels per split update) of the last batched FP32 factorization."""
+        a, b = C.c_int(), C.c_int()
+        self.check(lib().mhx_lu_config(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
 
     def solver_stats(self):
         """(systems solved in mixed precision, refinement steps, FP64 fallbacks, max residual)."""
