@@ -325,8 +325,9 @@ def main():
                     help="frequencies per step, spread over the 64 of the workload (fewer only for profiling)")
     ap.add_argument("--activation", default="silu", choices=["silu", "relu"],
                     help="surrogate network: the 4 x 128 SiLU network (the workload's), or its ReLU variant")
-    ap.add_argument("--workload", default="headline", choices=["headline", "forward_speed"],
-                    help="forward_speed: the forward-speed grid on the same body (IndirectBEM, exact or Wu term)")
+    ap.add_argument("--workload", default="headline", choices=["headline", "forward_speed", "multibody"],
+                    help="forward_speed: the forward-speed grid on the same body (IndirectBEM, exact or Wu term); "
+                         "multibody: the seeded uneven 8-body workload as one graph-captured sweep")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -350,6 +351,8 @@ def main():
         ctx.set_lu_streams(args.lu_streams)
     if args.workload == "forward_speed":
         return bench_forward_speed(args, mhx, _lib, ctx, body, names, mesh, omegas)
+    if args.workload == "multibody":
+        return bench_multibody(args, mhx, _lib, ctx, names)
     form = mhx.DirectBEM(args.gf)
     if args.freqs != len(omegas):
         omegas = omegas[np.linspace(0, len(omegas) - 1, max(1, args.freqs)).round().astype(int)]
@@ -513,6 +516,43 @@ def bench_forward_speed(args, mhx, _lib, ctx, body, names, mesh, omegas):
         "roofline": roof,
         "roofline_next_kernel": roof2,
         "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_multibody(args, mhx, _lib, ctx, names):
+    """The seeded uneven workload (8 bodies: spheres, cylinders, boxes of 256-4,096 panels, 16-64
+    log-uniform frequencies in 0.1-3.0 rad/s each, 6 dofs, headings 0, pi/4, pi/2, pi, direct
+    method) as one multi-body sweep captured into a HIP graph: the warmup captures it, each timed
+    step replays it.  A unit is one (body, frequency) omega-solve with its 4 headings."""
+    import math
+    from mhx import meshes
+    betas = [0.0, math.pi / 4, math.pi / 2, math.pi]
+    if args.gf == "Surrogate":
+        mhx.load_surrogate()
+    jobs = [(mhx.FloatingBody(m, names), {"wave_frequencies": w, "radiating_dofs": names, "wave_directions": betas})
+            for m, w, _ in meshes.random_bodies()]
+    units = sum(len(par["wave_frequencies"]) for _, par in jobs)
+    form = mhx.DirectBEM(args.gf)
+    log(f"bench: multibody, {len(jobs)} bodies, panels {[b.mesh.nfaces for b, _ in jobs]}, {units} omega-solves")
+    for _ in range(max(args.warmup, 1)):
+        mhx.hydrodynamic_coefficients_many(jobs, form, graph=True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res = mhx.hydrodynamic_coefficients_many(jobs, form, graph=True)
+    elapsed = time.perf_counter() - t0
+    assert all(np.all(np.isfinite(r["added_mass"])) for r in res)
+    value = units * args.steps / elapsed
+    log(f"bench: multibody {value:.1f} omega-solves/s")
+    out = {
+        "metric": "multi-body omega-solves/s (seeded 8-body workload, graph-captured sweep)", "value": value,
+        "unit": "omega-solves/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": elapsed * 1e3 / args.steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": DTYPE[args.gf], "data": "synthetic",
+        "config": {"workload": f"seeded uneven workload, 8 bodies of {sorted(b.mesh.nfaces for b, _ in jobs)} panels, "
+                               f"{units} (body, frequency) items, 6 dofs, 4 headings, direct method, wave term "
+                               f"{WAVE_DESC[args.gf]}; one step = the whole sweep replayed from its HIP graph",
+                   "bodies": len(jobs), "omega_solves_per_step": units, "parallelism": "single GPU"},
     }
     print(json.dumps(out), flush=True)
 

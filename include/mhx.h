@@ -84,12 +84,14 @@ int mhx_assemble(mhx_ctx* ctx, const mhx_mesh* mesh, double k, int gf, int direc
 
 /* solve(D, S, bc; direct) (src/matrix_assembly.jl:436-449): direct phi = D \ (S bc);
  * indirect sources = D \ bc, phi = S sources.  bc, phi_out, sources_out: n x nrhs complex
- * (sources_out may be NULL; it is left untouched for the direct method). */
+ * (sources_out may be NULL; it is left untouched for the direct method).  More than 16 columns
+ * are solved in chunks of 16 (each with its own factorization). */
 int mhx_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* D, const double* S, const double* bc, int direct,
               double* phi_out, double* sources_out);
 
 /* zgetrf + zgetrs on the device (LinearSolve LUFactorization, src/matrix_assembly.jl:411-416):
- * X = A \ B (trans = 0) or A^H \ B (trans = 2); B overwritten by X.  Returns MHX_ESINGULAR on a zero pivot. */
+ * X = A \ B (trans = 0) or A^H \ B (trans = 2); B overwritten by X.  Returns MHX_ESINGULAR on a zero pivot.
+ * More than 16 right-hand sides are solved in column chunks of 16 (each with its own factorization). */
 int mhx_lu_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* A, int trans, double* B);
 
 /* No counterpart in the reference.  For callers that cannot allocate device memory (code under
@@ -117,7 +119,10 @@ int mhx_lu_solve_batched(mhx_ctx* ctx, int64_t n, int64_t nrhs, int64_t batch, c
 /* hydrodynamic_coefficients(body, (wave_frequencies, radiating_dofs=all, wave_directions), formulation)
  * (src/problems_and_results.jl:464-495) with every dof radiating and influenced:
  *   A_out, B_out  ndof x ndof x nw          (influenced, radiating, omega)
- *   Fd_out, Fk_out ndof x nw x nb complex   (influenced, omega, beta); excitation = Fd + Fk */
+ *   Fd_out, Fk_out ndof x nw x nb complex   (influenced, omega, beta); excitation = Fd + Fk
+ * One solve takes ndof + nb <= 16 right-hand sides; more headings are solved in chunks of
+ * 16 - ndof (each chunk with its own assembly and factorization; ndof <= 15).  MHX_MODE_GRAPH
+ * takes ndof + nb <= 16. */
 int mhx_hydro_coeffs(mhx_ctx* ctx, const mhx_mesh* mesh, int64_t ndof, const double* dofs, int64_t nw,
                      const double* omegas, int64_t nb, const double* betas, int gf, int direct, int mode,
                      double* A_out, double* B_out, double* Fd_out, double* Fk_out);

@@ -1081,7 +1081,15 @@ static int lu_solve_host(mhx_ctx* ctx, int64_t n, int64_t nrhs, int64_t batch, c
 int mhx_lu_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* A, int trans, double* Bh) {
     if (!ctx) return MHX_EINVAL;
     if (n <= 0 || nrhs <= 0) return fail(ctx, MHX_EDIM, "lu_solve: empty system");
-    if (nrhs > 16) return fail(ctx, MHX_EDIM, "lu_solve: at most 16 right-hand sides per call");
+    if (nrhs > 16 && A && Bh) {
+        // the right-hand sides in column chunks of 16 (contiguous in the column-major B), each
+        // with its own factorization of A
+        for (int64_t c0 = 0; c0 < nrhs; c0 += 16) {
+            const int rc = mhx_lu_solve(ctx, n, std::min<int64_t>(16, nrhs - c0), A, trans, Bh + 2 * n * c0);
+            if (rc) return rc;
+        }
+        return MHX_OK;
+    }
     if (n > mhx::kMaxOrder) return fail(ctx, MHX_EDIM, "lu_solve: order above 16384 is not supported");
     if (!A || !Bh) return fail(ctx, MHX_EINVAL, "lu_solve: null buffer");
     if (trans != 0 && trans != 2) return fail(ctx, MHX_EINVAL, "lu_solve: trans must be 0 or 2");
@@ -1191,7 +1199,16 @@ int mhx_solve(mhx_ctx* ctx, int64_t n, int64_t nrhs, const double* D, const doub
               double* phi_out, double* sources_out) {
     if (!ctx) return MHX_EINVAL;
     if (n <= 0 || nrhs <= 0) return fail(ctx, MHX_EDIM, "solve: empty system");
-    if (nrhs > 16) return fail(ctx, MHX_EDIM, "solve: at most 16 right-hand sides per call");
+    if (nrhs > 16 && D && S && bc && phi_out) {
+        // the boundary conditions in column chunks of 16 (contiguous in the column-major bc,
+        // phi and sources), each chunk a full solve
+        for (int64_t c0 = 0; c0 < nrhs; c0 += 16) {
+            const int rc = mhx_solve(ctx, n, std::min<int64_t>(16, nrhs - c0), D, S, bc + 2 * n * c0, direct,
+                                     phi_out + 2 * n * c0, sources_out ? sources_out + 2 * n * c0 : nullptr);
+            if (rc) return rc;
+        }
+        return MHX_OK;
+    }
     if (n > mhx::kMaxOrder) return fail(ctx, MHX_EDIM, "solve: order above 16384 is not supported");
     if (!D || !S || !bc || !phi_out) return fail(ctx, MHX_EINVAL, "solve: null buffer");
     hipSetDevice(ctx->device);
@@ -1257,9 +1274,32 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
         return mhx_sweep_coeffs(ctx, &sb, 1, nb, betas, gf, direct, MHX_SWEEP_GRAPH);
     }
     const int ndof = b->ndof, n = b->n;
+    if (ndof + nb > 16 && ndof < 16) {
+        // more right-hand sides than one solve takes (16): the headings in chunks of 16 - ndof,
+        // each chunk a full pass (its own assembly and factorization; the radiation problems are
+        // solved again and only the first chunk's A and B are kept).  The excitation outputs are
+        // (dof, omega, heading) with the heading slowest, so a chunk's are one contiguous slab.
+        const int64_t per = 16 - ndof;
+        std::vector<double> Ad, Bd;
+        for (int64_t b0 = 0; b0 < nb; b0 += per) {
+            const int64_t nbg = std::min<int64_t>(per, nb - b0);
+            const size_t off = 2 * (size_t)ndof * (size_t)nw * (size_t)b0;
+            double *Ao = A_out, *Bo = B_out;
+            if (b0 > 0 && ndof > 0) {
+                Ad.resize((size_t)ndof * ndof * nw);
+                Bd.resize((size_t)ndof * ndof * nw);
+                Ao = Ad.data();
+                Bo = Bd.data();
+            }
+            const int rc = mhx_body_coeffs(ctx, b, nw, omegas, nbg, betas + b0, gf, direct, mode, Ao, Bo,
+                                           Fd_out ? Fd_out + off : nullptr, Fk_out ? Fk_out + off : nullptr);
+            if (rc) return rc;
+        }
+        return MHX_OK;
+    }
     const int R = ndof + (int)nb;
     if (R <= 0) return fail(ctx, MHX_EDIM, "coeffs: no radiating dof and no heading");
-    if (R > 16) return fail(ctx, MHX_EDIM, "coeffs: ndof + nbeta must be at most 16");
+    if (R > 16) return fail(ctx, MHX_EDIM, "coeffs: at most 15 dofs");
     if (n > mhx::kMaxOrder) return fail(ctx, MHX_EDIM, "coeffs: more than 16384 panels is not supported");
     if ((ndof && (!A_out || !B_out)) || (nb && ndof && (!Fd_out || !Fk_out)))
         return fail(ctx, MHX_EINVAL, "coeffs: null output");

@@ -106,6 +106,49 @@ def test_hydrodynamic_coefficients_plain_body(lib, oracle, gf):
         assert ok, (key, rel, elem)
 
 
+def test_more_headings_than_one_solve_takes(lib, oracle):
+    """Six dofs and twelve headings through mhx_body_coeffs (the glue passes every heading at
+    once): 18 right-hand sides, more than one solve takes, so the library solves the headings in
+    chunks of ten and two; every block equals the oracle's."""
+    L, ctx, _lib = lib
+    import mhx
+    m = mhx.meshes.cylinder(1.0, 1.5, 16, 6, 3)
+    body = mhx.FloatingBody(m, ["Surge", "Sway", "Heave", "Roll", "Pitch", "Yaw"])
+    names = body.dof_names
+    jm = julia_mesh(m)
+    dofs = packed_dofs(body.dofs, names)
+    h = C.c_void_p()
+    ws, bs = [0.8, 1.9], list(np.linspace(0.0, 2 * math.pi, 12, endpoint=False))
+    print("glue replay: 6 dofs, 12 headings", flush=True)
+    ctx.check(L.mhx_body_create(ctx.h, C.byref(mhxmesh(_lib, jm)), len(names), dp(dofs), C.byref(h)))
+    try:
+        got = body_coeffs(L, ctx, h, ws, bs, 0, 1, len(names))
+    finally:
+        L.mhx_body_destroy(h)
+    ref = oracle.hydro_coeffs(oracle.Mesh.from_mhx(m), [body.dofs[k] for k in names], ws, bs, rho=RHO, g=G,
+                              gf=oracle.GF_WU, direct=True)
+    for key in KEYS:
+        rel, elem, ok = close(got[key], ref[key])
+        assert ok, (key, rel, elem)
+
+
+def test_backend_ldiv_many_columns(lib):
+    """_backend_ldiv with 40 right-hand sides (more than one solve takes): mhx_lu_solve solves
+    them in column chunks of 16, 16 and 8; the result equals LAPACK's (A and A^H)."""
+    L, ctx, _lib = lib
+    rng = np.random.default_rng(7)
+    n, k = 300, 40
+    A = 0.5 * np.eye(n) + (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / (2 * math.sqrt(n))
+    Bm = rng.standard_normal((n, k)) + 1j * rng.standard_normal((n, k))
+    print("glue replay: backend_ldiv, 40 columns", flush=True)
+    for trans in (0, 2):
+        Af = np.asfortranarray(A)
+        X = np.asfortranarray(Bm.copy())
+        ctx.check(L.mhx_lu_solve(ctx.h, n, k, dp(Af), trans, dp(X)))
+        ref = np.linalg.solve(A if trans == 0 else A.conj().T, Bm)
+        assert np.linalg.norm(X - ref) / np.linalg.norm(ref) < 1e-12
+
+
 def test_hydrodynamic_coefficients_lid_body(lib, oracle):
     """FloatingBody(HIPMesh(hull), HIPMesh(lid), dofs): hull + lid panels, dofs padded with zero
     rows on the lid, mhx_body_create_lid with the hull's panel count."""
