@@ -437,23 +437,23 @@ __device__ __forceinline__ void surr_load_lds(SurrLds& L, const unsigned short* 
     __syncthreads();
 }
 
-// one hidden layer on the two column tiles: act (B operands, KB input blocks) -> act (8 blocks).
-// Software-pipelined over the four output neuron tiles: the MFMAs of tile t + 1 are issued
-// between the activation instructions of tile t, so the matrix cores and the VALU work together
-// within one wave.
+// Part of the products of neuron tile t of one layer on the two column tiles: input blocks
+// kb0 .. kb1 - 1 of `act` (B operands) into acc0, acc1, the bias put in first when kb0 = 0.
 template <int KB>
-__device__ __forceinline__ void surr_mfma_tile(const bf16x8* W, const float* bias, int t, int lane, int h,
-                                               const bf16x8 act[2][8], f32x16& acc0, f32x16& acc1) {
+__device__ __forceinline__ void surr_mfma_part(const bf16x8* W, const float* bias, int t, int kb0, int kb1, int lane,
+                                               int h, const bf16x8 act[2][8], f32x16& acc0, f32x16& acc1) {
+    if (kb0 == 0) {
 #pragma unroll
-    for (int g = 0; g < 4; g++) {
-        const float4 b = *reinterpret_cast<const float4*>(bias + 32 * t + 8 * g + 4 * h);
-        acc0[4 * g + 0] = acc1[4 * g + 0] = b.x;
-        acc0[4 * g + 1] = acc1[4 * g + 1] = b.y;
-        acc0[4 * g + 2] = acc1[4 * g + 2] = b.z;
-        acc0[4 * g + 3] = acc1[4 * g + 3] = b.w;
+        for (int g = 0; g < 4; g++) {
+            const float4 b = *reinterpret_cast<const float4*>(bias + 32 * t + 8 * g + 4 * h);
+            acc0[4 * g + 0] = acc1[4 * g + 0] = b.x;
+            acc0[4 * g + 1] = acc1[4 * g + 1] = b.y;
+            acc0[4 * g + 2] = acc1[4 * g + 2] = b.z;
+            acc0[4 * g + 3] = acc1[4 * g + 3] = b.w;
+        }
     }
 #pragma unroll
-    for (int kb = 0; kb < KB; kb++) {
+    for (int kb = kb0; kb < kb1; kb++) {
         const bf16x8 a = W[(t * KB + kb) * 64 + lane];
         acc0 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, act[0][kb], acc0, 0, 0, 0);
         acc1 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, act[1][kb], acc1, 0, 0, 0);
@@ -466,7 +466,7 @@ template <int ACT> __device__ __forceinline__ float act_log2(float y) {
     return y * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(y));
 }
 
-// activation of neuron tile t's accumulators into the B operands of the next layer
+// activation of neuron tile t's accumulators into blocks 2t, 2t + 1 of the next layer's B operands
 template <int ACT>
 __device__ __forceinline__ void surr_act_tile(int t, const f32x16& acc0, const f32x16& acc1, bf16x8 nxt[2][8]) {
 #pragma unroll
@@ -483,33 +483,37 @@ __device__ __forceinline__ void surr_act_tile(int t, const f32x16& acc0, const f
     }
 }
 
-template <int KB, int ACT>
-__device__ __forceinline__ void surr_hidden(const SurrLds& L, int layer, int lane, bf16x8 act[2][8]) {
+// Hidden layer `layer` (KB input blocks in X, outputs to Y), software-pipelined over its four
+// neuron tiles: the MFMAs of tile t + 1 run between the activation instructions of tile t, and
+// the last tile's activations run beside the next layer's first tile on its input blocks 0 .. 5
+// (which the activations of tiles 0 .. 2 have finished; blocks 6, 7 follow in the next call).
+// On entry a0, a1 hold tile 0 of this layer with input blocks 0 .. kb_done - 1 already added
+// (kb_done = 0: none, the bias not yet in); on return they hold the next layer's tile 0 (NKB
+// input blocks per tile, blocks 0 .. 5 added) and Y this layer's outputs.
+template <int KB, int NKB, int ACT>
+__device__ __forceinline__ void surr_layer(const SurrLds& L, int layer, int kb_done, int lane, const bf16x8 X[2][8],
+                                           bf16x8 Y[2][8], f32x16& a0, f32x16& a1, f32x16& b0, f32x16& b1) {
     const int h = lane >> 5;
     const bf16x8* W = L.w + sw_layer_off(layer);
     const float* bias = L.bias + 128 * layer;
-    bf16x8 nxt[2][8];
-    f32x16 a0, a1, b0, b1;
-    // scheduling regions {MFMAs of tile t + 1, activations of tile t}; the barriers keep a later
-    // region's LDS loads from being hoisted, which would raise the register pressure
+    const bf16x8* Wn = L.w + sw_layer_off(layer + 1);
+    const float* bn = L.bias + 128 * (layer + 1);
+    // scheduling regions; the barriers keep a later region's LDS loads from being hoisted, which
+    // would raise the register pressure
     __builtin_amdgcn_sched_barrier(0);
-    surr_mfma_tile<KB>(W, bias, 0, lane, h, act, a0, a1);
+    surr_mfma_part<KB>(W, bias, 0, kb_done, KB, lane, h, X, a0, a1);
     __builtin_amdgcn_sched_barrier(0);
-    surr_mfma_tile<KB>(W, bias, 1, lane, h, act, b0, b1);
-    surr_act_tile<ACT>(0, a0, a1, nxt);
+    surr_mfma_part<KB>(W, bias, 1, 0, KB, lane, h, X, b0, b1);
+    surr_act_tile<ACT>(0, a0, a1, Y);
     __builtin_amdgcn_sched_barrier(0);
-    surr_mfma_tile<KB>(W, bias, 2, lane, h, act, a0, a1);
-    surr_act_tile<ACT>(1, b0, b1, nxt);
+    surr_mfma_part<KB>(W, bias, 2, 0, KB, lane, h, X, a0, a1);
+    surr_act_tile<ACT>(1, b0, b1, Y);
     __builtin_amdgcn_sched_barrier(0);
-    surr_mfma_tile<KB>(W, bias, 3, lane, h, act, b0, b1);
-    surr_act_tile<ACT>(2, a0, a1, nxt);
+    surr_mfma_part<KB>(W, bias, 3, 0, KB, lane, h, X, b0, b1);
+    surr_act_tile<ACT>(2, a0, a1, Y);
     __builtin_amdgcn_sched_barrier(0);
-    surr_act_tile<ACT>(3, b0, b1, nxt);
-#pragma unroll
-    for (int kb = 0; kb < 8; kb++) {
-        act[0][kb] = nxt[0][kb];
-        act[1][kb] = nxt[1][kb];
-    }
+    surr_mfma_part<NKB>(Wn, bn, 0, 0, 6, lane, h, Y, a0, a1);
+    surr_act_tile<ACT>(3, b0, b1, Y);
 }
 
 // the network on the lane's two points (u of column tile 0 and 1); out[pt] holds the
@@ -518,7 +522,7 @@ template <int ACT>
 __device__ __forceinline__ void surr_mlp64(const SurrLds& L, int lane, const float u1[2], const float u2[2],
                                            f32x16 out[2]) {
     const int h = lane >> 5;
-    bf16x8 act[2][8];
+    bf16x8 P[2][8], Q[2][8];
 #pragma unroll
     for (int pt = 0; pt < 2; pt++)
 #pragma unroll
@@ -528,30 +532,56 @@ __device__ __forceinline__ void surr_mlp64(const SurrLds& L, int lane, const flo
                 const int f = 16 * h + 4 * kb + e;
                 // v_sin/v_cos take revolutions: sin(2 pi x)
                 const float x = L.fourier[2 * f] * u1[pt] + L.fourier[2 * f + 1] * u2[pt];
-                act[pt][kb][e] = (__bf16)__builtin_amdgcn_sinf(x);
-                act[pt][kb][4 + e] = (__bf16)__builtin_amdgcn_cosf(x);
+                P[pt][kb][e] = (__bf16)__builtin_amdgcn_sinf(x);
+                P[pt][kb][4 + e] = (__bf16)__builtin_amdgcn_cosf(x);
             }
-    surr_hidden<4, ACT>(L, 0, lane, act);
-    surr_hidden<8, ACT>(L, 1, lane, act);
-    surr_hidden<8, ACT>(L, 2, lane, act);
-    surr_hidden<8, ACT>(L, 3, lane, act);
-    const bf16x8* W = L.w + sw_layer_off(4);
+    f32x16 a0, a1, b0, b1;
+    surr_layer<4, 8, ACT>(L, 0, 0, lane, P, Q, a0, a1, b0, b1);  // features P -> Q
+    surr_layer<8, 8, ACT>(L, 1, 6, lane, Q, P, a0, a1, b0, b1);  // Q -> P
+    surr_layer<8, 8, ACT>(L, 2, 6, lane, P, Q, a0, a1, b0, b1);  // P -> Q
+    // layer 3 (Q -> P), its last tile's activations beside the output layer's blocks 0 .. 5; the
+    // output layer's bias: rows 0..3 on lanes 0-31, rows 4, 5 on lanes 32-63 (the rest padding)
+    {
+        const bf16x8* W = L.w + sw_layer_off(3);
+        const float* bias = L.bias + 128 * 3;
+        const bf16x8* Wo = L.w + sw_layer_off(4);
+        __builtin_amdgcn_sched_barrier(0);
+        surr_mfma_part<8>(W, bias, 0, 6, 8, lane, h, Q, a0, a1);
+        __builtin_amdgcn_sched_barrier(0);
+        surr_mfma_part<8>(W, bias, 1, 0, 8, lane, h, Q, b0, b1);
+        surr_act_tile<ACT>(0, a0, a1, P);
+        __builtin_amdgcn_sched_barrier(0);
+        surr_mfma_part<8>(W, bias, 2, 0, 8, lane, h, Q, a0, a1);
+        surr_act_tile<ACT>(1, b0, b1, P);
+        __builtin_amdgcn_sched_barrier(0);
+        surr_mfma_part<8>(W, bias, 3, 0, 8, lane, h, Q, b0, b1);
+        surr_act_tile<ACT>(2, a0, a1, P);
+        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int pt = 0; pt < 2; pt++)
+        for (int pt = 0; pt < 2; pt++)
 #pragma unroll
-        for (int q = 0; q < 16; q++) out[pt][q] = 0.f;
-    if (h == 0) {
+            for (int q = 0; q < 16; q++) out[pt][q] = 0.f;
+        if (h == 0) {
 #pragma unroll
-        for (int q = 0; q < 4; q++) out[0][q] = out[1][q] = L.bias[512 + q];
-    } else {
-        out[0][0] = out[1][0] = L.bias[512 + 4];
-        out[0][1] = out[1][1] = L.bias[512 + 5];
-    }
+            for (int q = 0; q < 4; q++) out[0][q] = out[1][q] = L.bias[512 + q];
+        } else {
+            out[0][0] = out[1][0] = L.bias[512 + 4];
+            out[0][1] = out[1][1] = L.bias[512 + 5];
+        }
 #pragma unroll
-    for (int kb = 0; kb < 8; kb++) {
-        const bf16x8 a = W[kb * 64 + lane];
-        out[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, act[0][kb], out[0], 0, 0, 0);
-        out[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, act[1][kb], out[1], 0, 0, 0);
+        for (int kb = 0; kb < 6; kb++) {
+            const bf16x8 a = Wo[kb * 64 + lane];
+            out[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, P[0][kb], out[0], 0, 0, 0);
+            out[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, P[1][kb], out[1], 0, 0, 0);
+        }
+        surr_act_tile<ACT>(3, b0, b1, P);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int kb = 6; kb < 8; kb++) {
+            const bf16x8 a = Wo[kb * 64 + lane];
+            out[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, P[0][kb], out[0], 0, 0, 0);
+            out[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, P[1][kb], out[1], 0, 0, 0);
+        }
     }
 }
 
