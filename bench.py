@@ -231,16 +231,23 @@ def lu_kind(mode, gf):
 
 
 def measure(ctx, run, ws, n, gf, nlu=None, units=None, kind="fp64", calls=1):
-    """`calls` timed calls of run(ws); returns rate, phase table and roofline of the dominant
-    kernel (per-launch HIP events on the library's stream).  nlu: factorizations per call
-    (default one per frequency); units: what the rate counts per call (default the frequencies);
-    kind: the trailing-update kernel (lu_kind)."""
+    """`calls` timed calls of run(ws) with the library's phase timing off (the rate), then one
+    call with it on: the phase table and the roofline of the dominant kernel (per-launch HIP
+    events on the library's stream) come from that call.  nlu: factorizations per call (default
+    one per frequency); units: what the rate counts per call (default the frequencies); kind: the
+    trailing-update kernel (lu_kind)."""
     ctx.reset_timing()
+    ctx.set_timing(False)
     t0 = time.perf_counter()
     for _ in range(calls):
         A, B, Fd, Fk = run(ws)
     elapsed = time.perf_counter() - t0
     assert np.all(np.isfinite(A)) and np.all(np.isfinite(Fd))
+    rate = (len(ws) if units is None else units) * calls / elapsed
+    # the instrumented call
+    ctx.reset_timing()
+    ctx.set_timing(True)
+    run(ws)
     phases = {p: ctx.phase_time(p) for p in ("wave_table", "wave", "lu", "lu_update", "solve", "forces", "rankine",
                                              "forward_speed")}
     counts, flops = ctx.exact_work()
@@ -248,8 +255,7 @@ def measure(ctx, run, ws, n, gf, nlu=None, units=None, kind="fp64", calls=1):
     tu, cu = phases["lu_update"]
     tl, cl = phases["lu"]
     nw1 = len(ws) if nlu is None else nlu       # factorizations per call
-    nw = nw1 * calls
-    units = (len(ws) if units is None else units) * calls
+    nw = nw1
     fu = lu_update_flops(n, two_level=kind == "split") * nw
     if kind == "split":
         # the split-bf16 update moves the trailing matrix once per step: HBM-bound
@@ -303,7 +309,7 @@ def measure(ctx, run, ws, n, gf, nlu=None, units=None, kind="fp64", calls=1):
         roof, other = rl_update, rl_table
     lu_tflops = 8.0 / 3.0 * n ** 3 * nw / (tl * 1e-3) / 1e12 if tl else None
     per_w = {k: v[0] / nw for k, v in phases.items()}
-    return units / elapsed, elapsed, per_w, roof, lu_tflops, (A, B, Fd, Fk), other
+    return rate, elapsed, per_w, roof, lu_tflops, (A, B, Fd, Fk), other
 
 
 def main():

@@ -36,8 +36,8 @@ struct mhx_ctx {
     hipStream_t side[kMaxLuStreams - 1] = {};
     hipEvent_t fork = nullptr, join[kMaxLuStreams - 1] = {};
     int lu_streams = 2;
-    // panels per split-bf16 trailing update of a batched FP32 factorization (1 .. 4)
-    int lu_super = mhx::kSplitSuperMax;
+    // panels per split-bf16 trailing update of a batched FP32 factorization (1 .. 8)
+    int lu_super = mhx::kSplitSuperDefault;
     // autotuning of the batched FP32 factorization's streams and super-panel width (per order,
     // batch, update kind and pivoting, on the first factorization of that signature); streams set
     // by mhx_set_lu_streams are kept
@@ -490,7 +490,7 @@ int getrf_f32_two_streams(mhx_ctx* ctx, mhx_body* b, int batch, mhx::LaunchTimer
 
 // The LU tuner: on the first batched FP32 factorization of a signature (order, batch, update kind,
 // pivoting) it factors the batch's own matrices (FP32 copies made again from the intact FP64 D
-// before every trial) with 1-3 streams and 2-4 panels per split update, timed with events on the
+// before every trial) with 1-3 streams and 2-8 panels per split update, timed with events on the
 // library's stream, and keeps the fastest for every later call of that signature.  Results do not
 // depend on the choice beyond the FP32 rounding of the factors (the refinement takes the solutions
 // to the same tolerance).  Returns whether Df still holds the FP32 copy of D.
@@ -538,7 +538,7 @@ bool tune_lu(mhx_ctx* ctx, mhx_body* b, int batch, unsigned short* split, int pi
     hipEventDestroy(e1);
     if (best < 0) {
         ctx->lu_streams = keep_streams;
-        ctx->lu_super = mhx::kSplitSuperMax;
+        ctx->lu_super = mhx::kSplitSuperDefault;
         return false;
     }
     ctx->tuned[key] = best;
@@ -2417,6 +2417,12 @@ int mhx_phase_time(mhx_ctx* ctx, const char* phase, double* total_ms, int64_t* c
     auto it = ctx->phases.find(phase);
     if (total_ms) *total_ms = it == ctx->phases.end() ? 0.0 : it->second.ms;
     if (count) *count = it == ctx->phases.end() ? 0 : it->second.count;
+    return MHX_OK;
+}
+
+int mhx_set_timing(mhx_ctx* ctx, int on) {
+    if (!ctx) return MHX_EINVAL;
+    ctx->timing = on ? 1 : 0;
     return MHX_OK;
 }
 

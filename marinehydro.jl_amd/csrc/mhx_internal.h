@@ -98,7 +98,8 @@ void zgemm(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cp
 constexpr int kPivotStrict = 0;
 constexpr int kPivotVerified = 1;
 constexpr int kMaxOrder = 16384;  // largest order of the factorization (leaf register tiles)
-constexpr int kSplitSuperMax = 4;  // most panels per split-bf16 trailing update (split_work is sized for it)
+constexpr int kSplitSuperMax = 8;  // most panels per split-bf16 trailing update (split_work is sized for it)
+constexpr int kSplitSuperDefault = 4;  // panels per split-bf16 update when the LU tuner has not chosen
 // inv64 (verified pass, optional): where the FP64 copies of the diagonal blocks' triangular
 // inverses go, in the layout of getrs_batched's inverse tables (getrs_inverses(work, n, R) of the
 // getrs work of each matrix, stride inv64_stride), so that the solves need not form them again.
@@ -123,7 +124,7 @@ int getrs_batched(hipStream_t st, int trans, int n, int R, const cplx* LU, int l
 int getrf_batched(hipStream_t st, int n, cplxf* A, int lda, int64_t strideA, int batch, int* ipiv, int64_t ipiv_stride,
                   int* work_perm, int* info, LaunchTimer* update_timer = nullptr, unsigned short* split = nullptr,
                   int pivot = kPivotStrict, cplxf* inv = nullptr, int64_t inv_stride = 0, cplx* inv64 = nullptr,
-                  int64_t inv64_stride = 0, int super_panels = kSplitSuperMax);
+                  int64_t inv64_stride = 0, int super_panels = kSplitSuperDefault);
 // super_panels: 64-column panels factored left-looking before each split-bf16 trailing update
 // (1 .. kSplitSuperMax; the update's k is 64 times that), split path only
 int64_t split_work(int n);

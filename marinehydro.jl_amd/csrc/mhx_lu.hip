@@ -896,7 +896,7 @@ void zgemm(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cp
 //   q = 4 .. 7  U12 columns: re hi, re lo, im hi, im lo
 // ---------------------------------------------------------------------------
 constexpr int SPLIT_TILE = 128;
-constexpr int SPLIT_SUPER = kSplitSuperMax;  // most panels per trailing update on the split path (k <= 256)
+constexpr int SPLIT_SUPER = kSplitSuperMax;  // most panels per trailing update on the split path (k <= 512)
 constexpr int SPLIT_KC = 16;    // k chunk of the layout and of the update's LDS stages
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x16_t __attribute__((ext_vector_type(16)));
