@@ -1661,22 +1661,35 @@ __global__ void __launch_bounds__(TRS_B) invert_diag_kernel(const C* __restrict_
 // Block blk's x is op(Inv_blk) times its y after the updates by every block solved before it.
 // ---------------------------------------------------------------------------
 
-// element (a, q) of the dependency tile (a: index contiguous in memory), e = 0..15 per thread
-template <class C>
-__device__ __forceinline__ void trs_load_tile(const C* __restrict__ A, int lda, int conj, int b0, int nbk, int c0,
-                                              int ncb, int tid, C (&reg)[TRS_B * TRS_B / 256]) {
+// element (a, q) of the dependency tile (a: index contiguous in memory), e = 0..15 per thread;
+// FULL: the tile lies inside the matrix (no bounds checks)
+template <class C, bool CONJ, bool FULL>
+__device__ __forceinline__ void trs_load_tile(const C* __restrict__ A, int lda, int b0, int nbk, int c0, int ncb,
+                                              int tid, C (&reg)[TRS_B * TRS_B / 256]) {
 #pragma unroll
     for (int e = 0; e < TRS_B * TRS_B / 256; e++) {
         const int idx = tid + 256 * e;
         const int a = idx & (TRS_B - 1), q = idx / TRS_B;
-        C w = tmk<C>(0, 0);
-        if (!conj) {
-            if (a < nbk && q < ncb) w = A[(b0 + a) + (int64_t)lda * (c0 + q)];
+        if (!CONJ) {
+            reg[e] = (FULL || (a < nbk && q < ncb)) ? A[(b0 + a) + (int64_t)lda * (c0 + q)] : tmk<C>(0, 0);
         } else {
-            if (q < nbk && a < ncb) w = A[(c0 + a) + (int64_t)lda * (b0 + q)];
+            reg[e] = (FULL || (q < nbk && a < ncb)) ? A[(c0 + a) + (int64_t)lda * (b0 + q)] : tmk<C>(0, 0);
         }
-        reg[e] = w;
     }
+}
+
+// acc -= t x and acc += t x, four FMAs each
+__device__ __forceinline__ void cfms(cplx& acc, cplx t, cplx x) {
+    acc.x = fma(-t.x, x.x, acc.x);
+    acc.x = fma(t.y, x.y, acc.x);
+    acc.y = fma(-t.x, x.y, acc.y);
+    acc.y = fma(-t.y, x.x, acc.y);
+}
+__device__ __forceinline__ void cfma(cplx& acc, cplx t, cplx x) {
+    acc.x = fma(t.x, x.x, acc.x);
+    acc.x = fma(-t.y, x.y, acc.x);
+    acc.y = fma(t.x, x.y, acc.y);
+    acc.y = fma(t.y, x.x, acc.y);
 }
 
 // One block step of the blocked triangular solve X = op(T)^{-1} Y, all items of the batch
@@ -1687,10 +1700,11 @@ __device__ __forceinline__ void trs_load_tile(const C* __restrict__ A, int lda, 
 //              forms x_rb = op(Inv_rb) y_rb.
 // The launches' order carries the dependencies (x_blk and y_rb were written by earlier launches),
 // so there are no waits inside a launch.  Each element of y is reduced in the order of the block
-// steps, tile by tile, in FP64.
-template <int RB, class C>
+// steps, tile by tile, in FP64.  CONJ: op(T) = T^H (else T); FULL: every block is TRS_B rows (the
+// order is a multiple of TRS_B), so no element needs a bounds check.
+template <int RB, class C, bool CONJ, bool FULL>
 __global__ void __launch_bounds__(256) trs_step_kernel(const C* __restrict__ Ab, int lda, int64_t sA, int n, int R,
-                                                       int lower, int conj, int blk, int first,
+                                                       int lower, int blk, int first,
                                                        const cplx* __restrict__ Invb, int64_t sInv,
                                                        cplx* __restrict__ Yb, int64_t sY, cplx* __restrict__ Xb,
                                                        int64_t sX) {
@@ -1705,12 +1719,12 @@ __global__ void __launch_bounds__(256) trs_step_kernel(const C* __restrict__ Ab,
     cplx(*invs)[TRS_B] = reinterpret_cast<cplx(*)[TRS_B]>(smem);
     const int tid = threadIdx.x, i = tid & (TRS_B - 1), rq = tid / TRS_B;
     const int item = blockIdx.z, j = blockIdx.x;
-    const int b0 = blk * TRS_B, nbk = min(TRS_B, n - b0);
+    const int b0 = blk * TRS_B, nbk = FULL ? TRS_B : min(TRS_B, n - b0);
     const C* A = Ab + sA * item;
     cplx* Y = Yb + sY * item;
     cplx* X = Xb + sX * item;
     const int rb = first ? blk : (lower ? blk + 1 + j : blk - 1 - j);
-    const int r0 = rb * TRS_B, nrb = min(TRS_B, n - r0);
+    const int r0 = rb * TRS_B, nrb = FULL ? TRS_B : min(TRS_B, n - r0);
     const bool diag = first || j == 0;  // this workgroup forms x of block rb
     // the inverse of block rb, loaded at once (its latency hides behind the update)
     cplx ireg[TE];
@@ -1726,19 +1740,19 @@ __global__ void __launch_bounds__(256) trs_step_kernel(const C* __restrict__ Ab,
             const int idx = tid + 256 * e;
             const int a = idx & (TRS_B - 1), q = idx / TRS_B;  // element (a, q) of Inv_rb
             const cplx v = ireg[e];
-            if (!conj) invs[q][a] = v;
+            if (!CONJ) invs[q][a] = v;
             else invs[a][q] = cmk(v.x, -v.y);
         }
         __syncthreads();
-        if (i < nrb) {
+        if (FULL || i < nrb) {
             cplx x[CPT];
 #pragma unroll
             for (int cc = 0; cc < CPT; cc++) x[cc] = cmk(0.0, 0.0);
 #pragma unroll 4
-            for (int s = 0; s < nrb; s++) {
+            for (int s = 0; s < (FULL ? TRS_B : nrb); s++) {
                 const cplx v = invs[s][i];
 #pragma unroll
-                for (int cc = 0; cc < CPT; cc++) x[cc] = cadd(x[cc], cmul(v, xs[s][rq * CPT + cc]));
+                for (int cc = 0; cc < CPT; cc++) cfma(x[cc], v, xs[s][rq * CPT + cc]);
             }
 #pragma unroll
             for (int cc = 0; cc < CPT; cc++) {
@@ -1751,30 +1765,30 @@ __global__ void __launch_bounds__(256) trs_step_kernel(const C* __restrict__ Ab,
 #pragma unroll
         for (int cc = 0; cc < CPT; cc++) {
             const int col = rq * CPT + cc;
-            xs[i][col] = (i < nrb && col < R) ? Y[(r0 + i) + (int64_t)n * col] : cmk(0.0, 0.0);
+            xs[i][col] = ((FULL || i < nrb) && col < R) ? Y[(r0 + i) + (int64_t)n * col] : cmk(0.0, 0.0);
         }
         solve_diag();
         return;
     }
     C reg[TE];
-    trs_load_tile(A, lda, conj, r0, nrb, b0, nbk, tid, reg);
+    trs_load_tile<C, CONJ, FULL>(A, lda, r0, nrb, b0, nbk, tid, reg);
 #pragma unroll
     for (int cc = 0; cc < CPT; cc++) {
         const int col = rq * CPT + cc;
-        xs[i][col] = (i < nbk && col < R) ? X[(b0 + i) + (int64_t)n * col] : cmk(0.0, 0.0);
+        xs[i][col] = ((FULL || i < nbk) && col < R) ? X[(b0 + i) + (int64_t)n * col] : cmk(0.0, 0.0);
     }
     cplx acc[CPT];
 #pragma unroll
     for (int cc = 0; cc < CPT; cc++) {
         const int col = rq * CPT + cc;
-        acc[cc] = (i < nrb && col < R) ? Y[(r0 + i) + (int64_t)n * col] : cmk(0.0, 0.0);
+        acc[cc] = ((FULL || i < nrb) && col < R) ? Y[(r0 + i) + (int64_t)n * col] : cmk(0.0, 0.0);
     }
 #pragma unroll
     for (int e = 0; e < TE; e++) {
         const int idx = tid + 256 * e;
         const int a = idx & (TRS_B - 1), q = idx / TRS_B;
         C w = reg[e];
-        if (!conj) {
+        if (!CONJ) {
             tileT[q][a] = w;
         } else {
             w.y = -w.y;
@@ -1786,14 +1800,14 @@ __global__ void __launch_bounds__(256) trs_step_kernel(const C* __restrict__ Ab,
     for (int s = 0; s < TRS_B; s++) {
         const cplx t = to_d(tileT[s][i]);
 #pragma unroll
-        for (int cc = 0; cc < CPT; cc++) acc[cc] = csub(acc[cc], cmul(t, xs[s][rq * CPT + cc]));
+        for (int cc = 0; cc < CPT; cc++) cfms(acc[cc], t, xs[s][rq * CPT + cc]);
     }
     if (diag) {
         __syncthreads();  // every thread is done with tileT and with xs of block blk
 #pragma unroll
         for (int cc = 0; cc < CPT; cc++) xs[i][rq * CPT + cc] = acc[cc];
         solve_diag();
-    } else if (i < nrb) {
+    } else if (FULL || i < nrb) {
 #pragma unroll
         for (int cc = 0; cc < CPT; cc++) {
             const int col = rq * CPT + cc;
@@ -1802,17 +1816,30 @@ __global__ void __launch_bounds__(256) trs_step_kernel(const C* __restrict__ Ab,
     }
 }
 
-template <int RB, class C>
-static void trs_pass_t(hipStream_t st, const C* A, int lda, int64_t sA, int n, int R, int lower, int conj,
+template <int RB, class C, bool CONJ, bool FULL>
+static void trs_pass_k(hipStream_t st, const C* A, int lda, int64_t sA, int n, int R, int lower,
                        const cplx* Inv, int64_t sInv, cplx* Y, int64_t sY, cplx* X, int64_t sX, int batch) {
     const int nblk = (n + TRS_B - 1) / TRS_B;
     const int blk0 = lower ? 0 : nblk - 1;
-    trs_step_kernel<RB, C><<<dim3(1, 1, batch), 256, 0, st>>>(A, lda, sA, n, R, lower, conj, blk0, 1, Inv, sInv, Y, sY,
-                                                               X, sX);
+    trs_step_kernel<RB, C, CONJ, FULL><<<dim3(1, 1, batch), 256, 0, st>>>(A, lda, sA, n, R, lower, blk0, 1, Inv, sInv,
+                                                                          Y, sY, X, sX);
     for (int step = 0; step + 1 < nblk; step++) {
         const int blk = lower ? step : nblk - 1 - step;
-        trs_step_kernel<RB, C><<<dim3(nblk - 1 - step, 1, batch), 256, 0, st>>>(A, lda, sA, n, R, lower, conj, blk, 0,
-                                                                                 Inv, sInv, Y, sY, X, sX);
+        trs_step_kernel<RB, C, CONJ, FULL><<<dim3(nblk - 1 - step, 1, batch), 256, 0, st>>>(
+            A, lda, sA, n, R, lower, blk, 0, Inv, sInv, Y, sY, X, sX);
+    }
+}
+
+template <int RB, class C>
+static void trs_pass_t(hipStream_t st, const C* A, int lda, int64_t sA, int n, int R, int lower, int conj,
+                       const cplx* Inv, int64_t sInv, cplx* Y, int64_t sY, cplx* X, int64_t sX, int batch) {
+    const bool full = n % TRS_B == 0;
+    if (conj) {
+        if (full) trs_pass_k<RB, C, true, true>(st, A, lda, sA, n, R, lower, Inv, sInv, Y, sY, X, sX, batch);
+        else trs_pass_k<RB, C, true, false>(st, A, lda, sA, n, R, lower, Inv, sInv, Y, sY, X, sX, batch);
+    } else {
+        if (full) trs_pass_k<RB, C, false, true>(st, A, lda, sA, n, R, lower, Inv, sInv, Y, sY, X, sX, batch);
+        else trs_pass_k<RB, C, false, false>(st, A, lda, sA, n, R, lower, Inv, sInv, Y, sY, X, sX, batch);
     }
 }
 
