@@ -760,7 +760,7 @@ __global__ void __launch_bounds__(256) matmul_nr_kernel(int n, int R, const cplx
             for (int jj = 0; jj < cnt; jj++) {
                 const cplx a = S[i + (int64_t)n * (j0 + jj)];
 #pragma unroll
-                for (int c = 0; c < RB; c++) acc[c] = cadd(acc[c], cmul(a, xs[jj][c]));
+                for (int c = 0; c < RB; c++) cfma(acc[c], a, xs[jj][c]);
             }
     }
     if (i < n)

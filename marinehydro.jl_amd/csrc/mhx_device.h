@@ -26,6 +26,19 @@ __device__ __forceinline__ cplx cmk(double r, double i) { return make_double2(r,
 __device__ __forceinline__ cplx cadd(cplx a, cplx b) { return cmk(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ cplx csub(cplx a, cplx b) { return cmk(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ cplx cmul(cplx a, cplx b) { return cmk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// acc += t x and acc -= t x, four FMAs each
+__device__ __forceinline__ void cfma(cplx& acc, cplx t, cplx x) {
+    acc.x = fma(t.x, x.x, acc.x);
+    acc.x = fma(-t.y, x.y, acc.x);
+    acc.y = fma(t.x, x.y, acc.y);
+    acc.y = fma(t.y, x.x, acc.y);
+}
+__device__ __forceinline__ void cfms(cplx& acc, cplx t, cplx x) {
+    acc.x = fma(-t.x, x.x, acc.x);
+    acc.x = fma(t.y, x.y, acc.x);
+    acc.y = fma(-t.x, x.y, acc.y);
+    acc.y = fma(-t.y, x.x, acc.y);
+}
 __device__ __forceinline__ cplx cscale(double s, cplx a) { return cmk(s * a.x, s * a.y); }
 __device__ __forceinline__ double cabs_(cplx a) { return hypot(a.x, a.y); }
 // Smith's division
@@ -824,23 +837,29 @@ __device__ __forceinline__ bool hquad_pair(double r, double z, double rtol, doub
 #define MHX_FLOPS_PER_SERIES 8.0
 #define MHX_FLOPS_PER_CF 45.0
 
+// the quantities of an entry that depend only on the (symmetric) pair: 2 / |(hh, vv)| and k^2 / hh
+// (0 for hh <= 1e-6)
+struct PairQ {
+    double inv, kkh;
+};
+__device__ __forceinline__ PairQ pair_q(double k, double hh, double vv) {
+    PairQ q;
+    q.inv = 2.0 / hypot(hh, vv);
+    q.kkh = fabs(hh) > 1e-6 ? k * k / hh : 0.0;
+    return q;
+}
 // One entry (row panel with centre xr / normal nr, column panel g) of S and D (or K) from the
 // wave term G, dG/dr and the Rankine parts sr, dr: with_reduced_coordinates_derivative
 // (abstract_greens_function.jl:140-185) and the one-point integrals
 // (exact_Guevel_Delhommeau.jl:61-75), scaled by -1/(4 pi); `diag` adds the identity term 1/2.
-__device__ __forceinline__ void combine_entry(double k, double hh, double vv, const double* xr, const double* nr,
+__device__ __forceinline__ void combine_entry(double k, PairQ pq, const double* xr, const double* nr,
                                               const PanelGeom& g, int direct, bool diag, cplx G, cplx Gh, double sr,
                                               double dr, cplx& S, cplx& D) {
     const double kScaleD = -1.0 / (4 * MHX_PI);
-    const double inv = 2.0 / hypot(hh, vv);
-    const cplx Gv = cmk(G.x + inv, G.y);
+    const cplx Gv = cmk(G.x + pq.inv, G.y);
     double dx1 = direct ? g.c[0] - xr[0] : xr[0] - g.c[0];
     double dx2 = direct ? g.c[1] - xr[1] : xr[1] - g.c[1];
-    double dr1 = 0.0, dr2 = 0.0;
-    if (fabs(hh) > 1e-6) {
-        dr1 = k * k / hh * dx1;
-        dr2 = k * k / hh * dx2;
-    }
+    const double dr1 = pq.kkh * dx1, dr2 = pq.kkh * dx2;
     const double n0 = direct ? g.nrm[0] : nr[0], n1 = direct ? g.nrm[1] : nr[1], n2 = direct ? g.nrm[2] : nr[2];
     const double a2 = g.area;
     cplx g0 = cscale(k, cscale(dr1, Gh)), g1 = cscale(k, cscale(dr2, Gh)), g2 = cscale(k, cscale(k, Gv));
@@ -849,4 +868,9 @@ __device__ __forceinline__ void combine_entry(double k, double hh, double vv, co
     if (diag) dr += g.pad[0];
     S = cmk(sr + kScaleD * s.x, kScaleD * s.y);
     D = cmk(dr + kScaleD * d.x, kScaleD * d.y);
+}
+__device__ __forceinline__ void combine_entry(double k, double hh, double vv, const double* xr, const double* nr,
+                                              const PanelGeom& g, int direct, bool diag, cplx G, cplx Gh, double sr,
+                                              double dr, cplx& S, cplx& D) {
+    combine_entry(k, pair_q(k, hh, vv), xr, nr, g, direct, diag, G, Gh, sr, dr, S, D);
 }

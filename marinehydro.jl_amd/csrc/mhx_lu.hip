@@ -1678,20 +1678,6 @@ __device__ __forceinline__ void trs_load_tile(const C* __restrict__ A, int lda, 
     }
 }
 
-// acc -= t x and acc += t x, four FMAs each
-__device__ __forceinline__ void cfms(cplx& acc, cplx t, cplx x) {
-    acc.x = fma(-t.x, x.x, acc.x);
-    acc.x = fma(t.y, x.y, acc.x);
-    acc.y = fma(-t.x, x.y, acc.y);
-    acc.y = fma(-t.y, x.x, acc.y);
-}
-__device__ __forceinline__ void cfma(cplx& acc, cplx t, cplx x) {
-    acc.x = fma(t.x, x.x, acc.x);
-    acc.x = fma(-t.y, x.y, acc.x);
-    acc.y = fma(t.x, x.y, acc.y);
-    acc.y = fma(t.y, x.x, acc.y);
-}
-
 // One block step of the blocked triangular solve X = op(T)^{-1} Y, all items of the batch
 // (blockIdx.z), in sweep order (lower: block rows upwards from 0, upper: downwards from the last).
 //   first = 1: x_blk = op(Inv_blk) y_blk (one workgroup per item; the first block of the sweep);

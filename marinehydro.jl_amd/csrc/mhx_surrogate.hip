@@ -668,8 +668,8 @@ __device__ __forceinline__ void store_stream(cplxf* p, double re, double im) {
 // store one entry (row r, column c) of the item's output from the pair's wave term: the (G, dG/dr)
 // table, or (fused) the S and D entries with their Rankine parts, and the FP32 copy of D for the
 // mixed-precision solver
-__device__ __forceinline__ void surr_store_entry(const SurrDst& o, int n, int r, int c, double k, double hh,
-                                                 double vv, const PanelGeom* __restrict__ geom, cplx G, cplx Gr) {
+__device__ __forceinline__ void surr_store_entry(const SurrDst& o, int n, int r, int c, double k, PairQ pq,
+                                                 const PanelGeom* __restrict__ geom, cplx G, cplx Gr) {
     const int64_t rc = r + (int64_t)n * c;
     if (!o.S) {
         store_stream(o.WG + rc, G);
@@ -678,7 +678,7 @@ __device__ __forceinline__ void surr_store_entry(const SurrDst& o, int n, int r,
     }
     const PanelGeom& gr = geom[r];
     cplx Sv, Dv;
-    combine_entry(k, hh, vv, gr.c, gr.nrm, geom[c], o.direct, o.include_identity && r == c, G, Gr, o.SR[rc],
+    combine_entry(k, pq, gr.c, gr.nrm, geom[c], o.direct, o.include_identity && r == c, G, Gr, o.SR[rc],
                   o.DR[rc], Sv, Dv);
     store_stream(o.S + rc, Sv);
     store_stream(o.D + rc, Dv);
@@ -688,8 +688,9 @@ __device__ __forceinline__ void surr_store_entry(const SurrDst& o, int n, int r,
 // both entries (i, j) and (j, i) of a pair from one thread (the exact fallback)
 __device__ __forceinline__ void surr_store_pair(const SurrDst& o, int n, int i, int j, double k, double hh, double vv,
                                                 const PanelGeom* __restrict__ geom, cplx G, cplx Gr) {
-    surr_store_entry(o, n, i, j, k, hh, vv, geom, G, Gr);
-    if (i != j) surr_store_entry(o, n, j, i, k, hh, vv, geom, G, Gr);
+    const PairQ pq = pair_q(k, hh, vv);
+    surr_store_entry(o, n, i, j, k, pq, geom, G, Gr);
+    if (i != j) surr_store_entry(o, n, j, i, k, pq, geom, G, Gr);
 }
 
 // One work unit of a wave: the 8 x 8 block of panel pairs (i in 8 bi.., j in 8 bj.., bi <= bj;
@@ -753,6 +754,7 @@ __device__ __forceinline__ void surr_unit(const SurrLds& L, int lane, int n, int
     }
     if (!__ballot(net)) return;
     cplx G = cmk(0.0, 0.0), Gr = cmk(0.0, 0.0);
+    PairQ pq = {0.0, 0.0};
     if (net) {
         const WuGh w = wu_term_gh_f32sf(hh_me > 0 ? hh_me : 1e-300, vv_me);
         const double is = 1.0 / SURR_SCALE;
@@ -762,16 +764,17 @@ __device__ __forceinline__ void surr_unit(const SurrLds& L, int lane, int n, int
         G = cmk(w.G.x + is * o[0], w.G.y + ii * o[1]);
         Gr = cmk(w.Gh.x + is * o[2], w.Gh.y + ii * o[3]);
         // entry (i, j): lanes 8m .. 8m + 7 write rows 8 bi .. 8 bi + 7 of column 8 bj + m
-        surr_store_entry(dst, n, i, j, k, hh_me, vv_me, geom, G, Gr);
+        pq = pair_q(k, hh_me, vv_me);
+        surr_store_entry(dst, n, i, j, k, pq, geom, G, Gr);
     }
     // entry (j, i) from the transposed lane t, whose pair is (8 bi + lane / 8, 8 bj + lane % 8): lanes
     // 8m .. 8m + 7 again write eight consecutive rows of one column (hh and vv are symmetric in the pair)
     const int t = ((lane & 7) << 3) | (lane >> 3);
-    const double hh_t = __shfl(hh_me, t), vv_t = __shfl(vv_me, t);
+    const PairQ pq_t = {__shfl(pq.inv, t), __shfl(pq.kkh, t)};
     const cplx G_t = cmk(__shfl(G.x, t), __shfl(G.y, t)), Gr_t = cmk(__shfl(Gr.x, t), __shfl(Gr.y, t));
     const bool net_t = __shfl((int)net, t) != 0;
     const int mi = 8 * bi + (lane >> 3), mj = 8 * bj + (lane & 7);
-    if (net_t && mi != mj) surr_store_entry(dst, n, mj, mi, k, hh_t, vv_t, geom, G_t, Gr_t);
+    if (net_t && mi != mj) surr_store_entry(dst, n, mj, mi, k, pq_t, geom, G_t, Gr_t);
 }
 
 // Persistent pair-table kernel over all items of the batch: each wave takes units in a strided
