@@ -187,7 +187,7 @@ def profiled_traffic(name):
 
 def lu_update_steps(n, nb=64, two_level=False, super_=4):
     """(K, KE) of the LU's timed trailing updates: one per 64-column panel, or (split updates) one
-    per group of up to 4 panels that leaves a trailing matrix."""
+    per group of up to super_ panels that leaves a trailing matrix."""
     steps, K = [], 0
     while K < n:
         S = 1
@@ -205,17 +205,17 @@ def lu_update_steps(n, nb=64, two_level=False, super_=4):
     return steps
 
 
-def lu_update_flops(n, nb=64, two_level=False):
+def lu_update_flops(n, nb=64, two_level=False, super_=4):
     """Algorithmic flops of the trailing updates of one blocked LU (complex multiply-add = 8 flops)."""
-    return sum(8.0 * (n - KE) ** 2 * (KE - K) for K, KE in lu_update_steps(n, nb, two_level))
+    return sum(8.0 * (n - KE) ** 2 * (KE - K) for K, KE in lu_update_steps(n, nb, two_level, super_))
 
 
-def lu_update_bytes(n, nb=64, tile=128):
+def lu_update_bytes(n, nb=64, tile=128, super_=4):
     """Algorithmic HBM bytes of the split-bf16 trailing updates of one blocked LU (multi-panel
     steps): the trailing matrix read and written once per update (FP32 complex, 16 B per
     element) and the update's split panels (8 bf16 arrays of mpad x ks) read once."""
     b = 0.0
-    for K, KE in lu_update_steps(n, nb, True):
+    for K, KE in lu_update_steps(n, nb, True, super_):
         m = n - KE
         mpad = (m + tile - 1) // tile * tile
         ks = (KE - K + nb - 1) // nb * nb
@@ -256,12 +256,14 @@ def measure(ctx, run, ws, n, gf, nlu=None, units=None, kind="fp64", calls=1):
     tl, cl = phases["lu"]
     nw1 = len(ws) if nlu is None else nlu       # factorizations per call
     nw = nw1
-    fu = lu_update_flops(n, two_level=kind == "split") * nw
+    _, panels = ctx.lu_config()                  # panels per split update the library used
+    sp = panels if panels else 4
+    fu = lu_update_flops(n, two_level=kind == "split", super_=sp) * nw
     if kind == "split":
         # the split-bf16 update moves the trailing matrix once per step: HBM-bound
-        bu = lu_update_bytes(n) * nw
+        bu = lu_update_bytes(n, super_=sp) * nw
         traffic = None
-        items = nw / cu * len(lu_update_steps(n, two_level=True)) if cu else 0
+        items = nw / cu * len(lu_update_steps(n, two_level=True, super_=sp)) if cu else 0
         if n == 4096 and items:  # profiled at 32 matrices per launch; the traffic scales with the matrices
             t32 = profiled_traffic("lu_update_split_4096x64_traffic.md")
             traffic = t32 / 32 * items if t32 else None
