@@ -964,18 +964,21 @@ __device__ __forceinline__ bf16x8_t bf16x8_neg(bf16x8_t v) {
 }
 
 // C[0:m, 0:m] -= L21 U12 from the split arrays.  Workgroup tile 128 x 128 of C, four waves of
-// 64 x 64 (2 x 2 accumulator tiles of 32 x 32, real and imaginary); k in chunks of 16 staged
-// through LDS (rows padded to 48 B: the eight lanes of a 16-byte read phase hit distinct banks),
-// the next chunk prefetched into registers during the current chunk's MFMAs: each thread loads
-// one 16-byte unit per array and panel, a chunk of 128 rows of one array being 4 KB contiguous.
-// Two workgroups per CU (operands loaded per column tile, C read in the epilogue): one
-// workgroup's C read-modify-write overlaps the other's MFMAs.
-constexpr int SPLIT_LDS_ROW = SPLIT_KC / 8 + 1;  // bf16x8 units per row (2 used + 1 pad)
+// 64 x 64 (2 x 2 accumulator tiles of 32 x 32, real and imaginary); k in chunks of 16.  The
+// operands of a chunk go from HBM straight into LDS (global_load_lds_dwordx4: a wave's 64 lanes
+// write 1 KB contiguous, a chunk of 128 rows of one array being 4 KB contiguous in the
+// chunk-major layout), into two LDS stages: chunk c + 1 is in flight while chunk c's MFMAs run,
+// one barrier per chunk.  A chunk's operands are read into registers before the next chunk's
+// loads are issued.  Rows are 32 B (two 16-byte halves): the eight lanes of a 16-byte read phase
+// read eight consecutive rows and hit distinct banks.  Two workgroups per CU (64 KB of LDS each;
+// C read in the epilogue): one workgroup's C read-modify-write overlaps the other's MFMAs.
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
 
 __global__ void __launch_bounds__(256, 2) zgemm_split_kernel(int m, int mpad, int ks, const unsigned short* __restrict__ Sb,
                                                              int64_t sS, cplxf* __restrict__ Cb, int ldc, int64_t sC) {
-    __shared__ bf16x8_t La[4][SPLIT_TILE][SPLIT_LDS_ROW];  // L21 rows   (MFMA B operand, C rows)
-    __shared__ bf16x8_t Lb[4][SPLIT_TILE][SPLIT_LDS_ROW];  // U12 columns (MFMA A operand, C columns)
+    // [stage][L21 rows, U12 columns][array q][row * 2 + half]; L21 rows are the MFMA B operand (C
+    // rows), U12 columns the A operand (C columns)
+    __shared__ bf16x8_t Ls[2 * 2 * 4 * 2 * SPLIT_TILE];
     // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin by linear id, so the
     // id is remapped to give each XCD a contiguous run of tiles (row tile fastest: the tiles an XCD
     // holds at once share their column panel and walk down the same column-major C columns)
@@ -987,7 +990,8 @@ __global__ void __launch_bounds__(256, 2) zgemm_split_kernel(int m, int mpad, in
     const int tx = L % gridDim.x, ty = (L / gridDim.x) % gridDim.y, tz = L / (gridDim.x * gridDim.y);
     const unsigned short* S = Sb + sS * tz;
     cplxf* Cm = Cb + sC * tz;
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, h = lane >> 5, l32 = lane & 31;
+    const int tid = threadIdx.x, lane = tid & 63, h = lane >> 5, l32 = lane & 31;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int row0 = tx * SPLIT_TILE, col0 = ty * SPLIT_TILE;
     const int wr = (wave & 1) * 64, wc = (wave >> 1) * 64;
     f32x16_t acc_re[2][2], acc_im[2][2];  // [column tile][row tile]
@@ -1000,60 +1004,61 @@ __global__ void __launch_bounds__(256, 2) zgemm_split_kernel(int m, int mpad, in
     const int nkc = ks / SPLIT_KC;
     const bf16x8_t* Sa = reinterpret_cast<const bf16x8_t*>(S);                              // arrays 0..3
     const bf16x8_t* Sbv = reinterpret_cast<const bf16x8_t*>(S + (int64_t)4 * mpad * ks);  // arrays 4..7
-    // one chunk: 4 arrays x 128 rows x 2 units for each panel; thread tid takes unit tid of each
-    // array's 256 (row tid / 2, half tid % 2)
-    bf16x8_t pa[4], pb[4];
-    auto load_chunk = [&](int k0) {
-        const int kc = k0 / SPLIT_KC;
+    constexpr int kArr = 2 * SPLIT_TILE;  // 16-byte units of one array's chunk
+    // chunk kc into stage st: thread tid loads unit tid of each array's chunk, wave w's lanes
+    // landing at units 64 w .. 64 w + 63 of the stage's array
+    auto issue = [&](int kc, int st) {
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            pa[q] = Sa[(((int64_t)q * nkc + kc) * mpad + row0) * 2 + tid];
-            pb[q] = Sbv[(((int64_t)q * nkc + kc) * mpad + col0) * 2 + tid];
+            const bf16x8_t* ga = Sa + (((int64_t)q * nkc + kc) * mpad + row0) * 2 + tid;
+            const bf16x8_t* gb = Sbv + (((int64_t)q * nkc + kc) * mpad + col0) * 2 + tid;
+            __builtin_amdgcn_global_load_lds((const void*)ga, (lds_void_ptr)(Ls + ((st * 2 + 0) * 4 + q) * kArr + 64 * wave),
+                                             16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const void*)gb, (lds_void_ptr)(Ls + ((st * 2 + 1) * 4 + q) * kArr + 64 * wave),
+                                             16, 0, 0);
         }
     };
-    load_chunk(0);
-    for (int k0 = 0; k0 < ks; k0 += SPLIT_KC) {
-#pragma unroll
-        for (int q = 0; q < 4; q++) {
-            La[q][tid >> 1][tid & 1] = pa[q];
-            Lb[q][tid >> 1][tid & 1] = pb[q];
-        }
+    issue(0, 0);
+    for (int kc = 0; kc < nkc; kc++) {
+        const int st = kc & 1;
+        // this wave's loads of chunk kc have landed; after the barrier every wave's have, and every
+        // wave has read stage st ^ 1 (chunk kc - 1) into registers
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        if (k0 + SPLIT_KC < ks) load_chunk(k0 + SPLIT_KC);
-        bf16x8_t arh[2], arl[2], aih[2], ail[2];
+        const bf16x8_t* La = Ls + (st * 2 + 0) * 4 * kArr;
+        const bf16x8_t* Lb = Ls + (st * 2 + 1) * 4 * kArr;
+        bf16x8_t ar[4][2], br[4][2];  // [array: re hi, re lo, im hi, im lo][32-row tile]
 #pragma unroll
-        for (int t = 0; t < 2; t++) {
-            const int r = wr + 32 * t + l32;
-            arh[t] = La[0][r][h];
-            arl[t] = La[1][r][h];
-            aih[t] = La[2][r][h];
-            ail[t] = La[3][r][h];
-        }
+        for (int q = 0; q < 4; q++)
+#pragma unroll
+            for (int t = 0; t < 2; t++) {
+                ar[q][t] = La[q * kArr + (wr + 32 * t + l32) * 2 + h];
+                br[q][t] = Lb[q * kArr + (wc + 32 * t + l32) * 2 + h];
+            }
+        if (kc + 1 < nkc) issue(kc + 1, st ^ 1);
 #pragma unroll
         for (int a = 0; a < 2; a++) {
-            const int c = wc + 32 * a + l32;
-            const bf16x8_t brh = Lb[0][c][h], brl = Lb[1][c][h], bih = Lb[2][c][h], bil = Lb[3][c][h];
+            const bf16x8_t brh = br[0][a], brl = br[1][a], bih = br[2][a], bil = br[3][a];
             const bf16x8_t nbih = bf16x8_neg(bih), nbil = bf16x8_neg(bil);
 #pragma unroll
             for (int b = 0; b < 2; b++) {
                 f32x16_t re = acc_re[a][b], im = acc_im[a][b];
-                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh, arh[b], re, 0, 0, 0);
-                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh, arl[b], re, 0, 0, 0);
-                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brl, arh[b], re, 0, 0, 0);
-                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbih, aih[b], re, 0, 0, 0);
-                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbih, ail[b], re, 0, 0, 0);
-                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbil, aih[b], re, 0, 0, 0);
-                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bih, arh[b], im, 0, 0, 0);
-                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bih, arl[b], im, 0, 0, 0);
-                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bil, arh[b], im, 0, 0, 0);
-                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh, aih[b], im, 0, 0, 0);
-                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh, ail[b], im, 0, 0, 0);
-                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brl, aih[b], im, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh, ar[0][b], re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh, ar[1][b], re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brl, ar[0][b], re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbih, ar[2][b], re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbih, ar[3][b], re, 0, 0, 0);
+                re = __builtin_amdgcn_mfma_f32_32x32x16_bf16(nbil, ar[2][b], re, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bih, ar[0][b], im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bih, ar[1][b], im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(bil, ar[0][b], im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh, ar[2][b], im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brh, ar[3][b], im, 0, 0, 0);
+                im = __builtin_amdgcn_mfma_f32_32x32x16_bf16(brl, ar[2][b], im, 0, 0, 0);
                 acc_re[a][b] = re;
                 acc_im[a][b] = im;
             }
         }
-        __syncthreads();
     }
     // epilogue: accumulator register v of lane (l32 + 32 h) is C row l32 (N index), column
     // 8 (v / 4) + 4 h + v % 4 (M index) of its 32 x 32 tile; each access covers 32 consecutive

@@ -122,3 +122,28 @@ def test_tuned_factorization_matches_fp64(mhx, mixed):
     for key in ("added_mass", "radiation_damping", "excitation_force"):
         err = np.linalg.norm(got[key] - ref[key]) / np.linalg.norm(ref[key])
         assert err <= 1e-9, (key, err)
+
+
+def test_phase_timing_switch(mhx):
+    """mhx_set_timing: a call made with the phase events off records no phase, one made with them
+    on records its LU and solve phases, and the coefficients are the same either way."""
+    print("phase timing switch", flush=True)
+    from mhx import meshes
+    ctx = mhx.context()
+    body = mhx.FloatingBody(meshes.sphere(1.0, 8, 16, immersed=True), ["Heave"])
+    par = dict(wave_frequencies=[0.8, 1.4], radiating_dofs=["Heave"], wave_directions=[0.0])
+    try:
+        ctx.set_timing(False)
+        ctx.reset_timing()
+        off = mhx.hydrodynamic_coefficients(body, par, mhx.DirectBEM("Wu"), schedule="batched")
+        assert ctx.phase_time("lu")[1] == 0 and ctx.phase_time("solve")[1] == 0
+        ctx.set_timing(True)
+        ctx.reset_timing()
+        on = mhx.hydrodynamic_coefficients(body, par, mhx.DirectBEM("Wu"), schedule="batched")
+        assert ctx.phase_time("lu")[1] > 0 and ctx.phase_time("solve")[1] > 0
+        assert ctx.phase_time("lu")[0] > 0.0
+        for key in ("added_mass", "radiation_damping", "excitation_force"):
+            assert np.array_equal(off[key], on[key]), key
+    finally:
+        ctx.set_timing(False)
+        ctx.reset_timing()
