@@ -190,7 +190,7 @@ int mhx_solver_stats(mhx_ctx* ctx, int64_t counts[3], double* max_residual);
 int mhx_set_lu_streams(mhx_ctx* ctx, int streams);
 /* Autotuning of the batched FP32 factorization (on by default): on the first batched factorization
  * of a signature (order, batch size, update kind, pivoting) the library factors that batch with
- * 1-3 streams and 2-8 panels per split-bf16 update, keeps the fastest and uses it for every later
+ * 1-3 streams and 2-8 panels per split-bf16 update (each timed twice), keeps the fastest and uses it for every later
  * call of the signature (streams set with mhx_set_lu_streams are kept).  mhx_lu_config: the
  * streams and panels per update of the last batched FP32 factorization (0, 0 before any).  No
  * counterpart in the reference; tuning controls of this build's batched solver. */
