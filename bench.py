@@ -46,8 +46,9 @@ BF16_PEAK_TFLOPS = 2500.0
 
 WAVE_DESC = {"Surrogate": "learned surrogate (Wu + BF16 MFMA residual network, exact FP64 fallback)",
              "ExactGuevelDelhommeau": "exact FP64 Guevel-Delhommeau", "Wu": "FP64 Liang-Wu-Noblesse fit"}
-DTYPE = {"Surrogate": "f64 (wave-term residual network: bf16 MFMA, f32 accumulate; dense solve: FP32 LU with "
-                      "split-bf16 MFMA trailing updates, FP64 residuals and refinement to 1e-12)",
+DTYPE = {"Surrogate": "f64 (wave-term residual network: bf16 MFMA, f32 accumulate; S and D stored in FP32; "
+                      "dense solve: FP32 LU with split-bf16 MFMA trailing updates, FP64-summed residuals and "
+                      "refinement to 1e-12)",
          "ExactGuevelDelhommeau": "f64", "Wu": "f64"}
 
 
@@ -173,14 +174,15 @@ def surrogate_flops_per_pair():
     return 2.0 * (64 * 128 + 3 * 128 * 128 + 128 * 6)
 
 
-def profiled_traffic(name):
-    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated on a copy
-    kernel) for the default workload, read from the committed summary profiles/<name>."""
+def profiled_value(name, key="traffic_bytes_per_launch"):
+    """A figure of the committed summary profiles/<name> (a line "key: value"): by default the HBM
+    bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE (calibrated on a copy kernel)
+    for the default workload."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
+        if line.startswith(key + ":"):
             return float(line.split(":")[1])
     return None
 
@@ -264,8 +266,10 @@ def measure(ctx, run, ws, n, gf, nlu=None, units=None, kind="fp64", calls=1):
         bu = lu_update_bytes(n, super_=sp) * nw
         traffic = None
         items = nw / cu * len(lu_update_steps(n, two_level=True, super_=sp)) if cu else 0
-        if n == 4096 and items:  # profiled at 32 matrices per launch; the traffic scales with the matrices
-            t32 = profiled_traffic("lu_update_split_4096x64_traffic.md")
+        prof = "lu_update_split_4096x64_traffic.md"
+        if n == 4096 and items and profiled_value(prof, "panels_per_update") == sp:
+            # profiled at 32 matrices per launch and the same update width; the traffic scales with the matrices
+            t32 = profiled_value(prof)
             traffic = t32 / 32 * items if t32 else None
         rl_update = {"bound": "hbm", "kernel": "zgemm_split_kernel (LU trailing update, split-bf16 MFMA)",
                      "achieved": bu / (tu * 1e-3) / 1e9 if tu else None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -290,7 +294,7 @@ def measure(ctx, run, ws, n, gf, nlu=None, units=None, kind="fp64", calls=1):
         ach = pairs * surrogate_flops_per_pair() / (tw * 1e-3) / 1e12
         traffic = None
         if n == 4096 and cw and abs(nw / cw - 64) < 1e-9:   # the profiled configuration: 64 frequencies per launch
-            traffic = profiled_traffic("surrogate_table_4096x64_traffic.md")
+            traffic = profiled_value("surrogate_table_4096x64_traffic.md")
         rl_table = {"bound": "mfma", "kernel": "surr_table_kernel (+ exact fallback)", "achieved": ach,
                     "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / BF16_PEAK_TFLOPS, "traffic": traffic,
                     "avg_launch_ms": tw / cw, "launches": cw, "frequencies_per_launch": nw / cw,

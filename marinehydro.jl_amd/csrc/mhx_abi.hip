@@ -98,6 +98,9 @@ struct mhx_body {
     cplx *Rr = nullptr, *Tr = nullptr;
     double* norms = nullptr;
     bool df_ready = false;  // Df already holds the FP32 copy of D (written by the fused surrogate assembly)
+    // S and D of the assembled batch are stored as FP32 complex, item q's matrix at the start of its
+    // FP64 slot ((cplxf*)(S + q nn)): the surrogate path with the mixed-precision solver
+    bool sd32 = false;
     // generation of the per-frequency buffers above: a new value whenever any of them is freed or
     // reallocated, so that a captured sweep graph holding their addresses is not replayed
     uint64_t gen = 0;
@@ -280,8 +283,9 @@ int ensure_pair_list(mhx_ctx* ctx, mhx_body* b, int batch) {
 // the surrogate writes its (G, dG/dr) table into b->WG, b->WR (allocated by the caller) and S, D
 // are combined from it, as for the exact term, so that later kernels can read dG/dr.
 int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int include_identity,
-                  bool keep_table = false) {
+                  bool keep_table = false, bool sd32_ok = false) {
     b->df_ready = false;
+    b->sd32 = false;
     if (gf == MHX_GF_WU) {
         Phase ph(ctx, "wave");
         mhx::launch_combine_wu(ctx->stream, b->n, b->geom, b->SR, b->DR[direct], b->d_ks, batch, direct,
@@ -308,6 +312,10 @@ int assemble_into(mhx_ctx* ctx, mhx_body* b, int batch, int gf, int direct, int 
             int rc = dalloc(ctx, &b->Df, (size_t)b->nn() * b->cap_batch);
             if (rc) return rc;
             out.Df = b->Df;
+            // the caller solves with the mixed-precision solver: S and D in FP32 (their entries carry
+            // the network's BF16-level error; the residual products then read half the bytes)
+            out.sd32 = sd32_ok ? 1 : 0;
+            b->sd32 = sd32_ok;
         }
         mhx::launch_zero(ctx->stream, b->status, 4 * sizeof(unsigned long long));
         {
@@ -375,6 +383,42 @@ int read_pivot_flags(mhx_ctx* ctx, const int* wperm, int batch, std::vector<int>
 // FP64 LU of D (in place) with the context's pivoting.  Verified diagonal pivots: D is copied to
 // D0 first, and every item whose factorization breaks LAPACK's rule is copied back and factored
 // again with row interchanges (so the pivots are LAPACK's either way).
+// Y = M X for a batch's S or D (M = b->S or b->D), in the precision the assembly stored them in
+void matmul_sd(hipStream_t st, const mhx_body* b, int R, const cplx* M, const cplx* X, cplx* Y, int batch) {
+    if (b->sd32)
+        mhx::launch_matmul_nr(st, b->n, R, reinterpret_cast<const cplxf*>(M), 2 * b->nn(), X, b->nR(), Y, b->nR(),
+                              b->work, b->sW(), batch);
+    else
+        mhx::launch_matmul_nr(st, b->n, R, M, b->nn(), X, b->nR(), Y, b->nR(), b->work, b->sW(), batch);
+}
+
+// the FP32 working copy Df of items [q0, q0 + batch) of D: converted from the FP64 D, or copied from
+// its FP32 storage
+int df_from_d(mhx_ctx* ctx, mhx_body* b, int64_t q0, int batch) {
+    const int64_t nn = b->nn();
+    if (!b->sd32) {
+        mhx::convert_to_f32(ctx->stream, nn * batch, b->D + q0 * nn, b->Df + q0 * nn);
+        return MHX_OK;
+    }
+    HCHK(hipMemcpy2DAsync(b->Df + q0 * nn, sizeof(cplxf) * nn, reinterpret_cast<const cplxf*>(b->D + q0 * nn),
+                          sizeof(cplx) * nn, sizeof(cplxf) * nn, batch, hipMemcpyDeviceToDevice, ctx->stream));
+    return MHX_OK;
+}
+
+// S and D of a batch stored in FP32 widened back to FP64 in place (through D0), for the FP64 path
+int widen_sd(mhx_ctx* ctx, mhx_body* b, int batch) {
+    if (!b->sd32) return MHX_OK;
+    int rc;
+    if ((rc = dalloc(ctx, &b->D0, (size_t)b->nn() * b->cap_batch))) return rc;
+    for (cplx* m : {b->S, b->D}) {
+        if (!m) continue;
+        mhx::widen_to_f64(ctx->stream, b->nn(), batch, reinterpret_cast<const cplxf*>(m), 2 * b->nn(), b->D0, b->nn());
+        HCHK(hipMemcpyAsync(m, b->D0, sizeof(cplx) * b->nn() * batch, hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    b->sd32 = false;
+    return MHX_OK;
+}
+
 int getrf_fp64(mhx_ctx* ctx, mhx_body* b, int batch, mhx::LaunchTimer* lt) {
     const int n = b->n;
     hipStream_t st = ctx->stream;
@@ -410,6 +454,10 @@ int getrf_fp64(mhx_ctx* ctx, mhx_body* b, int batch, mhx::LaunchTimer* lt) {
 // trans = 2 then solves with D^H.
 int solve_fp64(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, bool form_phi = true, int trans = 0) {
     const int n = b->n;
+    if (b->sd32) {
+        const int rc = widen_sd(ctx, b, batch);
+        if (rc) return rc;
+    }
     mhx::launch_zero(ctx->stream, b->info, sizeof(int) * batch);
     {
         Phase ph(ctx, "lu");
@@ -523,7 +571,7 @@ bool tune_lu(mhx_ctx* ctx, mhx_body* b, int batch, unsigned short* split, int pi
             for (int sp = split ? 2 : mhx::kSplitSuperMax; sp <= mhx::kSplitSuperMax; sp++) {
                 ctx->lu_streams = streams;
                 ctx->lu_super = sp;
-                mhx::convert_to_f32(ctx->stream, b->nn() * batch, b->D, b->Df);
+                if (df_from_d(ctx, b, 0, batch)) continue;
                 hipEventRecord(e0, ctx->stream);
                 getrf_f32_two_streams(ctx, b, batch, nullptr, split, pivot, inv64);
                 hipEventRecord(e1, ctx->stream);
@@ -577,7 +625,7 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf,
         ctx->last_streams = lu_parts(ctx, n, batch);
         ctx->last_super = split ? ctx->lu_super : 1;
         Phase ph(ctx, "lu");
-        if (!b->df_ready) mhx::convert_to_f32(st, b->nn() * batch, b->D, b->Df);
+        if (!b->df_ready && (rc = df_from_d(ctx, b, 0, batch))) return rc;
         b->df_ready = false;
         mhx::LaunchTimer lt;
         // with verified pivots the factorization also leaves the FP64 diagonal-block inverses in
@@ -600,7 +648,7 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf,
                 }
                 ctx->lu_interchanged++;
                 inverses_ready = false;  // the refactored item's blocks differ from the verified pass
-                mhx::convert_to_f32(st, b->nn(), b->D + q * b->nn(), b->Df + q * b->nn());
+                if ((rc = df_from_d(ctx, b, q, 1))) return rc;
                 mhx::getrf_batched(st, n, b->Df + q * b->nn(), n, b->nn(), 1, b->ipiv + (size_t)q * n, n,
                                    b->wperm + (size_t)q * mhx::kPermWork, b->info + q, ctx->timing ? &lt : nullptr,
                                    split ? b->Ps + (size_t)q * mhx::split_work(n) : nullptr);
@@ -619,14 +667,13 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf,
         Phase ph(ctx, "solve");
         cplx* W2 = b->work + 16 * b->nR();
         cplx* X = direct ? b->Y : b->B;  // the system's unknowns, solved in place
-        if (direct)
-            mhx::launch_matmul_nr(st, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), batch);
+        if (direct) matmul_sd(st, b, R, b->S, b->B, b->Y, batch);
         HCHK(hipMemcpyAsync(b->Rr, X, sizeof(cplx) * nRb, hipMemcpyDeviceToDevice, st));
         mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, X, b->nR(), W2, b->sW(), batch,
                            inverses_ready ? mhx::kGetrsInverses : mhx::kGetrsFresh);
         std::vector<double> nr(2 * batch);
         for (int it = 0; it <= refine_max; it++) {
-            mhx::launch_matmul_nr(st, n, R, b->D, b->nn(), X, b->nR(), b->Tr, b->nR(), b->work, b->sW(), batch);
+            matmul_sd(st, b, R, b->D, X, b->Tr, batch);
             mhx::launch_zero(st, b->norms, sizeof(double) * 2 * batch);
             mhx::launch_residual(st, (int64_t)n * R, b->Rr, b->nR(), b->Tr, b->nR(), b->norms, batch);
             HCHK(hipMemcpyAsync(nr.data(), b->norms, sizeof(double) * 2 * batch, hipMemcpyDeviceToHost, st));
@@ -647,8 +694,7 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf,
             mhx::launch_add(st, (int64_t)n * R, b->Tr, b->nR(), X, b->nR(), batch);
             ctx->refine_steps += batch;
         }
-        if (converged && !direct && form_phi)
-            mhx::launch_matmul_nr(st, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), batch);
+        if (converged && !direct && form_phi) matmul_sd(st, b, R, b->S, b->B, b->Y, batch);
         HCHK(hipGetLastError());
     }
     if (converged) {
@@ -866,25 +912,23 @@ void sweep_enqueue(mhx_ctx* ctx, SweepState& s) {
                 cplx* W2 = b->work + 16 * b->nR();
                 cplx* X = s.direct ? b->Y : b->B;
                 const int64_t nRb = b->nR() * bt;
-                if (s.direct)
-                    mhx::launch_matmul_nr(st, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), bt);
+                if (s.direct) matmul_sd(st, b, R, b->S, b->B, b->Y, bt);
                 mhx::launch_copy(st, X, b->Rr, sizeof(cplx) * nRb);
                 // (a flagged item's results are replaced by the pivoted redo after the sweep; its
                 // verified factors and their inverses are consistent with each other meanwhile)
                 mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, X, b->nR(), W2, b->sW(), bt,
                                    s.verified ? mhx::kGetrsInverses : mhx::kGetrsFresh);
-                mhx::launch_matmul_nr(st, n, R, b->D, b->nn(), X, b->nR(), b->Tr, b->nR(), b->work, b->sW(), bt);
+                matmul_sd(st, b, R, b->D, X, b->Tr, bt);
                 mhx::launch_zero(st, b->norms, sizeof(double) * 2 * bt);
                 mhx::launch_residual(st, (int64_t)n * R, b->Rr, b->nR(), b->Tr, b->nR(), b->norms, bt);
                 mhx::getrs_batched(st, 0, n, R, b->Df, n, b->nn(), b->ipiv, n, b->perm, n, b->Tr, b->nR(), W2, b->sW(),
                                    bt, true);
                 mhx::launch_add(st, (int64_t)n * R, b->Tr, b->nR(), X, b->nR(), bt);
                 double* rn = s.rnorms + 2 * (s.flag_off[ib] + w0);
-                mhx::launch_matmul_nr(st, n, R, b->D, b->nn(), X, b->nR(), b->Tr, b->nR(), b->work, b->sW(), bt);
+                matmul_sd(st, b, R, b->D, X, b->Tr, bt);
                 mhx::launch_zero(st, rn, sizeof(double) * 2 * bt);
                 mhx::launch_residual(st, (int64_t)n * R, b->Rr, b->nR(), b->Tr, b->nR(), rn, bt);
-                if (!s.direct)
-                    mhx::launch_matmul_nr(st, n, R, b->S, b->nn(), b->B, b->nR(), b->Y, b->nR(), b->work, b->sW(), bt);
+                if (!s.direct) matmul_sd(st, b, R, b->S, b->B, b->Y, bt);
                 if (b->ndof > 0)
                     mhx::launch_forces(st, n, b->geom, b->dofs, b->ndof, b->Y, b->nR(), R, ctx->rho, b->d_ws, b->d_ks, bt,
                                        g, b->betas, (int)s.nb, s.F_all[ib] + w0 * b->ndof * R,
@@ -1332,7 +1376,7 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
         if ((rc = set_freqs(ctx, b, omegas + w0, bt))) break;
         mhx::launch_bc(st, n, b->geom, b->dofs, ndof, b->betas, (int)nb, b->d_ws, b->d_ks, bt, ctx->g, b->B, b->nR());
         if (mode != MHX_MODE_PER_PROBLEM) {
-            if ((rc = assemble_into(ctx, b, bt, gf, direct, 1)) || (rc = check_status(ctx, b, gf))) break;
+            if ((rc = assemble_into(ctx, b, bt, gf, direct, 1, false, true)) || (rc = check_status(ctx, b, gf))) break;
             if ((rc = factor_and_solve(ctx, b, R, direct, bt, gf))) break;
         } else {
             // reference semantics: one assembly (Rankine part included, as the reference recomputes
@@ -1343,7 +1387,7 @@ int mhx_body_coeffs(mhx_ctx* ctx, mhx_body* b, int64_t nw, const double* omegas,
             for (int r = 0; r < R && !rc; r++) {
                 b->have_rankine[direct] = false;
                 if ((rc = ensure_rankine(ctx, b, direct))) break;
-                if ((rc = assemble_into(ctx, b, 1, gf, direct, 1)) || (rc = check_status(ctx, b, gf))) break;
+                if ((rc = assemble_into(ctx, b, 1, gf, direct, 1, false, true)) || (rc = check_status(ctx, b, gf))) break;
                 HCHK(hipMemcpyAsync(b->B, hB.data() + (size_t)n * r, sizeof(cplx) * n, hipMemcpyHostToDevice, st));
                 if ((rc = factor_and_solve(ctx, b, 1, direct, 1, gf))) break;
                 HCHK(hipMemcpyAsync(hphi.data() + (size_t)n * r, b->Y, sizeof(cplx) * n, hipMemcpyDeviceToHost, st));
@@ -2173,6 +2217,7 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
                 it.WG = table ? b->WG + q * b->nn() : nullptr;
                 it.WR = table ? b->WR + q * b->nn() : nullptr;
                 it.Df = surr && mixed ? b->Df + q * b->nn() : nullptr;
+                it.sd32 = surr && mixed ? 1 : 0;  // the mixed solver reads S and D in FP32
                 it.k = b->d_ks + q;
                 it.n = b->n;
                 it.unit0 = u;
@@ -2195,6 +2240,7 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
     std::vector<int64_t> nws(nbodies);
     std::vector<uint64_t> gens(nbodies);
     for (int64_t ib = 0; ib < nbodies; ib++) {
+        in[ib].body->sd32 = surr && mixed;  // the queue's pair kernel stores S and D in FP32 (items' sd32)
         bodies[ib] = in[ib].body;
         nws[ib] = in[ib].nw;
         gens[ib] = in[ib].body->gen;

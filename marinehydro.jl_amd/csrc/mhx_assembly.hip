@@ -733,11 +733,15 @@ void launch_bc(hipStream_t st, int n, const PanelGeom* geom, const double* dofs,
 #define NSPLIT 16
 #define XCH 64
 
-template <int RB>
-__global__ void __launch_bounds__(256) matmul_nr_kernel(int n, int R, const cplx* __restrict__ Sb, int64_t sS,
+// matrix element as FP64 complex (the matrix in FP64, or in FP32 on the surrogate path)
+__device__ __forceinline__ cplx mat_d(cplx v) { return v; }
+__device__ __forceinline__ cplx mat_d(cplxf v) { return cmk((double)v.x, (double)v.y); }
+
+template <int RB, class M>
+__global__ void __launch_bounds__(256) matmul_nr_kernel(int n, int R, const M* __restrict__ Sb, int64_t sS,
                                                         const cplx* __restrict__ Xb, int64_t sX,
                                                         cplx* __restrict__ partb, int64_t sP) {
-    const cplx* S = Sb + sS * blockIdx.z;
+    const M* S = Sb + sS * blockIdx.z;
     const cplx* X = Xb + sX * blockIdx.z;
     cplx* part = partb + sP * blockIdx.z;
     __shared__ cplx xs[XCH][RB];
@@ -758,7 +762,7 @@ __global__ void __launch_bounds__(256) matmul_nr_kernel(int n, int R, const cplx
         __syncthreads();
         if (i < n)
             for (int jj = 0; jj < cnt; jj++) {
-                const cplx a = S[i + (int64_t)n * (j0 + jj)];
+                const cplx a = mat_d(S[i + (int64_t)n * (j0 + jj)]);
 #pragma unroll
                 for (int c = 0; c < RB; c++) cfma(acc[c], a, xs[jj][c]);
             }
@@ -780,17 +784,28 @@ __global__ void reduce_parts_kernel(int n, int R, const cplx* __restrict__ partb
     Y[t] = s;
 }
 
-void launch_matmul_nr(hipStream_t st, int n, int R, const cplx* S, int64_t sS, const cplx* X, int64_t sX, cplx* Y,
-                      int64_t sY, cplx* work, int64_t sW, int batch) {
+template <class M>
+static void matmul_nr_t(hipStream_t st, int n, int R, const M* S, int64_t sS, const cplx* X, int64_t sX, cplx* Y,
+                        int64_t sY, cplx* work, int64_t sW, int batch) {
     dim3 grid((n + 255) / 256, NSPLIT, batch);
     if (R <= 4)
-        matmul_nr_kernel<4><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW);
+        matmul_nr_kernel<4, M><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW);
     else if (R <= 8)
-        matmul_nr_kernel<8><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW);
+        matmul_nr_kernel<8, M><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW);
     else
-        matmul_nr_kernel<16><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW);
+        matmul_nr_kernel<16, M><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW);
     const int64_t tot = (int64_t)n * R;
     reduce_parts_kernel<<<dim3((tot + 255) / 256, 1, batch), 256, 0, st>>>(n, R, work, sW, Y, sY);
+}
+
+void launch_matmul_nr(hipStream_t st, int n, int R, const cplx* S, int64_t sS, const cplx* X, int64_t sX, cplx* Y,
+                      int64_t sY, cplx* work, int64_t sW, int batch) {
+    matmul_nr_t(st, n, R, S, sS, X, sX, Y, sY, work, sW, batch);
+}
+
+void launch_matmul_nr(hipStream_t st, int n, int R, const cplxf* S, int64_t sS, const cplx* X, int64_t sX, cplx* Y,
+                      int64_t sY, cplx* work, int64_t sW, int batch) {
+    matmul_nr_t(st, n, R, S, sS, X, sX, Y, sY, work, sW, batch);
 }
 
 // ---------------------------------------------------------------------------

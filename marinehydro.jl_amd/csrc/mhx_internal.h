@@ -54,7 +54,8 @@ struct SweepItem {
     cplxf* Df;                    // FP32 copy of D (the surrogate's mixed-precision solve), or null
     const double* k;
     int64_t unit0, tunit0, sunit0;  // first combine / exact-table / surrogate unit of the item
-    int n, pad;
+    int n;
+    int sd32;  // the surrogate pair kernel stores S and D as FP32 (at the items' FP64 slots)
 };
 // combine units of an item (256 rows x 8 columns) and table units (16 x 16 blocks, upper triangle)
 int64_t sweep_combine_units(int n);
@@ -73,8 +74,11 @@ void launch_sticky_info(hipStream_t st, const int* info, int batch, int* sticky)
 
 void launch_bc(hipStream_t st, int n, const PanelGeom* geom, const double* dofs, int ndof, const double* betas, int nb,
                const double* ws, const double* ks, int batch, double g, cplx* B, int64_t sB);
-// Y = S X (n x n times n x R); split-K partials in work (16 * n * R complex per item, stride sW)
+// Y = S X (n x n times n x R); split-K partials in work (16 * n * R complex per item, stride sW);
+// S in FP64, or in FP32 (the products and sums in FP64 either way)
 void launch_matmul_nr(hipStream_t st, int n, int R, const cplx* S, int64_t sS, const cplx* X, int64_t sX, cplx* Y,
+                      int64_t sY, cplx* work, int64_t sW, int batch);
+void launch_matmul_nr(hipStream_t st, int n, int R, const cplxf* S, int64_t sS, const cplx* X, int64_t sX, cplx* Y,
                       int64_t sY, cplx* work, int64_t sW, int batch);
 // F[d + ndof*r] = sum_i (i rho omega phi[i, r]) * -(dof_d,i . n_i) * a_i ; FK[d + ndof*b] for the Airy
 // pressure; item z writes F + z*ndof*R and FK + z*ndof*nb
@@ -132,6 +136,8 @@ int getrs_batched(hipStream_t st, int trans, int n, int R, const cplxf* LU, int 
                   int64_t ipiv_stride, int* perm, int64_t perm_stride, cplx* B, int64_t sB, cplx* work, int64_t sW,
                   int batch, int prepared = kGetrsFresh);
 void convert_to_f32(hipStream_t st, int64_t count, const cplx* src, cplxf* dst);
+// per item z < batch: dst[z sD + e] = (FP64) src[z sS + e], e < count
+void widen_to_f64(hipStream_t st, int64_t count, int batch, const cplxf* src, int64_t sS, cplx* dst, int64_t sD);
 // T <- Bv - T (n R elements per item) and per-item squared norms |T|^2, |Bv|^2 added into norms[2z], norms[2z+1]
 void launch_residual(hipStream_t st, int64_t cnt, const cplx* Bv, int64_t sB, cplx* T, int64_t sT, double* norms,
                      int batch);

@@ -1915,6 +1915,21 @@ void convert_to_f32(hipStream_t st, int64_t count, const cplx* src, cplxf* dst) 
     to_f32_kernel<<<(int)blocks, 256, 0, st>>>(count, src, dst);
 }
 
+// dst[z * sD + e] = src[z * sS + e] widened, e < count, item z = blockIdx.y
+__global__ void widen_kernel(int64_t count, const cplxf* __restrict__ src, int64_t sS, cplx* __restrict__ dst,
+                             int64_t sD) {
+    const cplxf* a = src + sS * blockIdx.y;
+    cplx* o = dst + sD * blockIdx.y;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        o[i] = to_d(a[i]);
+}
+
+void widen_to_f64(hipStream_t st, int64_t count, int batch, const cplxf* src, int64_t sS, cplx* dst, int64_t sD) {
+    if (count <= 0 || batch <= 0) return;
+    const int64_t blocks = std::min<int64_t>((count + 255) / 256, 2048);
+    widen_kernel<<<dim3((unsigned)blocks, (unsigned)batch), 256, 0, st>>>(count, src, sS, dst, sD);
+}
+
 // per item z: T <- Bv - T over n*R elements; norms[2z] += |T|^2, norms[2z+1] += |Bv|^2
 __global__ void residual_kernel(int64_t cnt, const cplx* __restrict__ Bvb, int64_t sB, cplx* __restrict__ Tb,
                                 int64_t sT, double* __restrict__ norms) {

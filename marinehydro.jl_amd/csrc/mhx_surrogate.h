@@ -45,6 +45,9 @@ struct SurrOut {
     cplxf* Df = nullptr;
     int direct = 1, include_identity = 1;
     int64_t sMat = 0;
+    // S and D stored as FP32 complex: entry rc of item q at ((cplxf*)(S + sMat q))[rc] (the first half
+    // of the item's FP64 slot)
+    int sd32 = 0;
 };
 void launch_surr_table(hipStream_t st, int n, const PanelGeom* geom, const double* ks, int batch,
                        const unsigned short* Wb, const float* bias, const float* Bf, const SurrOut& out,

@@ -624,6 +624,7 @@ struct SurrDst {
     cplx *S, *D;
     cplxf* Df;
     int direct, include_identity;
+    int sd32;  // S and D as FP32 complex at the item's slots
 };
 
 __device__ __forceinline__ SurrDst surr_dst(const SurrOut& o, int item) {
@@ -637,6 +638,7 @@ __device__ __forceinline__ SurrDst surr_dst(const SurrOut& o, int item) {
     d.Df = o.Df ? o.Df + o.sMat * item : nullptr;
     d.direct = o.direct;
     d.include_identity = o.include_identity;
+    d.sd32 = o.sd32;
     return d;
 }
 
@@ -650,6 +652,7 @@ __device__ __forceinline__ SurrDst surr_dst(const SweepItem& it, int direct, int
     d.Df = it.Df;
     d.direct = direct;
     d.include_identity = include_identity;
+    d.sd32 = it.sd32;
     return d;
 }
 
@@ -680,8 +683,13 @@ __device__ __forceinline__ void surr_store_entry(const SurrDst& o, int n, int r,
     cplx Sv, Dv;
     combine_entry(k, pq, gr.c, gr.nrm, geom[c], o.direct, o.include_identity && r == c, G, Gr, o.SR[rc],
                   o.DR[rc], Sv, Dv);
-    store_stream(o.S + rc, Sv);
-    store_stream(o.D + rc, Dv);
+    if (o.sd32) {
+        store_stream(reinterpret_cast<cplxf*>(o.S) + rc, Sv.x, Sv.y);
+        store_stream(reinterpret_cast<cplxf*>(o.D) + rc, Dv.x, Dv.y);
+    } else {
+        store_stream(o.S + rc, Sv);
+        store_stream(o.D + rc, Dv);
+    }
     if (o.Df) store_stream(o.Df + rc, Dv.x, Dv.y);
 }
 
