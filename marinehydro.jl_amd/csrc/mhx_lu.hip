@@ -896,6 +896,9 @@ void zgemm(hipStream_t st, bool sub, bool accum, int m, int nn, int kk, const cp
 //   q = 4 .. 7  U12 columns: re hi, re lo, im hi, im lo
 // ---------------------------------------------------------------------------
 constexpr int SPLIT_TILE = 128;
+#ifndef SPLIT_GROUP
+#define SPLIT_GROUP 8
+#endif
 constexpr int SPLIT_SUPER = kSplitSuperMax;  // most panels per trailing update on the split path (k <= 512)
 constexpr int SPLIT_KC = 16;    // k chunk of the layout and of the update's LDS stages
 typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
@@ -980,14 +983,18 @@ __global__ void __launch_bounds__(256, 2) zgemm_split_kernel(int m, int mpad, in
     // rows), U12 columns the A operand (C columns)
     __shared__ bf16x8_t Ls[2 * 2 * 4 * 2 * SPLIT_TILE];
     // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin by linear id, so the
-    // id is remapped to give each XCD a contiguous run of tiles (row tile fastest: the tiles an XCD
-    // holds at once share their column panel and walk down the same column-major C columns)
+    // id is remapped to give each XCD a contiguous run of tiles; within a matrix the run walks
+    // groups of SPLIT_GROUP row tiles, row tile fastest, so that the 64 tiles an XCD holds at once
+    // form a SPLIT_GROUP x (64 / SPLIT_GROUP) block and share its row and column panels in the L2
     int L = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
     {
         const int T = gridDim.x * gridDim.y * gridDim.z, q = T / 8;
         if (L < 8 * q) L = (L % 8) * q + L / 8;
     }
-    const int tx = L % gridDim.x, ty = (L / gridDim.x) % gridDim.y, tz = L / (gridDim.x * gridDim.y);
+    const int per = gridDim.x * gridDim.y, tz = L / per, t = L - tz * per;
+    const int grp = t / (SPLIT_GROUP * gridDim.y), first = grp * SPLIT_GROUP;
+    const int gs = min((int)gridDim.x - first, SPLIT_GROUP), r = t - grp * SPLIT_GROUP * gridDim.y;
+    const int tx = first + r % gs, ty = r / gs;
     const unsigned short* S = Sb + sS * tz;
     cplxf* Cm = Cb + sC * tz;
     const int tid = threadIdx.x, lane = tid & 63, h = lane >> 5, l32 = lane & 31;
