@@ -224,6 +224,13 @@ void mhx_reset_timing(mhx_ctx* ctx);
  * kept); off, no events are recorded.  No counterpart in the reference; instrumentation of this
  * build. */
 int mhx_set_timing(mhx_ctx* ctx, int on);
+/* Refinement steps the mixed-precision solver may take before an item is redone with the FP64 LU:
+ * -1 (default) restores the solver's own limit (4 with FP32 trailing updates, 8 with split-bf16
+ * ones), 0 .. 8 sets it.  The sweep (mhx_sweep_coeffs) always takes one step and redoes an item
+ * that missed 1e-12 through the per-frequency FP64 path; with 0 it redoes every item.  0 is the
+ * diagnostic setting that sends every system down the FP64 path.  No counterpart in the
+ * reference (its solves are one LAPACK call); a control of this build's mixed-precision solver. */
+int mhx_set_refine_limit(mhx_ctx* ctx, int steps);
 
 /* work of the exact Guevel-Delhommeau wave term since the last mhx_reset_timing: counts =
  * {integrand evaluations, E1 power-series terms, E1 continued-fraction steps}; flops = the

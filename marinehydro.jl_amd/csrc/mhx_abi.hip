@@ -45,6 +45,7 @@ struct mhx_ctx {
     int last_streams = 0, last_super = 0;  // the configuration of the last batched FP32 factorization
     double rho = 1000.0, g = 9.81;
     int timing = 0;
+    int refine_limit = -1;  // refinement steps before an FP64 redo (-1: the solver's default)
     std::string err;
     std::map<std::string, PhaseAcc> phases;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -613,7 +614,7 @@ int solve_mixed(mhx_ctx* ctx, mhx_body* b, int R, int direct, int batch, int gf,
         return rc;
     const bool split = use_split(ctx, gf);
     if (split && (rc = dalloc(ctx, &b->Ps, (size_t)mhx::split_work(n) * b->cap_batch))) return rc;
-    const int refine_max = split ? kRefineMaxSplit : kRefineMax;
+    const int refine_max = ctx->refine_limit >= 0 ? ctx->refine_limit : split ? kRefineMaxSplit : kRefineMax;
     bool inverses_ready = true;  // the verified factorization wrote the solves' diagonal-block inverses
     mhx::launch_zero(st, b->info, sizeof(int) * batch);
     {
@@ -2414,7 +2415,7 @@ int mhx_sweep_coeffs(mhx_ctx* ctx, const mhx_sweep_body* in, int64_t nbodies, in
                 bool missed = false;
                 if (s->mixed) {
                     const double rel = rn[2 * f + 1] > 0 ? std::sqrt(rn[2 * f] / rn[2 * f + 1]) : std::sqrt(rn[2 * f]);
-                    missed = !(rel <= kRefineTol);
+                    missed = !(rel <= kRefineTol) || ctx->refine_limit == 0;
                     if (!missed) {
                         ctx->mixed_solves++;
                         ctx->refine_steps++;
@@ -2477,6 +2478,13 @@ int mhx_phase_time(mhx_ctx* ctx, const char* phase, double* total_ms, int64_t* c
 int mhx_set_timing(mhx_ctx* ctx, int on) {
     if (!ctx) return MHX_EINVAL;
     ctx->timing = on ? 1 : 0;
+    return MHX_OK;
+}
+
+int mhx_set_refine_limit(mhx_ctx* ctx, int steps) {
+    if (!ctx) return MHX_EINVAL;
+    if (steps < -1 || steps > kRefineMaxSplit) return fail(ctx, MHX_EINVAL, "refinement limit outside -1 .. 8");
+    ctx->refine_limit = steps;
     return MHX_OK;
 }
 

@@ -40,7 +40,7 @@ EXPORTED = ["mhx_create", "mhx_destroy", "mhx_last_error", "mhx_set_constants", 
             "mhx_lu_solve", "mhx_lu_solve_batched", "mhx_lu_solve_noalloc", "mhx_hydro_coeffs", "mhx_body_create", "mhx_body_destroy", "mhx_body_coeffs",
             "mhx_phase_time", "mhx_reset_timing", "mhx_exact_work", "mhx_set_batch", "mhx_surrogate_train",
             "mhx_surrogate_save", "mhx_surrogate_load", "mhx_surrogate_params", "mhx_surrogate_eval",
-            "mhx_surrogate_loss_grad", "mhx_surrogate_set_activation", "mhx_surrogate_get_activation", "mhx_hydro_coeffs_fs_jvp", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_set_pivoting", "mhx_set_lu_streams", "mhx_set_autotune", "mhx_lu_config", "mhx_set_timing",
+            "mhx_surrogate_loss_grad", "mhx_surrogate_set_activation", "mhx_surrogate_get_activation", "mhx_hydro_coeffs_fs_jvp", "mhx_set_lu_mode", "mhx_solver_stats", "mhx_set_pivoting", "mhx_set_lu_streams", "mhx_set_autotune", "mhx_lu_config", "mhx_set_timing", "mhx_set_refine_limit",
             "mhx_pivot_stats", "mhx_hydro_coeffs_jvp",
             "mhx_hydro_coeffs_vjp", "mhx_sweep_coeffs", "mhx_body_create_lid", "mhx_body_coeffs_fs"]
 
@@ -135,6 +135,7 @@ def lib():
         L.mhx_set_autotune.argtypes = [vp, C.c_int]
         L.mhx_lu_config.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.mhx_set_timing.argtypes = [vp, C.c_int]
+        L.mhx_set_refine_limit.argtypes = [vp, C.c_int]
         L.mhx_solver_stats.argtypes = [vp, _ip, _dp]
         L.mhx_set_pivoting.argtypes = [vp, C.c_int]
         L.mhx_pivot_stats.argtypes = [vp, _ip]
@@ -209,6 +210,11 @@ class Context:
     def set_timing(self, on):
         """Per-phase timing events on or off (from the next call on)."""
         self.check(lib().mhx_set_timing(self.h, int(bool(on))))
+
+    def set_refine_limit(self, steps):
+        """Refinement steps of the mixed-precision solver before an item is redone in FP64 (-1:
+        the solver's default; 0 sends every system down the FP64 path)."""
+        self.check(lib().mhx_set_refine_limit(self.h, int(steps)))
 
     def reset_timing(self):
         lib().mhx_reset_timing(self.h)

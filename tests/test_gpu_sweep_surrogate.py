@@ -119,3 +119,40 @@ def test_graph_schedule_equals_batched(mhx, gf):
         for key in ("added_mass", "radiation_damping", "diffraction_force", "Froude_Krylov_force"):
             rel, elem, ok = close(g[key], b[key], tol=1e-10)
             assert ok, (gf, key, rel, elem)
+
+
+def test_fp64_path_of_the_surrogate_storage(mhx):
+    """Refinement limit 0 sends every system down the FP64 path.  The batched schedule then widens
+    the FP32 S and D the pair kernel stored and factors them in FP64: the same matrices the mixed
+    solver refines to 1e-12, so the coefficients agree to 1e-9.  The sweep redoes every item
+    through the per-frequency FP64 path, which assembles S and D again in FP64: within FP32
+    rounding (1e-5) of the sweep's own results."""
+    form = mhx.DirectBEM("Surrogate")
+    body, par = _small_jobs(mhx)[1]
+    nw = len(par["wave_frequencies"])
+    ctx = mhx.context()
+    keys = ("added_mass", "radiation_damping", "diffraction_force", "Froude_Krylov_force")
+    print("refine limit: reference runs", flush=True)
+    ref = mhx.hydrodynamic_coefficients(body, par, form, schedule="batched")
+    ref_sweep = mhx.hydrodynamic_coefficients_many([(body, par)], form)[0]
+    try:
+        ctx.set_refine_limit(0)
+        ctx.reset_timing()
+        print("refine limit 0: batched", flush=True)
+        got = mhx.hydrodynamic_coefficients(body, par, form, schedule="batched")
+        solved, _, fallbacks, _ = ctx.solver_stats()
+        assert solved == 0 and fallbacks == nw, (solved, fallbacks)
+        ctx.reset_timing()
+        print("refine limit 0: sweep", flush=True)
+        sweep = mhx.hydrodynamic_coefficients_many([(body, par)], form)[0]
+        solved, _, fallbacks, _ = ctx.solver_stats()
+        assert solved == 0 and fallbacks == nw, (solved, fallbacks)
+    finally:
+        ctx.set_refine_limit(-1)
+    for key in keys:
+        rel, elem, ok = close(got[key], ref[key], tol=1e-9, abs_floor=1e-10)
+        assert ok, ("batched", key, rel, elem)
+        rel, elem, ok = close(sweep[key], ref_sweep[key], tol=1e-5, abs_floor=1e-6)
+        assert ok, ("sweep", key, rel, elem)
+    with pytest.raises(ValueError):
+        ctx.set_refine_limit(9)
