@@ -788,12 +788,19 @@ template <class M>
 static void matmul_nr_t(hipStream_t st, int n, int R, const M* S, int64_t sS, const cplx* X, int64_t sX, cplx* Y,
                         int64_t sY, cplx* work, int64_t sW, int batch) {
     dim3 grid((n + 255) / 256, NSPLIT, batch);
-    if (R <= 4)
-        matmul_nr_kernel<4, M><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW);
-    else if (R <= 8)
-        matmul_nr_kernel<8, M><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW);
-    else
-        matmul_nr_kernel<16, M><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW);
+    // one instantiation per column count up to 8 (the FP64 multiply-adds are as much of the cost
+    // as the matrix stream, so padding columns are not free), 16 above
+    switch (R) {
+        case 1: matmul_nr_kernel<1, M><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW); break;
+        case 2: matmul_nr_kernel<2, M><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW); break;
+        case 3: matmul_nr_kernel<3, M><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW); break;
+        case 4: matmul_nr_kernel<4, M><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW); break;
+        case 5: matmul_nr_kernel<5, M><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW); break;
+        case 6: matmul_nr_kernel<6, M><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW); break;
+        case 7: matmul_nr_kernel<7, M><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW); break;
+        case 8: matmul_nr_kernel<8, M><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW); break;
+        default: matmul_nr_kernel<16, M><<<grid, 256, 0, st>>>(n, R, S, sS, X, sX, work, sW); break;
+    }
     const int64_t tot = (int64_t)n * R;
     reduce_parts_kernel<<<dim3((tot + 255) / 256, 1, batch), 256, 0, st>>>(n, R, work, sW, Y, sY);
 }
