@@ -110,7 +110,7 @@ def cpu_baseline(mesh, omegas, nfreq, threads, sample_rows=32):
         t_asm += t1 - t0
         t_lu += t2 - t1
     per = (t_asm + t_lu) / len(ws)
-    host = f"host: {nproc} logical CPUs ({model}); {threads} threads used (the GPU box's CPU share)"
+    host = f"host: {nproc} logical CPUs ({model}); {threads} threads used (the host's CPU share for one GPU)"
     best = {"value": 1.0 / per, "unit": "omega-solves/s", "cores": threads, "kind": "port", "mode": "best CPU",
             "sample": f"{len(ws)} complete omega-solves at omega = {np.round(ws, 3).tolist()} on {n} panels: "
                       f"exact assembly of S and D once per frequency (oracle, {threads} OpenMP threads, "
@@ -342,7 +342,7 @@ def main():
                          "multibody: the seeded uneven 8-body workload as one graph-captured sweep")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: this benchmark measures one GPU; --gpus {args.gpus} is not supported "
               "(frequency items would shard across GPUs, one process per GPU)", file=sys.stderr)
         sys.exit(2)
 

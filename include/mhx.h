@@ -186,14 +186,14 @@ int mhx_solver_stats(mhx_ctx* ctx, int64_t counts[3], double* max_residual);
  * (default 2): a batch of 8 or more systems of order >= 512 is factored in that many parts of at
  * least 4 systems, on as many streams, so that one part's panel phases overlap the others'
  * trailing updates.  Results do not depend on it.  No counterpart in the reference (its LU is one
- * LAPACK call per system); a tuning control of this build's batched solver. */
+ * LAPACK call per system); a tuning control of this library's batched solver. */
 int mhx_set_lu_streams(mhx_ctx* ctx, int streams);
 /* Autotuning of the batched FP32 factorization (on by default): on the first batched factorization
  * of a signature (order, batch size, update kind, pivoting) the library factors that batch with
  * 1-3 streams and 2-8 panels per split-bf16 update (each timed twice), keeps the fastest and uses it for every later
  * call of the signature (streams set with mhx_set_lu_streams are kept).  mhx_lu_config: the
  * streams and panels per update of the last batched FP32 factorization (0, 0 before any).  No
- * counterpart in the reference; tuning controls of this build's batched solver. */
+ * counterpart in the reference; tuning controls of this library's batched solver. */
 int mhx_set_autotune(mhx_ctx* ctx, int on);
 int mhx_lu_config(mhx_ctx* ctx, int* streams, int* super_panels);
 
@@ -222,14 +222,14 @@ int mhx_phase_time(mhx_ctx* ctx, const char* phase, double* total_ms, int64_t* c
 void mhx_reset_timing(mhx_ctx* ctx);
 /* per-phase timing on (1) or off (0) from the next call on (the events of calls already made are
  * kept); off, no events are recorded.  No counterpart in the reference; instrumentation of this
- * build. */
+ * library. */
 int mhx_set_timing(mhx_ctx* ctx, int on);
 /* Refinement steps the mixed-precision solver may take before an item is redone with the FP64 LU:
  * -1 (default) restores the solver's own limit (4 with FP32 trailing updates, 8 with split-bf16
  * ones), 0 .. 8 sets it.  The sweep (mhx_sweep_coeffs) always takes one step and redoes an item
  * that missed 1e-12 through the per-frequency FP64 path; with 0 it redoes every item.  0 is the
  * diagnostic setting that sends every system down the FP64 path.  No counterpart in the
- * reference (its solves are one LAPACK call); a control of this build's mixed-precision solver. */
+ * reference (its solves are one LAPACK call); a control of this library's mixed-precision solver. */
 int mhx_set_refine_limit(mhx_ctx* ctx, int steps);
 
 /* work of the exact Guevel-Delhommeau wave term since the last mhx_reset_timing: counts =
